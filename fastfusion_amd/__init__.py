@@ -1,0 +1,247 @@
+"""fastfusion_amd -- MI355X-native TSDF fusion + incremental meshing.
+
+Host-side mirror of the reference's ``FusionMipMapCPU`` interface
+(src/fusion/geometryfusion_mipmap_cpu.hpp:111-163) over the C ABI of
+``lib/libffgpu.so`` (declared in include/ff_abi.h).  Every method maps to one
+ABI call; all compute runs in the HIP library.  There is no CPU fallback:
+importing the binding raises if the library has not been built.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_DIR = os.path.join(_HERE, "lib")
+_lib_handle = None
+
+
+class FFError(RuntimeError):
+    pass
+
+
+class _Params(ctypes.Structure):
+    _fields_ = [("scale", ctypes.c_float), ("distance_threshold", ctypes.c_float),
+                ("use_color", ctypes.c_int), ("incremental_meshing", ctypes.c_int),
+                ("max_leaves", ctypes.c_uint32), ("max_branches", ctypes.c_uint32),
+                ("device", ctypes.c_int)]
+
+
+class _Counts(ctypes.Structure):
+    _fields_ = [("n_leaves_used", ctypes.c_uint32), ("n_branches_used", ctypes.c_uint32),
+                ("n_queued", ctypes.c_uint32), ("side", ctypes.c_uint32),
+                ("offset", ctypes.c_float * 3), ("frames", ctypes.c_uint32),
+                ("grows", ctypes.c_uint32)]
+
+
+# name -> (restype, argtypes); exactly the symbols of include/ff_abi.h
+_P = ctypes.c_void_p
+_SZ = ctypes.c_size_t
+_DP = ctypes.POINTER(ctypes.c_double)
+ABI = {
+    "ff_create": (_P, [ctypes.POINTER(_Params)]),
+    "ff_destroy": (None, [_P]),
+    "ff_last_error": (ctypes.c_char_p, [_P]),
+    "ff_add_map": (ctypes.c_int, [_P, _P, _SZ, _P, _SZ, ctypes.c_int, ctypes.c_int, _DP, _DP, _DP,
+                                  ctypes.c_float, ctypes.c_float]),
+    "ff_add_map_device": (ctypes.c_int, [_P, _P, _P, ctypes.c_int, ctypes.c_int, _DP, _DP, _DP,
+                                         ctypes.c_float, ctypes.c_float]),
+    "ff_update_meshes": (ctypes.c_int, [_P]),
+    "ff_mesh_sizes": (ctypes.c_int, [_P, ctypes.POINTER(_SZ), ctypes.POINTER(_SZ)]),
+    "ff_copy_mesh": (ctypes.c_int, [_P, _P, _P, _P]),
+    "ff_cell_count": (ctypes.c_int, [_P, ctypes.POINTER(_SZ)]),
+    "ff_copy_cells": (ctypes.c_int, [_P, _P, _P, _P]),
+    "ff_set_incremental_meshing": (ctypes.c_int, [_P, ctypes.c_int]),
+    "ff_get_counts": (ctypes.c_int, [_P, ctypes.POINTER(_Counts)]),
+    "ff_copy_tree": (ctypes.c_int, [_P, _P, _SZ]),
+    "ff_copy_leaves": (ctypes.c_int, [_P, _P, _P, _P, _SZ]),
+    "ff_copy_bricks": (ctypes.c_int, [_P, _P, _SZ, _P, _P, _P]),
+    "ff_copy_queue": (ctypes.c_int, [_P, _P, _SZ]),
+    "ff_copy_processed": (ctypes.c_int, [_P, _P, _SZ]),
+    "ff_stream": (_P, [_P]),
+}
+
+
+def lib_path():
+    return os.path.join(LIB_DIR, "libffgpu.so")
+
+
+def load_library():
+    """Loads libffgpu.so; raises FFError if it has not been built."""
+    global _lib_handle
+    if _lib_handle is not None:
+        return _lib_handle
+    path = lib_path()
+    if not os.path.exists(path):
+        raise FFError(f"{path} is missing: build it with `python -m fastfusion_amd.build` "
+                      "(the HIP library is the only implementation; there is no CPU fallback)")
+    lib = ctypes.CDLL(path)
+    for name, (res, args) in ABI.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib_handle = lib
+    return lib
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p) if a is not None else None
+
+
+def _dptr(a):
+    return np.ascontiguousarray(a, dtype=np.float64).ctypes.data_as(_DP)
+
+
+class MeshInterleaved:
+    """The reference's MeshInterleaved (src/fusion/mesh_interleaved.hpp:35-69):
+    vertices (N, 3) float32, colors (N, 3) uint8 as R, G, B, faces (M, 3) uint32."""
+
+    def __init__(self, vertices, colors, faces):
+        self.vertices = vertices
+        self.colors = colors
+        self.faces = faces
+
+    def writePLY(self, path):
+        """Binary little-endian PLY, like MeshInterleaved::writePLY
+        (src/fusion/mesh_interleaved.cpp:139-200)."""
+        nv, nf = len(self.vertices), len(self.faces)
+        with open(path, "wb") as f:
+            hdr = ("ply\nformat binary_little_endian 1.0\n"
+                   f"element vertex {nv}\nproperty float x\nproperty float y\nproperty float z\n"
+                   "property uchar red\nproperty uchar green\nproperty uchar blue\n"
+                   f"element face {nf}\nproperty list uchar uint vertex_indices\nend_header\n")
+            f.write(hdr.encode())
+            v = np.zeros(nv, dtype=[("p", "<f4", 3), ("c", "u1", 3)])
+            v["p"] = self.vertices
+            v["c"] = self.colors
+            f.write(v.tobytes())
+            fa = np.zeros(nf, dtype=[("n", "u1"), ("i", "<u4", 3)])
+            fa["n"] = 3
+            fa["i"] = self.faces
+            f.write(fa.tobytes())
+
+
+class FusionMipMapGPU:
+    """Drop-in for the reference's FusionMipMapCPU hot path.
+
+    ``FusionMipMapCPU(offsetX, offsetY, offsetZ, scale, distanceThreshold, n, color)``
+    (geometryfusion_mipmap_cpu.cpp:420-640): like the reference, the offset
+    and ``n`` arguments are ignored (the volume is set by the first frame).
+    """
+
+    def __init__(self, offsetX=0.0, offsetY=0.0, offsetZ=0.0, scale=0.005, distanceThreshold=0.005,
+                 n=0, color=True, max_leaves=0, max_branches=0, device=0, incremental=True):
+        self._lib = load_library()
+        p = _Params(float(scale), float(distanceThreshold), int(bool(color)), int(bool(incremental)),
+                    int(max_leaves), int(max_branches), int(device))
+        self._ctx = self._lib.ff_create(ctypes.byref(p))
+        if not self._ctx:
+            raise FFError("ff_create failed (no HIP device or allocation failure)")
+        self.color = bool(color)
+
+    def close(self):
+        if getattr(self, "_ctx", None):
+            self._lib.ff_destroy(self._ctx)
+            self._ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc):
+        if rc < 0:
+            raise FFError(self._lib.ff_last_error(self._ctx).decode() or f"error {rc}")
+        return rc
+
+    # -- FusionMipMapCPU::addMap (u16 overload), :1603-2034
+    def addMap(self, depth, R, t, K, rgb=None, scaling=1.0 / 5000.0, maxcamdistance=10.0):
+        depth = np.ascontiguousarray(depth, dtype=np.uint16)
+        h, w = depth.shape
+        if rgb is not None:
+            rgb = np.ascontiguousarray(rgb, dtype=np.uint8)
+            if rgb.shape != (h, w, 3):
+                raise FFError("rgb must be (h, w, 3) BGR8 like depth")
+        rc = self._lib.ff_add_map(self._ctx, _ptr(depth), w * 2, _ptr(rgb), w * 3, w, h, _dptr(R), _dptr(t),
+                                  _dptr(K), float(scaling), float(maxcamdistance))
+        return self._check(rc)
+
+    def addMapDevice(self, depth_ptr, rgb_ptr, w, h, R, t, K, scaling=1.0 / 5000.0, maxcamdistance=10.0):
+        rc = self._lib.ff_add_map_device(self._ctx, ctypes.c_void_p(depth_ptr), ctypes.c_void_p(rgb_ptr),
+                                         int(w), int(h), _dptr(R), _dptr(t), _dptr(K), float(scaling),
+                                         float(maxcamdistance))
+        return self._check(rc)
+
+    # -- updateMeshes, :2884-2954
+    def updateMeshes(self):
+        return self._check(self._lib.ff_update_meshes(self._ctx))
+
+    def setIncrementalMeshing(self, on=True):
+        self._check(self._lib.ff_set_incremental_meshing(self._ctx, int(bool(on))))
+
+    # -- getMeshInterleavedMarchingCubes, :3359-3374
+    def getMeshInterleavedMarchingCubes(self):
+        nv, nf = _SZ(), _SZ()
+        self._check(self._lib.ff_mesh_sizes(self._ctx, ctypes.byref(nv), ctypes.byref(nf)))
+        v = np.zeros((nv.value, 3), np.float32)
+        c = np.zeros((nv.value, 3), np.uint8)
+        f = np.zeros((nf.value, 3), np.uint32)
+        self._check(self._lib.ff_copy_mesh(self._ctx, _ptr(v), _ptr(c), _ptr(f)))
+        return MeshInterleaved(v, c, f)
+
+    def getMeshCells(self):
+        """Per-cell descriptors {type, ox, oy, oz, size} with vertex / face counts."""
+        n = _SZ()
+        self._check(self._lib.ff_cell_count(self._ctx, ctypes.byref(n)))
+        d = np.zeros((n.value, 5), np.int32)
+        nv = np.zeros(n.value, np.uint32)
+        nf = np.zeros(n.value, np.uint32)
+        self._check(self._lib.ff_copy_cells(self._ctx, _ptr(d), _ptr(nv), _ptr(nf)))
+        return d, nv, nf
+
+    # -- state export
+    def counts(self):
+        c = _Counts()
+        self._check(self._lib.ff_get_counts(self._ctx, ctypes.byref(c)))
+        return {"n_leaves_used": c.n_leaves_used, "n_branches_used": c.n_branches_used,
+                "n_queued": c.n_queued, "side": c.side, "offset": np.array(c.offset[:], np.float32),
+                "frames": c.frames, "grows": c.grows}
+
+    def tree(self, n_slots=None):
+        n = self.counts()["n_branches_used"] if n_slots is None else n_slots
+        out = np.zeros(n, np.uint32)
+        self._check(self._lib.ff_copy_tree(self._ctx, _ptr(out), n))
+        return out
+
+    def leaves(self, n=None):
+        n = self.counts()["n_leaves_used"] if n is None else n
+        pos = np.zeros((n, 3), np.uint16)
+        sc = np.zeros(n, np.uint16)
+        par = np.zeros(n, np.uint32)
+        self._check(self._lib.ff_copy_leaves(self._ctx, _ptr(pos), _ptr(sc), _ptr(par), n))
+        return pos, sc, par
+
+    def bricks(self, leaves):
+        leaves = np.ascontiguousarray(leaves, dtype=np.uint32)
+        n = len(leaves)
+        d = np.zeros((n, 512), np.float32)
+        w = np.zeros((n, 512), np.float32)
+        c = np.zeros((n, 3, 512), np.uint16)
+        self._check(self._lib.ff_copy_bricks(self._ctx, _ptr(leaves), n, _ptr(d), _ptr(w),
+                                             _ptr(c) if self.color else None))
+        return d, w, c
+
+    def queue(self):
+        n = self.counts()["n_queued"]
+        out = np.zeros(max(n, 1), np.uint32)
+        k = self._check(self._lib.ff_copy_queue(self._ctx, _ptr(out), n))
+        return out[:k]
+
+    def processed(self, w, h):
+        out = np.zeros((h, w), np.uint8)
+        self._check(self._lib.ff_copy_processed(self._ctx, _ptr(out), w * h))
+        return out
+
+    def stream(self):
+        return self._lib.ff_stream(self._ctx)

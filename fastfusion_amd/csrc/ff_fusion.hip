@@ -1,0 +1,1130 @@
+// ff_fusion.hip -- per-frame depth integration on gfx950.
+//
+// One frame (the reference's FusionMipMapCPU::addMap, u16 overload,
+// src/fusion/geometryfusion_mipmap_cpu.cpp:1603-2034) runs as:
+//   k_planes     per pixel: brick key of the back-projected point; three bit
+//                planes per row (valid, same key as left, same key as up)
+//   k_dedup      the reference's order-dependent de-dup recurrence
+//                (traversal_wrapper_functions.cpp:601-690), solved exactly by
+//                one wavefront sweeping rows with a lane skew, 32 columns per
+//                lane, bit-parallel within a lane
+//   k_descend    per processed pixel: octree descent through the existing
+//                tree, queueing the bricks on the path, recording every
+//                missing node with the smallest pixel index that reaches it
+//   k_resolve / scan / k_assign
+//                new branches and bricks get indices in the order the
+//                sequential reference creates them (pixel order, top-down)
+//   k_link       writes the new tree links and bricks, with the parent brick
+//                the reference sees at the creating pixel's turn
+//   k_sdf        the per-voxel SDF update (update_functions.cpp:1187-1444)
+//                with the reference's round-toward-zero + RCPPS arithmetic
+#include "ff_device.hpp"
+#include "ff_internal.hpp"
+#include "rcp_table.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+using namespace ff;
+
+namespace ff {
+
+__constant__ uint32_t c_rcp_table[2048];
+
+// ---------------------------------------------------------------------------
+// hash table of nodes created this frame
+// key: [type:1][depth:5][x:16][y:16][z:16]
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint64_t node_key(uint32_t type, uint32_t depth, uint32_t x, uint32_t y,
+                                             uint32_t z)
+{
+    return ((uint64_t)type << 53) | ((uint64_t)depth << 48) | ((uint64_t)x << 32) |
+           ((uint64_t)y << 16) | (uint64_t)z;
+}
+
+__device__ __forceinline__ uint32_t key_hash(uint64_t k, uint32_t mask)
+{
+    k ^= k >> 33;
+    k *= 0xff51afd7ed558ccdull;
+    k ^= k >> 33;
+    k *= 0xc4ceb9fe1a85ec53ull;
+    k ^= k >> 33;
+    return (uint32_t)k & mask;
+}
+
+__device__ void hash_insert_min(const HashView &h, uint64_t key, uint32_t pix, uint32_t aux,
+                                FrameCounters *fc)
+{
+    uint32_t s = key_hash(key, h.mask);
+    for (uint32_t probe = 0; probe <= h.mask; probe++) {
+        unsigned long long prev = atomicCAS((unsigned long long *)&h.keys[s], (unsigned long long)FF_HEMPTY,
+                                            (unsigned long long)key);
+        if (prev == FF_HEMPTY) {
+            uint32_t li = atomicAdd(&fc->listCount, 1u);
+            if (li < h.listCap) h.list[li] = s;
+            h.aux[s] = aux;
+            atomicMin(&h.minpix[s], pix);
+            return;
+        }
+        if (prev == key) {
+            atomicMin(&h.minpix[s], pix);
+            return;
+        }
+        s = (s + 1) & h.mask;
+    }
+    atomicOr(&fc->flags, FF_FLAG_HASH_FULL);
+}
+
+__device__ __forceinline__ int hash_find(const HashView &h, uint64_t key)
+{
+    uint32_t s = key_hash(key, h.mask);
+    for (uint32_t probe = 0; probe <= h.mask; probe++) {
+        uint64_t k = h.keys[s];
+        if (k == key) return (int)s;
+        if (k == FF_HEMPTY) return -1;
+        s = (s + 1) & h.mask;
+    }
+    return -1;
+}
+
+// ---------------------------------------------------------------------------
+// first frame: integer bounding box (computeBoundingboxIntCPU, u16 overload,
+// geometryfusion_mipmap_cpu.cpp:1270-1303)
+// ---------------------------------------------------------------------------
+__global__ void k_bbox_first(const uint16_t *__restrict__ depth, int W, int H, float scaling, float maxcam,
+                             CamF p, int bw, FrameCounters *fc)
+{
+    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+    int mn[3] = {1000000, 1000000, 1000000}, mx[3] = {-1000000, -1000000, -1000000};
+    if (x < W) {
+        const float pz = __fmul_rn((float)depth[y * W + x], scaling);
+        if (pz > 0.0f && pz < maxcam) {
+            const float px = __fmul_rn(__fdiv_rn(__fsub_rn((float)x, p.cx), p.fx), pz);
+            const float py = __fmul_rn(__fdiv_rn(__fsub_rn((float)y, p.cy), p.fy), pz);
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                const float s = __fadd_rn(__fadd_rn(__fadd_rn(__fmul_rn(p.r[3 * k], px), __fmul_rn(p.r[3 * k + 1], py)),
+                                                    __fmul_rn(p.r[3 * k + 2], pz)),
+                                          p.t[k]);
+                const int v = cvt_x86(s);
+                mn[k] = v - bw;
+                mx[k] = v + bw;
+            }
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        int a = mn[k], b = mx[k];
+        for (int o = 32; o > 0; o >>= 1) {
+            a = min(a, __shfl_xor(a, o));
+            b = max(b, __shfl_xor(b, o));
+        }
+        if ((threadIdx.x & 63) == 0) {
+            atomicMin(&fc->bmin[k], a);
+            atomicMax(&fc->bmax[k], b);
+        }
+    }
+}
+
+// v_k = (int)((((qx_k[x] + qy_k[y]) * D) * scaling) + t_k), RN, no contraction
+// (traversal_wrapper_functions.cpp:613-630)
+__device__ __forceinline__ void voxel_of(const float *__restrict__ qx, const float *__restrict__ qy, int W, int H,
+                                         int x, int y, float D, float scaling, const float t[3], int v[3])
+{
+#pragma unroll
+    for (int k = 0; k < 3; k++)
+        v[k] = cvt_x86(__fadd_rn(__fmul_rn(__fmul_rn(__fadd_rn(qx[k * W + x], qy[k * H + y]), D), scaling), t[k]));
+}
+
+// ---------------------------------------------------------------------------
+// k_planes: per 64-pixel row chunk, ballot the three relations into bit
+// planes: plane[(y*words + w)*3 + {0: valid, 1: key == left key, 2: key == up key}]
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(64) void k_planes(const uint16_t *__restrict__ depth, int W, int H, int words,
+                                               float scaling, float maxcam, const float *__restrict__ qx,
+                                               const float *__restrict__ qy, float t0, float t1, float t2,
+                                               uint32_t *__restrict__ planes, uint32_t *__restrict__ cntB,
+                                               uint32_t *__restrict__ cntL, uint32_t *__restrict__ firstD)
+{
+    const int lane = threadIdx.x, y = blockIdx.y, x = blockIdx.x * 64 + lane;
+    const float t[3] = {t0, t1, t2};
+    bool valid = false, el = false, eu = false;
+    if (x < W) {
+        const float D = (float)depth[y * W + x];
+        const float z = __fmul_rn(D, scaling);
+        valid = z > 0.0f && z < maxcam;
+        int v[3];
+        voxel_of(qx, qy, W, H, x, y, D, scaling, t, v);
+        if (x > 0) {
+            int u[3];
+            voxel_of(qx, qy, W, H, x - 1, y, (float)depth[y * W + x - 1], scaling, t, u);
+            el = ((u[0] & -8) == (v[0] & -8)) && ((u[1] & -8) == (v[1] & -8)) && ((u[2] & -8) == (v[2] & -8));
+        }
+        if (y > 0) {
+            int u[3];
+            voxel_of(qx, qy, W, H, x, y - 1, (float)depth[(y - 1) * W + x], scaling, t, u);
+            eu = ((u[0] & -8) == (v[0] & -8)) && ((u[1] & -8) == (v[1] & -8)) && ((u[2] & -8) == (v[2] & -8));
+        }
+        const size_t i = (size_t)y * W + x;
+        cntB[i] = 0;
+        cntL[i] = 0;
+        firstD[i] = 0xffu;
+    }
+    const uint64_t bv = __ballot(valid), bl = __ballot(el), bu = __ballot(eu);
+    if (lane < 2) {
+        const int w = blockIdx.x * 2 + lane;
+        if (w < words) {
+            const int sh = lane * 32;
+            uint32_t *o = planes + ((size_t)y * words + w) * 3;
+            o[0] = (uint32_t)(bv >> sh);
+            o[1] = (uint32_t)(bl >> sh);
+            o[2] = (uint32_t)(bu >> sh);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// k_dedup: P(x,y) = valid & !(Eu & P(x,y-1)) & !(El & P(x-1,y)).
+// Lane l owns columns [32 l, 32 l + 32) and handles row s - l at step s, so
+// the left neighbour's result for the same row arrives from lane l-1's
+// previous step through a lane shuffle; rows of a lane are carried in a
+// register.  Within a lane the horizontal chain is solved bit-parallel:
+// maximal runs linked by El alternate 1,0,1,... from their start.
+// ---------------------------------------------------------------------------
+template <int PF>
+__global__ __launch_bounds__(64) void k_dedup(const uint32_t *__restrict__ planes, int H, int words,
+                                              uint32_t *__restrict__ pbits)
+{
+    const int l = threadIdx.x;
+    const bool lane_ok = l < words;
+    const int nsteps = H + words - 1;
+    uint32_t ra[PF], rb[PF], rc[PF];
+#pragma unroll
+    for (int k = 0; k < PF; k++) {
+        const int y = k - l;
+        if (lane_ok && y >= 0 && y < H) {
+            const uint32_t *p = planes + ((size_t)y * words + l) * 3;
+            ra[k] = p[0]; rb[k] = p[1]; rc[k] = p[2];
+        } else {
+            ra[k] = rb[k] = rc[k] = 0u;
+        }
+    }
+    uint32_t pprev = 0u, pout = 0u;
+    for (int s0 = 0; s0 < nsteps; s0 += PF) {
+#pragma unroll
+        for (int k = 0; k < PF; k++) {
+            const int s = s0 + k, y = s - l;
+            const uint32_t va = ra[k], el = rb[k], eu = rc[k];
+            {
+                const int yn = y + PF;
+                if (lane_ok && yn >= 0 && yn < H) {
+                    const uint32_t *p = planes + ((size_t)yn * words + l) * 3;
+                    ra[k] = p[0]; rb[k] = p[1]; rc[k] = p[2];
+                } else {
+                    ra[k] = rb[k] = rc[k] = 0u;
+                }
+            }
+            uint32_t pin = __shfl_up(pout, 1);
+            if (l == 0) pin = 0u;
+            const bool active = lane_ok && y >= 0 && y < H && s < nsteps;
+            uint32_t P = 0u;
+            if (active) {
+                const uint32_t a = va & ~(eu & pprev);
+                const uint32_t virt = a & el & pin & 1u;          // bit 0 continues a chain whose last was processed
+                const uint32_t cont = (a & el & (a << 1)) | virt;  // bit j continues from j-1
+                const uint32_t S = a & ~cont;                      // chain starts
+                const uint32_t ev = S & 0x55555555u, od = S & 0xaaaaaaaau;
+                const uint32_t fe = ev | (((cont + (ev << 1)) ^ cont) & cont);
+                const uint32_t fo = od | (((cont + ((od << 1) | virt)) ^ cont) & cont);
+                P = a & ((fe & 0x55555555u) | (fo & 0xaaaaaaaau));
+                pbits[(size_t)y * words + l] = P;
+                pprev = P;
+            }
+            pout = active ? (P >> 31) : 0u;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// k_descend: queryPointDepthSingle_func_subtree (traversal_functions.cpp:308-395)
+// read-only against the tree as it was before this frame.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void queue_leaf(const MapView &m, uint32_t leaf, uint32_t stamp, FrameCounters *fc)
+{
+    if (m.stamp[leaf] == stamp) return;
+    if (atomicExch(&m.stamp[leaf], stamp) != stamp) {
+        const uint32_t q = atomicAdd(&fc->nQueued, 1u);
+        m.queue[q] = leaf;
+    }
+}
+
+__global__ void k_descend(const uint16_t *__restrict__ depth, int W, int H, int words, float scaling,
+                          const float *__restrict__ qx, const float *__restrict__ qy, float t0, float t1,
+                          float t2, const uint32_t *__restrict__ pbits, MapView m, HashView hv, uint32_t n,
+                          int bw, uint32_t stamp, uint32_t *__restrict__ pl0, FrameCounters *fc)
+{
+    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+    if (x >= W) return;
+    if (!((pbits[(size_t)y * words + (x >> 5)] >> (x & 31)) & 1u)) return;
+    const uint32_t pix = (uint32_t)(y * W + x);
+    const float t[3] = {t0, t1, t2};
+    const float D = (float)depth[pix];
+    int v[3];
+    voxel_of(qx, qy, W, H, x, y, D, scaling, t, v);
+    // bounding-box tracking (traversal_wrapper_functions.cpp:642-676)
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        const int lo = v[k] - bw, hi = v[k] + bw;
+        if (lo < 0) atomicMin(&fc->bmin[k], lo);
+        if (hi >= (int)n) atomicMax(&fc->bmax[k], hi);
+    }
+    if (!(v[0] >= 0 && (uint32_t)(uint16_t)v[0] < n && v[1] >= 0 && (uint32_t)(uint16_t)v[1] < n && v[2] >= 0 &&
+          (uint32_t)(uint16_t)v[2] < n))
+        return;
+    const uint32_t X = (uint16_t)v[0], Y = (uint16_t)v[1], Z = (uint16_t)v[2];
+    const uint32_t target = leaf_scale_target(__fmul_rn(D, scaling));
+    uint32_t ox = 0, oy = 0, oz = 0, half = n >> 1, branch = 0, depthLvl = 1;
+    uint32_t pl = FF_BRANCHINIT, pld = 0;
+    bool missing = false;
+    uint32_t parentIdx = 0;  // tree index of the branch holding the current octant
+    while ((half >> 1) >= target) {
+        const uint32_t co = (X >= ox + half) * 1u + (Y >= oy + half) * 2u + (Z >= oz + half) * 4u;
+        const uint32_t nx = ox + half * (co & 1u), ny = oy + half * ((co >> 1) & 1u), nz = oz + half * ((co >> 2) & 1u);
+        if (!missing) {
+            const uint32_t ptr = branch + co + 8;
+            const uint32_t child = m.tree[ptr];
+            const uint32_t leaf = m.tree[ptr - 8];
+            if (leaf < m.nLeavesTotal) {
+                pl = leaf;
+                pld = depthLvl;
+                queue_leaf(m, leaf, stamp, fc);
+            }
+            if (child == FF_BRANCHINIT) {
+                missing = true;
+                hash_insert_min(hv, node_key(0, depthLvl, nx, ny, nz), pix, branch, fc);
+            } else {
+                parentIdx = child;
+            }
+            branch = child;
+        } else {
+            hash_insert_min(hv, node_key(0, depthLvl, nx, ny, nz), pix, FF_BRANCHINIT, fc);
+        }
+        ox = nx; oy = ny; oz = nz;
+        half >>= 1;
+        depthLvl++;
+    }
+    const uint32_t co = (X >= ox + half) * 1u + (Y >= oy + half) * 2u + (Z >= oz + half) * 4u;
+    const uint32_t nx = ox + half * (co & 1u), ny = oy + half * ((co >> 1) & 1u), nz = oz + half * ((co >> 2) & 1u);
+    if (!missing) {
+        const uint32_t leaf = m.tree[branch + co];
+        if (leaf >= FF_BRANCHINIT)
+            hash_insert_min(hv, node_key(1, depthLvl, nx, ny, nz), pix, branch, fc);
+        else if (leaf < m.nLeavesTotal)
+            queue_leaf(m, leaf, stamp, fc);
+    } else {
+        hash_insert_min(hv, node_key(1, depthLvl, nx, ny, nz), pix, FF_BRANCHINIT, fc);
+    }
+    (void)parentIdx;
+    pl0[2 * pix] = pl;
+    pl0[2 * pix + 1] = pld;
+}
+
+// per new node: its creating pixel counts it
+__global__ void k_resolve(HashView hv, const FrameCounters *fc, uint32_t *__restrict__ cntB,
+                          uint32_t *__restrict__ cntL, uint32_t *__restrict__ firstD)
+{
+    const uint32_t n = min(fc->listCount, hv.listCap);
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const uint32_t s = hv.list[i];
+        const uint64_t k = hv.keys[s];
+        const uint32_t pix = hv.minpix[s];
+        const uint32_t type = (uint32_t)(k >> 53) & 1u, d = (uint32_t)(k >> 48) & 31u;
+        if (type == 0) {
+            atomicAdd(&cntB[pix], 1u);
+            atomicMin(&firstD[pix], d);
+        } else {
+            cntL[pix] = 1u;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// exclusive scan of (cntB, cntL) over pixels, 4096 pixels per block
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint64_t wave_incl_scan(uint64_t v)
+{
+    const int lane = threadIdx.x & 63;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint64_t u = __shfl_up(v, o);
+        if (lane >= o) v += u;
+    }
+    return v;
+}
+
+__device__ uint64_t block_excl_scan(uint64_t v, uint64_t *total)
+{
+    __shared__ uint64_t ws[17];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    const uint64_t inc = wave_incl_scan(v);
+    if (lane == 63) ws[w] = inc;
+    __syncthreads();
+    if (w == 0) {
+        uint64_t t = lane < nw ? ws[lane] : 0;
+        const uint64_t ti = wave_incl_scan(t);
+        if (lane < nw) ws[lane] = ti - t;
+        if (lane == nw - 1) ws[16] = ti;
+    }
+    __syncthreads();
+    const uint64_t r = ws[w] + inc - v;
+    *total = ws[16];
+    __syncthreads();
+    return r;
+}
+
+__global__ __launch_bounds__(1024) void k_scan_a(const uint32_t *__restrict__ cntB, const uint32_t *__restrict__ cntL,
+                                                 uint32_t n, uint64_t *__restrict__ pref, uint64_t *__restrict__ bsum)
+{
+    const uint32_t base = blockIdx.x * 4096 + threadIdx.x * 4;
+    uint64_t e[4], s = 0;
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const uint32_t i = base + k;
+        e[k] = i < n ? ((uint64_t)cntB[i] | ((uint64_t)cntL[i] << 32)) : 0ull;
+        s += e[k];
+    }
+    uint64_t tot;
+    uint64_t ex = block_excl_scan(s, &tot);
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        const uint32_t i = base + k;
+        if (i < n) pref[i] = ex;
+        ex += e[k];
+    }
+    if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
+}
+
+__global__ __launch_bounds__(1024) void k_scan_b(uint64_t *__restrict__ bsum, uint32_t nb, FrameCounters *fc)
+{
+    const uint32_t per = (nb + 1023) / 1024;
+    uint64_t loc[8], s = 0;
+    for (uint32_t k = 0; k < per && k < 8; k++) {
+        const uint32_t i = threadIdx.x * per + k;
+        loc[k] = i < nb ? bsum[i] : 0ull;
+        s += loc[k];
+    }
+    uint64_t tot;
+    uint64_t ex = block_excl_scan(s, &tot);
+    for (uint32_t k = 0; k < per && k < 8; k++) {
+        const uint32_t i = threadIdx.x * per + k;
+        if (i < nb) bsum[i] = ex;
+        ex += loc[k];
+    }
+    if (threadIdx.x == 0) {
+        fc->newB = (uint32_t)tot;
+        fc->newL = (uint32_t)(tot >> 32);
+    }
+}
+
+__global__ void k_assign(HashView hv, const FrameCounters *fc, const uint64_t *__restrict__ pref,
+                         const uint64_t *__restrict__ bsum, const uint32_t *__restrict__ firstD,
+                         uint32_t nBranchesUsed, uint32_t nLeavesUsed)
+{
+    const uint32_t n = min(fc->listCount, hv.listCap);
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const uint32_t s = hv.list[i];
+        const uint64_t k = hv.keys[s];
+        const uint32_t pix = hv.minpix[s];
+        const uint32_t type = (uint32_t)(k >> 53) & 1u, d = (uint32_t)(k >> 48) & 31u;
+        const uint64_t p = pref[pix] + bsum[pix >> 12];
+        if (type == 0)
+            hv.assigned[s] = nBranchesUsed + 16u * ((uint32_t)p + d - firstD[pix]);
+        else
+            hv.assigned[s] = nLeavesUsed + (uint32_t)(p >> 32);
+    }
+}
+
+// parent branch's tree index of a node created this frame
+__device__ __forceinline__ uint32_t parent_branch(const HashView &hv, uint32_t aux, uint32_t d, uint32_t x,
+                                                  uint32_t y, uint32_t z, uint32_t n)
+{
+    if (aux != FF_BRANCHINIT) return aux;
+    const uint32_t psize = (n >> (d - 1));
+    const uint32_t msk = ~(psize - 1u);
+    const int s = hash_find(hv, node_key(0, d - 1, x & msk, y & msk, z & msk));
+    return s >= 0 ? hv.assigned[s] : FF_BRANCHINIT;
+}
+
+__global__ void k_link(HashView hv, MapView m, const FrameCounters *fc, const uint32_t *__restrict__ pl0,
+                       uint32_t n, uint32_t stamp, FrameCounters *fcw)
+{
+    const uint32_t cnt = min(fc->listCount, hv.listCap);
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < cnt; i += gridDim.x * blockDim.x) {
+        const uint32_t s = hv.list[i];
+        const uint64_t k = hv.keys[s];
+        const uint32_t type = (uint32_t)(k >> 53) & 1u, d = (uint32_t)(k >> 48) & 31u;
+        const uint32_t x = (uint32_t)(k >> 32) & 0xffffu, y = (uint32_t)(k >> 16) & 0xffffu, z = (uint32_t)k & 0xffffu;
+        const uint32_t size = n >> d;
+        const uint32_t co = ((x / size) & 1u) | (((y / size) & 1u) << 1) | (((z / size) & 1u) << 2);
+        const uint32_t pb = parent_branch(hv, hv.aux[s], d, x, y, z, n);
+        const uint32_t idx = hv.assigned[s];
+        const bool pb_ok = pb != FF_BRANCHINIT && pb + 16u <= m.nBranchesTotal;
+        if (type == 0) {
+            if (pb_ok && idx + 16u <= m.nBranchesTotal) m.tree[pb + co + 8] = idx;
+            else atomicOr(&fcw->flags, FF_FLAG_POOL_FULL);
+            continue;
+        }
+        if (!pb_ok) {
+            atomicOr(&fcw->flags, FF_FLAG_POOL_FULL);
+            continue;
+        }
+        m.tree[pb + co] = idx;
+        if (idx >= m.nLeavesTotal) continue;
+        // parent brick at the creating pixel's turn: the deepest brick on the
+        // path that existed before this frame or was created by an earlier pixel
+        const uint32_t pix = hv.minpix[s];
+        uint32_t pl = pl0[2 * pix], pld = pl0[2 * pix + 1];
+        for (uint32_t dd = d - 1; dd > pld && dd >= 1; dd--) {
+            const uint32_t as = n >> dd, am = ~(as - 1u);
+            const int e = hash_find(hv, node_key(1, dd, x & am, y & am, z & am));
+            if (e >= 0 && hv.minpix[e] < pix && hv.assigned[e] < m.nLeavesTotal) {
+                pl = hv.assigned[e];
+                break;
+            }
+        }
+        m.leafinfo[idx] = make_ushort4((unsigned short)x, (unsigned short)y, (unsigned short)z,
+                                       (unsigned short)(size / FF_BL));
+        m.parent[idx] = pl;
+        m.stamp[idx] = stamp;
+        const uint32_t q = atomicAdd(&fcw->nQueued, 1u);
+        m.queue[q] = idx;
+    }
+}
+
+__global__ void k_hash_clear(HashView hv, const FrameCounters *fc)
+{
+    const uint32_t n = min(fc->listCount, hv.listCap);
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const uint32_t s = hv.list[i];
+        hv.keys[s] = FF_HEMPTY;
+        hv.minpix[s] = 0xffffffffu;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// grow (geometryfusion_mipmap_cpu.cpp:3071-3319): shift child-branch pointers,
+// move the tree, shift brick positions
+// ---------------------------------------------------------------------------
+__global__ void k_shift_children(uint32_t *tree, uint32_t nBranchesUsed, uint32_t shift)
+{
+    const uint32_t b = (blockIdx.x * blockDim.x + threadIdx.x);
+    const uint32_t slot = (b / 8) * 16 + 8 + (b % 8);
+    if (slot < nBranchesUsed && tree[slot] != FF_BRANCHINIT) tree[slot] += shift;
+}
+
+__global__ void k_copy_u32(const uint32_t *src, uint32_t *dst, uint32_t n)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[i];
+}
+
+__global__ void k_fill_u32(uint32_t *dst, uint32_t v, uint32_t n)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = v;
+}
+
+__global__ void k_shift_leaves(ushort4 *info, uint32_t n, uint32_t ax, uint32_t ay, uint32_t az)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        ushort4 v = info[i];
+        v.x = (unsigned short)(v.x + ax);
+        v.y = (unsigned short)(v.y + ay);
+        v.z = (unsigned short)(v.z + az);
+        info[i] = v;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// k_sdf: update8AddLoopAVXSingleInteger (update_functions.cpp:1187-1444).
+// One wave per brick; lane t = 8 z + y owns one x-row of 8 voxels.  The row
+// origin is accumulated with the same sequential adds as the reference
+// (z steps, then y steps), lanes add k * d1j like the AVX lanes.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_sdf(const uint32_t *__restrict__ queue, const FrameCounters *fc,
+                                             MapView m, const uint16_t *__restrict__ depth,
+                                             const uint8_t *__restrict__ bgr, int W, int H, SdfParams P)
+{
+    __shared__ uint32_t tab[2048];
+    for (int i = threadIdx.x; i < 2048; i += blockDim.x) tab[i] = c_rcp_table[i];
+    __syncthreads();
+    const uint32_t nq = fc->nQueued;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int zr = lane >> 3, yr = lane & 7;
+    const float EPS = 0.005f;
+    for (uint32_t qi = blockIdx.x * 4 + wave; qi < nq; qi += gridDim.x * 4) {
+        const uint32_t leaf = queue[qi];
+        const ushort4 info = m.leafinfo[leaf];
+        const float ls = (float)info.w;
+        const float fls = rz_mul(ls, P.scale);
+        const float ofx = (float)info.x, ofy = (float)info.y, ofz = (float)info.z;
+        float O[3], d1[3], d2[3], d3[3];
+#pragma unroll
+        for (int r = 0; r < 3; r++) {
+            O[r] = rz_add(rz_mul(rz_add(rz_add(rz_mul(P.m[4 * r], ofx), rz_mul(P.m[4 * r + 1], ofy)),
+                                        rz_mul(P.m[4 * r + 2], ofz)),
+                                 P.scale),
+                          P.m[4 * r + 3]);
+            d1[r] = rz_mul(P.m[4 * r], fls);
+            d2[r] = rz_mul(P.m[4 * r + 1], fls);
+            d3[r] = rz_mul(P.m[4 * r + 2], fls);
+        }
+        const float td = rz_mul(P.dT, ls);
+        const float tw = rz_mul(rz_mul(3.0f, P.dT), ls);
+        const float rfall = rcp_x86(rz_sub(tw, EPS), tab);
+        float R[3] = {O[0], O[1], O[2]};
+        for (int k = 0; k < zr; k++) {
+            R[0] = rz_add(R[0], d3[0]); R[1] = rz_add(R[1], d3[1]); R[2] = rz_add(R[2], d3[2]);
+        }
+        for (int k = 0; k < yr; k++) {
+            R[0] = rz_add(R[0], d2[0]); R[1] = rz_add(R[1], d2[1]); R[2] = rz_add(R[2], d2[2]);
+        }
+        const size_t vb = (size_t)leaf * FF_BRICK + (size_t)lane * 8;
+        float dA[8], wA[8];
+        *(float4 *)&dA[0] = *(const float4 *)&m.dist[vb];
+        *(float4 *)&dA[4] = *(const float4 *)&m.dist[vb + 4];
+        *(float4 *)&wA[0] = *(const float4 *)&m.wgt[vb];
+        *(float4 *)&wA[4] = *(const float4 *)&m.wgt[vb + 4];
+        uint16_t cA[3][8];
+        uint16_t *cbase = m.col + (size_t)leaf * FF_BRICK * 3 + (size_t)lane * 8;
+        if (P.useColor) {
+#pragma unroll
+            for (int ch = 0; ch < 3; ch++) *(uint4 *)&cA[ch][0] = *(const uint4 *)(cbase + ch * FF_BRICK);
+        }
+        float dN[8], wN[8];
+        uint16_t cN[3][8];
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            float px, py, pz;
+            if (k == 0) {
+                px = rz_add(R[0], 0.0f); py = rz_add(R[1], 0.0f); pz = rz_add(R[2], 0.0f);
+            } else {
+                const float fk = (float)k;
+                px = rz_add(R[0], rz_mul(fk, d1[0]));
+                py = rz_add(R[1], rz_mul(fk, d1[1]));
+                pz = rz_add(R[2], rz_mul(fk, d1[2]));
+            }
+            const float len = rz_sqrt(rz_add(rz_add(rz_mul(px, px), rz_mul(py, py)), rz_mul(pz, pz)));
+            const float r = rcp_x86(pz, tab);
+            const int imx = cvt_x86(rz_add(P.cx, rz_mul(P.fx, rz_mul(px, r))));
+            const int imy = cvt_x86(rz_add(P.cy, rz_mul(P.fy, rz_mul(py, r))));
+            const int cyi = max(min(imy, H - 1), 0), cxi = max(min(imx, W - 1), 0);
+            const int ii = W * cyi + cxi;
+            const float h8 = (float)depth[ii];
+            const float h = rz_mul(P.scaling, h8);
+            const float dI = rz_sub(len, rz_mul(rz_mul(len, r), h));
+            const bool mask = (pz > 0.0f) && (imy < H) && (imy > -1) && (imx < W) && (imx > -1) && (h < P.maxcam);
+            const float front = (dI < EPS) ? 1.0f : 0.0f;
+            const float back = (dI >= EPS && dI < tw) ? 1.0f : 0.0f;
+            const float fall = rz_mul(rz_sub(tw, dI), rfall);
+            const float wI = mask ? rz_add(front, rz_mul(fall, back)) : 0.0f;
+            const float wn = rz_add(wA[k], wI);
+            const float f = rcp_x86(rz_add(wn, (wn <= 0.0f) ? 1.0f : 0.0f), tab);
+            const float mn = (td < dI) ? td : dI;
+            const float ntd = -td;
+            const float mx = (ntd > mn) ? ntd : mn;
+            const float add = mask ? rz_mul(mx, wI) : 0.0f;
+            dN[k] = rz_mul(rz_add(rz_mul(dA[k], wA[k]), add), f);
+            wN[k] = wn;
+            if (P.useColor) {
+                const float cmax = 65280.0f;
+#pragma unroll
+                for (int ch = 0; ch < 3; ch++) {
+                    const float inc = (float)bgr[ii * 3 + (2 - ch)] * 256.0f;
+                    float vv = rz_mul(rz_add(rz_mul((float)cA[ch][k], wA[k]), rz_mul(inc, wI)), f);
+                    vv = (vv < cmax) ? vv : cmax;
+                    cN[ch][k] = (uint16_t)cvt_x86(vv);
+                }
+            }
+        }
+        *(float4 *)&m.dist[vb] = *(const float4 *)&dN[0];
+        *(float4 *)&m.dist[vb + 4] = *(const float4 *)&dN[4];
+        *(float4 *)&m.wgt[vb] = *(const float4 *)&wN[0];
+        *(float4 *)&m.wgt[vb + 4] = *(const float4 *)&wN[4];
+        if (P.useColor) {
+#pragma unroll
+            for (int ch = 0; ch < 3; ch++) *(uint4 *)(cbase + ch * FF_BRICK) = *(const uint4 *)&cN[ch][0];
+        }
+    }
+}
+
+}  // namespace ff
+
+// ===========================================================================
+// host side
+// ===========================================================================
+namespace ff {
+
+static bool hip_ok(ff_ctx *c, hipError_t e, const char *what)
+{
+    if (e == hipSuccess) return true;
+    c->err = std::string(what) + ": " + hipGetErrorString(e);
+    return false;
+}
+
+#define HIPCHK(call)                                  \
+    do {                                              \
+        if (!hip_ok(c, (call), #call)) return FF_E_DEVICE; \
+    } while (0)
+
+static CamF make_p(const ff_ctx *c, const double R[9], const double t[3], const double K[4])
+{
+    // geometryfusion_mipmap_cpu.cpp:1617-1625
+    CamF p;
+    const double s = (double)c->scale;
+    for (int i = 0; i < 9; i++) p.r[i] = (float)(R[i] / s);
+    for (int i = 0; i < 3; i++) p.t[i] = (float)((t[i] - (double)c->offset[i]) / s);
+    p.fx = (float)K[0]; p.fy = (float)K[1]; p.cx = (float)K[2]; p.cy = (float)K[3];
+    return p;
+}
+
+static CamF make_pinv(const ff_ctx *c, const double R[9], const double t[3], const double K[4])
+{
+    // geometryfusion_mipmap_cpu.cpp:1771-1793 with CameraInfo::getExtrinsicInverse
+    // (camerautils.cpp:153-160)
+    CamF q;
+    double Rt[9], tt[3];
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) Rt[i * 3 + j] = R[j * 3 + i];
+    for (int i = 0; i < 3; i++)
+        tt[i] = ((-Rt[i * 3 + 0]) * t[0] + (-Rt[i * 3 + 1]) * t[1]) + (-Rt[i * 3 + 2]) * t[2];
+    for (int i = 0; i < 9; i++) q.r[i] = (float)Rt[i];
+    const double o0 = c->offset[0], o1 = c->offset[1], o2 = c->offset[2];
+    for (int i = 0; i < 3; i++)
+        q.t[i] = (float)(((Rt[i * 3 + 0] * o0 + Rt[i * 3 + 1] * o1) + Rt[i * 3 + 2] * o2) + tt[i]);
+    q.fx = (float)K[0]; q.fy = (float)K[1]; q.cx = (float)K[2]; q.cy = (float)K[3];
+    return q;
+}
+
+static int ensure_image(ff_ctx *c, int W, int H)
+{
+    if (W <= 0 || H <= 0 || W > 2048) {
+        c->err = "image width must be in [1, 2048]";
+        return FF_E_ARG;
+    }
+    if (c->capW >= W && c->capH >= H) return FF_OK;
+    const size_t npix = (size_t)W * H;
+    const int words = (W + 31) / 32;
+    c->freeFrameBuffers();
+    HIPCHK(hipMalloc(&c->d_depth, npix * 2));
+    HIPCHK(hipMalloc(&c->d_bgr, npix * 3));
+    HIPCHK(hipMalloc(&c->d_qx, sizeof(float) * 3 * W));
+    HIPCHK(hipMalloc(&c->d_qy, sizeof(float) * 3 * H));
+    HIPCHK(hipMalloc(&c->d_planes, sizeof(uint32_t) * 3 * words * H));
+    HIPCHK(hipMalloc(&c->d_pbits, sizeof(uint32_t) * words * H));
+    HIPCHK(hipMalloc(&c->d_cntB, sizeof(uint32_t) * npix));
+    HIPCHK(hipMalloc(&c->d_cntL, sizeof(uint32_t) * npix));
+    HIPCHK(hipMalloc(&c->d_firstD, sizeof(uint32_t) * npix));
+    HIPCHK(hipMalloc(&c->d_pl0, sizeof(uint32_t) * 2 * npix));
+    HIPCHK(hipMalloc(&c->d_pref, sizeof(uint64_t) * npix));
+    HIPCHK(hipMalloc(&c->d_bsum, sizeof(uint64_t) * ((npix + 4095) / 4096 + 1)));
+    HIPCHK(hipMemsetAsync(c->d_pbits, 0, sizeof(uint32_t) * words * H, c->stream));
+    c->capW = W;
+    c->capH = H;
+    return FF_OK;
+}
+
+static void launch_grow(ff_ctx *c, const int bmin[3], const int bmax[3])
+{
+    // grow (geometryfusion_mipmap_cpu.cpp:3071-3319), device part
+    auto hp2 = [](uint16_t v) {
+        uint16_t r = 1;
+        while (r * 2 <= v) r *= 2;
+        return r;
+    };
+    const int n = c->n;
+    uint16_t p[3], mm[3];
+    for (int k = 0; k < 3; k++) {
+        p[k] = (uint16_t)((bmax[k] >= n) * ((bmax[k] - n) / n + 1));
+        mm[k] = (uint16_t)((bmin[k] < 0) * ((-bmin[k]) / n + 1));
+    }
+    for (int k = 0; k < 3; k++) {
+        const uint16_t cut = p[k] & mm[k];
+        if (cut) {
+            if (mm[k] > p[k]) mm[k] = (uint16_t)(2 * hp2(mm[k]) + (mm[k] & (uint16_t)~cut));
+            else p[k] = (uint16_t)(2 * hp2(p[k]) + (p[k] & (uint16_t)~cut));
+        }
+    }
+    uint16_t maxAdd = 0;
+    for (int k = 0; k < 3; k++) {
+        maxAdd = std::max(maxAdd, hp2(mm[k]));
+        maxAdd = std::max(maxAdd, hp2(p[k]));
+    }
+    const uint32_t add = (uint32_t)maxAdd * 16u;
+    if (add > c->nBranchesTotal - c->nBranchesUsed) return;
+    const uint32_t used = c->nBranchesUsed;
+    const uint32_t nb = (used / 16) * 8;
+    k_shift_children<<<(nb + 255) / 256, 256, 0, c->stream>>>(c->d_tree, used, add);
+    // move [0, used) to [add, add + used) through a scratch copy
+    k_copy_u32<<<(used + 255) / 256, 256, 0, c->stream>>>(c->d_tree, c->d_treeTmp, used);
+    k_copy_u32<<<(used + 255) / 256, 256, 0, c->stream>>>(c->d_treeTmp, c->d_tree + add, used);
+    k_fill_u32<<<(add + 255) / 256, 256, 0, c->stream>>>(c->d_tree, FF_BRANCHINIT, add);
+    c->nBranchesUsed += add;
+    uint32_t index = (maxAdd - 1) * 16u;
+    uint16_t addv[3] = {0, 0, 0};
+    std::vector<std::pair<uint32_t, uint32_t>> links;
+    for (uint16_t level = 0; level < maxAdd; level++) {
+        const unsigned dec = 1u << level;
+        uint32_t co = 0;
+        for (int k = 0; k < 3; k++)
+            if ((mm[k] & dec) || (!(p[k] & dec) && (p[k] > mm[k]))) {
+                addv[k] += c->n;
+                co += 1u << k;
+            }
+        links.push_back({index + co + 8, index + 16});
+        c->n *= 2;
+        index -= 16;
+    }
+    for (auto &l : links)
+        hipMemcpyAsync(c->d_tree + l.first, &l.second, sizeof(uint32_t), hipMemcpyHostToDevice, c->stream);
+    hipStreamSynchronize(c->stream);
+    for (int k = 0; k < 3; k++) c->offset[k] -= (float)addv[k] * c->scale;
+    const uint32_t nl = std::min(c->nLeavesUsed, c->nLeavesTotal);
+    if (nl) k_shift_leaves<<<(nl + 255) / 256, 256, 0, c->stream>>>(c->d_leafinfo, nl, addv[0], addv[1], addv[2]);
+    c->grows++;
+}
+
+}  // namespace ff
+
+using namespace ff;
+
+void ff_ctx::freeFrameBuffers()
+{
+    hipFree(d_depth); hipFree(d_bgr); hipFree(d_qx); hipFree(d_qy); hipFree(d_planes); hipFree(d_pbits);
+    hipFree(d_cntB); hipFree(d_cntL); hipFree(d_firstD); hipFree(d_pl0); hipFree(d_pref); hipFree(d_bsum);
+    d_depth = nullptr; d_bgr = nullptr; d_qx = d_qy = nullptr; d_planes = d_pbits = nullptr;
+    d_cntB = d_cntL = d_firstD = d_pl0 = nullptr; d_pref = d_bsum = nullptr;
+    capW = capH = 0;
+}
+
+extern "C" {
+
+ff_ctx *ff_create(const ff_params *prm)
+{
+    if (!prm || !(prm->scale > 0.0f) || !(prm->distance_threshold > 0.0f)) return nullptr;
+    ff_ctx *c = new ff_ctx();
+    c->scale = prm->scale;
+    c->dT = prm->distance_threshold;
+    c->useColor = prm->use_color != 0;
+    c->incremental = prm->incremental_meshing != 0;
+    c->device = prm->device;
+    // _bandwidth(_distanceThreshold / _scale*BANDWIDTHFACTOR) (ctor, :425)
+    c->bandwidth = (uint16_t)((double)(prm->distance_threshold / prm->scale) * 1.0);
+    c->nLeavesTotal = prm->max_leaves ? prm->max_leaves : (1u << 20);
+    c->nBranchesTotal = prm->max_branches ? prm->max_branches : (1u << 25);
+    c->nBranchesUsed = 16;
+    if (hipSetDevice(c->device) != hipSuccess || hipStreamCreate(&c->stream) != hipSuccess) {
+        delete c;
+        return nullptr;
+    }
+    const size_t NL = c->nLeavesTotal;
+    bool ok = hipMalloc(&c->d_tree, sizeof(uint32_t) * c->nBranchesTotal) == hipSuccess &&
+              hipMalloc(&c->d_treeTmp, sizeof(uint32_t) * c->nBranchesTotal) == hipSuccess &&
+              hipMalloc(&c->d_leafinfo, sizeof(ushort4) * NL) == hipSuccess &&
+              hipMalloc(&c->d_parent, sizeof(uint32_t) * NL) == hipSuccess &&
+              hipMalloc(&c->d_stamp, sizeof(uint32_t) * NL) == hipSuccess &&
+              hipMalloc(&c->d_queue, sizeof(uint32_t) * NL) == hipSuccess &&
+              hipMalloc(&c->d_dist, sizeof(float) * NL * FF_BRICK) == hipSuccess &&
+              hipMalloc(&c->d_wgt, sizeof(float) * NL * FF_BRICK) == hipSuccess &&
+              hipMalloc(&c->d_fc, sizeof(FrameCounters)) == hipSuccess &&
+              hipHostMalloc(&c->h_fc, sizeof(FrameCounters), hipHostMallocDefault) == hipSuccess;
+    if (ok && c->useColor) ok = hipMalloc(&c->d_col, sizeof(uint16_t) * NL * FF_BRICK * 3) == hipSuccess;
+    const uint32_t hcap = 1u << 22;
+    ok = ok && hipMalloc(&c->hv.keys, sizeof(uint64_t) * hcap) == hipSuccess &&
+         hipMalloc(&c->hv.minpix, sizeof(uint32_t) * hcap) == hipSuccess &&
+         hipMalloc(&c->hv.aux, sizeof(uint32_t) * hcap) == hipSuccess &&
+         hipMalloc(&c->hv.assigned, sizeof(uint32_t) * hcap) == hipSuccess &&
+         hipMalloc(&c->hv.list, sizeof(uint32_t) * (hcap / 2)) == hipSuccess;
+    if (!ok) {
+        ff_destroy(c);
+        return nullptr;
+    }
+    c->hv.mask = hcap - 1;
+    c->hv.listCap = hcap / 2;
+    hipMemsetAsync(c->hv.keys, 0xff, sizeof(uint64_t) * hcap, c->stream);
+    hipMemsetAsync(c->hv.minpix, 0xff, sizeof(uint32_t) * hcap, c->stream);
+    hipMemsetAsync(c->d_tree, 0xff, sizeof(uint32_t) * c->nBranchesTotal, c->stream);  // then fix to BRANCHINIT
+    k_fill_u32<<<(c->nBranchesTotal + 255) / 256, 256, 0, c->stream>>>(c->d_tree, FF_BRANCHINIT, c->nBranchesTotal);
+    hipMemsetAsync(c->d_stamp, 0, sizeof(uint32_t) * NL, c->stream);
+    hipMemsetAsync(c->d_dist, 0, sizeof(float) * NL * FF_BRICK, c->stream);
+    hipMemsetAsync(c->d_wgt, 0, sizeof(float) * NL * FF_BRICK, c->stream);
+    if (c->d_col) hipMemsetAsync(c->d_col, 0, sizeof(uint16_t) * NL * FF_BRICK * 3, c->stream);
+    hipMemcpyToSymbolAsync(HIP_SYMBOL(c_rcp_table), FF_RCP_TABLE, sizeof(FF_RCP_TABLE), 0, hipMemcpyHostToDevice,
+                           c->stream);
+    if (hipStreamSynchronize(c->stream) != hipSuccess) {
+        ff_destroy(c);
+        return nullptr;
+    }
+    if (mesh_init(c) != FF_OK) {
+        ff_destroy(c);
+        return nullptr;
+    }
+    return c;
+}
+
+void ff_destroy(ff_ctx *c)
+{
+    if (!c) return;
+    hipSetDevice(c->device);
+    if (c->stream) hipStreamSynchronize(c->stream);
+    mesh_free(c);
+    c->freeFrameBuffers();
+    hipFree(c->d_tree); hipFree(c->d_treeTmp); hipFree(c->d_leafinfo); hipFree(c->d_parent);
+    hipFree(c->d_stamp); hipFree(c->d_queue); hipFree(c->d_dist); hipFree(c->d_wgt); hipFree(c->d_col);
+    hipFree(c->d_fc); hipHostFree(c->h_fc);
+    hipFree(c->hv.keys); hipFree(c->hv.minpix); hipFree(c->hv.aux); hipFree(c->hv.assigned); hipFree(c->hv.list);
+    if (c->stream) hipStreamDestroy(c->stream);
+    delete c;
+}
+
+const char *ff_last_error(const ff_ctx *c) { return c ? c->err.c_str() : "null context"; }
+
+void *ff_stream(ff_ctx *c) { return c ? (void *)c->stream : nullptr; }
+
+static int add_map_impl(ff_ctx *c, int W, int H, const double R[9], const double t[3], const double K[4],
+                        float scaling, float maxcam)
+{
+    CamF p = make_p(c, R, t, K);
+    FrameCounters *hf = c->h_fc;
+    const uint32_t stamp = ++c->stampCounter;
+    if (!c->n) {
+        // first frame: initial volume (setInitialVolume, :1186-1202)
+        memset(hf, 0, sizeof(FrameCounters));
+        for (int k = 0; k < 3; k++) { hf->bmin[k] = 1000000; hf->bmax[k] = -1000000; }
+        HIPCHK(hipMemcpyAsync(c->d_fc, hf, sizeof(FrameCounters), hipMemcpyHostToDevice, c->stream));
+        k_bbox_first<<<dim3((W + 255) / 256, H), 256, 0, c->stream>>>(c->d_depth, W, H, scaling, maxcam, p,
+                                                                       c->bandwidth, c->d_fc);
+        HIPCHK(hipMemcpyAsync(hf, c->d_fc, sizeof(FrameCounters), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));
+        const int *mn = hf->bmin, *mx = hf->bmax;
+        if (!(mn[0] >= mx[0] || mn[1] >= mx[1] || mn[2] >= mx[2])) {
+            uint16_t hi[3];
+            for (int k = 0; k < 3; k++) hi[k] = (uint16_t)(mx[k] - mn[k]);
+            const uint16_t high = std::max(std::max(hi[0], hi[1]), hi[2]);
+            uint16_t n;
+            for (n = 2 * FF_BL; n <= high; n *= 2) ;
+            c->n = n;
+            for (int k = 0; k < 3; k++) c->offset[k] = (float)(mn[k] - (n - hi[k]) / 2) * c->scale;
+            mesh_on_initial_volume(c);
+        }
+        p = make_p(c, R, t, K);
+    }
+    if (c->imgW != W || c->imgH != H) {
+        // column/row tables built with the frame that introduced the size (:1745-1764)
+        c->imgW = W; c->imgH = H;
+        c->pxp.resize(W); c->pyp.resize(H);
+        for (int x = 0; x < W; x++) c->pxp[x] = ((float)x - p.cx) / p.fx;
+        for (int y = 0; y < H; y++) c->pyp[y] = ((float)y - p.cy) / p.fy;
+    }
+    const CamF q = make_pinv(c, R, t, K);
+    std::vector<float> &qx = c->h_qx, &qy = c->h_qy;
+    qx.resize(3 * W); qy.resize(3 * H);
+    for (int x = 0; x < W; x++) {
+        qx[x] = p.r[0] * c->pxp[x];
+        qx[W + x] = p.r[3] * c->pxp[x];
+        qx[2 * W + x] = p.r[6] * c->pxp[x];
+    }
+    for (int y = 0; y < H; y++) {
+        qy[y] = p.r[1] * c->pyp[y] + p.r[2];
+        qy[H + y] = p.r[4] * c->pyp[y] + p.r[5];
+        qy[2 * H + y] = p.r[7] * c->pyp[y] + p.r[8];
+    }
+    HIPCHK(hipMemcpyAsync(c->d_qx, qx.data(), sizeof(float) * 3 * W, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(c->d_qy, qy.data(), sizeof(float) * 3 * H, hipMemcpyHostToDevice, c->stream));
+
+    memset(hf, 0, sizeof(FrameCounters));
+    for (int k = 0; k < 3; k++) { hf->bmin[k] = 0; hf->bmax[k] = (int)c->n - 1; }
+    HIPCHK(hipMemcpyAsync(c->d_fc, hf, sizeof(FrameCounters), hipMemcpyHostToDevice, c->stream));
+    const int words = (W + 31) / 32;
+    const uint32_t npix = (uint32_t)W * H;
+    if (c->n) {
+        k_planes<<<dim3((W + 63) / 64, H), 64, 0, c->stream>>>(c->d_depth, W, H, words, scaling, maxcam, c->d_qx,
+                                                              c->d_qy, p.t[0], p.t[1], p.t[2], c->d_planes,
+                                                              c->d_cntB, c->d_cntL, c->d_firstD);
+        k_dedup<16><<<1, 64, 0, c->stream>>>(c->d_planes, H, words, c->d_pbits);
+        MapView mv = c->mapView();
+        k_descend<<<dim3((W + 127) / 128, H), 128, 0, c->stream>>>(c->d_depth, W, H, words, scaling, c->d_qx,
+                                                                   c->d_qy, p.t[0], p.t[1], p.t[2], c->d_pbits,
+                                                                   mv, c->hv, c->n, c->bandwidth, stamp,
+                                                                   c->d_pl0, c->d_fc);
+        k_resolve<<<256, 256, 0, c->stream>>>(c->hv, c->d_fc, c->d_cntB, c->d_cntL, c->d_firstD);
+        const uint32_t nb = (npix + 4095) / 4096;
+        k_scan_a<<<nb, 1024, 0, c->stream>>>(c->d_cntB, c->d_cntL, npix, c->d_pref, c->d_bsum);
+        k_scan_b<<<1, 1024, 0, c->stream>>>(c->d_bsum, nb, c->d_fc);
+        k_assign<<<256, 256, 0, c->stream>>>(c->hv, c->d_fc, c->d_pref, c->d_bsum, c->d_firstD, c->nBranchesUsed,
+                                             c->nLeavesUsed);
+        k_link<<<256, 256, 0, c->stream>>>(c->hv, mv, c->d_fc, c->d_pl0, c->n, stamp, c->d_fc);
+        k_hash_clear<<<256, 256, 0, c->stream>>>(c->hv, c->d_fc);
+    } else {
+        HIPCHK(hipMemsetAsync(c->d_pbits, 0, sizeof(uint32_t) * words * H, c->stream));
+    }
+    HIPCHK(hipMemcpyAsync(hf, c->d_fc, sizeof(FrameCounters), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if (hf->flags & FF_FLAG_HASH_FULL) {
+        c->err = "per-frame node table overflow";
+        return FF_E_POOL_FULL;
+    }
+    if (hf->listCount > c->hv.listCap) {
+        c->err = "per-frame node list overflow";
+        return FF_E_POOL_FULL;
+    }
+    c->nBranchesUsed += 16u * hf->newB;
+    c->nLeavesUsed += hf->newL;
+    c->nQueued = hf->nQueued;
+    if (hf->flags & FF_FLAG_POOL_FULL) {
+        c->err = "branch pool exhausted";
+        c->incremental = false;
+    }
+    // grow (:1874-1877), before the SDF update which keeps this frame's pInv
+    if (c->nLeavesUsed < c->nLeavesTotal && c->nBranchesUsed < c->nBranchesTotal &&
+        (hf->bmin[0] < 0 || hf->bmin[1] < 0 || hf->bmin[2] < 0 || hf->bmax[0] >= (int)c->n ||
+         hf->bmax[1] >= (int)c->n || hf->bmax[2] >= (int)c->n)) {
+        int bmin[3] = {hf->bmin[0], hf->bmin[1], hf->bmin[2]}, bmax[3] = {hf->bmax[0], hf->bmax[1], hf->bmax[2]};
+        launch_grow(c, bmin, bmax);
+        mesh_on_grow(c);
+    }
+    if (c->nQueued) {
+        SdfParams sp;
+        for (int i = 0; i < 3; i++) {
+            sp.m[4 * i] = q.r[3 * i]; sp.m[4 * i + 1] = q.r[3 * i + 1]; sp.m[4 * i + 2] = q.r[3 * i + 2];
+            sp.m[4 * i + 3] = q.t[i];
+        }
+        sp.fx = q.fx; sp.fy = q.fy; sp.cx = q.cx; sp.cy = q.cy;
+        sp.scale = c->scale; sp.dT = c->dT; sp.scaling = scaling; sp.maxcam = maxcam;
+        sp.useColor = c->useColor ? 1 : 0;
+        const uint32_t blocks = std::min<uint32_t>((c->nQueued + 3) / 4, 8192u);
+        k_sdf<<<blocks, 256, 0, c->stream>>>(c->d_queue, c->d_fc, c->mapView(), c->d_depth, c->d_bgr, W, H, sp);
+        HIPCHK(hipGetLastError());
+    }
+    if (c->nLeavesUsed > c->nLeavesTotal || c->nBranchesUsed > c->nBranchesTotal) c->incremental = false;
+    c->frames++;
+    mesh_after_frame(c);
+    return (int)c->nQueued;
+}
+
+int ff_add_map(ff_ctx *c, const uint16_t *depth, size_t depth_stride, const uint8_t *bgr, size_t bgr_stride,
+               int W, int H, const double R[9], const double t[3], const double K[4], float scaling, float maxcam)
+{
+    if (!c || !depth || !R || !t || !K) return FF_E_ARG;
+    if (c->useColor && !bgr) {
+        c->err = "color is enabled but no bgr image was given";
+        return FF_E_ARG;
+    }
+    hipSetDevice(c->device);
+    int rc = ensure_image(c, W, H);
+    if (rc) return rc;
+    if (!depth_stride) depth_stride = (size_t)W * 2;
+    if (!bgr_stride) bgr_stride = (size_t)W * 3;
+    HIPCHK(hipMemcpy2DAsync(c->d_depth, (size_t)W * 2, depth, depth_stride, (size_t)W * 2, H, hipMemcpyHostToDevice,
+                            c->stream));
+    if (c->useColor)
+        HIPCHK(hipMemcpy2DAsync(c->d_bgr, (size_t)W * 3, bgr, bgr_stride, (size_t)W * 3, H, hipMemcpyHostToDevice,
+                                c->stream));
+    return add_map_impl(c, W, H, R, t, K, scaling, maxcam);
+}
+
+int ff_add_map_device(ff_ctx *c, const uint16_t *depth_dev, const uint8_t *bgr_dev, int W, int H, const double R[9],
+                      const double t[3], const double K[4], float scaling, float maxcam)
+{
+    if (!c || !depth_dev || !R || !t || !K) return FF_E_ARG;
+    if (c->useColor && !bgr_dev) {
+        c->err = "color is enabled but no bgr image was given";
+        return FF_E_ARG;
+    }
+    hipSetDevice(c->device);
+    int rc = ensure_image(c, W, H);
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(c->d_depth, depth_dev, (size_t)W * H * 2, hipMemcpyDeviceToDevice, c->stream));
+    if (c->useColor)
+        HIPCHK(hipMemcpyAsync(c->d_bgr, bgr_dev, (size_t)W * H * 3, hipMemcpyDeviceToDevice, c->stream));
+    return add_map_impl(c, W, H, R, t, K, scaling, maxcam);
+}
+
+int ff_get_counts(ff_ctx *c, ff_counts *o)
+{
+    if (!c || !o) return FF_E_ARG;
+    o->n_leaves_used = c->nLeavesUsed;
+    o->n_branches_used = c->nBranchesUsed;
+    o->n_queued = c->nQueued;
+    o->side = c->n;
+    for (int k = 0; k < 3; k++) o->offset[k] = c->offset[k];
+    o->frames = c->frames;
+    o->grows = c->grows;
+    return FF_OK;
+}
+
+int ff_copy_tree(ff_ctx *c, uint32_t *out, size_t n)
+{
+    if (!c || !out || n > c->nBranchesTotal) return FF_E_ARG;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    HIPCHK(hipMemcpy(out, c->d_tree, n * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    return FF_OK;
+}
+
+int ff_copy_leaves(ff_ctx *c, uint16_t *pos3, uint16_t *scale, uint32_t *parent, size_t n)
+{
+    if (!c || n > c->nLeavesTotal) return FF_E_ARG;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    std::vector<ushort4> info(n);
+    HIPCHK(hipMemcpy(info.data(), c->d_leafinfo, n * sizeof(ushort4), hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < n; i++) {
+        if (pos3) { pos3[3 * i] = info[i].x; pos3[3 * i + 1] = info[i].y; pos3[3 * i + 2] = info[i].z; }
+        if (scale) scale[i] = info[i].w;
+    }
+    if (parent) HIPCHK(hipMemcpy(parent, c->d_parent, n * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    return FF_OK;
+}
+
+int ff_copy_bricks(ff_ctx *c, const uint32_t *leaves, size_t n, float *dist, float *weight, uint16_t *color)
+{
+    if (!c || (!leaves && n)) return FF_E_ARG;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (size_t i = 0; i < n; i++) {
+        const uint32_t l = leaves[i];
+        if (l >= c->nLeavesTotal) return FF_E_ARG;
+        if (dist) HIPCHK(hipMemcpy(dist + i * FF_BRICK, c->d_dist + (size_t)l * FF_BRICK, FF_BRICK * 4, hipMemcpyDeviceToHost));
+        if (weight) HIPCHK(hipMemcpy(weight + i * FF_BRICK, c->d_wgt + (size_t)l * FF_BRICK, FF_BRICK * 4, hipMemcpyDeviceToHost));
+        if (color && c->d_col)
+            HIPCHK(hipMemcpy(color + i * FF_BRICK * 3, c->d_col + (size_t)l * FF_BRICK * 3, FF_BRICK * 6,
+                             hipMemcpyDeviceToHost));
+    }
+    return FF_OK;
+}
+
+int ff_copy_queue(ff_ctx *c, uint32_t *out, size_t cap)
+{
+    if (!c || !out) return FF_E_ARG;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    const size_t n = std::min<size_t>(cap, c->nQueued);
+    HIPCHK(hipMemcpy(out, c->d_queue, n * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    std::sort(out, out + n);
+    return (int)n;
+}
+
+int ff_copy_processed(ff_ctx *c, uint8_t *out, size_t n)
+{
+    if (!c || !out || n < (size_t)c->imgW * c->imgH) return FF_E_ARG;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    const int W = c->imgW, H = c->imgH, words = (W + 31) / 32;
+    std::vector<uint32_t> bits((size_t)words * H);
+    HIPCHK(hipMemcpy(bits.data(), c->d_pbits, bits.size() * 4, hipMemcpyDeviceToHost));
+    for (int y = 0; y < H; y++)
+        for (int x = 0; x < W; x++) out[(size_t)y * W + x] = (bits[(size_t)y * words + (x >> 5)] >> (x & 31)) & 1u;
+    return FF_OK;
+}
+
+}  // extern "C"

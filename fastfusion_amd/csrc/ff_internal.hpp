@@ -1,0 +1,107 @@
+// ff_internal.hpp -- context and shared types of the fusion library.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/ff_abi.h"
+
+#define FF_HEMPTY 0xffffffffffffffffull
+#define FF_FLAG_HASH_FULL 1u
+#define FF_FLAG_POOL_FULL 2u
+
+namespace ff {
+
+// camPamsFloat (src/fusion/definitions.h:70-89)
+struct CamF {
+    float r[9], t[3], fx, fy, cx, cy;
+};
+
+struct FrameCounters {
+    uint32_t nQueued, listCount, newB, newL;
+    int bmin[3], bmax[3];
+    uint32_t flags, pad;
+};
+
+// device views of the map arrays
+struct MapView {
+    uint32_t *tree;
+    ushort4 *leafinfo;  // x, y, z, scale  (_leafPos, _leafScale)
+    uint32_t *parent;   // _leafParent
+    uint32_t *stamp;    // frame stamp of the last queueing
+    uint32_t *queue;    // _leafNumberSurface
+    float *dist;        // 512 per brick
+    float *wgt;         // 512 per brick
+    uint16_t *col;      // R, G, B planes of 512 per brick
+    uint32_t nLeavesTotal, nBranchesTotal;
+};
+
+// nodes created during one frame
+struct HashView {
+    uint64_t *keys;
+    uint32_t *minpix, *aux, *assigned, *list;
+    uint32_t mask, listCap;
+};
+
+struct SdfParams {
+    float m[12];
+    float fx, fy, cx, cy, scale, dT, scaling, maxcam;
+    int useColor;
+};
+
+struct MeshState;
+
+}  // namespace ff
+
+struct ff_ctx {
+    // parameters
+    float scale = 0, dT = 0;
+    bool useColor = true, incremental = true;
+    int device = 0;
+    uint16_t bandwidth = 0;
+    // map scalars (host mirror)
+    uint16_t n = 0;
+    float offset[3] = {0, 0, 0};
+    uint32_t nLeavesTotal = 0, nBranchesTotal = 0, nLeavesUsed = 0, nBranchesUsed = 16, nQueued = 0;
+    uint32_t frames = 0, grows = 0, stampCounter = 0;
+    int imgW = 0, imgH = 0, capW = 0, capH = 0;
+    std::vector<float> pxp, pyp, h_qx, h_qy;
+    hipStream_t stream = nullptr;
+    // map arrays
+    uint32_t *d_tree = nullptr, *d_treeTmp = nullptr, *d_parent = nullptr, *d_stamp = nullptr, *d_queue = nullptr;
+    ushort4 *d_leafinfo = nullptr;
+    float *d_dist = nullptr, *d_wgt = nullptr;
+    uint16_t *d_col = nullptr;
+    // frame buffers
+    uint16_t *d_depth = nullptr;
+    uint8_t *d_bgr = nullptr;
+    float *d_qx = nullptr, *d_qy = nullptr;
+    uint32_t *d_planes = nullptr, *d_pbits = nullptr, *d_cntB = nullptr, *d_cntL = nullptr, *d_firstD = nullptr,
+             *d_pl0 = nullptr;
+    uint64_t *d_pref = nullptr, *d_bsum = nullptr;
+    ff::FrameCounters *d_fc = nullptr, *h_fc = nullptr;
+    ff::HashView hv{};
+    ff::MeshState *mesh = nullptr;
+    std::string err;
+
+    ff::MapView mapView() const
+    {
+        ff::MapView m;
+        m.tree = d_tree; m.leafinfo = d_leafinfo; m.parent = d_parent; m.stamp = d_stamp; m.queue = d_queue;
+        m.dist = d_dist; m.wgt = d_wgt; m.col = d_col;
+        m.nLeavesTotal = nLeavesTotal; m.nBranchesTotal = nBranchesTotal;
+        return m;
+    }
+    void freeFrameBuffers();
+};
+
+namespace ff {
+// meshing module hooks (ff_mesh.hip)
+int mesh_init(ff_ctx *c);
+void mesh_free(ff_ctx *c);
+void mesh_on_initial_volume(ff_ctx *c);
+void mesh_on_grow(ff_ctx *c);
+void mesh_after_frame(ff_ctx *c);
+}  // namespace ff

@@ -1,0 +1,121 @@
+/*
+ * ff_abi.h -- C ABI of the MI355X fusion library (libffgpu.so).
+ *
+ * The reference has no FFI layer: its integration path sits behind the C++
+ * class FusionMipMapCPU (src/fusion/geometryfusion_mipmap_cpu.hpp:111-163),
+ * a subclass of Fusion_AoS (src/fusion/geometryfusion_aos.hpp:20-48).  Each
+ * entry point below replaces one member of that class; the comment names it.
+ * All arguments are plain pointers and sizes; no exception crosses the ABI.
+ * One context is used by one thread at a time, like the reference's fusion
+ * thread.  Negative return values are FF_E* error codes; ff_last_error()
+ * returns the message of the last failure on that context.
+ */
+#ifndef FF_ABI_H
+#define FF_ABI_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FF_OK 0
+#define FF_E_ARG -1        /* invalid argument */
+#define FF_E_DEVICE -2     /* HIP runtime failure */
+#define FF_E_POOL_FULL -3  /* brick or branch pool exhausted */
+#define FF_E_STATE -4      /* call not valid in the current state */
+
+typedef struct ff_ctx ff_ctx;
+
+/* Constructor arguments of FusionMipMapCPU (geometryfusion_mipmap_cpu.hpp:114-116:
+ * scale, distanceThreshold, color) plus the brick-pool capacity that the
+ * reference derives from host memory (geometryfusion_mipmap_cpu.cpp:497-509)
+ * and the HIP device ordinal. */
+typedef struct ff_params {
+    float scale;              /* voxel edge in metres */
+    float distance_threshold; /* truncation in metres */
+    int use_color;            /* fuse BGR color (the reference's `color`) */
+    int incremental_meshing;  /* setIncrementalMeshing */
+    uint32_t max_leaves;      /* brick capacity (_nLeavesTotal); 0 = default */
+    uint32_t max_branches;    /* tree capacity in 32-bit slots (_nBranchesTotal); 0 = default */
+    int device;               /* HIP device ordinal */
+} ff_params;
+
+/* State scalars after a frame, for inspection and parity checks. */
+typedef struct ff_counts {
+    uint32_t n_leaves_used;   /* _nLeavesUsed */
+    uint32_t n_branches_used; /* _nBranchesUsed */
+    uint32_t n_queued;        /* _nLeavesQueuedSurface: bricks updated by the last frame */
+    uint32_t side;            /* _n: tree side length in voxels */
+    float offset[3];          /* _offset: world position of voxel (0,0,0) */
+    uint32_t frames;          /* _framesAdded */
+    uint32_t grows;           /* number of grow() calls so far */
+} ff_counts;
+
+/* FusionMipMapCPU::FusionMipMapCPU (geometryfusion_mipmap_cpu.cpp:420-640). */
+ff_ctx *ff_create(const ff_params *params);
+/* FusionMipMapCPU::~FusionMipMapCPU */
+void ff_destroy(ff_ctx *ctx);
+const char *ff_last_error(const ff_ctx *ctx);
+
+/* FusionMipMapCPU::addMap(const cv::Mat &depth, CameraInfo, const cv::Mat &rgb,
+ * float scaling, float maxcamdistance), u16 overload
+ * (geometryfusion_mipmap_cpu.cpp:1603-2034).  depth: CV_16UC1 host image with
+ * row stride depth_stride (bytes); bgr: CV_8UC3 host image, row stride
+ * bgr_stride (bytes), may be NULL when color is off; R, t: CameraInfo
+ * rotation (row-major) and translation, camera->world; K = {fx, fy, cx, cy}.
+ * Returns the number of bricks updated (the reference's return value) or FF_E*. */
+int ff_add_map(ff_ctx *ctx, const uint16_t *depth, size_t depth_stride, const uint8_t *bgr,
+               size_t bgr_stride, int width, int height, const double R[9], const double t[3],
+               const double K[4], float scaling, float maxcam);
+
+/* Same, with depth and bgr already resident in device memory (dense rows). */
+int ff_add_map_device(ff_ctx *ctx, const uint16_t *depth_dev, const uint8_t *bgr_dev, int width,
+                      int height, const double R[9], const double t[3], const double K[4],
+                      float scaling, float maxcam);
+
+/* FusionMipMapCPU::updateMeshes (geometryfusion_mipmap_cpu.cpp:2884-2954), run
+ * synchronously: re-meshes the cells touched by the frames since the last
+ * call.  Returns the number of cells re-meshed or FF_E*. */
+int ff_update_meshes(ff_ctx *ctx);
+
+/* getMeshInterleavedMarchingCubes (geometryfusion_mipmap_cpu.cpp:3359-3374):
+ * the MeshInterleaved layout -- vertices 3 x f32, colors 3 x u8 (R, G, B),
+ * faces 3 x u32 per triangle.  ff_mesh_sizes returns the counts; ff_copy_mesh
+ * fills caller buffers of those sizes (any pointer may be NULL to skip it). */
+int ff_mesh_sizes(ff_ctx *ctx, size_t *n_vertices, size_t *n_faces);
+int ff_copy_mesh(ff_ctx *ctx, float *xyz, uint8_t *rgb, uint32_t *faces);
+
+/* Per-cell output (getMeshCellsUpdate, geometryfusion_mipmap_cpu.cpp:3006-3047):
+ * the number of cells with a mesh, and for each cell its descriptor
+ * {type, ox, oy, oz, size} and its vertex and face counts.  Cells are listed
+ * in the order their geometry appears in ff_copy_mesh. */
+int ff_cell_count(ff_ctx *ctx, size_t *n_cells);
+int ff_copy_cells(ff_ctx *ctx, int32_t *desc5, uint32_t *n_vertices, uint32_t *n_faces);
+
+/* setIncrementalMeshing (geometryfusion_mipmap_cpu.hpp:131). */
+int ff_set_incremental_meshing(ff_ctx *ctx, int on);
+
+/* Map state, for inspection and parity (the reference's protected members
+ * _tree, _leafPos, _leafScale, _leafParent, _distance, _weights, _color). */
+int ff_get_counts(ff_ctx *ctx, ff_counts *out);
+int ff_copy_tree(ff_ctx *ctx, uint32_t *out, size_t n_slots);
+int ff_copy_leaves(ff_ctx *ctx, uint16_t *pos3, uint16_t *scale, uint32_t *parent, size_t n);
+/* Copies TSDF data of the given leaves: 512 distances, 512 weights and
+ * 3 x 512 color channels (R, G, B planes) per leaf; any output may be NULL. */
+int ff_copy_bricks(ff_ctx *ctx, const uint32_t *leaves, size_t n, float *dist, float *weight,
+                   uint16_t *color);
+/* The queue of bricks updated by the last frame (_leafNumberSurface), in
+ * ascending leaf order. */
+int ff_copy_queue(ff_ctx *ctx, uint32_t *out, size_t cap);
+/* De-dup result of the last frame: one byte per pixel, 1 = processed. */
+int ff_copy_processed(ff_ctx *ctx, uint8_t *out, size_t n);
+
+/* Device stream the context launches on (a hipStream_t), for callers that
+ * time or order work around it. */
+void *ff_stream(ff_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

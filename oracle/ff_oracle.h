@@ -1,0 +1,73 @@
+/*
+ * ff_oracle.h -- CPU restatement of fastfusion's per-frame integration path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the parity oracle: the tests,
+ * the smoke check and the CPU-baseline leg of the benchmark load it to check
+ * or time the reference algorithm.  The product (fastfusion_amd) never links
+ * or calls it.
+ *
+ * It restates, in plain scalar C, the reference's FusionMipMapCPU::addMap
+ * u16 path (src/fusion/geometryfusion_mipmap_cpu.cpp:1603-2034), its brick
+ * allocation (src/fusion/traversal_wrapper_functions.cpp:576-694,
+ * src/fusion/traversal_functions.cpp:185-211, 308-395), the volume set-up and
+ * growth (geometryfusion_mipmap_cpu.cpp:1186-1303, 3071-3319) and the AVX2
+ * voxel update (src/fusion/update_functions.cpp:1187-1444, 1633-1732), which
+ * runs in round-toward-zero with the RCPPS approximate reciprocal.
+ */
+#ifndef FF_ORACLE_H
+#define FF_ORACLE_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FFO_BRANCHINIT 4294967294u
+#define FFO_MAXLEAFNUMBER 10000000u
+
+typedef struct ffo_ctx ffo_ctx;
+
+ffo_ctx *ffo_create(float scale, float distance_threshold, int use_color,
+                    uint32_t n_leaves_total, uint32_t n_branches_total);
+void ffo_destroy(ffo_ctx *c);
+
+/* One frame: depth u16 (row stride w), BGR8 (row stride 3w), pose camera->world
+ * (R row-major, t), K = {fx, fy, cx, cy}.  Returns the number of queued bricks. */
+int ffo_add_map(ffo_ctx *c, const uint16_t *depth, const uint8_t *bgr, int w, int h,
+                const double R[9], const double t[3], const double K[4],
+                float scaling, float maxcam);
+
+/* state accessors */
+uint32_t ffo_n_leaves(const ffo_ctx *c);
+uint32_t ffo_n_branches(const ffo_ctx *c);
+uint32_t ffo_side(const ffo_ctx *c);
+void ffo_offset(const ffo_ctx *c, float out[3]);
+uint32_t ffo_n_queued(const ffo_ctx *c);
+const uint32_t *ffo_queue(const ffo_ctx *c);
+const uint32_t *ffo_tree(const ffo_ctx *c);
+const uint16_t *ffo_leaf_pos(const ffo_ctx *c);    /* 3 per leaf */
+const uint16_t *ffo_leaf_scale(const ffo_ctx *c);
+const uint32_t *ffo_leaf_parent(const ffo_ctx *c);
+const float *ffo_distance(const ffo_ctx *c);       /* 512 per leaf */
+const float *ffo_weights(const ffo_ctx *c);
+const uint16_t *ffo_color(const ffo_ctx *c);       /* 3 per voxel: R, G, B (x, y, z) */
+const uint8_t *ffo_processed(const ffo_ctx *c);    /* last frame's de-dup result, w*h */
+int ffo_grow_count(const ffo_ctx *c);
+
+/* unit-level entry points used by the tests */
+float ffo_rcp(float x);
+uint16_t ffo_leaf_scale_func(float depth);
+/* Processed-pixel mask of the de-dup recurrence for given per-pixel keys. */
+void ffo_dedup(const int32_t *vx, const int32_t *vy, const int32_t *vz, const uint8_t *valid,
+               int w, int h, uint8_t *processed);
+/* Updates one brick (512 voxels) in place with the reference's voxel kernel. */
+void ffo_update_brick(const uint16_t *depth, float scaling, float maxcam, const uint8_t *bgr,
+                      int w, int h, const float m[12], const float K[4], float scale,
+                      float distance_threshold, const uint16_t pos[3], uint16_t leaf_scale,
+                      float *dist, float *weight, uint16_t *color);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,159 @@
+"""ctypes binding of the oracle (oracle/lib/libfforacle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+CPU-baseline leg of bench.py, as the checker.  The product never imports it.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+_lib = None
+
+
+def load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    path = os.path.join(HERE, "lib", "libfforacle.so")
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} missing: run `python oracle/build.py`")
+    lib = ctypes.CDLL(path)
+    P, U32 = ctypes.c_void_p, ctypes.c_uint32
+    D = ctypes.POINTER(ctypes.c_double)
+    lib.ffo_create.restype = P
+    lib.ffo_create.argtypes = [ctypes.c_float, ctypes.c_float, ctypes.c_int, U32, U32]
+    lib.ffo_destroy.argtypes = [P]
+    lib.ffo_add_map.restype = ctypes.c_int
+    lib.ffo_add_map.argtypes = [P, P, P, ctypes.c_int, ctypes.c_int, D, D, D, ctypes.c_float, ctypes.c_float]
+    for name in ("ffo_n_leaves", "ffo_n_branches", "ffo_side", "ffo_n_queued"):
+        getattr(lib, name).restype = U32
+        getattr(lib, name).argtypes = [P]
+    lib.ffo_grow_count.restype = ctypes.c_int
+    lib.ffo_grow_count.argtypes = [P]
+    lib.ffo_offset.argtypes = [P, P]
+    for name in ("ffo_queue", "ffo_tree", "ffo_leaf_pos", "ffo_leaf_scale", "ffo_leaf_parent",
+                 "ffo_distance", "ffo_weights", "ffo_color", "ffo_processed"):
+        getattr(lib, name).restype = P
+        getattr(lib, name).argtypes = [P]
+    lib.ffo_rcp.restype = ctypes.c_float
+    lib.ffo_rcp.argtypes = [ctypes.c_float]
+    lib.ffo_leaf_scale_func.restype = ctypes.c_uint16
+    lib.ffo_leaf_scale_func.argtypes = [ctypes.c_float]
+    lib.ffo_dedup.argtypes = [P, P, P, P, ctypes.c_int, ctypes.c_int, P]
+    lib.ffo_update_brick.argtypes = [P, ctypes.c_float, ctypes.c_float, P, ctypes.c_int, ctypes.c_int, P, P,
+                                     ctypes.c_float, ctypes.c_float, P, ctypes.c_uint16, P, P, P]
+    if hasattr(lib, "ffo_mesh_update"):
+        _mesh_sigs(lib)
+    _lib = lib
+    return lib
+
+
+def _mesh_sigs(lib):
+    P = ctypes.c_void_p
+    lib.ffo_mesh_update.restype = ctypes.c_int
+    lib.ffo_mesh_update.argtypes = [P]
+    lib.ffo_mesh_counts.argtypes = [P, P]
+    lib.ffo_mesh_copy.argtypes = [P, P, P, P, P, P, P]
+
+
+def _vp(a):
+    return a.ctypes.data_as(ctypes.c_void_p) if a is not None else None
+
+
+def _dp(a):
+    return np.ascontiguousarray(a, np.float64).ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _arr(ptr, dtype, n):
+    if n == 0:
+        return np.zeros(0, dtype)
+    buf = (ctypes.c_char * (n * np.dtype(dtype).itemsize)).from_address(ptr)
+    return np.frombuffer(buf, dtype=dtype, count=n).copy()
+
+
+class Oracle:
+    """Sequential CPU restatement of FusionMipMapCPU's integration path."""
+
+    def __init__(self, scale, threshold, color=True, max_leaves=1 << 16, max_branches=1 << 21):
+        self.lib = load()
+        self.h = self.lib.ffo_create(float(scale), float(threshold), int(bool(color)), int(max_leaves),
+                                     int(max_branches))
+        self.color = color
+
+    def __del__(self):
+        try:
+            self.lib.ffo_destroy(self.h)
+        except Exception:
+            pass
+
+    def add_map(self, depth, R, t, K, bgr=None, scaling=1.0 / 5000.0, maxcam=10.0):
+        depth = np.ascontiguousarray(depth, np.uint16)
+        h, w = depth.shape
+        bgr = np.ascontiguousarray(bgr, np.uint8) if bgr is not None else np.zeros((h, w, 3), np.uint8)
+        return self.lib.ffo_add_map(self.h, _vp(depth), _vp(bgr), w, h, _dp(R), _dp(t), _dp(K),
+                                    float(scaling), float(maxcam))
+
+    def counts(self):
+        off = np.zeros(3, np.float32)
+        self.lib.ffo_offset(self.h, _vp(off))
+        return {"n_leaves_used": self.lib.ffo_n_leaves(self.h), "n_branches_used": self.lib.ffo_n_branches(self.h),
+                "n_queued": self.lib.ffo_n_queued(self.h), "side": self.lib.ffo_side(self.h), "offset": off,
+                "grows": self.lib.ffo_grow_count(self.h)}
+
+    def queue(self):
+        n = self.lib.ffo_n_queued(self.h)
+        return _arr(self.lib.ffo_queue(self.h), np.uint32, n)
+
+    def tree(self):
+        return _arr(self.lib.ffo_tree(self.h), np.uint32, self.lib.ffo_n_branches(self.h))
+
+    def leaves(self):
+        n = self.lib.ffo_n_leaves(self.h)
+        pos = _arr(self.lib.ffo_leaf_pos(self.h), np.uint16, 3 * n).reshape(n, 3)
+        sc = _arr(self.lib.ffo_leaf_scale(self.h), np.uint16, n)
+        par = _arr(self.lib.ffo_leaf_parent(self.h), np.uint32, n)
+        return pos, sc, par
+
+    def bricks(self, leaves):
+        n = self.lib.ffo_n_leaves(self.h)
+        d = _arr(self.lib.ffo_distance(self.h), np.float32, 512 * n).reshape(n, 512)
+        w = _arr(self.lib.ffo_weights(self.h), np.float32, 512 * n).reshape(n, 512)
+        c = _arr(self.lib.ffo_color(self.h), np.uint16, 3 * 512 * n).reshape(n, 512, 3).transpose(0, 2, 1)
+        idx = np.asarray(leaves, np.int64)
+        return d[idx], w[idx], c[idx]
+
+    def processed(self, w, h):
+        return _arr(self.lib.ffo_processed(self.h), np.uint8, w * h).reshape(h, w)
+
+
+def rcp(x):
+    lib = load()
+    return np.array([lib.ffo_rcp(float(v)) for v in np.asarray(x, np.float32).ravel()], np.float32)
+
+
+def leaf_scale_func(z):
+    return load().ffo_leaf_scale_func(float(z))
+
+
+def dedup(vx, vy, vz, valid):
+    h, w = valid.shape
+    out = np.zeros((h, w), np.uint8)
+    load().ffo_dedup(_vp(np.ascontiguousarray(vx, np.int32)), _vp(np.ascontiguousarray(vy, np.int32)),
+                     _vp(np.ascontiguousarray(vz, np.int32)), _vp(np.ascontiguousarray(valid, np.uint8)),
+                     w, h, _vp(out))
+    return out
+
+
+def update_brick(depth, bgr, scaling, maxcam, m12, K, scale, threshold, pos, leaf_scale, dist, weight, color):
+    """One brick through the oracle's voxel update; arrays are updated in place."""
+    depth = np.ascontiguousarray(depth, np.uint16)
+    h, w = depth.shape
+    bgr = np.ascontiguousarray(bgr, np.uint8)
+    m = np.ascontiguousarray(m12, np.float32)
+    Kf = np.ascontiguousarray(K, np.float32)
+    p = np.ascontiguousarray(pos, np.uint16)
+    load().ffo_update_brick(_vp(depth), float(scaling), float(maxcam), _vp(bgr), w, h, _vp(m), _vp(Kf),
+                            float(scale), float(threshold), _vp(p), int(leaf_scale), _vp(dist), _vp(weight),
+                            _vp(color))

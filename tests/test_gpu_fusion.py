@@ -1,0 +1,42 @@
+"""GPU parity of the integration path (allocation + SDF update) against the
+oracle, bit for bit: allocated tree, brick positions/scales/parents, queued
+set, de-dup mask, TSDF distances, weights and colors."""
+import numpy as np
+import pytest
+
+from fastfusion_amd import synth
+from tests.helpers import make_pair, run_pair
+
+pytestmark = pytest.mark.gpu
+
+
+def test_first_frame_c1_640():
+    sc, R, t, cfg = synth.config_c1(n_frames=200)
+    gpu, ora = make_pair(cfg["scale"], cfg["threshold"])
+    run_pair(sc, R, t, [0], cfg, gpu, ora)
+
+
+def test_c1_sequence_640():
+    sc, R, t, cfg = synth.config_c1(n_frames=200)
+    gpu, ora = make_pair(cfg["scale"], cfg["threshold"])
+    run_pair(sc, R, t, list(range(0, 24)), cfg, gpu, ora, check_every=4)
+
+
+def test_c2_walk_5mm():
+    sc, R, t, cfg = synth.config_c2(n_frames=60)
+    gpu, ora = make_pair(cfg["scale"], cfg["threshold"], max_leaves=1 << 18)
+    run_pair(sc, R, t, list(range(0, 30)), cfg, gpu, ora, check_every=5)
+
+
+def test_small_image_no_color_320():
+    sc, R, t, cfg = synth.config_c1(n_frames=50)
+    cfg = dict(cfg, w=320, h=240)
+    K = (262.5, 262.5, 159.5, 119.5)
+    gpu, ora = make_pair(cfg["scale"], cfg["threshold"], color=False)
+    from tests import helpers
+    old = helpers.K
+    helpers.K = K
+    try:
+        run_pair(sc, R, t, list(range(0, 8)), cfg, gpu, ora, check_every=2)
+    finally:
+        helpers.K = old
