@@ -59,6 +59,9 @@ ABI = {
     "ff_copy_queue": (ctypes.c_int, [_P, _P, _SZ]),
     "ff_copy_processed": (ctypes.c_int, [_P, _P, _SZ]),
     "ff_stream": (_P, [_P]),
+    "ff_set_timing": (ctypes.c_int, [_P, ctypes.c_int]),
+    "ff_get_timing": (ctypes.c_int, [_P, _P, _P, _P, _P]),
+    "ff_reset_timing": (ctypes.c_int, [_P]),
 }
 
 
@@ -245,3 +248,20 @@ class FusionMipMapGPU:
 
     def stream(self):
         return self._lib.ff_stream(self._ctx)
+
+    def set_timing(self, on=True):
+        self._check(self._lib.ff_set_timing(self._ctx, int(bool(on))))
+
+    def reset_timing(self):
+        self._check(self._lib.ff_reset_timing(self._ctx))
+
+    def timing(self):
+        """Per phase (allocation, sdf, mesh): ms, launches, algorithmic bytes, units."""
+        ms = np.zeros(3, np.float64)
+        calls = np.zeros(3, np.uint64)
+        by = np.zeros(3, np.float64)
+        un = np.zeros(3, np.uint64)
+        self._check(self._lib.ff_get_timing(self._ctx, _ptr(ms), _ptr(calls), _ptr(by), _ptr(un)))
+        names = ("alloc", "sdf", "mesh")
+        return {n: {"ms": float(ms[i]), "calls": int(calls[i]), "bytes": float(by[i]), "units": int(un[i])}
+                for i, n in enumerate(names)}

@@ -198,51 +198,45 @@ template <int PF>
 __global__ __launch_bounds__(64) void k_dedup(const uint32_t *__restrict__ planes, int H, int words,
                                               uint32_t *__restrict__ pbits)
 {
+    // Branch-free on purpose: loads come from clamped addresses and stores of
+    // inactive steps go to the sink row H, so the compiler keeps PF loads in
+    // flight instead of waiting for each one.
     const int l = threadIdx.x;
     const bool lane_ok = l < words;
+    const int lw = lane_ok ? l : words - 1;
     const int nsteps = H + words - 1;
     uint32_t ra[PF], rb[PF], rc[PF];
 #pragma unroll
     for (int k = 0; k < PF; k++) {
-        const int y = k - l;
-        if (lane_ok && y >= 0 && y < H) {
-            const uint32_t *p = planes + ((size_t)y * words + l) * 3;
-            ra[k] = p[0]; rb[k] = p[1]; rc[k] = p[2];
-        } else {
-            ra[k] = rb[k] = rc[k] = 0u;
-        }
+        const int y = min(max(k - l, 0), H - 1);
+        const uint32_t *p = planes + ((size_t)y * words + lw) * 3;
+        ra[k] = p[0]; rb[k] = p[1]; rc[k] = p[2];
     }
     uint32_t pprev = 0u, pout = 0u;
     for (int s0 = 0; s0 < nsteps; s0 += PF) {
 #pragma unroll
         for (int k = 0; k < PF; k++) {
             const int s = s0 + k, y = s - l;
-            const uint32_t va = ra[k], el = rb[k], eu = rc[k];
+            const bool active = lane_ok && y >= 0 && y < H && s < nsteps;
+            const uint32_t va = active ? ra[k] : 0u, el = rb[k], eu = rc[k];
             {
-                const int yn = y + PF;
-                if (lane_ok && yn >= 0 && yn < H) {
-                    const uint32_t *p = planes + ((size_t)yn * words + l) * 3;
-                    ra[k] = p[0]; rb[k] = p[1]; rc[k] = p[2];
-                } else {
-                    ra[k] = rb[k] = rc[k] = 0u;
-                }
+                const int yn = min(max(y + PF, 0), H - 1);
+                const uint32_t *p = planes + ((size_t)yn * words + lw) * 3;
+                ra[k] = p[0]; rb[k] = p[1]; rc[k] = p[2];
             }
             uint32_t pin = __shfl_up(pout, 1);
-            if (l == 0) pin = 0u;
-            const bool active = lane_ok && y >= 0 && y < H && s < nsteps;
-            uint32_t P = 0u;
-            if (active) {
-                const uint32_t a = va & ~(eu & pprev);
-                const uint32_t virt = a & el & pin & 1u;          // bit 0 continues a chain whose last was processed
-                const uint32_t cont = (a & el & (a << 1)) | virt;  // bit j continues from j-1
-                const uint32_t S = a & ~cont;                      // chain starts
-                const uint32_t ev = S & 0x55555555u, od = S & 0xaaaaaaaau;
-                const uint32_t fe = ev | (((cont + (ev << 1)) ^ cont) & cont);
-                const uint32_t fo = od | (((cont + ((od << 1) | virt)) ^ cont) & cont);
-                P = a & ((fe & 0x55555555u) | (fo & 0xaaaaaaaau));
-                pbits[(size_t)y * words + l] = P;
-                pprev = P;
-            }
+            pin = (l == 0) ? 0u : pin;
+            const uint32_t a = va & ~(eu & pprev);
+            const uint32_t virt = a & el & pin & 1u;          // bit 0 continues a chain whose last was processed
+            const uint32_t cont = (a & el & (a << 1)) | virt;  // bit j continues from j-1
+            const uint32_t S = a & ~cont;                      // chain starts
+            const uint32_t ev = S & 0x55555555u, od = S & 0xaaaaaaaau;
+            const uint32_t fe = ev | (((cont + (ev << 1)) ^ cont) & cont);
+            const uint32_t fo = od | (((cont + ((od << 1) | virt)) ^ cont) & cont);
+            const uint32_t P = a & ((fe & 0x55555555u) | (fo & 0xaaaaaaaau));
+            const size_t dst = active ? (size_t)y * words + l : (size_t)H * words + lw;
+            pbits[dst] = P;
+            pprev = active ? P : pprev;
             pout = active ? (P >> 31) : 0u;
         }
     }
@@ -267,25 +261,55 @@ __global__ void k_descend(const uint16_t *__restrict__ depth, int W, int H, int 
                           int bw, uint32_t stamp, uint32_t *__restrict__ pl0, FrameCounters *fc)
 {
     const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
-    if (x >= W) return;
-    if (!((pbits[(size_t)y * words + (x >> 5)] >> (x & 31)) & 1u)) return;
+    const int lane = threadIdx.x & 63;
+    const bool proc = x < W && ((pbits[(size_t)y * words + (x >> 5)] >> (x & 31)) & 1u);
+    const uint64_t pb = __ballot(proc);
+    if (pb == 0) return;  // wave-uniform
+    if (lane == 0) atomicAdd(&fc->nProcessed, (uint32_t)__popcll(pb));
     const uint32_t pix = (uint32_t)(y * W + x);
     const float t[3] = {t0, t1, t2};
-    const float D = (float)depth[pix];
-    int v[3];
-    voxel_of(qx, qy, W, H, x, y, D, scaling, t, v);
-    // bounding-box tracking (traversal_wrapper_functions.cpp:642-676)
+    int v[3] = {0, 0, 0};
+    float D = 0.0f;
+    if (proc) {
+        D = (float)depth[pix];
+        voxel_of(qx, qy, W, H, x, y, D, scaling, t, v);
+    }
+    // bounding-box tracking (traversal_wrapper_functions.cpp:642-676), one
+    // atomic per wave and axis
 #pragma unroll
     for (int k = 0; k < 3; k++) {
-        const int lo = v[k] - bw, hi = v[k] + bw;
-        if (lo < 0) atomicMin(&fc->bmin[k], lo);
-        if (hi >= (int)n) atomicMax(&fc->bmax[k], hi);
+        int lo = proc ? v[k] - bw : 0, hi = proc ? v[k] + bw : 0;
+        for (int o = 32; o > 0; o >>= 1) {
+            lo = min(lo, __shfl_xor(lo, o));
+            hi = max(hi, __shfl_xor(hi, o));
+        }
+        if (lane == 0) {
+            if (lo < 0) atomicMin(&fc->bmin[k], lo);
+            if (hi >= (int)n) atomicMax(&fc->bmax[k], hi);
+        }
     }
-    if (!(v[0] >= 0 && (uint32_t)(uint16_t)v[0] < n && v[1] >= 0 && (uint32_t)(uint16_t)v[1] < n && v[2] >= 0 &&
-          (uint32_t)(uint16_t)v[2] < n))
-        return;
+    const bool inb = proc && v[0] >= 0 && (uint32_t)(uint16_t)v[0] < n && v[1] >= 0 &&
+                     (uint32_t)(uint16_t)v[1] < n && v[2] >= 0 && (uint32_t)(uint16_t)v[2] < n;
     const uint32_t X = (uint16_t)v[0], Y = (uint16_t)v[1], Z = (uint16_t)v[2];
-    const uint32_t target = leaf_scale_target(__fmul_rn(D, scaling));
+    const uint32_t target = inb ? leaf_scale_target(__fmul_rn(D, scaling)) : 0u;
+    // target node of this pixel; pixels of a wave with the same target node
+    // walk the same path, so only the lowest lane (smallest pixel index) does
+    uint32_t tsize = n >> 1, tdepth = 1;
+    while ((tsize >> 1) >= target && tsize > 1) {
+        tsize >>= 1;
+        tdepth++;
+    }
+    const uint64_t tkey = inb ? node_key(1, tdepth, X & ~(tsize - 1u), Y & ~(tsize - 1u), Z & ~(tsize - 1u)) : 0ull;
+    bool leader = false;
+    uint64_t todo = __ballot(inb);
+    while (todo) {
+        const int l0 = __ffsll((long long)todo) - 1;
+        const uint64_t k0 = __shfl(tkey, l0);
+        const uint64_t same = __ballot(inb && tkey == k0);
+        if (lane == l0) leader = true;
+        todo &= ~same;
+    }
+    if (!leader) return;
     uint32_t ox = 0, oy = 0, oz = 0, half = n >> 1, branch = 0, depthLvl = 1;
     uint32_t pl = FF_BRANCHINIT, pld = 0;
     bool missing = false;
@@ -551,113 +575,89 @@ __global__ void k_shift_leaves(ushort4 *info, uint32_t n, uint32_t ax, uint32_t 
 
 // ---------------------------------------------------------------------------
 // k_sdf: update8AddLoopAVXSingleInteger (update_functions.cpp:1187-1444).
-// One wave per brick; lane t = 8 z + y owns one x-row of 8 voxels.  The row
-// origin is accumulated with the same sequential adds as the reference
-// (z steps, then y steps), lanes add k * d1j like the AVX lanes.
+// One 512-thread block per queued brick, one voxel per thread.  Each thread
+// rebuilds its row origin with the same sequential adds as the reference
+// (z steps, then y steps) and adds k * d1j like the AVX lane k.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_sdf(const uint32_t *__restrict__ queue, const FrameCounters *fc,
-                                             MapView m, const uint16_t *__restrict__ depth,
+__global__ __launch_bounds__(512) void k_sdf(const uint32_t *__restrict__ queue, MapView m,
+                                             const uint16_t *__restrict__ depth,
                                              const uint8_t *__restrict__ bgr, int W, int H, SdfParams P)
 {
     __shared__ uint32_t tab[2048];
     for (int i = threadIdx.x; i < 2048; i += blockDim.x) tab[i] = c_rcp_table[i];
-    __syncthreads();
-    const uint32_t nq = fc->nQueued;
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int zr = lane >> 3, yr = lane & 7;
+    const int tid = threadIdx.x;
+    const int zr = tid >> 6, yr = (tid >> 3) & 7, k = tid & 7;
     const float EPS = 0.005f;
-    for (uint32_t qi = blockIdx.x * 4 + wave; qi < nq; qi += gridDim.x * 4) {
-        const uint32_t leaf = queue[qi];
-        const ushort4 info = m.leafinfo[leaf];
-        const float ls = (float)info.w;
-        const float fls = rz_mul(ls, P.scale);
-        const float ofx = (float)info.x, ofy = (float)info.y, ofz = (float)info.z;
-        float O[3], d1[3], d2[3], d3[3];
+    const uint32_t leaf = queue[blockIdx.x];
+    const ushort4 info = m.leafinfo[leaf];
+    const size_t vi = (size_t)leaf * FF_BRICK + tid;
+    const float wA = m.wgt[vi], dA = m.dist[vi];
+    uint16_t cA[3] = {0, 0, 0};
+    uint16_t *cbase = m.col + (size_t)leaf * FF_BRICK * 3 + tid;
+    if (P.useColor) {
+        cA[0] = cbase[0];
+        cA[1] = cbase[FF_BRICK];
+        cA[2] = cbase[2 * FF_BRICK];
+    }
+    const float ls = (float)info.w;
+    const float fls = rz_mul(ls, P.scale);
+    const float ofx = (float)info.x, ofy = (float)info.y, ofz = (float)info.z;
+    float R[3], d1[3], d2[3], d3[3];
 #pragma unroll
-        for (int r = 0; r < 3; r++) {
-            O[r] = rz_add(rz_mul(rz_add(rz_add(rz_mul(P.m[4 * r], ofx), rz_mul(P.m[4 * r + 1], ofy)),
-                                        rz_mul(P.m[4 * r + 2], ofz)),
-                                 P.scale),
-                          P.m[4 * r + 3]);
-            d1[r] = rz_mul(P.m[4 * r], fls);
-            d2[r] = rz_mul(P.m[4 * r + 1], fls);
-            d3[r] = rz_mul(P.m[4 * r + 2], fls);
-        }
-        const float td = rz_mul(P.dT, ls);
-        const float tw = rz_mul(rz_mul(3.0f, P.dT), ls);
-        const float rfall = rcp_x86(rz_sub(tw, EPS), tab);
-        float R[3] = {O[0], O[1], O[2]};
-        for (int k = 0; k < zr; k++) {
-            R[0] = rz_add(R[0], d3[0]); R[1] = rz_add(R[1], d3[1]); R[2] = rz_add(R[2], d3[2]);
-        }
-        for (int k = 0; k < yr; k++) {
-            R[0] = rz_add(R[0], d2[0]); R[1] = rz_add(R[1], d2[1]); R[2] = rz_add(R[2], d2[2]);
-        }
-        const size_t vb = (size_t)leaf * FF_BRICK + (size_t)lane * 8;
-        float dA[8], wA[8];
-        *(float4 *)&dA[0] = *(const float4 *)&m.dist[vb];
-        *(float4 *)&dA[4] = *(const float4 *)&m.dist[vb + 4];
-        *(float4 *)&wA[0] = *(const float4 *)&m.wgt[vb];
-        *(float4 *)&wA[4] = *(const float4 *)&m.wgt[vb + 4];
-        uint16_t cA[3][8];
-        uint16_t *cbase = m.col + (size_t)leaf * FF_BRICK * 3 + (size_t)lane * 8;
-        if (P.useColor) {
+    for (int r = 0; r < 3; r++) {
+        R[r] = rz_add(rz_mul(rz_add(rz_add(rz_mul(P.m[4 * r], ofx), rz_mul(P.m[4 * r + 1], ofy)),
+                                    rz_mul(P.m[4 * r + 2], ofz)),
+                             P.scale),
+                      P.m[4 * r + 3]);
+        d1[r] = rz_mul(P.m[4 * r], fls);
+        d2[r] = rz_mul(P.m[4 * r + 1], fls);
+        d3[r] = rz_mul(P.m[4 * r + 2], fls);
+    }
+    const float td = rz_mul(P.dT, ls);
+    const float tw = rz_mul(rz_mul(3.0f, P.dT), ls);
+    for (int i = 0; i < zr; i++) {
+        R[0] = rz_add(R[0], d3[0]); R[1] = rz_add(R[1], d3[1]); R[2] = rz_add(R[2], d3[2]);
+    }
+    for (int i = 0; i < yr; i++) {
+        R[0] = rz_add(R[0], d2[0]); R[1] = rz_add(R[1], d2[1]); R[2] = rz_add(R[2], d2[2]);
+    }
+    const float fk = (float)k;
+    // lane 0 adds +0.0f like _mm256_setr_ps(0.0f, ...)
+    const float px = rz_add(R[0], k ? rz_mul(fk, d1[0]) : 0.0f);
+    const float py = rz_add(R[1], k ? rz_mul(fk, d1[1]) : 0.0f);
+    const float pz = rz_add(R[2], k ? rz_mul(fk, d1[2]) : 0.0f);
+    __syncthreads();  // rcp table ready
+    const float rfall = rcp_x86(rz_sub(tw, EPS), tab);
+    const float len = rz_sqrt(rz_add(rz_add(rz_mul(px, px), rz_mul(py, py)), rz_mul(pz, pz)));
+    const float r = rcp_x86(pz, tab);
+    const int imx = cvt_x86(rz_add(P.cx, rz_mul(P.fx, rz_mul(px, r))));
+    const int imy = cvt_x86(rz_add(P.cy, rz_mul(P.fy, rz_mul(py, r))));
+    const int cyi = max(min(imy, H - 1), 0), cxi = max(min(imx, W - 1), 0);
+    const int ii = W * cyi + cxi;
+    const float h8 = (float)depth[ii];
+    const float h = rz_mul(P.scaling, h8);
+    const float dI = rz_sub(len, rz_mul(rz_mul(len, r), h));
+    const bool mask = (pz > 0.0f) && (imy < H) && (imy > -1) && (imx < W) && (imx > -1) && (h < P.maxcam);
+    const float front = (dI < EPS) ? 1.0f : 0.0f;
+    const float back = (dI >= EPS && dI < tw) ? 1.0f : 0.0f;
+    const float fall = rz_mul(rz_sub(tw, dI), rfall);
+    const float wI = mask ? rz_add(front, rz_mul(fall, back)) : 0.0f;
+    const float wn = rz_add(wA, wI);
+    const float f = rcp_x86(rz_add(wn, (wn <= 0.0f) ? 1.0f : 0.0f), tab);
+    const float mn = (td < dI) ? td : dI;                 // _mm256_min_ps(td, dI)
+    const float ntd = -td;
+    const float mx = (ntd > mn) ? ntd : mn;               // _mm256_max_ps(-td, mn)
+    const float add = mask ? rz_mul(mx, wI) : 0.0f;
+    m.dist[vi] = rz_mul(rz_add(rz_mul(dA, wA), add), f);
+    m.wgt[vi] = wn;
+    if (P.useColor) {
+        const float cmax = 65280.0f;
 #pragma unroll
-            for (int ch = 0; ch < 3; ch++) *(uint4 *)&cA[ch][0] = *(const uint4 *)(cbase + ch * FF_BRICK);
-        }
-        float dN[8], wN[8];
-        uint16_t cN[3][8];
-#pragma unroll
-        for (int k = 0; k < 8; k++) {
-            float px, py, pz;
-            if (k == 0) {
-                px = rz_add(R[0], 0.0f); py = rz_add(R[1], 0.0f); pz = rz_add(R[2], 0.0f);
-            } else {
-                const float fk = (float)k;
-                px = rz_add(R[0], rz_mul(fk, d1[0]));
-                py = rz_add(R[1], rz_mul(fk, d1[1]));
-                pz = rz_add(R[2], rz_mul(fk, d1[2]));
-            }
-            const float len = rz_sqrt(rz_add(rz_add(rz_mul(px, px), rz_mul(py, py)), rz_mul(pz, pz)));
-            const float r = rcp_x86(pz, tab);
-            const int imx = cvt_x86(rz_add(P.cx, rz_mul(P.fx, rz_mul(px, r))));
-            const int imy = cvt_x86(rz_add(P.cy, rz_mul(P.fy, rz_mul(py, r))));
-            const int cyi = max(min(imy, H - 1), 0), cxi = max(min(imx, W - 1), 0);
-            const int ii = W * cyi + cxi;
-            const float h8 = (float)depth[ii];
-            const float h = rz_mul(P.scaling, h8);
-            const float dI = rz_sub(len, rz_mul(rz_mul(len, r), h));
-            const bool mask = (pz > 0.0f) && (imy < H) && (imy > -1) && (imx < W) && (imx > -1) && (h < P.maxcam);
-            const float front = (dI < EPS) ? 1.0f : 0.0f;
-            const float back = (dI >= EPS && dI < tw) ? 1.0f : 0.0f;
-            const float fall = rz_mul(rz_sub(tw, dI), rfall);
-            const float wI = mask ? rz_add(front, rz_mul(fall, back)) : 0.0f;
-            const float wn = rz_add(wA[k], wI);
-            const float f = rcp_x86(rz_add(wn, (wn <= 0.0f) ? 1.0f : 0.0f), tab);
-            const float mn = (td < dI) ? td : dI;
-            const float ntd = -td;
-            const float mx = (ntd > mn) ? ntd : mn;
-            const float add = mask ? rz_mul(mx, wI) : 0.0f;
-            dN[k] = rz_mul(rz_add(rz_mul(dA[k], wA[k]), add), f);
-            wN[k] = wn;
-            if (P.useColor) {
-                const float cmax = 65280.0f;
-#pragma unroll
-                for (int ch = 0; ch < 3; ch++) {
-                    const float inc = (float)bgr[ii * 3 + (2 - ch)] * 256.0f;
-                    float vv = rz_mul(rz_add(rz_mul((float)cA[ch][k], wA[k]), rz_mul(inc, wI)), f);
-                    vv = (vv < cmax) ? vv : cmax;
-                    cN[ch][k] = (uint16_t)cvt_x86(vv);
-                }
-            }
-        }
-        *(float4 *)&m.dist[vb] = *(const float4 *)&dN[0];
-        *(float4 *)&m.dist[vb + 4] = *(const float4 *)&dN[4];
-        *(float4 *)&m.wgt[vb] = *(const float4 *)&wN[0];
-        *(float4 *)&m.wgt[vb + 4] = *(const float4 *)&wN[4];
-        if (P.useColor) {
-#pragma unroll
-            for (int ch = 0; ch < 3; ch++) *(uint4 *)(cbase + ch * FF_BRICK) = *(const uint4 *)&cN[ch][0];
+        for (int ch = 0; ch < 3; ch++) {
+            const float inc = (float)bgr[ii * 3 + (2 - ch)] * 256.0f;
+            float vv = rz_mul(rz_add(rz_mul((float)cA[ch], wA), rz_mul(inc, wI)), f);
+            vv = (vv < cmax) ? vv : cmax;
+            cbase[ch * FF_BRICK] = (uint16_t)cvt_x86(vv);
         }
     }
 }
@@ -725,7 +725,7 @@ static int ensure_image(ff_ctx *c, int W, int H)
     HIPCHK(hipMalloc(&c->d_qx, sizeof(float) * 3 * W));
     HIPCHK(hipMalloc(&c->d_qy, sizeof(float) * 3 * H));
     HIPCHK(hipMalloc(&c->d_planes, sizeof(uint32_t) * 3 * words * H));
-    HIPCHK(hipMalloc(&c->d_pbits, sizeof(uint32_t) * words * H));
+    HIPCHK(hipMalloc(&c->d_pbits, sizeof(uint32_t) * words * (H + 1)));  // + sink row
     HIPCHK(hipMalloc(&c->d_cntB, sizeof(uint32_t) * npix));
     HIPCHK(hipMalloc(&c->d_cntL, sizeof(uint32_t) * npix));
     HIPCHK(hipMalloc(&c->d_firstD, sizeof(uint32_t) * npix));
@@ -736,6 +736,20 @@ static int ensure_image(ff_ctx *c, int W, int H)
     c->capW = W;
     c->capH = H;
     return FF_OK;
+}
+
+// accumulates event pairs of completed SDF / mesh launches
+void resolve_timing(ff_ctx *c)
+{
+    for (int i = 0; i < 2; i++) {
+        if (!c->sdfPending[i]) continue;
+        float ms = 0.0f;
+        if (hipEventElapsedTime(&ms, c->ev[2 + 2 * i], c->ev[3 + 2 * i]) == hipSuccess) {
+            c->phaseMs[1] += ms;
+            c->phaseCalls[1]++;
+        }
+        c->sdfPending[i] = false;
+    }
 }
 
 static void launch_grow(ff_ctx *c, const int bmin[3], const int bmax[3])
@@ -887,6 +901,8 @@ void ff_destroy(ff_ctx *c)
     hipFree(c->d_stamp); hipFree(c->d_queue); hipFree(c->d_dist); hipFree(c->d_wgt); hipFree(c->d_col);
     hipFree(c->d_fc); hipHostFree(c->h_fc);
     hipFree(c->hv.keys); hipFree(c->hv.minpix); hipFree(c->hv.aux); hipFree(c->hv.assigned); hipFree(c->hv.list);
+    for (int i = 0; i < 6; i++)
+        if (c->ev[i]) hipEventDestroy(c->ev[i]);
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;
 }
@@ -952,6 +968,7 @@ static int add_map_impl(ff_ctx *c, int W, int H, const double R[9], const double
     const int words = (W + 31) / 32;
     const uint32_t npix = (uint32_t)W * H;
     if (c->n) {
+        if (c->timing) hipEventRecord(c->ev[0], c->stream);
         k_planes<<<dim3((W + 63) / 64, H), 64, 0, c->stream>>>(c->d_depth, W, H, words, scaling, maxcam, c->d_qx,
                                                               c->d_qy, p.t[0], p.t[1], p.t[2], c->d_planes,
                                                               c->d_cntB, c->d_cntL, c->d_firstD);
@@ -969,11 +986,20 @@ static int add_map_impl(ff_ctx *c, int W, int H, const double R[9], const double
                                              c->nLeavesUsed);
         k_link<<<256, 256, 0, c->stream>>>(c->hv, mv, c->d_fc, c->d_pl0, c->n, stamp, c->d_fc);
         k_hash_clear<<<256, 256, 0, c->stream>>>(c->hv, c->d_fc);
+        if (c->timing) hipEventRecord(c->ev[1], c->stream);
     } else {
         HIPCHK(hipMemsetAsync(c->d_pbits, 0, sizeof(uint32_t) * words * H, c->stream));
     }
     HIPCHK(hipMemcpyAsync(hf, c->d_fc, sizeof(FrameCounters), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
+    if (c->timing) {
+        float ms = 0.0f;
+        if (c->n && hipEventElapsedTime(&ms, c->ev[0], c->ev[1]) == hipSuccess) {
+            c->phaseMs[0] += ms;
+            c->phaseCalls[0]++;
+        }
+        resolve_timing(c);  // the previous frame's SDF update completed before this sync
+    }
     if (hf->flags & FF_FLAG_HASH_FULL) {
         c->err = "per-frame node table overflow";
         return FF_E_POOL_FULL;
@@ -1006,10 +1032,22 @@ static int add_map_impl(ff_ctx *c, int W, int H, const double R[9], const double
         sp.fx = q.fx; sp.fy = q.fy; sp.cx = q.cx; sp.cy = q.cy;
         sp.scale = c->scale; sp.dT = c->dT; sp.scaling = scaling; sp.maxcam = maxcam;
         sp.useColor = c->useColor ? 1 : 0;
-        const uint32_t blocks = std::min<uint32_t>((c->nQueued + 3) / 4, 8192u);
-        k_sdf<<<blocks, 256, 0, c->stream>>>(c->d_queue, c->d_fc, c->mapView(), c->d_depth, c->d_bgr, W, H, sp);
+        const int e = 2 + 2 * (int)(c->frames & 1u);  // alternate event pairs 2/3 and 4/5
+        if (c->timing) hipEventRecord(c->ev[e], c->stream);
+        k_sdf<<<c->nQueued, 512, 0, c->stream>>>(c->d_queue, c->mapView(), c->d_depth, c->d_bgr, W, H, sp);
+        if (c->timing) {
+            hipEventRecord(c->ev[e + 1], c->stream);
+            c->sdfPending[e == 2 ? 0 : 1] = true;
+        }
         HIPCHK(hipGetLastError());
     }
+    // algorithmic bytes: images read once, 16 B per processed pixel, 512 voxels
+    // x 14 B (8 B without color) read and written per queued brick
+    c->phaseBytes[0] += (double)W * H * 2 + 16.0 * hf->nProcessed;
+    c->phaseBytes[1] += (double)c->nQueued * (c->useColor ? 14336.0 : 8192.0) + (c->useColor ? 3.0 * W * H : 0.0);
+    c->phaseUnits[0] += hf->nProcessed;
+    c->phaseUnits[1] += c->nQueued;
+
     if (c->nLeavesUsed > c->nLeavesTotal || c->nBranchesUsed > c->nBranchesTotal) c->incremental = false;
     c->frames++;
     mesh_after_frame(c);
@@ -1052,6 +1090,41 @@ int ff_add_map_device(ff_ctx *c, const uint16_t *depth_dev, const uint8_t *bgr_d
     if (c->useColor)
         HIPCHK(hipMemcpyAsync(c->d_bgr, bgr_dev, (size_t)W * H * 3, hipMemcpyDeviceToDevice, c->stream));
     return add_map_impl(c, W, H, R, t, K, scaling, maxcam);
+}
+
+int ff_set_timing(ff_ctx *c, int on)
+{
+    if (!c) return FF_E_ARG;
+    if (on && !c->ev[0])
+        for (int i = 0; i < 6; i++) HIPCHK(hipEventCreate(&c->ev[i]));
+    c->timing = on != 0;
+    return FF_OK;
+}
+
+int ff_get_timing(ff_ctx *c, double *ms, uint64_t *calls, double *bytes, uint64_t *units)
+{
+    if (!c) return FF_E_ARG;
+    HIPCHK(hipStreamSynchronize(c->stream));
+    resolve_timing(c);
+    for (int i = 0; i < 3; i++) {
+        if (ms) ms[i] = c->phaseMs[i];
+        if (calls) calls[i] = c->phaseCalls[i];
+        if (bytes) bytes[i] = c->phaseBytes[i];
+        if (units) units[i] = c->phaseUnits[i];
+    }
+    return FF_OK;
+}
+
+int ff_reset_timing(ff_ctx *c)
+{
+    if (!c) return FF_E_ARG;
+    for (int i = 0; i < 3; i++) {
+        c->phaseMs[i] = 0.0;
+        c->phaseCalls[i] = 0;
+        c->phaseBytes[i] = 0.0;
+        c->phaseUnits[i] = 0;
+    }
+    return FF_OK;
 }
 
 int ff_get_counts(ff_ctx *c, ff_counts *o)

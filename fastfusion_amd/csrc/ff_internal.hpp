@@ -22,7 +22,7 @@ struct CamF {
 struct FrameCounters {
     uint32_t nQueued, listCount, newB, newL;
     int bmin[3], bmax[3];
-    uint32_t flags, pad;
+    uint32_t flags, nProcessed;
 };
 
 // device views of the map arrays
@@ -85,6 +85,12 @@ struct ff_ctx {
     ff::HashView hv{};
     ff::MeshState *mesh = nullptr;
     std::string err;
+    // per-phase timing (0 allocation, 1 SDF update, 2 meshing)
+    bool timing = false;
+    hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    double phaseMs[3] = {0, 0, 0}, phaseBytes[3] = {0, 0, 0};
+    uint64_t phaseCalls[3] = {0, 0, 0}, phaseUnits[3] = {0, 0, 0};
+    bool sdfPending[2] = {false, false};
 
     ff::MapView mapView() const
     {
@@ -104,4 +110,5 @@ void mesh_free(ff_ctx *c);
 void mesh_on_initial_volume(ff_ctx *c);
 void mesh_on_grow(ff_ctx *c);
 void mesh_after_frame(ff_ctx *c);
+void resolve_timing(ff_ctx *c);
 }  // namespace ff

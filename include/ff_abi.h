@@ -111,6 +111,17 @@ int ff_copy_queue(ff_ctx *ctx, uint32_t *out, size_t cap);
 /* De-dup result of the last frame: one byte per pixel, 1 = processed. */
 int ff_copy_processed(ff_ctx *ctx, uint8_t *out, size_t n);
 
+/* Per-phase GPU timing (no counterpart in the reference, which prints
+ * cv::getTickCount averages, geometryfusion_mipmap_cpu.cpp:979-991).
+ * When on, each frame records HIP events around its phases on the context
+ * stream: 0 = allocation kernels, 1 = SDF update kernel, 2 = meshing.
+ * ff_get_timing returns per phase the summed milliseconds, the number of
+ * timed launches, the summed algorithmic bytes and work units (processed
+ * pixels, queued bricks, re-meshed cells); arrays have 3 entries. */
+int ff_set_timing(ff_ctx *ctx, int on);
+int ff_get_timing(ff_ctx *ctx, double *ms, uint64_t *calls, double *bytes, uint64_t *units);
+int ff_reset_timing(ff_ctx *ctx);
+
 /* Device stream the context launches on (a hipStream_t), for callers that
  * time or order work around it. */
 void *ff_stream(ff_ctx *ctx);

@@ -527,3 +527,5 @@ const float *ffo_weights(const ffo_ctx *c) { return c->wgt; }
 const uint16_t *ffo_color(const ffo_ctx *c) { return c->col; }
 const uint8_t *ffo_processed(const ffo_ctx *c) { return c->processed; }
 int ffo_grow_count(const ffo_ctx *c) { return c->grows; }
+uint32_t ffo_leaves_total(const ffo_ctx *c) { return c->nLeavesTotal; }
+float ffo_scale(const ffo_ctx *c) { return c->scale; }

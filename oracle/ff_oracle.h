@@ -54,6 +54,15 @@ const float *ffo_weights(const ffo_ctx *c);
 const uint16_t *ffo_color(const ffo_ctx *c);       /* 3 per voxel: R, G, B (x, y, z) */
 const uint8_t *ffo_processed(const ffo_ctx *c);    /* last frame's de-dup result, w*h */
 int ffo_grow_count(const ffo_ctx *c);
+uint32_t ffo_leaves_total(const ffo_ctx *c);
+float ffo_scale(const ffo_ctx *c);
+
+/* Meshing (ff_oracle_mesh.c): meshes every meshable cell of the current map
+ * from scratch; cells are sorted by (ox, oy, oz, size, type). */
+int ffo_mesh_all(ffo_ctx *c);
+void ffo_mesh_counts(const ffo_ctx *c, size_t out[3]);
+void ffo_mesh_copy(const ffo_ctx *c, int32_t *desc13, uint32_t *nv, uint32_t *nf, float *xyz, uint8_t *rgb,
+                   uint32_t *faces);
 
 /* unit-level entry points used by the tests */
 float ffo_rcp(float x);

@@ -44,16 +44,15 @@ def load():
     lib.ffo_dedup.argtypes = [P, P, P, P, ctypes.c_int, ctypes.c_int, P]
     lib.ffo_update_brick.argtypes = [P, ctypes.c_float, ctypes.c_float, P, ctypes.c_int, ctypes.c_int, P, P,
                                      ctypes.c_float, ctypes.c_float, P, ctypes.c_uint16, P, P, P]
-    if hasattr(lib, "ffo_mesh_update"):
-        _mesh_sigs(lib)
+    _mesh_sigs(lib)
     _lib = lib
     return lib
 
 
 def _mesh_sigs(lib):
     P = ctypes.c_void_p
-    lib.ffo_mesh_update.restype = ctypes.c_int
-    lib.ffo_mesh_update.argtypes = [P]
+    lib.ffo_mesh_all.restype = ctypes.c_int
+    lib.ffo_mesh_all.argtypes = [P]
     lib.ffo_mesh_counts.argtypes = [P, P]
     lib.ffo_mesh_copy.argtypes = [P, P, P, P, P, P, P]
 
@@ -126,6 +125,22 @@ class Oracle:
 
     def processed(self, w, h):
         return _arr(self.lib.ffo_processed(self.h), np.uint8, w * h).reshape(h, w)
+
+    def mesh_all(self):
+        """Meshes every meshable cell from scratch.  Returns (desc (n, 13): type,
+        ox, oy, oz, size, lastLeaf[8]; nv; nf; xyz; rgb; faces local per cell)."""
+        self.lib.ffo_mesh_all(self.h)
+        cnt = np.zeros(3, np.uint64)
+        self.lib.ffo_mesh_counts(self.h, _vp(cnt))
+        nc, nv, nf = (int(v) for v in cnt)
+        desc = np.zeros((nc, 13), np.int32)
+        cv = np.zeros(nc, np.uint32)
+        cf = np.zeros(nc, np.uint32)
+        xyz = np.zeros((nv, 3), np.float32)
+        rgb = np.zeros((nv, 3), np.uint8)
+        faces = np.zeros((nf, 3), np.uint32)
+        self.lib.ffo_mesh_copy(self.h, _vp(desc), _vp(cv), _vp(cf), _vp(xyz), _vp(rgb), _vp(faces))
+        return desc, cv, cf, xyz, rgb, faces
 
 
 def rcp(x):
