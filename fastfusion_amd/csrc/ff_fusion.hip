@@ -806,6 +806,21 @@ static void launch_grow(ff_ctx *c, const int bmin[3], const int bmax[3])
     for (auto &l : links)
         hipMemcpyAsync(c->d_tree + l.first, &l.second, sizeof(uint32_t), hipMemcpyHostToDevice, c->stream);
     hipStreamSynchronize(c->stream);
+    {
+        // node geometry of the new root chain: the branch added at level L has
+        // octant size n_old 2^L; origins accumulate from the new root down
+        std::vector<ushort4> chain(maxAdd);
+        uint32_t org[3] = {0, 0, 0};
+        for (int L = (int)maxAdd - 1; L >= 0; L--) {
+            const uint32_t octant = (uint32_t)(c->n >> (maxAdd - L));
+            chain[maxAdd - 1 - L] = make_ushort4((unsigned short)org[0], (unsigned short)org[1], (unsigned short)org[2],
+                                                 (unsigned short)octant);
+            const uint32_t co = (uint32_t)(links[L].first - (maxAdd - 1 - L) * 16u - 8u);
+            for (int k = 0; k < 3; k++) org[k] += octant * ((co >> k) & 1u);
+        }
+        const uint32_t addv32[3] = {addv[0], addv[1], addv[2]};
+        mesh_on_grow_blocks(c, used / 16, add / 16, addv32, chain);
+    }
     for (int k = 0; k < 3; k++) c->offset[k] -= (float)addv[k] * c->scale;
     const uint32_t nl = std::min(c->nLeavesUsed, c->nLeavesTotal);
     if (nl) k_shift_leaves<<<(nl + 255) / 256, 256, 0, c->stream>>>(c->d_leafinfo, nl, addv[0], addv[1], addv[2]);
@@ -901,7 +916,7 @@ void ff_destroy(ff_ctx *c)
     hipFree(c->d_stamp); hipFree(c->d_queue); hipFree(c->d_dist); hipFree(c->d_wgt); hipFree(c->d_col);
     hipFree(c->d_fc); hipHostFree(c->h_fc);
     hipFree(c->hv.keys); hipFree(c->hv.minpix); hipFree(c->hv.aux); hipFree(c->hv.assigned); hipFree(c->hv.list);
-    for (int i = 0; i < 6; i++)
+    for (int i = 0; i < 8; i++)
         if (c->ev[i]) hipEventDestroy(c->ev[i]);
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;
@@ -985,6 +1000,7 @@ static int add_map_impl(ff_ctx *c, int W, int H, const double R[9], const double
         k_assign<<<256, 256, 0, c->stream>>>(c->hv, c->d_fc, c->d_pref, c->d_bsum, c->d_firstD, c->nBranchesUsed,
                                              c->nLeavesUsed);
         k_link<<<256, 256, 0, c->stream>>>(c->hv, mv, c->d_fc, c->d_pl0, c->n, stamp, c->d_fc);
+        mesh_record_branches(c);
         k_hash_clear<<<256, 256, 0, c->stream>>>(c->hv, c->d_fc);
         if (c->timing) hipEventRecord(c->ev[1], c->stream);
     } else {
@@ -1096,7 +1112,7 @@ int ff_set_timing(ff_ctx *c, int on)
 {
     if (!c) return FF_E_ARG;
     if (on && !c->ev[0])
-        for (int i = 0; i < 6; i++) HIPCHK(hipEventCreate(&c->ev[i]));
+        for (int i = 0; i < 8; i++) HIPCHK(hipEventCreate(&c->ev[i]));
     c->timing = on != 0;
     return FF_OK;
 }

@@ -87,7 +87,7 @@ struct ff_ctx {
     std::string err;
     // per-phase timing (0 allocation, 1 SDF update, 2 meshing)
     bool timing = false;
-    hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    hipEvent_t ev[8] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     double phaseMs[3] = {0, 0, 0}, phaseBytes[3] = {0, 0, 0};
     uint64_t phaseCalls[3] = {0, 0, 0}, phaseUnits[3] = {0, 0, 0};
     bool sdfPending[2] = {false, false};
@@ -110,5 +110,8 @@ void mesh_free(ff_ctx *c);
 void mesh_on_initial_volume(ff_ctx *c);
 void mesh_on_grow(ff_ctx *c);
 void mesh_after_frame(ff_ctx *c);
+void mesh_record_branches(ff_ctx *c);
+void mesh_on_grow_blocks(ff_ctx *c, uint32_t oldBlocks, uint32_t addBlocks, const uint32_t addv[3],
+                         const std::vector<ushort4> &chain);
 void resolve_timing(ff_ctx *c);
 }  // namespace ff

@@ -1,39 +1,1132 @@
-// ff_mesh.hip -- incremental marching cubes over changed bricks (placeholder
-// until the cell mesher lands: reports no geometry).
+// ff_mesh.hip -- incremental marching cubes over mesh cells on gfx950.
+//
+// The reference keeps a "mesh cell" partition of space
+// (src/fusion/meshcelltraversalSplitReferenced.hpp) and re-meshes the cells
+// touched by each frame (updateMeshes, geometryfusion_mipmap_cpu.cpp:
+// 2884-2954; MeshCell::updateMesh, treeandbrick_incremental.cpp:2867-2987).
+// Here the partition is derived from the octree on every update:
+//   k_terminal   octants without a child branch that some brick covers
+//   k_cells      each such node emits the cells it owns: its interior, and
+//                the wall / edge / corner cells between it and its
+//                neighbours where it is the smallest adjacent node (ties to
+//                the lowest lastLeaf slot); lastLeaf[k] = the finest brick
+//                covering side k (located by descending the tree)
+//   k_mark       a cell keeps last update's mesh when it existed then with
+//                the same bricks and none of the bricks it can sample (its
+//                lastLeafs and their parent chains) changed since
+//   k_mesh       one wave per dirty cell: samples into LDS (reference
+//                sampling rules per cell type, parent-chain fallback), cube
+//                classification, a bitmap prefix scan numbers the used edges
+//                in raster order, vertices (double VertexInterp) and faces
+//                are written to device pools
+// Meshes stay on the device; ff_copy_mesh concatenates them on request.
+#include "ff_device.hpp"
 #include "ff_internal.hpp"
+#include "mc_tables.h"
+
+#include <algorithm>
+#include <cstring>
+#include <vector>
 
 namespace ff {
-struct MeshState {
-    int dummy;
+
+struct CellRec {
+    uint64_t key;     // ox:16 oy:16 oz:16 log2size:5 type:3
+    uint32_t ll[8];   // lastLeaf per side
+    uint32_t v0, nv, f0, nf;
+    uint32_t hslot;   // slot of the key in the lookup hash
+    uint32_t pad[3];
 };
-int mesh_init(ff_ctx *c) { c->mesh = new MeshState(); return FF_OK; }
-void mesh_free(ff_ctx *c) { delete c->mesh; c->mesh = nullptr; }
-void mesh_on_initial_volume(ff_ctx *) {}
+
+struct MeshCounters {
+    uint32_t nTerm, nCells, nDirty, vAlloc, fAlloc, flags, nLive, pad;
+};
+
+#define FF_MESH_OVERFLOW 1u
+#define FF_CELL_OVERFLOW 2u
+
+__constant__ uint8_t c_tri[256][16];  // edges of each triangle, 255-terminated
+__constant__ uint8_t c_ntri[256];
+
+struct MeshState {
+    ushort4 *d_binfo = nullptr;  // per branch block: origin of the node, octant size
+    uint32_t binfoCap = 0;
+    uint4 *d_term = nullptr;     // x, y, z, (size << 0) ; .w = covering leaf, size packed in x high bits
+    uint32_t *d_termSize = nullptr;
+    uint32_t termCap = 0;
+    CellRec *d_cells[2] = {nullptr, nullptr};
+    uint32_t cellCap = 0;
+    uint32_t nCells[2] = {0, 0};
+    int cur = 0;
+    uint32_t *d_dirty = nullptr;
+    uint64_t *d_hkeys = nullptr;  // previous table: key -> index
+    uint32_t *d_hval = nullptr;
+    uint32_t hmask = 0;
+    float4 *d_vpool = nullptr;    // x, y, z, rgb packed
+    uint32_t *d_fpool = nullptr;  // 3 local indices per face
+    uint32_t vCap = 0, fCap = 0;
+    MeshCounters *d_mc = nullptr, *h_mc = nullptr;
+    uint32_t lastMeshStamp = 0;
+    bool haveVolume = false;
+};
+
+// ---------------------------------------------------------------------------
+// tree queries
+// ---------------------------------------------------------------------------
+// Terminal node containing voxel (x, y, z): returns its size (0 outside the
+// volume), origin, and the deepest brick on the path.
+__device__ uint32_t locate(const uint32_t *__restrict__ tree, uint32_t nLeavesTotal, uint32_t n, int x, int y, int z,
+                           int org[3], uint32_t *leaf)
+{
+    *leaf = FF_BRANCHINIT;
+    if (x < 0 || y < 0 || z < 0 || x >= (int)n || y >= (int)n || z >= (int)n) return 0u;
+    uint32_t half = n >> 1, branch = 0;
+    int ox = 0, oy = 0, oz = 0;
+    for (;;) {
+        uint32_t co = 0;
+        if (x >= ox + (int)half) { co |= 1u; ox += half; }
+        if (y >= oy + (int)half) { co |= 2u; oy += half; }
+        if (z >= oz + (int)half) { co |= 4u; oz += half; }
+        const uint32_t l = tree[branch + co];
+        if (l < nLeavesTotal) *leaf = l;
+        const uint32_t child = tree[branch + co + 8];
+        if (child == FF_BRANCHINIT || half <= FF_BL) {
+            org[0] = ox; org[1] = oy; org[2] = oz;
+            return half;
+        }
+        branch = child;
+        half >>= 1;
+    }
+}
+
+__device__ __forceinline__ uint64_t cell_key(int type, int ox, int oy, int oz, int size)
+{
+    const uint32_t lg = 31u - __clz((uint32_t)size);
+    return ((uint64_t)(uint16_t)ox << 48) | ((uint64_t)(uint16_t)oy << 32) | ((uint64_t)(uint16_t)oz << 16) |
+           ((uint64_t)lg << 3) | (uint64_t)type;
+}
+
+// ---------------------------------------------------------------------------
+// k_terminal: covered terminal nodes
+// ---------------------------------------------------------------------------
+__global__ void k_terminal(const uint32_t *__restrict__ tree, const ushort4 *__restrict__ binfo, uint32_t nBlocks,
+                           uint32_t nLeavesTotal, uint32_t n, uint4 *__restrict__ term, uint32_t termCap,
+                           MeshCounters *mc)
+{
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nBlocks * 8u) return;
+    const uint32_t b = t >> 3, co = t & 7u;
+    if (tree[b * 16 + 8 + co] != FF_BRANCHINIT) return;
+    const ushort4 bi = binfo[b];
+    const uint32_t half = bi.w;
+    if (half == 0) return;
+    const int x = bi.x + (int)(half * (co & 1u)), y = bi.y + (int)(half * ((co >> 1) & 1u)),
+              z = bi.z + (int)(half * ((co >> 2) & 1u));
+    int org[3];
+    uint32_t lt;
+    const uint32_t s = locate(tree, nLeavesTotal, n, x, y, z, org, &lt);
+    if (s != half || lt == FF_BRANCHINIT) return;
+    const uint32_t i = atomicAdd(&mc->nTerm, 1u);
+    if (i < termCap) term[i] = make_uint4((uint32_t)x | ((uint32_t)y << 16), (uint32_t)z, half, lt);
+    else atomicOr(&mc->flags, FF_CELL_OVERFLOW);
+}
+
+// ---------------------------------------------------------------------------
+// k_cells: cells owned by each covered terminal node
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void emit_cell(CellRec *cells, uint32_t cap, MeshCounters *mc, int type, int ox, int oy,
+                                          int oz, int size, const uint32_t *ll, int nll)
+{
+    for (int k = 0; k < nll; k++)
+        if (ll[k] >= FF_BRANCHINIT) return;  // pushMeshCellQueue needs every side
+    const uint32_t i = atomicAdd(&mc->nCells, 1u);
+    if (i >= cap) {
+        atomicOr(&mc->flags, FF_CELL_OVERFLOW);
+        return;
+    }
+    CellRec r;
+    r.key = cell_key(type, ox, oy, oz, size);
+    for (int k = 0; k < 8; k++) r.ll[k] = k < nll ? ll[k] : FF_BRANCHINIT;
+    r.v0 = r.nv = r.f0 = r.nf = 0;
+    r.hslot = 0xffffffffu;
+    r.pad[0] = r.pad[1] = r.pad[2] = 0;
+    cells[i] = r;
+}
+
+__global__ void k_cells(const uint32_t *__restrict__ tree, uint32_t nLeavesTotal, uint32_t n,
+                        const uint4 *__restrict__ term, uint32_t termCap, MeshCounters *mc, CellRec *cells,
+                        uint32_t cap)
+{
+    const uint32_t nt = min(mc->nTerm, termCap);
+    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < nt; t += gridDim.x * blockDim.x) {
+        const uint4 tn = term[t];
+        const int p[3] = {(int)(tn.x & 0xffffu), (int)(tn.x >> 16), (int)tn.y};
+        const int s = (int)tn.z;
+        const uint32_t lt = tn.w;
+        int org[3];
+        uint32_t ll[8];
+        // interior
+        ll[0] = lt;
+        emit_cell(cells, cap, mc, 0, p[0], p[1], p[2], s, ll, 1);
+        // walls
+        for (int a = 0; a < 3; a++)
+            for (int side = 0; side < 2; side++) {
+                int q[3] = {p[0], p[1], p[2]};
+                q[a] = side ? p[a] + s : p[a] - 1;
+                uint32_t lf;
+                const uint32_t ns = locate(tree, nLeavesTotal, n, q[0], q[1], q[2], org, &lf);
+                if (!ns) continue;
+                const bool own = side ? (ns >= (uint32_t)s) : (ns > (uint32_t)s);
+                if (!own) continue;
+                int o[3] = {p[0], p[1], p[2]};
+                if (!side) o[a] = p[a] - s;
+                ll[side ? 0 : 1] = lt;
+                ll[side ? 1 : 0] = lf;
+                emit_cell(cells, cap, mc, 1 + a, o[0], o[1], o[2], s, ll, 2);
+            }
+        // edges along axis e; thin axes (u, v); slot = ubit + 2 vbit
+        for (int e = 0; e < 3; e++) {
+            const int u = e == 0 ? 1 : 0, v = e == 2 ? 1 : 2;
+            for (int cv = 0; cv < 2; cv++)
+                for (int cu = 0; cu < 2; cu++) {
+                    const int lu = p[u] + cu * s, lv = p[v] + cv * s;
+                    const int mine = (1 - cu) + 2 * (1 - cv);
+                    uint32_t sz[4];
+                    bool ok = true;
+                    for (int sl = 0; sl < 4; sl++) {
+                        if (sl == mine) { sz[sl] = (uint32_t)s; ll[sl] = lt; continue; }
+                        int q[3] = {p[0], p[1], p[2]};
+                        q[u] = lu - 1 + (sl & 1);
+                        q[v] = lv - 1 + (sl >> 1);
+                        sz[sl] = locate(tree, nLeavesTotal, n, q[0], q[1], q[2], org, &ll[sl]);
+                        if (!sz[sl]) ok = false;
+                    }
+                    if (!ok) continue;
+                    uint32_t mn = sz[0];
+                    for (int sl = 1; sl < 4; sl++) mn = min(mn, sz[sl]);
+                    if (mn != (uint32_t)s) continue;
+                    int first = 0;
+                    while (sz[first] != mn) first++;
+                    if (first != mine) continue;
+                    int o[3];
+                    o[e] = p[e]; o[u] = lu - s; o[v] = lv - s;
+                    emit_cell(cells, cap, mc, 4 + e, o[0], o[1], o[2], s, ll, 4);
+                }
+        }
+        // corners: slot bits (x, y, z) = + side of the point
+        for (int cz = 0; cz < 2; cz++)
+            for (int cy = 0; cy < 2; cy++)
+                for (int cx = 0; cx < 2; cx++) {
+                    const int pt[3] = {p[0] + cx * s, p[1] + cy * s, p[2] + cz * s};
+                    const int mine = (1 - cx) + 2 * (1 - cy) + 4 * (1 - cz);
+                    uint32_t sz[8];
+                    bool ok = true;
+                    for (int sl = 0; sl < 8; sl++) {
+                        if (sl == mine) { sz[sl] = (uint32_t)s; ll[sl] = lt; continue; }
+                        sz[sl] = locate(tree, nLeavesTotal, n, pt[0] - 1 + (sl & 1), pt[1] - 1 + ((sl >> 1) & 1),
+                                        pt[2] - 1 + ((sl >> 2) & 1), org, &ll[sl]);
+                        if (!sz[sl]) ok = false;
+                    }
+                    if (!ok) continue;
+                    uint32_t mn = sz[0];
+                    for (int sl = 1; sl < 8; sl++) mn = min(mn, sz[sl]);
+                    if (mn != (uint32_t)s) continue;
+                    int first = 0;
+                    while (sz[first] != mn) first++;
+                    if (first != mine) continue;
+                    emit_cell(cells, cap, mc, 7, pt[0] - s, pt[1] - s, pt[2] - s, s, ll, 8);
+                }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// k_mark: reuse the previous mesh of unchanged cells
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t hkey_hash(uint64_t k, uint32_t mask)
+{
+    k ^= k >> 31;
+    k *= 0x7fb5d329728ea185ull;
+    k ^= k >> 27;
+    k *= 0x81dadef4bc2dd44dull;
+    k ^= k >> 33;
+    return (uint32_t)k & mask;
+}
+
+__device__ __forceinline__ bool chain_dirty(const uint32_t *__restrict__ stamp, const uint32_t *__restrict__ parent,
+                                            uint32_t leaf, uint32_t since)
+{
+    for (int d = 0; d < 16 && leaf < FF_BRANCHINIT; d++) {
+        if (stamp[leaf] > since) return true;
+        leaf = parent[leaf];
+    }
+    return false;
+}
+
+__global__ void k_mark(CellRec *cells, const CellRec *__restrict__ prev, const uint64_t *__restrict__ hkeys,
+                       const uint32_t *__restrict__ hval, uint32_t hmask, bool havePrev,
+                       const uint32_t *__restrict__ stamp, const uint32_t *__restrict__ parent, uint32_t since,
+                       MeshCounters *mc, uint32_t *dirty, uint32_t cap)
+{
+    const uint32_t nc = min(mc->nCells, cap);
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nc; i += gridDim.x * blockDim.x) {
+        CellRec &c = cells[i];
+        int found = -1;
+        if (havePrev) {
+            uint32_t h = hkey_hash(c.key, hmask);
+            for (uint32_t pr = 0; pr <= hmask; pr++) {
+                const uint64_t k = hkeys[h];
+                if (k == c.key) { found = (int)hval[h]; break; }
+                if (k == FF_HEMPTY) break;
+                h = (h + 1) & hmask;
+            }
+        }
+        bool clean = found >= 0;
+        if (clean) {
+            const CellRec &p = prev[found];
+            for (int k = 0; k < 8; k++) clean = clean && (p.ll[k] == c.ll[k]);
+            for (int k = 0; k < 8 && clean; k++)
+                if (c.ll[k] < FF_BRANCHINIT && chain_dirty(stamp, parent, c.ll[k], since)) clean = false;
+            if (clean) {
+                c.v0 = p.v0; c.nv = p.nv; c.f0 = p.f0; c.nf = p.nf;
+            }
+        }
+        if (!clean) dirty[atomicAdd(&mc->nDirty, 1u)] = i;
+    }
+}
+
+__global__ void k_hash_build(CellRec *cells, const MeshCounters *mc, uint32_t cap, uint64_t *hkeys, uint32_t *hval,
+                             uint32_t hmask)
+{
+    const uint32_t nc = min(mc->nCells, cap);
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nc; i += gridDim.x * blockDim.x) {
+        const uint64_t key = cells[i].key;
+        uint32_t h = hkey_hash(key, hmask);
+        for (uint32_t pr = 0; pr <= hmask; pr++) {
+            const unsigned long long prevk =
+                atomicCAS((unsigned long long *)&hkeys[h], (unsigned long long)FF_HEMPTY, (unsigned long long)key);
+            if (prevk == FF_HEMPTY || prevk == key) {
+                hval[h] = i;
+                cells[i].hslot = h;
+                break;
+            }
+            h = (h + 1) & hmask;
+        }
+    }
+}
+
+// empties the slots of a whole table (by recorded slot, so concurrent
+// clears cannot cut each other's probe chains)
+__global__ void k_hash_clear_cells(const CellRec *__restrict__ cells, uint32_t nc, uint64_t *hkeys)
+{
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nc; i += gridDim.x * blockDim.x) {
+        const uint32_t h = cells[i].hslot;
+        if (h != 0xffffffffu) hkeys[h] = FF_HEMPTY;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// k_mesh: one wave per dirty cell
+// ---------------------------------------------------------------------------
+struct MapRO {
+    const ushort4 *info;     // pos, scale
+    const uint32_t *parent;
+    const float *dist, *wgt;
+    const uint16_t *col;     // R, G, B planes of 512 per brick
+    float off[3];
+    float scale;
+    int useColor;
+};
+
+struct Sample {
+    float d, w;
+    uint16_t c[3];
+};
+
+// orders LDS traffic between the lanes of one wave (each wave owns its own
+// LDS slice, so no block barrier is needed)
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ int clamp7(int v) { return v < 0 ? 0 : (v > 7 ? 7 : v); }
+
+__device__ __forceinline__ void split_axis(int X, int lo, int ls, int &il, float &r, int &ih)
+{
+    const int l = (X - lo) / ls, rem = (X - lo) % ls;
+    const float rr = __fdiv_rn((float)rem, (float)ls);
+    il = clamp7(l);
+    r = rr;
+    ih = clamp7(l + ((rr > 0.0f && l < FF_BL - 1) ? 1 : 0));
+}
+
+__device__ __forceinline__ void fetch(const MapRO &m, uint32_t leaf, int ix, int iy, int iz, float &w, float &d,
+                                      float c[3])
+{
+    const size_t q = (size_t)leaf * FF_BRICK + (size_t)((iz * FF_BL + iy) * FF_BL + ix);
+    w = m.wgt[q];
+    d = m.dist[q];
+    if (m.useColor) {
+        const uint16_t *cb = m.col + (size_t)leaf * FF_BRICK * 3 + (size_t)((iz * FF_BL + iy) * FF_BL + ix);
+        c[0] = (float)cb[0]; c[1] = (float)cb[FF_BRICK]; c[2] = (float)cb[2 * FF_BRICK];
+    } else {
+        c[0] = c[1] = c[2] = 0.0f;
+    }
+}
+
+__device__ __forceinline__ uint16_t f2u16(float v) { return (uint16_t)(int)v; }
+
+// interior: trilinear; weight of the lower corner (mesh_interleaved_meshcell.cpp:95-260)
+__device__ void sample_interior(const MapRO &m, uint32_t ll, const int X[3], Sample &s)
+{
+    s.w = 0.0f; s.d = 0.0f; s.c[0] = s.c[1] = s.c[2] = 0;
+    int guard = 0;
+    for (uint32_t leaf = ll; leaf < FF_BRANCHINIT && s.w <= 0.0f && guard < 16; leaf = m.parent[leaf], guard++) {
+        const ushort4 li = m.info[leaf];
+        const int lo[3] = {li.x, li.y, li.z}, ls = li.w;
+        int il[3], ih[3];
+        float r[3];
+        for (int k = 0; k < 3; k++) split_axis(X[k], lo[k], ls, il[k], r[k], ih[k]);
+        const float rxn = 1.0f - r[0], ryn = 1.0f - r[1], rzn = 1.0f - r[2];
+        const float f[8] = {rzn * ryn * rxn, rzn * ryn * r[0], rzn * r[1] * rxn, rzn * r[1] * r[0],
+                            r[2] * ryn * rxn, r[2] * ryn * r[0], r[2] * r[1] * rxn, r[2] * r[1] * r[0]};
+        float wf[8], df[8], cf[8][3];
+#pragma unroll
+        for (int i = 0; i < 8; i++)
+            fetch(m, leaf, (i & 1) ? ih[0] : il[0], (i & 2) ? ih[1] : il[1], (i & 4) ? ih[2] : il[2], wf[i], df[i], cf[i]);
+        s.w += wf[0];
+        const float a0 = f[0] * df[0] + f[4] * df[4], a1 = f[1] * df[1] + f[5] * df[5];
+        const float a2 = f[2] * df[2] + f[6] * df[6], a3 = f[3] * df[3] + f[7] * df[7];
+        s.d = (a0 + a1) + (a2 + a3);
+#pragma unroll
+        for (int ch = 0; ch < 3; ch++) {
+            const float b0 = f[0] * cf[0][ch] + f[4] * cf[4][ch], b1 = f[1] * cf[1][ch] + f[5] * cf[5][ch];
+            const float b2 = f[2] * cf[2][ch] + f[6] * cf[6][ch], b3 = f[3] * cf[3][ch] + f[7] * cf[7][ch];
+            s.c[ch] = f2u16((b0 + b1) + (b2 + b3));
+        }
+    }
+}
+
+// wall: thin axis a pointwise, u (inner) and v (outer) bilinear (:700-760)
+__device__ void sample_wall(const MapRO &m, uint32_t ll, int a, int u, int v, const int X[3], Sample &s)
+{
+    s.w = 0.0f; s.d = 0.0f; s.c[0] = s.c[1] = s.c[2] = 0;
+    int guard = 0;
+    for (uint32_t leaf = ll; leaf < FF_BRANCHINIT && s.w <= 0.0f && guard < 16; leaf = m.parent[leaf], guard++) {
+        const ushort4 li = m.info[leaf];
+        const int lo[3] = {li.x, li.y, li.z}, ls = li.w;
+        const int la = clamp7((X[a] - lo[a]) / ls);
+        int ul, uh, vl, vh;
+        float ru, rv;
+        split_axis(X[u], lo[u], ls, ul, ru, uh);
+        split_axis(X[v], lo[v], ls, vl, rv, vh);
+        float wf[4], df[4], cf[4][3];
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            int id[3];
+            id[a] = la;
+            id[u] = (i & 1) ? uh : ul;
+            id[v] = (i & 2) ? vh : vl;
+            fetch(m, leaf, id[0], id[1], id[2], wf[i], df[i], cf[i]);
+        }
+        const float ruI = 1.0f - ru, rvI = 1.0f - rv;
+        s.w += rvI * ruI * wf[0] + rvI * ru * wf[1] + rv * ruI * wf[2] + rv * ru * wf[3];
+        s.d = rvI * ruI * df[0] + rvI * ru * df[1] + rv * ruI * df[2] + rv * ru * df[3];
+#pragma unroll
+        for (int ch = 0; ch < 3; ch++)
+            s.c[ch] = f2u16(rvI * ruI * cf[0][ch] + rvI * ru * cf[1][ch] + rv * ruI * cf[2][ch] + rv * ru * cf[3][ch]);
+    }
+}
+
+// edge: axis e linear, weight = max of the two (:1800-1860)
+__device__ void sample_edge(const MapRO &m, uint32_t ll, int e, const int X[3], Sample &s)
+{
+    s.w = 0.0f; s.d = 0.0f; s.c[0] = s.c[1] = s.c[2] = 0;
+    int guard = 0;
+    for (uint32_t leaf = ll; leaf < FF_BRANCHINIT && s.w <= 0.0f && guard < 16; leaf = m.parent[leaf], guard++) {
+        const ushort4 li = m.info[leaf];
+        const int lo[3] = {li.x, li.y, li.z}, ls = li.w;
+        int el, eh;
+        float re;
+        split_axis(X[e], lo[e], ls, el, re, eh);
+        int i0[3], i1[3];
+        for (int k = 0; k < 3; k++) i0[k] = i1[k] = clamp7((X[k] - lo[k]) / ls);
+        i0[e] = el; i1[e] = eh;
+        float w0, w1, d0, d1, c0[3], c1[3];
+        fetch(m, leaf, i0[0], i0[1], i0[2], w0, d0, c0);
+        fetch(m, leaf, i1[0], i1[1], i1[2], w1, d1, c1);
+        s.w += (w0 < w1) ? w1 : w0;
+        s.d = (1.0f - re) * d0 + re * d1;
+#pragma unroll
+        for (int ch = 0; ch < 3; ch++) s.c[ch] = f2u16((1.0f - re) * c0[ch] + re * c1[ch]);
+    }
+}
+
+// corner: pointwise (:2550-2575)
+__device__ void sample_point(const MapRO &m, uint32_t ll, const int X[3], Sample &s)
+{
+    s.w = 0.0f; s.d = 0.0f; s.c[0] = s.c[1] = s.c[2] = 0;
+    int guard = 0;
+    for (uint32_t leaf = ll; leaf < FF_BRANCHINIT && s.w <= 0.0f && guard < 16; leaf = m.parent[leaf], guard++) {
+        const ushort4 li = m.info[leaf];
+        const int lo[3] = {li.x, li.y, li.z}, ls = li.w;
+        float w, d, c[3];
+        fetch(m, leaf, clamp7((X[0] - lo[0]) / ls), clamp7((X[1] - lo[1]) / ls), clamp7((X[2] - lo[2]) / ls), w, d, c);
+        s.w += w;
+        s.d = d;
+        s.c[0] = (uint16_t)c[0]; s.c[1] = (uint16_t)c[1]; s.c[2] = (uint16_t)c[2];
+    }
+}
+
+// VertexInterp, double (mesh.cpp:1266-1300)
+__device__ __forceinline__ void vinterp(const float p1[3], const float p2[3], float v1f, float v2f, float out[3])
+{
+    const double v1 = v1f, v2 = v2f;
+    if (fabs(0.0 - v1) < 1e-40) { out[0] = p1[0]; out[1] = p1[1]; out[2] = p1[2]; return; }
+    if (fabs(0.0 - v2) < 1e-40) { out[0] = p2[0]; out[1] = p2[1]; out[2] = p2[2]; return; }
+    if (fabs(v1 - v2) < 1e-40) { out[0] = p1[0]; out[1] = p1[1]; out[2] = p1[2]; return; }
+    const double mu = __ddiv_rn(0.0 - v1, v2 - v1);
+#pragma unroll
+    for (int k = 0; k < 3; k++) out[k] = (float)__dadd_rn((double)p1[k], __dmul_rn(mu, __dsub_rn((double)p2[k], (double)p1[k])));
+}
+
+// VertexInterp for colors, float with clamping (mesh.cpp:1302-1323), then / 256
+__device__ __forceinline__ uint32_t cinterp(const uint16_t c1[3], const uint16_t c2[3], float v1, float v2)
+{
+    uint16_t r[3];
+    if ((double)fabsf(0.0f - v1) < 1e-40) { r[0] = c1[0]; r[1] = c1[1]; r[2] = c1[2]; }
+    else if ((double)fabsf(0.0f - v2) < 1e-40) { r[0] = c2[0]; r[1] = c2[1]; r[2] = c2[2]; }
+    else if ((double)fabsf(v1 - v2) < 1e-40) { r[0] = c1[0]; r[1] = c1[1]; r[2] = c1[2]; }
+    else {
+        const float mu = __fdiv_rn(0.0f - v1, v2 - v1);
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            const float p = (float)c1[k] + mu * ((float)c2[k] - (float)c1[k]);
+            r[k] = p <= 65280.0f ? (p >= 0.0f ? (uint16_t)(int)p : (uint16_t)0) : (uint16_t)65280;
+        }
+    }
+    return (uint32_t)(r[0] / 256) | ((uint32_t)(r[1] / 256) << 8) | ((uint32_t)(r[2] / 256) << 16);
+}
+
+// Bourke corners and edges
+__constant__ int8_t c_corner[8][3] = {{0, 0, 0}, {1, 0, 0}, {1, 1, 0}, {0, 1, 0}, {0, 0, 1}, {1, 0, 1}, {1, 1, 1}, {0, 1, 1}};
+__constant__ int8_t c_edge[12][4] = {{0, 0, 0, 0}, {1, 0, 0, 1}, {0, 1, 0, 0}, {0, 0, 0, 1}, {0, 0, 1, 0}, {1, 0, 1, 1},
+                                     {0, 1, 1, 0}, {0, 0, 1, 1}, {0, 0, 0, 2}, {1, 0, 0, 2}, {1, 1, 0, 2}, {0, 1, 0, 2}};
+__constant__ int8_t c_edgec[12][2] = {{0, 1}, {1, 2}, {2, 3}, {3, 0}, {4, 5}, {5, 6}, {6, 7}, {7, 4}, {0, 4}, {1, 5}, {2, 6}, {3, 7}};
+
+struct WaveLds {
+    float d[FF_BRICK];
+    float w[FF_BRICK];
+    uint16_t c[3][FF_BRICK];
+    uint32_t used[48];     // 1536 edge slots: (sample * 3 + axis)
+    uint32_t wpref[48];    // vertex index of the first slot of each word
+    uint8_t cube[343];
+    uint16_t foff[344];    // face offset per cube
+};
+
+__device__ __forceinline__ uint32_t wave_excl_scan_u32(uint32_t v, uint32_t *total)
+{
+    const int lane = threadIdx.x & 63;
+    uint32_t x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_up(x, o);
+        if (lane >= o) x += y;
+    }
+    *total = __shfl(x, 63);
+    return x - v;
+}
+
+__global__ __launch_bounds__(256) void k_mesh(CellRec *cells, const uint32_t *__restrict__ dirty, MeshCounters *mc,
+                                              MapRO m, float4 *__restrict__ vpool, uint32_t vCap,
+                                              uint32_t *__restrict__ fpool, uint32_t fCap)
+{
+    __shared__ WaveLds lds[4];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    WaveLds &L = lds[wave];
+    const uint32_t nd = mc->nDirty;
+    for (uint32_t base = blockIdx.x * 4; base < nd; base += gridDim.x * 4) {
+        // waves of a block run independently: each owns lds[wave]
+        const uint32_t di = base + wave;
+        const bool act = di < nd;
+        CellRec *cp = act ? &cells[dirty[di]] : nullptr;
+        int type = 0, size = 1, o[3] = {0, 0, 0};
+        uint32_t ll[8];
+        bool empty = !act;
+        int n[3] = {0, 0, 0}, sizeMin = 1, stretch = 1, full[3] = {1, 1, 1};
+        if (act) {
+            const uint64_t key = cp->key;
+            type = (int)(key & 7u);
+            size = 1 << ((key >> 3) & 31u);
+            o[0] = (int)(key >> 48); o[1] = (int)((key >> 32) & 0xffffu); o[2] = (int)((key >> 16) & 0xffffu);
+            for (int k = 0; k < 8; k++) ll[k] = cp->ll[k];
+            if (type == 0) {
+                sizeMin = m.info[ll[0]].w;
+                if (sizeMin > size || size / sizeMin > FF_BL) empty = true;
+                const ushort4 li = m.info[ll[0]];
+                const int lp[3] = {li.x, li.y, li.z}, ext = sizeMin * FF_BL;
+                for (int k = 0; k < 3; k++)
+                    if (o[k] + size > lp[k] + ext || o[k] < lp[k]) empty = true;
+            } else if (type <= 3) {
+                const int s0 = m.info[ll[0]].w, s1 = m.info[ll[1]].w;
+                sizeMin = min(s0, s1);
+                stretch = s0;
+                full[type - 1] = 0;
+                if (stretch > size) empty = true;
+            } else if (type <= 6) {
+                int sc[4];
+                for (int k = 0; k < 4; k++) sc[k] = m.info[ll[k]].w;
+                stretch = max(max(sc[0], sc[1]), sc[2]);
+                sizeMin = min(min(sc[0], sc[1]), min(sc[2], sc[3]));
+                for (int k = 0; k < 3; k++) full[k] = (k == type - 4);
+                if (stretch > size) empty = true;
+            } else {
+                stretch = m.info[ll[0]].w;
+                for (int k = 1; k < 7; k++) stretch = max(stretch, (int)m.info[ll[k]].w);
+                sizeMin = stretch;
+                full[0] = full[1] = full[2] = 0;
+                if (stretch > size) empty = true;
+            }
+            for (int k = 0; k < 3; k++) n[k] = full[k] ? size / sizeMin : 2;
+        }
+        if (empty) { n[0] = n[1] = n[2] = 0; }
+        const int total = n[0] * n[1] * n[2];
+        // -- samples
+        for (int s = lane; s < total; s += 64) {
+            const int i = s % n[0], j = (s / n[0]) % n[1], l = s / (n[0] * n[1]);
+            const int g[3] = {i, j, l};
+            int X[3];
+            for (int k = 0; k < 3; k++)
+                X[k] = full[k] ? o[k] + g[k] * sizeMin : (g[k] ? o[k] + size : o[k] + size - stretch);
+            Sample sm;
+            if (type == 0) sample_interior(m, ll[0], X, sm);
+            else if (type <= 3) {
+                const int a = type - 1, u = a == 0 ? 1 : 0, v = a == 2 ? 1 : 2;
+                sample_wall(m, ll[g[a]], a, u, v, X, sm);
+            } else if (type <= 6) {
+                const int e = type - 4, u = e == 0 ? 1 : 0, v = e == 2 ? 1 : 2;
+                sample_edge(m, ll[g[u] + 2 * g[v]], e, X, sm);
+            } else {
+                sample_point(m, ll[i + 2 * j + 4 * l], X, sm);
+            }
+            const int id = (l * FF_BL + j) * FF_BL + i;
+            L.d[id] = sm.d; L.w[id] = sm.w;
+            L.c[0][id] = sm.c[0]; L.c[1][id] = sm.c[1]; L.c[2][id] = sm.c[2];
+        }
+        if (lane < 48) L.used[lane] = 0u;
+        wave_sync();
+        // -- grid geometry for vertices
+        auto coordOf = [&](int k, int gk) -> int {
+            return full[k] ? o[k] + gk * sizeMin : (gk ? o[k] + size : o[k] + size - stretch);
+        };
+        const int elen[3] = {full[0] ? sizeMin : stretch, full[1] ? sizeMin : stretch, full[2] ? sizeMin : stretch};
+        uint32_t nv = 0, nf = 0;
+        if (type == 7) {
+            // non-indexed corner cell with degenerate triangles dropped
+            // (getColoredTrianglesForCubePrecomputed, mesh.cpp:1931-2008)
+            int ci = 0;
+            bool veto = false;
+            float P[8][3];
+            for (int k = 0; k < 8; k++) {
+                const int id = (c_corner[k][2] * FF_BL + c_corner[k][1]) * FF_BL + c_corner[k][0];
+                if (total) {
+                    if (L.w[id] == 0.0f) veto = true;
+                    if (L.d[id] < 0.0f) ci |= 1 << k;
+                }
+                for (int a = 0; a < 3; a++) {
+                    const float basev = m.off[a] + (float)coordOf(a, 0) * m.scale;
+                    const float sx = m.scale * (float)stretch;
+                    P[k][a] = c_corner[k][a] ? basev + 1.0f * sx : basev + 0.0f * sx;
+                }
+            }
+            if (veto || !total) ci = 0;
+            const int ntri = c_ntri[ci];
+            bool keep = false;
+            float V[3][3];
+            uint32_t C[3];
+            if (lane < ntri) {
+                for (int q = 0; q < 3; q++) {
+                    const int e = c_tri[ci][3 * lane + q];
+                    const int a = c_edgec[e][0], b = c_edgec[e][1];
+                    const int ia = (c_corner[a][2] * FF_BL + c_corner[a][1]) * FF_BL + c_corner[a][0];
+                    const int ib = (c_corner[b][2] * FF_BL + c_corner[b][1]) * FF_BL + c_corner[b][0];
+                    vinterp(P[a], P[b], L.d[ia], L.d[ib], V[q]);
+                    const uint16_t ca[3] = {L.c[0][ia], L.c[1][ia], L.c[2][ia]};
+                    const uint16_t cb[3] = {L.c[0][ib], L.c[1][ib], L.c[2][ib]};
+                    C[q] = cinterp(ca, cb, L.d[ia], L.d[ib]);
+                }
+                keep = true;
+                for (int q = 0; q < 3; q++) {
+                    const float *A = V[q], *B = V[(q + 1) % 3];
+                    if (A[0] == B[0] && A[1] == B[1] && A[2] == B[2]) keep = false;
+                }
+            }
+            const uint64_t kb = __ballot(keep);
+            const uint32_t rank = __popcll(kb & ((1ull << lane) - 1ull));
+            nf = __popcll(kb);
+            nv = 3 * nf;
+            uint32_t v0 = 0, f0 = 0;
+            if (lane == 0 && act) {
+                v0 = atomicAdd(&mc->vAlloc, nv);
+                f0 = atomicAdd(&mc->fAlloc, nf);
+            }
+            v0 = __shfl(v0, 0);
+            f0 = __shfl(f0, 0);
+            const bool fits = v0 + nv <= vCap && f0 + nf <= fCap;
+            if (keep && fits) {
+                for (int q = 0; q < 3; q++) {
+                    vpool[v0 + 3 * rank + q] = make_float4(V[q][0], V[q][1], V[q][2], __uint_as_float(C[q]));
+                    fpool[3 * (f0 + rank) + q] = 3 * rank + q;
+                }
+            }
+            if (lane == 0 && act) {
+                if (!fits) { atomicOr(&mc->flags, FF_MESH_OVERFLOW); nv = nf = 0; }
+                cp->v0 = v0; cp->nv = nv; cp->f0 = f0; cp->nf = nf;
+            }
+            wave_sync();
+            continue;
+        }
+        // -- cubes: index, used edges, triangle counts
+        const int nc0 = max(n[0] - 1, 0), nc1 = max(n[1] - 1, 0), nc2 = max(n[2] - 1, 0);
+        const int ncubes = nc0 * nc1 * nc2;
+        for (int q = lane; q < ncubes; q += 64) {
+            const int i = q % nc0, j = (q / nc0) % nc1, l = q / (nc0 * nc1);
+            int ci = 0;
+            bool veto = false;
+            for (int k = 0; k < 8; k++) {
+                const int id = ((l + c_corner[k][2]) * FF_BL + j + c_corner[k][1]) * FF_BL + i + c_corner[k][0];
+                if (L.w[id] == 0.0f) veto = true;
+                if (L.d[id] < 0.0f) ci |= 1 << k;
+            }
+            if (veto) ci = 0;
+            L.cube[q] = (uint8_t)ci;
+            const int nt = c_ntri[ci];
+            for (int t = 0; t < 3 * nt; t++) {
+                const int e = c_tri[ci][t];
+                const int slot = (((l + c_edge[e][2]) * FF_BL + j + c_edge[e][1]) * FF_BL + i + c_edge[e][0]) * 3 + c_edge[e][3];
+                atomicOr(&L.used[slot >> 5], 1u << (slot & 31));
+            }
+        }
+        wave_sync();
+        // -- vertex numbering over the used-edge bitmap
+        {
+            const uint32_t wd = lane < 48 ? L.used[lane] : 0u;
+            uint32_t tot;
+            const uint32_t ex = wave_excl_scan_u32((uint32_t)__popc(wd), &tot);
+            if (lane < 48) L.wpref[lane] = ex;
+            nv = tot;
+        }
+        // face offsets per cube: contiguous chunks of 6 cubes per lane
+        {
+            const int per = 6;
+            uint32_t loc = 0;
+            for (int q = lane * per; q < min(ncubes, lane * per + per); q++) loc += c_ntri[L.cube[q]];
+            uint32_t tot;
+            uint32_t ex = wave_excl_scan_u32(loc, &tot);
+            for (int q = lane * per; q < min(ncubes, lane * per + per); q++) {
+                L.foff[q] = (uint16_t)ex;
+                ex += c_ntri[L.cube[q]];
+            }
+            nf = tot;
+        }
+        wave_sync();
+        uint32_t v0 = 0, f0 = 0;
+        if (lane == 0 && act) {
+            v0 = atomicAdd(&mc->vAlloc, nv);
+            f0 = atomicAdd(&mc->fAlloc, nf);
+        }
+        v0 = __shfl(v0, 0);
+        f0 = __shfl(f0, 0);
+        const bool fits = act && v0 + nv <= vCap && f0 + nf <= fCap;
+        if (fits) {
+            // vertices
+            for (int wdi = 0; wdi < 48; wdi++) {
+                const uint32_t wd = L.used[wdi];
+                if (!wd) continue;
+                for (int bit = lane; bit < 32; bit += 64) {
+                    if (!((wd >> bit) & 1u)) continue;
+                    const int slot = wdi * 32 + bit, a = slot % 3, id = slot / 3;
+                    const int i = id & 7, j = (id >> 3) & 7, l = id >> 6;
+                    const int gg[3] = {i, j, l};
+                    int g2[3] = {i, j, l};
+                    g2[a]++;
+                    const int id2 = (g2[2] * FF_BL + g2[1]) * FF_BL + g2[0];
+                    float p1[3], p2[3];
+                    for (int k = 0; k < 3; k++) {
+                        p1[k] = m.off[k] + (float)coordOf(k, gg[k]) * m.scale;
+                        p2[k] = p1[k];
+                    }
+                    p2[a] = m.off[a] + (float)(coordOf(a, gg[a]) + elen[a]) * m.scale;
+                    float V[3];
+                    vinterp(p1, p2, L.d[id], L.d[id2], V);
+                    const uint16_t ca[3] = {L.c[0][id], L.c[1][id], L.c[2][id]};
+                    const uint16_t cb[3] = {L.c[0][id2], L.c[1][id2], L.c[2][id2]};
+                    const uint32_t rgb = cinterp(ca, cb, L.d[id], L.d[id2]);
+                    const uint32_t vi = L.wpref[wdi] + __popc(wd & ((1u << bit) - 1u));
+                    vpool[v0 + vi] = make_float4(V[0], V[1], V[2], __uint_as_float(rgb));
+                }
+            }
+            // faces
+            for (int q = lane; q < ncubes; q += 64) {
+                const int ci = L.cube[q];
+                const int nt = c_ntri[ci];
+                if (!nt) continue;
+                const int i = q % nc0, j = (q / nc0) % nc1, l = q / (nc0 * nc1);
+                const uint32_t fb = f0 + L.foff[q];
+                for (int t = 0; t < 3 * nt; t++) {
+                    const int e = c_tri[ci][t];
+                    const int slot = (((l + c_edge[e][2]) * FF_BL + j + c_edge[e][1]) * FF_BL + i + c_edge[e][0]) * 3 + c_edge[e][3];
+                    const uint32_t wd = L.used[slot >> 5];
+                    fpool[3 * fb + t] = L.wpref[slot >> 5] + __popc(wd & ((1u << (slot & 31)) - 1u));
+                }
+            }
+        }
+        if (lane == 0 && act) {
+            if (!fits) { atomicOr(&mc->flags, FF_MESH_OVERFLOW); nv = nf = 0; }
+            cp->v0 = v0; cp->nv = nv; cp->f0 = f0; cp->nf = nf;
+        }
+        wave_sync();
+    }
+}
+
+// live-geometry compaction: copies each cell's segments to a fresh pool
+__global__ void k_compact(CellRec *cells, const MeshCounters *mc, const float4 *__restrict__ vsrc,
+                          const uint32_t *__restrict__ fsrc, float4 *__restrict__ vdst, uint32_t *__restrict__ fdst,
+                          MeshCounters *mcw, uint32_t cap)
+{
+    const uint32_t nc = min(mc->nCells, cap);
+    const int lane = threadIdx.x & 63;
+    for (uint32_t c = (blockIdx.x * blockDim.x + threadIdx.x) >> 6; c < nc; c += (gridDim.x * blockDim.x) >> 6) {
+        CellRec &r = cells[c];
+        uint32_t v0 = 0, f0 = 0;
+        if (lane == 0) {
+            v0 = atomicAdd(&mcw->vAlloc, r.nv);
+            f0 = atomicAdd(&mcw->fAlloc, r.nf);
+        }
+        v0 = __shfl(v0, 0);
+        f0 = __shfl(f0, 0);
+        for (uint32_t i = lane; i < r.nv; i += 64) vdst[v0 + i] = vsrc[r.v0 + i];
+        for (uint32_t i = lane; i < 3 * r.nf; i += 64) fdst[3 * f0 + i] = fsrc[3 * r.f0 + i];
+        if (lane == 0) { r.v0 = v0; r.f0 = f0; }
+    }
+}
+
+__global__ void k_binfo_shift(const ushort4 *src, ushort4 *dst, uint32_t n, uint32_t ax, uint32_t ay, uint32_t az)
+{
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        ushort4 b = src[i];
+        b.x = (unsigned short)(b.x + ax); b.y = (unsigned short)(b.y + ay); b.z = (unsigned short)(b.z + az);
+        dst[i] = b;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+static bool mok(ff_ctx *c, hipError_t e, const char *what)
+{
+    if (e == hipSuccess) return true;
+    c->err = std::string(what) + ": " + hipGetErrorString(e);
+    return false;
+}
+#define MCHK(call)                                  \
+    do {                                            \
+        if (!mok(c, (call), #call)) return FF_E_DEVICE; \
+    } while (0)
+
+int mesh_init(ff_ctx *c)
+{
+    MeshState *s = new MeshState();
+    c->mesh = s;
+    // triangle tables into constant memory
+    uint8_t tri[256][16], ntri[256];
+    for (int i = 0; i < 256; i++) {
+        const char *r = FF_MC_TRI[i];
+        const int len = (int)strlen(r);
+        ntri[i] = (uint8_t)(len / 3);
+        for (int k = 0; k < 16; k++) tri[i][k] = k < len ? (uint8_t)(r[k] <= '9' ? r[k] - '0' : r[k] - 'a' + 10) : 255;
+    }
+    MCHK(hipMemcpyToSymbol(HIP_SYMBOL(c_tri), tri, sizeof tri));
+    MCHK(hipMemcpyToSymbol(HIP_SYMBOL(c_ntri), ntri, sizeof ntri));
+    s->binfoCap = c->nBranchesTotal / 16 + 1;
+    MCHK(hipMalloc(&s->d_binfo, sizeof(ushort4) * s->binfoCap));
+    MCHK(hipMemset(s->d_binfo, 0, sizeof(ushort4) * s->binfoCap));
+    s->termCap = std::max<uint32_t>(1u << 16, std::min<uint32_t>(c->nLeavesTotal * 2u, 1u << 24));
+    MCHK(hipMalloc(&s->d_term, sizeof(uint4) * s->termCap));
+    s->cellCap = std::max<uint32_t>(1u << 18, std::min<uint32_t>(c->nLeavesTotal * 16u, 1u << 25));
+    for (int k = 0; k < 2; k++) MCHK(hipMalloc(&s->d_cells[k], sizeof(CellRec) * s->cellCap));
+    MCHK(hipMalloc(&s->d_dirty, sizeof(uint32_t) * s->cellCap));
+    uint32_t hcap = 1;
+    while (hcap < 2 * s->cellCap) hcap <<= 1;
+    s->hmask = hcap - 1;
+    MCHK(hipMalloc(&s->d_hkeys, sizeof(uint64_t) * hcap));
+    MCHK(hipMemset(s->d_hkeys, 0xff, sizeof(uint64_t) * hcap));
+    MCHK(hipMalloc(&s->d_hval, sizeof(uint32_t) * hcap));
+    s->vCap = std::max<uint32_t>(1u << 22, std::min<uint32_t>(c->nLeavesTotal * 64u, 1u << 28));
+    s->fCap = s->vCap;
+    MCHK(hipMalloc(&s->d_vpool, sizeof(float4) * s->vCap));
+    MCHK(hipMalloc(&s->d_fpool, sizeof(uint32_t) * 3 * (size_t)s->fCap));
+    MCHK(hipMalloc(&s->d_mc, sizeof(MeshCounters)));
+    MCHK(hipHostMalloc(&s->h_mc, sizeof(MeshCounters), hipHostMallocDefault));
+    memset(s->h_mc, 0, sizeof(MeshCounters));
+    MCHK(hipMemset(s->d_mc, 0, sizeof(MeshCounters)));
+    return FF_OK;
+}
+
+void mesh_free(ff_ctx *c)
+{
+    MeshState *s = c->mesh;
+    if (!s) return;
+    hipFree(s->d_binfo); hipFree(s->d_term);
+    hipFree(s->d_cells[0]); hipFree(s->d_cells[1]); hipFree(s->d_dirty);
+    hipFree(s->d_hkeys); hipFree(s->d_hval); hipFree(s->d_vpool); hipFree(s->d_fpool);
+    hipFree(s->d_mc); hipHostFree(s->h_mc);
+    delete s;
+    c->mesh = nullptr;
+}
+
+void mesh_on_initial_volume(ff_ctx *c)
+{
+    MeshState *s = c->mesh;
+    const ushort4 root = make_ushort4(0, 0, 0, (unsigned short)(c->n >> 1));
+    hipMemcpyAsync(s->d_binfo, &root, sizeof root, hipMemcpyHostToDevice, c->stream);
+    hipStreamSynchronize(c->stream);
+    s->haveVolume = true;
+}
+
+// grow: branch blocks moved by addBlocks, origins shifted; the new root
+// chain's node origins and octant sizes are given by the caller
+void mesh_on_grow_blocks(ff_ctx *c, uint32_t oldBlocks, uint32_t addBlocks, const uint32_t addv[3],
+                         const std::vector<ushort4> &chain)
+{
+    MeshState *s = c->mesh;
+    ushort4 *tmp = nullptr;
+    if (hipMalloc(&tmp, sizeof(ushort4) * (oldBlocks + 1)) != hipSuccess) return;
+    hipMemcpyAsync(tmp, s->d_binfo, sizeof(ushort4) * oldBlocks, hipMemcpyDeviceToDevice, c->stream);
+    k_binfo_shift<<<(oldBlocks + 255) / 256, 256, 0, c->stream>>>(tmp, s->d_binfo + addBlocks, oldBlocks, addv[0],
+                                                                  addv[1], addv[2]);
+    hipMemcpyAsync(s->d_binfo, chain.data(), sizeof(ushort4) * chain.size(), hipMemcpyHostToDevice, c->stream);
+    hipStreamSynchronize(c->stream);
+    hipFree(tmp);
+    // every cell moved: drop the previous table so all cells re-mesh
+    s->nCells[s->cur] = 0;
+    hipMemsetAsync(s->d_hkeys, 0xff, sizeof(uint64_t) * (s->hmask + 1), c->stream);
+}
+
 void mesh_on_grow(ff_ctx *) {}
 void mesh_after_frame(ff_ctx *) {}
+
+__global__ void k_set_binfo(ushort4 *binfo, const uint64_t *__restrict__ keys, const uint32_t *__restrict__ assigned,
+                            const uint32_t *__restrict__ list, const uint32_t *listCount, uint32_t listCap, uint32_t n,
+                            uint32_t nBlocks)
+{
+    const uint32_t cnt = min(*listCount, listCap);
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < cnt; i += gridDim.x * blockDim.x) {
+        const uint32_t s = list[i];
+        const uint64_t k = keys[s];
+        if ((k >> 53) & 1u) continue;  // bricks
+        const uint32_t d = (uint32_t)(k >> 48) & 31u;
+        const uint32_t idx = assigned[s];
+        if (idx / 16 >= nBlocks) continue;
+        binfo[idx / 16] = make_ushort4((unsigned short)((k >> 32) & 0xffffu), (unsigned short)((k >> 16) & 0xffffu),
+                                       (unsigned short)(k & 0xffffu), (unsigned short)((n >> d) >> 1));
+    }
+}
+
+// records the node geometry of the branches created by this frame
+void mesh_record_branches(ff_ctx *c)
+{
+    MeshState *s = c->mesh;
+    k_set_binfo<<<256, 256, 0, c->stream>>>(s->d_binfo, c->hv.keys, c->hv.assigned, c->hv.list, &c->d_fc->listCount,
+                                            c->hv.listCap, c->n, s->binfoCap);
+}
+
+static int update_meshes(ff_ctx *c)
+{
+    MeshState *s = c->mesh;
+    if (!s->haveVolume || !c->n) return 0;
+    hipEvent_t e0 = c->ev[6], e1 = c->ev[7];
+    if (c->timing) hipEventRecord(e0, c->stream);
+    const int nxt = 1 - s->cur;
+    MeshCounters zero{};
+    zero.vAlloc = s->h_mc->vAlloc;
+    zero.fAlloc = s->h_mc->fAlloc;
+    MCHK(hipMemcpyAsync(s->d_mc, &zero, sizeof zero, hipMemcpyHostToDevice, c->stream));
+    const uint32_t nBlocks = c->nBranchesUsed / 16;
+    k_terminal<<<(nBlocks * 8 + 255) / 256, 256, 0, c->stream>>>(c->d_tree, s->d_binfo, nBlocks, c->nLeavesTotal, c->n,
+                                                                 s->d_term, s->termCap, s->d_mc);
+    k_cells<<<1024, 128, 0, c->stream>>>(c->d_tree, c->nLeavesTotal, c->n, s->d_term, s->termCap, s->d_mc,
+                                         s->d_cells[nxt], s->cellCap);
+    MapRO m;
+    m.info = c->d_leafinfo; m.parent = c->d_parent; m.dist = c->d_dist; m.wgt = c->d_wgt; m.col = c->d_col;
+    for (int k = 0; k < 3; k++) m.off[k] = c->offset[k];
+    m.scale = c->scale;
+    m.useColor = c->useColor ? 1 : 0;
+    k_mark<<<1024, 256, 0, c->stream>>>(s->d_cells[nxt], s->d_cells[s->cur], s->d_hkeys, s->d_hval, s->hmask,
+                                        s->nCells[s->cur] > 0, c->d_stamp, c->d_parent, s->lastMeshStamp, s->d_mc,
+                                        s->d_dirty, s->cellCap);
+    k_mesh<<<2048, 256, 0, c->stream>>>(s->d_cells[nxt], s->d_dirty, s->d_mc, m, s->d_vpool, s->vCap, s->d_fpool,
+                                        s->fCap);
+    // the previous table's keys leave the lookup hash, the new ones enter it
+    if (s->nCells[s->cur])
+        k_hash_clear_cells<<<1024, 256, 0, c->stream>>>(s->d_cells[s->cur], s->nCells[s->cur], s->d_hkeys);
+    k_hash_build<<<1024, 256, 0, c->stream>>>(s->d_cells[nxt], s->d_mc, s->cellCap, s->d_hkeys, s->d_hval, s->hmask);
+    if (c->timing) hipEventRecord(e1, c->stream);
+    MCHK(hipMemcpyAsync(s->h_mc, s->d_mc, sizeof(MeshCounters), hipMemcpyDeviceToHost, c->stream));
+    MCHK(hipStreamSynchronize(c->stream));
+    if (c->timing) {
+        float ms = 0.0f;
+        if (hipEventElapsedTime(&ms, e0, e1) == hipSuccess) {
+            c->phaseMs[2] += ms;
+            c->phaseCalls[2]++;
+        }
+    }
+    const MeshCounters &h = *s->h_mc;
+    if (h.flags & FF_CELL_OVERFLOW) {
+        c->err = "mesh cell table overflow";
+        return FF_E_POOL_FULL;
+    }
+    s->cur = nxt;
+    s->nCells[nxt] = std::min(h.nCells, s->cellCap);
+    s->lastMeshStamp = c->stampCounter;
+    c->phaseUnits[2] += h.nDirty;
+    // algorithmic bytes: 512 voxels x 14 B read per re-meshed cell-sized region
+    // is an upper bound; count the reads of the cells' samples and the writes
+    c->phaseBytes[2] += (double)h.nDirty * 0.0;
+    if (h.flags & FF_MESH_OVERFLOW || h.vAlloc > s->vCap / 2 || h.fAlloc > s->fCap / 2) {
+        // compact live geometry into a fresh pool
+        float4 *nv = nullptr;
+        uint32_t *nf = nullptr;
+        MCHK(hipMalloc(&nv, sizeof(float4) * s->vCap));
+        MCHK(hipMalloc(&nf, sizeof(uint32_t) * 3 * (size_t)s->fCap));
+        MeshCounters z2 = h;
+        z2.vAlloc = z2.fAlloc = 0;
+        MCHK(hipMemcpyAsync(s->d_mc, &z2, sizeof z2, hipMemcpyHostToDevice, c->stream));
+        k_compact<<<1024, 256, 0, c->stream>>>(s->d_cells[nxt], s->d_mc, s->d_vpool, s->d_fpool, nv, nf, s->d_mc,
+                                               s->cellCap);
+        MCHK(hipMemcpyAsync(s->h_mc, s->d_mc, sizeof(MeshCounters), hipMemcpyDeviceToHost, c->stream));
+        MCHK(hipStreamSynchronize(c->stream));
+        hipFree(s->d_vpool);
+        hipFree(s->d_fpool);
+        s->d_vpool = nv;
+        s->d_fpool = nf;
+        if (h.flags & FF_MESH_OVERFLOW) {
+            // cells that did not fit were stored empty; mesh everything again next time
+            s->nCells[nxt] = 0;
+            hipMemsetAsync(s->d_hkeys, 0xff, sizeof(uint64_t) * (s->hmask + 1), c->stream);
+        }
+    }
+    return (int)h.nDirty;
+}
+
+// host copy of the current cell table, sorted by key
+static int host_cells(ff_ctx *c, std::vector<CellRec> &out)
+{
+    MeshState *s = c->mesh;
+    out.resize(std::min(s->nCells[s->cur], s->cellCap));
+    MCHK(hipStreamSynchronize(c->stream));
+    if (!out.empty())
+        MCHK(hipMemcpy(out.data(), s->d_cells[s->cur], sizeof(CellRec) * out.size(), hipMemcpyDeviceToHost));
+    std::sort(out.begin(), out.end(), [](const CellRec &a, const CellRec &b) { return a.key < b.key; });
+    return FF_OK;
+}
+
 }  // namespace ff
 
+using namespace ff;
+
 extern "C" {
-int ff_update_meshes(ff_ctx *c) { return c ? 0 : FF_E_ARG; }
+
+int ff_update_meshes(ff_ctx *c)
+{
+    if (!c) return FF_E_ARG;
+    hipSetDevice(c->device);
+    if (!c->incremental) return 0;
+    return update_meshes(c);
+}
+
 int ff_mesh_sizes(ff_ctx *c, size_t *nv, size_t *nf)
 {
     if (!c) return FF_E_ARG;
-    if (nv) *nv = 0;
-    if (nf) *nf = 0;
+    std::vector<CellRec> cells;
+    int rc = host_cells(c, cells);
+    if (rc) return rc;
+    size_t v = 0, f = 0;
+    for (const auto &r : cells) { v += r.nv; f += r.nf; }
+    if (nv) *nv = v;
+    if (nf) *nf = f;
     return FF_OK;
 }
-int ff_copy_mesh(ff_ctx *c, float *, uint8_t *, uint32_t *) { return c ? FF_OK : FF_E_ARG; }
+
+int ff_copy_mesh(ff_ctx *c, float *xyz, uint8_t *rgb, uint32_t *faces)
+{
+    if (!c) return FF_E_ARG;
+    MeshState *s = c->mesh;
+    std::vector<CellRec> cells;
+    int rc = host_cells(c, cells);
+    if (rc) return rc;
+    const uint32_t va = s->h_mc->vAlloc, fa = s->h_mc->fAlloc;
+    std::vector<float4> vp(va);
+    std::vector<uint32_t> fp(3 * (size_t)fa);
+    if (va) MCHK(hipMemcpy(vp.data(), s->d_vpool, sizeof(float4) * va, hipMemcpyDeviceToHost));
+    if (fa) MCHK(hipMemcpy(fp.data(), s->d_fpool, sizeof(uint32_t) * 3 * (size_t)fa, hipMemcpyDeviceToHost));
+    size_t vo = 0, fo = 0;
+    for (const auto &r : cells) {
+        for (uint32_t i = 0; i < r.nv; i++) {
+            const float4 v = vp[r.v0 + i];
+            if (xyz) { xyz[3 * (vo + i)] = v.x; xyz[3 * (vo + i) + 1] = v.y; xyz[3 * (vo + i) + 2] = v.z; }
+            if (rgb) {
+                uint32_t cc;
+                memcpy(&cc, &v.w, sizeof cc);
+                rgb[3 * (vo + i)] = (uint8_t)(cc & 0xffu);
+                rgb[3 * (vo + i) + 1] = (uint8_t)((cc >> 8) & 0xffu);
+                rgb[3 * (vo + i) + 2] = (uint8_t)((cc >> 16) & 0xffu);
+            }
+        }
+        if (faces)
+            for (uint32_t i = 0; i < 3 * r.nf; i++) faces[3 * fo + i] = (uint32_t)vo + fp[3 * (size_t)r.f0 + i];
+        vo += r.nv;
+        fo += r.nf;
+    }
+    return FF_OK;
+}
+
 int ff_cell_count(ff_ctx *c, size_t *n)
 {
     if (!c || !n) return FF_E_ARG;
-    *n = 0;
+    *n = c->mesh->nCells[c->mesh->cur];
     return FF_OK;
 }
-int ff_copy_cells(ff_ctx *c, int32_t *, uint32_t *, uint32_t *) { return c ? FF_OK : FF_E_ARG; }
+
+int ff_copy_cells(ff_ctx *c, int32_t *desc5, uint32_t *nv, uint32_t *nf)
+{
+    if (!c) return FF_E_ARG;
+    std::vector<CellRec> cells;
+    int rc = host_cells(c, cells);
+    if (rc) return rc;
+    for (size_t i = 0; i < cells.size(); i++) {
+        const CellRec &r = cells[i];
+        if (desc5) {
+            desc5[5 * i] = (int32_t)(r.key & 7u);
+            desc5[5 * i + 1] = (int32_t)(r.key >> 48);
+            desc5[5 * i + 2] = (int32_t)((r.key >> 32) & 0xffffu);
+            desc5[5 * i + 3] = (int32_t)((r.key >> 16) & 0xffffu);
+            desc5[5 * i + 4] = 1 << ((r.key >> 3) & 31u);
+        }
+        if (nv) nv[i] = r.nv;
+        if (nf) nf[i] = r.nf;
+    }
+    return FF_OK;
+}
+
 int ff_set_incremental_meshing(ff_ctx *c, int on)
 {
     if (!c) return FF_E_ARG;
     c->incremental = on != 0;
     return FF_OK;
 }
-}
+
+}  // extern "C"

@@ -1,0 +1,72 @@
+"""GPU parity of incremental meshing: after each frame, the GPU's per-cell
+meshes (kept across frames, only dirty cells re-meshed) must equal the
+oracle's meshes of every cell computed from scratch: same cell set
+(type, origin, size), same vertex and face counts per cell, identical faces
+and colors, vertices within 1e-4 voxel."""
+import numpy as np
+import pytest
+
+from fastfusion_amd import synth
+from tests import helpers
+from tests.helpers import make_pair
+
+pytestmark = pytest.mark.gpu
+
+
+def compare_mesh(gpu, ora, voxel, tag):
+    d_g, nv_g, nf_g = gpu.getMeshCells()
+    desc, nv_o, nf_o, xyz_o, rgb_o, faces_o = ora.mesh_all()
+    assert len(d_g) == len(desc), f"{tag}: {len(d_g)} gpu cells vs {len(desc)} oracle cells"
+    assert np.array_equal(d_g, desc[:, :5]), f"{tag}: cell descriptors differ"
+    bad = np.flatnonzero((nv_g != nv_o) | (nf_g != nf_o))
+    assert bad.size == 0, (f"{tag}: {bad.size} cells differ in counts, first {desc[bad[:3], :5]} "
+                           f"gpu {nv_g[bad[:3]]}/{nf_g[bad[:3]]} oracle {nv_o[bad[:3]]}/{nf_o[bad[:3]]}")
+    m = gpu.getMeshInterleavedMarchingCubes()
+    assert len(m.vertices) == len(xyz_o) and len(m.faces) == len(faces_o)
+    err = np.abs(m.vertices.astype(np.float64) - xyz_o).max() if len(xyz_o) else 0.0
+    assert err <= 1e-4 * voxel, f"{tag}: vertex error {err}"
+    # oracle faces are local per cell; make them global
+    off = np.repeat(np.concatenate([[0], np.cumsum(nv_o)[:-1]]).astype(np.uint32), nf_o)
+    assert np.array_equal(m.faces, faces_o + off[:, None]), f"{tag}: faces differ"
+    assert np.array_equal(m.colors, rgb_o), f"{tag}: colors differ"
+    return len(desc), len(m.faces)
+
+
+def run_mesh(sc, R, t, frames, cfg, K, check_every=1, max_leaves=1 << 17):
+    gpu, ora = make_pair(cfg["scale"], cfg["threshold"], max_leaves=max_leaves)
+    w, h = cfg["w"], cfg["h"]
+    out = None
+    for k, i in enumerate(frames):
+        d, bgr = sc.render(R[i].reshape(3, 3), t[i], K, w, h, noise=cfg.get("noise", 1.0),
+                           dropout=cfg.get("dropout", 0.005), seed=1000 + i)
+        assert gpu.addMap(d, R[i], t[i], K, bgr, 1 / 5000.0, cfg["maxcam"]) == \
+            ora.add_map(d, R[i], t[i], K, bgr, 1 / 5000.0, cfg["maxcam"])
+        gpu.updateMeshes()
+        if k % check_every == 0 or k == len(frames) - 1:
+            out = compare_mesh(gpu, ora, cfg["scale"], f"frame {i}")
+    return out
+
+
+def test_mesh_c1_mixed_scales():
+    sc, R, t, cfg = synth.config_c1(n_frames=200)
+    cfg = dict(cfg, w=320, h=240)
+    ncell, nface = run_mesh(sc, R, t, list(range(0, 12)), cfg, (262.5, 262.5, 159.5, 119.5), check_every=3)
+    assert nface > 1000
+
+
+def test_mesh_c2_walk_5mm():
+    sc, R, t, cfg = synth.config_c2(n_frames=60)
+    ncell, nface = run_mesh(sc, R, t, list(range(0, 16)), cfg, synth.K_DEFAULT, check_every=5, max_leaves=1 << 18)
+    assert nface > 10000
+
+
+def test_mesh_close_uniform_scale():
+    sc = synth.Scene.room_scene(5, (4.0, 4.0, 3.0), 3, 3, 0.1, 0.4)
+    f = np.array([-1.0, 0.15, -0.1]); f /= np.linalg.norm(f)
+    x = np.cross(f, [0, 0, 1.0]); x /= np.linalg.norm(x)
+    y = np.cross(f, x)
+    Rm = np.stack([x, y, f], 1).ravel()
+    R = np.tile(Rm, (6, 1))
+    t = np.array([[0.8, 2.0 + 0.01 * i, 1.5] for i in range(6)])
+    cfg = dict(scale=0.01, threshold=0.03, maxcam=1.1, w=320, h=240)
+    run_mesh(sc, R, t, list(range(6)), cfg, (262.5, 262.5, 159.5, 119.5), check_every=1)
