@@ -157,85 +157,68 @@ __global__ void k_cells(const uint32_t *__restrict__ tree, uint32_t nLeavesTotal
                         const uint4 *__restrict__ term, uint32_t termCap, MeshCounters *mc, CellRec *cells,
                         uint32_t cap)
 {
+    // one thread per (terminal node, candidate): 0 interior, 1-6 walls,
+    // 7-18 edges, 19-26 corners
     const uint32_t nt = min(mc->nTerm, termCap);
-    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < nt; t += gridDim.x * blockDim.x) {
+    for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < nt * 27u; g += gridDim.x * blockDim.x) {
+        const uint32_t t = g / 27u, cand = g % 27u;
         const uint4 tn = term[t];
         const int p[3] = {(int)(tn.x & 0xffffu), (int)(tn.x >> 16), (int)tn.y};
         const int s = (int)tn.z;
         const uint32_t lt = tn.w;
         int org[3];
         uint32_t ll[8];
-        // interior
-        ll[0] = lt;
-        emit_cell(cells, cap, mc, 0, p[0], p[1], p[2], s, ll, 1);
-        // walls
-        for (int a = 0; a < 3; a++)
-            for (int side = 0; side < 2; side++) {
-                int q[3] = {p[0], p[1], p[2]};
-                q[a] = side ? p[a] + s : p[a] - 1;
-                uint32_t lf;
-                const uint32_t ns = locate(tree, nLeavesTotal, n, q[0], q[1], q[2], org, &lf);
-                if (!ns) continue;
-                const bool own = side ? (ns >= (uint32_t)s) : (ns > (uint32_t)s);
-                if (!own) continue;
-                int o[3] = {p[0], p[1], p[2]};
-                if (!side) o[a] = p[a] - s;
-                ll[side ? 0 : 1] = lt;
-                ll[side ? 1 : 0] = lf;
-                emit_cell(cells, cap, mc, 1 + a, o[0], o[1], o[2], s, ll, 2);
-            }
-        // edges along axis e; thin axes (u, v); slot = ubit + 2 vbit
-        for (int e = 0; e < 3; e++) {
+        if (cand == 0) {
+            ll[0] = lt;
+            emit_cell(cells, cap, mc, 0, p[0], p[1], p[2], s, ll, 1);
+        } else if (cand <= 6) {
+            const int a = (int)(cand - 1) >> 1, side = (int)(cand - 1) & 1;
+            int q[3] = {p[0], p[1], p[2]};
+            q[a] = side ? p[a] + s : p[a] - 1;
+            uint32_t lf;
+            const uint32_t ns = locate(tree, nLeavesTotal, n, q[0], q[1], q[2], org, &lf);
+            if (!ns) continue;
+            const bool own = side ? (ns >= (uint32_t)s) : (ns > (uint32_t)s);
+            if (!own) continue;
+            int o[3] = {p[0], p[1], p[2]};
+            if (!side) o[a] = p[a] - s;
+            ll[side ? 0 : 1] = lt;
+            ll[side ? 1 : 0] = lf;
+            emit_cell(cells, cap, mc, 1 + a, o[0], o[1], o[2], s, ll, 2);
+        } else if (cand <= 18) {
+            const int e = (int)(cand - 7) >> 2, cu = (int)(cand - 7) & 1, cv = ((int)(cand - 7) >> 1) & 1;
             const int u = e == 0 ? 1 : 0, v = e == 2 ? 1 : 2;
-            for (int cv = 0; cv < 2; cv++)
-                for (int cu = 0; cu < 2; cu++) {
-                    const int lu = p[u] + cu * s, lv = p[v] + cv * s;
-                    const int mine = (1 - cu) + 2 * (1 - cv);
-                    uint32_t sz[4];
-                    bool ok = true;
-                    for (int sl = 0; sl < 4; sl++) {
-                        if (sl == mine) { sz[sl] = (uint32_t)s; ll[sl] = lt; continue; }
-                        int q[3] = {p[0], p[1], p[2]};
-                        q[u] = lu - 1 + (sl & 1);
-                        q[v] = lv - 1 + (sl >> 1);
-                        sz[sl] = locate(tree, nLeavesTotal, n, q[0], q[1], q[2], org, &ll[sl]);
-                        if (!sz[sl]) ok = false;
-                    }
-                    if (!ok) continue;
-                    uint32_t mn = sz[0];
-                    for (int sl = 1; sl < 4; sl++) mn = min(mn, sz[sl]);
-                    if (mn != (uint32_t)s) continue;
-                    int first = 0;
-                    while (sz[first] != mn) first++;
-                    if (first != mine) continue;
-                    int o[3];
-                    o[e] = p[e]; o[u] = lu - s; o[v] = lv - s;
-                    emit_cell(cells, cap, mc, 4 + e, o[0], o[1], o[2], s, ll, 4);
-                }
+            const int lu = p[u] + cu * s, lv = p[v] + cv * s;
+            const int mine = (1 - cu) + 2 * (1 - cv);
+            uint32_t sz[4];
+            bool ok = true;
+            for (int sl = 0; sl < 4 && ok; sl++) {
+                if (sl == mine) { sz[sl] = (uint32_t)s; ll[sl] = lt; continue; }
+                int q[3] = {p[0], p[1], p[2]};
+                q[u] = lu - 1 + (sl & 1);
+                q[v] = lv - 1 + (sl >> 1);
+                sz[sl] = locate(tree, nLeavesTotal, n, q[0], q[1], q[2], org, &ll[sl]);
+                // a smaller node ahead of this one owns the cell: stop early
+                if (!sz[sl] || sz[sl] < (uint32_t)s || (sl < mine && sz[sl] == (uint32_t)s)) ok = false;
+            }
+            if (!ok) continue;
+            int o[3];
+            o[e] = p[e]; o[u] = lu - s; o[v] = lv - s;
+            emit_cell(cells, cap, mc, 4 + e, o[0], o[1], o[2], s, ll, 4);
+        } else {
+            const int cb = (int)(cand - 19), cx = cb & 1, cy = (cb >> 1) & 1, cz = cb >> 2;
+            const int pt[3] = {p[0] + cx * s, p[1] + cy * s, p[2] + cz * s};
+            const int mine = (1 - cx) + 2 * (1 - cy) + 4 * (1 - cz);
+            bool ok = true;
+            for (int sl = 0; sl < 8 && ok; sl++) {
+                if (sl == mine) { ll[sl] = lt; continue; }
+                const uint32_t szs = locate(tree, nLeavesTotal, n, pt[0] - 1 + (sl & 1), pt[1] - 1 + ((sl >> 1) & 1),
+                                            pt[2] - 1 + ((sl >> 2) & 1), org, &ll[sl]);
+                if (!szs || szs < (uint32_t)s || (sl < mine && szs == (uint32_t)s)) ok = false;
+            }
+            if (!ok) continue;
+            emit_cell(cells, cap, mc, 7, pt[0] - s, pt[1] - s, pt[2] - s, s, ll, 8);
         }
-        // corners: slot bits (x, y, z) = + side of the point
-        for (int cz = 0; cz < 2; cz++)
-            for (int cy = 0; cy < 2; cy++)
-                for (int cx = 0; cx < 2; cx++) {
-                    const int pt[3] = {p[0] + cx * s, p[1] + cy * s, p[2] + cz * s};
-                    const int mine = (1 - cx) + 2 * (1 - cy) + 4 * (1 - cz);
-                    uint32_t sz[8];
-                    bool ok = true;
-                    for (int sl = 0; sl < 8; sl++) {
-                        if (sl == mine) { sz[sl] = (uint32_t)s; ll[sl] = lt; continue; }
-                        sz[sl] = locate(tree, nLeavesTotal, n, pt[0] - 1 + (sl & 1), pt[1] - 1 + ((sl >> 1) & 1),
-                                        pt[2] - 1 + ((sl >> 2) & 1), org, &ll[sl]);
-                        if (!sz[sl]) ok = false;
-                    }
-                    if (!ok) continue;
-                    uint32_t mn = sz[0];
-                    for (int sl = 1; sl < 8; sl++) mn = min(mn, sz[sl]);
-                    if (mn != (uint32_t)s) continue;
-                    int first = 0;
-                    while (sz[first] != mn) first++;
-                    if (first != mine) continue;
-                    emit_cell(cells, cap, mc, 7, pt[0] - s, pt[1] - s, pt[2] - s, s, ll, 8);
-                }
     }
 }
 
@@ -594,7 +577,35 @@ __global__ __launch_bounds__(256) void k_mesh(CellRec *cells, const uint32_t *__
         if (empty) { n[0] = n[1] = n[2] = 0; }
         const int total = n[0] * n[1] * n[2];
         // -- samples
-        for (int s = lane; s < total; s += 64) {
+        // full interior cell (the cell is its brick): samples are the brick's
+        // voxels exactly; only zero-weight samples fall back along the chain
+        const bool fullInterior = type == 0 && total == FF_BRICK && sizeMin * FF_BL == size;
+        if (fullInterior) {
+            const ushort4 li = m.info[ll[0]];
+            const bool aligned = li.x == o[0] && li.y == o[1] && li.z == o[2];
+            const size_t vb = (size_t)ll[0] * FF_BRICK;
+            for (int id = lane; id < FF_BRICK; id += 64) {
+                const float w = m.wgt[vb + id];
+                if (aligned && w > 0.0f) {
+                    L.d[id] = m.dist[vb + id];
+                    L.w[id] = w;
+                    if (m.useColor) {
+                        const uint16_t *cb = m.col + vb * 3 + id;
+                        L.c[0][id] = cb[0]; L.c[1][id] = cb[FF_BRICK]; L.c[2][id] = cb[2 * FF_BRICK];
+                    } else {
+                        L.c[0][id] = L.c[1][id] = L.c[2][id] = 0;
+                    }
+                } else {
+                    const int X[3] = {o[0] + (id & 7) * sizeMin, o[1] + ((id >> 3) & 7) * sizeMin,
+                                      o[2] + (id >> 6) * sizeMin};
+                    Sample sm;
+                    sample_interior(m, ll[0], X, sm);
+                    L.d[id] = sm.d; L.w[id] = sm.w;
+                    L.c[0][id] = sm.c[0]; L.c[1][id] = sm.c[1]; L.c[2][id] = sm.c[2];
+                }
+            }
+        }
+        for (int s = fullInterior ? total : lane; s < total; s += 64) {
             const int i = s % n[0], j = (s / n[0]) % n[1], l = s / (n[0] * n[1]);
             const int g[3] = {i, j, l};
             int X[3];
@@ -958,7 +969,7 @@ static int update_meshes(ff_ctx *c)
     const uint32_t nBlocks = c->nBranchesUsed / 16;
     k_terminal<<<(nBlocks * 8 + 255) / 256, 256, 0, c->stream>>>(c->d_tree, s->d_binfo, nBlocks, c->nLeavesTotal, c->n,
                                                                  s->d_term, s->termCap, s->d_mc);
-    k_cells<<<1024, 128, 0, c->stream>>>(c->d_tree, c->nLeavesTotal, c->n, s->d_term, s->termCap, s->d_mc,
+    k_cells<<<2048, 256, 0, c->stream>>>(c->d_tree, c->nLeavesTotal, c->n, s->d_term, s->termCap, s->d_mc,
                                          s->d_cells[nxt], s->cellCap);
     MapRO m;
     m.info = c->d_leafinfo; m.parent = c->d_parent; m.dist = c->d_dist; m.wgt = c->d_wgt; m.col = c->d_col;

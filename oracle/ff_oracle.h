@@ -63,6 +63,12 @@ int ffo_mesh_all(ffo_ctx *c);
 void ffo_mesh_counts(const ffo_ctx *c, size_t out[3]);
 void ffo_mesh_copy(const ffo_ctx *c, int32_t *desc13, uint32_t *nv, uint32_t *nf, float *xyz, uint8_t *rgb,
                    uint32_t *faces);
+/* Incremental meshing for the CPU baseline: ffo_mesh_note_frame after each
+ * ffo_add_map records the queued bricks; ffo_mesh_update re-meshes the cells
+ * they affect and concatenates all cell meshes (one context at a time). */
+void ffo_mesh_note_frame(ffo_ctx *c);
+int ffo_mesh_update(ffo_ctx *c);
+size_t ffo_mesh_update_faces(void);
 
 /* unit-level entry points used by the tests */
 float ffo_rcp(float x);

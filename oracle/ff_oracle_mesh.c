@@ -719,3 +719,132 @@ void ffo_mesh_copy(const ffo_ctx *c, int32_t *desc, uint32_t *nv, uint32_t *nf, 
     memcpy(rgb, g_out.mb.rgb, g_out.mb.nv * 3);
     memcpy(faces, g_out.mb.faces, g_out.mb.nf * 3 * sizeof(uint32_t));
 }
+
+/* ---------------- incremental update (CPU baseline) ----------------
+ * updateMeshes semantics for timing: re-mesh the cells whose bricks (their
+ * lastLeafs or parent chains) were updated since the previous call, or that
+ * are new or changed their lastLeafs; keep the other cells' meshes; then
+ * concatenate all cell meshes into one mesh like meshWrapperInterleaved
+ * (geometryfusion_mipmap_cpu.cpp:2820-2880). */
+typedef struct {
+    uint64_t key;
+    uint32_t ll[8];
+    float *xyz;
+    uint8_t *rgb;
+    uint32_t *faces;
+    uint32_t nv, nf;
+    int used;
+} centry;
+
+typedef struct {
+    centry *tab;
+    size_t cap, n;
+    uint32_t *last_queue_frame; /* per leaf: update counter when it was last queued */
+    uint32_t nleaves_cap;
+    uint32_t epoch;
+    meshbuf all;
+} inc_state;
+
+static inc_state g_inc;
+
+static size_t inc_find(uint64_t key)
+{
+    size_t h = (size_t)((key * 0x9e3779b97f4a7c15ull) >> 17) & (g_inc.cap - 1);
+    while (g_inc.tab[h].used && g_inc.tab[h].key != key) h = (h + 1) & (g_inc.cap - 1);
+    return h;
+}
+
+static void inc_rehash(size_t newcap)
+{
+    centry *old = g_inc.tab;
+    size_t oc = g_inc.cap;
+    g_inc.tab = (centry *)calloc(newcap, sizeof(centry));
+    g_inc.cap = newcap;
+    for (size_t i = 0; i < oc; i++)
+        if (old[i].used) g_inc.tab[inc_find(old[i].key)] = old[i];
+    free(old);
+}
+
+static int chain_changed(const mapview *m, uint32_t leaf, uint32_t since)
+{
+    for (int d = 0; d < 16 && leaf < FFO_BRANCHINIT; d++) {
+        if (g_inc.last_queue_frame[leaf] > since) return 1;
+        leaf = m->par[leaf];
+    }
+    return 0;
+}
+
+/* Records the bricks queued by the last add_map (call after each frame). */
+void ffo_mesh_note_frame(ffo_ctx *c)
+{
+    uint32_t cap = ffo_leaves_total(c);
+    if (g_inc.nleaves_cap != cap) {
+        free(g_inc.last_queue_frame);
+        g_inc.last_queue_frame = (uint32_t *)calloc(cap, sizeof(uint32_t));
+        g_inc.nleaves_cap = cap;
+    }
+    g_inc.epoch++;
+    const uint32_t *q = ffo_queue(c);
+    for (uint32_t i = 0; i < ffo_n_queued(c); i++) g_inc.last_queue_frame[q[i]] = g_inc.epoch;
+}
+
+/* Incremental updateMeshes; returns the number of cells re-meshed. */
+int ffo_mesh_update(ffo_ctx *c)
+{
+    static uint32_t last_epoch = 0;
+    mapview m;
+    make_view(c, &m);
+    if (!m.n) return 0;
+    if (!g_inc.tab) {
+        g_inc.cap = 1 << 16;
+        g_inc.tab = (centry *)calloc(g_inc.cap, sizeof(centry));
+    }
+    cellvec all = {0};
+    int o[3] = {0, 0, 0};
+    walk(&m, 0, o, m.n >> 1, FFO_BRANCHINIT, &all);
+    size_t k = 0;
+    for (size_t i = 0; i < all.n; i++)
+        if (required_present(&all.cells[i])) all.cells[k++] = all.cells[i];
+    all.n = k;
+    while (2 * all.n > g_inc.cap) inc_rehash(2 * g_inc.cap);
+    int remeshed = 0;
+    meshbuf tmp = {0};
+    for (size_t i = 0; i < all.n; i++) {
+        const cell_t *cl = &all.cells[i];
+        uint64_t key = cell_key(cl);
+        size_t h = inc_find(key);
+        centry *e = &g_inc.tab[h];
+        int dirty = !e->used || memcmp(e->ll, cl->ll, sizeof e->ll) != 0;
+        for (int s = 0; s < 8 && !dirty; s++)
+            if (cl->ll[s] < FFO_BRANCHINIT && chain_changed(&m, cl->ll[s], last_epoch)) dirty = 1;
+        if (!dirty) continue;
+        tmp.nv = tmp.nf = 0;
+        mesh_cell(&m, cl, &tmp);
+        if (!e->used) { e->used = 1; e->key = key; g_inc.n++; }
+        memcpy(e->ll, cl->ll, sizeof e->ll);
+        e->xyz = (float *)realloc(e->xyz, (tmp.nv ? tmp.nv : 1) * 3 * sizeof(float));
+        e->rgb = (uint8_t *)realloc(e->rgb, (tmp.nv ? tmp.nv : 1) * 3);
+        e->faces = (uint32_t *)realloc(e->faces, (tmp.nf ? tmp.nf : 1) * 3 * sizeof(uint32_t));
+        memcpy(e->xyz, tmp.xyz, tmp.nv * 3 * sizeof(float));
+        memcpy(e->rgb, tmp.rgb, tmp.nv * 3);
+        memcpy(e->faces, tmp.faces, tmp.nf * 3 * sizeof(uint32_t));
+        e->nv = (uint32_t)tmp.nv;
+        e->nf = (uint32_t)tmp.nf;
+        remeshed++;
+    }
+    free(tmp.xyz); free(tmp.rgb); free(tmp.faces);
+    /* concatenate every current cell's mesh (the reference's per-update copy) */
+    g_inc.all.nv = g_inc.all.nf = 0;
+    for (size_t i = 0; i < all.n; i++) {
+        centry *e = &g_inc.tab[inc_find(cell_key(&all.cells[i]))];
+        uint32_t base = (uint32_t)g_inc.all.nv;
+        for (uint32_t v = 0; v < e->nv; v++) mb_vertex(&g_inc.all, e->xyz + 3 * v, e->rgb + 3 * v);
+        for (uint32_t f = 0; f < e->nf; f++)
+            mb_face(&g_inc.all, base + e->faces[3 * f], base + e->faces[3 * f + 1], base + e->faces[3 * f + 2]);
+    }
+    free(all.cells);
+    last_epoch = g_inc.epoch;
+    return remeshed;
+}
+
+size_t ffo_mesh_update_faces(void) { return g_inc.all.nf; }

@@ -55,6 +55,11 @@ def _mesh_sigs(lib):
     lib.ffo_mesh_all.argtypes = [P]
     lib.ffo_mesh_counts.argtypes = [P, P]
     lib.ffo_mesh_copy.argtypes = [P, P, P, P, P, P, P]
+    lib.ffo_mesh_note_frame.argtypes = [P]
+    lib.ffo_mesh_update.restype = ctypes.c_int
+    lib.ffo_mesh_update.argtypes = [P]
+    lib.ffo_mesh_update_faces.restype = ctypes.c_size_t
+    lib.ffo_mesh_update_faces.argtypes = []
 
 
 def _vp(a):
@@ -91,8 +96,10 @@ class Oracle:
         depth = np.ascontiguousarray(depth, np.uint16)
         h, w = depth.shape
         bgr = np.ascontiguousarray(bgr, np.uint8) if bgr is not None else np.zeros((h, w, 3), np.uint8)
-        return self.lib.ffo_add_map(self.h, _vp(depth), _vp(bgr), w, h, _dp(R), _dp(t), _dp(K),
-                                    float(scaling), float(maxcam))
+        nq = self.lib.ffo_add_map(self.h, _vp(depth), _vp(bgr), w, h, _dp(R), _dp(t), _dp(K),
+                                  float(scaling), float(maxcam))
+        self.lib.ffo_mesh_note_frame(self.h)
+        return nq
 
     def counts(self):
         off = np.zeros(3, np.float32)
@@ -125,6 +132,11 @@ class Oracle:
 
     def processed(self, w, h):
         return _arr(self.lib.ffo_processed(self.h), np.uint8, w * h).reshape(h, w)
+
+    def update_meshes(self):
+        """Incremental re-mesh of the cells affected since the last call plus the
+        concatenation of all cell meshes (CPU-baseline timing path)."""
+        return self.lib.ffo_mesh_update(self.h)
 
     def mesh_all(self):
         """Meshes every meshable cell from scratch.  Returns (desc (n, 13): type,

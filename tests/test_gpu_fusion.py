@@ -40,3 +40,21 @@ def test_small_image_no_color_320():
         run_pair(sc, R, t, list(range(0, 8)), cfg, gpu, ora, check_every=2)
     finally:
         helpers.K = old
+
+
+def test_grow_parity():
+    """A first view close to a wall, then views across the room: the tree
+    grows upward (grow(), geometryfusion_mipmap_cpu.cpp:3071-3319), and the
+    GPU's tree, bricks and TSDF stay bit-identical to the oracle's."""
+    from tests.test_cpu_oracle import grow_sequence
+    sc, R, t = grow_sequence()
+    gpu, ora = make_pair(0.02, 0.06)
+    cfg = dict(w=320, h=240, maxcam=6.0)
+    from tests import helpers
+    old = helpers.K
+    helpers.K = (262.5, 262.5, 159.5, 119.5)
+    try:
+        run_pair(sc, R, t, list(range(len(R))), cfg, gpu, ora)
+    finally:
+        helpers.K = old
+    assert gpu.counts()["grows"] >= 1

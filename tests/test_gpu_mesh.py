@@ -70,3 +70,10 @@ def test_mesh_close_uniform_scale():
     t = np.array([[0.8, 2.0 + 0.01 * i, 1.5] for i in range(6)])
     cfg = dict(scale=0.01, threshold=0.03, maxcam=1.1, w=320, h=240)
     run_mesh(sc, R, t, list(range(6)), cfg, (262.5, 262.5, 159.5, 119.5), check_every=1)
+
+
+def test_mesh_after_grow():
+    from tests.test_cpu_oracle import grow_sequence
+    sc, R, t = grow_sequence()
+    cfg = dict(scale=0.02, threshold=0.06, maxcam=6.0, w=320, h=240)
+    run_mesh(sc, R, t, list(range(len(R))), cfg, (262.5, 262.5, 159.5, 119.5), check_every=1)
