@@ -64,6 +64,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--calib-copy", type=int, default=0,
+                    help="MiB copied device-to-device once before the run (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported in this build "
@@ -80,6 +82,12 @@ def main():
     print(f"[bench] rendered {n_frames} frames in {time.perf_counter() - t_r:.1f} s", file=sys.stderr, flush=True)
 
     w, h = cfg["w"], cfg["h"]
+    if args.calib_copy:
+        nb = args.calib_copy << 20
+        src, dst = hiprt.DeviceBuffer(nb), hiprt.DeviceBuffer(nb)
+        hiprt.check(hiprt.hip().hipMemcpy(dst.ptr, src.ptr, nb, hiprt.D2D), "calibration copy")
+        hiprt.synchronize()
+        del src, dst
     d_depth = hiprt.DeviceBuffer.from_array(depth)
     d_bgr = hiprt.DeviceBuffer.from_array(bgr)
     fz = FusionMipMapGPU(0, 0, 0, cfg["scale"], cfg["threshold"], 0, True, max_leaves=1 << 21,

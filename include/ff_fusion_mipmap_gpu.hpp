@@ -1,0 +1,117 @@
+// ff_fusion_mipmap_gpu.hpp -- header-only C++ adapter with the member names
+// of the reference's FusionMipMapCPU (src/fusion/geometryfusion_mipmap_cpu.hpp:
+// 111-163) over the C ABI of libffgpu.so (ff_abi.h).
+//
+// A caller of the reference swaps
+//     FusionMipMapCPU *fusion = new FusionMipMapCPU(0,0,0, scale, threshold, 0, true);
+//     fusion->addMap(depth, caminfo, rgb, 1.0f/5000.0f, maxcam);
+//     fusion->updateMeshes();
+//     MeshInterleaved mesh = fusion->getMeshInterleavedMarchingCubes();
+// for
+//     ff::FusionMipMapGPU *fusion = new ff::FusionMipMapGPU(0,0,0, scale, threshold, 0, true);
+//     fusion->addMap(ff::DepthView{...}, pose, ff::ColorView{...}, 1.0f/5000.0f, maxcam);
+//     fusion->updateMeshes();
+//     ff::Mesh mesh = fusion->getMeshInterleavedMarchingCubes();
+// where the views carry the cv::Mat data pointer, size and step, and the pose
+// carries CameraInfo's rotation, translation (camera -> world) and intrinsics.
+#pragma once
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "ff_abi.h"
+
+namespace ff {
+
+struct DepthView {           // CV_16UC1
+    const uint16_t *data;
+    int cols, rows;
+    size_t step;             // bytes per row
+};
+
+struct ColorView {           // CV_8UC3, BGR
+    const uint8_t *data;
+    int cols, rows;
+    size_t step;
+};
+
+struct CameraPose {          // CameraInfo: R, t map camera to world
+    double R[9];             // row-major
+    double t[3];
+    double fx, fy, cx, cy;
+};
+
+struct Mesh {                // MeshInterleaved (src/fusion/mesh_interleaved.hpp:35-69)
+    std::vector<float> vertices;   // 3 per vertex
+    std::vector<uint8_t> colors;   // R, G, B per vertex
+    std::vector<uint32_t> faces;   // 3 per triangle
+};
+
+class FusionMipMapGPU {
+public:
+    // FusionMipMapCPU(offsetX, offsetY, offsetZ, scale, distanceThreshold, n, color):
+    // as in the reference, the offset and n are set by the first frame.
+    FusionMipMapGPU(float /*offsetX*/, float /*offsetY*/, float /*offsetZ*/, float scale, float distanceThreshold,
+                    uint16_t /*n*/ = 0, bool color = true, uint32_t maxLeaves = 0, int device = 0)
+    {
+        ff_params p{};
+        p.scale = scale;
+        p.distance_threshold = distanceThreshold;
+        p.use_color = color ? 1 : 0;
+        p.incremental_meshing = 1;
+        p.max_leaves = maxLeaves;
+        p.device = device;
+        ctx_ = ff_create(&p);
+        if (!ctx_) throw std::runtime_error("ff_create failed");
+    }
+    ~FusionMipMapGPU() { ff_destroy(ctx_); }
+    FusionMipMapGPU(const FusionMipMapGPU &) = delete;
+    FusionMipMapGPU &operator=(const FusionMipMapGPU &) = delete;
+
+    // int addMap(const cv::Mat &depth, CameraInfo caminfo, const cv::Mat &rgb,
+    //            float scaling, float maxcamdistance)
+    int addMap(const DepthView &depth, const CameraPose &cam, const ColorView &rgb, float scaling,
+               float maxcamdistance = 10.0f)
+    {
+        const double K[4] = {cam.fx, cam.fy, cam.cx, cam.cy};
+        return check(ff_add_map(ctx_, depth.data, depth.step, rgb.data, rgb.step, depth.cols, depth.rows, cam.R,
+                                cam.t, K, scaling, maxcamdistance));
+    }
+
+    // bool updateMeshes(): synchronous; always true
+    bool updateMeshes()
+    {
+        check(ff_update_meshes(ctx_));
+        return true;
+    }
+
+    // MeshInterleaved getMeshInterleavedMarchingCubes()
+    Mesh getMeshInterleavedMarchingCubes()
+    {
+        size_t nv = 0, nf = 0;
+        check(ff_mesh_sizes(ctx_, &nv, &nf));
+        Mesh m;
+        m.vertices.resize(3 * nv);
+        m.colors.resize(3 * nv);
+        m.faces.resize(3 * nf);
+        check(ff_copy_mesh(ctx_, m.vertices.data(), m.colors.data(), m.faces.data()));
+        return m;
+    }
+
+    void setIncrementalMeshing(bool on) { check(ff_set_incremental_meshing(ctx_, on ? 1 : 0)); }
+    void setThreadMeshing(bool) {}  // meshing is a device pass; there is no meshing thread
+    void setDepthChecks(int) {}     // the reference's depth checks are commented out (:1627-1700)
+
+    ff_ctx *handle() { return ctx_; }
+
+private:
+    int check(int rc)
+    {
+        if (rc < 0) throw std::runtime_error(std::string("fastfusion_amd: ") + ff_last_error(ctx_));
+        return rc;
+    }
+    ff_ctx *ctx_ = nullptr;
+};
+
+}  // namespace ff
