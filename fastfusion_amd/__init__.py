@@ -52,6 +52,7 @@ ABI = {
     "ff_cell_count": (ctypes.c_int, [_P, ctypes.POINTER(_SZ)]),
     "ff_copy_cells": (ctypes.c_int, [_P, _P, _P, _P]),
     "ff_set_incremental_meshing": (ctypes.c_int, [_P, ctypes.c_int]),
+    "ff_set_mesh_sample_tiles": (ctypes.c_int, [_P, ctypes.c_uint32]),
     "ff_get_counts": (ctypes.c_int, [_P, ctypes.POINTER(_Counts)]),
     "ff_copy_tree": (ctypes.c_int, [_P, _P, _SZ]),
     "ff_copy_leaves": (ctypes.c_int, [_P, _P, _P, _P, _SZ]),
@@ -182,6 +183,11 @@ class FusionMipMapGPU:
 
     def setIncrementalMeshing(self, on=True):
         self._check(self._lib.ff_set_incremental_meshing(self._ctx, int(bool(on))))
+
+    def setMeshSampleTiles(self, tiles):
+        """Capacity of the meshing sample buffer in 64-sample tiles (no
+        counterpart in the reference; cells beyond it are meshed in place)."""
+        self._check(self._lib.ff_set_mesh_sample_tiles(self._ctx, int(tiles)))
 
     # -- getMeshInterleavedMarchingCubes, :3359-3374
     def getMeshInterleavedMarchingCubes(self):

@@ -13,9 +13,11 @@
 //                covering side k (located by descending the tree)
 //   k_mark       a cell keeps last update's mesh when it existed then with
 //                the same bricks and none of the bricks it can sample (its
-//                lastLeafs and their parent chains) changed since
-//   k_mesh       one wave per dirty cell: samples into LDS (reference
-//                sampling rules per cell type, parent-chain fallback), cube
+//                lastLeafs and their parent chains) changed since; a dirty
+//                cell gets its sampling grid and a run of 64-sample tiles
+//   k_sample     one thread per sample of every dirty cell (reference
+//                sampling rules per cell type, parent-chain fallback)
+//   k_mc         one wave per dirty cell: samples into LDS, cube
 //                classification, a bitmap prefix scan numbers the used edges
 //                in raster order, vertices (double VertexInterp) and faces
 //                are written to device pools
@@ -39,7 +41,7 @@ struct CellRec {
 };
 
 struct MeshCounters {
-    uint32_t nTerm, nCells, nDirty, vAlloc, fAlloc, flags, nLive, pad;
+    uint32_t nTerm, nCells, nDirty, vAlloc, fAlloc, flags, nLive, nTiles;
 };
 
 #define FF_MESH_OVERFLOW 1u
@@ -47,6 +49,8 @@ struct MeshCounters {
 
 __constant__ uint8_t c_tri[256][16];  // edges of each triangle, 255-terminated
 __constant__ uint8_t c_ntri[256];
+
+#define FF_TILE_CAP_MAX (1u << 22)  // 4 GiB of samples
 
 struct MeshState {
     ushort4 *d_binfo = nullptr;  // per branch block: origin of the node, octant size
@@ -58,7 +62,10 @@ struct MeshState {
     uint32_t cellCap = 0;
     uint32_t nCells[2] = {0, 0};
     int cur = 0;
-    uint32_t *d_dirty = nullptr;
+    struct CellGeom *d_geom = nullptr;  // per dirty cell
+    uint32_t *d_tiles = nullptr;        // sample tile -> (dirty cell, tile in cell)
+    uint4 *d_sbuf = nullptr;            // 64 samples per tile: d, w, R | G << 16, B
+    uint32_t tileCap = 0;
     uint64_t *d_hkeys = nullptr;  // previous table: key -> index
     uint32_t *d_hval = nullptr;
     uint32_t hmask = 0;
@@ -106,6 +113,20 @@ __device__ __forceinline__ uint64_t cell_key(int type, int ox, int oy, int oz, i
            ((uint64_t)lg << 3) | (uint64_t)type;
 }
 
+// Appends one element per participating lane with a single atomic per wave.
+// Every lane of the wave must call it (want may be false).
+__device__ __forceinline__ uint32_t wave_append(uint32_t *counter, bool want)
+{
+    const uint64_t m = __ballot(want);
+    if (!m) return 0xffffffffu;
+    const int lane = threadIdx.x & 63;
+    const int leader = __ffsll((long long)m) - 1;
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(m));
+    base = __shfl(base, leader);
+    return want ? base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull)) : 0xffffffffu;
+}
+
 // ---------------------------------------------------------------------------
 // k_terminal: covered terminal nodes
 // ---------------------------------------------------------------------------
@@ -114,110 +135,129 @@ __global__ void k_terminal(const uint32_t *__restrict__ tree, const ushort4 *__r
                            MeshCounters *mc)
 {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= nBlocks * 8u) return;
-    const uint32_t b = t >> 3, co = t & 7u;
-    if (tree[b * 16 + 8 + co] != FF_BRANCHINIT) return;
-    const ushort4 bi = binfo[b];
-    const uint32_t half = bi.w;
-    if (half == 0) return;
-    const int x = bi.x + (int)(half * (co & 1u)), y = bi.y + (int)(half * ((co >> 1) & 1u)),
-              z = bi.z + (int)(half * ((co >> 2) & 1u));
-    int org[3];
-    uint32_t lt;
-    const uint32_t s = locate(tree, nLeavesTotal, n, x, y, z, org, &lt);
-    if (s != half || lt == FF_BRANCHINIT) return;
-    const uint32_t i = atomicAdd(&mc->nTerm, 1u);
-    if (i < termCap) term[i] = make_uint4((uint32_t)x | ((uint32_t)y << 16), (uint32_t)z, half, lt);
-    else atomicOr(&mc->flags, FF_CELL_OVERFLOW);
+    bool want = false;
+    uint4 rec = make_uint4(0, 0, 0, 0);
+    if (t < nBlocks * 8u) {
+        const uint32_t b = t >> 3, co = t & 7u;
+        const ushort4 bi = binfo[b];
+        const uint32_t half = bi.w;
+        if (tree[b * 16 + 8 + co] == FF_BRANCHINIT && half != 0) {
+            const int x = bi.x + (int)(half * (co & 1u)), y = bi.y + (int)(half * ((co >> 1) & 1u)),
+                      z = bi.z + (int)(half * ((co >> 2) & 1u));
+            int org[3];
+            uint32_t lt;
+            const uint32_t sz = locate(tree, nLeavesTotal, n, x, y, z, org, &lt);
+            if (sz == half && lt != FF_BRANCHINIT) {
+                want = true;
+                rec = make_uint4((uint32_t)x | ((uint32_t)y << 16), (uint32_t)z, half, lt);
+            }
+        }
+    }
+    const uint32_t i = wave_append(&mc->nTerm, want);
+    if (want) {
+        if (i < termCap) term[i] = rec;
+        else atomicOr(&mc->flags, FF_CELL_OVERFLOW);
+    }
 }
 
 // ---------------------------------------------------------------------------
 // k_cells: cells owned by each covered terminal node
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ void emit_cell(CellRec *cells, uint32_t cap, MeshCounters *mc, int type, int ox, int oy,
-                                          int oz, int size, const uint32_t *ll, int nll)
-{
-    for (int k = 0; k < nll; k++)
-        if (ll[k] >= FF_BRANCHINIT) return;  // pushMeshCellQueue needs every side
-    const uint32_t i = atomicAdd(&mc->nCells, 1u);
-    if (i >= cap) {
-        atomicOr(&mc->flags, FF_CELL_OVERFLOW);
-        return;
-    }
-    CellRec r;
-    r.key = cell_key(type, ox, oy, oz, size);
-    for (int k = 0; k < 8; k++) r.ll[k] = k < nll ? ll[k] : FF_BRANCHINIT;
-    r.v0 = r.nv = r.f0 = r.nf = 0;
-    r.hslot = 0xffffffffu;
-    r.pad[0] = r.pad[1] = r.pad[2] = 0;
-    cells[i] = r;
-}
-
 __global__ void k_cells(const uint32_t *__restrict__ tree, uint32_t nLeavesTotal, uint32_t n,
                         const uint4 *__restrict__ term, uint32_t termCap, MeshCounters *mc, CellRec *cells,
                         uint32_t cap)
 {
-    // one thread per (terminal node, candidate): 0 interior, 1-6 walls,
-    // 7-18 edges, 19-26 corners
+    // one lane per (terminal node, candidate): 0 interior, 1-6 walls,
+    // 7-18 edges, 19-26 corners; the loop bound is uniform per wave so that
+    // the emission ballot sees every lane
     const uint32_t nt = min(mc->nTerm, termCap);
-    for (uint32_t g = blockIdx.x * blockDim.x + threadIdx.x; g < nt * 27u; g += gridDim.x * blockDim.x) {
-        const uint32_t t = g / 27u, cand = g % 27u;
-        const uint4 tn = term[t];
-        const int p[3] = {(int)(tn.x & 0xffffu), (int)(tn.x >> 16), (int)tn.y};
-        const int s = (int)tn.z;
-        const uint32_t lt = tn.w;
-        int org[3];
+    const uint32_t total = nt * 27u;
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t gb = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); gb < total; gb += stride) {
+        const uint32_t g = gb + lane;
+        bool emit = false;
+        int type = 0, o[3] = {0, 0, 0}, s = 1, nll = 0;
         uint32_t ll[8];
-        if (cand == 0) {
-            ll[0] = lt;
-            emit_cell(cells, cap, mc, 0, p[0], p[1], p[2], s, ll, 1);
-        } else if (cand <= 6) {
-            const int a = (int)(cand - 1) >> 1, side = (int)(cand - 1) & 1;
-            int q[3] = {p[0], p[1], p[2]};
-            q[a] = side ? p[a] + s : p[a] - 1;
-            uint32_t lf;
-            const uint32_t ns = locate(tree, nLeavesTotal, n, q[0], q[1], q[2], org, &lf);
-            if (!ns) continue;
-            const bool own = side ? (ns >= (uint32_t)s) : (ns > (uint32_t)s);
-            if (!own) continue;
-            int o[3] = {p[0], p[1], p[2]};
-            if (!side) o[a] = p[a] - s;
-            ll[side ? 0 : 1] = lt;
-            ll[side ? 1 : 0] = lf;
-            emit_cell(cells, cap, mc, 1 + a, o[0], o[1], o[2], s, ll, 2);
-        } else if (cand <= 18) {
-            const int e = (int)(cand - 7) >> 2, cu = (int)(cand - 7) & 1, cv = ((int)(cand - 7) >> 1) & 1;
-            const int u = e == 0 ? 1 : 0, v = e == 2 ? 1 : 2;
-            const int lu = p[u] + cu * s, lv = p[v] + cv * s;
-            const int mine = (1 - cu) + 2 * (1 - cv);
-            uint32_t sz[4];
-            bool ok = true;
-            for (int sl = 0; sl < 4 && ok; sl++) {
-                if (sl == mine) { sz[sl] = (uint32_t)s; ll[sl] = lt; continue; }
+        if (g < total) {
+            const uint32_t t = g / 27u, cand = g % 27u;
+            const uint4 tn = term[t];
+            const int p[3] = {(int)(tn.x & 0xffffu), (int)(tn.x >> 16), (int)tn.y};
+            s = (int)tn.z;
+            const uint32_t lt = tn.w;
+            int org[3];
+            if (cand == 0) {
+                ll[0] = lt;
+                type = 0; o[0] = p[0]; o[1] = p[1]; o[2] = p[2]; nll = 1;
+                emit = true;
+            } else if (cand <= 6) {
+                const int a = (int)(cand - 1) >> 1, side = (int)(cand - 1) & 1;
                 int q[3] = {p[0], p[1], p[2]};
-                q[u] = lu - 1 + (sl & 1);
-                q[v] = lv - 1 + (sl >> 1);
-                sz[sl] = locate(tree, nLeavesTotal, n, q[0], q[1], q[2], org, &ll[sl]);
-                // a smaller node ahead of this one owns the cell: stop early
-                if (!sz[sl] || sz[sl] < (uint32_t)s || (sl < mine && sz[sl] == (uint32_t)s)) ok = false;
+                q[a] = side ? p[a] + s : p[a] - 1;
+                uint32_t lf;
+                const uint32_t ns = locate(tree, nLeavesTotal, n, q[0], q[1], q[2], org, &lf);
+                if (ns && (side ? (ns >= (uint32_t)s) : (ns > (uint32_t)s))) {
+                    o[0] = p[0]; o[1] = p[1]; o[2] = p[2];
+                    if (!side) o[a] = p[a] - s;
+                    ll[side ? 0 : 1] = lt;
+                    ll[side ? 1 : 0] = lf;
+                    type = 1 + a; nll = 2;
+                    emit = true;
+                }
+            } else if (cand <= 18) {
+                const int e = (int)(cand - 7) >> 2, cu = (int)(cand - 7) & 1, cv = ((int)(cand - 7) >> 1) & 1;
+                const int u = e == 0 ? 1 : 0, v = e == 2 ? 1 : 2;
+                const int lu = p[u] + cu * s, lv = p[v] + cv * s;
+                const int mine = (1 - cu) + 2 * (1 - cv);
+                bool ok = true;
+                for (int sl = 0; sl < 4 && ok; sl++) {
+                    if (sl == mine) { ll[sl] = lt; continue; }
+                    int q[3] = {p[0], p[1], p[2]};
+                    q[u] = lu - 1 + (sl & 1);
+                    q[v] = lv - 1 + (sl >> 1);
+                    const uint32_t sz = locate(tree, nLeavesTotal, n, q[0], q[1], q[2], org, &ll[sl]);
+                    // a smaller node, or an equal one in an earlier slot, owns the cell
+                    if (!sz || sz < (uint32_t)s || (sl < mine && sz == (uint32_t)s)) ok = false;
+                }
+                if (ok) {
+                    o[e] = p[e]; o[u] = lu - s; o[v] = lv - s;
+                    type = 4 + e; nll = 4;
+                    emit = true;
+                }
+            } else {
+                const int cb = (int)(cand - 19), cx = cb & 1, cy = (cb >> 1) & 1, cz = cb >> 2;
+                const int pt[3] = {p[0] + cx * s, p[1] + cy * s, p[2] + cz * s};
+                const int mine = (1 - cx) + 2 * (1 - cy) + 4 * (1 - cz);
+                bool ok = true;
+                for (int sl = 0; sl < 8 && ok; sl++) {
+                    if (sl == mine) { ll[sl] = lt; continue; }
+                    const uint32_t sz = locate(tree, nLeavesTotal, n, pt[0] - 1 + (sl & 1),
+                                               pt[1] - 1 + ((sl >> 1) & 1), pt[2] - 1 + ((sl >> 2) & 1), org, &ll[sl]);
+                    if (!sz || sz < (uint32_t)s || (sl < mine && sz == (uint32_t)s)) ok = false;
+                }
+                if (ok) {
+                    o[0] = pt[0] - s; o[1] = pt[1] - s; o[2] = pt[2] - s;
+                    type = 7; nll = 8;
+                    emit = true;
+                }
             }
-            if (!ok) continue;
-            int o[3];
-            o[e] = p[e]; o[u] = lu - s; o[v] = lv - s;
-            emit_cell(cells, cap, mc, 4 + e, o[0], o[1], o[2], s, ll, 4);
-        } else {
-            const int cb = (int)(cand - 19), cx = cb & 1, cy = (cb >> 1) & 1, cz = cb >> 2;
-            const int pt[3] = {p[0] + cx * s, p[1] + cy * s, p[2] + cz * s};
-            const int mine = (1 - cx) + 2 * (1 - cy) + 4 * (1 - cz);
-            bool ok = true;
-            for (int sl = 0; sl < 8 && ok; sl++) {
-                if (sl == mine) { ll[sl] = lt; continue; }
-                const uint32_t szs = locate(tree, nLeavesTotal, n, pt[0] - 1 + (sl & 1), pt[1] - 1 + ((sl >> 1) & 1),
-                                            pt[2] - 1 + ((sl >> 2) & 1), org, &ll[sl]);
-                if (!szs || szs < (uint32_t)s || (sl < mine && szs == (uint32_t)s)) ok = false;
+            // pushMeshCellQueue needs every side's brick
+            for (int k = 0; k < nll && emit; k++)
+                if (ll[k] >= FF_BRANCHINIT) emit = false;
+        }
+        const uint32_t i = wave_append(&mc->nCells, emit);
+        if (emit) {
+            if (i >= cap) {
+                atomicOr(&mc->flags, FF_CELL_OVERFLOW);
+            } else {
+                CellRec r;
+                r.key = cell_key(type, o[0], o[1], o[2], s);
+                for (int k = 0; k < 8; k++) r.ll[k] = k < nll ? ll[k] : FF_BRANCHINIT;
+                r.v0 = r.nv = r.f0 = r.nf = 0;
+                r.hslot = 0xffffffffu;
+                r.pad[0] = r.pad[1] = r.pad[2] = 0;
+                cells[i] = r;
             }
-            if (!ok) continue;
-            emit_cell(cells, cap, mc, 7, pt[0] - s, pt[1] - s, pt[2] - s, s, ll, 8);
         }
     }
 }
@@ -245,70 +285,8 @@ __device__ __forceinline__ bool chain_dirty(const uint32_t *__restrict__ stamp, 
     return false;
 }
 
-__global__ void k_mark(CellRec *cells, const CellRec *__restrict__ prev, const uint64_t *__restrict__ hkeys,
-                       const uint32_t *__restrict__ hval, uint32_t hmask, bool havePrev,
-                       const uint32_t *__restrict__ stamp, const uint32_t *__restrict__ parent, uint32_t since,
-                       MeshCounters *mc, uint32_t *dirty, uint32_t cap)
-{
-    const uint32_t nc = min(mc->nCells, cap);
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nc; i += gridDim.x * blockDim.x) {
-        CellRec &c = cells[i];
-        int found = -1;
-        if (havePrev) {
-            uint32_t h = hkey_hash(c.key, hmask);
-            for (uint32_t pr = 0; pr <= hmask; pr++) {
-                const uint64_t k = hkeys[h];
-                if (k == c.key) { found = (int)hval[h]; break; }
-                if (k == FF_HEMPTY) break;
-                h = (h + 1) & hmask;
-            }
-        }
-        bool clean = found >= 0;
-        if (clean) {
-            const CellRec &p = prev[found];
-            for (int k = 0; k < 8; k++) clean = clean && (p.ll[k] == c.ll[k]);
-            for (int k = 0; k < 8 && clean; k++)
-                if (c.ll[k] < FF_BRANCHINIT && chain_dirty(stamp, parent, c.ll[k], since)) clean = false;
-            if (clean) {
-                c.v0 = p.v0; c.nv = p.nv; c.f0 = p.f0; c.nf = p.nf;
-            }
-        }
-        if (!clean) dirty[atomicAdd(&mc->nDirty, 1u)] = i;
-    }
-}
-
-__global__ void k_hash_build(CellRec *cells, const MeshCounters *mc, uint32_t cap, uint64_t *hkeys, uint32_t *hval,
-                             uint32_t hmask)
-{
-    const uint32_t nc = min(mc->nCells, cap);
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nc; i += gridDim.x * blockDim.x) {
-        const uint64_t key = cells[i].key;
-        uint32_t h = hkey_hash(key, hmask);
-        for (uint32_t pr = 0; pr <= hmask; pr++) {
-            const unsigned long long prevk =
-                atomicCAS((unsigned long long *)&hkeys[h], (unsigned long long)FF_HEMPTY, (unsigned long long)key);
-            if (prevk == FF_HEMPTY || prevk == key) {
-                hval[h] = i;
-                cells[i].hslot = h;
-                break;
-            }
-            h = (h + 1) & hmask;
-        }
-    }
-}
-
-// empties the slots of a whole table (by recorded slot, so concurrent
-// clears cannot cut each other's probe chains)
-__global__ void k_hash_clear_cells(const CellRec *__restrict__ cells, uint32_t nc, uint64_t *hkeys)
-{
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nc; i += gridDim.x * blockDim.x) {
-        const uint32_t h = cells[i].hslot;
-        if (h != 0xffffffffu) hkeys[h] = FF_HEMPTY;
-    }
-}
-
 // ---------------------------------------------------------------------------
-// k_mesh: one wave per dirty cell
+// sampling (per cell type)
 // ---------------------------------------------------------------------------
 struct MapRO {
     const ushort4 *info;     // pos, scale
@@ -463,6 +441,265 @@ __device__ void sample_point(const MapRO &m, uint32_t ll, const int X[3], Sample
     }
 }
 
+__device__ __forceinline__ uint32_t wave_excl_scan_u32(uint32_t v, uint32_t *total)
+{
+    const int lane = threadIdx.x & 63;
+    uint32_t x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_up(x, o);
+        if (lane >= o) x += y;
+    }
+    *total = __shfl(x, 63);
+    return x - v;
+}
+
+
+// Geometry of a dirty cell, fixed when it is marked: origin, size, the
+// sample spacing on full axes (sizeMin), the offset of the two sample planes
+// on thin axes (stretch) and the sample grid n (0 when the cell meshes to
+// nothing).  tile0 is the first 64-sample tile of its samples in the sample
+// buffer, or FF_NO_TILE when the buffer was full and k_mc samples in place.
+struct CellGeom {
+    uint32_t cell;
+    uint32_t tile0;
+    uint16_t o[3];
+    uint16_t size, sizeMin, stretch;
+    uint8_t type, n[3];
+    uint32_t pad;
+};
+
+#define FF_NO_TILE 0xffffffffu     // samples did not fit the buffer: sampled in place
+#define FF_EMPTY_CELL 0xfffffffeu  // the cell meshes to nothing
+
+__device__ __forceinline__ bool axis_full(int type, int k)
+{
+    return type == 0 || (type <= 3 ? k != type - 1 : (type <= 6 ? k == type - 4 : false));
+}
+
+// sampling grid of a cell (per type, from the scales of its lastLeaf bricks;
+// mesh_interleaved_meshcell.cpp interior / wall / edge / corner meshers)
+__device__ void cell_geom(const MapRO &m, uint64_t key, const uint32_t ll[8], CellGeom &g)
+{
+    const int type = (int)(key & 7u);
+    const int size = 1 << ((key >> 3) & 31u);
+    const int o[3] = {(int)(key >> 48), (int)((key >> 32) & 0xffffu), (int)((key >> 16) & 0xffffu)};
+    bool empty = false;
+    int sizeMin = 1, stretch = 1;
+    if (type == 0) {
+        const ushort4 li = m.info[ll[0]];
+        sizeMin = li.w;
+        if (sizeMin > size || size / sizeMin > FF_BL) empty = true;
+        const int lp[3] = {li.x, li.y, li.z}, ext = sizeMin * FF_BL;
+        for (int k = 0; k < 3; k++)
+            if (o[k] + size > lp[k] + ext || o[k] < lp[k]) empty = true;
+    } else if (type <= 3) {
+        const int s0 = m.info[ll[0]].w, s1 = m.info[ll[1]].w;
+        sizeMin = min(s0, s1);
+        stretch = s0;
+        if (stretch > size) empty = true;
+    } else if (type <= 6) {
+        int sc[4];
+        for (int k = 0; k < 4; k++) sc[k] = m.info[ll[k]].w;
+        stretch = max(max(sc[0], sc[1]), sc[2]);
+        sizeMin = min(min(sc[0], sc[1]), min(sc[2], sc[3]));
+        if (stretch > size) empty = true;
+    } else {
+        stretch = m.info[ll[0]].w;
+        for (int k = 1; k < 7; k++) stretch = max(stretch, (int)m.info[ll[k]].w);
+        sizeMin = stretch;
+        if (stretch > size) empty = true;
+    }
+    int n[3];
+    for (int k = 0; k < 3; k++) {
+        n[k] = axis_full(type, k) ? size / sizeMin : 2;
+        if (n[k] > FF_BL) empty = true;  // cannot happen for a tree-derived cell; keeps LDS indexing in range
+    }
+    g.type = (uint8_t)type;
+    for (int k = 0; k < 3; k++) {
+        g.o[k] = (uint16_t)o[k];
+        g.n[k] = empty ? 0 : (uint8_t)n[k];
+    }
+    g.size = (uint16_t)size;
+    g.sizeMin = (uint16_t)sizeMin;
+    g.stretch = (uint16_t)stretch;
+}
+
+__device__ __forceinline__ int geom_total(const CellGeom &g) { return (int)g.n[0] * g.n[1] * g.n[2]; }
+
+// grid coordinate k of sample index gk: full axes step by sizeMin, thin axes
+// take the two planes size - stretch and size past the origin
+__device__ __forceinline__ int geom_coord(const CellGeom &g, int k, int gk)
+{
+    return axis_full(g.type, k) ? (int)g.o[k] + gk * g.sizeMin
+                                : (gk ? (int)g.o[k] + g.size : (int)g.o[k] + g.size - g.stretch);
+}
+
+// sample s (raster order over n) of a cell; ll points at the cell's lastLeafs
+__device__ Sample sample_cell(const MapRO &m, const CellGeom &g, const uint32_t *ll, int s)
+{
+    const int n0 = g.n[0], n1 = g.n[1];
+    const int i = s % n0, j = (s / n0) % n1, l = s / (n0 * n1);
+    const int gg[3] = {i, j, l};
+    int X[3];
+    for (int k = 0; k < 3; k++) X[k] = geom_coord(g, k, gg[k]);
+    Sample sm;
+    const int type = g.type;
+    if (type == 0) {
+        const uint32_t l0 = ll[0];
+        // full interior cell (the cell is its brick): the samples are the
+        // brick's voxels; only zero-weight ones fall back along the chain
+        if (g.size == g.sizeMin * FF_BL && n0 == FF_BL && n1 == FF_BL && g.n[2] == FF_BL) {
+            const ushort4 li = m.info[l0];
+            const size_t q = (size_t)l0 * FF_BRICK + s;
+            const float w = m.wgt[q], d = m.dist[q];
+            uint16_t c0 = 0, c1 = 0, c2 = 0;
+            if (m.useColor) {
+                const uint16_t *cb = m.col + (size_t)l0 * FF_BRICK * 3 + s;
+                c0 = cb[0]; c1 = cb[FF_BRICK]; c2 = cb[2 * FF_BRICK];
+            }
+            if (li.x == g.o[0] && li.y == g.o[1] && li.z == g.o[2] && w > 0.0f) {
+                sm.d = d; sm.w = w;
+                sm.c[0] = c0; sm.c[1] = c1; sm.c[2] = c2;
+                return sm;
+            }
+        }
+        sample_interior(m, l0, X, sm);
+    } else if (type <= 3) {
+        const int a = type - 1, u = a == 0 ? 1 : 0, v = a == 2 ? 1 : 2;
+        sample_wall(m, ll[gg[a]], a, u, v, X, sm);
+    } else if (type <= 6) {
+        const int e = type - 4, u = e == 0 ? 1 : 0, v = e == 2 ? 1 : 2;
+        sample_edge(m, ll[gg[u] + 2 * gg[v]], e, X, sm);
+    } else {
+        sample_point(m, ll[i + 2 * j + 4 * l], X, sm);
+    }
+    return sm;
+}
+
+__device__ __forceinline__ uint4 pack_sample(const Sample &s)
+{
+    return make_uint4(__float_as_uint(s.d), __float_as_uint(s.w), (uint32_t)s.c[0] | ((uint32_t)s.c[1] << 16),
+                      (uint32_t)s.c[2]);
+}
+
+__global__ void k_mark(CellRec *cells, const CellRec *__restrict__ prev, const uint64_t *__restrict__ hkeys,
+                       const uint32_t *__restrict__ hval, uint32_t hmask, bool havePrev,
+                       const uint32_t *__restrict__ stamp, const uint32_t *__restrict__ parent, uint32_t since,
+                       MeshCounters *mc, CellGeom *geom, uint32_t cap, MapRO m, uint32_t *tiles, uint32_t tileCap)
+{
+    const uint32_t nc = min(mc->nCells, cap);
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t ib = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); ib < nc; ib += stride) {
+        const uint32_t i = ib + lane;
+        bool isDirty = false;
+        CellGeom g;
+        uint32_t nt = 0;
+        if (i < nc) {
+            CellRec &c = cells[i];
+            int found = -1;
+            if (havePrev) {
+                uint32_t h = hkey_hash(c.key, hmask);
+                for (uint32_t pr = 0; pr <= hmask; pr++) {
+                    const uint64_t k = hkeys[h];
+                    if (k == c.key) { found = (int)hval[h]; break; }
+                    if (k == FF_HEMPTY) break;
+                    h = (h + 1) & hmask;
+                }
+            }
+            bool clean = found >= 0;
+            if (clean) {
+                const CellRec &p = prev[found];
+                for (int k = 0; k < 8; k++) clean = clean && (p.ll[k] == c.ll[k]);
+                for (int k = 0; k < 8 && clean; k++)
+                    if (c.ll[k] < FF_BRANCHINIT && chain_dirty(stamp, parent, c.ll[k], since)) clean = false;
+                if (clean) {
+                    c.v0 = p.v0; c.nv = p.nv; c.f0 = p.f0; c.nf = p.nf;
+                }
+            }
+            isDirty = !clean;
+            if (isDirty) {
+                uint32_t ll[8];
+                for (int k = 0; k < 8; k++) ll[k] = c.ll[k];
+                cell_geom(m, c.key, ll, g);
+                g.cell = i;
+                g.pad = 0;
+                nt = ((uint32_t)geom_total(g) + 63u) >> 6;
+            }
+        }
+        const uint32_t d = wave_append(&mc->nDirty, isDirty);
+        // sample tiles: one atomic per wave for the wave's total
+        uint32_t ntot;
+        const uint32_t ex = wave_excl_scan_u32(nt, &ntot);
+        uint32_t tb = 0;
+        if (ntot) {
+            if (lane == 0) tb = atomicAdd(&mc->nTiles, ntot);
+            tb = __shfl(tb, 0);
+        }
+        if (isDirty) {
+            const uint32_t t0 = tb + ex;
+            const bool fits = t0 + nt <= tileCap;
+            g.tile0 = nt == 0 ? FF_EMPTY_CELL : (fits ? t0 : FF_NO_TILE);
+            for (uint32_t k = 0; k < nt && t0 + k < tileCap; k++) tiles[t0 + k] = fits ? ((d << 3) | k) : FF_NO_TILE;
+            geom[d] = g;
+        }
+    }
+}
+
+__global__ void k_hash_build(CellRec *cells, const MeshCounters *mc, uint32_t cap, uint64_t *hkeys, uint32_t *hval,
+                             uint32_t hmask)
+{
+    const uint32_t nc = min(mc->nCells, cap);
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nc; i += gridDim.x * blockDim.x) {
+        const uint64_t key = cells[i].key;
+        uint32_t h = hkey_hash(key, hmask);
+        for (uint32_t pr = 0; pr <= hmask; pr++) {
+            const unsigned long long prevk =
+                atomicCAS((unsigned long long *)&hkeys[h], (unsigned long long)FF_HEMPTY, (unsigned long long)key);
+            if (prevk == FF_HEMPTY || prevk == key) {
+                hval[h] = i;
+                cells[i].hslot = h;
+                break;
+            }
+            h = (h + 1) & hmask;
+        }
+    }
+}
+
+// empties the slots of a whole table (by recorded slot, so concurrent
+// clears cannot cut each other's probe chains)
+__global__ void k_hash_clear_cells(const CellRec *__restrict__ cells, uint32_t nc, uint64_t *hkeys)
+{
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nc; i += gridDim.x * blockDim.x) {
+        const uint32_t h = cells[i].hslot;
+        if (h != 0xffffffffu) hkeys[h] = FF_HEMPTY;
+    }
+}
+
+
+// ---------------------------------------------------------------------------
+// k_sample: one thread per sample of the dirty cells' 64-sample tiles
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_sample(const CellGeom *__restrict__ geom, const CellRec *__restrict__ cells,
+                                                const uint32_t *__restrict__ tiles, const MeshCounters *mc,
+                                                uint32_t tileCap, MapRO m, uint4 *__restrict__ sbuf)
+{
+    const uint32_t nt = min(mc->nTiles, tileCap);
+    for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < nt * 64u; t += gridDim.x * blockDim.x) {
+        const uint32_t e = tiles[t >> 6];
+        if (e >= FF_EMPTY_CELL) continue;
+        const CellGeom g = geom[e >> 3];
+        const int s = (int)((e & 7u) * 64u + (t & 63u));
+        if (s >= geom_total(g)) continue;
+        sbuf[t] = pack_sample(sample_cell(m, g, cells[g.cell].ll, s));
+    }
+}
+
+// ---------------------------------------------------------------------------
+// k_mc: one wave per dirty cell -- cube classification, edge numbering,
+// vertices and faces
+// ---------------------------------------------------------------------------
 // VertexInterp, double (mesh.cpp:1266-1300)
 __device__ __forceinline__ void vinterp(const float p1[3], const float p2[3], float v1f, float v2f, float out[3])
 {
@@ -501,30 +738,34 @@ __constant__ int8_t c_edgec[12][2] = {{0, 1}, {1, 2}, {2, 3}, {3, 0}, {4, 5}, {5
 
 struct WaveLds {
     float d[FF_BRICK];
-    float w[FF_BRICK];
     uint16_t c[3][FF_BRICK];
+    uint8_t sf[FF_BRICK];  // bit 0: distance < 0, bit 1: zero weight
     uint32_t used[48];     // 1536 edge slots: (sample * 3 + axis)
     uint32_t wpref[48];    // vertex index of the first slot of each word
-    uint8_t cube[343];
+    uint8_t cube[344];
     uint16_t foff[344];    // face offset per cube
+    uint16_t list[1344];   // used slots in vertex order (at most 3 * 7 * 8 * 8 edges)
 };
 
-__device__ __forceinline__ uint32_t wave_excl_scan_u32(uint32_t v, uint32_t *total)
+__device__ __forceinline__ void put_sample(WaveLds &L, const CellGeom &g, int s, float d, float w, uint32_t c01,
+                                           uint32_t c2)
 {
-    const int lane = threadIdx.x & 63;
-    uint32_t x = v;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        const uint32_t y = __shfl_up(x, o);
-        if (lane >= o) x += y;
-    }
-    *total = __shfl(x, 63);
-    return x - v;
+    const int n0 = g.n[0], n1 = g.n[1];
+    const int i = s % n0, j = (s / n0) % n1, l = s / (n0 * n1);
+    const int id = (l * FF_BL + j) * FF_BL + i;
+    L.d[id] = d;
+    L.c[0][id] = (uint16_t)(c01 & 0xffffu);
+    L.c[1][id] = (uint16_t)(c01 >> 16);
+    L.c[2][id] = (uint16_t)c2;
+    L.sf[id] = (uint8_t)((d < 0.0f ? 1u : 0u) | (w == 0.0f ? 2u : 0u));
 }
 
-__global__ __launch_bounds__(256) void k_mesh(CellRec *cells, const uint32_t *__restrict__ dirty, MeshCounters *mc,
-                                              MapRO m, float4 *__restrict__ vpool, uint32_t vCap,
-                                              uint32_t *__restrict__ fpool, uint32_t fCap)
+// InPlace = false: cells whose samples k_sample wrote to the buffer;
+// InPlace = true: cells that did not fit the buffer, sampled by the wave itself
+template <bool InPlace>
+__global__ __launch_bounds__(256) void k_mc(CellRec *cells, const CellGeom *__restrict__ geom, MeshCounters *mc,
+                                            const uint4 *__restrict__ sbuf, MapRO m, float4 *__restrict__ vpool,
+                                            uint32_t vCap, uint32_t *__restrict__ fpool, uint32_t fCap)
 {
     __shared__ WaveLds lds[4];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -534,104 +775,40 @@ __global__ __launch_bounds__(256) void k_mesh(CellRec *cells, const uint32_t *__
         // waves of a block run independently: each owns lds[wave]
         const uint32_t di = base + wave;
         const bool act = di < nd;
-        CellRec *cp = act ? &cells[dirty[di]] : nullptr;
-        int type = 0, size = 1, o[3] = {0, 0, 0};
-        uint32_t ll[8];
-        bool empty = !act;
-        int n[3] = {0, 0, 0}, sizeMin = 1, stretch = 1, full[3] = {1, 1, 1};
+        CellGeom g;
         if (act) {
-            const uint64_t key = cp->key;
-            type = (int)(key & 7u);
-            size = 1 << ((key >> 3) & 31u);
-            o[0] = (int)(key >> 48); o[1] = (int)((key >> 32) & 0xffffu); o[2] = (int)((key >> 16) & 0xffffu);
-            for (int k = 0; k < 8; k++) ll[k] = cp->ll[k];
-            if (type == 0) {
-                sizeMin = m.info[ll[0]].w;
-                if (sizeMin > size || size / sizeMin > FF_BL) empty = true;
-                const ushort4 li = m.info[ll[0]];
-                const int lp[3] = {li.x, li.y, li.z}, ext = sizeMin * FF_BL;
-                for (int k = 0; k < 3; k++)
-                    if (o[k] + size > lp[k] + ext || o[k] < lp[k]) empty = true;
-            } else if (type <= 3) {
-                const int s0 = m.info[ll[0]].w, s1 = m.info[ll[1]].w;
-                sizeMin = min(s0, s1);
-                stretch = s0;
-                full[type - 1] = 0;
-                if (stretch > size) empty = true;
-            } else if (type <= 6) {
-                int sc[4];
-                for (int k = 0; k < 4; k++) sc[k] = m.info[ll[k]].w;
-                stretch = max(max(sc[0], sc[1]), sc[2]);
-                sizeMin = min(min(sc[0], sc[1]), min(sc[2], sc[3]));
-                for (int k = 0; k < 3; k++) full[k] = (k == type - 4);
-                if (stretch > size) empty = true;
-            } else {
-                stretch = m.info[ll[0]].w;
-                for (int k = 1; k < 7; k++) stretch = max(stretch, (int)m.info[ll[k]].w);
-                sizeMin = stretch;
-                full[0] = full[1] = full[2] = 0;
-                if (stretch > size) empty = true;
-            }
-            for (int k = 0; k < 3; k++) n[k] = full[k] ? size / sizeMin : 2;
+            g = geom[di];
+        } else {
+            g.cell = 0; g.tile0 = FF_EMPTY_CELL; g.type = 0;
+            g.n[0] = g.n[1] = g.n[2] = 0;
+            g.o[0] = g.o[1] = g.o[2] = 0;
+            g.size = g.sizeMin = g.stretch = 1;
         }
-        if (empty) { n[0] = n[1] = n[2] = 0; }
+        // the wave's cell is uniform: skip cells of the other pass (empty
+        // cells keep the zero counts k_cells gave them)
+        if (InPlace ? g.tile0 != FF_NO_TILE : g.tile0 >= FF_EMPTY_CELL) continue;
+        const int type = g.type;
+        const int n[3] = {g.n[0], g.n[1], g.n[2]};
         const int total = n[0] * n[1] * n[2];
-        // -- samples
-        // full interior cell (the cell is its brick): samples are the brick's
-        // voxels exactly; only zero-weight samples fall back along the chain
-        const bool fullInterior = type == 0 && total == FF_BRICK && sizeMin * FF_BL == size;
-        if (fullInterior) {
-            const ushort4 li = m.info[ll[0]];
-            const bool aligned = li.x == o[0] && li.y == o[1] && li.z == o[2];
-            const size_t vb = (size_t)ll[0] * FF_BRICK;
-            for (int id = lane; id < FF_BRICK; id += 64) {
-                const float w = m.wgt[vb + id];
-                if (aligned && w > 0.0f) {
-                    L.d[id] = m.dist[vb + id];
-                    L.w[id] = w;
-                    if (m.useColor) {
-                        const uint16_t *cb = m.col + vb * 3 + id;
-                        L.c[0][id] = cb[0]; L.c[1][id] = cb[FF_BRICK]; L.c[2][id] = cb[2 * FF_BRICK];
-                    } else {
-                        L.c[0][id] = L.c[1][id] = L.c[2][id] = 0;
-                    }
-                } else {
-                    const int X[3] = {o[0] + (id & 7) * sizeMin, o[1] + ((id >> 3) & 7) * sizeMin,
-                                      o[2] + (id >> 6) * sizeMin};
-                    Sample sm;
-                    sample_interior(m, ll[0], X, sm);
-                    L.d[id] = sm.d; L.w[id] = sm.w;
-                    L.c[0][id] = sm.c[0]; L.c[1][id] = sm.c[1]; L.c[2][id] = sm.c[2];
-                }
+        if (!InPlace) {
+            const uint4 *src = sbuf + (size_t)g.tile0 * 64u;
+            for (int s = lane; s < total; s += 64) {
+                const uint4 v = src[s];
+                put_sample(L, g, s, __uint_as_float(v.x), __uint_as_float(v.y), v.z, v.w);
             }
-        }
-        for (int s = fullInterior ? total : lane; s < total; s += 64) {
-            const int i = s % n[0], j = (s / n[0]) % n[1], l = s / (n[0] * n[1]);
-            const int g[3] = {i, j, l};
-            int X[3];
-            for (int k = 0; k < 3; k++)
-                X[k] = full[k] ? o[k] + g[k] * sizeMin : (g[k] ? o[k] + size : o[k] + size - stretch);
-            Sample sm;
-            if (type == 0) sample_interior(m, ll[0], X, sm);
-            else if (type <= 3) {
-                const int a = type - 1, u = a == 0 ? 1 : 0, v = a == 2 ? 1 : 2;
-                sample_wall(m, ll[g[a]], a, u, v, X, sm);
-            } else if (type <= 6) {
-                const int e = type - 4, u = e == 0 ? 1 : 0, v = e == 2 ? 1 : 2;
-                sample_edge(m, ll[g[u] + 2 * g[v]], e, X, sm);
-            } else {
-                sample_point(m, ll[i + 2 * j + 4 * l], X, sm);
+        } else {
+            const uint32_t *ll = cells[g.cell].ll;
+            for (int s = lane; s < total; s += 64) {
+                const uint4 v = pack_sample(sample_cell(m, g, ll, s));
+                put_sample(L, g, s, __uint_as_float(v.x), __uint_as_float(v.y), v.z, v.w);
             }
-            const int id = (l * FF_BL + j) * FF_BL + i;
-            L.d[id] = sm.d; L.w[id] = sm.w;
-            L.c[0][id] = sm.c[0]; L.c[1][id] = sm.c[1]; L.c[2][id] = sm.c[2];
         }
         if (lane < 48) L.used[lane] = 0u;
         wave_sync();
-        // -- grid geometry for vertices
-        auto coordOf = [&](int k, int gk) -> int {
-            return full[k] ? o[k] + gk * sizeMin : (gk ? o[k] + size : o[k] + size - stretch);
-        };
+        CellRec *cp = act ? &cells[g.cell] : nullptr;
+        const int sizeMin = g.sizeMin, stretch = g.stretch;
+        bool full[3];
+        for (int k = 0; k < 3; k++) full[k] = axis_full(type, k);
         const int elen[3] = {full[0] ? sizeMin : stretch, full[1] ? sizeMin : stretch, full[2] ? sizeMin : stretch};
         uint32_t nv = 0, nf = 0;
         if (type == 7) {
@@ -639,19 +816,17 @@ __global__ __launch_bounds__(256) void k_mesh(CellRec *cells, const uint32_t *__
             // (getColoredTrianglesForCubePrecomputed, mesh.cpp:1931-2008)
             int ci = 0;
             bool veto = false;
-            float P[8][3];
             for (int k = 0; k < 8; k++) {
                 const int id = (c_corner[k][2] * FF_BL + c_corner[k][1]) * FF_BL + c_corner[k][0];
                 if (total) {
-                    if (L.w[id] == 0.0f) veto = true;
-                    if (L.d[id] < 0.0f) ci |= 1 << k;
-                }
-                for (int a = 0; a < 3; a++) {
-                    const float basev = m.off[a] + (float)coordOf(a, 0) * m.scale;
-                    const float sx = m.scale * (float)stretch;
-                    P[k][a] = c_corner[k][a] ? basev + 1.0f * sx : basev + 0.0f * sx;
+                    const uint32_t f = L.sf[id];
+                    if (f & 2u) veto = true;
+                    if (f & 1u) ci |= 1 << k;
                 }
             }
+            float basev[3];
+            const float sx = m.scale * (float)stretch;
+            for (int a = 0; a < 3; a++) basev[a] = m.off[a] + (float)geom_coord(g, a, 0) * m.scale;
             if (veto || !total) ci = 0;
             const int ntri = c_ntri[ci];
             bool keep = false;
@@ -663,7 +838,12 @@ __global__ __launch_bounds__(256) void k_mesh(CellRec *cells, const uint32_t *__
                     const int a = c_edgec[e][0], b = c_edgec[e][1];
                     const int ia = (c_corner[a][2] * FF_BL + c_corner[a][1]) * FF_BL + c_corner[a][0];
                     const int ib = (c_corner[b][2] * FF_BL + c_corner[b][1]) * FF_BL + c_corner[b][0];
-                    vinterp(P[a], P[b], L.d[ia], L.d[ib], V[q]);
+                    float Pa[3], Pb[3];
+                    for (int k = 0; k < 3; k++) {
+                        Pa[k] = c_corner[a][k] ? basev[k] + 1.0f * sx : basev[k] + 0.0f * sx;
+                        Pb[k] = c_corner[b][k] ? basev[k] + 1.0f * sx : basev[k] + 0.0f * sx;
+                    }
+                    vinterp(Pa, Pb, L.d[ia], L.d[ib], V[q]);
                     const uint16_t ca[3] = {L.c[0][ia], L.c[1][ia], L.c[2][ia]};
                     const uint16_t cb[3] = {L.c[0][ib], L.c[1][ib], L.c[2][ib]};
                     C[q] = cinterp(ca, cb, L.d[ia], L.d[ib]);
@@ -704,29 +884,39 @@ __global__ __launch_bounds__(256) void k_mesh(CellRec *cells, const uint32_t *__
         const int ncubes = nc0 * nc1 * nc2;
         for (int q = lane; q < ncubes; q += 64) {
             const int i = q % nc0, j = (q / nc0) % nc1, l = q / (nc0 * nc1);
+            const int b0 = (l * FF_BL + j) * FF_BL + i;
             int ci = 0;
-            bool veto = false;
+            uint32_t zw = 0;
+#pragma unroll
             for (int k = 0; k < 8; k++) {
-                const int id = ((l + c_corner[k][2]) * FF_BL + j + c_corner[k][1]) * FF_BL + i + c_corner[k][0];
-                if (L.w[id] == 0.0f) veto = true;
-                if (L.d[id] < 0.0f) ci |= 1 << k;
+                const uint32_t f = L.sf[b0 + (c_corner[k][2] * FF_BL + c_corner[k][1]) * FF_BL + c_corner[k][0]];
+                zw |= f;
+                ci |= (int)(f & 1u) << k;
             }
-            if (veto) ci = 0;
+            if (zw & 2u) ci = 0;
             L.cube[q] = (uint8_t)ci;
             const int nt = c_ntri[ci];
             for (int t = 0; t < 3 * nt; t++) {
                 const int e = c_tri[ci][t];
-                const int slot = (((l + c_edge[e][2]) * FF_BL + j + c_edge[e][1]) * FF_BL + i + c_edge[e][0]) * 3 + c_edge[e][3];
+                const int slot = (b0 + (c_edge[e][2] * FF_BL + c_edge[e][1]) * FF_BL + c_edge[e][0]) * 3 + c_edge[e][3];
                 atomicOr(&L.used[slot >> 5], 1u << (slot & 31));
             }
         }
         wave_sync();
-        // -- vertex numbering over the used-edge bitmap
+        // -- vertex numbering over the used-edge bitmap, and the slot list
         {
             const uint32_t wd = lane < 48 ? L.used[lane] : 0u;
             uint32_t tot;
             const uint32_t ex = wave_excl_scan_u32((uint32_t)__popc(wd), &tot);
-            if (lane < 48) L.wpref[lane] = ex;
+            if (lane < 48) {
+                L.wpref[lane] = ex;
+                uint32_t bits = wd, k = ex;
+                while (bits) {
+                    const uint32_t b = (uint32_t)__ffs(bits) - 1u;
+                    L.list[k++] = (uint16_t)(lane * 32 + b);
+                    bits &= bits - 1u;
+                }
+            }
             nv = tot;
         }
         // face offsets per cube: contiguous chunks of 6 cubes per lane
@@ -752,32 +942,24 @@ __global__ __launch_bounds__(256) void k_mesh(CellRec *cells, const uint32_t *__
         f0 = __shfl(f0, 0);
         const bool fits = act && v0 + nv <= vCap && f0 + nf <= fCap;
         if (fits) {
-            // vertices
-            for (int wdi = 0; wdi < 48; wdi++) {
-                const uint32_t wd = L.used[wdi];
-                if (!wd) continue;
-                for (int bit = lane; bit < 32; bit += 64) {
-                    if (!((wd >> bit) & 1u)) continue;
-                    const int slot = wdi * 32 + bit, a = slot % 3, id = slot / 3;
-                    const int i = id & 7, j = (id >> 3) & 7, l = id >> 6;
-                    const int gg[3] = {i, j, l};
-                    int g2[3] = {i, j, l};
-                    g2[a]++;
-                    const int id2 = (g2[2] * FF_BL + g2[1]) * FF_BL + g2[0];
-                    float p1[3], p2[3];
-                    for (int k = 0; k < 3; k++) {
-                        p1[k] = m.off[k] + (float)coordOf(k, gg[k]) * m.scale;
-                        p2[k] = p1[k];
-                    }
-                    p2[a] = m.off[a] + (float)(coordOf(a, gg[a]) + elen[a]) * m.scale;
-                    float V[3];
-                    vinterp(p1, p2, L.d[id], L.d[id2], V);
-                    const uint16_t ca[3] = {L.c[0][id], L.c[1][id], L.c[2][id]};
-                    const uint16_t cb[3] = {L.c[0][id2], L.c[1][id2], L.c[2][id2]};
-                    const uint32_t rgb = cinterp(ca, cb, L.d[id], L.d[id2]);
-                    const uint32_t vi = L.wpref[wdi] + __popc(wd & ((1u << bit) - 1u));
-                    vpool[v0 + vi] = make_float4(V[0], V[1], V[2], __uint_as_float(rgb));
+            // vertices: one used edge per lane
+            for (uint32_t vi = lane; vi < nv; vi += 64) {
+                const int slot = L.list[vi], a = slot % 3, id = slot / 3;
+                const int gg[3] = {id & 7, (id >> 3) & 7, id >> 6};
+                const int id2 = id + (a == 0 ? 1 : (a == 1 ? FF_BL : FF_BL * FF_BL));
+                float p1[3], p2[3];
+#pragma unroll
+                for (int k = 0; k < 3; k++) {
+                    const int ck = geom_coord(g, k, gg[k]);
+                    p1[k] = m.off[k] + (float)ck * m.scale;
+                    p2[k] = k == a ? m.off[k] + (float)(ck + elen[k]) * m.scale : p1[k];
                 }
+                float V[3];
+                vinterp(p1, p2, L.d[id], L.d[id2], V);
+                const uint16_t ca[3] = {L.c[0][id], L.c[1][id], L.c[2][id]};
+                const uint16_t cb[3] = {L.c[0][id2], L.c[1][id2], L.c[2][id2]};
+                const uint32_t rgb = cinterp(ca, cb, L.d[id], L.d[id2]);
+                vpool[v0 + vi] = make_float4(V[0], V[1], V[2], __uint_as_float(rgb));
             }
             // faces
             for (int q = lane; q < ncubes; q += 64) {
@@ -785,10 +967,11 @@ __global__ __launch_bounds__(256) void k_mesh(CellRec *cells, const uint32_t *__
                 const int nt = c_ntri[ci];
                 if (!nt) continue;
                 const int i = q % nc0, j = (q / nc0) % nc1, l = q / (nc0 * nc1);
+                const int b0 = (l * FF_BL + j) * FF_BL + i;
                 const uint32_t fb = f0 + L.foff[q];
                 for (int t = 0; t < 3 * nt; t++) {
                     const int e = c_tri[ci][t];
-                    const int slot = (((l + c_edge[e][2]) * FF_BL + j + c_edge[e][1]) * FF_BL + i + c_edge[e][0]) * 3 + c_edge[e][3];
+                    const int slot = (b0 + (c_edge[e][2] * FF_BL + c_edge[e][1]) * FF_BL + c_edge[e][0]) * 3 + c_edge[e][3];
                     const uint32_t wd = L.used[slot >> 5];
                     fpool[3 * fb + t] = L.wpref[slot >> 5] + __popc(wd & ((1u << (slot & 31)) - 1u));
                 }
@@ -869,7 +1052,10 @@ int mesh_init(ff_ctx *c)
     MCHK(hipMalloc(&s->d_term, sizeof(uint4) * s->termCap));
     s->cellCap = std::max<uint32_t>(1u << 18, std::min<uint32_t>(c->nLeavesTotal * 16u, 1u << 25));
     for (int k = 0; k < 2; k++) MCHK(hipMalloc(&s->d_cells[k], sizeof(CellRec) * s->cellCap));
-    MCHK(hipMalloc(&s->d_dirty, sizeof(uint32_t) * s->cellCap));
+    MCHK(hipMalloc(&s->d_geom, sizeof(CellGeom) * s->cellCap));
+    s->tileCap = std::max<uint32_t>(1u << 16, std::min<uint32_t>(c->nLeavesTotal * 8u, 1u << 19));
+    MCHK(hipMalloc(&s->d_tiles, sizeof(uint32_t) * s->tileCap));
+    MCHK(hipMalloc(&s->d_sbuf, sizeof(uint4) * 64 * (size_t)s->tileCap));
     uint32_t hcap = 1;
     while (hcap < 2 * s->cellCap) hcap <<= 1;
     s->hmask = hcap - 1;
@@ -892,7 +1078,8 @@ void mesh_free(ff_ctx *c)
     MeshState *s = c->mesh;
     if (!s) return;
     hipFree(s->d_binfo); hipFree(s->d_term);
-    hipFree(s->d_cells[0]); hipFree(s->d_cells[1]); hipFree(s->d_dirty);
+    hipFree(s->d_cells[0]); hipFree(s->d_cells[1]);
+    hipFree(s->d_geom); hipFree(s->d_tiles); hipFree(s->d_sbuf);
     hipFree(s->d_hkeys); hipFree(s->d_hval); hipFree(s->d_vpool); hipFree(s->d_fpool);
     hipFree(s->d_mc); hipHostFree(s->h_mc);
     delete s;
@@ -978,9 +1165,10 @@ static int update_meshes(ff_ctx *c)
     m.useColor = c->useColor ? 1 : 0;
     k_mark<<<1024, 256, 0, c->stream>>>(s->d_cells[nxt], s->d_cells[s->cur], s->d_hkeys, s->d_hval, s->hmask,
                                         s->nCells[s->cur] > 0, c->d_stamp, c->d_parent, s->lastMeshStamp, s->d_mc,
-                                        s->d_dirty, s->cellCap);
-    k_mesh<<<2048, 256, 0, c->stream>>>(s->d_cells[nxt], s->d_dirty, s->d_mc, m, s->d_vpool, s->vCap, s->d_fpool,
-                                        s->fCap);
+                                        s->d_geom, s->cellCap, m, s->d_tiles, s->tileCap);
+    k_sample<<<4096, 256, 0, c->stream>>>(s->d_geom, s->d_cells[nxt], s->d_tiles, s->d_mc, s->tileCap, m, s->d_sbuf);
+    k_mc<false><<<2048, 256, 0, c->stream>>>(s->d_cells[nxt], s->d_geom, s->d_mc, s->d_sbuf, m, s->d_vpool, s->vCap,
+                                             s->d_fpool, s->fCap);
     // the previous table's keys leave the lookup hash, the new ones enter it
     if (s->nCells[s->cur])
         k_hash_clear_cells<<<1024, 256, 0, c->stream>>>(s->d_cells[s->cur], s->nCells[s->cur], s->d_hkeys);
@@ -993,6 +1181,31 @@ static int update_meshes(ff_ctx *c)
         if (hipEventElapsedTime(&ms, e0, e1) == hipSuccess) {
             c->phaseMs[2] += ms;
             c->phaseCalls[2]++;
+        }
+    }
+    if (s->h_mc->nTiles > s->tileCap && !(s->h_mc->flags & FF_CELL_OVERFLOW)) {
+        // some cells' samples did not fit the buffer: mesh them in place,
+        // then grow the buffer for the next updates
+        k_mc<true><<<2048, 256, 0, c->stream>>>(s->d_cells[nxt], s->d_geom, s->d_mc, s->d_sbuf, m, s->d_vpool,
+                                                s->vCap, s->d_fpool, s->fCap);
+        MCHK(hipMemcpyAsync(s->h_mc, s->d_mc, sizeof(MeshCounters), hipMemcpyDeviceToHost, c->stream));
+        MCHK(hipStreamSynchronize(c->stream));
+        uint32_t cap = s->tileCap;
+        while (cap < s->h_mc->nTiles && cap < FF_TILE_CAP_MAX) cap <<= 1;
+        if (cap > s->tileCap) {
+            uint32_t *nt = nullptr;
+            uint4 *nb = nullptr;
+            if (hipMalloc(&nt, sizeof(uint32_t) * cap) == hipSuccess &&
+                hipMalloc(&nb, sizeof(uint4) * 64 * (size_t)cap) == hipSuccess) {
+                hipFree(s->d_tiles);
+                hipFree(s->d_sbuf);
+                s->d_tiles = nt;
+                s->d_sbuf = nb;
+                s->tileCap = cap;
+            } else {
+                hipFree(nt);  // keep the old buffer; in-place meshing covers the rest
+                (void)hipGetLastError();
+            }
         }
     }
     const MeshCounters &h = *s->h_mc;
@@ -1137,6 +1350,30 @@ int ff_set_incremental_meshing(ff_ctx *c, int on)
 {
     if (!c) return FF_E_ARG;
     c->incremental = on != 0;
+    return FF_OK;
+}
+
+int ff_set_mesh_sample_tiles(ff_ctx *c, uint32_t tiles)
+{
+    if (!c || !c->mesh || tiles == 0 || tiles > FF_TILE_CAP_MAX) {
+        if (c) c->err = "ff_set_mesh_sample_tiles: tiles must be in [1, 4194304]";
+        return FF_E_ARG;
+    }
+    hipSetDevice(c->device);
+    MeshState *s = c->mesh;
+    MCHK(hipStreamSynchronize(c->stream));
+    uint32_t *nt = nullptr;
+    uint4 *nb = nullptr;
+    MCHK(hipMalloc(&nt, sizeof(uint32_t) * tiles));
+    if (!mok(c, hipMalloc(&nb, sizeof(uint4) * 64 * (size_t)tiles), "hipMalloc(sample buffer)")) {
+        hipFree(nt);
+        return FF_E_DEVICE;
+    }
+    hipFree(s->d_tiles);
+    hipFree(s->d_sbuf);
+    s->d_tiles = nt;
+    s->d_sbuf = nb;
+    s->tileCap = tiles;
     return FF_OK;
 }
 

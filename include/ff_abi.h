@@ -96,6 +96,13 @@ int ff_copy_cells(ff_ctx *ctx, int32_t *desc5, uint32_t *n_vertices, uint32_t *n
 /* setIncrementalMeshing (geometryfusion_mipmap_cpu.hpp:131). */
 int ff_set_incremental_meshing(ff_ctx *ctx, int on);
 
+/* Device memory knob with no counterpart in the reference (whose meshing
+ * samples cells on the host): the capacity, in tiles of 64 samples (1 KiB),
+ * of the buffer that holds the samples of the cells re-meshed by one
+ * updateMeshes.  Cells beyond it are sampled in place by a second meshing
+ * pass and the buffer then grows; the default is sized from max_leaves. */
+int ff_set_mesh_sample_tiles(ff_ctx *ctx, uint32_t tiles);
+
 /* Map state, for inspection and parity (the reference's protected members
  * _tree, _leafPos, _leafScale, _leafParent, _distance, _weights, _color). */
 int ff_get_counts(ff_ctx *ctx, ff_counts *out);

@@ -32,8 +32,10 @@ def compare_mesh(gpu, ora, voxel, tag):
     return len(desc), len(m.faces)
 
 
-def run_mesh(sc, R, t, frames, cfg, K, check_every=1, max_leaves=1 << 17):
+def run_mesh(sc, R, t, frames, cfg, K, check_every=1, max_leaves=1 << 17, sample_tiles=None):
     gpu, ora = make_pair(cfg["scale"], cfg["threshold"], max_leaves=max_leaves)
+    if sample_tiles is not None:
+        gpu.setMeshSampleTiles(sample_tiles)
     w, h = cfg["w"], cfg["h"]
     out = None
     for k, i in enumerate(frames):
@@ -77,3 +79,14 @@ def test_mesh_after_grow():
     sc, R, t = grow_sequence()
     cfg = dict(scale=0.02, threshold=0.06, maxcam=6.0, w=320, h=240)
     run_mesh(sc, R, t, list(range(len(R))), cfg, (262.5, 262.5, 159.5, 119.5), check_every=1)
+
+
+def test_mesh_sample_buffer_overflow():
+    """A sample buffer of 8 tiles cannot hold the first update's cells: the
+    cells beyond it are meshed in place by the second pass, the buffer grows,
+    and every frame's meshes still equal the oracle's."""
+    sc, R, t, cfg = synth.config_c1(n_frames=200)
+    cfg = dict(cfg, w=320, h=240)
+    ncell, nface = run_mesh(sc, R, t, list(range(0, 6)), cfg, (262.5, 262.5, 159.5, 119.5), check_every=1,
+                            sample_tiles=8)
+    assert nface > 1000
