@@ -41,11 +41,45 @@ struct CellRec {
 };
 
 struct MeshCounters {
-    uint32_t nTerm, nCells, nDirty, vAlloc, fAlloc, flags, nLive, nTiles;
+    uint32_t nTerm, nCells, nDirty, flags, nTiles, pad[3];
 };
 
 #define FF_MESH_OVERFLOW 1u
 #define FF_CELL_OVERFLOW 2u
+#define FF_MESH_HALF 4u  // some arena passed half of its capacity
+
+// The vertex and face pools are split into FF_ARENAS arenas with one
+// allocation counter each (vertices in the low, faces in the high 32 bits),
+// each on its own 128-byte line: cells spread their allocations over them,
+// so no single counter serialises the meshing waves.
+#define FF_ARENAS 64u
+#define FF_ARENA_STRIDE 16u
+
+struct Pools {
+    float4 *v;                    // x, y, z, rgb packed
+    uint32_t *f;                  // 3 cell-local indices per face
+    unsigned long long *arena;    // FF_ARENAS counters, FF_ARENA_STRIDE apart
+    uint32_t vA, fA;              // capacity of one arena
+};
+
+// reserves nv vertices and nf faces in arena k; false when they do not fit
+__device__ __forceinline__ bool pool_alloc(const Pools &P, uint32_t k, uint32_t nv, uint32_t nf, uint32_t &v0,
+                                           uint32_t &f0, MeshCounters *mc)
+{
+    v0 = f0 = 0;
+    if (nv == 0 && nf == 0) return true;
+    const unsigned long long old = atomicAdd(&P.arena[k * FF_ARENA_STRIDE], ((unsigned long long)nf << 32) | nv);
+    const uint32_t vo = (uint32_t)old, fo = (uint32_t)(old >> 32);
+    if (vo + nv > P.vA || fo + nf > P.fA) {
+        atomicOr(&mc->flags, FF_MESH_OVERFLOW);
+        return false;
+    }
+    const uint32_t vh = P.vA / 2, fh = P.fA / 2;
+    if ((vo <= vh && vo + nv > vh) || (fo <= fh && fo + nf > fh)) atomicOr(&mc->flags, FF_MESH_HALF);
+    v0 = k * P.vA + vo;
+    f0 = k * P.fA + fo;
+    return true;
+}
 
 __constant__ uint8_t c_tri[256][16];  // edges of each triangle, 255-terminated
 __constant__ uint8_t c_ntri[256];
@@ -71,7 +105,8 @@ struct MeshState {
     uint32_t hmask = 0;
     float4 *d_vpool = nullptr;    // x, y, z, rgb packed
     uint32_t *d_fpool = nullptr;  // 3 local indices per face
-    uint32_t vCap = 0, fCap = 0;
+    uint32_t vCap = 0, fCap = 0;  // FF_ARENAS arenas of vCap / FF_ARENAS, fCap / FF_ARENAS
+    unsigned long long *d_arena = nullptr;
     MeshCounters *d_mc = nullptr, *h_mc = nullptr;
     uint32_t lastMeshStamp = 0;
     bool haveVolume = false;
@@ -163,102 +198,132 @@ __global__ void k_terminal(const uint32_t *__restrict__ tree, const ushort4 *__r
 // ---------------------------------------------------------------------------
 // k_cells: cells owned by each covered terminal node
 // ---------------------------------------------------------------------------
-__global__ void k_cells(const uint32_t *__restrict__ tree, uint32_t nLeavesTotal, uint32_t n,
-                        const uint4 *__restrict__ term, uint32_t termCap, MeshCounters *mc, CellRec *cells,
-                        uint32_t cap)
+// Every ownership test looks at terminal nodes containing points next to the
+// node p (size s): per axis the coordinate is one of p - 1, p, p + s - 1,
+// p + s (option 0..3).  A block takes 4 nodes at a time: one thread locates
+// each of the 64 option points of a node (the 8 inside ones are the node
+// itself), then one thread per (node, candidate) decides ownership from them.
+__device__ __forceinline__ int opt_code(int ox, int oy, int oz) { return ox + 4 * oy + 16 * oz; }
+
+__global__ __launch_bounds__(256) void k_cells(const uint32_t *__restrict__ tree, uint32_t nLeavesTotal, uint32_t n,
+                                               const uint4 *__restrict__ term, uint32_t termCap, MeshCounters *mc,
+                                               CellRec *cells, uint32_t cap)
 {
-    // one lane per (terminal node, candidate): 0 interior, 1-6 walls,
-    // 7-18 edges, 19-26 corners; the loop bound is uniform per wave so that
-    // the emission ballot sees every lane
+    __shared__ uint2 nb[4][64];  // (size, lastLeaf) per node and option point
     const uint32_t nt = min(mc->nTerm, termCap);
-    const uint32_t total = nt * 27u;
-    const uint32_t lane = threadIdx.x & 63u;
-    const uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t gb = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); gb < total; gb += stride) {
-        const uint32_t g = gb + lane;
-        bool emit = false;
-        int type = 0, o[3] = {0, 0, 0}, s = 1, nll = 0;
-        uint32_t ll[8];
-        if (g < total) {
-            const uint32_t t = g / 27u, cand = g % 27u;
-            const uint4 tn = term[t];
-            const int p[3] = {(int)(tn.x & 0xffffu), (int)(tn.x >> 16), (int)tn.y};
-            s = (int)tn.z;
-            const uint32_t lt = tn.w;
-            int org[3];
-            if (cand == 0) {
-                ll[0] = lt;
-                type = 0; o[0] = p[0]; o[1] = p[1]; o[2] = p[2]; nll = 1;
-                emit = true;
-            } else if (cand <= 6) {
-                const int a = (int)(cand - 1) >> 1, side = (int)(cand - 1) & 1;
-                int q[3] = {p[0], p[1], p[2]};
-                q[a] = side ? p[a] + s : p[a] - 1;
-                uint32_t lf;
-                const uint32_t ns = locate(tree, nLeavesTotal, n, q[0], q[1], q[2], org, &lf);
-                if (ns && (side ? (ns >= (uint32_t)s) : (ns > (uint32_t)s))) {
-                    o[0] = p[0]; o[1] = p[1]; o[2] = p[2];
-                    if (!side) o[a] = p[a] - s;
-                    ll[side ? 0 : 1] = lt;
-                    ll[side ? 1 : 0] = lf;
-                    type = 1 + a; nll = 2;
-                    emit = true;
-                }
-            } else if (cand <= 18) {
-                const int e = (int)(cand - 7) >> 2, cu = (int)(cand - 7) & 1, cv = ((int)(cand - 7) >> 1) & 1;
-                const int u = e == 0 ? 1 : 0, v = e == 2 ? 1 : 2;
-                const int lu = p[u] + cu * s, lv = p[v] + cv * s;
-                const int mine = (1 - cu) + 2 * (1 - cv);
-                bool ok = true;
-                for (int sl = 0; sl < 4 && ok; sl++) {
-                    if (sl == mine) { ll[sl] = lt; continue; }
-                    int q[3] = {p[0], p[1], p[2]};
-                    q[u] = lu - 1 + (sl & 1);
-                    q[v] = lv - 1 + (sl >> 1);
-                    const uint32_t sz = locate(tree, nLeavesTotal, n, q[0], q[1], q[2], org, &ll[sl]);
-                    // a smaller node, or an equal one in an earlier slot, owns the cell
-                    if (!sz || sz < (uint32_t)s || (sl < mine && sz == (uint32_t)s)) ok = false;
-                }
-                if (ok) {
-                    o[e] = p[e]; o[u] = lu - s; o[v] = lv - s;
-                    type = 4 + e; nll = 4;
-                    emit = true;
-                }
-            } else {
-                const int cb = (int)(cand - 19), cx = cb & 1, cy = (cb >> 1) & 1, cz = cb >> 2;
-                const int pt[3] = {p[0] + cx * s, p[1] + cy * s, p[2] + cz * s};
-                const int mine = (1 - cx) + 2 * (1 - cy) + 4 * (1 - cz);
-                bool ok = true;
-                for (int sl = 0; sl < 8 && ok; sl++) {
-                    if (sl == mine) { ll[sl] = lt; continue; }
-                    const uint32_t sz = locate(tree, nLeavesTotal, n, pt[0] - 1 + (sl & 1),
-                                               pt[1] - 1 + ((sl >> 1) & 1), pt[2] - 1 + ((sl >> 2) & 1), org, &ll[sl]);
-                    if (!sz || sz < (uint32_t)s || (sl < mine && sz == (uint32_t)s)) ok = false;
-                }
-                if (ok) {
-                    o[0] = pt[0] - s; o[1] = pt[1] - s; o[2] = pt[2] - s;
-                    type = 7; nll = 8;
-                    emit = true;
+    const int tid = threadIdx.x;
+    for (uint32_t n0 = blockIdx.x * 4u; n0 < nt; n0 += gridDim.x * 4u) {
+        {
+            const uint32_t node = n0 + (uint32_t)(tid >> 6);
+            const int code = tid & 63;
+            uint2 r = make_uint2(0u, FF_BRANCHINIT);
+            if (node < nt) {
+                const uint4 tn = term[node];
+                const int p[3] = {(int)(tn.x & 0xffffu), (int)(tn.x >> 16), (int)tn.y};
+                const int sz = (int)tn.z;
+                const int op[3] = {code & 3, (code >> 2) & 3, code >> 4};
+                const bool inside = (op[0] == 1 || op[0] == 2) && (op[1] == 1 || op[1] == 2) && (op[2] == 1 || op[2] == 2);
+                if (inside) {
+                    r = make_uint2((uint32_t)sz, tn.w);
+                } else {
+                    int q[3];
+                    for (int k = 0; k < 3; k++)
+                        q[k] = op[k] == 0 ? p[k] - 1 : (op[k] == 1 ? p[k] : (op[k] == 2 ? p[k] + sz - 1 : p[k] + sz));
+                    int org[3];
+                    uint32_t lf;
+                    r.x = locate(tree, nLeavesTotal, n, q[0], q[1], q[2], org, &lf);
+                    r.y = lf;
                 }
             }
-            // pushMeshCellQueue needs every side's brick
-            for (int k = 0; k < nll && emit; k++)
-                if (ll[k] >= FF_BRANCHINIT) emit = false;
+            nb[tid >> 6][code] = r;
         }
-        const uint32_t i = wave_append(&mc->nCells, emit);
-        if (emit) {
-            if (i >= cap) {
-                atomicOr(&mc->flags, FF_CELL_OVERFLOW);
-            } else {
-                CellRec r;
-                r.key = cell_key(type, o[0], o[1], o[2], s);
-                for (int k = 0; k < 8; k++) r.ll[k] = k < nll ? ll[k] : FF_BRANCHINIT;
-                r.v0 = r.nv = r.f0 = r.nf = 0;
-                r.hslot = 0xffffffffu;
-                r.pad[0] = r.pad[1] = r.pad[2] = 0;
-                cells[i] = r;
+        __syncthreads();
+        if (tid < 128) {
+            // candidates: 0 interior, 1-6 walls, 7-18 edges, 19-26 corners
+            const uint32_t node = n0 + (uint32_t)(tid / 27);
+            const int cand = tid % 27;
+            bool emit = false;
+            int type = 0, o[3] = {0, 0, 0}, s = 1, nll = 0;
+            uint32_t ll[8];
+            if (tid < 108 && node < nt) {
+                const uint2 *N = nb[tid / 27];
+                const uint4 tn = term[node];
+                const int p[3] = {(int)(tn.x & 0xffffu), (int)(tn.x >> 16), (int)tn.y};
+                s = (int)tn.z;
+                const uint32_t lt = tn.w;
+                if (cand == 0) {
+                    ll[0] = lt;
+                    type = 0; o[0] = p[0]; o[1] = p[1]; o[2] = p[2]; nll = 1;
+                    emit = true;
+                } else if (cand <= 6) {
+                    const int a = (cand - 1) >> 1, side = (cand - 1) & 1;
+                    int op[3] = {1, 1, 1};
+                    op[a] = side ? 3 : 0;
+                    const uint2 r = N[opt_code(op[0], op[1], op[2])];
+                    const uint32_t ns = r.x;
+                    if (ns && (side ? (ns >= (uint32_t)s) : (ns > (uint32_t)s))) {
+                        o[0] = p[0]; o[1] = p[1]; o[2] = p[2];
+                        if (!side) o[a] = p[a] - s;
+                        ll[side ? 0 : 1] = lt;
+                        ll[side ? 1 : 0] = r.y;
+                        type = 1 + a; nll = 2;
+                        emit = true;
+                    }
+                } else if (cand <= 18) {
+                    const int e = (cand - 7) >> 2, cu = (cand - 7) & 1, cv = ((cand - 7) >> 1) & 1;
+                    const int u = e == 0 ? 1 : 0, v = e == 2 ? 1 : 2;
+                    const int mine = (1 - cu) + 2 * (1 - cv);
+                    bool ok = true;
+                    for (int sl = 0; sl < 4; sl++) {
+                        int op[3];
+                        op[e] = 1;
+                        op[u] = 2 * cu + (sl & 1);
+                        op[v] = 2 * cv + (sl >> 1);
+                        const uint2 r = N[opt_code(op[0], op[1], op[2])];
+                        ll[sl] = r.y;
+                        // a smaller node, or an equal one in an earlier slot, owns the cell
+                        if (sl != mine && (!r.x || r.x < (uint32_t)s || (sl < mine && r.x == (uint32_t)s))) ok = false;
+                    }
+                    if (ok) {
+                        o[e] = p[e]; o[u] = p[u] + cu * s - s; o[v] = p[v] + cv * s - s;
+                        type = 4 + e; nll = 4;
+                        emit = true;
+                    }
+                } else {
+                    const int cb = cand - 19, c3[3] = {cb & 1, (cb >> 1) & 1, cb >> 2};
+                    const int mine = (1 - c3[0]) + 2 * (1 - c3[1]) + 4 * (1 - c3[2]);
+                    bool ok = true;
+                    for (int sl = 0; sl < 8; sl++) {
+                        const uint2 r = N[opt_code(2 * c3[0] + (sl & 1), 2 * c3[1] + ((sl >> 1) & 1), 2 * c3[2] + (sl >> 2))];
+                        ll[sl] = r.y;
+                        if (sl != mine && (!r.x || r.x < (uint32_t)s || (sl < mine && r.x == (uint32_t)s))) ok = false;
+                    }
+                    if (ok) {
+                        for (int k = 0; k < 3; k++) o[k] = p[k] + c3[k] * s - s;
+                        type = 7; nll = 8;
+                        emit = true;
+                    }
+                }
+                // pushMeshCellQueue needs every side's brick
+                for (int k = 0; k < nll && emit; k++)
+                    if (ll[k] >= FF_BRANCHINIT) emit = false;
+            }
+            const uint32_t i = wave_append(&mc->nCells, emit);
+            if (emit) {
+                if (i >= cap) {
+                    atomicOr(&mc->flags, FF_CELL_OVERFLOW);
+                } else {
+                    CellRec r;
+                    r.key = cell_key(type, o[0], o[1], o[2], s);
+                    for (int k = 0; k < 8; k++) r.ll[k] = k < nll ? ll[k] : FF_BRANCHINIT;
+                    r.v0 = r.nv = r.f0 = r.nf = 0;
+                    r.hslot = 0xffffffffu;
+                    r.pad[0] = r.pad[1] = r.pad[2] = 0;
+                    cells[i] = r;
+                }
             }
         }
+        __syncthreads();
     }
 }
 
@@ -764,8 +829,7 @@ __device__ __forceinline__ void put_sample(WaveLds &L, const CellGeom &g, int s,
 // InPlace = true: cells that did not fit the buffer, sampled by the wave itself
 template <bool InPlace>
 __global__ __launch_bounds__(256) void k_mc(CellRec *cells, const CellGeom *__restrict__ geom, MeshCounters *mc,
-                                            const uint4 *__restrict__ sbuf, MapRO m, float4 *__restrict__ vpool,
-                                            uint32_t vCap, uint32_t *__restrict__ fpool, uint32_t fCap)
+                                            const uint4 *__restrict__ sbuf, MapRO m, Pools P)
 {
     __shared__ WaveLds lds[4];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -858,22 +922,19 @@ __global__ __launch_bounds__(256) void k_mc(CellRec *cells, const CellGeom *__re
             const uint32_t rank = __popcll(kb & ((1ull << lane) - 1ull));
             nf = __popcll(kb);
             nv = 3 * nf;
-            uint32_t v0 = 0, f0 = 0;
-            if (lane == 0 && act) {
-                v0 = atomicAdd(&mc->vAlloc, nv);
-                f0 = atomicAdd(&mc->fAlloc, nf);
-            }
+            uint32_t v0 = 0, f0 = 0, ok = 1;
+            if (lane == 0) ok = pool_alloc(P, di % FF_ARENAS, nv, nf, v0, f0, mc) ? 1u : 0u;
             v0 = __shfl(v0, 0);
             f0 = __shfl(f0, 0);
-            const bool fits = v0 + nv <= vCap && f0 + nf <= fCap;
+            const bool fits = __shfl(ok, 0) != 0;
             if (keep && fits) {
                 for (int q = 0; q < 3; q++) {
-                    vpool[v0 + 3 * rank + q] = make_float4(V[q][0], V[q][1], V[q][2], __uint_as_float(C[q]));
-                    fpool[3 * (f0 + rank) + q] = 3 * rank + q;
+                    P.v[v0 + 3 * rank + q] = make_float4(V[q][0], V[q][1], V[q][2], __uint_as_float(C[q]));
+                    P.f[3 * (f0 + rank) + q] = 3 * rank + q;
                 }
             }
-            if (lane == 0 && act) {
-                if (!fits) { atomicOr(&mc->flags, FF_MESH_OVERFLOW); nv = nf = 0; }
+            if (lane == 0) {
+                if (!fits) nv = nf = 0;
                 cp->v0 = v0; cp->nv = nv; cp->f0 = f0; cp->nf = nf;
             }
             wave_sync();
@@ -933,14 +994,11 @@ __global__ __launch_bounds__(256) void k_mc(CellRec *cells, const CellGeom *__re
             nf = tot;
         }
         wave_sync();
-        uint32_t v0 = 0, f0 = 0;
-        if (lane == 0 && act) {
-            v0 = atomicAdd(&mc->vAlloc, nv);
-            f0 = atomicAdd(&mc->fAlloc, nf);
-        }
+        uint32_t v0 = 0, f0 = 0, ok = 1;
+        if (lane == 0) ok = pool_alloc(P, di % FF_ARENAS, nv, nf, v0, f0, mc) ? 1u : 0u;
         v0 = __shfl(v0, 0);
         f0 = __shfl(f0, 0);
-        const bool fits = act && v0 + nv <= vCap && f0 + nf <= fCap;
+        const bool fits = __shfl(ok, 0) != 0;
         if (fits) {
             // vertices: one used edge per lane
             for (uint32_t vi = lane; vi < nv; vi += 64) {
@@ -959,7 +1017,7 @@ __global__ __launch_bounds__(256) void k_mc(CellRec *cells, const CellGeom *__re
                 const uint16_t ca[3] = {L.c[0][id], L.c[1][id], L.c[2][id]};
                 const uint16_t cb[3] = {L.c[0][id2], L.c[1][id2], L.c[2][id2]};
                 const uint32_t rgb = cinterp(ca, cb, L.d[id], L.d[id2]);
-                vpool[v0 + vi] = make_float4(V[0], V[1], V[2], __uint_as_float(rgb));
+                P.v[v0 + vi] = make_float4(V[0], V[1], V[2], __uint_as_float(rgb));
             }
             // faces
             for (int q = lane; q < ncubes; q += 64) {
@@ -973,37 +1031,54 @@ __global__ __launch_bounds__(256) void k_mc(CellRec *cells, const CellGeom *__re
                     const int e = c_tri[ci][t];
                     const int slot = (b0 + (c_edge[e][2] * FF_BL + c_edge[e][1]) * FF_BL + c_edge[e][0]) * 3 + c_edge[e][3];
                     const uint32_t wd = L.used[slot >> 5];
-                    fpool[3 * fb + t] = L.wpref[slot >> 5] + __popc(wd & ((1u << (slot & 31)) - 1u));
+                    P.f[3 * fb + t] = L.wpref[slot >> 5] + __popc(wd & ((1u << (slot & 31)) - 1u));
                 }
             }
         }
-        if (lane == 0 && act) {
-            if (!fits) { atomicOr(&mc->flags, FF_MESH_OVERFLOW); nv = nf = 0; }
+        if (lane == 0) {
+            if (!fits) nv = nf = 0;
             cp->v0 = v0; cp->nv = nv; cp->f0 = f0; cp->nf = nf;
         }
         wave_sync();
     }
 }
 
-// live-geometry compaction: copies each cell's segments to a fresh pool
-__global__ void k_compact(CellRec *cells, const MeshCounters *mc, const float4 *__restrict__ vsrc,
-                          const uint32_t *__restrict__ fsrc, float4 *__restrict__ vdst, uint32_t *__restrict__ fdst,
-                          MeshCounters *mcw, uint32_t cap)
+// live-geometry compaction: copies each cell's segments to fresh pools
+__global__ void k_compact(CellRec *cells, MeshCounters *mc, uint32_t cap, const float4 *__restrict__ vsrc,
+                          const uint32_t *__restrict__ fsrc, Pools dst)
 {
     const uint32_t nc = min(mc->nCells, cap);
     const int lane = threadIdx.x & 63;
     for (uint32_t c = (blockIdx.x * blockDim.x + threadIdx.x) >> 6; c < nc; c += (gridDim.x * blockDim.x) >> 6) {
         CellRec &r = cells[c];
-        uint32_t v0 = 0, f0 = 0;
-        if (lane == 0) {
-            v0 = atomicAdd(&mcw->vAlloc, r.nv);
-            f0 = atomicAdd(&mcw->fAlloc, r.nf);
-        }
+        const uint32_t nv = r.nv, nf = r.nf;
+        uint32_t v0 = 0, f0 = 0, ok = 1;
+        if (lane == 0) ok = pool_alloc(dst, c % FF_ARENAS, nv, nf, v0, f0, mc) ? 1u : 0u;
         v0 = __shfl(v0, 0);
         f0 = __shfl(f0, 0);
-        for (uint32_t i = lane; i < r.nv; i += 64) vdst[v0 + i] = vsrc[r.v0 + i];
-        for (uint32_t i = lane; i < 3 * r.nf; i += 64) fdst[3 * f0 + i] = fsrc[3 * r.f0 + i];
-        if (lane == 0) { r.v0 = v0; r.f0 = f0; }
+        if (__shfl(ok, 0)) {
+            for (uint32_t i = lane; i < nv; i += 64) dst.v[v0 + i] = vsrc[r.v0 + i];
+            for (uint32_t i = lane; i < 3 * nf; i += 64) dst.f[3 * f0 + i] = fsrc[3 * r.f0 + i];
+        }
+        if (lane == 0) {
+            if (!ok) { r.nv = 0; r.nf = 0; }
+            r.v0 = v0; r.f0 = f0;
+        }
+    }
+}
+
+// concatenation for the host: cell k's vertices go to vo[k], its faces to
+// fo[k] with indices made global
+__global__ void k_gather(const uint4 *__restrict__ segs, const uint2 *__restrict__ outOff, uint32_t n,
+                         const float4 *__restrict__ vsrc, const uint32_t *__restrict__ fsrc, float4 *__restrict__ vdst,
+                         uint32_t *__restrict__ fdst)
+{
+    const int lane = threadIdx.x & 63;
+    for (uint32_t c = (blockIdx.x * blockDim.x + threadIdx.x) >> 6; c < n; c += (gridDim.x * blockDim.x) >> 6) {
+        const uint4 sg = segs[c];  // v0, nv, f0, nf
+        const uint2 oo = outOff[c];
+        for (uint32_t i = lane; i < sg.y; i += 64) vdst[oo.x + i] = vsrc[sg.x + i];
+        for (uint32_t i = lane; i < 3 * sg.w; i += 64) fdst[3 * (size_t)oo.y + i] = fsrc[3 * (size_t)sg.z + i] + oo.x;
     }
 }
 
@@ -1063,9 +1138,12 @@ int mesh_init(ff_ctx *c)
     MCHK(hipMemset(s->d_hkeys, 0xff, sizeof(uint64_t) * hcap));
     MCHK(hipMalloc(&s->d_hval, sizeof(uint32_t) * hcap));
     s->vCap = std::max<uint32_t>(1u << 22, std::min<uint32_t>(c->nLeavesTotal * 64u, 1u << 28));
+    s->vCap -= s->vCap % FF_ARENAS;
     s->fCap = s->vCap;
     MCHK(hipMalloc(&s->d_vpool, sizeof(float4) * s->vCap));
     MCHK(hipMalloc(&s->d_fpool, sizeof(uint32_t) * 3 * (size_t)s->fCap));
+    MCHK(hipMalloc(&s->d_arena, sizeof(unsigned long long) * FF_ARENAS * FF_ARENA_STRIDE));
+    MCHK(hipMemset(s->d_arena, 0, sizeof(unsigned long long) * FF_ARENAS * FF_ARENA_STRIDE));
     MCHK(hipMalloc(&s->d_mc, sizeof(MeshCounters)));
     MCHK(hipHostMalloc(&s->h_mc, sizeof(MeshCounters), hipHostMallocDefault));
     memset(s->h_mc, 0, sizeof(MeshCounters));
@@ -1081,6 +1159,7 @@ void mesh_free(ff_ctx *c)
     hipFree(s->d_cells[0]); hipFree(s->d_cells[1]);
     hipFree(s->d_geom); hipFree(s->d_tiles); hipFree(s->d_sbuf);
     hipFree(s->d_hkeys); hipFree(s->d_hval); hipFree(s->d_vpool); hipFree(s->d_fpool);
+    hipFree(s->d_arena);
     hipFree(s->d_mc); hipHostFree(s->h_mc);
     delete s;
     c->mesh = nullptr;
@@ -1142,6 +1221,17 @@ void mesh_record_branches(ff_ctx *c)
                                             c->hv.listCap, c->n, s->binfoCap);
 }
 
+static Pools pools(const MeshState *s)
+{
+    Pools P;
+    P.v = s->d_vpool;
+    P.f = s->d_fpool;
+    P.arena = s->d_arena;
+    P.vA = s->vCap / FF_ARENAS;
+    P.fA = s->fCap / FF_ARENAS;
+    return P;
+}
+
 static int update_meshes(ff_ctx *c)
 {
     MeshState *s = c->mesh;
@@ -1150,8 +1240,6 @@ static int update_meshes(ff_ctx *c)
     if (c->timing) hipEventRecord(e0, c->stream);
     const int nxt = 1 - s->cur;
     MeshCounters zero{};
-    zero.vAlloc = s->h_mc->vAlloc;
-    zero.fAlloc = s->h_mc->fAlloc;
     MCHK(hipMemcpyAsync(s->d_mc, &zero, sizeof zero, hipMemcpyHostToDevice, c->stream));
     const uint32_t nBlocks = c->nBranchesUsed / 16;
     k_terminal<<<(nBlocks * 8 + 255) / 256, 256, 0, c->stream>>>(c->d_tree, s->d_binfo, nBlocks, c->nLeavesTotal, c->n,
@@ -1167,8 +1255,8 @@ static int update_meshes(ff_ctx *c)
                                         s->nCells[s->cur] > 0, c->d_stamp, c->d_parent, s->lastMeshStamp, s->d_mc,
                                         s->d_geom, s->cellCap, m, s->d_tiles, s->tileCap);
     k_sample<<<4096, 256, 0, c->stream>>>(s->d_geom, s->d_cells[nxt], s->d_tiles, s->d_mc, s->tileCap, m, s->d_sbuf);
-    k_mc<false><<<2048, 256, 0, c->stream>>>(s->d_cells[nxt], s->d_geom, s->d_mc, s->d_sbuf, m, s->d_vpool, s->vCap,
-                                             s->d_fpool, s->fCap);
+    const Pools P = pools(s);
+    k_mc<false><<<2048, 256, 0, c->stream>>>(s->d_cells[nxt], s->d_geom, s->d_mc, s->d_sbuf, m, P);
     // the previous table's keys leave the lookup hash, the new ones enter it
     if (s->nCells[s->cur])
         k_hash_clear_cells<<<1024, 256, 0, c->stream>>>(s->d_cells[s->cur], s->nCells[s->cur], s->d_hkeys);
@@ -1186,8 +1274,7 @@ static int update_meshes(ff_ctx *c)
     if (s->h_mc->nTiles > s->tileCap && !(s->h_mc->flags & FF_CELL_OVERFLOW)) {
         // some cells' samples did not fit the buffer: mesh them in place,
         // then grow the buffer for the next updates
-        k_mc<true><<<2048, 256, 0, c->stream>>>(s->d_cells[nxt], s->d_geom, s->d_mc, s->d_sbuf, m, s->d_vpool,
-                                                s->vCap, s->d_fpool, s->fCap);
+        k_mc<true><<<2048, 256, 0, c->stream>>>(s->d_cells[nxt], s->d_geom, s->d_mc, s->d_sbuf, m, P);
         MCHK(hipMemcpyAsync(s->h_mc, s->d_mc, sizeof(MeshCounters), hipMemcpyDeviceToHost, c->stream));
         MCHK(hipStreamSynchronize(c->stream));
         uint32_t cap = s->tileCap;
@@ -1220,30 +1307,33 @@ static int update_meshes(ff_ctx *c)
     // algorithmic bytes: 512 voxels x 14 B read per re-meshed cell-sized region
     // is an upper bound; count the reads of the cells' samples and the writes
     c->phaseBytes[2] += (double)h.nDirty * 0.0;
-    if (h.flags & FF_MESH_OVERFLOW || h.vAlloc > s->vCap / 2 || h.fAlloc > s->fCap / 2) {
-        // compact live geometry into a fresh pool
+    const uint32_t flags = h.flags, nDirty = h.nDirty;
+    if (flags & (FF_MESH_OVERFLOW | FF_MESH_HALF)) {
+        // compact live geometry into fresh pools (cell k to arena k mod FF_ARENAS)
         float4 *nv = nullptr;
         uint32_t *nf = nullptr;
         MCHK(hipMalloc(&nv, sizeof(float4) * s->vCap));
-        MCHK(hipMalloc(&nf, sizeof(uint32_t) * 3 * (size_t)s->fCap));
-        MeshCounters z2 = h;
-        z2.vAlloc = z2.fAlloc = 0;
-        MCHK(hipMemcpyAsync(s->d_mc, &z2, sizeof z2, hipMemcpyHostToDevice, c->stream));
-        k_compact<<<1024, 256, 0, c->stream>>>(s->d_cells[nxt], s->d_mc, s->d_vpool, s->d_fpool, nv, nf, s->d_mc,
-                                               s->cellCap);
-        MCHK(hipMemcpyAsync(s->h_mc, s->d_mc, sizeof(MeshCounters), hipMemcpyDeviceToHost, c->stream));
+        if (!mok(c, hipMalloc(&nf, sizeof(uint32_t) * 3 * (size_t)s->fCap), "hipMalloc(face pool)")) {
+            hipFree(nv);
+            return FF_E_DEVICE;
+        }
+        MCHK(hipMemsetAsync(s->d_arena, 0, sizeof(unsigned long long) * FF_ARENAS * FF_ARENA_STRIDE, c->stream));
+        Pools dst = pools(s);
+        dst.v = nv;
+        dst.f = nf;
+        k_compact<<<1024, 256, 0, c->stream>>>(s->d_cells[nxt], s->d_mc, s->cellCap, s->d_vpool, s->d_fpool, dst);
         MCHK(hipStreamSynchronize(c->stream));
         hipFree(s->d_vpool);
         hipFree(s->d_fpool);
         s->d_vpool = nv;
         s->d_fpool = nf;
-        if (h.flags & FF_MESH_OVERFLOW) {
+        if (flags & FF_MESH_OVERFLOW) {
             // cells that did not fit were stored empty; mesh everything again next time
             s->nCells[nxt] = 0;
-            hipMemsetAsync(s->d_hkeys, 0xff, sizeof(uint64_t) * (s->hmask + 1), c->stream);
+            MCHK(hipMemsetAsync(s->d_hkeys, 0xff, sizeof(uint64_t) * (s->hmask + 1), c->stream));
         }
     }
-    return (int)h.nDirty;
+    return (int)nDirty;
 }
 
 // host copy of the current cell table, sorted by key
@@ -1292,29 +1382,56 @@ int ff_copy_mesh(ff_ctx *c, float *xyz, uint8_t *rgb, uint32_t *faces)
     std::vector<CellRec> cells;
     int rc = host_cells(c, cells);
     if (rc) return rc;
-    const uint32_t va = s->h_mc->vAlloc, fa = s->h_mc->fAlloc;
-    std::vector<float4> vp(va);
-    std::vector<uint32_t> fp(3 * (size_t)fa);
-    if (va) MCHK(hipMemcpy(vp.data(), s->d_vpool, sizeof(float4) * va, hipMemcpyDeviceToHost));
-    if (fa) MCHK(hipMemcpy(fp.data(), s->d_fpool, sizeof(uint32_t) * 3 * (size_t)fa, hipMemcpyDeviceToHost));
-    size_t vo = 0, fo = 0;
-    for (const auto &r : cells) {
-        for (uint32_t i = 0; i < r.nv; i++) {
-            const float4 v = vp[r.v0 + i];
-            if (xyz) { xyz[3 * (vo + i)] = v.x; xyz[3 * (vo + i) + 1] = v.y; xyz[3 * (vo + i) + 2] = v.z; }
-            if (rgb) {
-                uint32_t cc;
-                memcpy(&cc, &v.w, sizeof cc);
-                rgb[3 * (vo + i)] = (uint8_t)(cc & 0xffu);
-                rgb[3 * (vo + i) + 1] = (uint8_t)((cc >> 8) & 0xffu);
-                rgb[3 * (vo + i) + 2] = (uint8_t)((cc >> 16) & 0xffu);
-            }
-        }
-        if (faces)
-            for (uint32_t i = 0; i < 3 * r.nf; i++) faces[3 * fo + i] = (uint32_t)vo + fp[3 * (size_t)r.f0 + i];
-        vo += r.nv;
-        fo += r.nf;
+    // device gather of the cells' segments (in key order) into dense buffers
+    std::vector<uint4> segs(cells.size());
+    std::vector<uint2> offs(cells.size());
+    size_t vt = 0, ft = 0;
+    for (size_t k = 0; k < cells.size(); k++) {
+        const CellRec &r = cells[k];
+        segs[k] = make_uint4(r.v0, r.nv, r.f0, r.nf);
+        offs[k] = make_uint2((uint32_t)vt, (uint32_t)ft);
+        vt += r.nv;
+        ft += r.nf;
     }
+    if (vt >= 0xffffffffull || ft >= 0xffffffffull) {
+        c->err = "ff_copy_mesh: mesh larger than 2^32 vertices";
+        return FF_E_ARG;
+    }
+    std::vector<float4> vp(vt);
+    std::vector<uint32_t> fp(3 * ft);
+    if (!cells.empty() && (vt || ft)) {
+        uint4 *d_segs = nullptr;
+        uint2 *d_offs = nullptr;
+        float4 *d_v = nullptr;
+        uint32_t *d_f = nullptr;
+        hipError_t e = hipMalloc(&d_segs, sizeof(uint4) * segs.size());
+        if (e == hipSuccess) e = hipMalloc(&d_offs, sizeof(uint2) * offs.size());
+        if (e == hipSuccess) e = hipMalloc(&d_v, sizeof(float4) * std::max<size_t>(vt, 1));
+        if (e == hipSuccess) e = hipMalloc(&d_f, sizeof(uint32_t) * 3 * std::max<size_t>(ft, 1));
+        if (e == hipSuccess) e = hipMemcpyAsync(d_segs, segs.data(), sizeof(uint4) * segs.size(), hipMemcpyHostToDevice, c->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(d_offs, offs.data(), sizeof(uint2) * offs.size(), hipMemcpyHostToDevice, c->stream);
+        if (e == hipSuccess) {
+            k_gather<<<1024, 256, 0, c->stream>>>(d_segs, d_offs, (uint32_t)cells.size(), s->d_vpool, s->d_fpool, d_v, d_f);
+            e = hipGetLastError();
+        }
+        if (e == hipSuccess && vt) e = hipMemcpyAsync(vp.data(), d_v, sizeof(float4) * vt, hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess && ft) e = hipMemcpyAsync(fp.data(), d_f, sizeof(uint32_t) * 3 * ft, hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        hipFree(d_segs); hipFree(d_offs); hipFree(d_v); hipFree(d_f);
+        MCHK(e);
+    }
+    for (size_t i = 0; i < vt; i++) {
+        const float4 v = vp[i];
+        if (xyz) { xyz[3 * i] = v.x; xyz[3 * i + 1] = v.y; xyz[3 * i + 2] = v.z; }
+        if (rgb) {
+            uint32_t cc;
+            memcpy(&cc, &v.w, sizeof cc);
+            rgb[3 * i] = (uint8_t)(cc & 0xffu);
+            rgb[3 * i + 1] = (uint8_t)((cc >> 8) & 0xffu);
+            rgb[3 * i + 2] = (uint8_t)((cc >> 16) & 0xffu);
+        }
+    }
+    if (faces) memcpy(faces, fp.data(), sizeof(uint32_t) * fp.size());
     return FF_OK;
 }
 
