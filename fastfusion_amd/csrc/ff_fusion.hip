@@ -140,6 +140,36 @@ __device__ __forceinline__ void voxel_of(const float *__restrict__ qx, const flo
 }
 
 // ---------------------------------------------------------------------------
+// k_frame_begin: the frame's ray tables qx = R * pxp (per column) and
+// qy = R * pyp + R_z (per row), as the reference forms them in float
+// (geometryfusion_mipmap_cpu.cpp:1765-1790), and the frame counters reset
+// ---------------------------------------------------------------------------
+__global__ void k_frame_begin(const float *__restrict__ pxp, const float *__restrict__ pyp, int W, int H, CamF p,
+                              float *__restrict__ qx, float *__restrict__ qy, FrameCounters *fc, uint32_t n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < W) {
+        const float v = pxp[i];
+        qx[i] = __fmul_rn(p.r[0], v);
+        qx[W + i] = __fmul_rn(p.r[3], v);
+        qx[2 * W + i] = __fmul_rn(p.r[6], v);
+    }
+    if (i < H) {
+        const float v = pyp[i];
+        qy[i] = __fadd_rn(__fmul_rn(p.r[1], v), p.r[2]);
+        qy[H + i] = __fadd_rn(__fmul_rn(p.r[4], v), p.r[5]);
+        qy[2 * H + i] = __fadd_rn(__fmul_rn(p.r[7], v), p.r[8]);
+    }
+    if (i == 0) {
+        FrameCounters z;
+        z.nQueued = z.listCount = z.newB = z.newL = 0u;
+        for (int k = 0; k < 3; k++) { z.bmin[k] = 0; z.bmax[k] = (int)n - 1; }
+        z.flags = z.nProcessed = z.sdfRan = 0u;
+        *fc = z;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // k_planes: per 64-pixel row chunk, ballot the three relations into bit
 // planes: plane[(y*words + w)*3 + {0: valid, 1: key == left key, 2: key == up key}]
 // ---------------------------------------------------------------------------
@@ -188,57 +218,100 @@ __global__ __launch_bounds__(64) void k_planes(const uint16_t *__restrict__ dept
 
 // ---------------------------------------------------------------------------
 // k_dedup: P(x,y) = valid & !(Eu & P(x,y-1)) & !(El & P(x-1,y)).
-// Lane l owns columns [32 l, 32 l + 32) and handles row s - l at step s, so
-// the left neighbour's result for the same row arrives from lane l-1's
-// previous step through a lane shuffle; rows of a lane are carried in a
-// register.  Within a lane the horizontal chain is solved bit-parallel:
-// maximal runs linked by El alternate 1,0,1,... from their start.
+// Wave 0 of the block solves the recurrence: lane l owns columns
+// [32 l, 32 l + 32) and handles row s - l at step s, so the left neighbour's
+// result for the same row arrives from lane l-1's previous step through a
+// one-lane DPP shift; rows of a lane are carried in a register.  Within a
+// lane the horizontal chain is solved bit-parallel: maximal runs linked by
+// El alternate 1,0,1,... from their start.  The other waves stream the bit
+// planes through an LDS ring of 8 bands of R rows, three bands ahead of the
+// sweep, with one block barrier per band.  Rows outside the image read ring
+// rows of other bands: their valid bits are masked, so they yield P = 0.
+// pbits rows are FF_PB_STRIDE words apart with FF_PB_PAD rows of padding
+// above and below, so every step stores unconditionally.
 // ---------------------------------------------------------------------------
-template <int PF>
-__global__ __launch_bounds__(64) void k_dedup(const uint32_t *__restrict__ planes, int H, int words,
-                                              uint32_t *__restrict__ pbits)
+#define FF_DD_NB 8
+#define FF_DD_THREADS 1024
+
+// R = 32 while one uint4 per loader thread covers a band (24 words <= 960)
+static int dedup_rows(int words) { return words <= 40 ? 32 : 16; }
+size_t dedup_lds_bytes(int words) { return (size_t)FF_DD_NB * dedup_rows(words) * words * 3 * sizeof(uint32_t); }
+int dedup_plane_rows(int H) { return (H + 31) / 32 * 32; }
+
+__device__ __forceinline__ uint32_t lane_shr1(uint32_t v)
 {
-    // Branch-free on purpose: loads come from clamped addresses and stores of
-    // inactive steps go to the sink row H, so the compiler keeps PF loads in
-    // flight instead of waiting for each one.
-    const int l = threadIdx.x;
+    // wave_shr:1 -- lane l receives lane l-1's value, lane 0 keeps 0
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x138, 0xf, 0xf, false);
+}
+
+template <int R>
+__global__ __launch_bounds__(FF_DD_THREADS) void k_dedup(const uint32_t *__restrict__ planes, int H, int words,
+                                                         uint32_t *__restrict__ pbits, FrameCounters *fc)
+{
+    extern __shared__ uint32_t ring[];
+    constexpr int RING = FF_DD_NB * R;  // rows, a power of two
+    const int tid = threadIdx.x;
+    const bool compute = tid < 64;
+    const int rowDw = words * 3;
+    const int bandV = R * rowDw / 4;  // uint4 per band (R * 3 * words is a multiple of 4)
+    const int nBands = (H + R - 1) / R;
+    const int nsteps = H + words - 1;
+    const int niter = (nsteps + R - 1) / R;
+    const uint4 *src = reinterpret_cast<const uint4 *>(planes);
+    uint4 *ringV = reinterpret_cast<uint4 *>(ring);
+    const int lt = tid - 64, nl = (int)blockDim.x - 64;  // loader index and count (nl >= bandV)
+    uint4 pre = make_uint4(0, 0, 0, 0);
+    if (!compute) {
+        for (int b = 0; b < 3 && b < nBands; b++)
+            for (int i = lt; i < bandV; i += nl) ringV[b * bandV + i] = src[(size_t)b * bandV + i];
+        if (3 < nBands && lt < bandV) pre = src[(size_t)3 * bandV + lt];
+    }
+    __syncthreads();
+    const int l = tid & 63;
     const bool lane_ok = l < words;
     const int lw = lane_ok ? l : words - 1;
-    const int nsteps = H + words - 1;
-    uint32_t ra[PF], rb[PF], rc[PF];
+    const uint32_t Hl = lane_ok ? (uint32_t)H : 0u;
+    uint32_t *out = pbits + (size_t)(FF_PB_PAD - l) * FF_PB_STRIDE + l;  // row 0 - l: the lane's row at step 0
+    uint32_t pprev = 0u, pout = 0u, nproc = 0u;
+    for (int it = 0; it < niter; it++) {
+        if (compute) {
+            // the band's plane words first, so the LDS latency is paid once
+            uint32_t rva[R], rel[R], reu[R];
+            const int r0 = it * R - l;
 #pragma unroll
-    for (int k = 0; k < PF; k++) {
-        const int y = min(max(k - l, 0), H - 1);
-        const uint32_t *p = planes + ((size_t)y * words + lw) * 3;
-        ra[k] = p[0]; rb[k] = p[1]; rc[k] = p[2];
-    }
-    uint32_t pprev = 0u, pout = 0u;
-    for (int s0 = 0; s0 < nsteps; s0 += PF) {
-#pragma unroll
-        for (int k = 0; k < PF; k++) {
-            const int s = s0 + k, y = s - l;
-            const bool active = lane_ok && y >= 0 && y < H && s < nsteps;
-            const uint32_t va = active ? ra[k] : 0u, el = rb[k], eu = rc[k];
-            {
-                const int yn = min(max(y + PF, 0), H - 1);
-                const uint32_t *p = planes + ((size_t)yn * words + lw) * 3;
-                ra[k] = p[0]; rb[k] = p[1]; rc[k] = p[2];
+            for (int k = 0; k < R; k++) {
+                const uint32_t *r = ring + ((r0 + k) & (RING - 1)) * rowDw + lw * 3;
+                rva[k] = r[0]; rel[k] = r[1]; reu[k] = r[2];
             }
-            uint32_t pin = __shfl_up(pout, 1);
-            pin = (l == 0) ? 0u : pin;
-            const uint32_t a = va & ~(eu & pprev);
-            const uint32_t virt = a & el & pin & 1u;          // bit 0 continues a chain whose last was processed
-            const uint32_t cont = (a & el & (a << 1)) | virt;  // bit j continues from j-1
-            const uint32_t S = a & ~cont;                      // chain starts
-            const uint32_t ev = S & 0x55555555u, od = S & 0xaaaaaaaau;
-            const uint32_t fe = ev | (((cont + (ev << 1)) ^ cont) & cont);
-            const uint32_t fo = od | (((cont + ((od << 1) | virt)) ^ cont) & cont);
-            const uint32_t P = a & ((fe & 0x55555555u) | (fo & 0xaaaaaaaau));
-            const size_t dst = active ? (size_t)y * words + l : (size_t)H * words + lw;
-            pbits[dst] = P;
-            pprev = active ? P : pprev;
-            pout = active ? (P >> 31) : 0u;
+            uint32_t *o = out + (size_t)it * R * FF_PB_STRIDE;
+#pragma unroll
+            for (int k = 0; k < R; k++) {
+                const uint32_t va = (uint32_t)(r0 + k) < Hl ? rva[k] : 0u, el = rel[k], eu = reu[k];
+                const uint32_t pin = lane_shr1(pout);
+                const uint32_t a = va & ~(eu & pprev);
+                const uint32_t virt = a & el & pin & 1u;          // bit 0 continues a chain whose last was processed
+                const uint32_t cont = (a & el & (a << 1)) | virt;  // bit j continues from j-1
+                const uint32_t S = a & ~cont;                      // chain starts
+                const uint32_t ev = S & 0x55555555u, od = S & 0xaaaaaaaau;
+                const uint32_t fe = ev | (((cont + (ev << 1)) ^ cont) & cont);
+                const uint32_t fo = od | (((cont + ((od << 1) | virt)) ^ cont) & cont);
+                const uint32_t P = a & ((fe & 0x55555555u) | (fo & 0xaaaaaaaau));
+                o[k * FF_PB_STRIDE] = P;
+                nproc += (uint32_t)__popc(P);
+                pprev = P;
+                pout = P >> 31;
+            }
+        } else {
+            // band it + 3 into the slot of band it - 5; band it + 4 into registers
+            const int wb = it + 3, lb = it + 4;
+            if (wb < nBands && lt < bandV) ringV[(wb % FF_DD_NB) * bandV + lt] = pre;
+            if (lb < nBands && lt < bandV) pre = src[(size_t)lb * bandV + lt];
         }
+        __syncthreads();
+    }
+    if (compute) {
+        for (int o = 32; o > 0; o >>= 1) nproc += __shfl_xor(nproc, o);
+        if (l == 0) fc->nProcessed = nproc;
     }
 }
 
@@ -262,10 +335,9 @@ __global__ void k_descend(const uint16_t *__restrict__ depth, int W, int H, int 
 {
     const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
     const int lane = threadIdx.x & 63;
-    const bool proc = x < W && ((pbits[(size_t)y * words + (x >> 5)] >> (x & 31)) & 1u);
+    const bool proc = x < W && ((pbits[(size_t)(y + FF_PB_PAD) * FF_PB_STRIDE + (x >> 5)] >> (x & 31)) & 1u);
     const uint64_t pb = __ballot(proc);
     if (pb == 0) return;  // wave-uniform
-    if (lane == 0) atomicAdd(&fc->nProcessed, (uint32_t)__popcll(pb));
     const uint32_t pix = (uint32_t)(y * W + x);
     const float t[3] = {t0, t1, t2};
     int v[3] = {0, 0, 0};
@@ -579,16 +651,13 @@ __global__ void k_shift_leaves(ushort4 *info, uint32_t n, uint32_t ax, uint32_t 
 // rebuilds its row origin with the same sequential adds as the reference
 // (z steps, then y steps) and adds k * d1j like the AVX lane k.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(512) void k_sdf(const uint32_t *__restrict__ queue, MapView m,
-                                             const uint16_t *__restrict__ depth,
-                                             const uint8_t *__restrict__ bgr, int W, int H, SdfParams P)
+__device__ __forceinline__ void sdf_brick(uint32_t leaf, const MapView &m, const uint16_t *__restrict__ depth,
+                                          const uint8_t *__restrict__ bgr, int W, int H, const SdfParams &P,
+                                          const uint32_t *tab)
 {
-    __shared__ uint32_t tab[2048];
-    for (int i = threadIdx.x; i < 2048; i += blockDim.x) tab[i] = c_rcp_table[i];
     const int tid = threadIdx.x;
     const int zr = tid >> 6, yr = (tid >> 3) & 7, k = tid & 7;
     const float EPS = 0.005f;
-    const uint32_t leaf = queue[blockIdx.x];
     const ushort4 info = m.leafinfo[leaf];
     const size_t vi = (size_t)leaf * FF_BRICK + tid;
     const float wA = m.wgt[vi], dA = m.dist[vi];
@@ -626,7 +695,6 @@ __global__ __launch_bounds__(512) void k_sdf(const uint32_t *__restrict__ queue,
     const float px = rz_add(R[0], k ? rz_mul(fk, d1[0]) : 0.0f);
     const float py = rz_add(R[1], k ? rz_mul(fk, d1[1]) : 0.0f);
     const float pz = rz_add(R[2], k ? rz_mul(fk, d1[2]) : 0.0f);
-    __syncthreads();  // rcp table ready
     const float rfall = rcp_x86(rz_sub(tw, EPS), tab);
     const float len = rz_sqrt(rz_add(rz_add(rz_mul(px, px), rz_mul(py, py)), rz_mul(pz, pz)));
     const float r = rcp_x86(pz, tab);
@@ -661,6 +729,36 @@ __global__ __launch_bounds__(512) void k_sdf(const uint32_t *__restrict__ queue,
         }
     }
 }
+
+#define FF_SDF_GRID 1024  // blocks of the SDF update; each loops over queued bricks
+
+__device__ __forceinline__ bool sdf_gate_open(const FrameCounters *fc, const SdfGate &g)
+{
+    if (g.force) return true;
+    if ((fc->flags & FF_FLAG_HASH_FULL) || fc->listCount > g.listCap) return false;
+    const bool out = fc->bmin[0] < 0 || fc->bmin[1] < 0 || fc->bmin[2] < 0 || fc->bmax[0] >= (int)g.n ||
+                     fc->bmax[1] >= (int)g.n || fc->bmax[2] >= (int)g.n;
+    const bool grows = g.leavesUsed + fc->newL < g.leavesTotal && g.branchesUsed + 16u * fc->newB < g.branchesTotal;
+    return !(out && grows);
+}
+
+__global__ __launch_bounds__(512) void k_sdf(const uint32_t *__restrict__ queue, MapView m,
+                                             const uint16_t *__restrict__ depth,
+                                             const uint8_t *__restrict__ bgr, int W, int H, SdfParams P,
+                                             FrameCounters *fc, SdfGate gate)
+{
+    // blocks loop over the queued bricks; the count comes from the device
+    if (!sdf_gate_open(fc, gate)) return;
+    const uint32_t nQ = fc->nQueued;
+    if (blockIdx.x == 0 && threadIdx.x == 0) fc->sdfRan = 1u;
+    if (blockIdx.x >= nQ) return;
+    __shared__ uint32_t tab[2048];
+    for (int i = threadIdx.x; i < 2048; i += blockDim.x) tab[i] = c_rcp_table[i];
+    __syncthreads();  // rcp table ready
+    for (uint32_t b = blockIdx.x; b < nQ; b += gridDim.x)
+        sdf_brick(queue[b], m, depth, bgr, W, H, P, tab);
+}
+
 
 }  // namespace ff
 
@@ -724,33 +822,27 @@ static int ensure_image(ff_ctx *c, int W, int H)
     HIPCHK(hipMalloc(&c->d_bgr, npix * 3));
     HIPCHK(hipMalloc(&c->d_qx, sizeof(float) * 3 * W));
     HIPCHK(hipMalloc(&c->d_qy, sizeof(float) * 3 * H));
-    HIPCHK(hipMalloc(&c->d_planes, sizeof(uint32_t) * 3 * words * H));
-    HIPCHK(hipMalloc(&c->d_pbits, sizeof(uint32_t) * words * (H + 1)));  // + sink row
+    HIPCHK(hipMalloc(&c->d_pxp, sizeof(float) * W));
+    HIPCHK(hipMalloc(&c->d_pyp, sizeof(float) * H));
+    // bit planes padded to whole bands of the de-dup ring
+    HIPCHK(hipMalloc(&c->d_planes, sizeof(uint32_t) * 3 * words * (size_t)dedup_plane_rows(H)));
+    HIPCHK(hipFuncSetAttribute((const void *)k_dedup<32>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)dedup_lds_bytes(40)));
+    HIPCHK(hipFuncSetAttribute((const void *)k_dedup<16>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)dedup_lds_bytes(64)));
+    HIPCHK(hipMalloc(&c->d_pbits, pbits_bytes(H)));
     HIPCHK(hipMalloc(&c->d_cntB, sizeof(uint32_t) * npix));
     HIPCHK(hipMalloc(&c->d_cntL, sizeof(uint32_t) * npix));
     HIPCHK(hipMalloc(&c->d_firstD, sizeof(uint32_t) * npix));
     HIPCHK(hipMalloc(&c->d_pl0, sizeof(uint32_t) * 2 * npix));
     HIPCHK(hipMalloc(&c->d_pref, sizeof(uint64_t) * npix));
     HIPCHK(hipMalloc(&c->d_bsum, sizeof(uint64_t) * ((npix + 4095) / 4096 + 1)));
-    HIPCHK(hipMemsetAsync(c->d_pbits, 0, sizeof(uint32_t) * words * H, c->stream));
+    HIPCHK(hipMemsetAsync(c->d_pbits, 0, pbits_bytes(H), c->stream));
     c->capW = W;
     c->capH = H;
     return FF_OK;
 }
 
-// accumulates event pairs of completed SDF / mesh launches
-void resolve_timing(ff_ctx *c)
-{
-    for (int i = 0; i < 2; i++) {
-        if (!c->sdfPending[i]) continue;
-        float ms = 0.0f;
-        if (hipEventElapsedTime(&ms, c->ev[2 + 2 * i], c->ev[3 + 2 * i]) == hipSuccess) {
-            c->phaseMs[1] += ms;
-            c->phaseCalls[1]++;
-        }
-        c->sdfPending[i] = false;
-    }
-}
 
 static void launch_grow(ff_ctx *c, const int bmin[3], const int bmax[3])
 {
@@ -833,9 +925,9 @@ using namespace ff;
 
 void ff_ctx::freeFrameBuffers()
 {
-    hipFree(d_depth); hipFree(d_bgr); hipFree(d_qx); hipFree(d_qy); hipFree(d_planes); hipFree(d_pbits);
+    hipFree(d_depth); hipFree(d_bgr); hipFree(d_qx); hipFree(d_qy); hipFree(d_pxp); hipFree(d_pyp); hipFree(d_planes); hipFree(d_pbits);
     hipFree(d_cntB); hipFree(d_cntL); hipFree(d_firstD); hipFree(d_pl0); hipFree(d_pref); hipFree(d_bsum);
-    d_depth = nullptr; d_bgr = nullptr; d_qx = d_qy = nullptr; d_planes = d_pbits = nullptr;
+    d_depth = nullptr; d_bgr = nullptr; d_qx = d_qy = d_pxp = d_pyp = nullptr; d_planes = d_pbits = nullptr;
     d_cntB = d_cntL = d_firstD = d_pl0 = nullptr; d_pref = d_bsum = nullptr;
     capW = capH = 0;
 }
@@ -926,8 +1018,9 @@ const char *ff_last_error(const ff_ctx *c) { return c ? c->err.c_str() : "null c
 
 void *ff_stream(ff_ctx *c) { return c ? (void *)c->stream : nullptr; }
 
-static int add_map_impl(ff_ctx *c, int W, int H, const double R[9], const double t[3], const double K[4],
-                        float scaling, float maxcam)
+// depth / bgr: the frame in device memory, dense rows
+static int add_map_impl(ff_ctx *c, const uint16_t *depth, const uint8_t *bgr, int W, int H, const double R[9],
+                        const double t[3], const double K[4], float scaling, float maxcam)
 {
     CamF p = make_p(c, R, t, K);
     FrameCounters *hf = c->h_fc;
@@ -937,7 +1030,7 @@ static int add_map_impl(ff_ctx *c, int W, int H, const double R[9], const double
         memset(hf, 0, sizeof(FrameCounters));
         for (int k = 0; k < 3; k++) { hf->bmin[k] = 1000000; hf->bmax[k] = -1000000; }
         HIPCHK(hipMemcpyAsync(c->d_fc, hf, sizeof(FrameCounters), hipMemcpyHostToDevice, c->stream));
-        k_bbox_first<<<dim3((W + 255) / 256, H), 256, 0, c->stream>>>(c->d_depth, W, H, scaling, maxcam, p,
+        k_bbox_first<<<dim3((W + 255) / 256, H), 256, 0, c->stream>>>(depth, W, H, scaling, maxcam, p,
                                                                        c->bandwidth, c->d_fc);
         HIPCHK(hipMemcpyAsync(hf, c->d_fc, sizeof(FrameCounters), hipMemcpyDeviceToHost, c->stream));
         HIPCHK(hipStreamSynchronize(c->stream));
@@ -960,36 +1053,28 @@ static int add_map_impl(ff_ctx *c, int W, int H, const double R[9], const double
         c->pxp.resize(W); c->pyp.resize(H);
         for (int x = 0; x < W; x++) c->pxp[x] = ((float)x - p.cx) / p.fx;
         for (int y = 0; y < H; y++) c->pyp[y] = ((float)y - p.cy) / p.fy;
+        HIPCHK(hipMemcpyAsync(c->d_pxp, c->pxp.data(), sizeof(float) * W, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(hipMemcpyAsync(c->d_pyp, c->pyp.data(), sizeof(float) * H, hipMemcpyHostToDevice, c->stream));
+        HIPCHK(hipStreamSynchronize(c->stream));  // the host tables are reused
     }
     const CamF q = make_pinv(c, R, t, K);
-    std::vector<float> &qx = c->h_qx, &qy = c->h_qy;
-    qx.resize(3 * W); qy.resize(3 * H);
-    for (int x = 0; x < W; x++) {
-        qx[x] = p.r[0] * c->pxp[x];
-        qx[W + x] = p.r[3] * c->pxp[x];
-        qx[2 * W + x] = p.r[6] * c->pxp[x];
-    }
-    for (int y = 0; y < H; y++) {
-        qy[y] = p.r[1] * c->pyp[y] + p.r[2];
-        qy[H + y] = p.r[4] * c->pyp[y] + p.r[5];
-        qy[2 * H + y] = p.r[7] * c->pyp[y] + p.r[8];
-    }
-    HIPCHK(hipMemcpyAsync(c->d_qx, qx.data(), sizeof(float) * 3 * W, hipMemcpyHostToDevice, c->stream));
-    HIPCHK(hipMemcpyAsync(c->d_qy, qy.data(), sizeof(float) * 3 * H, hipMemcpyHostToDevice, c->stream));
-
-    memset(hf, 0, sizeof(FrameCounters));
-    for (int k = 0; k < 3; k++) { hf->bmin[k] = 0; hf->bmax[k] = (int)c->n - 1; }
-    HIPCHK(hipMemcpyAsync(c->d_fc, hf, sizeof(FrameCounters), hipMemcpyHostToDevice, c->stream));
+    k_frame_begin<<<(std::max(W, H) + 255) / 256, 256, 0, c->stream>>>(c->d_pxp, c->d_pyp, W, H, p, c->d_qx,
+                                                                       c->d_qy, c->d_fc, c->n);
     const int words = (W + 31) / 32;
     const uint32_t npix = (uint32_t)W * H;
     if (c->n) {
         if (c->timing) hipEventRecord(c->ev[0], c->stream);
-        k_planes<<<dim3((W + 63) / 64, H), 64, 0, c->stream>>>(c->d_depth, W, H, words, scaling, maxcam, c->d_qx,
+        k_planes<<<dim3((W + 63) / 64, H), 64, 0, c->stream>>>(depth, W, H, words, scaling, maxcam, c->d_qx,
                                                               c->d_qy, p.t[0], p.t[1], p.t[2], c->d_planes,
                                                               c->d_cntB, c->d_cntL, c->d_firstD);
-        k_dedup<16><<<1, 64, 0, c->stream>>>(c->d_planes, H, words, c->d_pbits);
+        if (dedup_rows(words) == 32)
+            k_dedup<32><<<1, FF_DD_THREADS, dedup_lds_bytes(words), c->stream>>>(c->d_planes, H, words, c->d_pbits,
+                                                                                 c->d_fc);
+        else
+            k_dedup<16><<<1, FF_DD_THREADS, dedup_lds_bytes(words), c->stream>>>(c->d_planes, H, words, c->d_pbits,
+                                                                                 c->d_fc);
         MapView mv = c->mapView();
-        k_descend<<<dim3((W + 127) / 128, H), 128, 0, c->stream>>>(c->d_depth, W, H, words, scaling, c->d_qx,
+        k_descend<<<dim3((W + 127) / 128, H), 128, 0, c->stream>>>(depth, W, H, words, scaling, c->d_qx,
                                                                    c->d_qy, p.t[0], p.t[1], p.t[2], c->d_pbits,
                                                                    mv, c->hv, c->n, c->bandwidth, stamp,
                                                                    c->d_pl0, c->d_fc);
@@ -1004,17 +1089,42 @@ static int add_map_impl(ff_ctx *c, int W, int H, const double R[9], const double
         k_hash_clear<<<256, 256, 0, c->stream>>>(c->hv, c->d_fc);
         if (c->timing) hipEventRecord(c->ev[1], c->stream);
     } else {
-        HIPCHK(hipMemsetAsync(c->d_pbits, 0, sizeof(uint32_t) * words * H, c->stream));
+        HIPCHK(hipMemsetAsync(c->d_pbits, 0, pbits_bytes(H), c->stream));
+    }
+    // the SDF update follows on the device; it waits for the host only when
+    // the node table overflowed or the volume must grow first
+    SdfParams sp;
+    for (int i = 0; i < 3; i++) {
+        sp.m[4 * i] = q.r[3 * i]; sp.m[4 * i + 1] = q.r[3 * i + 1]; sp.m[4 * i + 2] = q.r[3 * i + 2];
+        sp.m[4 * i + 3] = q.t[i];
+    }
+    sp.fx = q.fx; sp.fy = q.fy; sp.cx = q.cx; sp.cy = q.cy;
+    sp.scale = c->scale; sp.dT = c->dT; sp.scaling = scaling; sp.maxcam = maxcam;
+    sp.useColor = c->useColor ? 1 : 0;
+    SdfGate gate;
+    gate.listCap = c->hv.listCap;
+    gate.leavesUsed = c->nLeavesUsed; gate.leavesTotal = c->nLeavesTotal;
+    gate.branchesUsed = c->nBranchesUsed; gate.branchesTotal = c->nBranchesTotal;
+    gate.n = c->n;
+    gate.force = 0;
+    if (c->n) {
+        if (c->timing) hipEventRecord(c->ev[2], c->stream);
+        k_sdf<<<FF_SDF_GRID, 512, 0, c->stream>>>(c->d_queue, c->mapView(), depth, bgr, W, H, sp, c->d_fc, gate);
+        if (c->timing) hipEventRecord(c->ev[3], c->stream);
+        HIPCHK(hipGetLastError());
     }
     HIPCHK(hipMemcpyAsync(hf, c->d_fc, sizeof(FrameCounters), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
-    if (c->timing) {
+    if (c->timing && c->n) {
         float ms = 0.0f;
-        if (c->n && hipEventElapsedTime(&ms, c->ev[0], c->ev[1]) == hipSuccess) {
+        if (hipEventElapsedTime(&ms, c->ev[0], c->ev[1]) == hipSuccess) {
             c->phaseMs[0] += ms;
             c->phaseCalls[0]++;
         }
-        resolve_timing(c);  // the previous frame's SDF update completed before this sync
+        if (hf->sdfRan && hipEventElapsedTime(&ms, c->ev[2], c->ev[3]) == hipSuccess) {
+            c->phaseMs[1] += ms;
+            c->phaseCalls[1]++;
+        }
     }
     if (hf->flags & FF_FLAG_HASH_FULL) {
         c->err = "per-frame node table overflow";
@@ -1039,23 +1149,19 @@ static int add_map_impl(ff_ctx *c, int W, int H, const double R[9], const double
         launch_grow(c, bmin, bmax);
         mesh_on_grow(c);
     }
-    if (c->nQueued) {
-        SdfParams sp;
-        for (int i = 0; i < 3; i++) {
-            sp.m[4 * i] = q.r[3 * i]; sp.m[4 * i + 1] = q.r[3 * i + 1]; sp.m[4 * i + 2] = q.r[3 * i + 2];
-            sp.m[4 * i + 3] = q.t[i];
-        }
-        sp.fx = q.fx; sp.fy = q.fy; sp.cx = q.cx; sp.cy = q.cy;
-        sp.scale = c->scale; sp.dT = c->dT; sp.scaling = scaling; sp.maxcam = maxcam;
-        sp.useColor = c->useColor ? 1 : 0;
-        const int e = 2 + 2 * (int)(c->frames & 1u);  // alternate event pairs 2/3 and 4/5
-        if (c->timing) hipEventRecord(c->ev[e], c->stream);
-        k_sdf<<<c->nQueued, 512, 0, c->stream>>>(c->d_queue, c->mapView(), c->d_depth, c->d_bgr, W, H, sp);
-        if (c->timing) {
-            hipEventRecord(c->ev[e + 1], c->stream);
-            c->sdfPending[e == 2 ? 0 : 1] = true;
-        }
+    if (c->nQueued && !hf->sdfRan) {
+        // the gated launch stood down (volume grown above): update now
+        gate.force = 1;
+        if (c->timing) hipEventRecord(c->ev[4], c->stream);
+        k_sdf<<<FF_SDF_GRID, 512, 0, c->stream>>>(c->d_queue, c->mapView(), depth, bgr, W, H, sp, c->d_fc, gate);
+        if (c->timing) hipEventRecord(c->ev[5], c->stream);
         HIPCHK(hipGetLastError());
+        HIPCHK(hipStreamSynchronize(c->stream));
+        float ms = 0.0f;
+        if (c->timing && hipEventElapsedTime(&ms, c->ev[4], c->ev[5]) == hipSuccess) {
+            c->phaseMs[1] += ms;
+            c->phaseCalls[1]++;
+        }
     }
     // algorithmic bytes: images read once, 16 B per processed pixel, 512 voxels
     // x 14 B (8 B without color) read and written per queued brick
@@ -1088,7 +1194,7 @@ int ff_add_map(ff_ctx *c, const uint16_t *depth, size_t depth_stride, const uint
     if (c->useColor)
         HIPCHK(hipMemcpy2DAsync(c->d_bgr, (size_t)W * 3, bgr, bgr_stride, (size_t)W * 3, H, hipMemcpyHostToDevice,
                                 c->stream));
-    return add_map_impl(c, W, H, R, t, K, scaling, maxcam);
+    return add_map_impl(c, c->d_depth, c->useColor ? c->d_bgr : nullptr, W, H, R, t, K, scaling, maxcam);
 }
 
 int ff_add_map_device(ff_ctx *c, const uint16_t *depth_dev, const uint8_t *bgr_dev, int W, int H, const double R[9],
@@ -1102,10 +1208,8 @@ int ff_add_map_device(ff_ctx *c, const uint16_t *depth_dev, const uint8_t *bgr_d
     hipSetDevice(c->device);
     int rc = ensure_image(c, W, H);
     if (rc) return rc;
-    HIPCHK(hipMemcpyAsync(c->d_depth, depth_dev, (size_t)W * H * 2, hipMemcpyDeviceToDevice, c->stream));
-    if (c->useColor)
-        HIPCHK(hipMemcpyAsync(c->d_bgr, bgr_dev, (size_t)W * H * 3, hipMemcpyDeviceToDevice, c->stream));
-    return add_map_impl(c, W, H, R, t, K, scaling, maxcam);
+    // read in place: the call returns after the frame's kernels completed
+    return add_map_impl(c, depth_dev, c->useColor ? bgr_dev : nullptr, W, H, R, t, K, scaling, maxcam);
 }
 
 int ff_set_timing(ff_ctx *c, int on)
@@ -1121,7 +1225,6 @@ int ff_get_timing(ff_ctx *c, double *ms, uint64_t *calls, double *bytes, uint64_
 {
     if (!c) return FF_E_ARG;
     HIPCHK(hipStreamSynchronize(c->stream));
-    resolve_timing(c);
     for (int i = 0; i < 3; i++) {
         if (ms) ms[i] = c->phaseMs[i];
         if (calls) calls[i] = c->phaseCalls[i];
@@ -1209,10 +1312,12 @@ int ff_copy_processed(ff_ctx *c, uint8_t *out, size_t n)
     if (!c || !out || n < (size_t)c->imgW * c->imgH) return FF_E_ARG;
     HIPCHK(hipStreamSynchronize(c->stream));
     const int W = c->imgW, H = c->imgH, words = (W + 31) / 32;
-    std::vector<uint32_t> bits((size_t)words * H);
+    (void)words;
+    std::vector<uint32_t> bits((size_t)(H + 2 * FF_PB_PAD) * FF_PB_STRIDE);
     HIPCHK(hipMemcpy(bits.data(), c->d_pbits, bits.size() * 4, hipMemcpyDeviceToHost));
     for (int y = 0; y < H; y++)
-        for (int x = 0; x < W; x++) out[(size_t)y * W + x] = (bits[(size_t)y * words + (x >> 5)] >> (x & 31)) & 1u;
+        for (int x = 0; x < W; x++)
+            out[(size_t)y * W + x] = (bits[(size_t)(y + FF_PB_PAD) * FF_PB_STRIDE + (x >> 5)] >> (x & 31)) & 1u;
     return FF_OK;
 }
 

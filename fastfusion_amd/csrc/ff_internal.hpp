@@ -12,6 +12,13 @@
 #define FF_FLAG_HASH_FULL 1u
 #define FF_FLAG_POOL_FULL 2u
 
+// processed-pixel bitmap: 32 pixels per word, rows FF_PB_STRIDE words apart,
+// FF_PB_PAD rows of padding above the image and 2 FF_PB_PAD below it (the
+// de-dup sweep's skewed lanes store up to H + 158 rows past the top)
+#define FF_PB_STRIDE 64
+#define FF_PB_PAD 64
+inline size_t pbits_bytes(int H) { return sizeof(uint32_t) * FF_PB_STRIDE * (size_t)(H + 3 * FF_PB_PAD); }
+
 namespace ff {
 
 // camPamsFloat (src/fusion/definitions.h:70-89)
@@ -23,6 +30,14 @@ struct FrameCounters {
     uint32_t nQueued, listCount, newB, newL;
     int bmin[3], bmax[3];
     uint32_t flags, nProcessed;
+    uint32_t sdfRan;  // set by the SDF update when it ran for the frame
+};
+
+// what the SDF update checks on the device before running: a frame whose
+// node table overflowed, or that must grow the volume first, waits for the host
+struct SdfGate {
+    uint32_t listCap, leavesUsed, leavesTotal, branchesUsed, branchesTotal, n;
+    int force;
 };
 
 // device views of the map arrays
@@ -78,6 +93,7 @@ struct ff_ctx {
     uint16_t *d_depth = nullptr;
     uint8_t *d_bgr = nullptr;
     float *d_qx = nullptr, *d_qy = nullptr;
+    float *d_pxp = nullptr, *d_pyp = nullptr;  // per-column / per-row ray tables of the current image size
     uint32_t *d_planes = nullptr, *d_pbits = nullptr, *d_cntB = nullptr, *d_cntL = nullptr, *d_firstD = nullptr,
              *d_pl0 = nullptr;
     uint64_t *d_pref = nullptr, *d_bsum = nullptr;
@@ -90,7 +106,6 @@ struct ff_ctx {
     hipEvent_t ev[8] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     double phaseMs[3] = {0, 0, 0}, phaseBytes[3] = {0, 0, 0};
     uint64_t phaseCalls[3] = {0, 0, 0}, phaseUnits[3] = {0, 0, 0};
-    bool sdfPending[2] = {false, false};
 
     ff::MapView mapView() const
     {
@@ -113,5 +128,4 @@ void mesh_after_frame(ff_ctx *c);
 void mesh_record_branches(ff_ctx *c);
 void mesh_on_grow_blocks(ff_ctx *c, uint32_t oldBlocks, uint32_t addBlocks, const uint32_t addv[3],
                          const std::vector<ushort4> &chain);
-void resolve_timing(ff_ctx *c);
 }  // namespace ff

@@ -1221,6 +1221,16 @@ void mesh_record_branches(ff_ctx *c)
                                             c->hv.listCap, c->n, s->binfoCap);
 }
 
+__global__ void k_mesh_begin(MeshCounters *mc)
+{
+    if (threadIdx.x == 0) {
+        MeshCounters z;
+        z.nTerm = z.nCells = z.nDirty = z.flags = z.nTiles = 0u;
+        z.pad[0] = z.pad[1] = z.pad[2] = 0u;
+        *mc = z;
+    }
+}
+
 static Pools pools(const MeshState *s)
 {
     Pools P;
@@ -1239,8 +1249,7 @@ static int update_meshes(ff_ctx *c)
     hipEvent_t e0 = c->ev[6], e1 = c->ev[7];
     if (c->timing) hipEventRecord(e0, c->stream);
     const int nxt = 1 - s->cur;
-    MeshCounters zero{};
-    MCHK(hipMemcpyAsync(s->d_mc, &zero, sizeof zero, hipMemcpyHostToDevice, c->stream));
+    k_mesh_begin<<<1, 64, 0, c->stream>>>(s->d_mc);
     const uint32_t nBlocks = c->nBranchesUsed / 16;
     k_terminal<<<(nBlocks * 8 + 255) / 256, 256, 0, c->stream>>>(c->d_tree, s->d_binfo, nBlocks, c->nLeavesTotal, c->n,
                                                                  s->d_term, s->termCap, s->d_mc);
