@@ -124,6 +124,11 @@ def main():
     frame_bytes = sum(v["bytes"] for v in tm.values()) / args.steps
     print(f"[bench] phases {json.dumps(tm)}", file=sys.stderr, flush=True)
 
+    # surface error of the final mesh against the analytic scene (GT)
+    mesh = fz.getMeshInterleavedMarchingCubes()
+    err = sc.sdf(mesh.vertices) if len(mesh.vertices) else np.zeros(1, np.float32)
+    rms = float(np.sqrt(np.mean(err.astype(np.float64) ** 2)))
+
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(depth, bgr, R, t, cfg, args.cpu_seconds)
@@ -151,6 +156,9 @@ def main():
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                      "avg_launch_us": round(sdf_ms * 1e3, 3), "bytes_per_launch": round(sdf_bytes)},
         "phases_ms_per_frame": {k: round(v["ms"] / args.steps, 4) for k, v in tm.items()},
+        "rms_surface_err_m": round(rms, 6),
+        "rms_surface_err_voxels": round(rms / cfg["scale"], 4),
+        "mesh": {"vertices": int(len(mesh.vertices)), "triangles": int(len(mesh.faces))},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

@@ -109,6 +109,7 @@ struct MeshState {
     unsigned long long *d_arena = nullptr;
     MeshCounters *d_mc = nullptr, *h_mc = nullptr;
     uint32_t lastMeshStamp = 0;
+    uint64_t lastVAlloc = 0, lastFAlloc = 0;  // arena totals after the previous update (timing)
     bool haveVolume = false;
 };
 
@@ -1221,6 +1222,20 @@ void mesh_record_branches(ff_ctx *c)
                                             c->hv.listCap, c->n, s->binfoCap);
 }
 
+// totals of the arena counters (vertices, faces allocated since the last
+// compaction) into pad[0], pad[1]
+__global__ void k_arena_sum(const unsigned long long *arena, MeshCounters *mc)
+{
+    const int lane = threadIdx.x;
+    const unsigned long long a = lane < (int)FF_ARENAS ? arena[lane * FF_ARENA_STRIDE] : 0ull;
+    uint32_t v = (uint32_t)a, f = (uint32_t)(a >> 32);
+    for (int o = 32; o > 0; o >>= 1) {
+        v += __shfl_xor(v, o);
+        f += __shfl_xor(f, o);
+    }
+    if (lane == 0) { mc->pad[0] = v; mc->pad[1] = f; }
+}
+
 __global__ void k_mesh_begin(MeshCounters *mc)
 {
     if (threadIdx.x == 0) {
@@ -1270,7 +1285,10 @@ static int update_meshes(ff_ctx *c)
     if (s->nCells[s->cur])
         k_hash_clear_cells<<<1024, 256, 0, c->stream>>>(s->d_cells[s->cur], s->nCells[s->cur], s->d_hkeys);
     k_hash_build<<<1024, 256, 0, c->stream>>>(s->d_cells[nxt], s->d_mc, s->cellCap, s->d_hkeys, s->d_hval, s->hmask);
-    if (c->timing) hipEventRecord(e1, c->stream);
+    if (c->timing) {
+        hipEventRecord(e1, c->stream);
+        k_arena_sum<<<1, 64, 0, c->stream>>>(s->d_arena, s->d_mc);
+    }
     MCHK(hipMemcpyAsync(s->h_mc, s->d_mc, sizeof(MeshCounters), hipMemcpyDeviceToHost, c->stream));
     MCHK(hipStreamSynchronize(c->stream));
     if (c->timing) {
@@ -1313,9 +1331,16 @@ static int update_meshes(ff_ctx *c)
     s->nCells[nxt] = std::min(h.nCells, s->cellCap);
     s->lastMeshStamp = c->stampCounter;
     c->phaseUnits[2] += h.nDirty;
-    // algorithmic bytes: 512 voxels x 14 B read per re-meshed cell-sized region
-    // is an upper bound; count the reads of the cells' samples and the writes
-    c->phaseBytes[2] += (double)h.nDirty * 0.0;
+    // algorithmic bytes: the 512 source voxels (14 B) of each re-meshed cell,
+    // 15 B per vertex and 12 B per triangle written (vertex and face counts
+    // from the arena totals, which only grow between compactions)
+    if (c->timing) {
+        const uint64_t va = h.pad[0], fa = h.pad[1];
+        const uint64_t dv = va >= s->lastVAlloc ? va - s->lastVAlloc : va, df = fa >= s->lastFAlloc ? fa - s->lastFAlloc : fa;
+        c->phaseBytes[2] += (double)h.nDirty * 512.0 * 14.0 + 15.0 * (double)dv + 12.0 * (double)df;
+        s->lastVAlloc = va;
+        s->lastFAlloc = fa;
+    }
     const uint32_t flags = h.flags, nDirty = h.nDirty;
     if (flags & (FF_MESH_OVERFLOW | FF_MESH_HALF)) {
         // compact live geometry into fresh pools (cell k to arena k mod FF_ARENAS)
@@ -1331,7 +1356,15 @@ static int update_meshes(ff_ctx *c)
         dst.v = nv;
         dst.f = nf;
         k_compact<<<1024, 256, 0, c->stream>>>(s->d_cells[nxt], s->d_mc, s->cellCap, s->d_vpool, s->d_fpool, dst);
+        if (c->timing) {
+            k_arena_sum<<<1, 64, 0, c->stream>>>(s->d_arena, s->d_mc);
+            MCHK(hipMemcpyAsync(s->h_mc, s->d_mc, sizeof(MeshCounters), hipMemcpyDeviceToHost, c->stream));
+        }
         MCHK(hipStreamSynchronize(c->stream));
+        if (c->timing) {
+            s->lastVAlloc = s->h_mc->pad[0];
+            s->lastFAlloc = s->h_mc->pad[1];
+        }
         hipFree(s->d_vpool);
         hipFree(s->d_fpool);
         s->d_vpool = nv;
