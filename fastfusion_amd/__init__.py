@@ -63,7 +63,24 @@ ABI = {
     "ff_set_timing": (ctypes.c_int, [_P, ctypes.c_int]),
     "ff_get_timing": (ctypes.c_int, [_P, _P, _P, _P, _P]),
     "ff_reset_timing": (ctypes.c_int, [_P]),
+    "ff_set_learned_weighting": (ctypes.c_int, [_P, _P, _SZ]),
+    "ff_copy_confidence": (ctypes.c_int, [_P, _P, _SZ]),
+    "ff_trainer_create": (_P, [ctypes.c_int, _SZ, _P, _SZ, ctypes.c_uint64]),
+    "ff_trainer_destroy": (None, [_P]),
+    "ff_trainer_last_error": (ctypes.c_char_p, [_P]),
+    "ff_trainer_collect": (ctypes.c_int, [_P, _P, _P, ctypes.c_int, ctypes.c_int, _DP, _DP, _DP,
+                                          ctypes.c_float, ctypes.c_float, ctypes.c_float]),
+    "ff_trainer_size": (ctypes.c_int, [_P, ctypes.POINTER(_SZ)]),
+    "ff_trainer_copy_samples": (ctypes.c_int, [_P, _P, _P, _SZ]),
+    "ff_trainer_set_samples": (ctypes.c_int, [_P, _P, _P, _SZ]),
+    "ff_trainer_train": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_uint32, ctypes.c_float,
+                                        ctypes.POINTER(ctypes.c_float)]),
+    "ff_trainer_grad": (ctypes.c_int, [_P, _P, ctypes.c_uint32, _P, ctypes.POINTER(ctypes.c_float)]),
+    "ff_trainer_params": (ctypes.c_int, [_P, _P, _SZ]),
 }
+
+MLP_NPARAMS = 10177       # FF_MLP_NPARAMS
+MLP_RES_SCALE = 0.0025    # FF_MLP_RES_SCALE
 
 
 def lib_path():
@@ -271,3 +288,95 @@ class FusionMipMapGPU:
         names = ("alloc", "sdf", "mesh")
         return {n: {"ms": float(ms[i]), "calls": int(calls[i]), "bytes": float(by[i]), "units": int(un[i])}
                 for i, n in enumerate(names)}
+
+    # -- learned confidence (no counterpart in the reference)
+    def setLearnedWeighting(self, params):
+        """Turns learned per-pixel confidence on (params: MLP_NPARAMS floats)
+        or off (None: the plain method)."""
+        if params is None:
+            self._check(self._lib.ff_set_learned_weighting(self._ctx, None, 0))
+            return
+        p = np.ascontiguousarray(params, np.float32).ravel()
+        self._check(self._lib.ff_set_learned_weighting(self._ctx, _ptr(p), p.size))
+
+    def confidence(self, w, h):
+        out = np.zeros(w * h, np.float32)
+        self._check(self._lib.ff_copy_confidence(self._ctx, _ptr(out), out.size))
+        return out.reshape(h, w)
+
+    @property
+    def handle(self):
+        return self._ctx
+
+
+class Trainer:
+    """Device trainer of the confidence network (ff_trainer_* in ff_abi.h)."""
+
+    def __init__(self, init_params, capacity=1 << 22, seed=1, device=0):
+        self._lib = load_library()
+        p = np.ascontiguousarray(init_params, np.float32).ravel()
+        if p.size != MLP_NPARAMS:
+            raise FFError(f"expected {MLP_NPARAMS} parameters")
+        self._h = self._lib.ff_trainer_create(int(device), int(capacity), _ptr(p), p.size, int(seed))
+        if not self._h:
+            raise FFError("ff_trainer_create failed")
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.ff_trainer_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc):
+        if rc < 0:
+            raise FFError(f"fastfusion_amd trainer error {rc}: "
+                          f"{self._lib.ff_trainer_last_error(self._h).decode(errors='replace')}")
+        return rc
+
+    def collect(self, fusion, depth, R, t, K, scaling=1.0 / 5000.0, maxcamdistance=10.0, keep=1.0):
+        depth = np.ascontiguousarray(depth, np.uint16)
+        h, w = depth.shape
+        return self._check(self._lib.ff_trainer_collect(self._h, fusion.handle, _ptr(depth), w, h,
+                                                        _dptr(np.asarray(R, np.float64).reshape(9)),
+                                                        _dptr(np.asarray(t, np.float64).reshape(3)),
+                                                        _dptr(np.asarray(K, np.float64).reshape(4)),
+                                                        float(scaling), float(maxcamdistance), float(keep)))
+
+    def size(self):
+        n = _SZ()
+        self._check(self._lib.ff_trainer_size(self._h, ctypes.byref(n)))
+        return n.value
+
+    def samples(self, n=None):
+        n = self.size() if n is None else n
+        feat = np.zeros((n, 32), np.uint16)
+        res = np.zeros(n, np.float32)
+        self._check(self._lib.ff_trainer_copy_samples(self._h, _ptr(feat), _ptr(res), n))
+        return feat, res
+
+    def set_samples(self, feat_bits, resid):
+        feat_bits = np.ascontiguousarray(feat_bits, np.uint16)
+        resid = np.ascontiguousarray(resid, np.float32)
+        self._check(self._lib.ff_trainer_set_samples(self._h, _ptr(feat_bits), _ptr(resid), len(resid)))
+
+    def train(self, steps, batch=65536, lr=1e-3):
+        loss = ctypes.c_float(0.0)
+        self._check(self._lib.ff_trainer_train(self._h, int(steps), int(batch), float(lr), ctypes.byref(loss)))
+        return loss.value
+
+    def grad(self, index):
+        index = np.ascontiguousarray(index, np.uint32)
+        g = np.zeros(MLP_NPARAMS, np.float32)
+        loss = ctypes.c_float(0.0)
+        self._check(self._lib.ff_trainer_grad(self._h, _ptr(index), len(index), _ptr(g), ctypes.byref(loss)))
+        return g, loss.value
+
+    def params(self):
+        out = np.zeros(MLP_NPARAMS, np.float32)
+        self._check(self._lib.ff_trainer_params(self._h, _ptr(out), out.size))
+        return out

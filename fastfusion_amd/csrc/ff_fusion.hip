@@ -709,7 +709,8 @@ __device__ __forceinline__ void sdf_brick(uint32_t leaf, const MapView &m, const
     const float front = (dI < EPS) ? 1.0f : 0.0f;
     const float back = (dI >= EPS && dI < tw) ? 1.0f : 0.0f;
     const float fall = rz_mul(rz_sub(tw, dI), rfall);
-    const float wI = mask ? rz_add(front, rz_mul(fall, back)) : 0.0f;
+    float wI = mask ? rz_add(front, rz_mul(fall, back)) : 0.0f;
+    if (P.conf) wI = rz_mul(wI, P.conf[ii]);  // learned per-pixel confidence
     const float wn = rz_add(wA, wI);
     const float f = rcp_x86(rz_add(wn, (wn <= 0.0f) ? 1.0f : 0.0f), tab);
     const float mn = (td < dI) ? td : dI;                 // _mm256_min_ps(td, dI)
@@ -1003,6 +1004,7 @@ void ff_destroy(ff_ctx *c)
     hipSetDevice(c->device);
     if (c->stream) hipStreamSynchronize(c->stream);
     mesh_free(c);
+    learned_free(c);
     c->freeFrameBuffers();
     hipFree(c->d_tree); hipFree(c->d_treeTmp); hipFree(c->d_leafinfo); hipFree(c->d_parent);
     hipFree(c->d_stamp); hipFree(c->d_queue); hipFree(c->d_dist); hipFree(c->d_wgt); hipFree(c->d_col);
@@ -1101,6 +1103,7 @@ static int add_map_impl(ff_ctx *c, const uint16_t *depth, const uint8_t *bgr, in
     sp.fx = q.fx; sp.fy = q.fy; sp.cx = q.cx; sp.cy = q.cy;
     sp.scale = c->scale; sp.dT = c->dT; sp.scaling = scaling; sp.maxcam = maxcam;
     sp.useColor = c->useColor ? 1 : 0;
+    sp.conf = nullptr;
     SdfGate gate;
     gate.listCap = c->hv.listCap;
     gate.leavesUsed = c->nLeavesUsed; gate.leavesTotal = c->nLeavesTotal;
@@ -1109,6 +1112,10 @@ static int add_map_impl(ff_ctx *c, const uint16_t *depth, const uint8_t *bgr, in
     gate.force = 0;
     if (c->n) {
         if (c->timing) hipEventRecord(c->ev[2], c->stream);
+        if (c->learned) {
+            sp.conf = learned_confidence(c, depth, W, H, K, scaling, maxcam);
+            if (!sp.conf) return FF_E_DEVICE;
+        }
         k_sdf<<<FF_SDF_GRID, 512, 0, c->stream>>>(c->d_queue, c->mapView(), depth, bgr, W, H, sp, c->d_fc, gate);
         if (c->timing) hipEventRecord(c->ev[3], c->stream);
         HIPCHK(hipGetLastError());

@@ -64,9 +64,11 @@ struct SdfParams {
     float m[12];
     float fx, fy, cx, cy, scale, dT, scaling, maxcam;
     int useColor;
+    const float *conf;  // per-pixel confidence multiplying the weight, or nullptr
 };
 
 struct MeshState;
+struct LearnedState;
 
 }  // namespace ff
 
@@ -100,6 +102,7 @@ struct ff_ctx {
     ff::FrameCounters *d_fc = nullptr, *h_fc = nullptr;
     ff::HashView hv{};
     ff::MeshState *mesh = nullptr;
+    ff::LearnedState *learned = nullptr;  // learned confidence (off: nullptr)
     std::string err;
     // per-phase timing (0 allocation, 1 SDF update, 2 meshing)
     bool timing = false;
@@ -119,6 +122,10 @@ struct ff_ctx {
 };
 
 namespace ff {
+// learned confidence (ff_learned.hip)
+void learned_free(ff_ctx *c);
+const float *learned_confidence(ff_ctx *c, const uint16_t *depth, int W, int H, const double K[4], float scaling,
+                                float maxcam);
 // meshing module hooks (ff_mesh.hip)
 int mesh_init(ff_ctx *c);
 void mesh_free(ff_ctx *c);

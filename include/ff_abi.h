@@ -133,6 +133,52 @@ int ff_reset_timing(ff_ctx *ctx);
  * time or order work around it. */
 void *ff_stream(ff_ctx *ctx);
 
+/* ---- Learned per-pixel confidence -------------------------------------
+ * No counterpart in the reference, whose per-sample weight is the fixed
+ * linear fall-off of getDistanceWeight_AoS_variables_h
+ * (inline_functions.hpp:83-124, update_functions.cpp:1355-1403).  With
+ * learned weighting on, that weight is multiplied by c(pixel) in (0, 1) from
+ * an MLP over 27 per-pixel features (5x5 depth patch relative to the centre
+ * depth, depth / 4 m, incidence cosine): 27 -> 64 -> 64 -> 64 -> 1, ReLU,
+ * sigmoid, BF16 weights and activations with FP32 accumulation.
+ * Parameters are FP32, FF_MLP_NPARAMS of them, laid out as W1[64][27], b1[64],
+ * W2[64][64], b2[64], W3[64][64], b3[64], w4[64], b4 (row = output unit). */
+#define FF_MLP_NPARAMS 10177
+#define FF_MLP_RES_SCALE 0.0025f /* s of the training loss c r^2 / (2 s^2) - log(c) / 2, metres */
+
+/* Turns learned weighting on with the given parameters, or off (params NULL:
+ * the plain method, bit-identical to the reference's arithmetic). */
+int ff_set_learned_weighting(ff_ctx *ctx, const float *params, size_t n_params);
+/* The last frame's per-pixel confidence (W x H floats, 0 at invalid depth). */
+int ff_copy_confidence(ff_ctx *ctx, float *out, size_t n);
+
+/* Trainer of the confidence network on the device: samples are the features
+ * of valid pixels with the residual r = TSDF of a converged map at the
+ * pixel's back-projected point (clamped to the truncation; the truncation
+ * where the map has no weight).  Training is Adam on the mean loss over
+ * batches drawn uniformly from the samples. */
+typedef struct ff_trainer ff_trainer;
+ff_trainer *ff_trainer_create(int device, size_t capacity, const float *init_params, size_t n_params,
+                              uint64_t seed);
+void ff_trainer_destroy(ff_trainer *tr);
+const char *ff_trainer_last_error(const ff_trainer *tr);
+/* Adds samples from one frame against the converged map `map` (host depth,
+ * dense rows; pose camera -> world); keeps each valid pixel with probability
+ * keep.  Returns the number of samples added. */
+int ff_trainer_collect(ff_trainer *tr, ff_ctx *map, const uint16_t *depth, int w, int h, const double R[9],
+                       const double t[3], const double K[4], float scaling, float maxcam, float keep);
+int ff_trainer_size(const ff_trainer *tr, size_t *n);
+/* Samples as 32 BF16 features (bit patterns; 27 used) and a residual each. */
+int ff_trainer_copy_samples(ff_trainer *tr, uint16_t *feat, float *resid, size_t n);
+int ff_trainer_set_samples(ff_trainer *tr, const uint16_t *feat, const float *resid, size_t n);
+/* steps of Adam (lr, beta 0.9 / 0.999) on batches of `batch` samples
+ * (a multiple of 2048); last_loss gets the mean loss of the last batch. */
+int ff_trainer_train(ff_trainer *tr, int steps, uint32_t batch, float lr, float *last_loss);
+/* Gradient of the mean loss over the given samples (n a multiple of 2048),
+ * without an update: the check of the training kernels. */
+int ff_trainer_grad(ff_trainer *tr, const uint32_t *index, uint32_t n, float *grad, float *loss);
+int ff_trainer_params(ff_trainer *tr, float *out, size_t n_params);
+
 #ifdef __cplusplus
 }
 #endif

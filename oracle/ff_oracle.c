@@ -42,6 +42,7 @@ struct ffo_ctx {
     float *pxp, *pyp;
     uint8_t *processed;
     int grows;
+    const float *conf; /* per-pixel confidence for the next frames, or NULL */
 };
 
 static uint32_t f2u(float f) { uint32_t u; memcpy(&u, &f, 4); return u; }
@@ -370,7 +371,7 @@ static int cvt_rz(float v)
 static void update_brick_rz(const uint16_t *depth, float scaling, float maxcam, const uint8_t *rgb,
                             int W, int H, const float *m, float fx, float fy, float cx, float cy,
                             float scale, float dT, const uint16_t *o, uint16_t leafScale,
-                            float *dist, float *wgt, uint16_t *col)
+                            float *dist, float *wgt, uint16_t *col, const float *conf)
 {
     volatile float vs = scale; /* keep the compiler from folding across fesetround */
     float s = vs;
@@ -414,6 +415,8 @@ static void update_brick_rz(const uint16_t *depth, float scaling, float maxcam, 
                 float back = (dI >= EPS_W && dI < tw) ? 1.0f : 0.0f;
                 float fall = (tw - dI) * rfall;
                 float wI = mask ? (front + fall * back) : 0.0f;
+                /* learned confidence: a factor on the reference's weight */
+                if (conf) wI = wI * conf[ii];
                 float wN = wA + wI;
                 float f = ffo_rcp(wN + ((wN <= 0.0f) ? 1.0f : 0.0f));
                 float mn = (td < dI) ? td : dI;         /* _mm256_min_ps(td, dI) */
@@ -444,7 +447,7 @@ void ffo_update_brick(const uint16_t *depth, float scaling, float maxcam, const 
     int old = fegetround();
     fesetround(FE_TOWARDZERO);
     update_brick_rz(depth, scaling, maxcam, bgr, w, h, m, K[0], K[1], K[2], K[3], scale,
-                    distance_threshold, pos, leaf_scale, dist, weight, color);
+                    distance_threshold, pos, leaf_scale, dist, weight, color, NULL);
     fesetround(old);
 }
 
@@ -506,13 +509,17 @@ int ffo_add_map(ffo_ctx *c, const uint16_t *depth, const uint8_t *bgr, int w, in
         uint32_t b = c->queue[l];
         update_brick_rz(depth, scaling, maxcam, bgr, w, h, mm, q.fx, q.fy, q.cx, q.cy, c->scale, c->dT,
                         c->leafPos + (size_t)b * 3, c->leafScale[b], c->dist + (size_t)b * BRICK,
-                        c->wgt + (size_t)b * BRICK, c->use_color ? c->col + (size_t)b * BRICK * 3 : NULL);
+                        c->wgt + (size_t)b * BRICK, c->use_color ? c->col + (size_t)b * BRICK * 3 : NULL, c->conf);
     }
     fesetround(old);
     return (int)c->nQueued;
 }
 
 uint32_t ffo_n_leaves(const ffo_ctx *c) { return c->nLeavesUsed; }
+
+/* Per-pixel confidence multiplying the weight of the following frames
+ * (W x H floats that stay valid while used), or NULL for the plain method. */
+void ffo_set_confidence(ffo_ctx *c, const float *conf) { c->conf = conf; }
 uint32_t ffo_n_branches(const ffo_ctx *c) { return c->nBranchesUsed; }
 uint32_t ffo_side(const ffo_ctx *c) { return c->n; }
 void ffo_offset(const ffo_ctx *c, float out[3]) { memcpy(out, c->offset, sizeof(float) * 3); }

@@ -38,6 +38,11 @@ int ffo_add_map(ffo_ctx *c, const uint16_t *depth, const uint8_t *bgr, int w, in
                 const double R[9], const double t[3], const double K[4],
                 float scaling, float maxcam);
 
+/* Learned confidence (no counterpart in the reference): the weight of every
+ * sample of the following frames is multiplied by conf[pixel] (W x H floats,
+ * kept by the caller); NULL restores the plain method. */
+void ffo_set_confidence(ffo_ctx *c, const float *conf);
+
 /* state accessors */
 uint32_t ffo_n_leaves(const ffo_ctx *c);
 uint32_t ffo_n_branches(const ffo_ctx *c);

@@ -44,6 +44,7 @@ def load():
     lib.ffo_dedup.argtypes = [P, P, P, P, ctypes.c_int, ctypes.c_int, P]
     lib.ffo_update_brick.argtypes = [P, ctypes.c_float, ctypes.c_float, P, ctypes.c_int, ctypes.c_int, P, P,
                                      ctypes.c_float, ctypes.c_float, P, ctypes.c_uint16, P, P, P]
+    lib.ffo_set_confidence.argtypes = [P, P]
     _mesh_sigs(lib)
     _lib = lib
     return lib
@@ -92,12 +93,18 @@ class Oracle:
         except Exception:
             pass
 
-    def add_map(self, depth, R, t, K, bgr=None, scaling=1.0 / 5000.0, maxcam=10.0):
+    def add_map(self, depth, R, t, K, bgr=None, scaling=1.0 / 5000.0, maxcam=10.0, conf=None):
+        """One frame; conf: optional H x W per-pixel confidence multiplying the weight."""
         depth = np.ascontiguousarray(depth, np.uint16)
         h, w = depth.shape
         bgr = np.ascontiguousarray(bgr, np.uint8) if bgr is not None else np.zeros((h, w, 3), np.uint8)
+        cf = None
+        if conf is not None:
+            cf = np.ascontiguousarray(conf, np.float32).reshape(h, w)
+        self.lib.ffo_set_confidence(self.h, _vp(cf) if cf is not None else None)
         nq = self.lib.ffo_add_map(self.h, _vp(depth), _vp(bgr), w, h, _dp(R), _dp(t), _dp(K),
                                   float(scaling), float(maxcam))
+        self.lib.ffo_set_confidence(self.h, None)
         self.lib.ffo_mesh_note_frame(self.h)
         return nq
 
