@@ -64,6 +64,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--learned", action="store_true",
+                    help="learned per-pixel confidence on (committed weights): the throughput-cost run")
     ap.add_argument("--calib-copy", type=int, default=0,
                     help="MiB copied device-to-device once before the run (HBM counter calibration)")
     args = ap.parse_args()
@@ -92,6 +94,9 @@ def main():
     d_bgr = hiprt.DeviceBuffer.from_array(bgr)
     fz = FusionMipMapGPU(0, 0, 0, cfg["scale"], cfg["threshold"], 0, True, max_leaves=1 << 21,
                          max_branches=1 << 25)
+    if args.learned:
+        from fastfusion_amd import learned
+        fz.setLearnedWeighting(learned.load_weights())
     fz.set_timing(True)
     K = synth.K_DEFAULT
     fb, cb = w * h * 2, w * h * 3
@@ -145,7 +150,8 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic: seeded analytic room ray-cast on the host, u16 depth + BGR8, resident in HBM",
-        "config": {"workload": "plain method, 640x480, 5 mm voxels, 15 mm truncation, 4 m max depth, "
+        "config": {"workload": ("learned confidence" if args.learned else "plain method") +
+                               ", 640x480, 5 mm voxels, 15 mm truncation, 4 m max depth, "
                                "1000-frame seeded random walk (room seed 11, walk seed 12)",
                    "frames": n_frames, "resolution": [w, h], "voxel_m": cfg["scale"],
                    "truncation_m": cfg["threshold"], "maxcam_m": cfg["maxcam"]},

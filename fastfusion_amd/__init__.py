@@ -81,6 +81,7 @@ ABI = {
 
 MLP_NPARAMS = 10177       # FF_MLP_NPARAMS
 MLP_RES_SCALE = 0.0025    # FF_MLP_RES_SCALE
+MLP_GATE = 0.002          # FF_MLP_GATE
 
 
 def lib_path():

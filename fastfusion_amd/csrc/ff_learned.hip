@@ -15,6 +15,10 @@
 // Training minimises, per sample with residual r to the converged surface,
 //   l = c r^2 / (2 s^2) - log(c) / 2        (s = FF_MLP_RES_SCALE)
 // whose minimiser is c = s^2 / E[r^2 | features]: inverse-variance weighting.
+// At inference a confidence under FF_MLP_GATE (an expected residual above
+// ~56 mm, beyond the widest truncation band) rejects the pixel: a lone
+// outlier, such as a flying pixel in an occluded region, would otherwise
+// leave a surface that no later observation competes with.
 //
 // MFMA data layout.  A wave holds 16 pixels (samples) along N; hidden units
 // run along M.  The result of a 16x16x32 MFMA sits with lane l holding
@@ -193,6 +197,7 @@ __device__ __forceinline__ void mlp_forward(const MlpW &W, const bf16x8 &bx, con
 // ---------------------------------------------------------------------------
 struct FeatCam {
     float fx, fy, cx, cy, scaling, maxcam;
+    float ifx, ify;  // 1 / fx, 1 / fy (float division on the host)
 };
 
 __device__ __forceinline__ float depth_at(const uint16_t *__restrict__ depth, int W, int H, int x, int y,
@@ -202,16 +207,21 @@ __device__ __forceinline__ float depth_at(const uint16_t *__restrict__ depth, in
     return __fmul_rn((float)depth[(size_t)y * W + x], scaling);
 }
 
-__device__ float incidence_cos(const uint16_t *__restrict__ depth, int W, int H, int x, int y, float zc,
-                               const FeatCam &K)
+// patch entry: z_k / z_c - 1 with rc = 1 / z_c, clamped to [-1, 1]
+__device__ __forceinline__ float patch_feature(float zk, float rc)
 {
-    const float zl = depth_at(depth, W, H, x - 1, y, K.scaling), zr = depth_at(depth, W, H, x + 1, y, K.scaling);
-    const float zu = depth_at(depth, W, H, x, y - 1, K.scaling), zd = depth_at(depth, W, H, x, y + 1, K.scaling);
+    return fminf(fmaxf(__fsub_rn(__fmul_rn(zk, rc), 1.0f), -1.0f), 1.0f);
+}
+
+// |cos| between the viewing ray of (x, y) and the normal from the central
+// differences of the four neighbours' back-projections (0 if one is missing)
+__device__ float incidence_cos(float zl, float zr, float zu, float zd, float zc, int x, int y, const FeatCam &K)
+{
     if (!(zl > 0.0f && zr > 0.0f && zu > 0.0f && zd > 0.0f)) return 0.0f;
-    const float ul = __fdiv_rn((float)(x - 1) - K.cx, K.fx), ur = __fdiv_rn((float)(x + 1) - K.cx, K.fx);
-    const float uc = __fdiv_rn((float)x - K.cx, K.fx);
-    const float vu = __fdiv_rn((float)(y - 1) - K.cy, K.fy), vd = __fdiv_rn((float)(y + 1) - K.cy, K.fy);
-    const float vc = __fdiv_rn((float)y - K.cy, K.fy);
+    const float ul = __fmul_rn((float)(x - 1) - K.cx, K.ifx), ur = __fmul_rn((float)(x + 1) - K.cx, K.ifx);
+    const float uc = __fmul_rn((float)x - K.cx, K.ifx);
+    const float vu = __fmul_rn((float)(y - 1) - K.cy, K.ify), vd = __fmul_rn((float)(y + 1) - K.cy, K.ify);
+    const float vc = __fmul_rn((float)y - K.cy, K.ify);
     // tx = P(x+1) - P(x-1), ty = P(y+1) - P(y-1)
     const float tx0 = __fsub_rn(__fmul_rn(ur, zr), __fmul_rn(ul, zl));
     const float tx1 = __fsub_rn(__fmul_rn(vc, zr), __fmul_rn(vc, zl));
@@ -230,47 +240,74 @@ __device__ float incidence_cos(const uint16_t *__restrict__ depth, int W, int H,
     return __fdiv_rn(fabsf(dot), __fsqrt_rn(__fmul_rn(nn, rr)));
 }
 
-__device__ __forceinline__ float feature(const uint16_t *__restrict__ depth, int W, int H, int x, int y, float zc,
-                                         int k, const FeatCam &K)
+// feature k of pixel (x, y) read from the image (training-data path)
+__device__ float feature(const uint16_t *__restrict__ depth, int W, int H, int x, int y, float zc, int k,
+                         const FeatCam &K)
 {
-    if (k < 25) {
-        const float zk = depth_at(depth, W, H, x + k % 5 - 2, y + k / 5 - 2, K.scaling);
-        const float f = __fsub_rn(__fdiv_rn(zk, zc), 1.0f);
-        return fminf(fmaxf(f, -1.0f), 1.0f);
-    }
+    if (k < 25) return patch_feature(depth_at(depth, W, H, x + k % 5 - 2, y + k / 5 - 2, K.scaling), __frcp_rn(zc));
     if (k == 25) return __fmul_rn(zc, 0.25f);
-    if (k == 26) return incidence_cos(depth, W, H, x, y, zc, K);
+    if (k == 26)
+        return incidence_cos(depth_at(depth, W, H, x - 1, y, K.scaling), depth_at(depth, W, H, x + 1, y, K.scaling),
+                             depth_at(depth, W, H, x, y - 1, K.scaling), depth_at(depth, W, H, x, y + 1, K.scaling), zc,
+                             x, y, K);
     return 0.0f;
 }
 
 // ---------------------------------------------------------------------------
 // inference: per-pixel confidence of a frame (0 where the depth is invalid)
+// A wave takes 16 pixels of a row; the 5 x 20 depth window they need is
+// staged in the wave's LDS slice, every lane of a pixel (one per 8-feature
+// group) computes the same incidence, and feature selection is branch-free.
 // ---------------------------------------------------------------------------
+#define CONF_WIN_W 20
+#define CONF_WIN 100
+
 __global__ __launch_bounds__(256) void k_confidence(const uint16_t *__restrict__ depth, int W, int H, FeatCam K,
                                                     const bf16x8 *__restrict__ frag, const float *__restrict__ bias,
                                                     const float *__restrict__ w4, float *__restrict__ conf)
 {
     __shared__ float sb[3 * 64 + 1], sw4[64];
+    __shared__ float win[4][CONF_WIN + 4];
     for (int i = threadIdx.x; i < 3 * 64 + 1; i += blockDim.x) sb[i] = bias[i];
     for (int i = threadIdx.x; i < 64; i += blockDim.x) sw4[i] = w4[i];
     __syncthreads();
-    const int lane = threadIdx.x & 63, g = lane >> 4;
+    const int lane = threadIdx.x & 63, g = lane >> 4, p = lane & 15;
+    float *zt = win[threadIdx.x >> 6];
     MlpW Wt;
     load_forward(Wt, frag, lane);
     const int tilesPerRow = (W + 15) / 16;
     const int nTiles = tilesPerRow * H;
     const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6, nWaves = (gridDim.x * blockDim.x) >> 6;
     for (int tile = wave; tile < nTiles; tile += nWaves) {
-        const int y = tile / tilesPerRow, x = (tile % tilesPerRow) * 16 + (lane & 15);
+        const int y = tile / tilesPerRow, x0 = (tile % tilesPerRow) * 16, x = x0 + p;
+        // window rows y-2..y+2, columns x0-2..x0+17
+        for (int i = lane; i < CONF_WIN; i += 64)
+            zt[i] = depth_at(depth, W, H, x0 - 2 + i % CONF_WIN_W, y - 2 + i / CONF_WIN_W, K.scaling);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         const bool inImg = x < W;
-        const float zc = inImg ? depth_at(depth, W, H, x, y, K.scaling) : 0.0f;
-        const bool valid = zc > 0.0f && zc < K.maxcam;
+        const float zc = zt[2 * CONF_WIN_W + p + 2];
+        const bool valid = inImg && zc > 0.0f && zc < K.maxcam;
+        const float rc = __frcp_rn(valid ? zc : 1.0f);
+        const float inc = incidence_cos(zt[2 * CONF_WIN_W + p + 1], zt[2 * CONF_WIN_W + p + 3],
+                                        zt[CONF_WIN_W + p + 2], zt[3 * CONF_WIN_W + p + 2], zc, x, y, K);
         bf16x8 bx;
 #pragma unroll
-        for (int q = 0; q < 8; q++) bx[q] = (__bf16)(valid ? feature(depth, W, H, x, y, zc, 8 * g + q, K) : 0.0f);
+        for (int q = 0; q < 8; q++) {
+            const int k = 8 * g + q, kk = k < 25 ? k : 0;
+            const float fp = patch_feature(zt[(kk / 5) * CONF_WIN_W + p + kk % 5], rc);
+            const float f = k < 25 ? fp : (k == 25 ? __fmul_rn(zc, 0.25f) : (k == 26 ? inc : 0.0f));
+            bx[q] = (__bf16)(valid ? f : 0.0f);
+        }
         Fwd F;
         mlp_forward(Wt, bx, sb, sw4, lane, F);
-        if (g == 0 && inImg) conf[(size_t)y * W + x] = valid ? F.c : 0.0f;
+        // a pixel whose predicted residual is beyond every truncation band
+        // (confidence under the gate) is not fused at all
+        if (g == 0 && inImg) conf[(size_t)y * W + x] = (valid && F.c >= FF_MLP_GATE) ? F.c : 0.0f;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
 }
 
@@ -286,8 +323,9 @@ struct MapRead {
     float off[3], scale, trunc;
 };
 
-// trilinear TSDF at voxel coordinates v of the finest brick with weight there
-__device__ bool tsdf_at(const MapRead &M, const float v[3], float &d)
+// trilinear TSDF at voxel coordinates v of the finest brick with weight
+// there, and that brick's truncation (distance threshold x leaf scale)
+__device__ bool tsdf_at(const MapRead &M, const float v[3], float &d, float &td)
 {
     if (!(v[0] >= 0.0f && v[1] >= 0.0f && v[2] >= 0.0f && v[0] < (float)M.n && v[1] < (float)M.n &&
           v[2] < (float)M.n))
@@ -334,6 +372,7 @@ __device__ bool tsdf_at(const MapRead &M, const float v[3], float &d)
         }
         if (ok) {
             d = acc / wsum;
+            td = M.trunc * ls;
             return true;
         }
     }
@@ -373,8 +412,14 @@ __global__ __launch_bounds__(256) void k_collect(const uint16_t *__restrict__ de
                 const float wv = pw.r[3 * k] * cxm + pw.r[3 * k + 1] * cym + pw.r[3 * k + 2] * zc + pw.t[k];
                 v[k] = (wv - M.off[k]) / M.scale;
             }
-            float d;
-            r = tsdf_at(M, v, d) ? fminf(fmaxf(d, -M.trunc), M.trunc) : M.trunc;
+            // residual to the converged surface: the TSDF inside its band; a
+            // point the band does not reach (saturated, or never observed)
+            // is an outlier at FF_MLP_RES_OUT
+            float d, td;
+            if (tsdf_at(M, v, d, td))
+                r = fabsf(d) < 0.9f * td ? d : (d < 0.0f ? -FF_MLP_RES_OUT : FF_MLP_RES_OUT);
+            else
+                r = FF_MLP_RES_OUT;
         }
     }
     const uint64_t m = __ballot(take);
@@ -623,7 +668,8 @@ const float *learned_confidence(ff_ctx *c, const uint16_t *depth, int W, int H, 
         if (!lok(c, hipMalloc(&L->conf, sizeof(float) * np), "hipMalloc(confidence)")) return nullptr;
         L->confCap = np;
     }
-    FeatCam fk{(float)K[0], (float)K[1], (float)K[2], (float)K[3], scaling, maxcam};
+    FeatCam fk{(float)K[0], (float)K[1], (float)K[2], (float)K[3], scaling, maxcam, 1.0f / (float)K[0],
+               1.0f / (float)K[1]};
     const int tiles = (W + 15) / 16 * H;
     const int blocks = std::min(2048, (tiles + 3) / 4);
     k_confidence<<<blocks, 256, 0, c->stream>>>(depth, W, H, fk, L->w.frag, L->w.bias, L->w.w4, L->conf);
@@ -834,7 +880,8 @@ int ff_trainer_collect(ff_trainer *tr, ff_ctx *map, const uint16_t *depth, int W
     if (e == hipSuccess) e = hipMemcpyAsync(&before, tr->count, 4, hipMemcpyDeviceToHost, tr->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(tr->stream);
     if (e == hipSuccess) {
-        FeatCam fk{(float)K[0], (float)K[1], (float)K[2], (float)K[3], scaling, maxcam};
+        FeatCam fk{(float)K[0], (float)K[1], (float)K[2], (float)K[3], scaling, maxcam, 1.0f / (float)K[0],
+                   1.0f / (float)K[1]};
         CamF pw;
         for (int i = 0; i < 9; i++) pw.r[i] = (float)R[i];
         for (int i = 0; i < 3; i++) pw.t[i] = (float)t[i];

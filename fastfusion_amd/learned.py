@@ -78,14 +78,17 @@ def fuse(sc, n, seed, params=None, cfg=CFG, max_leaves=1 << 20, keep_frames=None
     return fz, kept
 
 
-def rms_error(fz, sc):
+def mesh_errors(fz, sc):
+    """Analytic scene SDF at the vertices of the map's full mesh."""
     fz.setIncrementalMeshing(True)
     fz.updateMeshes()
     m = fz.getMeshInterleavedMarchingCubes()
-    if len(m.vertices) == 0:
-        return float("nan"), 0
-    e = sc.sdf(m.vertices).astype(np.float64)
-    return float(np.sqrt(np.mean(e * e))), len(m.vertices)
+    return sc.sdf(m.vertices).astype(np.float64) if len(m.vertices) else np.zeros(0)
+
+
+def rms_error(fz, sc):
+    e = mesh_errors(fz, sc)
+    return (float(np.sqrt(np.mean(e * e))) if len(e) else float("nan")), len(e)
 
 
 def collect(trainer, seeds, n_frames, every, keep, cfg=CFG, log=print):
@@ -128,14 +131,22 @@ def evaluate(params, seeds, n_frames, cfg=CFG, log=print):
         out = {"seed": s}
         for name, p in (("plain", None), ("learned", params)):
             fz, _ = fuse(sc, n_frames, s, p, cfg)
-            out[name], out[name + "_vertices"] = rms_error(fz, sc)
+            e = mesh_errors(fz, sc)
             fz.close()
+            out[name] = float(np.sqrt(np.mean(e * e))) if len(e) else float("nan")
+            out[name + "_vertices"] = int(len(e))
+            # accuracy of the vertices within 2 cm of a surface, and the share beyond 5 cm
+            inl = np.abs(e) < 0.02
+            out[name + "_rms_within_2cm"] = float(np.sqrt(np.mean(e[inl] ** 2))) if inl.any() else float("nan")
+            out[name + "_frac_beyond_5cm"] = float(np.mean(np.abs(e) > 0.05)) if len(e) else 0.0
         rows.append(out)
         log(f"[learned] room {s}: plain {out['plain'] * 1e3:.3f} mm, learned {out['learned'] * 1e3:.3f} mm")
     plain = float(np.sqrt(np.mean([r["plain"] ** 2 for r in rows])))
     learned = float(np.sqrt(np.mean([r["learned"] ** 2 for r in rows])))
+    vp = sum(r["plain_vertices"] for r in rows)
+    vl = sum(r["learned_vertices"] for r in rows)
     return {"rooms": rows, "rms_plain_m": plain, "rms_learned_m": learned,
-            "reduction": 1.0 - learned / plain}
+            "reduction": 1.0 - learned / plain, "vertex_ratio": vl / max(vp, 1)}
 
 
 def main(argv=None):
@@ -151,7 +162,7 @@ def main(argv=None):
     a.add_argument("--capacity", type=int, default=1 << 23)
     a.add_argument("--steps", type=int, default=20000)
     a.add_argument("--batch", type=int, default=65536)
-    a.add_argument("--lr", type=float, default=1e-3)
+    a.add_argument("--lr", type=float, default=3e-3)
     a.add_argument("--seed", type=int, default=1)
     e = sub.add_parser("eval")
     e.add_argument("--weights", default=WEIGHTS)

@@ -145,17 +145,21 @@ void *ff_stream(ff_ctx *ctx);
  * W2[64][64], b2[64], W3[64][64], b3[64], w4[64], b4 (row = output unit). */
 #define FF_MLP_NPARAMS 10177
 #define FF_MLP_RES_SCALE 0.0025f /* s of the training loss c r^2 / (2 s^2) - log(c) / 2, metres */
+#define FF_MLP_RES_OUT 0.3f      /* residual of a sample outside the converged map's band, metres */
+#define FF_MLP_GATE 0.002f       /* confidences below this reject the pixel (weight 0) */
 
 /* Turns learned weighting on with the given parameters, or off (params NULL:
  * the plain method, bit-identical to the reference's arithmetic). */
 int ff_set_learned_weighting(ff_ctx *ctx, const float *params, size_t n_params);
-/* The last frame's per-pixel confidence (W x H floats, 0 at invalid depth). */
+/* The last frame's per-pixel confidence (W x H floats; 0 at invalid depth
+ * and where the confidence is under FF_MLP_GATE). */
 int ff_copy_confidence(ff_ctx *ctx, float *out, size_t n);
 
 /* Trainer of the confidence network on the device: samples are the features
  * of valid pixels with the residual r = TSDF of a converged map at the
- * pixel's back-projected point (clamped to the truncation; the truncation
- * where the map has no weight).  Training is Adam on the mean loss over
+ * pixel's back-projected point when that lies inside the map's truncation
+ * band (|TSDF| < 0.9 x the brick's truncation), else +-FF_MLP_RES_OUT (the
+ * sign of the saturated TSDF; + where the map has no weight).  Training is Adam on the mean loss over
  * batches drawn uniformly from the samples. */
 typedef struct ff_trainer ff_trainer;
 ff_trainer *ff_trainer_create(int device, size_t capacity, const float *init_params, size_t n_params,

@@ -7,8 +7,8 @@ its weight is the fixed linear fall-off of getDistanceWeight_AoS_variables_h
 
 Three restatements:
   * features(): the 27 per-pixel inputs, float32 in the kernels' order of
-    operations (5x5 patch z_k / z_c - 1 clamped to [-1, 1], z_c / 4,
-    incidence cosine from central differences);
+    operations (5x5 patch z_k * (1 / z_c) - 1 clamped to [-1, 1], z_c / 4,
+    incidence cosine from central differences with u = (x - cx) * (1 / fx));
   * forward_bf16() / grads_bf16(): the network and the gradient of the mean
     loss with the kernels' BF16 rounding points (weights, activations and
     back-propagated deltas rounded to BF16, sums in float32);
@@ -20,6 +20,7 @@ import numpy as np
 
 NPARAMS = 10177
 RES_SCALE = 0.0025
+GATE = 0.002  # inference: confidences below it are 0
 NIN, NH = 27, 64
 OFF = {"W1": 0, "b1": 64 * 27, "W2": 64 * 27 + 64, "b2": 64 * 27 + 64 + 4096,
        "W3": 64 * 27 + 128 + 4096, "b3": 64 * 27 + 128 + 8192, "w4": 64 * 27 + 192 + 8192,
@@ -71,26 +72,28 @@ def features(depth, K, scaling, maxcam):
     depth = np.asarray(depth, np.uint16)
     H, W = depth.shape
     fx, fy, cx, cy = (np.float32(k) for k in K)
+    ifx, ify = np.float32(1) / fx, np.float32(1) / fy
     z = depth.astype(np.float32) * np.float32(scaling)
     zp = np.zeros((H + 4, W + 4), np.float32)
     zp[2:H + 2, 2:W + 2] = z
     valid = (z > 0) & (z < np.float32(maxcam))
     zc = np.where(valid, z, np.float32(1.0))
+    rc = np.float32(1.0) / zc
     F = np.zeros((H, W, 32), np.float32)
     with np.errstate(divide="ignore", invalid="ignore"):
         for k in range(25):
             dy, dx = k // 5 - 2, k % 5 - 2
             zk = zp[2 + dy:2 + dy + H, 2 + dx:2 + dx + W]
-            F[:, :, k] = np.clip(zk / zc - np.float32(1.0), np.float32(-1.0), np.float32(1.0))
+            F[:, :, k] = np.clip(zk * rc - np.float32(1.0), np.float32(-1.0), np.float32(1.0))
         F[:, :, 25] = z * np.float32(0.25)
         xs = np.arange(W, dtype=np.float32)
         ys = np.arange(H, dtype=np.float32)
-        uc = ((xs - cx) / fx)[None, :]
-        ul = ((xs - np.float32(1) - cx) / fx)[None, :]
-        ur = ((xs + np.float32(1) - cx) / fx)[None, :]
-        vc = ((ys - cy) / fy)[:, None]
-        vu = ((ys - np.float32(1) - cy) / fy)[:, None]
-        vd = ((ys + np.float32(1) - cy) / fy)[:, None]
+        uc = ((xs - cx) * ifx)[None, :]
+        ul = ((xs - np.float32(1) - cx) * ifx)[None, :]
+        ur = ((xs + np.float32(1) - cx) * ifx)[None, :]
+        vc = ((ys - cy) * ify)[:, None]
+        vu = ((ys - np.float32(1) - cy) * ify)[:, None]
+        vd = ((ys + np.float32(1) - cy) * ify)[:, None]
         zl = zp[2:H + 2, 1:W + 1]
         zr = zp[2:H + 2, 3:W + 3]
         zu = zp[1:H + 1, 2:W + 2]
