@@ -64,6 +64,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=50)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--graphs", action="store_true", help="replay each frame from captured HIP graphs")
     ap.add_argument("--learned", action="store_true",
                     help="learned per-pixel confidence on (committed weights): the throughput-cost run")
     ap.add_argument("--calib-copy", type=int, default=0,
@@ -94,6 +95,8 @@ def main():
     d_bgr = hiprt.DeviceBuffer.from_array(bgr)
     fz = FusionMipMapGPU(0, 0, 0, cfg["scale"], cfg["threshold"], 0, True, max_leaves=1 << 21,
                          max_branches=1 << 25)
+    if args.graphs:
+        fz.setGraphCapture(True)
     if args.learned:
         from fastfusion_amd import learned
         fz.setLearnedWeighting(learned.load_weights())

@@ -60,6 +60,7 @@ ABI = {
     "ff_copy_queue": (ctypes.c_int, [_P, _P, _SZ]),
     "ff_copy_processed": (ctypes.c_int, [_P, _P, _SZ]),
     "ff_stream": (_P, [_P]),
+    "ff_set_graph_capture": (ctypes.c_int, [_P, ctypes.c_int]),
     "ff_set_timing": (ctypes.c_int, [_P, ctypes.c_int]),
     "ff_get_timing": (ctypes.c_int, [_P, _P, _P, _P, _P]),
     "ff_reset_timing": (ctypes.c_int, [_P]),
@@ -272,6 +273,10 @@ class FusionMipMapGPU:
 
     def stream(self):
         return self._lib.ff_stream(self._ctx)
+
+    def setGraphCapture(self, on=True):
+        """Replay each frame's launches from captured HIP graphs, or launch directly (default)."""
+        self._check(self._lib.ff_set_graph_capture(self._ctx, int(bool(on))))
 
     def set_timing(self, on=True):
         self._check(self._lib.ff_set_timing(self._ctx, int(bool(on))))

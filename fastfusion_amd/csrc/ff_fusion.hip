@@ -144,10 +144,13 @@ __device__ __forceinline__ void voxel_of(const float *__restrict__ qx, const flo
 // qy = R * pyp + R_z (per row), as the reference forms them in float
 // (geometryfusion_mipmap_cpu.cpp:1765-1790), and the frame counters reset
 // ---------------------------------------------------------------------------
-__global__ void k_frame_begin(const float *__restrict__ pxp, const float *__restrict__ pyp, int W, int H, CamF p,
-                              float *__restrict__ qx, float *__restrict__ qy, FrameCounters *fc, uint32_t n)
+__global__ void k_frame_begin(const float *__restrict__ pxp, const float *__restrict__ pyp, int W, int H,
+                              const FrameDev *__restrict__ fd, float *__restrict__ qx, float *__restrict__ qy,
+                              FrameCounters *fc)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const CamF &p = fd->p;
+    const uint32_t n = fd->n;
     if (i < W) {
         const float v = pxp[i];
         qx[i] = __fmul_rn(p.r[0], v);
@@ -173,14 +176,15 @@ __global__ void k_frame_begin(const float *__restrict__ pxp, const float *__rest
 // k_planes: per 64-pixel row chunk, ballot the three relations into bit
 // planes: plane[(y*words + w)*3 + {0: valid, 1: key == left key, 2: key == up key}]
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(64) void k_planes(const uint16_t *__restrict__ depth, int W, int H, int words,
-                                               float scaling, float maxcam, const float *__restrict__ qx,
-                                               const float *__restrict__ qy, float t0, float t1, float t2,
+__global__ __launch_bounds__(64) void k_planes(const FrameDev *__restrict__ fd, int W, int H, int words,
+                                               const float *__restrict__ qx, const float *__restrict__ qy,
                                                uint32_t *__restrict__ planes, uint32_t *__restrict__ cntB,
                                                uint32_t *__restrict__ cntL, uint32_t *__restrict__ firstD)
 {
     const int lane = threadIdx.x, y = blockIdx.y, x = blockIdx.x * 64 + lane;
-    const float t[3] = {t0, t1, t2};
+    const uint16_t *__restrict__ depth = fd->depth;
+    const float scaling = fd->scaling, maxcam = fd->maxcam;
+    const float t[3] = {fd->p.t[0], fd->p.t[1], fd->p.t[2]};
     bool valid = false, el = false, eu = false;
     if (x < W) {
         const float D = (float)depth[y * W + x];
@@ -328,18 +332,21 @@ __device__ __forceinline__ void queue_leaf(const MapView &m, uint32_t leaf, uint
     }
 }
 
-__global__ void k_descend(const uint16_t *__restrict__ depth, int W, int H, int words, float scaling,
-                          const float *__restrict__ qx, const float *__restrict__ qy, float t0, float t1,
-                          float t2, const uint32_t *__restrict__ pbits, MapView m, HashView hv, uint32_t n,
-                          int bw, uint32_t stamp, uint32_t *__restrict__ pl0, FrameCounters *fc)
+__global__ void k_descend(const FrameDev *__restrict__ fd, int W, int H, int words, const float *__restrict__ qx,
+                          const float *__restrict__ qy, const uint32_t *__restrict__ pbits, MapView m, HashView hv,
+                          uint32_t *__restrict__ pl0, FrameCounters *fc)
 {
+    const uint16_t *__restrict__ depth = fd->depth;
+    const float scaling = fd->scaling;
+    const uint32_t n = fd->n, stamp = fd->stamp;
+    const int bw = (int)fd->bandwidth;
     const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
     const int lane = threadIdx.x & 63;
     const bool proc = x < W && ((pbits[(size_t)(y + FF_PB_PAD) * FF_PB_STRIDE + (x >> 5)] >> (x & 31)) & 1u);
     const uint64_t pb = __ballot(proc);
     if (pb == 0) return;  // wave-uniform
     const uint32_t pix = (uint32_t)(y * W + x);
-    const float t[3] = {t0, t1, t2};
+    const float t[3] = {fd->p.t[0], fd->p.t[1], fd->p.t[2]};
     int v[3] = {0, 0, 0};
     float D = 0.0f;
     if (proc) {
@@ -527,8 +534,9 @@ __global__ __launch_bounds__(1024) void k_scan_b(uint64_t *__restrict__ bsum, ui
 
 __global__ void k_assign(HashView hv, const FrameCounters *fc, const uint64_t *__restrict__ pref,
                          const uint64_t *__restrict__ bsum, const uint32_t *__restrict__ firstD,
-                         uint32_t nBranchesUsed, uint32_t nLeavesUsed)
+                         const MapCounters *__restrict__ mapc)
 {
+    const uint32_t nBranchesUsed = mapc->nBranchesUsed, nLeavesUsed = mapc->nLeavesUsed;
     const uint32_t n = min(fc->listCount, hv.listCap);
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         const uint32_t s = hv.list[i];
@@ -555,8 +563,9 @@ __device__ __forceinline__ uint32_t parent_branch(const HashView &hv, uint32_t a
 }
 
 __global__ void k_link(HashView hv, MapView m, const FrameCounters *fc, const uint32_t *__restrict__ pl0,
-                       uint32_t n, uint32_t stamp, FrameCounters *fcw)
+                       const FrameDev *__restrict__ fd, FrameCounters *fcw)
 {
+    const uint32_t n = fd->n, stamp = fd->stamp;
     const uint32_t cnt = min(fc->listCount, hv.listCap);
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < cnt; i += gridDim.x * blockDim.x) {
         const uint32_t s = hv.list[i];
@@ -607,6 +616,16 @@ __global__ void k_hash_clear(HashView hv, const FrameCounters *fc)
         const uint32_t s = hv.list[i];
         hv.keys[s] = FF_HEMPTY;
         hv.minpix[s] = 0xffffffffu;
+    }
+}
+
+// the frame's new branches and bricks join the map's used counts (a frame
+// whose node table overflowed commits nothing; the host reports it)
+__global__ void k_commit(const FrameCounters *__restrict__ fc, MapCounters *mapc, uint32_t listCap)
+{
+    if (threadIdx.x == 0 && !(fc->flags & FF_FLAG_HASH_FULL) && fc->listCount <= listCap) {
+        mapc->nBranchesUsed += 16u * fc->newB;
+        mapc->nLeavesUsed += fc->newL;
     }
 }
 
@@ -733,23 +752,27 @@ __device__ __forceinline__ void sdf_brick(uint32_t leaf, const MapView &m, const
 
 #define FF_SDF_GRID 1024  // blocks of the SDF update; each loops over queued bricks
 
-__device__ __forceinline__ bool sdf_gate_open(const FrameCounters *fc, const SdfGate &g)
+// the map's used counts are the committed ones (after this frame)
+__device__ __forceinline__ bool sdf_gate_open(const FrameCounters *fc, const MapCounters *mapc, uint32_t n,
+                                              const SdfGate &g)
 {
     if (g.force) return true;
     if ((fc->flags & FF_FLAG_HASH_FULL) || fc->listCount > g.listCap) return false;
-    const bool out = fc->bmin[0] < 0 || fc->bmin[1] < 0 || fc->bmin[2] < 0 || fc->bmax[0] >= (int)g.n ||
-                     fc->bmax[1] >= (int)g.n || fc->bmax[2] >= (int)g.n;
-    const bool grows = g.leavesUsed + fc->newL < g.leavesTotal && g.branchesUsed + 16u * fc->newB < g.branchesTotal;
+    const bool out = fc->bmin[0] < 0 || fc->bmin[1] < 0 || fc->bmin[2] < 0 || fc->bmax[0] >= (int)n ||
+                     fc->bmax[1] >= (int)n || fc->bmax[2] >= (int)n;
+    const bool grows = mapc->nLeavesUsed < g.leavesTotal && mapc->nBranchesUsed < g.branchesTotal;
     return !(out && grows);
 }
 
 __global__ __launch_bounds__(512) void k_sdf(const uint32_t *__restrict__ queue, MapView m,
-                                             const uint16_t *__restrict__ depth,
-                                             const uint8_t *__restrict__ bgr, int W, int H, SdfParams P,
-                                             FrameCounters *fc, SdfGate gate)
+                                             const FrameDev *__restrict__ fd, int W, int H, FrameCounters *fc,
+                                             const MapCounters *__restrict__ mapc, SdfGate gate)
 {
     // blocks loop over the queued bricks; the count comes from the device
-    if (!sdf_gate_open(fc, gate)) return;
+    if (!sdf_gate_open(fc, mapc, fd->n, gate)) return;
+    const uint16_t *__restrict__ depth = fd->depth;
+    const uint8_t *__restrict__ bgr = fd->bgr;
+    const SdfParams P = fd->sp;
     const uint32_t nQ = fc->nQueued;
     if (blockIdx.x == 0 && threadIdx.x == 0) fc->sdfRan = 1u;
     if (blockIdx.x >= nQ) return;
@@ -841,9 +864,30 @@ static int ensure_image(ff_ctx *c, int W, int H)
     HIPCHK(hipMemsetAsync(c->d_pbits, 0, pbits_bytes(H), c->stream));
     c->capW = W;
     c->capH = H;
+    c->bufGen++;  // captured frame graphs refer to the old buffers
     return FF_OK;
 }
 
+
+void drop_frame_graphs(ff_ctx *c)
+{
+    for (int i = 0; i < 2; i++) {
+        if (c->frameExec[i]) hipGraphExecDestroy(c->frameExec[i]);
+        if (c->frameGraph[i]) hipGraphDestroy(c->frameGraph[i]);
+        c->frameExec[i] = nullptr;
+        c->frameGraph[i] = nullptr;
+    }
+}
+
+// writes the host's used counts to the device copy (after a host-side change)
+void sync_map_counters(ff_ctx *c)
+{
+    c->h_mapc->nBranchesUsed = c->nBranchesUsed;
+    c->h_mapc->nLeavesUsed = c->nLeavesUsed;
+    c->h_mapc->pad[0] = c->h_mapc->pad[1] = 0;
+    hipMemcpyAsync(c->d_mapc, c->h_mapc, sizeof(MapCounters), hipMemcpyHostToDevice, c->stream);
+    hipStreamSynchronize(c->stream);  // h_mapc is reused
+}
 
 static void launch_grow(ff_ctx *c, const int bmin[3], const int bmax[3])
 {
@@ -881,6 +925,7 @@ static void launch_grow(ff_ctx *c, const int bmin[3], const int bmax[3])
     k_copy_u32<<<(used + 255) / 256, 256, 0, c->stream>>>(c->d_treeTmp, c->d_tree + add, used);
     k_fill_u32<<<(add + 255) / 256, 256, 0, c->stream>>>(c->d_tree, FF_BRANCHINIT, add);
     c->nBranchesUsed += add;
+    sync_map_counters(c);
     uint32_t index = (maxAdd - 1) * 16u;
     uint16_t addv[3] = {0, 0, 0};
     std::vector<std::pair<uint32_t, uint32_t>> links;
@@ -963,7 +1008,11 @@ ff_ctx *ff_create(const ff_params *prm)
               hipMalloc(&c->d_dist, sizeof(float) * NL * FF_BRICK) == hipSuccess &&
               hipMalloc(&c->d_wgt, sizeof(float) * NL * FF_BRICK) == hipSuccess &&
               hipMalloc(&c->d_fc, sizeof(FrameCounters)) == hipSuccess &&
-              hipHostMalloc(&c->h_fc, sizeof(FrameCounters), hipHostMallocDefault) == hipSuccess;
+              hipHostMalloc(&c->h_fc, sizeof(FrameCounters), hipHostMallocDefault) == hipSuccess &&
+              hipMalloc(&c->d_fd, sizeof(FrameDev)) == hipSuccess &&
+              hipHostMalloc(&c->h_fd, sizeof(FrameDev), hipHostMallocDefault) == hipSuccess &&
+              hipMalloc(&c->d_mapc, sizeof(MapCounters)) == hipSuccess &&
+              hipHostMalloc(&c->h_mapc, sizeof(MapCounters), hipHostMallocDefault) == hipSuccess;
     if (ok && c->useColor) ok = hipMalloc(&c->d_col, sizeof(uint16_t) * NL * FF_BRICK * 3) == hipSuccess;
     const uint32_t hcap = 1u << 22;
     ok = ok && hipMalloc(&c->hv.keys, sizeof(uint64_t) * hcap) == hipSuccess &&
@@ -987,6 +1036,8 @@ ff_ctx *ff_create(const ff_params *prm)
     if (c->d_col) hipMemsetAsync(c->d_col, 0, sizeof(uint16_t) * NL * FF_BRICK * 3, c->stream);
     hipMemcpyToSymbolAsync(HIP_SYMBOL(c_rcp_table), FF_RCP_TABLE, sizeof(FF_RCP_TABLE), 0, hipMemcpyHostToDevice,
                            c->stream);
+    memset(c->h_fd, 0, sizeof(FrameDev));
+    sync_map_counters(c);
     if (hipStreamSynchronize(c->stream) != hipSuccess) {
         ff_destroy(c);
         return nullptr;
@@ -1009,6 +1060,9 @@ void ff_destroy(ff_ctx *c)
     hipFree(c->d_tree); hipFree(c->d_treeTmp); hipFree(c->d_leafinfo); hipFree(c->d_parent);
     hipFree(c->d_stamp); hipFree(c->d_queue); hipFree(c->d_dist); hipFree(c->d_wgt); hipFree(c->d_col);
     hipFree(c->d_fc); hipHostFree(c->h_fc);
+    hipFree(c->d_fd); hipHostFree(c->h_fd);
+    hipFree(c->d_mapc); hipHostFree(c->h_mapc);
+    drop_frame_graphs(c);
     hipFree(c->hv.keys); hipFree(c->hv.minpix); hipFree(c->hv.aux); hipFree(c->hv.assigned); hipFree(c->hv.list);
     for (int i = 0; i < 8; i++)
         if (c->ev[i]) hipEventDestroy(c->ev[i]);
@@ -1019,6 +1073,97 @@ void ff_destroy(ff_ctx *c)
 const char *ff_last_error(const ff_ctx *c) { return c ? c->err.c_str() : "null context"; }
 
 void *ff_stream(ff_ctx *c) { return c ? (void *)c->stream : nullptr; }
+
+// A frame's device work reads every per-frame value from c->d_fd, so that
+// it can be captured once and replayed as graphs: the allocation (ending
+// with the commit of the used counts), then the learned confidence when on,
+// the gated SDF update and the counters' copy back.
+static void launch_alloc(ff_ctx *c, int W, int H)
+{
+    const int words = (W + 31) / 32;
+    const uint32_t npix = (uint32_t)W * H;
+    const FrameDev *fd = c->d_fd;
+    k_frame_begin<<<(std::max(W, H) + 255) / 256, 256, 0, c->stream>>>(c->d_pxp, c->d_pyp, W, H, fd, c->d_qx,
+                                                                       c->d_qy, c->d_fc);
+    k_planes<<<dim3((W + 63) / 64, H), 64, 0, c->stream>>>(fd, W, H, words, c->d_qx, c->d_qy, c->d_planes, c->d_cntB,
+                                                          c->d_cntL, c->d_firstD);
+    if (dedup_rows(words) == 32)
+        k_dedup<32><<<1, FF_DD_THREADS, dedup_lds_bytes(words), c->stream>>>(c->d_planes, H, words, c->d_pbits,
+                                                                             c->d_fc);
+    else
+        k_dedup<16><<<1, FF_DD_THREADS, dedup_lds_bytes(words), c->stream>>>(c->d_planes, H, words, c->d_pbits,
+                                                                             c->d_fc);
+    const MapView mv = c->mapView();
+    k_descend<<<dim3((W + 127) / 128, H), 128, 0, c->stream>>>(fd, W, H, words, c->d_qx, c->d_qy, c->d_pbits, mv,
+                                                               c->hv, c->d_pl0, c->d_fc);
+    k_resolve<<<256, 256, 0, c->stream>>>(c->hv, c->d_fc, c->d_cntB, c->d_cntL, c->d_firstD);
+    const uint32_t nb = (npix + 4095) / 4096;
+    k_scan_a<<<nb, 1024, 0, c->stream>>>(c->d_cntB, c->d_cntL, npix, c->d_pref, c->d_bsum);
+    k_scan_b<<<1, 1024, 0, c->stream>>>(c->d_bsum, nb, c->d_fc);
+    k_assign<<<256, 256, 0, c->stream>>>(c->hv, c->d_fc, c->d_pref, c->d_bsum, c->d_firstD, c->d_mapc);
+    k_link<<<256, 256, 0, c->stream>>>(c->hv, mv, c->d_fc, c->d_pl0, fd, c->d_fc);
+    mesh_record_branches(c);
+    k_hash_clear<<<256, 256, 0, c->stream>>>(c->hv, c->d_fc);
+    k_commit<<<1, 64, 0, c->stream>>>(c->d_fc, c->d_mapc, c->hv.listCap);
+}
+
+// the SDF update waits for the host only when the node table overflowed or
+// the volume must grow first
+static void launch_sdf(ff_ctx *c, int W, int H)
+{
+    if (c->learned) learned_launch(c, W, H);
+    SdfGate gate;
+    gate.listCap = c->hv.listCap;
+    gate.leavesTotal = c->nLeavesTotal;
+    gate.branchesTotal = c->nBranchesTotal;
+    gate.force = 0;
+    k_sdf<<<FF_SDF_GRID, 512, 0, c->stream>>>(c->d_queue, c->mapView(), c->d_fd, W, H, c->d_fc, c->d_mapc, gate);
+    hipMemcpyAsync(c->h_fc, c->d_fc, sizeof(FrameCounters), hipMemcpyDeviceToHost, c->stream);
+}
+
+
+static bool capture(ff_ctx *c, void (*fn)(ff_ctx *, int, int), int W, int H, hipGraph_t *g, hipGraphExec_t *x)
+{
+    hipError_t e = hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal);
+    if (e == hipSuccess) {
+        fn(c, W, H);
+        e = hipStreamEndCapture(c->stream, g);
+    }
+    if (e == hipSuccess) e = hipGraphInstantiate(x, *g, nullptr, nullptr, 0);
+    if (e != hipSuccess) (void)hipGetLastError();
+    return e == hipSuccess;
+}
+
+// the frame's two launch sequences, from graphs captured for this
+// configuration (timing events are recorded between them)
+static int run_frame(ff_ctx *c, int W, int H)
+{
+    const uint64_t key = ((uint64_t)W << 48) ^ ((uint64_t)H << 32) ^ ((uint64_t)c->bufGen << 1) ^
+                         (uint64_t)(c->learned != nullptr);
+    if (c->useGraphs && (!c->frameExec[0] || c->frameKey != key)) {
+        drop_frame_graphs(c);
+        if (capture(c, launch_alloc, W, H, &c->frameGraph[0], &c->frameExec[0]) &&
+            capture(c, launch_sdf, W, H, &c->frameGraph[1], &c->frameExec[1])) {
+            c->frameKey = key;
+        } else {
+            drop_frame_graphs(c);  // capture is an optimisation: launch directly from now on
+            c->useGraphs = false;
+        }
+    }
+    const bool g = c->useGraphs && c->frameExec[0];
+    if (c->timing) hipEventRecord(c->ev[0], c->stream);
+    if (g) HIPCHK(hipGraphLaunch(c->frameExec[0], c->stream));
+    else launch_alloc(c, W, H);
+    if (c->timing) {
+        hipEventRecord(c->ev[1], c->stream);
+        hipEventRecord(c->ev[2], c->stream);
+    }
+    if (g) HIPCHK(hipGraphLaunch(c->frameExec[1], c->stream));
+    else launch_sdf(c, W, H);
+    if (c->timing) hipEventRecord(c->ev[3], c->stream);
+    HIPCHK(hipGetLastError());
+    return FF_OK;
+}
 
 // depth / bgr: the frame in device memory, dense rows
 static int add_map_impl(ff_ctx *c, const uint16_t *depth, const uint8_t *bgr, int W, int H, const double R[9],
@@ -1060,67 +1205,41 @@ static int add_map_impl(ff_ctx *c, const uint16_t *depth, const uint8_t *bgr, in
         HIPCHK(hipStreamSynchronize(c->stream));  // the host tables are reused
     }
     const CamF q = make_pinv(c, R, t, K);
-    k_frame_begin<<<(std::max(W, H) + 255) / 256, 256, 0, c->stream>>>(c->d_pxp, c->d_pyp, W, H, p, c->d_qx,
-                                                                       c->d_qy, c->d_fc, c->n);
-    const int words = (W + 31) / 32;
-    const uint32_t npix = (uint32_t)W * H;
-    if (c->n) {
-        if (c->timing) hipEventRecord(c->ev[0], c->stream);
-        k_planes<<<dim3((W + 63) / 64, H), 64, 0, c->stream>>>(depth, W, H, words, scaling, maxcam, c->d_qx,
-                                                              c->d_qy, p.t[0], p.t[1], p.t[2], c->d_planes,
-                                                              c->d_cntB, c->d_cntL, c->d_firstD);
-        if (dedup_rows(words) == 32)
-            k_dedup<32><<<1, FF_DD_THREADS, dedup_lds_bytes(words), c->stream>>>(c->d_planes, H, words, c->d_pbits,
-                                                                                 c->d_fc);
-        else
-            k_dedup<16><<<1, FF_DD_THREADS, dedup_lds_bytes(words), c->stream>>>(c->d_planes, H, words, c->d_pbits,
-                                                                                 c->d_fc);
-        MapView mv = c->mapView();
-        k_descend<<<dim3((W + 127) / 128, H), 128, 0, c->stream>>>(depth, W, H, words, scaling, c->d_qx,
-                                                                   c->d_qy, p.t[0], p.t[1], p.t[2], c->d_pbits,
-                                                                   mv, c->hv, c->n, c->bandwidth, stamp,
-                                                                   c->d_pl0, c->d_fc);
-        k_resolve<<<256, 256, 0, c->stream>>>(c->hv, c->d_fc, c->d_cntB, c->d_cntL, c->d_firstD);
-        const uint32_t nb = (npix + 4095) / 4096;
-        k_scan_a<<<nb, 1024, 0, c->stream>>>(c->d_cntB, c->d_cntL, npix, c->d_pref, c->d_bsum);
-        k_scan_b<<<1, 1024, 0, c->stream>>>(c->d_bsum, nb, c->d_fc);
-        k_assign<<<256, 256, 0, c->stream>>>(c->hv, c->d_fc, c->d_pref, c->d_bsum, c->d_firstD, c->nBranchesUsed,
-                                             c->nLeavesUsed);
-        k_link<<<256, 256, 0, c->stream>>>(c->hv, mv, c->d_fc, c->d_pl0, c->n, stamp, c->d_fc);
-        mesh_record_branches(c);
-        k_hash_clear<<<256, 256, 0, c->stream>>>(c->hv, c->d_fc);
-        if (c->timing) hipEventRecord(c->ev[1], c->stream);
-    } else {
-        HIPCHK(hipMemsetAsync(c->d_pbits, 0, pbits_bytes(H), c->stream));
-    }
-    // the SDF update follows on the device; it waits for the host only when
-    // the node table overflowed or the volume must grow first
-    SdfParams sp;
+    // the frame's parameters, staged in pinned memory for the device
+    FrameDev &f = *c->h_fd;
+    f.p = p;
     for (int i = 0; i < 3; i++) {
-        sp.m[4 * i] = q.r[3 * i]; sp.m[4 * i + 1] = q.r[3 * i + 1]; sp.m[4 * i + 2] = q.r[3 * i + 2];
-        sp.m[4 * i + 3] = q.t[i];
+        f.sp.m[4 * i] = q.r[3 * i]; f.sp.m[4 * i + 1] = q.r[3 * i + 1]; f.sp.m[4 * i + 2] = q.r[3 * i + 2];
+        f.sp.m[4 * i + 3] = q.t[i];
     }
-    sp.fx = q.fx; sp.fy = q.fy; sp.cx = q.cx; sp.cy = q.cy;
-    sp.scale = c->scale; sp.dT = c->dT; sp.scaling = scaling; sp.maxcam = maxcam;
-    sp.useColor = c->useColor ? 1 : 0;
-    sp.conf = nullptr;
-    SdfGate gate;
-    gate.listCap = c->hv.listCap;
-    gate.leavesUsed = c->nLeavesUsed; gate.leavesTotal = c->nLeavesTotal;
-    gate.branchesUsed = c->nBranchesUsed; gate.branchesTotal = c->nBranchesTotal;
-    gate.n = c->n;
-    gate.force = 0;
+    f.sp.fx = q.fx; f.sp.fy = q.fy; f.sp.cx = q.cx; f.sp.cy = q.cy;
+    f.sp.scale = c->scale; f.sp.dT = c->dT; f.sp.scaling = scaling; f.sp.maxcam = maxcam;
+    f.sp.useColor = c->useColor ? 1 : 0;
+    f.sp.conf = nullptr;
+    if (c->learned) {
+        f.sp.conf = learned_prepare(c, W, H);
+        if (!f.sp.conf) return FF_E_DEVICE;
+    }
+    f.depth = depth;
+    f.bgr = bgr;
+    f.scaling = scaling;
+    f.maxcam = maxcam;
+    f.ifx = 1.0f / (float)K[0];
+    f.ify = 1.0f / (float)K[1];
+    f.stamp = stamp;
+    f.n = c->n;
+    f.bandwidth = c->bandwidth;
+    f.pad = 0;
+    HIPCHK(hipMemcpyAsync(c->d_fd, c->h_fd, sizeof(FrameDev), hipMemcpyHostToDevice, c->stream));
     if (c->n) {
-        if (c->timing) hipEventRecord(c->ev[2], c->stream);
-        if (c->learned) {
-            sp.conf = learned_confidence(c, depth, W, H, K, scaling, maxcam);
-            if (!sp.conf) return FF_E_DEVICE;
-        }
-        k_sdf<<<FF_SDF_GRID, 512, 0, c->stream>>>(c->d_queue, c->mapView(), depth, bgr, W, H, sp, c->d_fc, gate);
-        if (c->timing) hipEventRecord(c->ev[3], c->stream);
-        HIPCHK(hipGetLastError());
+        int rc = run_frame(c, W, H);
+        if (rc) return rc;
+    } else {
+        k_frame_begin<<<(std::max(W, H) + 255) / 256, 256, 0, c->stream>>>(c->d_pxp, c->d_pyp, W, H, c->d_fd,
+                                                                           c->d_qx, c->d_qy, c->d_fc);
+        HIPCHK(hipMemsetAsync(c->d_pbits, 0, pbits_bytes(H), c->stream));
+        HIPCHK(hipMemcpyAsync(hf, c->d_fc, sizeof(FrameCounters), hipMemcpyDeviceToHost, c->stream));
     }
-    HIPCHK(hipMemcpyAsync(hf, c->d_fc, sizeof(FrameCounters), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     if (c->timing && c->n) {
         float ms = 0.0f;
@@ -1141,6 +1260,7 @@ static int add_map_impl(ff_ctx *c, const uint16_t *depth, const uint8_t *bgr, in
         c->err = "per-frame node list overflow";
         return FF_E_POOL_FULL;
     }
+    // the host mirror of the counts k_commit added on the device
     c->nBranchesUsed += 16u * hf->newB;
     c->nLeavesUsed += hf->newL;
     c->nQueued = hf->nQueued;
@@ -1157,10 +1277,15 @@ static int add_map_impl(ff_ctx *c, const uint16_t *depth, const uint8_t *bgr, in
         mesh_on_grow(c);
     }
     if (c->nQueued && !hf->sdfRan) {
-        // the gated launch stood down (volume grown above): update now
+        // the gated launch stood down (volume grown above): update now with
+        // this frame's parameters, still in c->d_fd
+        SdfGate gate;
+        gate.listCap = c->hv.listCap;
+        gate.leavesTotal = c->nLeavesTotal;
+        gate.branchesTotal = c->nBranchesTotal;
         gate.force = 1;
         if (c->timing) hipEventRecord(c->ev[4], c->stream);
-        k_sdf<<<FF_SDF_GRID, 512, 0, c->stream>>>(c->d_queue, c->mapView(), depth, bgr, W, H, sp, c->d_fc, gate);
+        k_sdf<<<FF_SDF_GRID, 512, 0, c->stream>>>(c->d_queue, c->mapView(), c->d_fd, W, H, c->d_fc, c->d_mapc, gate);
         if (c->timing) hipEventRecord(c->ev[5], c->stream);
         HIPCHK(hipGetLastError());
         HIPCHK(hipStreamSynchronize(c->stream));
@@ -1217,6 +1342,15 @@ int ff_add_map_device(ff_ctx *c, const uint16_t *depth_dev, const uint8_t *bgr_d
     if (rc) return rc;
     // read in place: the call returns after the frame's kernels completed
     return add_map_impl(c, depth_dev, c->useColor ? bgr_dev : nullptr, W, H, R, t, K, scaling, maxcam);
+}
+
+int ff_set_graph_capture(ff_ctx *c, int on)
+{
+    if (!c) return FF_E_ARG;
+    hipSetDevice(c->device);
+    drop_frame_graphs(c);
+    c->useGraphs = on != 0;
+    return FF_OK;
 }
 
 int ff_set_timing(ff_ctx *c, int on)

@@ -36,7 +36,7 @@ struct FrameCounters {
 // what the SDF update checks on the device before running: a frame whose
 // node table overflowed, or that must grow the volume first, waits for the host
 struct SdfGate {
-    uint32_t listCap, leavesUsed, leavesTotal, branchesUsed, branchesTotal, n;
+    uint32_t listCap, leavesTotal, branchesTotal;
     int force;
 };
 
@@ -65,6 +65,23 @@ struct SdfParams {
     float fx, fy, cx, cy, scale, dT, scaling, maxcam;
     int useColor;
     const float *conf;  // per-pixel confidence multiplying the weight, or nullptr
+};
+
+// per-frame values the kernels read from device memory, so that a frame's
+// launches can be captured once and replayed from a graph
+struct FrameDev {
+    CamF p;                 // camera -> voxel coordinates (back-projection)
+    SdfParams sp;           // SDF update (voxel -> camera, intrinsics, confidence)
+    const uint16_t *depth;  // the frame in device memory
+    const uint8_t *bgr;
+    float scaling, maxcam, ifx, ify;  // ifx, ify: 1 / fx, 1 / fy
+    uint32_t stamp, n, bandwidth, pad;
+};
+
+// the map's used counts, committed on the device at the end of each
+// allocation (the host mirror follows after the frame's synchronisation)
+struct MapCounters {
+    uint32_t nBranchesUsed, nLeavesUsed, pad[2];
 };
 
 struct MeshState;
@@ -100,6 +117,14 @@ struct ff_ctx {
              *d_pl0 = nullptr;
     uint64_t *d_pref = nullptr, *d_bsum = nullptr;
     ff::FrameCounters *d_fc = nullptr, *h_fc = nullptr;
+    ff::FrameDev *d_fd = nullptr, *h_fd = nullptr;     // h_fd: pinned staging
+    ff::MapCounters *d_mapc = nullptr, *h_mapc = nullptr;
+    // the captured per-frame launches: [0] allocation, [1] confidence + SDF
+    hipGraph_t frameGraph[2] = {nullptr, nullptr};
+    hipGraphExec_t frameExec[2] = {nullptr, nullptr};
+    uint64_t frameKey = 0;     // what the captured graph depends on
+    uint32_t bufGen = 0;       // bumped when buffers a graph refers to are replaced
+    bool useGraphs = false;    // direct launches measure faster for one stream
     ff::HashView hv{};
     ff::MeshState *mesh = nullptr;
     ff::LearnedState *learned = nullptr;  // learned confidence (off: nullptr)
@@ -122,10 +147,13 @@ struct ff_ctx {
 };
 
 namespace ff {
-// learned confidence (ff_learned.hip)
+// learned confidence (ff_learned.hip): the per-pixel buffer for a W x H
+// frame (allocated on first use), and the launch filling it from c->d_fd
 void learned_free(ff_ctx *c);
-const float *learned_confidence(ff_ctx *c, const uint16_t *depth, int W, int H, const double K[4], float scaling,
-                                float maxcam);
+const float *learned_prepare(ff_ctx *c, int W, int H);
+void learned_launch(ff_ctx *c, int W, int H);
+void sync_map_counters(ff_ctx *c);
+void drop_frame_graphs(ff_ctx *c);
 // meshing module hooks (ff_mesh.hip)
 int mesh_init(ff_ctx *c);
 void mesh_free(ff_ctx *c);

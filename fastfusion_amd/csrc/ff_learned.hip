@@ -262,10 +262,12 @@ __device__ float feature(const uint16_t *__restrict__ depth, int W, int H, int x
 #define CONF_WIN_W 20
 #define CONF_WIN 100
 
-__global__ __launch_bounds__(256) void k_confidence(const uint16_t *__restrict__ depth, int W, int H, FeatCam K,
+__global__ __launch_bounds__(256) void k_confidence(const FrameDev *__restrict__ fd, int W, int H,
                                                     const bf16x8 *__restrict__ frag, const float *__restrict__ bias,
                                                     const float *__restrict__ w4, float *__restrict__ conf)
 {
+    const uint16_t *__restrict__ depth = fd->depth;
+    const FeatCam K{fd->sp.fx, fd->sp.fy, fd->sp.cx, fd->sp.cy, fd->scaling, fd->maxcam, fd->ifx, fd->ify};
     __shared__ float sb[3 * 64 + 1], sw4[64];
     __shared__ float win[4][CONF_WIN + 4];
     for (int i = threadIdx.x; i < 3 * 64 + 1; i += blockDim.x) sb[i] = bias[i];
@@ -648,15 +650,15 @@ static bool lok(ff_ctx *c, hipError_t e, const char *what)
 void learned_free(ff_ctx *c)
 {
     if (!c->learned) return;
+    c->bufGen++;
     c->learned->w.free_all();
     hipFree(c->learned->conf);
     delete c->learned;
     c->learned = nullptr;
 }
 
-// per-pixel confidence of the frame, or nullptr with learned weighting off
-const float *learned_confidence(ff_ctx *c, const uint16_t *depth, int W, int H, const double K[4], float scaling,
-                                float maxcam)
+// the confidence buffer for a W x H frame, or nullptr on failure
+const float *learned_prepare(ff_ctx *c, int W, int H)
 {
     LearnedState *L = c->learned;
     if (!L) return nullptr;
@@ -667,13 +669,18 @@ const float *learned_confidence(ff_ctx *c, const uint16_t *depth, int W, int H, 
         L->confCap = 0;
         if (!lok(c, hipMalloc(&L->conf, sizeof(float) * np), "hipMalloc(confidence)")) return nullptr;
         L->confCap = np;
+        c->bufGen++;
     }
-    FeatCam fk{(float)K[0], (float)K[1], (float)K[2], (float)K[3], scaling, maxcam, 1.0f / (float)K[0],
-               1.0f / (float)K[1]};
+    return L->conf;
+}
+
+// per-pixel confidence of the frame described by c->d_fd
+void learned_launch(ff_ctx *c, int W, int H)
+{
+    LearnedState *L = c->learned;
     const int tiles = (W + 15) / 16 * H;
     const int blocks = std::min(2048, (tiles + 3) / 4);
-    k_confidence<<<blocks, 256, 0, c->stream>>>(depth, W, H, fk, L->w.frag, L->w.bias, L->w.w4, L->conf);
-    return L->conf;
+    k_confidence<<<blocks, 256, 0, c->stream>>>(c->d_fd, W, H, L->w.frag, L->w.bias, L->w.w4, L->conf);
 }
 
 }  // namespace ff
@@ -789,6 +796,7 @@ int ff_set_learned_weighting(ff_ctx *c, const float *params, size_t n)
         return FF_E_ARG;
     }
     if (!c->learned) {
+        c->bufGen++;
         c->learned = new LearnedState();
         if (c->learned->w.alloc(c->stream)) {
             learned_free(c);

@@ -1198,9 +1198,10 @@ void mesh_on_grow(ff_ctx *) {}
 void mesh_after_frame(ff_ctx *) {}
 
 __global__ void k_set_binfo(ushort4 *binfo, const uint64_t *__restrict__ keys, const uint32_t *__restrict__ assigned,
-                            const uint32_t *__restrict__ list, const uint32_t *listCount, uint32_t listCap, uint32_t n,
-                            uint32_t nBlocks)
+                            const uint32_t *__restrict__ list, const uint32_t *listCount, uint32_t listCap,
+                            const FrameDev *__restrict__ fd, uint32_t nBlocks)
 {
+    const uint32_t n = fd->n;
     const uint32_t cnt = min(*listCount, listCap);
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < cnt; i += gridDim.x * blockDim.x) {
         const uint32_t s = list[i];
@@ -1219,7 +1220,7 @@ void mesh_record_branches(ff_ctx *c)
 {
     MeshState *s = c->mesh;
     k_set_binfo<<<256, 256, 0, c->stream>>>(s->d_binfo, c->hv.keys, c->hv.assigned, c->hv.list, &c->d_fc->listCount,
-                                            c->hv.listCap, c->n, s->binfoCap);
+                                            c->hv.listCap, c->d_fd, s->binfoCap);
 }
 
 // totals of the arena counters (vertices, faces allocated since the last

@@ -125,6 +125,11 @@ int ff_copy_processed(ff_ctx *ctx, uint8_t *out, size_t n);
  * ff_get_timing returns per phase the summed milliseconds, the number of
  * timed launches, the summed algorithmic bytes and work units (processed
  * pixels, queued bricks, re-meshed cells); arrays have 3 entries. */
+/* Launch mode, no counterpart in the reference: with capture on, a frame's
+ * device work is captured once into HIP graphs and replayed; off (the
+ * default, measured faster for one stream), every kernel is launched
+ * directly. */
+int ff_set_graph_capture(ff_ctx *ctx, int on);
 int ff_set_timing(ff_ctx *ctx, int on);
 int ff_get_timing(ff_ctx *ctx, double *ms, uint64_t *calls, double *bytes, uint64_t *units);
 int ff_reset_timing(ff_ctx *ctx);

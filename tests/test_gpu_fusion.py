@@ -58,3 +58,21 @@ def test_grow_parity():
     finally:
         helpers.K = old
     assert gpu.counts()["grows"] >= 1
+
+
+def test_graph_capture_parity():
+    """Frames replayed from captured HIP graphs (per-frame values read from
+    device memory) give the same map as the oracle, through a grow."""
+    from tests.test_cpu_oracle import grow_sequence
+    sc, R, t = grow_sequence()
+    gpu, ora = make_pair(0.02, 0.06)
+    gpu.setGraphCapture(True)
+    cfg = dict(w=320, h=240, maxcam=6.0)
+    from tests import helpers
+    old = helpers.K
+    helpers.K = (262.5, 262.5, 159.5, 119.5)
+    try:
+        run_pair(sc, R, t, list(range(len(R))), cfg, gpu, ora)
+    finally:
+        helpers.K = old
+    assert gpu.counts()["grows"] >= 1
