@@ -34,6 +34,12 @@ class _Counts(ctypes.Structure):
                 ("grows", ctypes.c_uint32)]
 
 
+class _Frame(ctypes.Structure):
+    _fields_ = [("depth", ctypes.c_void_p), ("bgr", ctypes.c_void_p), ("width", ctypes.c_int),
+                ("height", ctypes.c_int), ("R", ctypes.c_double * 9), ("t", ctypes.c_double * 3),
+                ("K", ctypes.c_double * 4), ("scaling", ctypes.c_float), ("maxcam", ctypes.c_float)]
+
+
 # name -> (restype, argtypes); exactly the symbols of include/ff_abi.h
 _P = ctypes.c_void_p
 _SZ = ctypes.c_size_t
@@ -78,6 +84,12 @@ ABI = {
                                         ctypes.POINTER(ctypes.c_float)]),
     "ff_trainer_grad": (ctypes.c_int, [_P, _P, ctypes.c_uint32, _P, ctypes.POINTER(ctypes.c_float)]),
     "ff_trainer_params": (ctypes.c_int, [_P, _P, _SZ]),
+    "ff_add_map_batch": (ctypes.c_int, [_P, _P, ctypes.c_int]),
+    "ff_update_meshes_batch": (ctypes.c_int, [_P, ctypes.c_int]),
+    "ff_batch_set_config": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+    "ff_batch_autotune": (ctypes.c_int, [_P, ctypes.c_int]),
+    "ff_batch_get_config": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
+                                           ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]),
 }
 
 MLP_NPARAMS = 10177       # FF_MLP_NPARAMS
@@ -386,3 +398,48 @@ class Trainer:
         out = np.zeros(MLP_NPARAMS, np.float32)
         self._check(self._lib.ff_trainer_params(self._h, _ptr(out), out.size))
         return out
+
+
+class Batch:
+    """A fixed set of FusionMipMapGPU maps advanced one frame per tick
+    (ff_add_map_batch / ff_update_meshes_batch): one SDF launch and one
+    marching-cubes launch for all of them."""
+
+    def __init__(self, fusions):
+        self._lib = load_library()
+        self.fusions = list(fusions)
+        self._ctxs = (ctypes.c_void_p * len(self.fusions))(*[f.handle for f in self.fusions])
+        self._frames = (_Frame * len(self.fusions))()
+
+    def _check(self, rc):
+        if rc < 0:
+            msgs = [self._lib.ff_last_error(f.handle).decode(errors="replace") for f in self.fusions]
+            raise FFError(f"batch error {rc}: " + "; ".join(m for m in msgs if m))
+        return rc
+
+    def addMaps(self, frames):
+        """frames: per map (depth_ptr, bgr_ptr, w, h, R, t, K, scaling, maxcam), device pointers."""
+        for i, (dp, cp, w, h, R, t, K, sc, mc) in enumerate(frames):
+            f = self._frames[i]
+            f.depth, f.bgr, f.width, f.height = dp, cp, int(w), int(h)
+            f.R[:] = [float(v) for v in np.asarray(R, np.float64).ravel()]
+            f.t[:] = [float(v) for v in np.asarray(t, np.float64).ravel()]
+            f.K[:] = [float(v) for v in K]
+            f.scaling, f.maxcam = float(sc), float(mc)
+        return self._check(self._lib.ff_add_map_batch(self._ctxs, self._frames, len(self.fusions)))
+
+    def updateMeshes(self):
+        return self._check(self._lib.ff_update_meshes_batch(self._ctxs, len(self.fusions)))
+
+    def set_config(self, threads_per_brick=512, bricks_per_workgroup=1, graphs=False):
+        self._check(self._lib.ff_batch_set_config(self._ctxs[0], int(threads_per_brick),
+                                                  int(bricks_per_workgroup), int(bool(graphs))))
+
+    def autotune(self, ticks_per_candidate=2):
+        self._check(self._lib.ff_batch_autotune(self._ctxs[0], int(ticks_per_candidate)))
+
+    def config(self):
+        v = [ctypes.c_int() for _ in range(4)]
+        self._check(self._lib.ff_batch_get_config(self._ctxs[0], *[ctypes.byref(x) for x in v]))
+        return {"threads_per_brick": v[0].value, "bricks_per_workgroup": v[1].value, "graphs": bool(v[2].value),
+                "tuning": bool(v[3].value)}

@@ -1,0 +1,337 @@
+// ff_batch.hip -- many independent map streams per tick on one GPU.
+//
+// The reference fuses one map per FusionMipMapCPU object; a tick here
+// advances n such objects (contexts) by one frame each:
+//   host     each stream's frame preparation (pose, first-frame volume)
+//   device   each stream's allocation kernels, its learned confidence when
+//            on, then ONE SDF launch over the concatenation of every
+//            stream's queued bricks (k_sdf_batch: a prefix over the
+//            streams' gated counts maps worklist items to streams), and one
+//            gather of the streams' frame counters
+//   host     each stream's finish (used counts, errors, grow)
+// The device part can be captured into a HIP graph and replayed per tick.
+// Threads per brick (tpb), bricks per workgroup (bpw) and the launch mode are
+// picked by an autotuner that times each candidate on the first ticks;
+// every candidate computes the same maps.
+#include "ff_device.hpp"
+#include "ff_internal.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <vector>
+
+namespace ff {
+
+struct TuneCand {
+    int tpb, bpw;
+    bool graphs;
+    double ms;
+    int runs;
+};
+
+struct BatchState {
+    std::vector<ff_ctx *> ctxs;  // the batch (key)
+    int cap = 0;
+    SdfStream *d_ss = nullptr, *h_ss = nullptr;
+    uint32_t *d_prefix = nullptr;
+    FrameCounters *d_fcs = nullptr, *h_fcs = nullptr;
+    // launch configuration and its autotuning
+    int tpb = 512, bpw = 1;
+    bool graphs = false;
+    bool tuning = true;
+    int ticksPerCand = 2;
+    std::vector<TuneCand> cands;
+    size_t cur = 0;
+    // captured device work of a tick
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    uint64_t key = 0;
+    uint64_t ticks = 0;
+};
+
+static void drop_graph(BatchState *b)
+{
+    if (b->exec) hipGraphExecDestroy(b->exec);
+    if (b->graph) hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+}
+
+void batch_free(ff_ctx *c)
+{
+    BatchState *b = c->batch;
+    if (!b) return;
+    drop_graph(b);
+    hipFree(b->d_ss); hipHostFree(b->h_ss); hipFree(b->d_prefix); hipFree(b->d_fcs); hipHostFree(b->h_fcs);
+    delete b;
+    c->batch = nullptr;
+}
+
+static void default_cands(BatchState *b)
+{
+    b->cands.clear();
+    for (int g = 0; g < 2; g++)
+        for (int tpb : {512, 256})
+            for (int bpw : {1, 2, 4, 8}) b->cands.push_back({tpb, bpw, g == 1, 0.0, 0});
+    b->cur = 0;
+    b->tuning = true;
+}
+
+static BatchState *batch_for(ff_ctx *const *ctxs, int n)
+{
+    ff_ctx *lead = ctxs[0];
+    BatchState *b = lead->batch;
+    bool same = b && (int)b->ctxs.size() == n;
+    for (int i = 0; same && i < n; i++) same = b->ctxs[i] == ctxs[i];
+    if (same) return b;
+    if (!b) {
+        b = new BatchState();
+        lead->batch = b;
+        default_cands(b);
+    }
+    drop_graph(b);
+    b->ctxs.assign(ctxs, ctxs + n);
+    if (b->cap < n) {
+        hipFree(b->d_ss); hipHostFree(b->h_ss); hipFree(b->d_prefix); hipFree(b->d_fcs); hipHostFree(b->h_fcs);
+        b->d_ss = nullptr; b->h_ss = nullptr; b->d_prefix = nullptr; b->d_fcs = nullptr; b->h_fcs = nullptr;
+        b->cap = 0;
+        if (hipMalloc(&b->d_ss, sizeof(SdfStream) * n) != hipSuccess ||
+            hipHostMalloc(&b->h_ss, sizeof(SdfStream) * n, hipHostMallocDefault) != hipSuccess ||
+            hipMalloc(&b->d_prefix, sizeof(uint32_t) * (n + 1)) != hipSuccess ||
+            hipMalloc(&b->d_fcs, sizeof(FrameCounters) * n) != hipSuccess ||
+            hipHostMalloc(&b->h_fcs, sizeof(FrameCounters) * n, hipHostMallocDefault) != hipSuccess)
+            return nullptr;
+        b->cap = n;
+    }
+    return b;
+}
+
+struct TickPlan {
+    BatchState *b;
+    hipStream_t st;
+    std::vector<int> dev;  // contexts with a volume (device work), index into ctxs
+    std::vector<int> W, H;
+};
+
+static thread_local TickPlan *g_plan = nullptr;  // the plan being captured or launched
+
+static void launch_tick(ff_ctx *, int, int)
+{
+    TickPlan &P = *g_plan;
+    BatchState *b = P.b;
+    for (int i : P.dev) launch_alloc(b->ctxs[i], P.W[i], P.H[i]);
+    for (int i : P.dev)
+        if (b->ctxs[i]->learned) learned_launch(b->ctxs[i], P.W[i], P.H[i]);
+    const int nd = (int)P.dev.size();
+    launch_sdf_batch(P.st, b->d_ss, nd, b->d_prefix, b->tpb, b->bpw);
+    launch_gather_fc(P.st, b->d_ss, nd, b->d_fcs);
+    hipMemcpyAsync(b->h_fcs, b->d_fcs, sizeof(FrameCounters) * nd, hipMemcpyDeviceToHost, P.st);
+}
+
+static uint64_t tick_key(const TickPlan &P)
+{
+    uint64_t k = 1469598103934665603ull;
+    auto mix = [&](uint64_t v) { k = (k ^ v) * 1099511628211ull; };
+    mix((uint64_t)P.b->tpb);
+    mix((uint64_t)P.b->bpw);
+    for (int i : P.dev) {
+        const ff_ctx *c = P.b->ctxs[i];
+        mix((uint64_t)i);
+        mix((uint64_t)P.W[i] << 32 | (uint32_t)P.H[i]);
+        mix((uint64_t)c->bufGen << 1 | (c->learned != nullptr));
+    }
+    return k;
+}
+
+}  // namespace ff
+
+using namespace ff;
+
+extern "C" {
+
+int ff_add_map_batch(ff_ctx *const *ctxs, const ff_frame *frames, int n)
+{
+    if (!ctxs || !frames || n <= 0 || n > FF_BATCH_MAX) return FF_E_ARG;
+    for (int i = 0; i < n; i++) {
+        if (!ctxs[i]) return FF_E_ARG;
+        for (int j = 0; j < i; j++)
+            if (ctxs[j] == ctxs[i]) return FF_E_ARG;
+        if (ctxs[i]->device != ctxs[0]->device) {
+            ctxs[0]->err = "ff_add_map_batch: all contexts must be on one device";
+            return FF_E_ARG;
+        }
+    }
+    ff_ctx *lead = ctxs[0];
+    hipSetDevice(lead->device);
+    BatchState *b = batch_for(ctxs, n);
+    if (!b) {
+        lead->err = "ff_add_map_batch: out of device memory";
+        return FF_E_DEVICE;
+    }
+    const auto t0 = std::chrono::steady_clock::now();
+    // every context runs on the lead's stream for the tick
+    const hipStream_t st = lead->stream;
+    std::vector<hipStream_t> own(n);
+    for (int i = 0; i < n; i++) {
+        own[i] = ctxs[i]->stream;
+        if (own[i] != st) hipStreamSynchronize(own[i]);
+        ctxs[i]->stream = st;
+    }
+    TickPlan P;
+    P.b = b;
+    P.st = st;
+    P.W.assign(n, 0);
+    P.H.assign(n, 0);
+    std::vector<int> empty;
+    int rc = FF_OK;
+    for (int i = 0; i < n; i++) {
+        ff_ctx *c = ctxs[i];
+        const ff_frame &f = frames[i];
+        if (!f.depth || (c->useColor && !f.bgr)) {
+            c->err = "ff_add_map_batch: missing depth or color image";
+            if (rc == FF_OK) rc = FF_E_ARG;
+            continue;
+        }
+        int r = ensure_image(c, f.width, f.height);
+        if (!r) r = frame_prepare(c, f.depth, c->useColor ? f.bgr : nullptr, f.width, f.height, f.R, f.t, f.K,
+                                  f.scaling, f.maxcam);
+        if (r) {
+            if (rc == FF_OK) rc = r;
+            continue;
+        }
+        P.W[i] = f.width;
+        P.H[i] = f.height;
+        if (c->n) P.dev.push_back(i);
+        else empty.push_back(i);
+    }
+    // stream descriptors of the device work
+    const int nd = (int)P.dev.size();
+    for (int k = 0; k < nd; k++) {
+        ff_ctx *c = ctxs[P.dev[k]];
+        SdfStream &s = b->h_ss[k];
+        s.queue = c->d_queue;
+        s.m = c->mapView();
+        s.fd = c->d_fd;
+        s.fc = c->d_fc;
+        s.mapc = c->d_mapc;
+        s.gate.listCap = c->hv.listCap;
+        s.gate.leavesTotal = c->nLeavesTotal;
+        s.gate.branchesTotal = c->nBranchesTotal;
+        s.gate.force = 0;
+        s.W = P.W[P.dev[k]];
+        s.H = P.H[P.dev[k]];
+    }
+    TuneCand *cand = nullptr;
+    if (b->tuning && !b->cands.empty()) {
+        cand = &b->cands[b->cur];
+        b->tpb = cand->tpb;
+        b->bpw = cand->bpw;
+        b->graphs = cand->graphs;
+    }
+    if (nd) hipMemcpyAsync(b->d_ss, b->h_ss, sizeof(SdfStream) * nd, hipMemcpyHostToDevice, st);
+    for (int i : empty) launch_empty(ctxs[i], P.W[i], P.H[i]);
+    const auto t1 = std::chrono::steady_clock::now();
+    g_plan = &P;
+    if (nd) {
+        bool launched = false;
+        if (b->graphs) {
+            const uint64_t key = tick_key(P);
+            if (!b->exec || b->key != key) {
+                drop_graph(b);
+                hipError_t e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
+                if (e == hipSuccess) {
+                    launch_tick(nullptr, 0, 0);
+                    e = hipStreamEndCapture(st, &b->graph);
+                }
+                if (e == hipSuccess) e = hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0);
+                if (e != hipSuccess) {
+                    (void)hipGetLastError();
+                    drop_graph(b);
+                } else {
+                    b->key = key;
+                }
+            }
+            if (b->exec && hipGraphLaunch(b->exec, st) == hipSuccess) launched = true;
+        }
+        if (!launched) launch_tick(nullptr, 0, 0);
+    }
+    g_plan = nullptr;
+    hipError_t e = hipStreamSynchronize(st);
+    const auto t2 = std::chrono::steady_clock::now();
+    bool devOk = e == hipSuccess;
+    if (!devOk) {
+        lead->err = std::string("ff_add_map_batch: ") + hipGetErrorString(e);
+        rc = FF_E_DEVICE;
+    }
+    if (devOk) {
+        for (int k = 0; k < nd; k++) memcpy(ctxs[P.dev[k]]->h_fc, &b->h_fcs[k], sizeof(FrameCounters));
+        for (int i = 0; i < n; i++) {
+            if (!P.W[i]) continue;
+            const int r = frame_finish(ctxs[i], P.W[i], P.H[i], false);
+            if (r < 0 && rc == FF_OK) rc = r;
+        }
+    }
+    for (int i = 0; i < n; i++) ctxs[i]->stream = own[i];
+    // autotuning: the device part of each candidate's ticks, first tick of a
+    // candidate (capture, warm-up) not counted
+    if (cand && devOk) {
+        const double ms = std::chrono::duration<double, std::milli>(t2 - t1).count();
+        if (cand->runs++ > 0) cand->ms += ms;
+        if (cand->runs > b->ticksPerCand) {
+            b->cur++;
+            if (b->cur >= b->cands.size()) {
+                size_t best = 0;
+                for (size_t k = 1; k < b->cands.size(); k++)
+                    if (b->cands[k].ms < b->cands[best].ms) best = k;
+                b->tpb = b->cands[best].tpb;
+                b->bpw = b->cands[best].bpw;
+                b->graphs = b->cands[best].graphs;
+                b->tuning = false;
+            }
+        }
+    }
+    (void)t0;
+    b->ticks++;
+    return rc;
+}
+
+int ff_batch_set_config(ff_ctx *lead, int tpb, int bpw, int graphs)
+{
+    if (!lead || (tpb != 512 && tpb != 256) || bpw < 1 || bpw > 64) return FF_E_ARG;
+    if (!lead->batch) {
+        lead->batch = new BatchState();
+        default_cands(lead->batch);
+    }
+    BatchState *b = lead->batch;
+    drop_graph(b);
+    b->tpb = tpb;
+    b->bpw = bpw;
+    b->graphs = graphs != 0;
+    b->tuning = false;
+    return FF_OK;
+}
+
+int ff_batch_autotune(ff_ctx *lead, int ticks_per_candidate)
+{
+    if (!lead || ticks_per_candidate < 1) return FF_E_ARG;
+    if (!lead->batch) lead->batch = new BatchState();
+    BatchState *b = lead->batch;
+    drop_graph(b);
+    default_cands(b);
+    b->ticksPerCand = ticks_per_candidate;
+    return FF_OK;
+}
+
+int ff_batch_get_config(const ff_ctx *lead, int *tpb, int *bpw, int *graphs, int *tuning)
+{
+    if (!lead) return FF_E_ARG;
+    const BatchState *b = lead->batch;
+    if (tpb) *tpb = b ? b->tpb : 512;
+    if (bpw) *bpw = b ? b->bpw : 1;
+    if (graphs) *graphs = b ? (int)b->graphs : 0;
+    if (tuning) *tuning = b ? (int)b->tuning : 1;
+    return FF_OK;
+}
+
+}  // extern "C"

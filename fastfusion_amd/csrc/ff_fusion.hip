@@ -670,11 +670,12 @@ __global__ void k_shift_leaves(ushort4 *info, uint32_t n, uint32_t ax, uint32_t 
 // rebuilds its row origin with the same sequential adds as the reference
 // (z steps, then y steps) and adds k * d1j like the AVX lane k.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ void sdf_brick(uint32_t leaf, const MapView &m, const uint16_t *__restrict__ depth,
+// voxel v (z-major, 0..511) of brick `leaf`
+__device__ __forceinline__ void sdf_voxel(uint32_t leaf, int v, const MapView &m, const uint16_t *__restrict__ depth,
                                           const uint8_t *__restrict__ bgr, int W, int H, const SdfParams &P,
                                           const uint32_t *tab)
 {
-    const int tid = threadIdx.x;
+    const int tid = v;
     const int zr = tid >> 6, yr = (tid >> 3) & 7, k = tid & 7;
     const float EPS = 0.005f;
     const ushort4 info = m.leafinfo[leaf];
@@ -780,9 +781,85 @@ __global__ __launch_bounds__(512) void k_sdf(const uint32_t *__restrict__ queue,
     for (int i = threadIdx.x; i < 2048; i += blockDim.x) tab[i] = c_rcp_table[i];
     __syncthreads();  // rcp table ready
     for (uint32_t b = blockIdx.x; b < nQ; b += gridDim.x)
-        sdf_brick(queue[b], m, depth, bgr, W, H, P, tab);
+        sdf_voxel(queue[b], threadIdx.x, m, depth, bgr, W, H, P, tab);
 }
 
+
+
+// ---------------------------------------------------------------------------
+// batched SDF update: the queued bricks of n streams as one worklist
+// ---------------------------------------------------------------------------
+// per stream: its gate, its count (0 when the gate is closed), the exclusive
+// prefix over streams (n <= 64: one wave)
+__global__ void k_batch_prefix(const SdfStream *__restrict__ ss, int n, uint32_t *__restrict__ prefix)
+{
+    const int lane = threadIdx.x;
+    uint32_t cnt = 0;
+    if (lane < n) {
+        const SdfStream &S = ss[lane];
+        const bool open = sdf_gate_open(S.fc, S.mapc, S.fd->n, S.gate);
+        cnt = open ? S.fc->nQueued : 0u;
+        S.fc->sdfRan = open ? 1u : 0u;
+    }
+    uint32_t x = cnt;
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_up(x, o);
+        if (lane >= o) x += y;
+    }
+    if (lane < n) prefix[lane] = x - cnt;
+    if (lane == 63) prefix[n] = x;
+}
+
+// blocks take bpw consecutive bricks of the worklist at a time; tpb threads
+// per brick, 512 / tpb voxels per thread
+template <int TPB>
+__global__ __launch_bounds__(TPB) void k_sdf_batch(const SdfStream *__restrict__ ss, int n,
+                                                   const uint32_t *__restrict__ prefix, int bpw)
+{
+    const uint32_t total = prefix[n];
+    if ((uint32_t)blockIdx.x * (uint32_t)bpw >= total) return;
+    __shared__ uint32_t tab[2048];
+    for (int i = threadIdx.x; i < 2048; i += TPB) tab[i] = c_rcp_table[i];
+    __syncthreads();
+    for (uint32_t chunk = blockIdx.x; chunk * (uint32_t)bpw < total; chunk += gridDim.x) {
+        for (int k = 0; k < bpw; k++) {
+            const uint32_t item = chunk * (uint32_t)bpw + (uint32_t)k;
+            if (item >= total) break;
+            int lo = 0, hi = n;  // prefix[lo] <= item < prefix[hi]
+            while (hi - lo > 1) {
+                const int mid = (lo + hi) >> 1;
+                if (prefix[mid] <= item) lo = mid;
+                else hi = mid;
+            }
+            const SdfStream &S = ss[lo];
+            const FrameDev *fd = S.fd;
+            const uint32_t leaf = S.queue[item - prefix[lo]];
+#pragma unroll
+            for (int v = threadIdx.x; v < FF_BRICK; v += TPB)
+                sdf_voxel(leaf, v, S.m, fd->depth, fd->bgr, S.W, S.H, fd->sp, tab);
+        }
+    }
+}
+
+__global__ void k_gather_fc(const SdfStream *__restrict__ ss, int n, FrameCounters *__restrict__ out)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = *ss[i].fc;
+}
+
+#define FF_SDF_BATCH_GRID 2048
+
+void launch_sdf_batch(hipStream_t st, const SdfStream *d_ss, int n, uint32_t *d_prefix, int tpb, int bpw)
+{
+    k_batch_prefix<<<1, 64, 0, st>>>(d_ss, n, d_prefix);
+    if (tpb == 256) k_sdf_batch<256><<<FF_SDF_BATCH_GRID, 256, 0, st>>>(d_ss, n, d_prefix, bpw);
+    else k_sdf_batch<512><<<FF_SDF_BATCH_GRID, 512, 0, st>>>(d_ss, n, d_prefix, bpw);
+}
+
+void launch_gather_fc(hipStream_t st, const SdfStream *d_ss, int n, FrameCounters *d_out)
+{
+    k_gather_fc<<<1, 64, 0, st>>>(d_ss, n, d_out);
+}
 
 }  // namespace ff
 
@@ -832,7 +909,7 @@ static CamF make_pinv(const ff_ctx *c, const double R[9], const double t[3], con
     return q;
 }
 
-static int ensure_image(ff_ctx *c, int W, int H)
+int ensure_image(ff_ctx *c, int W, int H)
 {
     if (W <= 0 || H <= 0 || W > 2048) {
         c->err = "image width must be in [1, 2048]";
@@ -1054,6 +1131,7 @@ void ff_destroy(ff_ctx *c)
     if (!c) return;
     hipSetDevice(c->device);
     if (c->stream) hipStreamSynchronize(c->stream);
+    batch_free(c);
     mesh_free(c);
     learned_free(c);
     c->freeFrameBuffers();
@@ -1074,11 +1152,15 @@ const char *ff_last_error(const ff_ctx *c) { return c ? c->err.c_str() : "null c
 
 void *ff_stream(ff_ctx *c) { return c ? (void *)c->stream : nullptr; }
 
+}  // extern "C"
+
+namespace ff {
+
 // A frame's device work reads every per-frame value from c->d_fd, so that
 // it can be captured once and replayed as graphs: the allocation (ending
 // with the commit of the used counts), then the learned confidence when on,
 // the gated SDF update and the counters' copy back.
-static void launch_alloc(ff_ctx *c, int W, int H)
+void launch_alloc(ff_ctx *c, int W, int H)
 {
     const int words = (W + 31) / 32;
     const uint32_t npix = (uint32_t)W * H;
@@ -1109,7 +1191,7 @@ static void launch_alloc(ff_ctx *c, int W, int H)
 
 // the SDF update waits for the host only when the node table overflowed or
 // the volume must grow first
-static void launch_sdf(ff_ctx *c, int W, int H)
+void launch_sdf(ff_ctx *c, int W, int H)
 {
     if (c->learned) learned_launch(c, W, H);
     SdfGate gate;
@@ -1165,9 +1247,11 @@ static int run_frame(ff_ctx *c, int W, int H)
     return FF_OK;
 }
 
-// depth / bgr: the frame in device memory, dense rows
-static int add_map_impl(ff_ctx *c, const uint16_t *depth, const uint8_t *bgr, int W, int H, const double R[9],
-                        const double t[3], const double K[4], float scaling, float maxcam)
+// Host part of a frame up to the upload of its parameters (first-frame
+// volume, column/row tables, pose).  depth / bgr: the frame in device
+// memory, dense rows.
+int frame_prepare(ff_ctx *c, const uint16_t *depth, const uint8_t *bgr, int W, int H, const double R[9],
+                  const double t[3], const double K[4], float scaling, float maxcam)
 {
     CamF p = make_p(c, R, t, K);
     FrameCounters *hf = c->h_fc;
@@ -1231,17 +1315,24 @@ static int add_map_impl(ff_ctx *c, const uint16_t *depth, const uint8_t *bgr, in
     f.bandwidth = c->bandwidth;
     f.pad = 0;
     HIPCHK(hipMemcpyAsync(c->d_fd, c->h_fd, sizeof(FrameDev), hipMemcpyHostToDevice, c->stream));
-    if (c->n) {
-        int rc = run_frame(c, W, H);
-        if (rc) return rc;
-    } else {
-        k_frame_begin<<<(std::max(W, H) + 255) / 256, 256, 0, c->stream>>>(c->d_pxp, c->d_pyp, W, H, c->d_fd,
-                                                                           c->d_qx, c->d_qy, c->d_fc);
-        HIPCHK(hipMemsetAsync(c->d_pbits, 0, pbits_bytes(H), c->stream));
-        HIPCHK(hipMemcpyAsync(hf, c->d_fc, sizeof(FrameCounters), hipMemcpyDeviceToHost, c->stream));
-    }
-    HIPCHK(hipStreamSynchronize(c->stream));
-    if (c->timing && c->n) {
+    return FF_OK;
+}
+
+// the device work of a frame without a volume (an empty first frame)
+void launch_empty(ff_ctx *c, int W, int H)
+{
+    k_frame_begin<<<(std::max(W, H) + 255) / 256, 256, 0, c->stream>>>(c->d_pxp, c->d_pyp, W, H, c->d_fd, c->d_qx,
+                                                                       c->d_qy, c->d_fc);
+    hipMemsetAsync(c->d_pbits, 0, pbits_bytes(H), c->stream);
+    hipMemcpyAsync(c->h_fc, c->d_fc, sizeof(FrameCounters), hipMemcpyDeviceToHost, c->stream);
+}
+
+// Host part after the frame's device work completed and c->h_fc holds its
+// counters: errors, used counts, grow with the deferred SDF update, bytes.
+int frame_finish(ff_ctx *c, int W, int H, bool timed)
+{
+    FrameCounters *hf = c->h_fc;
+    if (timed && c->timing && c->n) {
         float ms = 0.0f;
         if (hipEventElapsedTime(&ms, c->ev[0], c->ev[1]) == hipSuccess) {
             c->phaseMs[0] += ms;
@@ -1307,6 +1398,25 @@ static int add_map_impl(ff_ctx *c, const uint16_t *depth, const uint8_t *bgr, in
     mesh_after_frame(c);
     return (int)c->nQueued;
 }
+
+static int add_map_impl(ff_ctx *c, const uint16_t *depth, const uint8_t *bgr, int W, int H, const double R[9],
+                        const double t[3], const double K[4], float scaling, float maxcam)
+{
+    int rc = frame_prepare(c, depth, bgr, W, H, R, t, K, scaling, maxcam);
+    if (rc) return rc;
+    if (c->n) {
+        rc = run_frame(c, W, H);
+        if (rc) return rc;
+    } else {
+        launch_empty(c, W, H);
+    }
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return frame_finish(c, W, H, true);
+}
+
+}  // namespace ff
+
+extern "C" {
 
 int ff_add_map(ff_ctx *c, const uint16_t *depth, size_t depth_stride, const uint8_t *bgr, size_t bgr_stride,
                int W, int H, const double R[9], const double t[3], const double K[4], float scaling, float maxcam)

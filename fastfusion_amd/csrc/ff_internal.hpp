@@ -84,8 +84,20 @@ struct MapCounters {
     uint32_t nBranchesUsed, nLeavesUsed, pad[2];
 };
 
+// one stream of a batched SDF launch
+struct SdfStream {
+    const uint32_t *queue;
+    MapView m;
+    const FrameDev *fd;
+    FrameCounters *fc;
+    const MapCounters *mapc;
+    SdfGate gate;
+    int W, H;
+};
+
 struct MeshState;
 struct LearnedState;
+struct BatchState;
 
 }  // namespace ff
 
@@ -128,6 +140,7 @@ struct ff_ctx {
     ff::HashView hv{};
     ff::MeshState *mesh = nullptr;
     ff::LearnedState *learned = nullptr;  // learned confidence (off: nullptr)
+    ff::BatchState *batch = nullptr;      // batched ticks led by this context
     std::string err;
     // per-phase timing (0 allocation, 1 SDF update, 2 meshing)
     bool timing = false;
@@ -154,6 +167,22 @@ const float *learned_prepare(ff_ctx *c, int W, int H);
 void learned_launch(ff_ctx *c, int W, int H);
 void sync_map_counters(ff_ctx *c);
 void drop_frame_graphs(ff_ctx *c);
+// a frame in phases (ff_fusion.hip): host preparation and parameter upload,
+// device work (allocation, then confidence + SDF; or the empty-frame path),
+// and the host part after the device work completed
+int frame_prepare(ff_ctx *c, const uint16_t *depth, const uint8_t *bgr, int W, int H, const double R[9],
+                  const double t[3], const double K[4], float scaling, float maxcam);
+void launch_alloc(ff_ctx *c, int W, int H);
+void launch_sdf(ff_ctx *c, int W, int H);
+void launch_empty(ff_ctx *c, int W, int H);
+int frame_finish(ff_ctx *c, int W, int H, bool timed);
+int ensure_image(ff_ctx *c, int W, int H);
+// batched SDF update over the concatenated brick lists of n streams
+// (ff_fusion.hip): per-stream gates and counts into prefix[0..n], then one
+// launch; threads per brick tpb (512 or 256), bricks per workgroup bpw
+void launch_sdf_batch(hipStream_t st, const SdfStream *d_ss, int n, uint32_t *d_prefix, int tpb, int bpw);
+void launch_gather_fc(hipStream_t st, const SdfStream *d_ss, int n, FrameCounters *d_out);
+void batch_free(ff_ctx *c);
 // meshing module hooks (ff_mesh.hip)
 int mesh_init(ff_ctx *c);
 void mesh_free(ff_ctx *c);

@@ -86,6 +86,17 @@ __constant__ uint8_t c_ntri[256];
 
 #define FF_TILE_CAP_MAX (1u << 22)  // 4 GiB of samples
 
+// read-only view of the map for sampling
+struct MapRO {
+    const ushort4 *info;     // pos, scale
+    const uint32_t *parent;
+    const float *dist, *wgt;
+    const uint16_t *col;     // R, G, B planes of 512 per brick
+    float off[3];
+    float scale;
+    int useColor;
+};
+
 struct MeshState {
     ushort4 *d_binfo = nullptr;  // per branch block: origin of the node, octant size
     uint32_t binfoCap = 0;
@@ -109,6 +120,12 @@ struct MeshState {
     unsigned long long *d_arena = nullptr;
     MeshCounters *d_mc = nullptr, *h_mc = nullptr;
     uint32_t lastMeshStamp = 0;
+    int nxt = 1;   // table being built by the update in flight
+    MapRO m{};     // its map view
+    // batched updates led by this context: stream descriptors and prefix
+    struct McStream *d_ms = nullptr, *h_ms = nullptr;
+    uint32_t *d_msPrefix = nullptr;
+    int msCap = 0;
     uint64_t lastVAlloc = 0, lastFAlloc = 0;  // arena totals after the previous update (timing)
     bool haveVolume = false;
 };
@@ -354,15 +371,6 @@ __device__ __forceinline__ bool chain_dirty(const uint32_t *__restrict__ stamp, 
 // ---------------------------------------------------------------------------
 // sampling (per cell type)
 // ---------------------------------------------------------------------------
-struct MapRO {
-    const ushort4 *info;     // pos, scale
-    const uint32_t *parent;
-    const float *dist, *wgt;
-    const uint16_t *col;     // R, G, B planes of 512 per brick
-    float off[3];
-    float scale;
-    int useColor;
-};
 
 struct Sample {
     float d, w;
@@ -826,8 +834,210 @@ __device__ __forceinline__ void put_sample(WaveLds &L, const CellGeom &g, int s,
     L.sf[id] = (uint8_t)((d < 0.0f ? 1u : 0u) | (w == 0.0f ? 2u : 0u));
 }
 
+// One dirty cell, by one wave (lane 0..63) with its LDS slice: samples into
+// LDS, cube classification, edge numbering, vertices and faces.  di is the
+// cell's dirty index (it picks the pool arena); act false for a wave without
+// a cell (it still takes part in the wave-wide steps).
 // InPlace = false: cells whose samples k_sample wrote to the buffer;
 // InPlace = true: cells that did not fit the buffer, sampled by the wave itself
+template <bool InPlace>
+__device__ void mc_cell(WaveLds &L, int lane, CellRec *cells, const CellGeom &g, MeshCounters *mc,
+                        const uint4 *__restrict__ sbuf, const MapRO &m, const Pools &P, uint32_t di, bool act)
+{
+    // the wave's cell is uniform: skip cells of the other pass (empty
+    // cells keep the zero counts k_cells gave them)
+    if (InPlace ? g.tile0 != FF_NO_TILE : g.tile0 >= FF_EMPTY_CELL) return;
+    const int type = g.type;
+    const int n[3] = {g.n[0], g.n[1], g.n[2]};
+    const int total = n[0] * n[1] * n[2];
+    if (!InPlace) {
+        const uint4 *src = sbuf + (size_t)g.tile0 * 64u;
+        for (int s = lane; s < total; s += 64) {
+            const uint4 v = src[s];
+            put_sample(L, g, s, __uint_as_float(v.x), __uint_as_float(v.y), v.z, v.w);
+        }
+    } else {
+        const uint32_t *ll = cells[g.cell].ll;
+        for (int s = lane; s < total; s += 64) {
+            const uint4 v = pack_sample(sample_cell(m, g, ll, s));
+            put_sample(L, g, s, __uint_as_float(v.x), __uint_as_float(v.y), v.z, v.w);
+        }
+    }
+    if (lane < 48) L.used[lane] = 0u;
+    wave_sync();
+    CellRec *cp = act ? &cells[g.cell] : nullptr;
+    const int sizeMin = g.sizeMin, stretch = g.stretch;
+    bool full[3];
+    for (int k = 0; k < 3; k++) full[k] = axis_full(type, k);
+    const int elen[3] = {full[0] ? sizeMin : stretch, full[1] ? sizeMin : stretch, full[2] ? sizeMin : stretch};
+    uint32_t nv = 0, nf = 0;
+    if (type == 7) {
+        // non-indexed corner cell with degenerate triangles dropped
+        // (getColoredTrianglesForCubePrecomputed, mesh.cpp:1931-2008)
+        int ci = 0;
+        bool veto = false;
+        for (int k = 0; k < 8; k++) {
+            const int id = (c_corner[k][2] * FF_BL + c_corner[k][1]) * FF_BL + c_corner[k][0];
+            if (total) {
+                const uint32_t f = L.sf[id];
+                if (f & 2u) veto = true;
+                if (f & 1u) ci |= 1 << k;
+            }
+        }
+        float basev[3];
+        const float sx = m.scale * (float)stretch;
+        for (int a = 0; a < 3; a++) basev[a] = m.off[a] + (float)geom_coord(g, a, 0) * m.scale;
+        if (veto || !total) ci = 0;
+        const int ntri = c_ntri[ci];
+        bool keep = false;
+        float V[3][3];
+        uint32_t C[3];
+        if (lane < ntri) {
+            for (int q = 0; q < 3; q++) {
+                const int e = c_tri[ci][3 * lane + q];
+                const int a = c_edgec[e][0], b = c_edgec[e][1];
+                const int ia = (c_corner[a][2] * FF_BL + c_corner[a][1]) * FF_BL + c_corner[a][0];
+                const int ib = (c_corner[b][2] * FF_BL + c_corner[b][1]) * FF_BL + c_corner[b][0];
+                float Pa[3], Pb[3];
+                for (int k = 0; k < 3; k++) {
+                    Pa[k] = c_corner[a][k] ? basev[k] + 1.0f * sx : basev[k] + 0.0f * sx;
+                    Pb[k] = c_corner[b][k] ? basev[k] + 1.0f * sx : basev[k] + 0.0f * sx;
+                }
+                vinterp(Pa, Pb, L.d[ia], L.d[ib], V[q]);
+                const uint16_t ca[3] = {L.c[0][ia], L.c[1][ia], L.c[2][ia]};
+                const uint16_t cb[3] = {L.c[0][ib], L.c[1][ib], L.c[2][ib]};
+                C[q] = cinterp(ca, cb, L.d[ia], L.d[ib]);
+            }
+            keep = true;
+            for (int q = 0; q < 3; q++) {
+                const float *A = V[q], *B = V[(q + 1) % 3];
+                if (A[0] == B[0] && A[1] == B[1] && A[2] == B[2]) keep = false;
+            }
+        }
+        const uint64_t kb = __ballot(keep);
+        const uint32_t rank = __popcll(kb & ((1ull << lane) - 1ull));
+        nf = __popcll(kb);
+        nv = 3 * nf;
+        uint32_t v0 = 0, f0 = 0, ok = 1;
+        if (lane == 0) ok = pool_alloc(P, di % FF_ARENAS, nv, nf, v0, f0, mc) ? 1u : 0u;
+        v0 = __shfl(v0, 0);
+        f0 = __shfl(f0, 0);
+        const bool fits = __shfl(ok, 0) != 0;
+        if (keep && fits) {
+            for (int q = 0; q < 3; q++) {
+                P.v[v0 + 3 * rank + q] = make_float4(V[q][0], V[q][1], V[q][2], __uint_as_float(C[q]));
+                P.f[3 * (f0 + rank) + q] = 3 * rank + q;
+            }
+        }
+        if (lane == 0) {
+            if (!fits) nv = nf = 0;
+            cp->v0 = v0; cp->nv = nv; cp->f0 = f0; cp->nf = nf;
+        }
+        wave_sync();
+        return;
+    }
+    // -- cubes: index, used edges, triangle counts
+    const int nc0 = max(n[0] - 1, 0), nc1 = max(n[1] - 1, 0), nc2 = max(n[2] - 1, 0);
+    const int ncubes = nc0 * nc1 * nc2;
+    for (int q = lane; q < ncubes; q += 64) {
+        const int i = q % nc0, j = (q / nc0) % nc1, l = q / (nc0 * nc1);
+        const int b0 = (l * FF_BL + j) * FF_BL + i;
+        int ci = 0;
+        uint32_t zw = 0;
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            const uint32_t f = L.sf[b0 + (c_corner[k][2] * FF_BL + c_corner[k][1]) * FF_BL + c_corner[k][0]];
+            zw |= f;
+            ci |= (int)(f & 1u) << k;
+        }
+        if (zw & 2u) ci = 0;
+        L.cube[q] = (uint8_t)ci;
+        const int nt = c_ntri[ci];
+        for (int t = 0; t < 3 * nt; t++) {
+            const int e = c_tri[ci][t];
+            const int slot = (b0 + (c_edge[e][2] * FF_BL + c_edge[e][1]) * FF_BL + c_edge[e][0]) * 3 + c_edge[e][3];
+            atomicOr(&L.used[slot >> 5], 1u << (slot & 31));
+        }
+    }
+    wave_sync();
+    // -- vertex numbering over the used-edge bitmap, and the slot list
+    {
+        const uint32_t wd = lane < 48 ? L.used[lane] : 0u;
+        uint32_t tot;
+        const uint32_t ex = wave_excl_scan_u32((uint32_t)__popc(wd), &tot);
+        if (lane < 48) {
+            L.wpref[lane] = ex;
+            uint32_t bits = wd, k = ex;
+            while (bits) {
+                const uint32_t b = (uint32_t)__ffs(bits) - 1u;
+                L.list[k++] = (uint16_t)(lane * 32 + b);
+                bits &= bits - 1u;
+            }
+        }
+        nv = tot;
+    }
+    // face offsets per cube: contiguous chunks of 6 cubes per lane
+    {
+        const int per = 6;
+        uint32_t loc = 0;
+        for (int q = lane * per; q < min(ncubes, lane * per + per); q++) loc += c_ntri[L.cube[q]];
+        uint32_t tot;
+        uint32_t ex = wave_excl_scan_u32(loc, &tot);
+        for (int q = lane * per; q < min(ncubes, lane * per + per); q++) {
+            L.foff[q] = (uint16_t)ex;
+            ex += c_ntri[L.cube[q]];
+        }
+        nf = tot;
+    }
+    wave_sync();
+    uint32_t v0 = 0, f0 = 0, ok = 1;
+    if (lane == 0) ok = pool_alloc(P, di % FF_ARENAS, nv, nf, v0, f0, mc) ? 1u : 0u;
+    v0 = __shfl(v0, 0);
+    f0 = __shfl(f0, 0);
+    const bool fits = __shfl(ok, 0) != 0;
+    if (fits) {
+        // vertices: one used edge per lane
+        for (uint32_t vi = lane; vi < nv; vi += 64) {
+            const int slot = L.list[vi], a = slot % 3, id = slot / 3;
+            const int gg[3] = {id & 7, (id >> 3) & 7, id >> 6};
+            const int id2 = id + (a == 0 ? 1 : (a == 1 ? FF_BL : FF_BL * FF_BL));
+            float p1[3], p2[3];
+#pragma unroll
+            for (int k = 0; k < 3; k++) {
+                const int ck = geom_coord(g, k, gg[k]);
+                p1[k] = m.off[k] + (float)ck * m.scale;
+                p2[k] = k == a ? m.off[k] + (float)(ck + elen[k]) * m.scale : p1[k];
+            }
+            float V[3];
+            vinterp(p1, p2, L.d[id], L.d[id2], V);
+            const uint16_t ca[3] = {L.c[0][id], L.c[1][id], L.c[2][id]};
+            const uint16_t cb[3] = {L.c[0][id2], L.c[1][id2], L.c[2][id2]};
+            const uint32_t rgb = cinterp(ca, cb, L.d[id], L.d[id2]);
+            P.v[v0 + vi] = make_float4(V[0], V[1], V[2], __uint_as_float(rgb));
+        }
+        // faces
+        for (int q = lane; q < ncubes; q += 64) {
+            const int ci = L.cube[q];
+            const int nt = c_ntri[ci];
+            if (!nt) continue;
+            const int i = q % nc0, j = (q / nc0) % nc1, l = q / (nc0 * nc1);
+            const int b0 = (l * FF_BL + j) * FF_BL + i;
+            const uint32_t fb = f0 + L.foff[q];
+            for (int t = 0; t < 3 * nt; t++) {
+                const int e = c_tri[ci][t];
+                const int slot = (b0 + (c_edge[e][2] * FF_BL + c_edge[e][1]) * FF_BL + c_edge[e][0]) * 3 + c_edge[e][3];
+                const uint32_t wd = L.used[slot >> 5];
+                P.f[3 * fb + t] = L.wpref[slot >> 5] + __popc(wd & ((1u << (slot & 31)) - 1u));
+            }
+        }
+    }
+    if (lane == 0) {
+        if (!fits) nv = nf = 0;
+        cp->v0 = v0; cp->nv = nv; cp->f0 = f0; cp->nf = nf;
+    }
+    wave_sync();
+}
+
 template <bool InPlace>
 __global__ __launch_bounds__(256) void k_mc(CellRec *cells, const CellGeom *__restrict__ geom, MeshCounters *mc,
                                             const uint4 *__restrict__ sbuf, MapRO m, Pools P)
@@ -849,198 +1059,63 @@ __global__ __launch_bounds__(256) void k_mc(CellRec *cells, const CellGeom *__re
             g.o[0] = g.o[1] = g.o[2] = 0;
             g.size = g.sizeMin = g.stretch = 1;
         }
-        // the wave's cell is uniform: skip cells of the other pass (empty
-        // cells keep the zero counts k_cells gave them)
-        if (InPlace ? g.tile0 != FF_NO_TILE : g.tile0 >= FF_EMPTY_CELL) continue;
-        const int type = g.type;
-        const int n[3] = {g.n[0], g.n[1], g.n[2]};
-        const int total = n[0] * n[1] * n[2];
-        if (!InPlace) {
-            const uint4 *src = sbuf + (size_t)g.tile0 * 64u;
-            for (int s = lane; s < total; s += 64) {
-                const uint4 v = src[s];
-                put_sample(L, g, s, __uint_as_float(v.x), __uint_as_float(v.y), v.z, v.w);
-            }
+        mc_cell<InPlace>(L, lane, cells, g, mc, sbuf, m, P, di, act);
+    }
+}
+
+// one stream of a batched marching-cubes launch
+struct McStream {
+    CellRec *cells;
+    const CellGeom *geom;
+    MeshCounters *mc;
+    const uint4 *sbuf;
+    MapRO m;
+    Pools P;
+};
+
+// exclusive prefix of the streams' dirty-cell counts (n <= 64: one wave)
+__global__ void k_mc_prefix(const McStream *__restrict__ ms, int n, uint32_t *__restrict__ prefix)
+{
+    const int lane = threadIdx.x;
+    const uint32_t cnt = lane < n ? ms[lane].mc->nDirty : 0u;
+    uint32_t x = cnt;
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_up(x, o);
+        if (lane >= o) x += y;
+    }
+    if (lane < n) prefix[lane] = x - cnt;
+    if (lane == 63) prefix[n] = x;
+}
+
+// the dirty cells of n streams as one worklist, one wave per cell
+__global__ __launch_bounds__(256) void k_mc_batch(const McStream *__restrict__ ms, int n,
+                                                  const uint32_t *__restrict__ prefix)
+{
+    __shared__ WaveLds lds[4];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    WaveLds &L = lds[wave];
+    const uint32_t total = prefix[n];
+    for (uint32_t base = blockIdx.x * 4; base < total; base += gridDim.x * 4) {
+        const uint32_t item = base + wave;
+        const bool act = item < total;
+        int lo = 0, hi = n;  // prefix[lo] <= item < prefix[hi]
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (prefix[mid] <= item) lo = mid;
+            else hi = mid;
+        }
+        const McStream &S = ms[lo];
+        const uint32_t di = act ? item - prefix[lo] : 0u;
+        CellGeom g;
+        if (act) {
+            g = S.geom[di];
         } else {
-            const uint32_t *ll = cells[g.cell].ll;
-            for (int s = lane; s < total; s += 64) {
-                const uint4 v = pack_sample(sample_cell(m, g, ll, s));
-                put_sample(L, g, s, __uint_as_float(v.x), __uint_as_float(v.y), v.z, v.w);
-            }
+            g.cell = 0; g.tile0 = FF_EMPTY_CELL; g.type = 0;
+            g.n[0] = g.n[1] = g.n[2] = 0;
+            g.o[0] = g.o[1] = g.o[2] = 0;
+            g.size = g.sizeMin = g.stretch = 1;
         }
-        if (lane < 48) L.used[lane] = 0u;
-        wave_sync();
-        CellRec *cp = act ? &cells[g.cell] : nullptr;
-        const int sizeMin = g.sizeMin, stretch = g.stretch;
-        bool full[3];
-        for (int k = 0; k < 3; k++) full[k] = axis_full(type, k);
-        const int elen[3] = {full[0] ? sizeMin : stretch, full[1] ? sizeMin : stretch, full[2] ? sizeMin : stretch};
-        uint32_t nv = 0, nf = 0;
-        if (type == 7) {
-            // non-indexed corner cell with degenerate triangles dropped
-            // (getColoredTrianglesForCubePrecomputed, mesh.cpp:1931-2008)
-            int ci = 0;
-            bool veto = false;
-            for (int k = 0; k < 8; k++) {
-                const int id = (c_corner[k][2] * FF_BL + c_corner[k][1]) * FF_BL + c_corner[k][0];
-                if (total) {
-                    const uint32_t f = L.sf[id];
-                    if (f & 2u) veto = true;
-                    if (f & 1u) ci |= 1 << k;
-                }
-            }
-            float basev[3];
-            const float sx = m.scale * (float)stretch;
-            for (int a = 0; a < 3; a++) basev[a] = m.off[a] + (float)geom_coord(g, a, 0) * m.scale;
-            if (veto || !total) ci = 0;
-            const int ntri = c_ntri[ci];
-            bool keep = false;
-            float V[3][3];
-            uint32_t C[3];
-            if (lane < ntri) {
-                for (int q = 0; q < 3; q++) {
-                    const int e = c_tri[ci][3 * lane + q];
-                    const int a = c_edgec[e][0], b = c_edgec[e][1];
-                    const int ia = (c_corner[a][2] * FF_BL + c_corner[a][1]) * FF_BL + c_corner[a][0];
-                    const int ib = (c_corner[b][2] * FF_BL + c_corner[b][1]) * FF_BL + c_corner[b][0];
-                    float Pa[3], Pb[3];
-                    for (int k = 0; k < 3; k++) {
-                        Pa[k] = c_corner[a][k] ? basev[k] + 1.0f * sx : basev[k] + 0.0f * sx;
-                        Pb[k] = c_corner[b][k] ? basev[k] + 1.0f * sx : basev[k] + 0.0f * sx;
-                    }
-                    vinterp(Pa, Pb, L.d[ia], L.d[ib], V[q]);
-                    const uint16_t ca[3] = {L.c[0][ia], L.c[1][ia], L.c[2][ia]};
-                    const uint16_t cb[3] = {L.c[0][ib], L.c[1][ib], L.c[2][ib]};
-                    C[q] = cinterp(ca, cb, L.d[ia], L.d[ib]);
-                }
-                keep = true;
-                for (int q = 0; q < 3; q++) {
-                    const float *A = V[q], *B = V[(q + 1) % 3];
-                    if (A[0] == B[0] && A[1] == B[1] && A[2] == B[2]) keep = false;
-                }
-            }
-            const uint64_t kb = __ballot(keep);
-            const uint32_t rank = __popcll(kb & ((1ull << lane) - 1ull));
-            nf = __popcll(kb);
-            nv = 3 * nf;
-            uint32_t v0 = 0, f0 = 0, ok = 1;
-            if (lane == 0) ok = pool_alloc(P, di % FF_ARENAS, nv, nf, v0, f0, mc) ? 1u : 0u;
-            v0 = __shfl(v0, 0);
-            f0 = __shfl(f0, 0);
-            const bool fits = __shfl(ok, 0) != 0;
-            if (keep && fits) {
-                for (int q = 0; q < 3; q++) {
-                    P.v[v0 + 3 * rank + q] = make_float4(V[q][0], V[q][1], V[q][2], __uint_as_float(C[q]));
-                    P.f[3 * (f0 + rank) + q] = 3 * rank + q;
-                }
-            }
-            if (lane == 0) {
-                if (!fits) nv = nf = 0;
-                cp->v0 = v0; cp->nv = nv; cp->f0 = f0; cp->nf = nf;
-            }
-            wave_sync();
-            continue;
-        }
-        // -- cubes: index, used edges, triangle counts
-        const int nc0 = max(n[0] - 1, 0), nc1 = max(n[1] - 1, 0), nc2 = max(n[2] - 1, 0);
-        const int ncubes = nc0 * nc1 * nc2;
-        for (int q = lane; q < ncubes; q += 64) {
-            const int i = q % nc0, j = (q / nc0) % nc1, l = q / (nc0 * nc1);
-            const int b0 = (l * FF_BL + j) * FF_BL + i;
-            int ci = 0;
-            uint32_t zw = 0;
-#pragma unroll
-            for (int k = 0; k < 8; k++) {
-                const uint32_t f = L.sf[b0 + (c_corner[k][2] * FF_BL + c_corner[k][1]) * FF_BL + c_corner[k][0]];
-                zw |= f;
-                ci |= (int)(f & 1u) << k;
-            }
-            if (zw & 2u) ci = 0;
-            L.cube[q] = (uint8_t)ci;
-            const int nt = c_ntri[ci];
-            for (int t = 0; t < 3 * nt; t++) {
-                const int e = c_tri[ci][t];
-                const int slot = (b0 + (c_edge[e][2] * FF_BL + c_edge[e][1]) * FF_BL + c_edge[e][0]) * 3 + c_edge[e][3];
-                atomicOr(&L.used[slot >> 5], 1u << (slot & 31));
-            }
-        }
-        wave_sync();
-        // -- vertex numbering over the used-edge bitmap, and the slot list
-        {
-            const uint32_t wd = lane < 48 ? L.used[lane] : 0u;
-            uint32_t tot;
-            const uint32_t ex = wave_excl_scan_u32((uint32_t)__popc(wd), &tot);
-            if (lane < 48) {
-                L.wpref[lane] = ex;
-                uint32_t bits = wd, k = ex;
-                while (bits) {
-                    const uint32_t b = (uint32_t)__ffs(bits) - 1u;
-                    L.list[k++] = (uint16_t)(lane * 32 + b);
-                    bits &= bits - 1u;
-                }
-            }
-            nv = tot;
-        }
-        // face offsets per cube: contiguous chunks of 6 cubes per lane
-        {
-            const int per = 6;
-            uint32_t loc = 0;
-            for (int q = lane * per; q < min(ncubes, lane * per + per); q++) loc += c_ntri[L.cube[q]];
-            uint32_t tot;
-            uint32_t ex = wave_excl_scan_u32(loc, &tot);
-            for (int q = lane * per; q < min(ncubes, lane * per + per); q++) {
-                L.foff[q] = (uint16_t)ex;
-                ex += c_ntri[L.cube[q]];
-            }
-            nf = tot;
-        }
-        wave_sync();
-        uint32_t v0 = 0, f0 = 0, ok = 1;
-        if (lane == 0) ok = pool_alloc(P, di % FF_ARENAS, nv, nf, v0, f0, mc) ? 1u : 0u;
-        v0 = __shfl(v0, 0);
-        f0 = __shfl(f0, 0);
-        const bool fits = __shfl(ok, 0) != 0;
-        if (fits) {
-            // vertices: one used edge per lane
-            for (uint32_t vi = lane; vi < nv; vi += 64) {
-                const int slot = L.list[vi], a = slot % 3, id = slot / 3;
-                const int gg[3] = {id & 7, (id >> 3) & 7, id >> 6};
-                const int id2 = id + (a == 0 ? 1 : (a == 1 ? FF_BL : FF_BL * FF_BL));
-                float p1[3], p2[3];
-#pragma unroll
-                for (int k = 0; k < 3; k++) {
-                    const int ck = geom_coord(g, k, gg[k]);
-                    p1[k] = m.off[k] + (float)ck * m.scale;
-                    p2[k] = k == a ? m.off[k] + (float)(ck + elen[k]) * m.scale : p1[k];
-                }
-                float V[3];
-                vinterp(p1, p2, L.d[id], L.d[id2], V);
-                const uint16_t ca[3] = {L.c[0][id], L.c[1][id], L.c[2][id]};
-                const uint16_t cb[3] = {L.c[0][id2], L.c[1][id2], L.c[2][id2]};
-                const uint32_t rgb = cinterp(ca, cb, L.d[id], L.d[id2]);
-                P.v[v0 + vi] = make_float4(V[0], V[1], V[2], __uint_as_float(rgb));
-            }
-            // faces
-            for (int q = lane; q < ncubes; q += 64) {
-                const int ci = L.cube[q];
-                const int nt = c_ntri[ci];
-                if (!nt) continue;
-                const int i = q % nc0, j = (q / nc0) % nc1, l = q / (nc0 * nc1);
-                const int b0 = (l * FF_BL + j) * FF_BL + i;
-                const uint32_t fb = f0 + L.foff[q];
-                for (int t = 0; t < 3 * nt; t++) {
-                    const int e = c_tri[ci][t];
-                    const int slot = (b0 + (c_edge[e][2] * FF_BL + c_edge[e][1]) * FF_BL + c_edge[e][0]) * 3 + c_edge[e][3];
-                    const uint32_t wd = L.used[slot >> 5];
-                    P.f[3 * fb + t] = L.wpref[slot >> 5] + __popc(wd & ((1u << (slot & 31)) - 1u));
-                }
-            }
-        }
-        if (lane == 0) {
-            if (!fits) nv = nf = 0;
-            cp->v0 = v0; cp->nv = nv; cp->f0 = f0; cp->nf = nf;
-        }
-        wave_sync();
+        mc_cell<false>(L, lane, S.cells, g, S.mc, S.sbuf, S.m, S.P, di, act);
     }
 }
 
@@ -1106,6 +1181,10 @@ static bool mok(ff_ctx *c, hipError_t e, const char *what)
     do {                                            \
         if (!mok(c, (call), #call)) return FF_E_DEVICE; \
     } while (0)
+#define MCHK_CTX(ctx, call)                            \
+    do {                                               \
+        if (!mok((ctx), (call), #call)) return FF_E_DEVICE; \
+    } while (0)
 
 int mesh_init(ff_ctx *c)
 {
@@ -1161,6 +1240,7 @@ void mesh_free(ff_ctx *c)
     hipFree(s->d_geom); hipFree(s->d_tiles); hipFree(s->d_sbuf);
     hipFree(s->d_hkeys); hipFree(s->d_hval); hipFree(s->d_vpool); hipFree(s->d_fpool);
     hipFree(s->d_arena);
+    hipFree(s->d_ms); hipHostFree(s->h_ms); hipFree(s->d_msPrefix);
     hipFree(s->d_mc); hipHostFree(s->h_mc);
     delete s;
     c->mesh = nullptr;
@@ -1258,13 +1338,16 @@ static Pools pools(const MeshState *s)
     return P;
 }
 
-static int update_meshes(ff_ctx *c)
+// update_meshes in phases (so that a batch can share the marching-cubes
+// launch): pre -- cells, dirty marks, samples; the marching cubes (mc);
+// post -- lookup hash, counters' copy back; finish -- after the device work.
+bool mesh_launch_pre(ff_ctx *c, bool timed)
 {
     MeshState *s = c->mesh;
-    if (!s->haveVolume || !c->n) return 0;
-    hipEvent_t e0 = c->ev[6], e1 = c->ev[7];
-    if (c->timing) hipEventRecord(e0, c->stream);
+    if (!s->haveVolume || !c->n) return false;
+    if (timed && c->timing) hipEventRecord(c->ev[6], c->stream);
     const int nxt = 1 - s->cur;
+    s->nxt = nxt;
     k_mesh_begin<<<1, 64, 0, c->stream>>>(s->d_mc);
     const uint32_t nBlocks = c->nBranchesUsed / 16;
     k_terminal<<<(nBlocks * 8 + 255) / 256, 256, 0, c->stream>>>(c->d_tree, s->d_binfo, nBlocks, c->nLeavesTotal, c->n,
@@ -1280,21 +1363,52 @@ static int update_meshes(ff_ctx *c)
                                         s->nCells[s->cur] > 0, c->d_stamp, c->d_parent, s->lastMeshStamp, s->d_mc,
                                         s->d_geom, s->cellCap, m, s->d_tiles, s->tileCap);
     k_sample<<<4096, 256, 0, c->stream>>>(s->d_geom, s->d_cells[nxt], s->d_tiles, s->d_mc, s->tileCap, m, s->d_sbuf);
-    const Pools P = pools(s);
-    k_mc<false><<<2048, 256, 0, c->stream>>>(s->d_cells[nxt], s->d_geom, s->d_mc, s->d_sbuf, m, P);
+    s->m = m;
+    return true;
+}
+
+McStream mesh_stream(ff_ctx *c)
+{
+    MeshState *s = c->mesh;
+    McStream ms;
+    ms.cells = s->d_cells[s->nxt];
+    ms.geom = s->d_geom;
+    ms.mc = s->d_mc;
+    ms.sbuf = s->d_sbuf;
+    ms.m = s->m;
+    ms.P = pools(s);
+    return ms;
+}
+
+void mesh_launch_mc(ff_ctx *c)
+{
+    MeshState *s = c->mesh;
+    k_mc<false><<<2048, 256, 0, c->stream>>>(s->d_cells[s->nxt], s->d_geom, s->d_mc, s->d_sbuf, s->m, pools(s));
+}
+
+void mesh_launch_post(ff_ctx *c, bool timed)
+{
+    MeshState *s = c->mesh;
+    const int nxt = s->nxt;
     // the previous table's keys leave the lookup hash, the new ones enter it
     if (s->nCells[s->cur])
         k_hash_clear_cells<<<1024, 256, 0, c->stream>>>(s->d_cells[s->cur], s->nCells[s->cur], s->d_hkeys);
     k_hash_build<<<1024, 256, 0, c->stream>>>(s->d_cells[nxt], s->d_mc, s->cellCap, s->d_hkeys, s->d_hval, s->hmask);
-    if (c->timing) {
-        hipEventRecord(e1, c->stream);
-        k_arena_sum<<<1, 64, 0, c->stream>>>(s->d_arena, s->d_mc);
-    }
-    MCHK(hipMemcpyAsync(s->h_mc, s->d_mc, sizeof(MeshCounters), hipMemcpyDeviceToHost, c->stream));
-    MCHK(hipStreamSynchronize(c->stream));
-    if (c->timing) {
+    if (timed && c->timing) hipEventRecord(c->ev[7], c->stream);
+    if (c->timing) k_arena_sum<<<1, 64, 0, c->stream>>>(s->d_arena, s->d_mc);
+    hipMemcpyAsync(s->h_mc, s->d_mc, sizeof(MeshCounters), hipMemcpyDeviceToHost, c->stream);
+}
+
+// after the device work of pre / mc / post completed
+int mesh_finish(ff_ctx *c, bool timed)
+{
+    MeshState *s = c->mesh;
+    const int nxt = s->nxt;
+    const MapRO m = s->m;
+    const Pools P = pools(s);
+    if (timed && c->timing) {
         float ms = 0.0f;
-        if (hipEventElapsedTime(&ms, e0, e1) == hipSuccess) {
+        if (hipEventElapsedTime(&ms, c->ev[6], c->ev[7]) == hipSuccess) {
             c->phaseMs[2] += ms;
             c->phaseCalls[2]++;
         }
@@ -1377,6 +1491,16 @@ static int update_meshes(ff_ctx *c)
         }
     }
     return (int)nDirty;
+}
+
+static int update_meshes(ff_ctx *c)
+{
+    if (!mesh_launch_pre(c, true)) return 0;
+    mesh_launch_mc(c);
+    mesh_launch_post(c, true);
+    MCHK(hipGetLastError());
+    MCHK(hipStreamSynchronize(c->stream));
+    return mesh_finish(c, true);
 }
 
 // host copy of the current cell table, sorted by key
@@ -1535,6 +1659,63 @@ int ff_set_mesh_sample_tiles(ff_ctx *c, uint32_t tiles)
     s->d_sbuf = nb;
     s->tileCap = tiles;
     return FF_OK;
+}
+
+int ff_update_meshes_batch(ff_ctx *const *ctxs, int n)
+{
+    if (!ctxs || n <= 0 || n > FF_BATCH_MAX) return FF_E_ARG;
+    for (int i = 0; i < n; i++) {
+        if (!ctxs[i] || !ctxs[i]->mesh) return FF_E_ARG;
+        for (int j = 0; j < i; j++)
+            if (ctxs[j] == ctxs[i]) return FF_E_ARG;
+        if (ctxs[i]->device != ctxs[0]->device) return FF_E_ARG;
+    }
+    ff_ctx *lead = ctxs[0];
+    hipSetDevice(lead->device);
+    MeshState *ls = lead->mesh;
+    if (ls->msCap < n) {
+        hipFree(ls->d_ms); hipHostFree(ls->h_ms); hipFree(ls->d_msPrefix);
+        ls->d_ms = ls->h_ms = nullptr;
+        ls->d_msPrefix = nullptr;
+        ls->msCap = 0;
+        MCHK_CTX(lead, hipMalloc(&ls->d_ms, sizeof(McStream) * n));
+        MCHK_CTX(lead, hipHostMalloc(&ls->h_ms, sizeof(McStream) * n, hipHostMallocDefault));
+        MCHK_CTX(lead, hipMalloc(&ls->d_msPrefix, sizeof(uint32_t) * (n + 1)));
+        ls->msCap = n;
+    }
+    // every context runs on the lead's stream for the update
+    const hipStream_t st = lead->stream;
+    std::vector<hipStream_t> own(n);
+    for (int i = 0; i < n; i++) {
+        own[i] = ctxs[i]->stream;
+        if (own[i] != st) hipStreamSynchronize(own[i]);
+        ctxs[i]->stream = st;
+    }
+    std::vector<int> act;
+    for (int i = 0; i < n; i++)
+        if (ctxs[i]->incremental && mesh_launch_pre(ctxs[i], false)) act.push_back(i);
+    const int na = (int)act.size();
+    for (int k = 0; k < na; k++) ls->h_ms[k] = mesh_stream(ctxs[act[k]]);
+    if (na) {
+        hipMemcpyAsync(ls->d_ms, ls->h_ms, sizeof(McStream) * na, hipMemcpyHostToDevice, st);
+        k_mc_prefix<<<1, 64, 0, st>>>(ls->d_ms, na, ls->d_msPrefix);
+        k_mc_batch<<<2048, 256, 0, st>>>(ls->d_ms, na, ls->d_msPrefix);
+    }
+    for (int i : act) mesh_launch_post(ctxs[i], false);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    int rc = FF_OK;
+    if (e != hipSuccess) {
+        lead->err = std::string("ff_update_meshes_batch: ") + hipGetErrorString(e);
+        rc = FF_E_DEVICE;
+    } else {
+        for (int i : act) {
+            const int r = mesh_finish(ctxs[i], false);
+            if (r < 0 && rc == FF_OK) rc = r;
+        }
+    }
+    for (int i = 0; i < n; i++) ctxs[i]->stream = own[i];
+    return rc;
 }
 
 }  // extern "C"

@@ -188,6 +188,34 @@ int ff_trainer_train(ff_trainer *tr, int steps, uint32_t batch, float lr, float 
 int ff_trainer_grad(ff_trainer *tr, const uint32_t *index, uint32_t n, float *grad, float *loss);
 int ff_trainer_params(ff_trainer *tr, float *out, size_t n_params);
 
+/* ---- Batched streams ---------------------------------------------------
+ * No counterpart in the reference (one FusionMipMapCPU per map): a tick
+ * advances n independent contexts on one device by one frame each, with one
+ * SDF launch over all of their queued bricks.  Per context the outcome is
+ * that of ff_add_map_device (its counts, ff_last_error on failure); the
+ * return value is FF_OK or the first error.  Inputs are device memory,
+ * dense rows, borrowed for the call. */
+#define FF_BATCH_MAX 64
+typedef struct ff_frame {
+    const uint16_t *depth;  /* u16 depth, device memory */
+    const uint8_t *bgr;     /* BGR8, device memory; NULL without color */
+    int width, height;
+    double R[9], t[3], K[4]; /* camera -> world, intrinsics fx fy cx cy */
+    float scaling, maxcam;
+} ff_frame;
+int ff_add_map_batch(ff_ctx *const *ctxs, const ff_frame *frames, int n);
+/* updateMeshes for the n contexts of a batch, with one marching-cubes
+ * launch over all of their re-meshed cells. */
+int ff_update_meshes_batch(ff_ctx *const *ctxs, int n);
+/* Launch configuration of the batch led by ctxs[0]: threads per brick (512
+ * or 256), bricks per workgroup, HIP-graph replay of the tick's device work.
+ * By default the autotuner times every candidate on the first ticks and
+ * keeps the fastest; set_config fixes one, autotune restarts the tuning. */
+int ff_batch_set_config(ff_ctx *lead, int threads_per_brick, int bricks_per_workgroup, int graphs);
+int ff_batch_autotune(ff_ctx *lead, int ticks_per_candidate);
+int ff_batch_get_config(const ff_ctx *lead, int *threads_per_brick, int *bricks_per_workgroup, int *graphs,
+                        int *tuning);
+
 #ifdef __cplusplus
 }
 #endif
