@@ -72,7 +72,7 @@ static void default_cands(BatchState *b)
 {
     b->cands.clear();
     for (int g = 0; g < 2; g++)
-        for (int tpb : {512, 256})
+        for (int tpb : {512, 256, 64})
             for (int bpw : {1, 2, 4, 8}) b->cands.push_back({tpb, bpw, g == 1, 0.0, 0});
     b->cur = 0;
     b->tuning = true;
@@ -116,7 +116,7 @@ struct TickPlan {
 
 static thread_local TickPlan *g_plan = nullptr;  // the plan being captured or launched
 
-static void launch_tick(ff_ctx *, int, int)
+static void launch_tick(ff_ctx *lead, int, int)
 {
     TickPlan &P = *g_plan;
     BatchState *b = P.b;
@@ -124,7 +124,10 @@ static void launch_tick(ff_ctx *, int, int)
     for (int i : P.dev)
         if (b->ctxs[i]->learned) learned_launch(b->ctxs[i], P.W[i], P.H[i]);
     const int nd = (int)P.dev.size();
+    // events only in direct launches (lead != nullptr): the batched SDF time
+    if (lead && lead->timing) hipEventRecord(lead->ev[2], P.st);
     launch_sdf_batch(P.st, b->d_ss, nd, b->d_prefix, b->tpb, b->bpw);
+    if (lead && lead->timing) hipEventRecord(lead->ev[3], P.st);
     launch_gather_fc(P.st, b->d_ss, nd, b->d_fcs);
     hipMemcpyAsync(b->h_fcs, b->d_fcs, sizeof(FrameCounters) * nd, hipMemcpyDeviceToHost, P.st);
 }
@@ -233,6 +236,7 @@ int ff_add_map_batch(ff_ctx *const *ctxs, const ff_frame *frames, int n)
     for (int i : empty) launch_empty(ctxs[i], P.W[i], P.H[i]);
     const auto t1 = std::chrono::steady_clock::now();
     g_plan = &P;
+    bool timedSdf = false;
     if (nd) {
         bool launched = false;
         if (b->graphs) {
@@ -254,7 +258,10 @@ int ff_add_map_batch(ff_ctx *const *ctxs, const ff_frame *frames, int n)
             }
             if (b->exec && hipGraphLaunch(b->exec, st) == hipSuccess) launched = true;
         }
-        if (!launched) launch_tick(nullptr, 0, 0);
+        if (!launched) {
+            launch_tick(lead, 0, 0);
+            timedSdf = lead->timing;
+        }
     }
     g_plan = nullptr;
     hipError_t e = hipStreamSynchronize(st);
@@ -263,6 +270,20 @@ int ff_add_map_batch(ff_ctx *const *ctxs, const ff_frame *frames, int n)
     if (!devOk) {
         lead->err = std::string("ff_add_map_batch: ") + hipGetErrorString(e);
         rc = FF_E_DEVICE;
+    }
+    if (devOk && timedSdf) {
+        // the batched SDF update on the lead's timing: time, launches, bytes
+        float ms = 0.0f;
+        if (hipEventElapsedTime(&ms, lead->ev[2], lead->ev[3]) == hipSuccess) {
+            lead->phaseMs[1] += ms;
+            lead->phaseCalls[1]++;
+            for (int k = 0; k < nd; k++) {
+                const ff_ctx *c = ctxs[P.dev[k]];
+                if (b->h_fcs[k].sdfRan)
+                    lead->phaseBytes[1] += (double)b->h_fcs[k].nQueued * (c->useColor ? 14336.0 : 8192.0);
+                lead->phaseUnits[1] += b->h_fcs[k].sdfRan ? b->h_fcs[k].nQueued : 0u;
+            }
+        }
     }
     if (devOk) {
         for (int k = 0; k < nd; k++) memcpy(ctxs[P.dev[k]]->h_fc, &b->h_fcs[k], sizeof(FrameCounters));
@@ -298,7 +319,7 @@ int ff_add_map_batch(ff_ctx *const *ctxs, const ff_frame *frames, int n)
 
 int ff_batch_set_config(ff_ctx *lead, int tpb, int bpw, int graphs)
 {
-    if (!lead || (tpb != 512 && tpb != 256) || bpw < 1 || bpw > 64) return FF_E_ARG;
+    if (!lead || (tpb != 512 && tpb != 256 && tpb != 64) || bpw < 1 || bpw > 64) return FF_E_ARG;
     if (!lead->batch) {
         lead->batch = new BatchState();
         default_cands(lead->batch);

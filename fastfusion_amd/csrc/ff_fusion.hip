@@ -670,52 +670,56 @@ __global__ void k_shift_leaves(ushort4 *info, uint32_t n, uint32_t ax, uint32_t 
 // rebuilds its row origin with the same sequential adds as the reference
 // (z steps, then y steps) and adds k * d1j like the AVX lane k.
 // ---------------------------------------------------------------------------
-// voxel v (z-major, 0..511) of brick `leaf`
-__device__ __forceinline__ void sdf_voxel(uint32_t leaf, int v, const MapView &m, const uint16_t *__restrict__ depth,
-                                          const uint8_t *__restrict__ bgr, int W, int H, const SdfParams &P,
-                                          const uint32_t *tab)
+// The per-row part of update8AddLoopAVXSingleInteger: the row origin of
+// brick row (zr, yr) accumulated by repeated adds in the reference's order,
+// the column step d1, the truncations and the fall-off reciprocal.
+struct SdfRow {
+    float R[3], d1[3], td, tw, rfall;
+};
+
+__device__ __forceinline__ SdfRow sdf_row_setup(const ushort4 info, int zr, int yr, const SdfParams &P,
+                                                const uint32_t *tab)
 {
-    const int tid = v;
-    const int zr = tid >> 6, yr = (tid >> 3) & 7, k = tid & 7;
     const float EPS = 0.005f;
-    const ushort4 info = m.leafinfo[leaf];
-    const size_t vi = (size_t)leaf * FF_BRICK + tid;
-    const float wA = m.wgt[vi], dA = m.dist[vi];
-    uint16_t cA[3] = {0, 0, 0};
-    uint16_t *cbase = m.col + (size_t)leaf * FF_BRICK * 3 + tid;
-    if (P.useColor) {
-        cA[0] = cbase[0];
-        cA[1] = cbase[FF_BRICK];
-        cA[2] = cbase[2 * FF_BRICK];
-    }
+    SdfRow w;
     const float ls = (float)info.w;
     const float fls = rz_mul(ls, P.scale);
     const float ofx = (float)info.x, ofy = (float)info.y, ofz = (float)info.z;
-    float R[3], d1[3], d2[3], d3[3];
+    float d2[3], d3[3];
 #pragma unroll
     for (int r = 0; r < 3; r++) {
-        R[r] = rz_add(rz_mul(rz_add(rz_add(rz_mul(P.m[4 * r], ofx), rz_mul(P.m[4 * r + 1], ofy)),
-                                    rz_mul(P.m[4 * r + 2], ofz)),
-                             P.scale),
-                      P.m[4 * r + 3]);
-        d1[r] = rz_mul(P.m[4 * r], fls);
+        w.R[r] = rz_add(rz_mul(rz_add(rz_add(rz_mul(P.m[4 * r], ofx), rz_mul(P.m[4 * r + 1], ofy)),
+                                      rz_mul(P.m[4 * r + 2], ofz)),
+                               P.scale),
+                        P.m[4 * r + 3]);
+        w.d1[r] = rz_mul(P.m[4 * r], fls);
         d2[r] = rz_mul(P.m[4 * r + 1], fls);
         d3[r] = rz_mul(P.m[4 * r + 2], fls);
     }
-    const float td = rz_mul(P.dT, ls);
-    const float tw = rz_mul(rz_mul(3.0f, P.dT), ls);
+    w.td = rz_mul(P.dT, ls);
+    w.tw = rz_mul(rz_mul(3.0f, P.dT), ls);
     for (int i = 0; i < zr; i++) {
-        R[0] = rz_add(R[0], d3[0]); R[1] = rz_add(R[1], d3[1]); R[2] = rz_add(R[2], d3[2]);
+        w.R[0] = rz_add(w.R[0], d3[0]); w.R[1] = rz_add(w.R[1], d3[1]); w.R[2] = rz_add(w.R[2], d3[2]);
     }
     for (int i = 0; i < yr; i++) {
-        R[0] = rz_add(R[0], d2[0]); R[1] = rz_add(R[1], d2[1]); R[2] = rz_add(R[2], d2[2]);
+        w.R[0] = rz_add(w.R[0], d2[0]); w.R[1] = rz_add(w.R[1], d2[1]); w.R[2] = rz_add(w.R[2], d2[2]);
     }
+    w.rfall = rcp_x86(rz_sub(w.tw, EPS), tab);
+    return w;
+}
+
+// one voxel (column k of its row): old (dA, wA, cA) -> new (dN, wN, cN)
+__device__ __forceinline__ void sdf_point(const SdfRow &w, int k, float dA, float wA, const uint16_t cA[3],
+                                          const uint16_t *__restrict__ depth, const uint8_t *__restrict__ bgr,
+                                          int W, int H, const SdfParams &P, const uint32_t *tab, float &dN,
+                                          float &wN, uint16_t cN[3])
+{
+    const float EPS = 0.005f;
     const float fk = (float)k;
     // lane 0 adds +0.0f like _mm256_setr_ps(0.0f, ...)
-    const float px = rz_add(R[0], k ? rz_mul(fk, d1[0]) : 0.0f);
-    const float py = rz_add(R[1], k ? rz_mul(fk, d1[1]) : 0.0f);
-    const float pz = rz_add(R[2], k ? rz_mul(fk, d1[2]) : 0.0f);
-    const float rfall = rcp_x86(rz_sub(tw, EPS), tab);
+    const float px = rz_add(w.R[0], k ? rz_mul(fk, w.d1[0]) : 0.0f);
+    const float py = rz_add(w.R[1], k ? rz_mul(fk, w.d1[1]) : 0.0f);
+    const float pz = rz_add(w.R[2], k ? rz_mul(fk, w.d1[2]) : 0.0f);
     const float len = rz_sqrt(rz_add(rz_add(rz_mul(px, px), rz_mul(py, py)), rz_mul(pz, pz)));
     const float r = rcp_x86(pz, tab);
     const int imx = cvt_x86(rz_add(P.cx, rz_mul(P.fx, rz_mul(px, r))));
@@ -727,18 +731,18 @@ __device__ __forceinline__ void sdf_voxel(uint32_t leaf, int v, const MapView &m
     const float dI = rz_sub(len, rz_mul(rz_mul(len, r), h));
     const bool mask = (pz > 0.0f) && (imy < H) && (imy > -1) && (imx < W) && (imx > -1) && (h < P.maxcam);
     const float front = (dI < EPS) ? 1.0f : 0.0f;
-    const float back = (dI >= EPS && dI < tw) ? 1.0f : 0.0f;
-    const float fall = rz_mul(rz_sub(tw, dI), rfall);
+    const float back = (dI >= EPS && dI < w.tw) ? 1.0f : 0.0f;
+    const float fall = rz_mul(rz_sub(w.tw, dI), w.rfall);
     float wI = mask ? rz_add(front, rz_mul(fall, back)) : 0.0f;
     if (P.conf) wI = rz_mul(wI, P.conf[ii]);  // learned per-pixel confidence
     const float wn = rz_add(wA, wI);
     const float f = rcp_x86(rz_add(wn, (wn <= 0.0f) ? 1.0f : 0.0f), tab);
-    const float mn = (td < dI) ? td : dI;                 // _mm256_min_ps(td, dI)
-    const float ntd = -td;
+    const float mn = (w.td < dI) ? w.td : dI;             // _mm256_min_ps(td, dI)
+    const float ntd = -w.td;
     const float mx = (ntd > mn) ? ntd : mn;               // _mm256_max_ps(-td, mn)
     const float add = mask ? rz_mul(mx, wI) : 0.0f;
-    m.dist[vi] = rz_mul(rz_add(rz_mul(dA, wA), add), f);
-    m.wgt[vi] = wn;
+    dN = rz_mul(rz_add(rz_mul(dA, wA), add), f);
+    wN = wn;
     if (P.useColor) {
         const float cmax = 65280.0f;
 #pragma unroll
@@ -746,8 +750,86 @@ __device__ __forceinline__ void sdf_voxel(uint32_t leaf, int v, const MapView &m
             const float inc = (float)bgr[ii * 3 + (2 - ch)] * 256.0f;
             float vv = rz_mul(rz_add(rz_mul((float)cA[ch], wA), rz_mul(inc, wI)), f);
             vv = (vv < cmax) ? vv : cmax;
-            cbase[ch * FF_BRICK] = (uint16_t)cvt_x86(vv);
+            cN[ch] = (uint16_t)cvt_x86(vv);
         }
+    }
+}
+
+// voxel v (z-major, 0..511) of brick `leaf`
+__device__ __forceinline__ void sdf_voxel(uint32_t leaf, int v, const MapView &m, const uint16_t *__restrict__ depth,
+                                          const uint8_t *__restrict__ bgr, int W, int H, const SdfParams &P,
+                                          const uint32_t *tab)
+{
+    const size_t vi = (size_t)leaf * FF_BRICK + v;
+    const float wA = m.wgt[vi], dA = m.dist[vi];
+    uint16_t cA[3] = {0, 0, 0}, cN[3];
+    uint16_t *cbase = m.col + (size_t)leaf * FF_BRICK * 3 + v;
+    if (P.useColor) {
+        cA[0] = cbase[0];
+        cA[1] = cbase[FF_BRICK];
+        cA[2] = cbase[2 * FF_BRICK];
+    }
+    const SdfRow w = sdf_row_setup(m.leafinfo[leaf], v >> 6, (v >> 3) & 7, P, tab);
+    float dN, wN;
+    sdf_point(w, v & 7, dA, wA, cA, depth, bgr, W, H, P, tab, dN, wN, cN);
+    m.dist[vi] = dN;
+    m.wgt[vi] = wN;
+    if (P.useColor) {
+        cbase[0] = cN[0];
+        cbase[FF_BRICK] = cN[1];
+        cbase[2 * FF_BRICK] = cN[2];
+    }
+}
+
+// the 8 voxels of row `row` (zr * 8 + yr) of brick `leaf`: the row set-up
+// once, 32-byte loads and stores of the row's distances and weights
+__device__ __forceinline__ void sdf_row(uint32_t leaf, int row, const MapView &m, const uint16_t *__restrict__ depth,
+                                        const uint8_t *__restrict__ bgr, int W, int H, const SdfParams &P,
+                                        const uint32_t *tab)
+{
+    const size_t vb = (size_t)leaf * FF_BRICK + (size_t)row * 8;
+    float4 *dp = reinterpret_cast<float4 *>(m.dist + vb);
+    float4 *wp = reinterpret_cast<float4 *>(m.wgt + vb);
+    const float4 d0 = dp[0], d1 = dp[1], w0 = wp[0], w1 = wp[1];
+    const float dA[8] = {d0.x, d0.y, d0.z, d0.w, d1.x, d1.y, d1.z, d1.w};
+    const float wA[8] = {w0.x, w0.y, w0.z, w0.w, w1.x, w1.y, w1.z, w1.w};
+    uint16_t cA[3][8];
+    uint4 *cp[3] = {nullptr, nullptr, nullptr};
+    if (P.useColor) {
+#pragma unroll
+        for (int ch = 0; ch < 3; ch++) {
+            cp[ch] = reinterpret_cast<uint4 *>(m.col + (size_t)leaf * FF_BRICK * 3 + (size_t)ch * FF_BRICK + row * 8);
+            const uint4 u = *cp[ch];
+            const uint32_t q[4] = {u.x, u.y, u.z, u.w};
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                cA[ch][2 * j] = (uint16_t)(q[j] & 0xffffu);
+                cA[ch][2 * j + 1] = (uint16_t)(q[j] >> 16);
+            }
+        }
+    }
+    const SdfRow w = sdf_row_setup(m.leafinfo[leaf], row >> 3, row & 7, P, tab);
+    float dN[8], wN[8];
+    uint16_t cN[3][8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) {
+        const uint16_t c3[3] = {P.useColor ? cA[0][k] : (uint16_t)0, P.useColor ? cA[1][k] : (uint16_t)0,
+                                P.useColor ? cA[2][k] : (uint16_t)0};
+        uint16_t o3[3] = {0, 0, 0};
+        sdf_point(w, k, dA[k], wA[k], c3, depth, bgr, W, H, P, tab, dN[k], wN[k], o3);
+        cN[0][k] = o3[0]; cN[1][k] = o3[1]; cN[2][k] = o3[2];
+    }
+    dp[0] = make_float4(dN[0], dN[1], dN[2], dN[3]);
+    dp[1] = make_float4(dN[4], dN[5], dN[6], dN[7]);
+    wp[0] = make_float4(wN[0], wN[1], wN[2], wN[3]);
+    wp[1] = make_float4(wN[4], wN[5], wN[6], wN[7]);
+    if (P.useColor) {
+#pragma unroll
+        for (int ch = 0; ch < 3; ch++)
+            *cp[ch] = make_uint4((uint32_t)cN[ch][0] | ((uint32_t)cN[ch][1] << 16),
+                                 (uint32_t)cN[ch][2] | ((uint32_t)cN[ch][3] << 16),
+                                 (uint32_t)cN[ch][4] | ((uint32_t)cN[ch][5] << 16),
+                                 (uint32_t)cN[ch][6] | ((uint32_t)cN[ch][7] << 16));
     }
 }
 
@@ -841,6 +923,37 @@ __global__ __launch_bounds__(TPB) void k_sdf_batch(const SdfStream *__restrict__
     }
 }
 
+// one wave per brick, lane = brick row (8 voxels): the row set-up is shared by
+// its 8 voxels; waves take bpw consecutive bricks of the worklist at a time
+__global__ __launch_bounds__(256) void k_sdf_batch_rows(const SdfStream *__restrict__ ss, int n,
+                                                        const uint32_t *__restrict__ prefix, int bpw)
+{
+    const uint32_t total = prefix[n];
+    const uint32_t wavesPerBlock = blockDim.x >> 6;
+    if ((uint32_t)blockIdx.x * wavesPerBlock * (uint32_t)bpw >= total) return;
+    __shared__ uint32_t tab[2048];
+    for (int i = threadIdx.x; i < 2048; i += blockDim.x) tab[i] = c_rcp_table[i];
+    __syncthreads();
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t nWaves = gridDim.x * wavesPerBlock;
+    for (uint32_t chunk = blockIdx.x * wavesPerBlock + (threadIdx.x >> 6); chunk * (uint32_t)bpw < total;
+         chunk += nWaves) {
+        for (int k = 0; k < bpw; k++) {
+            const uint32_t item = chunk * (uint32_t)bpw + (uint32_t)k;
+            if (item >= total) break;
+            int lo = 0, hi = n;  // prefix[lo] <= item < prefix[hi]
+            while (hi - lo > 1) {
+                const int mid = (lo + hi) >> 1;
+                if (prefix[mid] <= item) lo = mid;
+                else hi = mid;
+            }
+            const SdfStream &S = ss[lo];
+            const FrameDev *fd = S.fd;
+            sdf_row(S.queue[item - prefix[lo]], (int)lane, S.m, fd->depth, fd->bgr, S.W, S.H, fd->sp, tab);
+        }
+    }
+}
+
 __global__ void k_gather_fc(const SdfStream *__restrict__ ss, int n, FrameCounters *__restrict__ out)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -852,7 +965,8 @@ __global__ void k_gather_fc(const SdfStream *__restrict__ ss, int n, FrameCounte
 void launch_sdf_batch(hipStream_t st, const SdfStream *d_ss, int n, uint32_t *d_prefix, int tpb, int bpw)
 {
     k_batch_prefix<<<1, 64, 0, st>>>(d_ss, n, d_prefix);
-    if (tpb == 256) k_sdf_batch<256><<<FF_SDF_BATCH_GRID, 256, 0, st>>>(d_ss, n, d_prefix, bpw);
+    if (tpb == 64) k_sdf_batch_rows<<<FF_SDF_BATCH_GRID, 256, 0, st>>>(d_ss, n, d_prefix, bpw);
+    else if (tpb == 256) k_sdf_batch<256><<<FF_SDF_BATCH_GRID, 256, 0, st>>>(d_ss, n, d_prefix, bpw);
     else k_sdf_batch<512><<<FF_SDF_BATCH_GRID, 512, 0, st>>>(d_ss, n, d_prefix, bpw);
 }
 
@@ -1387,11 +1501,14 @@ int frame_finish(ff_ctx *c, int W, int H, bool timed)
         }
     }
     // algorithmic bytes: images read once, 16 B per processed pixel, 512 voxels
-    // x 14 B (8 B without color) read and written per queued brick
-    c->phaseBytes[0] += (double)W * H * 2 + 16.0 * hf->nProcessed;
-    c->phaseBytes[1] += (double)c->nQueued * (c->useColor ? 14336.0 : 8192.0) + (c->useColor ? 3.0 * W * H : 0.0);
-    c->phaseUnits[0] += hf->nProcessed;
-    c->phaseUnits[1] += c->nQueued;
+    // x 14 B (8 B without color) read and written per queued brick (batched
+    // ticks account their SDF launch on the lead context instead)
+    if (timed) {
+        c->phaseBytes[0] += (double)W * H * 2 + 16.0 * hf->nProcessed;
+        c->phaseBytes[1] += (double)c->nQueued * (c->useColor ? 14336.0 : 8192.0) + (c->useColor ? 3.0 * W * H : 0.0);
+        c->phaseUnits[0] += hf->nProcessed;
+        c->phaseUnits[1] += c->nQueued;
+    }
 
     if (c->nLeavesUsed > c->nLeavesTotal || c->nBranchesUsed > c->nBranchesTotal) c->incremental = false;
     c->frames++;

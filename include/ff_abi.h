@@ -208,7 +208,9 @@ int ff_add_map_batch(ff_ctx *const *ctxs, const ff_frame *frames, int n);
  * launch over all of their re-meshed cells. */
 int ff_update_meshes_batch(ff_ctx *const *ctxs, int n);
 /* Launch configuration of the batch led by ctxs[0]: threads per brick (512
- * or 256), bricks per workgroup, HIP-graph replay of the tick's device work.
+ * or 256: one or two voxels per thread; 64: one wave per brick, a row of 8
+ * voxels per lane), bricks per workgroup (per wave for 64), HIP-graph replay
+ * of the tick's device work.
  * By default the autotuner times every candidate on the first ticks and
  * keeps the fastest; set_config fixes one, autotune restarts the tuning. */
 int ff_batch_set_config(ff_ctx *lead, int threads_per_brick, int bricks_per_workgroup, int graphs);

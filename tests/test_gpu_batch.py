@@ -57,7 +57,7 @@ def _same_mesh(a, b, tag):
     return len(ma.faces)
 
 
-@pytest.mark.parametrize("config", [(512, 1, False), (256, 4, False), (512, 2, True), None])
+@pytest.mark.parametrize("config", [(512, 1, False), (256, 4, False), (64, 2, False), (512, 2, True), None])
 def test_batch_equals_single_streams(config):
     n_ticks = 5
     data = _frames(n_ticks)
