@@ -24,6 +24,11 @@ sys.path.insert(0, ROOT)
 
 METRIC = "fused+meshed frames/sec and HBM GB/s at 640x480, 5mm; RMS surface err vs GT"
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per k_sdf launch over the timed frames of the default command
+# (950 frames after 50), from rocprofv3 FETCH_SIZE and WRITE_SIZE passes of
+# that command, each calibrated against a 256 MiB copy
+# (profiles/hbm_traffic_640x480_5mm.md): 8 056 KiB read + 3 314 KiB written
+SDF_TRAFFIC_DEFAULT = 11_643_085
 
 
 def render_frames(sc, R, t, n, cfg):
@@ -191,6 +196,13 @@ def main():
               "(independent map streams shard by stream, a single map by Morton range: see DESIGN.md)",
               file=sys.stderr)
         sys.exit(2)
+    if args.calib_copy:
+        from fastfusion_amd import hiprt
+        nb = args.calib_copy << 20
+        src, dst = hiprt.DeviceBuffer(nb), hiprt.DeviceBuffer(nb)
+        hiprt.check(hiprt.hip().hipMemcpy(dst.ptr, src.ptr, nb, hiprt.D2D), "calibration copy")
+        hiprt.synchronize()
+        del src, dst
     if args.streams:
         return run_streams(args)
 
@@ -203,12 +215,6 @@ def main():
     print(f"[bench] rendered {n_frames} frames in {time.perf_counter() - t_r:.1f} s", file=sys.stderr, flush=True)
 
     w, h = cfg["w"], cfg["h"]
-    if args.calib_copy:
-        nb = args.calib_copy << 20
-        src, dst = hiprt.DeviceBuffer(nb), hiprt.DeviceBuffer(nb)
-        hiprt.check(hiprt.hip().hipMemcpy(dst.ptr, src.ptr, nb, hiprt.D2D), "calibration copy")
-        hiprt.synchronize()
-        del src, dst
     d_depth = hiprt.DeviceBuffer.from_array(depth)
     d_bgr = hiprt.DeviceBuffer.from_array(bgr)
     fz = FusionMipMapGPU(0, 0, 0, cfg["scale"], cfg["threshold"], 0, True, max_leaves=1 << 21,
@@ -258,6 +264,7 @@ def main():
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(depth, bgr, R, t, cfg, args.cpu_seconds)
+    traffic = (SDF_TRAFFIC_DEFAULT if (args.steps, args.warmup) == (950, 50) and not args.learned else None)
     line = {
         "metric": METRIC,
         "value": round(fps, 2),
@@ -280,7 +287,7 @@ def main():
         "bricks_per_frame": round(nq_sum / args.steps, 1),
         "map": {"bricks": counts["n_leaves_used"], "side": counts["side"]},
         "roofline": {"kernel": "k_sdf", "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "avg_launch_us": round(sdf_ms * 1e3, 3), "bytes_per_launch": round(sdf_bytes)},
         "phases_ms_per_frame": {k: round(v["ms"] / args.steps, 4) for k, v in tm.items()},
         "rms_surface_err_m": round(rms, 6),

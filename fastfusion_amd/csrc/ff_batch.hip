@@ -125,6 +125,7 @@ static void launch_tick(ff_ctx *lead, int, int)
         if (b->ctxs[i]->learned) learned_launch(b->ctxs[i], P.W[i], P.H[i]);
     const int nd = (int)P.dev.size();
     // events only in direct launches (lead != nullptr): the batched SDF time
+    launch_batch_prefix(P.st, b->d_ss, nd, b->d_prefix);
     if (lead && lead->timing) hipEventRecord(lead->ev[2], P.st);
     launch_sdf_batch(P.st, b->d_ss, nd, b->d_prefix, b->tpb, b->bpw);
     if (lead && lead->timing) hipEventRecord(lead->ev[3], P.st);

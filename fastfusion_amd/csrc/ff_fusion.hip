@@ -962,9 +962,13 @@ __global__ void k_gather_fc(const SdfStream *__restrict__ ss, int n, FrameCounte
 
 #define FF_SDF_BATCH_GRID 2048
 
-void launch_sdf_batch(hipStream_t st, const SdfStream *d_ss, int n, uint32_t *d_prefix, int tpb, int bpw)
+void launch_batch_prefix(hipStream_t st, const SdfStream *d_ss, int n, uint32_t *d_prefix)
 {
     k_batch_prefix<<<1, 64, 0, st>>>(d_ss, n, d_prefix);
+}
+
+void launch_sdf_batch(hipStream_t st, const SdfStream *d_ss, int n, uint32_t *d_prefix, int tpb, int bpw)
+{
     if (tpb == 64) k_sdf_batch_rows<<<FF_SDF_BATCH_GRID, 256, 0, st>>>(d_ss, n, d_prefix, bpw);
     else if (tpb == 256) k_sdf_batch<256><<<FF_SDF_BATCH_GRID, 256, 0, st>>>(d_ss, n, d_prefix, bpw);
     else k_sdf_batch<512><<<FF_SDF_BATCH_GRID, 512, 0, st>>>(d_ss, n, d_prefix, bpw);
@@ -1305,18 +1309,23 @@ void launch_alloc(ff_ctx *c, int W, int H)
 
 // the SDF update waits for the host only when the node table overflowed or
 // the volume must grow first
-void launch_sdf(ff_ctx *c, int W, int H)
+static void launch_sdf_only(ff_ctx *c, int W, int H)
 {
-    if (c->learned) learned_launch(c, W, H);
     SdfGate gate;
     gate.listCap = c->hv.listCap;
     gate.leavesTotal = c->nLeavesTotal;
     gate.branchesTotal = c->nBranchesTotal;
     gate.force = 0;
     k_sdf<<<FF_SDF_GRID, 512, 0, c->stream>>>(c->d_queue, c->mapView(), c->d_fd, W, H, c->d_fc, c->d_mapc, gate);
-    hipMemcpyAsync(c->h_fc, c->d_fc, sizeof(FrameCounters), hipMemcpyDeviceToHost, c->stream);
 }
 
+// (the learned confidence first when on; the counters' copy back follows
+// outside, after the timing event)
+void launch_sdf(ff_ctx *c, int W, int H)
+{
+    if (c->learned) learned_launch(c, W, H);
+    launch_sdf_only(c, W, H);
+}
 
 static bool capture(ff_ctx *c, void (*fn)(ff_ctx *, int, int), int W, int H, hipGraph_t *g, hipGraphExec_t *x)
 {
@@ -1350,13 +1359,14 @@ static int run_frame(ff_ctx *c, int W, int H)
     if (c->timing) hipEventRecord(c->ev[0], c->stream);
     if (g) HIPCHK(hipGraphLaunch(c->frameExec[0], c->stream));
     else launch_alloc(c, W, H);
-    if (c->timing) {
-        hipEventRecord(c->ev[1], c->stream);
-        hipEventRecord(c->ev[2], c->stream);
-    }
+    if (c->timing) hipEventRecord(c->ev[1], c->stream);
+    // the SDF phase's events bracket k_sdf alone when launched directly
+    if (!g && c->learned) learned_launch(c, W, H);
+    if (c->timing) hipEventRecord(c->ev[2], c->stream);
     if (g) HIPCHK(hipGraphLaunch(c->frameExec[1], c->stream));
-    else launch_sdf(c, W, H);
+    else launch_sdf_only(c, W, H);
     if (c->timing) hipEventRecord(c->ev[3], c->stream);
+    HIPCHK(hipMemcpyAsync(c->h_fc, c->d_fc, sizeof(FrameCounters), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipGetLastError());
     return FF_OK;
 }

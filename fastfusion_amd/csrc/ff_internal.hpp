@@ -180,6 +180,7 @@ int ensure_image(ff_ctx *c, int W, int H);
 // batched SDF update over the concatenated brick lists of n streams
 // (ff_fusion.hip): per-stream gates and counts into prefix[0..n], then one
 // launch; threads per brick tpb (512 or 256), bricks per workgroup bpw
+void launch_batch_prefix(hipStream_t st, const SdfStream *d_ss, int n, uint32_t *d_prefix);
 void launch_sdf_batch(hipStream_t st, const SdfStream *d_ss, int n, uint32_t *d_prefix, int tpb, int bpw);
 void launch_gather_fc(hipStream_t st, const SdfStream *d_ss, int n, FrameCounters *d_out);
 void batch_free(ff_ctx *c);

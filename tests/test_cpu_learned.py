@@ -138,3 +138,43 @@ def test_features_depth_edge_and_missing():
     # pixel (6, 8): left neighbours are the foreground: 2/3 - 1
     f = F[6, 8, :25].reshape(5, 5)
     assert np.allclose(f[:, :2], np.float32(2 / 3) - 1, atol=1e-7)
+
+
+def _oracle_frames(n=3):
+    from fastfusion_amd import synth
+    sc = synth.Scene.room_scene(7, (4.0, 4.0, 3.0), 3, 3, 0.1, 0.4)
+    K = (262.5, 262.5, 159.5, 119.5)
+    R, t = sc.walk(n, 8, 0.5, 3.0, 0.02, 2.0)
+    return K, R, t, [sc.render(R[i].reshape(3, 3), t[i], K, 320, 240, seed=40 + i) for i in range(n)]
+
+
+def _state(o):
+    n = o.counts()["n_leaves_used"]
+    return o.tree().copy(), o.bricks(np.arange(n))
+
+
+def test_oracle_confidence_hook():
+    """The confidence multiplies the weight increment only: c = 1 is the plain
+    update bit for bit; c = 1/2 on the first frame halves every weight exactly
+    and leaves distances and colors unchanged (power-of-two scaling is exact in
+    round-toward-zero products and in RCPPS).  Applying c to the distance, to
+    the accumulated weight or to the colour increment fails one of these."""
+    from oracle.pyoracle import Oracle
+    K, R, t, fr = _oracle_frames()
+    plain, ones, half = (Oracle(0.01, 0.03, True, 1 << 16, 1 << 20) for _ in range(3))
+    for i, (d, bgr) in enumerate(fr):
+        plain.add_map(d, R[i], t[i], K, bgr, 1 / 5000.0, 4.0)
+        ones.add_map(d, R[i], t[i], K, bgr, 1 / 5000.0, 4.0, conf=np.ones(d.shape, np.float32))
+        if i == 0:
+            half.add_map(d, R[i], t[i], K, bgr, 1 / 5000.0, 4.0, conf=np.full(d.shape, 0.5, np.float32))
+            (ta, (da, wa, ca)), (th, (dh, wh, ch)) = _state(plain), _state(half)
+            assert np.array_equal(ta, th)
+            assert np.array_equal(da.view(np.uint32), dh.view(np.uint32))
+            assert np.array_equal(ca, ch)
+            assert np.array_equal((wa * np.float32(0.5)).view(np.uint32), wh.view(np.uint32))
+            assert (wa > 0).sum() > 10000
+    (ta, (da, wa, ca)), (tb, (db, wb, cb)) = _state(plain), _state(ones)
+    assert np.array_equal(ta, tb)
+    assert np.array_equal(da.view(np.uint32), db.view(np.uint32))
+    assert np.array_equal(wa.view(np.uint32), wb.view(np.uint32))
+    assert np.array_equal(ca, cb)
