@@ -2,9 +2,12 @@
 //
 // The reference fuses one map per FusionMipMapCPU object; a tick here
 // advances n such objects (contexts) by one frame each:
-//   host     each stream's frame preparation (pose, first-frame volume)
-//   device   each stream's allocation kernels, its learned confidence when
-//            on, then ONE SDF launch over the concatenation of every
+//   host     each stream's frame preparation (pose, first-frame volume);
+//            the frame blocks and stream descriptors of all streams go to
+//            the device in one copy
+//   device   the allocation phase of all streams, one launch per kernel
+//            (grid z = stream), the learned confidence of streams that use
+//            it, then ONE SDF launch over the concatenation of every
 //            stream's queued bricks (k_sdf_batch: a prefix over the
 //            streams' gated counts maps worklist items to streams), and one
 //            gather of the streams' frame counters
@@ -30,9 +33,39 @@ struct TuneCand {
     int runs;
 };
 
+// per-tick device descriptors, one block so that a tick uploads them in one
+// copy: frame blocks, allocation descriptors, SDF descriptors (cap each)
+struct TickDesc {
+    FrameDev *fd;
+    AllocStream *as;
+    SdfStream *ss;
+};
+
+static size_t desc_bytes(int cap, TickDesc *d, char *base)
+{
+    size_t o = 0;
+    auto take = [&](size_t bytes) {
+        const size_t at = o;
+        o = (o + bytes + 255) & ~(size_t)255;
+        return base ? base + at : nullptr;
+    };
+    char *a = take(sizeof(FrameDev) * cap);
+    char *b = take(sizeof(AllocStream) * cap);
+    char *c = take(sizeof(SdfStream) * cap);
+    if (d) {
+        d->fd = reinterpret_cast<FrameDev *>(a);
+        d->as = reinterpret_cast<AllocStream *>(b);
+        d->ss = reinterpret_cast<SdfStream *>(c);
+    }
+    return o;
+}
+
 struct BatchState {
     std::vector<ff_ctx *> ctxs;  // the batch (key)
     int cap = 0;
+    char *d_desc = nullptr, *h_desc = nullptr;
+    size_t descBytes = 0;
+    TickDesc dd{}, hd{};  // device and host views of the descriptor block
     SdfStream *d_ss = nullptr, *h_ss = nullptr;
     uint32_t *d_prefix = nullptr;
     FrameCounters *d_fcs = nullptr, *h_fcs = nullptr;
@@ -63,7 +96,7 @@ void batch_free(ff_ctx *c)
     BatchState *b = c->batch;
     if (!b) return;
     drop_graph(b);
-    hipFree(b->d_ss); hipHostFree(b->h_ss); hipFree(b->d_prefix); hipFree(b->d_fcs); hipHostFree(b->h_fcs);
+    hipFree(b->d_desc); hipHostFree(b->h_desc); hipFree(b->d_prefix); hipFree(b->d_fcs); hipHostFree(b->h_fcs);
     delete b;
     c->batch = nullptr;
 }
@@ -93,15 +126,21 @@ static BatchState *batch_for(ff_ctx *const *ctxs, int n)
     drop_graph(b);
     b->ctxs.assign(ctxs, ctxs + n);
     if (b->cap < n) {
-        hipFree(b->d_ss); hipHostFree(b->h_ss); hipFree(b->d_prefix); hipFree(b->d_fcs); hipHostFree(b->h_fcs);
-        b->d_ss = nullptr; b->h_ss = nullptr; b->d_prefix = nullptr; b->d_fcs = nullptr; b->h_fcs = nullptr;
+        hipFree(b->d_desc); hipHostFree(b->h_desc); hipFree(b->d_prefix); hipFree(b->d_fcs); hipHostFree(b->h_fcs);
+        b->d_desc = nullptr; b->h_desc = nullptr; b->d_prefix = nullptr; b->d_fcs = nullptr; b->h_fcs = nullptr;
+        b->d_ss = nullptr; b->h_ss = nullptr;
         b->cap = 0;
-        if (hipMalloc(&b->d_ss, sizeof(SdfStream) * n) != hipSuccess ||
-            hipHostMalloc(&b->h_ss, sizeof(SdfStream) * n, hipHostMallocDefault) != hipSuccess ||
+        b->descBytes = desc_bytes(n, nullptr, nullptr);
+        if (hipMalloc(&b->d_desc, b->descBytes) != hipSuccess ||
+            hipHostMalloc(&b->h_desc, b->descBytes, hipHostMallocDefault) != hipSuccess ||
             hipMalloc(&b->d_prefix, sizeof(uint32_t) * (n + 1)) != hipSuccess ||
             hipMalloc(&b->d_fcs, sizeof(FrameCounters) * n) != hipSuccess ||
             hipHostMalloc(&b->h_fcs, sizeof(FrameCounters) * n, hipHostMallocDefault) != hipSuccess)
             return nullptr;
+        desc_bytes(n, &b->dd, b->d_desc);
+        desc_bytes(n, &b->hd, b->h_desc);
+        b->d_ss = b->dd.ss;
+        b->h_ss = b->hd.ss;
         b->cap = n;
     }
     return b;
@@ -120,10 +159,10 @@ static void launch_tick(ff_ctx *lead, int, int)
 {
     TickPlan &P = *g_plan;
     BatchState *b = P.b;
-    for (int i : P.dev) launch_alloc(b->ctxs[i], P.W[i], P.H[i]);
+    const int nd = (int)P.dev.size();
+    launch_alloc_batch(P.st, b->dd.as, b->hd.as, nd);
     for (int i : P.dev)
         if (b->ctxs[i]->learned) learned_launch(b->ctxs[i], P.W[i], P.H[i]);
-    const int nd = (int)P.dev.size();
     // events only in direct launches (lead != nullptr): the batched SDF time
     launch_batch_prefix(P.st, b->d_ss, nd, b->d_prefix);
     if (lead && lead->timing) hipEventRecord(lead->ev[2], P.st);
@@ -199,20 +238,32 @@ int ff_add_map_batch(ff_ctx *const *ctxs, const ff_frame *frames, int n)
         }
         int r = ensure_image(c, f.width, f.height);
         if (!r) r = frame_prepare(c, f.depth, c->useColor ? f.bgr : nullptr, f.width, f.height, f.R, f.t, f.K,
-                                  f.scaling, f.maxcam);
+                                  f.scaling, f.maxcam, false);
         if (r) {
             if (rc == FF_OK) rc = r;
             continue;
         }
         P.W[i] = f.width;
         P.H[i] = f.height;
-        if (c->n) P.dev.push_back(i);
-        else empty.push_back(i);
+        if (c->n) {
+            P.dev.push_back(i);
+        } else {
+            // no volume yet (an empty first frame): its own upload
+            const hipError_t ec = hipMemcpyAsync(c->d_fd, c->h_fd, sizeof(FrameDev), hipMemcpyHostToDevice, st);
+            if (ec != hipSuccess && rc == FF_OK) rc = FF_E_DEVICE;
+            empty.push_back(i);
+        }
     }
-    // stream descriptors of the device work
+    // stream descriptors of the device work; each context reads its frame
+    // block from the batch's block for the tick
     const int nd = (int)P.dev.size();
+    std::vector<FrameDev *> ownFd(n, nullptr);
     for (int k = 0; k < nd; k++) {
         ff_ctx *c = ctxs[P.dev[k]];
+        memcpy(&b->hd.fd[k], c->h_fd, sizeof(FrameDev));
+        ownFd[P.dev[k]] = c->d_fd;
+        c->d_fd = b->dd.fd + k;
+        b->hd.as[k] = alloc_stream(c, P.W[P.dev[k]], P.H[P.dev[k]]);
         SdfStream &s = b->h_ss[k];
         s.queue = c->d_queue;
         s.m = c->mapView();
@@ -233,7 +284,7 @@ int ff_add_map_batch(ff_ctx *const *ctxs, const ff_frame *frames, int n)
         b->bpw = cand->bpw;
         b->graphs = cand->graphs;
     }
-    if (nd) hipMemcpyAsync(b->d_ss, b->h_ss, sizeof(SdfStream) * nd, hipMemcpyHostToDevice, st);
+    if (nd) hipMemcpyAsync(b->d_desc, b->h_desc, b->descBytes, hipMemcpyHostToDevice, st);
     for (int i : empty) launch_empty(ctxs[i], P.W[i], P.H[i]);
     const auto t1 = std::chrono::steady_clock::now();
     g_plan = &P;
@@ -294,7 +345,10 @@ int ff_add_map_batch(ff_ctx *const *ctxs, const ff_frame *frames, int n)
             if (r < 0 && rc == FF_OK) rc = r;
         }
     }
-    for (int i = 0; i < n; i++) ctxs[i]->stream = own[i];
+    for (int i = 0; i < n; i++) {
+        ctxs[i]->stream = own[i];
+        if (ownFd[i]) ctxs[i]->d_fd = ownFd[i];
+    }
     // autotuning: the device part of each candidate's ticks, first tick of a
     // candidate (capture, warm-up) not counted
     if (cand && devOk) {

@@ -144,9 +144,9 @@ __device__ __forceinline__ void voxel_of(const float *__restrict__ qx, const flo
 // qy = R * pyp + R_z (per row), as the reference forms them in float
 // (geometryfusion_mipmap_cpu.cpp:1765-1790), and the frame counters reset
 // ---------------------------------------------------------------------------
-__global__ void k_frame_begin(const float *__restrict__ pxp, const float *__restrict__ pyp, int W, int H,
-                              const FrameDev *__restrict__ fd, float *__restrict__ qx, float *__restrict__ qy,
-                              FrameCounters *fc)
+__device__ __forceinline__ void frame_begin_body(const float *__restrict__ pxp, const float *__restrict__ pyp, int W,
+                                                 int H, const FrameDev *__restrict__ fd, float *__restrict__ qx,
+                                                 float *__restrict__ qy, FrameCounters *fc)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const CamF &p = fd->p;
@@ -172,14 +172,30 @@ __global__ void k_frame_begin(const float *__restrict__ pxp, const float *__rest
     }
 }
 
+__global__ void k_frame_begin(const float *__restrict__ pxp, const float *__restrict__ pyp, int W, int H,
+                              const FrameDev *__restrict__ fd, float *__restrict__ qx, float *__restrict__ qy,
+                              FrameCounters *fc)
+{
+    frame_begin_body(pxp, pyp, W, H, fd, qx, qy, fc);
+}
+
+// the batched forms of the allocation kernels: stream blockIdx.z of S, grid
+// x (and y) sized for the largest stream; blocks outside a stream's range
+// return at once
+__global__ void k_frame_begin_b(const AllocStream *__restrict__ S)
+{
+    const AllocStream &a = S[blockIdx.z];
+    frame_begin_body(a.pxp, a.pyp, a.W, a.H, a.fd, a.qx, a.qy, a.fc);
+}
+
 // ---------------------------------------------------------------------------
 // k_planes: per 64-pixel row chunk, ballot the three relations into bit
 // planes: plane[(y*words + w)*3 + {0: valid, 1: key == left key, 2: key == up key}]
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(64) void k_planes(const FrameDev *__restrict__ fd, int W, int H, int words,
-                                               const float *__restrict__ qx, const float *__restrict__ qy,
-                                               uint32_t *__restrict__ planes, uint32_t *__restrict__ cntB,
-                                               uint32_t *__restrict__ cntL, uint32_t *__restrict__ firstD)
+__device__ __forceinline__ void planes_body(const FrameDev *__restrict__ fd, int W, int H, int words,
+                                            const float *__restrict__ qx, const float *__restrict__ qy,
+                                            uint32_t *__restrict__ planes, uint32_t *__restrict__ cntB,
+                                            uint32_t *__restrict__ cntL, uint32_t *__restrict__ firstD)
 {
     const int lane = threadIdx.x, y = blockIdx.y, x = blockIdx.x * 64 + lane;
     const uint16_t *__restrict__ depth = fd->depth;
@@ -220,6 +236,21 @@ __global__ __launch_bounds__(64) void k_planes(const FrameDev *__restrict__ fd, 
     }
 }
 
+__global__ __launch_bounds__(64) void k_planes(const FrameDev *__restrict__ fd, int W, int H, int words,
+                                               const float *__restrict__ qx, const float *__restrict__ qy,
+                                               uint32_t *__restrict__ planes, uint32_t *__restrict__ cntB,
+                                               uint32_t *__restrict__ cntL, uint32_t *__restrict__ firstD)
+{
+    planes_body(fd, W, H, words, qx, qy, planes, cntB, cntL, firstD);
+}
+
+__global__ __launch_bounds__(64) void k_planes_b(const AllocStream *__restrict__ S)
+{
+    const AllocStream &a = S[blockIdx.z];
+    if ((int)blockIdx.y >= a.H || (int)blockIdx.x * 64 >= a.W) return;
+    planes_body(a.fd, a.W, a.H, a.words, a.qx, a.qy, a.planes, a.cntB, a.cntL, a.firstD);
+}
+
 // ---------------------------------------------------------------------------
 // k_dedup: P(x,y) = valid & !(Eu & P(x,y-1)) & !(El & P(x-1,y)).
 // Wave 0 of the block solves the recurrence: lane l owns columns
@@ -238,7 +269,7 @@ __global__ __launch_bounds__(64) void k_planes(const FrameDev *__restrict__ fd, 
 #define FF_DD_THREADS 1024
 
 // R = 32 while one uint4 per loader thread covers a band (24 words <= 960)
-static int dedup_rows(int words) { return words <= 40 ? 32 : 16; }
+__host__ __device__ static inline int dedup_rows(int words) { return words <= 40 ? 32 : 16; }
 size_t dedup_lds_bytes(int words) { return (size_t)FF_DD_NB * dedup_rows(words) * words * 3 * sizeof(uint32_t); }
 int dedup_plane_rows(int H) { return (H + 31) / 32 * 32; }
 
@@ -249,10 +280,9 @@ __device__ __forceinline__ uint32_t lane_shr1(uint32_t v)
 }
 
 template <int R>
-__global__ __launch_bounds__(FF_DD_THREADS) void k_dedup(const uint32_t *__restrict__ planes, int H, int words,
-                                                         uint32_t *__restrict__ pbits, FrameCounters *fc)
+__device__ __forceinline__ void dedup_body(uint32_t *ring, const uint32_t *__restrict__ planes, int H, int words,
+                                           uint32_t *__restrict__ pbits, FrameCounters *fc)
 {
-    extern __shared__ uint32_t ring[];
     constexpr int RING = FF_DD_NB * R;  // rows, a power of two
     const int tid = threadIdx.x;
     const bool compute = tid < 64;
@@ -319,6 +349,24 @@ __global__ __launch_bounds__(FF_DD_THREADS) void k_dedup(const uint32_t *__restr
     }
 }
 
+template <int R>
+__global__ __launch_bounds__(FF_DD_THREADS) void k_dedup(const uint32_t *__restrict__ planes, int H, int words,
+                                                         uint32_t *__restrict__ pbits, FrameCounters *fc)
+{
+    extern __shared__ uint32_t ring[];
+    dedup_body<R>(ring, planes, H, words, pbits, fc);
+}
+
+// one block per stream whose row count per band is R
+template <int R>
+__global__ __launch_bounds__(FF_DD_THREADS) void k_dedup_b(const AllocStream *__restrict__ S)
+{
+    extern __shared__ uint32_t ring[];
+    const AllocStream &a = S[blockIdx.z];
+    if (dedup_rows(a.words) != R) return;
+    dedup_body<R>(ring, a.planes, a.H, a.words, a.pbits, a.fc);
+}
+
 // ---------------------------------------------------------------------------
 // k_descend: queryPointDepthSingle_func_subtree (traversal_functions.cpp:308-395)
 // read-only against the tree as it was before this frame.
@@ -332,9 +380,10 @@ __device__ __forceinline__ void queue_leaf(const MapView &m, uint32_t leaf, uint
     }
 }
 
-__global__ void k_descend(const FrameDev *__restrict__ fd, int W, int H, int words, const float *__restrict__ qx,
-                          const float *__restrict__ qy, const uint32_t *__restrict__ pbits, MapView m, HashView hv,
-                          uint32_t *__restrict__ pl0, FrameCounters *fc)
+__device__ __forceinline__ void descend_body(const FrameDev *__restrict__ fd, int W, int H, int words,
+                                             const float *__restrict__ qx, const float *__restrict__ qy,
+                                             const uint32_t *__restrict__ pbits, const MapView &m, const HashView &hv,
+                                             uint32_t *__restrict__ pl0, FrameCounters *fc)
 {
     const uint16_t *__restrict__ depth = fd->depth;
     const float scaling = fd->scaling;
@@ -435,9 +484,23 @@ __global__ void k_descend(const FrameDev *__restrict__ fd, int W, int H, int wor
     pl0[2 * pix + 1] = pld;
 }
 
+__global__ void k_descend(const FrameDev *__restrict__ fd, int W, int H, int words, const float *__restrict__ qx,
+                          const float *__restrict__ qy, const uint32_t *__restrict__ pbits, MapView m, HashView hv,
+                          uint32_t *__restrict__ pl0, FrameCounters *fc)
+{
+    descend_body(fd, W, H, words, qx, qy, pbits, m, hv, pl0, fc);
+}
+
+__global__ void k_descend_b(const AllocStream *__restrict__ S)
+{
+    const AllocStream &a = S[blockIdx.z];
+    if ((int)blockIdx.y >= a.H || (int)(blockIdx.x * blockDim.x) >= a.W) return;
+    descend_body(a.fd, a.W, a.H, a.words, a.qx, a.qy, a.pbits, a.m, a.hv, a.pl0, a.fc);
+}
+
 // per new node: its creating pixel counts it
-__global__ void k_resolve(HashView hv, const FrameCounters *fc, uint32_t *__restrict__ cntB,
-                          uint32_t *__restrict__ cntL, uint32_t *__restrict__ firstD)
+__device__ __forceinline__ void resolve_body(const HashView &hv, const FrameCounters *fc, uint32_t *__restrict__ cntB,
+                                             uint32_t *__restrict__ cntL, uint32_t *__restrict__ firstD)
 {
     const uint32_t n = min(fc->listCount, hv.listCap);
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
@@ -452,6 +515,18 @@ __global__ void k_resolve(HashView hv, const FrameCounters *fc, uint32_t *__rest
             cntL[pix] = 1u;
         }
     }
+}
+
+__global__ void k_resolve(HashView hv, const FrameCounters *fc, uint32_t *__restrict__ cntB,
+                          uint32_t *__restrict__ cntL, uint32_t *__restrict__ firstD)
+{
+    resolve_body(hv, fc, cntB, cntL, firstD);
+}
+
+__global__ void k_resolve_b(const AllocStream *__restrict__ S)
+{
+    const AllocStream &a = S[blockIdx.z];
+    resolve_body(a.hv, a.fc, a.cntB, a.cntL, a.firstD);
 }
 
 // ---------------------------------------------------------------------------
@@ -488,8 +563,8 @@ __device__ uint64_t block_excl_scan(uint64_t v, uint64_t *total)
     return r;
 }
 
-__global__ __launch_bounds__(1024) void k_scan_a(const uint32_t *__restrict__ cntB, const uint32_t *__restrict__ cntL,
-                                                 uint32_t n, uint64_t *__restrict__ pref, uint64_t *__restrict__ bsum)
+__device__ __forceinline__ void scan_a_body(const uint32_t *__restrict__ cntB, const uint32_t *__restrict__ cntL,
+                                            uint32_t n, uint64_t *__restrict__ pref, uint64_t *__restrict__ bsum)
 {
     const uint32_t base = blockIdx.x * 4096 + threadIdx.x * 4;
     uint64_t e[4], s = 0;
@@ -510,7 +585,20 @@ __global__ __launch_bounds__(1024) void k_scan_a(const uint32_t *__restrict__ cn
     if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
 }
 
-__global__ __launch_bounds__(1024) void k_scan_b(uint64_t *__restrict__ bsum, uint32_t nb, FrameCounters *fc)
+__global__ __launch_bounds__(1024) void k_scan_a(const uint32_t *__restrict__ cntB, const uint32_t *__restrict__ cntL,
+                                                 uint32_t n, uint64_t *__restrict__ pref, uint64_t *__restrict__ bsum)
+{
+    scan_a_body(cntB, cntL, n, pref, bsum);
+}
+
+__global__ __launch_bounds__(1024) void k_scan_a_b(const AllocStream *__restrict__ S)
+{
+    const AllocStream &a = S[blockIdx.z];
+    if (blockIdx.x >= a.nb) return;
+    scan_a_body(a.cntB, a.cntL, a.npix, a.pref, a.bsum);
+}
+
+__device__ __forceinline__ void scan_b_body(uint64_t *__restrict__ bsum, uint32_t nb, FrameCounters *fc)
 {
     const uint32_t per = (nb + 1023) / 1024;
     uint64_t loc[8], s = 0;
@@ -532,9 +620,20 @@ __global__ __launch_bounds__(1024) void k_scan_b(uint64_t *__restrict__ bsum, ui
     }
 }
 
-__global__ void k_assign(HashView hv, const FrameCounters *fc, const uint64_t *__restrict__ pref,
-                         const uint64_t *__restrict__ bsum, const uint32_t *__restrict__ firstD,
-                         const MapCounters *__restrict__ mapc)
+__global__ __launch_bounds__(1024) void k_scan_b(uint64_t *__restrict__ bsum, uint32_t nb, FrameCounters *fc)
+{
+    scan_b_body(bsum, nb, fc);
+}
+
+__global__ __launch_bounds__(1024) void k_scan_b_b(const AllocStream *__restrict__ S)
+{
+    const AllocStream &a = S[blockIdx.z];
+    scan_b_body(a.bsum, a.nb, a.fc);
+}
+
+__device__ __forceinline__ void assign_body(const HashView &hv, const FrameCounters *fc,
+                                            const uint64_t *__restrict__ pref, const uint64_t *__restrict__ bsum,
+                                            const uint32_t *__restrict__ firstD, const MapCounters *__restrict__ mapc)
 {
     const uint32_t nBranchesUsed = mapc->nBranchesUsed, nLeavesUsed = mapc->nLeavesUsed;
     const uint32_t n = min(fc->listCount, hv.listCap);
@@ -551,6 +650,19 @@ __global__ void k_assign(HashView hv, const FrameCounters *fc, const uint64_t *_
     }
 }
 
+__global__ void k_assign(HashView hv, const FrameCounters *fc, const uint64_t *__restrict__ pref,
+                         const uint64_t *__restrict__ bsum, const uint32_t *__restrict__ firstD,
+                         const MapCounters *__restrict__ mapc)
+{
+    assign_body(hv, fc, pref, bsum, firstD, mapc);
+}
+
+__global__ void k_assign_b(const AllocStream *__restrict__ S)
+{
+    const AllocStream &a = S[blockIdx.z];
+    assign_body(a.hv, a.fc, a.pref, a.bsum, a.firstD, a.mapc);
+}
+
 // parent branch's tree index of a node created this frame
 __device__ __forceinline__ uint32_t parent_branch(const HashView &hv, uint32_t aux, uint32_t d, uint32_t x,
                                                   uint32_t y, uint32_t z, uint32_t n)
@@ -562,8 +674,9 @@ __device__ __forceinline__ uint32_t parent_branch(const HashView &hv, uint32_t a
     return s >= 0 ? hv.assigned[s] : FF_BRANCHINIT;
 }
 
-__global__ void k_link(HashView hv, MapView m, const FrameCounters *fc, const uint32_t *__restrict__ pl0,
-                       const FrameDev *__restrict__ fd, FrameCounters *fcw)
+__device__ __forceinline__ void link_body(const HashView &hv, const MapView &m, const FrameCounters *fc,
+                                          const uint32_t *__restrict__ pl0, const FrameDev *__restrict__ fd,
+                                          FrameCounters *fcw)
 {
     const uint32_t n = fd->n, stamp = fd->stamp;
     const uint32_t cnt = min(fc->listCount, hv.listCap);
@@ -609,7 +722,19 @@ __global__ void k_link(HashView hv, MapView m, const FrameCounters *fc, const ui
     }
 }
 
-__global__ void k_hash_clear(HashView hv, const FrameCounters *fc)
+__global__ void k_link(HashView hv, MapView m, const FrameCounters *fc, const uint32_t *__restrict__ pl0,
+                       const FrameDev *__restrict__ fd, FrameCounters *fcw)
+{
+    link_body(hv, m, fc, pl0, fd, fcw);
+}
+
+__global__ void k_link_b(const AllocStream *__restrict__ S)
+{
+    const AllocStream &a = S[blockIdx.z];
+    link_body(a.hv, a.m, a.fc, a.pl0, a.fd, a.fc);
+}
+
+__device__ __forceinline__ void hash_clear_body(const HashView &hv, const FrameCounters *fc)
 {
     const uint32_t n = min(fc->listCount, hv.listCap);
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
@@ -619,14 +744,33 @@ __global__ void k_hash_clear(HashView hv, const FrameCounters *fc)
     }
 }
 
+__global__ void k_hash_clear(HashView hv, const FrameCounters *fc) { hash_clear_body(hv, fc); }
+
+__global__ void k_hash_clear_b(const AllocStream *__restrict__ S)
+{
+    const AllocStream &a = S[blockIdx.z];
+    hash_clear_body(a.hv, a.fc);
+}
+
 // the frame's new branches and bricks join the map's used counts (a frame
 // whose node table overflowed commits nothing; the host reports it)
-__global__ void k_commit(const FrameCounters *__restrict__ fc, MapCounters *mapc, uint32_t listCap)
+__device__ __forceinline__ void commit_body(const FrameCounters *__restrict__ fc, MapCounters *mapc, uint32_t listCap)
 {
     if (threadIdx.x == 0 && !(fc->flags & FF_FLAG_HASH_FULL) && fc->listCount <= listCap) {
         mapc->nBranchesUsed += 16u * fc->newB;
         mapc->nLeavesUsed += fc->newL;
     }
+}
+
+__global__ void k_commit(const FrameCounters *__restrict__ fc, MapCounters *mapc, uint32_t listCap)
+{
+    commit_body(fc, mapc, listCap);
+}
+
+__global__ void k_commit_b(const AllocStream *__restrict__ S)
+{
+    const AllocStream &a = S[blockIdx.z];
+    commit_body(a.fc, a.mapc, a.hv.listCap);
 }
 
 // ---------------------------------------------------------------------------
@@ -1049,6 +1193,10 @@ int ensure_image(ff_ctx *c, int W, int H)
                                (int)dedup_lds_bytes(40)));
     HIPCHK(hipFuncSetAttribute((const void *)k_dedup<16>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                (int)dedup_lds_bytes(64)));
+    HIPCHK(hipFuncSetAttribute((const void *)k_dedup_b<32>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)dedup_lds_bytes(40)));
+    HIPCHK(hipFuncSetAttribute((const void *)k_dedup_b<16>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)dedup_lds_bytes(64)));
     HIPCHK(hipMalloc(&c->d_pbits, pbits_bytes(H)));
     HIPCHK(hipMalloc(&c->d_cntB, sizeof(uint32_t) * npix));
     HIPCHK(hipMalloc(&c->d_cntL, sizeof(uint32_t) * npix));
@@ -1307,6 +1455,62 @@ void launch_alloc(ff_ctx *c, int W, int H)
     k_commit<<<1, 64, 0, c->stream>>>(c->d_fc, c->d_mapc, c->hv.listCap);
 }
 
+AllocStream alloc_stream(ff_ctx *c, int W, int H)
+{
+    AllocStream a;
+    a.fd = c->d_fd;
+    a.pxp = c->d_pxp; a.pyp = c->d_pyp;
+    a.qx = c->d_qx; a.qy = c->d_qy;
+    a.fc = c->d_fc;
+    a.planes = c->d_planes; a.cntB = c->d_cntB; a.cntL = c->d_cntL; a.firstD = c->d_firstD;
+    a.pbits = c->d_pbits; a.pl0 = c->d_pl0;
+    a.pref = c->d_pref; a.bsum = c->d_bsum;
+    a.mapc = c->d_mapc;
+    mesh_binfo(c, &a.binfo, &a.binfoCap);
+    a.m = c->mapView();
+    a.hv = c->hv;
+    a.W = W; a.H = H; a.words = (W + 31) / 32;
+    a.npix = (uint32_t)W * H;
+    a.nb = (a.npix + 4095) / 4096;
+    return a;
+}
+
+// launch_alloc for n streams at once: every kernel one launch, grid z = stream
+void launch_alloc_batch(hipStream_t st, const AllocStream *d_as, const AllocStream *h_as, int n)
+{
+    int maxW = 1, maxH = 1;
+    uint32_t maxNb = 1;
+    bool r32 = false, r16 = false;
+    size_t lds32 = 0, lds16 = 0;
+    for (int k = 0; k < n; k++) {
+        const AllocStream &a = h_as[k];
+        maxW = std::max(maxW, a.W);
+        maxH = std::max(maxH, a.H);
+        maxNb = std::max(maxNb, a.nb);
+        if (dedup_rows(a.words) == 32) {
+            r32 = true;
+            lds32 = std::max(lds32, dedup_lds_bytes(a.words));
+        } else {
+            r16 = true;
+            lds16 = std::max(lds16, dedup_lds_bytes(a.words));
+        }
+    }
+    const int G = 64;  // blocks per stream of the node-list kernels
+    k_frame_begin_b<<<dim3((std::max(maxW, maxH) + 255) / 256, 1, n), 256, 0, st>>>(d_as);
+    k_planes_b<<<dim3((maxW + 63) / 64, maxH, n), 64, 0, st>>>(d_as);
+    if (r32) k_dedup_b<32><<<dim3(1, 1, n), FF_DD_THREADS, lds32, st>>>(d_as);
+    if (r16) k_dedup_b<16><<<dim3(1, 1, n), FF_DD_THREADS, lds16, st>>>(d_as);
+    k_descend_b<<<dim3((maxW + 127) / 128, maxH, n), 128, 0, st>>>(d_as);
+    k_resolve_b<<<dim3(G, 1, n), 256, 0, st>>>(d_as);
+    k_scan_a_b<<<dim3(maxNb, 1, n), 1024, 0, st>>>(d_as);
+    k_scan_b_b<<<dim3(1, 1, n), 1024, 0, st>>>(d_as);
+    k_assign_b<<<dim3(G, 1, n), 256, 0, st>>>(d_as);
+    k_link_b<<<dim3(G, 1, n), 256, 0, st>>>(d_as);
+    launch_set_binfo_batch(st, d_as, n, G);
+    k_hash_clear_b<<<dim3(G, 1, n), 256, 0, st>>>(d_as);
+    k_commit_b<<<dim3(1, 1, n), 64, 0, st>>>(d_as);
+}
+
 // the SDF update waits for the host only when the node table overflowed or
 // the volume must grow first
 static void launch_sdf_only(ff_ctx *c, int W, int H)
@@ -1375,7 +1579,7 @@ static int run_frame(ff_ctx *c, int W, int H)
 // volume, column/row tables, pose).  depth / bgr: the frame in device
 // memory, dense rows.
 int frame_prepare(ff_ctx *c, const uint16_t *depth, const uint8_t *bgr, int W, int H, const double R[9],
-                  const double t[3], const double K[4], float scaling, float maxcam)
+                  const double t[3], const double K[4], float scaling, float maxcam, bool upload)
 {
     CamF p = make_p(c, R, t, K);
     FrameCounters *hf = c->h_fc;
@@ -1438,7 +1642,7 @@ int frame_prepare(ff_ctx *c, const uint16_t *depth, const uint8_t *bgr, int W, i
     f.n = c->n;
     f.bandwidth = c->bandwidth;
     f.pad = 0;
-    HIPCHK(hipMemcpyAsync(c->d_fd, c->h_fd, sizeof(FrameDev), hipMemcpyHostToDevice, c->stream));
+    if (upload) HIPCHK(hipMemcpyAsync(c->d_fd, c->h_fd, sizeof(FrameDev), hipMemcpyHostToDevice, c->stream));
     return FF_OK;
 }
 

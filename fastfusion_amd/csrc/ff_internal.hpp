@@ -95,6 +95,23 @@ struct SdfStream {
     int W, H;
 };
 
+// one stream of a batched allocation phase: every buffer and size the
+// allocation kernels of that stream take (grid z = stream)
+struct AllocStream {
+    const FrameDev *fd;
+    const float *pxp, *pyp;
+    float *qx, *qy;
+    FrameCounters *fc;
+    uint32_t *planes, *cntB, *cntL, *firstD, *pbits, *pl0;
+    uint64_t *pref, *bsum;
+    MapCounters *mapc;
+    ushort4 *binfo;  // mesh-side branch geometry (k_set_binfo)
+    MapView m;
+    HashView hv;
+    int W, H, words;
+    uint32_t npix, nb, binfoCap;
+};
+
 struct MeshState;
 struct LearnedState;
 struct BatchState;
@@ -170,8 +187,9 @@ void drop_frame_graphs(ff_ctx *c);
 // a frame in phases (ff_fusion.hip): host preparation and parameter upload,
 // device work (allocation, then confidence + SDF; or the empty-frame path),
 // and the host part after the device work completed
+// upload = false: the frame block is left in c->h_fd for the caller to copy
 int frame_prepare(ff_ctx *c, const uint16_t *depth, const uint8_t *bgr, int W, int H, const double R[9],
-                  const double t[3], const double K[4], float scaling, float maxcam);
+                  const double t[3], const double K[4], float scaling, float maxcam, bool upload = true);
 void launch_alloc(ff_ctx *c, int W, int H);
 void launch_sdf(ff_ctx *c, int W, int H);
 void launch_empty(ff_ctx *c, int W, int H);
@@ -191,6 +209,12 @@ void mesh_on_initial_volume(ff_ctx *c);
 void mesh_on_grow(ff_ctx *c);
 void mesh_after_frame(ff_ctx *c);
 void mesh_record_branches(ff_ctx *c);
+void mesh_binfo(const ff_ctx *c, ushort4 **binfo, uint32_t *cap);
+void launch_set_binfo_batch(hipStream_t st, const AllocStream *d_as, int n, int grid);
+// the batched allocation phase of n streams (descriptors in device memory
+// d_as, host copy h_as for the grid sizes)
+AllocStream alloc_stream(ff_ctx *c, int W, int H);
+void launch_alloc_batch(hipStream_t st, const AllocStream *d_as, const AllocStream *h_as, int n);
 void mesh_on_grow_blocks(ff_ctx *c, uint32_t oldBlocks, uint32_t addBlocks, const uint32_t addv[3],
                          const std::vector<ushort4> &chain);
 }  // namespace ff

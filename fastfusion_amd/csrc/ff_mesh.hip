@@ -97,6 +97,33 @@ struct MapRO {
     int useColor;
 };
 
+// one stream of a batched mesh update: every buffer and size the per-stream
+// mesh kernels take (grid z = stream)
+struct MeshBatchStream {
+    const uint32_t *tree;
+    const ushort4 *binfo;
+    uint32_t nBlocks, nLeavesTotal, n;
+    uint4 *term;
+    uint32_t termCap;
+    MeshCounters *mc;
+    CellRec *cells;       // the table being built
+    const CellRec *prev;  // the previous table
+    uint32_t cellCap, nPrev;
+    uint64_t *hkeys;
+    uint32_t *hval;
+    uint32_t hmask;
+    int havePrev;
+    const uint32_t *stamp, *parent;
+    uint32_t since;
+    struct CellGeom *geom;
+    MapRO m;
+    uint32_t *tiles;
+    uint32_t tileCap;
+    uint4 *sbuf;
+    const unsigned long long *arena;
+    int arenaSum;  // totals of the arena counters into mc->pad (timing)
+};
+
 struct MeshState {
     ushort4 *d_binfo = nullptr;  // per branch block: origin of the node, octant size
     uint32_t binfoCap = 0;
@@ -123,7 +150,11 @@ struct MeshState {
     int nxt = 1;   // table being built by the update in flight
     MapRO m{};     // its map view
     // batched updates led by this context: stream descriptors and prefix
-    struct McStream *d_ms = nullptr, *h_ms = nullptr;
+    struct McStream *d_ms = nullptr, *h_ms = nullptr;  // inside d_mdesc / h_mdesc
+    MeshBatchStream *d_mbs = nullptr, *h_mbs = nullptr;
+    char *d_mdesc = nullptr, *h_mdesc = nullptr;  // one block: one upload per update
+    size_t mdescBytes = 0;
+    MeshCounters *d_mcs = nullptr, *h_mcs = nullptr;  // the streams' counters, gathered
     uint32_t *d_msPrefix = nullptr;
     int msCap = 0;
     uint64_t lastVAlloc = 0, lastFAlloc = 0;  // arena totals after the previous update (timing)
@@ -183,9 +214,9 @@ __device__ __forceinline__ uint32_t wave_append(uint32_t *counter, bool want)
 // ---------------------------------------------------------------------------
 // k_terminal: covered terminal nodes
 // ---------------------------------------------------------------------------
-__global__ void k_terminal(const uint32_t *__restrict__ tree, const ushort4 *__restrict__ binfo, uint32_t nBlocks,
-                           uint32_t nLeavesTotal, uint32_t n, uint4 *__restrict__ term, uint32_t termCap,
-                           MeshCounters *mc)
+__device__ __forceinline__ void terminal_body(const uint32_t *__restrict__ tree, const ushort4 *__restrict__ binfo,
+                                              uint32_t nBlocks, uint32_t nLeavesTotal, uint32_t n,
+                                              uint4 *__restrict__ term, uint32_t termCap, MeshCounters *mc)
 {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     bool want = false;
@@ -213,6 +244,21 @@ __global__ void k_terminal(const uint32_t *__restrict__ tree, const ushort4 *__r
     }
 }
 
+__global__ void k_terminal(const uint32_t *__restrict__ tree, const ushort4 *__restrict__ binfo, uint32_t nBlocks,
+                           uint32_t nLeavesTotal, uint32_t n, uint4 *__restrict__ term, uint32_t termCap,
+                           MeshCounters *mc)
+{
+    terminal_body(tree, binfo, nBlocks, nLeavesTotal, n, term, termCap, mc);
+}
+
+// the batched forms of the per-stream mesh kernels: stream blockIdx.z of S
+__global__ void k_terminal_b(const MeshBatchStream *__restrict__ S)
+{
+    const MeshBatchStream &a = S[blockIdx.z];
+    if (blockIdx.x * blockDim.x >= a.nBlocks * 8u) return;
+    terminal_body(a.tree, a.binfo, a.nBlocks, a.nLeavesTotal, a.n, a.term, a.termCap, a.mc);
+}
+
 // ---------------------------------------------------------------------------
 // k_cells: cells owned by each covered terminal node
 // ---------------------------------------------------------------------------
@@ -223,9 +269,9 @@ __global__ void k_terminal(const uint32_t *__restrict__ tree, const ushort4 *__r
 // itself), then one thread per (node, candidate) decides ownership from them.
 __device__ __forceinline__ int opt_code(int ox, int oy, int oz) { return ox + 4 * oy + 16 * oz; }
 
-__global__ __launch_bounds__(256) void k_cells(const uint32_t *__restrict__ tree, uint32_t nLeavesTotal, uint32_t n,
-                                               const uint4 *__restrict__ term, uint32_t termCap, MeshCounters *mc,
-                                               CellRec *cells, uint32_t cap)
+__device__ __forceinline__ void cells_body(const uint32_t *__restrict__ tree, uint32_t nLeavesTotal, uint32_t n,
+                                           const uint4 *__restrict__ term, uint32_t termCap, MeshCounters *mc,
+                                           CellRec *cells, uint32_t cap)
 {
     __shared__ uint2 nb[4][64];  // (size, lastLeaf) per node and option point
     const uint32_t nt = min(mc->nTerm, termCap);
@@ -343,6 +389,19 @@ __global__ __launch_bounds__(256) void k_cells(const uint32_t *__restrict__ tree
         }
         __syncthreads();
     }
+}
+
+__global__ __launch_bounds__(256) void k_cells(const uint32_t *__restrict__ tree, uint32_t nLeavesTotal, uint32_t n,
+                                               const uint4 *__restrict__ term, uint32_t termCap, MeshCounters *mc,
+                                               CellRec *cells, uint32_t cap)
+{
+    cells_body(tree, nLeavesTotal, n, term, termCap, mc, cells, cap);
+}
+
+__global__ __launch_bounds__(256) void k_cells_b(const MeshBatchStream *__restrict__ S)
+{
+    const MeshBatchStream &a = S[blockIdx.z];
+    cells_body(a.tree, a.nLeavesTotal, a.n, a.term, a.termCap, a.mc, a.cells, a.cellCap);
 }
 
 // ---------------------------------------------------------------------------
@@ -657,10 +716,12 @@ __device__ __forceinline__ uint4 pack_sample(const Sample &s)
                       (uint32_t)s.c[2]);
 }
 
-__global__ void k_mark(CellRec *cells, const CellRec *__restrict__ prev, const uint64_t *__restrict__ hkeys,
-                       const uint32_t *__restrict__ hval, uint32_t hmask, bool havePrev,
-                       const uint32_t *__restrict__ stamp, const uint32_t *__restrict__ parent, uint32_t since,
-                       MeshCounters *mc, CellGeom *geom, uint32_t cap, MapRO m, uint32_t *tiles, uint32_t tileCap)
+__device__ __forceinline__ void mark_body(CellRec *cells, const CellRec *__restrict__ prev,
+                                          const uint64_t *__restrict__ hkeys, const uint32_t *__restrict__ hval,
+                                          uint32_t hmask, bool havePrev, const uint32_t *__restrict__ stamp,
+                                          const uint32_t *__restrict__ parent, uint32_t since, MeshCounters *mc,
+                                          CellGeom *geom, uint32_t cap, const MapRO &m, uint32_t *tiles,
+                                          uint32_t tileCap)
 {
     const uint32_t nc = min(mc->nCells, cap);
     const uint32_t lane = threadIdx.x & 63u;
@@ -721,8 +782,23 @@ __global__ void k_mark(CellRec *cells, const CellRec *__restrict__ prev, const u
     }
 }
 
-__global__ void k_hash_build(CellRec *cells, const MeshCounters *mc, uint32_t cap, uint64_t *hkeys, uint32_t *hval,
-                             uint32_t hmask)
+__global__ void k_mark(CellRec *cells, const CellRec *__restrict__ prev, const uint64_t *__restrict__ hkeys,
+                       const uint32_t *__restrict__ hval, uint32_t hmask, bool havePrev,
+                       const uint32_t *__restrict__ stamp, const uint32_t *__restrict__ parent, uint32_t since,
+                       MeshCounters *mc, CellGeom *geom, uint32_t cap, MapRO m, uint32_t *tiles, uint32_t tileCap)
+{
+    mark_body(cells, prev, hkeys, hval, hmask, havePrev, stamp, parent, since, mc, geom, cap, m, tiles, tileCap);
+}
+
+__global__ void k_mark_b(const MeshBatchStream *__restrict__ S)
+{
+    const MeshBatchStream &a = S[blockIdx.z];
+    mark_body(a.cells, a.prev, a.hkeys, a.hval, a.hmask, a.havePrev != 0, a.stamp, a.parent, a.since, a.mc, a.geom,
+              a.cellCap, a.m, a.tiles, a.tileCap);
+}
+
+__device__ __forceinline__ void hash_build_body(CellRec *cells, const MeshCounters *mc, uint32_t cap, uint64_t *hkeys,
+                                                uint32_t *hval, uint32_t hmask)
 {
     const uint32_t nc = min(mc->nCells, cap);
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nc; i += gridDim.x * blockDim.x) {
@@ -741,9 +817,21 @@ __global__ void k_hash_build(CellRec *cells, const MeshCounters *mc, uint32_t ca
     }
 }
 
+__global__ void k_hash_build(CellRec *cells, const MeshCounters *mc, uint32_t cap, uint64_t *hkeys, uint32_t *hval,
+                             uint32_t hmask)
+{
+    hash_build_body(cells, mc, cap, hkeys, hval, hmask);
+}
+
+__global__ void k_hash_build_b(const MeshBatchStream *__restrict__ S)
+{
+    const MeshBatchStream &a = S[blockIdx.z];
+    hash_build_body(a.cells, a.mc, a.cellCap, a.hkeys, a.hval, a.hmask);
+}
+
 // empties the slots of a whole table (by recorded slot, so concurrent
 // clears cannot cut each other's probe chains)
-__global__ void k_hash_clear_cells(const CellRec *__restrict__ cells, uint32_t nc, uint64_t *hkeys)
+__device__ __forceinline__ void hash_clear_cells_body(const CellRec *__restrict__ cells, uint32_t nc, uint64_t *hkeys)
 {
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nc; i += gridDim.x * blockDim.x) {
         const uint32_t h = cells[i].hslot;
@@ -751,13 +839,25 @@ __global__ void k_hash_clear_cells(const CellRec *__restrict__ cells, uint32_t n
     }
 }
 
+__global__ void k_hash_clear_cells(const CellRec *__restrict__ cells, uint32_t nc, uint64_t *hkeys)
+{
+    hash_clear_cells_body(cells, nc, hkeys);
+}
+
+// (the previous table of each stream)
+__global__ void k_hash_clear_cells_b(const MeshBatchStream *__restrict__ S)
+{
+    const MeshBatchStream &a = S[blockIdx.z];
+    hash_clear_cells_body(a.prev, a.nPrev, a.hkeys);
+}
+
 
 // ---------------------------------------------------------------------------
 // k_sample: one thread per sample of the dirty cells' 64-sample tiles
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_sample(const CellGeom *__restrict__ geom, const CellRec *__restrict__ cells,
-                                                const uint32_t *__restrict__ tiles, const MeshCounters *mc,
-                                                uint32_t tileCap, MapRO m, uint4 *__restrict__ sbuf)
+__device__ __forceinline__ void sample_body(const CellGeom *__restrict__ geom, const CellRec *__restrict__ cells,
+                                            const uint32_t *__restrict__ tiles, const MeshCounters *mc,
+                                            uint32_t tileCap, const MapRO &m, uint4 *__restrict__ sbuf)
 {
     const uint32_t nt = min(mc->nTiles, tileCap);
     for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < nt * 64u; t += gridDim.x * blockDim.x) {
@@ -768,6 +868,19 @@ __global__ __launch_bounds__(256) void k_sample(const CellGeom *__restrict__ geo
         if (s >= geom_total(g)) continue;
         sbuf[t] = pack_sample(sample_cell(m, g, cells[g.cell].ll, s));
     }
+}
+
+__global__ __launch_bounds__(256) void k_sample(const CellGeom *__restrict__ geom, const CellRec *__restrict__ cells,
+                                                const uint32_t *__restrict__ tiles, const MeshCounters *mc,
+                                                uint32_t tileCap, MapRO m, uint4 *__restrict__ sbuf)
+{
+    sample_body(geom, cells, tiles, mc, tileCap, m, sbuf);
+}
+
+__global__ __launch_bounds__(256) void k_sample_b(const MeshBatchStream *__restrict__ S)
+{
+    const MeshBatchStream &a = S[blockIdx.z];
+    sample_body(a.geom, a.cells, a.tiles, a.mc, a.tileCap, a.m, a.sbuf);
 }
 
 // ---------------------------------------------------------------------------
@@ -1240,7 +1353,8 @@ void mesh_free(ff_ctx *c)
     hipFree(s->d_geom); hipFree(s->d_tiles); hipFree(s->d_sbuf);
     hipFree(s->d_hkeys); hipFree(s->d_hval); hipFree(s->d_vpool); hipFree(s->d_fpool);
     hipFree(s->d_arena);
-    hipFree(s->d_ms); hipHostFree(s->h_ms); hipFree(s->d_msPrefix);
+    hipFree(s->d_mdesc); hipHostFree(s->h_mdesc); hipFree(s->d_msPrefix);
+    hipFree(s->d_mcs); hipHostFree(s->h_mcs);
     hipFree(s->d_mc); hipHostFree(s->h_mc);
     delete s;
     c->mesh = nullptr;
@@ -1277,9 +1391,10 @@ void mesh_on_grow_blocks(ff_ctx *c, uint32_t oldBlocks, uint32_t addBlocks, cons
 void mesh_on_grow(ff_ctx *) {}
 void mesh_after_frame(ff_ctx *) {}
 
-__global__ void k_set_binfo(ushort4 *binfo, const uint64_t *__restrict__ keys, const uint32_t *__restrict__ assigned,
-                            const uint32_t *__restrict__ list, const uint32_t *listCount, uint32_t listCap,
-                            const FrameDev *__restrict__ fd, uint32_t nBlocks)
+__device__ __forceinline__ void set_binfo_body(ushort4 *binfo, const uint64_t *__restrict__ keys,
+                                               const uint32_t *__restrict__ assigned, const uint32_t *__restrict__ list,
+                                               const uint32_t *listCount, uint32_t listCap,
+                                               const FrameDev *__restrict__ fd, uint32_t nBlocks)
 {
     const uint32_t n = fd->n;
     const uint32_t cnt = min(*listCount, listCap);
@@ -1293,6 +1408,31 @@ __global__ void k_set_binfo(ushort4 *binfo, const uint64_t *__restrict__ keys, c
         binfo[idx / 16] = make_ushort4((unsigned short)((k >> 32) & 0xffffu), (unsigned short)((k >> 16) & 0xffffu),
                                        (unsigned short)(k & 0xffffu), (unsigned short)((n >> d) >> 1));
     }
+}
+
+__global__ void k_set_binfo(ushort4 *binfo, const uint64_t *__restrict__ keys, const uint32_t *__restrict__ assigned,
+                            const uint32_t *__restrict__ list, const uint32_t *listCount, uint32_t listCap,
+                            const FrameDev *__restrict__ fd, uint32_t nBlocks)
+{
+    set_binfo_body(binfo, keys, assigned, list, listCount, listCap, fd, nBlocks);
+}
+
+__global__ void k_set_binfo_b(const AllocStream *__restrict__ S)
+{
+    const AllocStream &a = S[blockIdx.z];
+    set_binfo_body(a.binfo, a.hv.keys, a.hv.assigned, a.hv.list, &a.fc->listCount, a.hv.listCap, a.fd, a.binfoCap);
+}
+
+// the branch-geometry table of a context (for a batched allocation phase)
+void mesh_binfo(const ff_ctx *c, ushort4 **binfo, uint32_t *cap)
+{
+    *binfo = c->mesh->d_binfo;
+    *cap = c->mesh->binfoCap;
+}
+
+void launch_set_binfo_batch(hipStream_t st, const AllocStream *d_as, int n, int grid)
+{
+    k_set_binfo_b<<<dim3(grid, 1, n), 256, 0, st>>>(d_as);
 }
 
 // records the node geometry of the branches created by this frame
@@ -1327,6 +1467,43 @@ __global__ void k_mesh_begin(MeshCounters *mc)
     }
 }
 
+__global__ void k_mesh_begin_b(const MeshBatchStream *__restrict__ S)
+{
+    if (threadIdx.x == 0) {
+        MeshCounters z;
+        z.nTerm = z.nCells = z.nDirty = z.flags = z.nTiles = 0u;
+        z.pad[0] = z.pad[1] = z.pad[2] = 0u;
+        *S[blockIdx.z].mc = z;
+    }
+}
+
+// every stream's counters into one array (one copy back), after the arena
+// totals of streams that account bytes
+__global__ void k_mesh_gather_b(const MeshBatchStream *__restrict__ S, MeshCounters *__restrict__ out)
+{
+    const MeshBatchStream &a = S[blockIdx.z];
+    const int lane = threadIdx.x;
+    uint32_t v = 0, f = 0;
+    if (a.arenaSum) {
+        const unsigned long long x = lane < (int)FF_ARENAS ? a.arena[lane * FF_ARENA_STRIDE] : 0ull;
+        v = (uint32_t)x;
+        f = (uint32_t)(x >> 32);
+        for (int o = 32; o > 0; o >>= 1) {
+            v += __shfl_xor(v, o);
+            f += __shfl_xor(f, o);
+        }
+    }
+    if (lane == 0) {
+        MeshCounters r = *a.mc;
+        if (a.arenaSum) {
+            r.pad[0] = v;
+            r.pad[1] = f;
+            *a.mc = r;
+        }
+        out[blockIdx.z] = r;
+    }
+}
+
 static Pools pools(const MeshState *s)
 {
     Pools P;
@@ -1341,30 +1518,81 @@ static Pools pools(const MeshState *s)
 // update_meshes in phases (so that a batch can share the marching-cubes
 // launch): pre -- cells, dirty marks, samples; the marching cubes (mc);
 // post -- lookup hash, counters' copy back; finish -- after the device work.
-bool mesh_launch_pre(ff_ctx *c, bool timed)
+// host part of pre: the table to build and the map view (false: nothing
+// to mesh)
+static bool mesh_pre_host(ff_ctx *c)
 {
     MeshState *s = c->mesh;
     if (!s->haveVolume || !c->n) return false;
+    s->nxt = 1 - s->cur;
+    MapRO m;
+    m.info = c->d_leafinfo; m.parent = c->d_parent; m.dist = c->d_dist; m.wgt = c->d_wgt; m.col = c->d_col;
+    for (int k = 0; k < 3; k++) m.off[k] = c->offset[k];
+    m.scale = c->scale;
+    m.useColor = c->useColor ? 1 : 0;
+    s->m = m;
+    return true;
+}
+
+bool mesh_launch_pre(ff_ctx *c, bool timed)
+{
+    MeshState *s = c->mesh;
+    if (!mesh_pre_host(c)) return false;
     if (timed && c->timing) hipEventRecord(c->ev[6], c->stream);
-    const int nxt = 1 - s->cur;
-    s->nxt = nxt;
+    const int nxt = s->nxt;
     k_mesh_begin<<<1, 64, 0, c->stream>>>(s->d_mc);
     const uint32_t nBlocks = c->nBranchesUsed / 16;
     k_terminal<<<(nBlocks * 8 + 255) / 256, 256, 0, c->stream>>>(c->d_tree, s->d_binfo, nBlocks, c->nLeavesTotal, c->n,
                                                                  s->d_term, s->termCap, s->d_mc);
     k_cells<<<2048, 256, 0, c->stream>>>(c->d_tree, c->nLeavesTotal, c->n, s->d_term, s->termCap, s->d_mc,
                                          s->d_cells[nxt], s->cellCap);
-    MapRO m;
-    m.info = c->d_leafinfo; m.parent = c->d_parent; m.dist = c->d_dist; m.wgt = c->d_wgt; m.col = c->d_col;
-    for (int k = 0; k < 3; k++) m.off[k] = c->offset[k];
-    m.scale = c->scale;
-    m.useColor = c->useColor ? 1 : 0;
+    const MapRO m = s->m;
     k_mark<<<1024, 256, 0, c->stream>>>(s->d_cells[nxt], s->d_cells[s->cur], s->d_hkeys, s->d_hval, s->hmask,
                                         s->nCells[s->cur] > 0, c->d_stamp, c->d_parent, s->lastMeshStamp, s->d_mc,
                                         s->d_geom, s->cellCap, m, s->d_tiles, s->tileCap);
     k_sample<<<4096, 256, 0, c->stream>>>(s->d_geom, s->d_cells[nxt], s->d_tiles, s->d_mc, s->tileCap, m, s->d_sbuf);
-    s->m = m;
     return true;
+}
+
+// the descriptor of a context in a batched update (after mesh_pre_host)
+static MeshBatchStream mesh_batch_stream(ff_ctx *c)
+{
+    MeshState *s = c->mesh;
+    MeshBatchStream a;
+    a.tree = c->d_tree;
+    a.binfo = s->d_binfo;
+    a.nBlocks = c->nBranchesUsed / 16;
+    a.nLeavesTotal = c->nLeavesTotal;
+    a.n = c->n;
+    a.term = s->d_term;
+    a.termCap = s->termCap;
+    a.mc = s->d_mc;
+    a.cells = s->d_cells[s->nxt];
+    a.prev = s->d_cells[s->cur];
+    a.cellCap = s->cellCap;
+    a.nPrev = s->nCells[s->cur];
+    a.hkeys = s->d_hkeys;
+    a.hval = s->d_hval;
+    a.hmask = s->hmask;
+    a.havePrev = s->nCells[s->cur] > 0;
+    a.stamp = c->d_stamp;
+    a.parent = c->d_parent;
+    a.since = s->lastMeshStamp;
+    a.geom = s->d_geom;
+    a.m = s->m;
+    a.tiles = s->d_tiles;
+    a.tileCap = s->tileCap;
+    a.sbuf = s->d_sbuf;
+    a.arena = s->d_arena;
+    a.arenaSum = c->timing ? 1 : 0;
+    return a;
+}
+
+// grid x per stream of a batched launch: the single-stream grid spread
+// over the streams (4x oversubscribed), within [lo, single]
+static int batch_grid(int single, int n, int lo)
+{
+    return std::max(lo, std::min(single, single * 4 / std::max(n, 1)));
 }
 
 McStream mesh_stream(ff_ctx *c)
@@ -1674,13 +1902,25 @@ int ff_update_meshes_batch(ff_ctx *const *ctxs, int n)
     hipSetDevice(lead->device);
     MeshState *ls = lead->mesh;
     if (ls->msCap < n) {
-        hipFree(ls->d_ms); hipHostFree(ls->h_ms); hipFree(ls->d_msPrefix);
+        hipFree(ls->d_mdesc); hipHostFree(ls->h_mdesc); hipFree(ls->d_msPrefix);
+        hipFree(ls->d_mcs); hipHostFree(ls->h_mcs);
+        ls->d_mdesc = ls->h_mdesc = nullptr;
         ls->d_ms = ls->h_ms = nullptr;
+        ls->d_mbs = ls->h_mbs = nullptr;
         ls->d_msPrefix = nullptr;
+        ls->d_mcs = ls->h_mcs = nullptr;
         ls->msCap = 0;
-        MCHK_CTX(lead, hipMalloc(&ls->d_ms, sizeof(McStream) * n));
-        MCHK_CTX(lead, hipHostMalloc(&ls->h_ms, sizeof(McStream) * n, hipHostMallocDefault));
+        const size_t a = (sizeof(McStream) * n + 255) & ~(size_t)255;
+        ls->mdescBytes = a + sizeof(MeshBatchStream) * n;
+        MCHK_CTX(lead, hipMalloc(&ls->d_mdesc, ls->mdescBytes));
+        MCHK_CTX(lead, hipHostMalloc(&ls->h_mdesc, ls->mdescBytes, hipHostMallocDefault));
         MCHK_CTX(lead, hipMalloc(&ls->d_msPrefix, sizeof(uint32_t) * (n + 1)));
+        MCHK_CTX(lead, hipMalloc(&ls->d_mcs, sizeof(MeshCounters) * n));
+        MCHK_CTX(lead, hipHostMalloc(&ls->h_mcs, sizeof(MeshCounters) * n, hipHostMallocDefault));
+        ls->d_ms = reinterpret_cast<McStream *>(ls->d_mdesc);
+        ls->h_ms = reinterpret_cast<McStream *>(ls->h_mdesc);
+        ls->d_mbs = reinterpret_cast<MeshBatchStream *>(ls->d_mdesc + a);
+        ls->h_mbs = reinterpret_cast<MeshBatchStream *>(ls->h_mdesc + a);
         ls->msCap = n;
     }
     // every context runs on the lead's stream for the update
@@ -1693,15 +1933,30 @@ int ff_update_meshes_batch(ff_ctx *const *ctxs, int n)
     }
     std::vector<int> act;
     for (int i = 0; i < n; i++)
-        if (ctxs[i]->incremental && mesh_launch_pre(ctxs[i], false)) act.push_back(i);
+        if (ctxs[i]->incremental && mesh_pre_host(ctxs[i])) act.push_back(i);
     const int na = (int)act.size();
-    for (int k = 0; k < na; k++) ls->h_ms[k] = mesh_stream(ctxs[act[k]]);
     if (na) {
-        hipMemcpyAsync(ls->d_ms, ls->h_ms, sizeof(McStream) * na, hipMemcpyHostToDevice, st);
+        // one upload of all descriptors, one launch per kernel, one copy back
+        uint32_t maxT = 1;
+        for (int k = 0; k < na; k++) {
+            ls->h_mbs[k] = mesh_batch_stream(ctxs[act[k]]);
+            ls->h_ms[k] = mesh_stream(ctxs[act[k]]);
+            maxT = std::max(maxT, (ls->h_mbs[k].nBlocks * 8u + 255u) / 256u);
+        }
+        hipMemcpyAsync(ls->d_mdesc, ls->h_mdesc, ls->mdescBytes, hipMemcpyHostToDevice, st);
+        const MeshBatchStream *S = ls->d_mbs;
+        k_mesh_begin_b<<<dim3(1, 1, na), 64, 0, st>>>(S);
+        k_terminal_b<<<dim3(maxT, 1, na), 256, 0, st>>>(S);
+        k_cells_b<<<dim3(batch_grid(2048, na, 64), 1, na), 256, 0, st>>>(S);
+        k_mark_b<<<dim3(batch_grid(1024, na, 32), 1, na), 256, 0, st>>>(S);
+        k_sample_b<<<dim3(batch_grid(4096, na, 64), 1, na), 256, 0, st>>>(S);
         k_mc_prefix<<<1, 64, 0, st>>>(ls->d_ms, na, ls->d_msPrefix);
         k_mc_batch<<<2048, 256, 0, st>>>(ls->d_ms, na, ls->d_msPrefix);
+        k_hash_clear_cells_b<<<dim3(batch_grid(1024, na, 32), 1, na), 256, 0, st>>>(S);
+        k_hash_build_b<<<dim3(batch_grid(1024, na, 32), 1, na), 256, 0, st>>>(S);
+        k_mesh_gather_b<<<dim3(1, 1, na), 64, 0, st>>>(S, ls->d_mcs);
+        hipMemcpyAsync(ls->h_mcs, ls->d_mcs, sizeof(MeshCounters) * na, hipMemcpyDeviceToHost, st);
     }
-    for (int i : act) mesh_launch_post(ctxs[i], false);
     hipError_t e = hipGetLastError();
     if (e == hipSuccess) e = hipStreamSynchronize(st);
     int rc = FF_OK;
@@ -1709,8 +1964,10 @@ int ff_update_meshes_batch(ff_ctx *const *ctxs, int n)
         lead->err = std::string("ff_update_meshes_batch: ") + hipGetErrorString(e);
         rc = FF_E_DEVICE;
     } else {
-        for (int i : act) {
-            const int r = mesh_finish(ctxs[i], false);
+        for (int k = 0; k < na; k++) {
+            ff_ctx *c = ctxs[act[k]];
+            memcpy(c->mesh->h_mc, &ls->h_mcs[k], sizeof(MeshCounters));
+            const int r = mesh_finish(c, false);
             if (r < 0 && rc == FF_OK) rc = r;
         }
     }
