@@ -4,11 +4,13 @@
 // The reference updates voxels with the SSE/AVX rounding mode set to
 // round-toward-zero (src/fusion/update_functions.cpp:1678-1679) and the
 // RCPPS approximate reciprocal (update_functions.cpp:1246, 1373, 1386).
-// Here every round-toward-zero operation is computed in the default
-// round-to-nearest mode together with its exact rounding error (TwoSum,
-// FMA-based TwoProduct); when the nearest result lies farther from zero than
-// the exact one, it is stepped one ulp toward zero.  This gives the
-// round-toward-zero result bit for bit without touching the MODE register.
+// The voxel kernels do the same: their first statement sets the f32
+// rounding mode of the MODE register to round-toward-zero (rz_mode_on), and
+// rz_add / rz_sub / rz_mul are then single hardware operations, named so
+// that the reference's operation order stays visible (the build does not
+// contract them: -ffp-contract=off).  The MODE register is per wave, so the
+// setting ends with the kernel's waves.  RCPPS is a table lookup and the
+// x86 truncating conversions are restated below.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -19,34 +21,18 @@
 
 namespace ff {
 
-// r: round-to-nearest result; e: exact error (exact value minus r), or NaN
-// when the operation involved an infinity.  Returns the round-toward-zero
-// result.
-__device__ __forceinline__ float rz_fix(float r, float e)
-{
-    const uint32_t rb = __float_as_uint(r);
-    const bool adjust = (e != 0.0f) && (e == e) && (((__float_as_uint(e) ^ rb) & 0x80000000u) != 0u);
-    return adjust ? __uint_as_float(rb - 1u) : r;
-}
+// sets the f32 round mode of the calling wave to round-toward-zero: the
+// first statement of every kernel that uses the functions below
+__device__ __forceinline__ void rz_mode_on() { __builtin_amdgcn_s_setreg(2049 /* hwreg(MODE, 0, 2) */, 3); }
 
-__device__ __forceinline__ float rz_mul(float a, float b)
-{
-    const float p = a * b;
-    return rz_fix(p, __builtin_fmaf(a, b, -p));
-}
-
-__device__ __forceinline__ float rz_add(float a, float b)
-{
-    const float s = a + b;
-    const float bb = s - a;
-    const float e = (a - (s - bb)) + (b - bb);
-    return rz_fix(s, e);
-}
-
-__device__ __forceinline__ float rz_sub(float a, float b) { return rz_add(a, -b); }
+// valid only after rz_mode_on()
+__device__ __forceinline__ float rz_mul(float a, float b) { return a * b; }
+__device__ __forceinline__ float rz_add(float a, float b) { return a + b; }
+__device__ __forceinline__ float rz_sub(float a, float b) { return a - b; }
 
 // sqrt rounded toward zero: start from the hardware estimate and correct it
-// with exact FMA residuals until s*s <= x < next(s)^2.
+// with FMA residuals until s*s <= x < next(s)^2 (only the residuals' signs
+// are used, which no rounding mode changes here).
 __device__ __forceinline__ float rz_sqrt(float x)
 {
     if (!(x > 0.0f) || x == __builtin_huge_valf()) return __builtin_sqrtf(x);

@@ -995,6 +995,7 @@ __global__ __launch_bounds__(512) void k_sdf(const uint32_t *__restrict__ queue,
                                              const FrameDev *__restrict__ fd, int W, int H, FrameCounters *fc,
                                              const MapCounters *__restrict__ mapc, SdfGate gate)
 {
+    rz_mode_on();  // first: every float operation below rounds toward zero
     // blocks loop over the queued bricks; the count comes from the device
     if (!sdf_gate_open(fc, mapc, fd->n, gate)) return;
     const uint16_t *__restrict__ depth = fd->depth;
@@ -1009,8 +1010,6 @@ __global__ __launch_bounds__(512) void k_sdf(const uint32_t *__restrict__ queue,
     for (uint32_t b = blockIdx.x; b < nQ; b += gridDim.x)
         sdf_voxel(queue[b], threadIdx.x, m, depth, bgr, W, H, P, tab);
 }
-
-
 
 // ---------------------------------------------------------------------------
 // batched SDF update: the queued bricks of n streams as one worklist
@@ -1042,6 +1041,7 @@ template <int TPB>
 __global__ __launch_bounds__(TPB) void k_sdf_batch(const SdfStream *__restrict__ ss, int n,
                                                    const uint32_t *__restrict__ prefix, int bpw)
 {
+    rz_mode_on();
     const uint32_t total = prefix[n];
     if ((uint32_t)blockIdx.x * (uint32_t)bpw >= total) return;
     __shared__ uint32_t tab[2048];
@@ -1072,6 +1072,7 @@ __global__ __launch_bounds__(TPB) void k_sdf_batch(const SdfStream *__restrict__
 __global__ __launch_bounds__(256) void k_sdf_batch_rows(const SdfStream *__restrict__ ss, int n,
                                                         const uint32_t *__restrict__ prefix, int bpw)
 {
+    rz_mode_on();
     const uint32_t total = prefix[n];
     const uint32_t wavesPerBlock = blockDim.x >> 6;
     if ((uint32_t)blockIdx.x * wavesPerBlock * (uint32_t)bpw >= total) return;
