@@ -81,8 +81,9 @@ __device__ __forceinline__ bool pool_alloc(const Pools &P, uint32_t k, uint32_t 
     return true;
 }
 
-__constant__ uint8_t c_tri[256][16];  // edges of each triangle, 255-terminated
-__constant__ uint8_t c_ntri[256];
+// per cube index: its triangles' edges, 4 bits each in order (bits 0-59),
+// and the triangle count (bits 60-63); one load per cube
+__constant__ uint64_t c_tri64[256];
 
 #define FF_TILE_CAP_MAX (1u << 22)  // 4 GiB of samples
 
@@ -923,22 +924,42 @@ __constant__ int8_t c_edge[12][4] = {{0, 0, 0, 0}, {1, 0, 0, 1}, {0, 1, 0, 0}, {
                                      {0, 1, 1, 0}, {0, 0, 1, 1}, {0, 0, 0, 2}, {1, 0, 0, 2}, {1, 1, 0, 2}, {0, 1, 0, 2}};
 __constant__ int8_t c_edgec[12][2] = {{0, 1}, {1, 2}, {2, 3}, {3, 0}, {4, 5}, {5, 6}, {6, 7}, {7, 4}, {0, 4}, {1, 5}, {2, 6}, {3, 7}};
 
+// edge e of a cube as an offset of used-edge slots from the cube's first
+// sample: 3 (dz * 64 + dy * 8 + dx) + axis (c_edge), 8 bits per edge, in
+// registers rather than a table
+__device__ __forceinline__ int edge_slot_off(int e)
+{
+    constexpr uint32_t E0 = 0u | (4u << 8) | (24u << 16) | (1u << 24);       // e0..e3
+    constexpr uint32_t E1 = 192u | (196u << 8) | (216u << 16) | (193u << 24);  // e4..e7
+    constexpr uint32_t E2 = 2u | (5u << 8) | (29u << 16) | (26u << 24);      // e8..e11
+    const uint32_t w = e < 4 ? E0 : (e < 8 ? E1 : E2);
+    return (int)((w >> (8 * (e & 3))) & 255u);
+}
+
+__device__ __forceinline__ int tri_edge(uint64_t T, int t) { return (int)((T >> (4 * t)) & 15u); }
+__device__ __forceinline__ int tri_count(uint64_t T) { return (int)(T >> 60); }
+
+// q / d for q < 512 and 1 <= d < 128 by a multiply with m = ceil(2^16 / d)
+// (the error q (m d - 2^16) / (d 2^16) < 1/128 stays below the gap 1/d)
+__device__ __forceinline__ uint32_t div_small(uint32_t q, uint32_t m) { return (q * m) >> 16; }
+
 struct WaveLds {
     float d[FF_BRICK];
     uint16_t c[3][FF_BRICK];
     uint8_t sf[FF_BRICK];  // bit 0: distance < 0, bit 1: zero weight
     uint32_t used[48];     // 1536 edge slots: (sample * 3 + axis)
     uint32_t wpref[48];    // vertex index of the first slot of each word
-    uint8_t cube[344];
+    uint16_t cube[344];    // cube index | triangle count << 8
     uint16_t foff[344];    // face offset per cube
     uint16_t list[1344];   // used slots in vertex order (at most 3 * 7 * 8 * 8 edges)
 };
 
-__device__ __forceinline__ void put_sample(WaveLds &L, const CellGeom &g, int s, float d, float w, uint32_t c01,
+// sample s of a cell with n0 x n1 samples per layer, n0 = 1 << lg0 and
+// n1 = 1 << lg1 (cell sample counts are powers of two)
+__device__ __forceinline__ void put_sample(WaveLds &L, int lg0, int lg1, int s, float d, float w, uint32_t c01,
                                            uint32_t c2)
 {
-    const int n0 = g.n[0], n1 = g.n[1];
-    const int i = s % n0, j = (s / n0) % n1, l = s / (n0 * n1);
+    const int i = s & ((1 << lg0) - 1), j = (s >> lg0) & ((1 << lg1) - 1), l = s >> (lg0 + lg1);
     const int id = (l * FF_BL + j) * FF_BL + i;
     L.d[id] = d;
     L.c[0][id] = (uint16_t)(c01 & 0xffffu);
@@ -963,17 +984,18 @@ __device__ void mc_cell(WaveLds &L, int lane, CellRec *cells, const CellGeom &g,
     const int type = g.type;
     const int n[3] = {g.n[0], g.n[1], g.n[2]};
     const int total = n[0] * n[1] * n[2];
+    const int lg0 = __ffs(max(n[0], 1)) - 1, lg1 = __ffs(max(n[1], 1)) - 1;
     if (!InPlace) {
         const uint4 *src = sbuf + (size_t)g.tile0 * 64u;
         for (int s = lane; s < total; s += 64) {
             const uint4 v = src[s];
-            put_sample(L, g, s, __uint_as_float(v.x), __uint_as_float(v.y), v.z, v.w);
+            put_sample(L, lg0, lg1, s, __uint_as_float(v.x), __uint_as_float(v.y), v.z, v.w);
         }
     } else {
         const uint32_t *ll = cells[g.cell].ll;
         for (int s = lane; s < total; s += 64) {
             const uint4 v = pack_sample(sample_cell(m, g, ll, s));
-            put_sample(L, g, s, __uint_as_float(v.x), __uint_as_float(v.y), v.z, v.w);
+            put_sample(L, lg0, lg1, s, __uint_as_float(v.x), __uint_as_float(v.y), v.z, v.w);
         }
     }
     if (lane < 48) L.used[lane] = 0u;
@@ -1001,13 +1023,14 @@ __device__ void mc_cell(WaveLds &L, int lane, CellRec *cells, const CellGeom &g,
         const float sx = m.scale * (float)stretch;
         for (int a = 0; a < 3; a++) basev[a] = m.off[a] + (float)geom_coord(g, a, 0) * m.scale;
         if (veto || !total) ci = 0;
-        const int ntri = c_ntri[ci];
+        const uint64_t T = c_tri64[ci];
+        const int ntri = tri_count(T);
         bool keep = false;
         float V[3][3];
         uint32_t C[3];
         if (lane < ntri) {
             for (int q = 0; q < 3; q++) {
-                const int e = c_tri[ci][3 * lane + q];
+                const int e = tri_edge(T, 3 * lane + q);
                 const int a = c_edgec[e][0], b = c_edgec[e][1];
                 const int ia = (c_corner[a][2] * FF_BL + c_corner[a][1]) * FF_BL + c_corner[a][0];
                 const int ib = (c_corner[b][2] * FF_BL + c_corner[b][1]) * FF_BL + c_corner[b][0];
@@ -1052,8 +1075,12 @@ __device__ void mc_cell(WaveLds &L, int lane, CellRec *cells, const CellGeom &g,
     // -- cubes: index, used edges, triangle counts
     const int nc0 = max(n[0] - 1, 0), nc1 = max(n[1] - 1, 0), nc2 = max(n[2] - 1, 0);
     const int ncubes = nc0 * nc1 * nc2;
+    // q -> (i, j, l) by multiplies (nc0, nc1 are 1, 3 or 7; q < 343)
+    const uint32_t m0 = nc0 ? (65536u + (uint32_t)nc0 - 1u) / (uint32_t)nc0 : 0u;
+    const uint32_t m01 = nc0 * nc1 ? (65536u + (uint32_t)(nc0 * nc1) - 1u) / (uint32_t)(nc0 * nc1) : 0u;
     for (int q = lane; q < ncubes; q += 64) {
-        const int i = q % nc0, j = (q / nc0) % nc1, l = q / (nc0 * nc1);
+        const int jl = (int)div_small((uint32_t)q, m0), l = (int)div_small((uint32_t)q, m01);
+        const int i = q - jl * nc0, j = jl - l * nc1;
         const int b0 = (l * FF_BL + j) * FF_BL + i;
         int ci = 0;
         uint32_t zw = 0;
@@ -1064,11 +1091,11 @@ __device__ void mc_cell(WaveLds &L, int lane, CellRec *cells, const CellGeom &g,
             ci |= (int)(f & 1u) << k;
         }
         if (zw & 2u) ci = 0;
-        L.cube[q] = (uint8_t)ci;
-        const int nt = c_ntri[ci];
+        const uint64_t T = ci ? c_tri64[ci] : 0ull;
+        const int nt = tri_count(T);
+        L.cube[q] = (uint16_t)(ci | (nt << 8));
         for (int t = 0; t < 3 * nt; t++) {
-            const int e = c_tri[ci][t];
-            const int slot = (b0 + (c_edge[e][2] * FF_BL + c_edge[e][1]) * FF_BL + c_edge[e][0]) * 3 + c_edge[e][3];
+            const int slot = b0 * 3 + edge_slot_off(tri_edge(T, t));
             atomicOr(&L.used[slot >> 5], 1u << (slot & 31));
         }
     }
@@ -1093,12 +1120,12 @@ __device__ void mc_cell(WaveLds &L, int lane, CellRec *cells, const CellGeom &g,
     {
         const int per = 6;
         uint32_t loc = 0;
-        for (int q = lane * per; q < min(ncubes, lane * per + per); q++) loc += c_ntri[L.cube[q]];
+        for (int q = lane * per; q < min(ncubes, lane * per + per); q++) loc += L.cube[q] >> 8;
         uint32_t tot;
         uint32_t ex = wave_excl_scan_u32(loc, &tot);
         for (int q = lane * per; q < min(ncubes, lane * per + per); q++) {
             L.foff[q] = (uint16_t)ex;
-            ex += c_ntri[L.cube[q]];
+            ex += L.cube[q] >> 8;
         }
         nf = tot;
     }
@@ -1130,15 +1157,16 @@ __device__ void mc_cell(WaveLds &L, int lane, CellRec *cells, const CellGeom &g,
         }
         // faces
         for (int q = lane; q < ncubes; q += 64) {
-            const int ci = L.cube[q];
-            const int nt = c_ntri[ci];
+            const int cq = L.cube[q];
+            const int nt = cq >> 8;
             if (!nt) continue;
-            const int i = q % nc0, j = (q / nc0) % nc1, l = q / (nc0 * nc1);
+            const uint64_t T = c_tri64[cq & 255];
+            const int jl = (int)div_small((uint32_t)q, m0), l = (int)div_small((uint32_t)q, m01);
+            const int i = q - jl * nc0, j = jl - l * nc1;
             const int b0 = (l * FF_BL + j) * FF_BL + i;
             const uint32_t fb = f0 + L.foff[q];
             for (int t = 0; t < 3 * nt; t++) {
-                const int e = c_tri[ci][t];
-                const int slot = (b0 + (c_edge[e][2] * FF_BL + c_edge[e][1]) * FF_BL + c_edge[e][0]) * 3 + c_edge[e][3];
+                const int slot = b0 * 3 + edge_slot_off(tri_edge(T, t));
                 const uint32_t wd = L.used[slot >> 5];
                 P.f[3 * fb + t] = L.wpref[slot >> 5] + __popc(wd & ((1u << (slot & 31)) - 1u));
             }
@@ -1304,15 +1332,15 @@ int mesh_init(ff_ctx *c)
     MeshState *s = new MeshState();
     c->mesh = s;
     // triangle tables into constant memory
-    uint8_t tri[256][16], ntri[256];
+    uint64_t tri[256];
     for (int i = 0; i < 256; i++) {
         const char *r = FF_MC_TRI[i];
-        const int len = (int)strlen(r);
-        ntri[i] = (uint8_t)(len / 3);
-        for (int k = 0; k < 16; k++) tri[i][k] = k < len ? (uint8_t)(r[k] <= '9' ? r[k] - '0' : r[k] - 'a' + 10) : 255;
+        const int len = (int)strlen(r);  // at most 15 edges (5 triangles)
+        uint64_t T = (uint64_t)(len / 3) << 60;
+        for (int k = 0; k < len; k++) T |= (uint64_t)(r[k] <= '9' ? r[k] - '0' : r[k] - 'a' + 10) << (4 * k);
+        tri[i] = T;
     }
-    MCHK(hipMemcpyToSymbol(HIP_SYMBOL(c_tri), tri, sizeof tri));
-    MCHK(hipMemcpyToSymbol(HIP_SYMBOL(c_ntri), ntri, sizeof ntri));
+    MCHK(hipMemcpyToSymbol(HIP_SYMBOL(c_tri64), tri, sizeof tri));
     s->binfoCap = c->nBranchesTotal / 16 + 1;
     MCHK(hipMalloc(&s->d_binfo, sizeof(ushort4) * s->binfoCap));
     MCHK(hipMemset(s->d_binfo, 0, sizeof(ushort4) * s->binfoCap));
