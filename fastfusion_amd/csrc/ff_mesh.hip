@@ -212,6 +212,32 @@ __device__ __forceinline__ uint32_t wave_append(uint32_t *counter, bool want)
     return want ? base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull)) : 0xffffffffu;
 }
 
+// The same for a whole block (every thread calls it; it holds two barriers):
+// one atomic per block, since appends of many waves to one counter
+// serialise on its address.
+template <int NW>
+__device__ __forceinline__ uint32_t block_append(uint32_t *counter, bool want)
+{
+    __shared__ uint32_t wc[NW + 1];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const uint64_t m = __ballot(want);
+    if (lane == 0) wc[w] = (uint32_t)__popcll(m);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t t = 0;
+        for (int k = 0; k < NW; k++) {
+            const uint32_t c = wc[k];
+            wc[k] = t;
+            t += c;
+        }
+        wc[NW] = t ? atomicAdd(counter, t) : 0u;
+    }
+    __syncthreads();
+    const uint32_t r = wc[NW] + wc[w] + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+    __syncthreads();  // wc is reused by the next call
+    return want ? r : 0xffffffffu;
+}
+
 // ---------------------------------------------------------------------------
 // k_terminal: covered terminal nodes
 // ---------------------------------------------------------------------------
@@ -265,19 +291,20 @@ __global__ void k_terminal_b(const MeshBatchStream *__restrict__ S)
 // ---------------------------------------------------------------------------
 // Every ownership test looks at terminal nodes containing points next to the
 // node p (size s): per axis the coordinate is one of p - 1, p, p + s - 1,
-// p + s (option 0..3).  A block takes 4 nodes at a time: one thread locates
-// each of the 64 option points of a node (the 8 inside ones are the node
-// itself), then one thread per (node, candidate) decides ownership from them.
+// p + s (option 0..3).  A block of 512 threads takes 8 nodes at a time: one
+// thread locates each of the 64 option points of a node (the 8 inside ones
+// are the node itself), then one thread per (node, candidate) decides
+// ownership from them; the block appends its cells with one atomic.
 __device__ __forceinline__ int opt_code(int ox, int oy, int oz) { return ox + 4 * oy + 16 * oz; }
 
 __device__ __forceinline__ void cells_body(const uint32_t *__restrict__ tree, uint32_t nLeavesTotal, uint32_t n,
                                            const uint4 *__restrict__ term, uint32_t termCap, MeshCounters *mc,
                                            CellRec *cells, uint32_t cap)
 {
-    __shared__ uint2 nb[4][64];  // (size, lastLeaf) per node and option point
+    __shared__ uint2 nb[8][64];  // (size, lastLeaf) per node and option point
     const uint32_t nt = min(mc->nTerm, termCap);
     const int tid = threadIdx.x;
-    for (uint32_t n0 = blockIdx.x * 4u; n0 < nt; n0 += gridDim.x * 4u) {
+    for (uint32_t n0 = blockIdx.x * 8u; n0 < nt; n0 += gridDim.x * 8u) {
         {
             const uint32_t node = n0 + (uint32_t)(tid >> 6);
             const int code = tid & 63;
@@ -303,14 +330,14 @@ __device__ __forceinline__ void cells_body(const uint32_t *__restrict__ tree, ui
             nb[tid >> 6][code] = r;
         }
         __syncthreads();
-        if (tid < 128) {
+        {
             // candidates: 0 interior, 1-6 walls, 7-18 edges, 19-26 corners
             const uint32_t node = n0 + (uint32_t)(tid / 27);
             const int cand = tid % 27;
             bool emit = false;
             int type = 0, o[3] = {0, 0, 0}, s = 1, nll = 0;
             uint32_t ll[8];
-            if (tid < 108 && node < nt) {
+            if (tid < 8 * 27 && node < nt) {
                 const uint2 *N = nb[tid / 27];
                 const uint4 tn = term[node];
                 const int p[3] = {(int)(tn.x & 0xffffu), (int)(tn.x >> 16), (int)tn.y};
@@ -373,7 +400,9 @@ __device__ __forceinline__ void cells_body(const uint32_t *__restrict__ tree, ui
                 for (int k = 0; k < nll && emit; k++)
                     if (ll[k] >= FF_BRANCHINIT) emit = false;
             }
-            const uint32_t i = wave_append(&mc->nCells, emit);
+            // (its barriers also order this iteration's nb reads before the
+            // next iteration's writes)
+            const uint32_t i = block_append<8>(&mc->nCells, emit);
             if (emit) {
                 if (i >= cap) {
                     atomicOr(&mc->flags, FF_CELL_OVERFLOW);
@@ -388,18 +417,17 @@ __device__ __forceinline__ void cells_body(const uint32_t *__restrict__ tree, ui
                 }
             }
         }
-        __syncthreads();
     }
 }
 
-__global__ __launch_bounds__(256) void k_cells(const uint32_t *__restrict__ tree, uint32_t nLeavesTotal, uint32_t n,
+__global__ __launch_bounds__(512) void k_cells(const uint32_t *__restrict__ tree, uint32_t nLeavesTotal, uint32_t n,
                                                const uint4 *__restrict__ term, uint32_t termCap, MeshCounters *mc,
                                                CellRec *cells, uint32_t cap)
 {
     cells_body(tree, nLeavesTotal, n, term, termCap, mc, cells, cap);
 }
 
-__global__ __launch_bounds__(256) void k_cells_b(const MeshBatchStream *__restrict__ S)
+__global__ __launch_bounds__(512) void k_cells_b(const MeshBatchStream *__restrict__ S)
 {
     const MeshBatchStream &a = S[blockIdx.z];
     cells_body(a.tree, a.nLeavesTotal, a.n, a.term, a.termCap, a.mc, a.cells, a.cellCap);
@@ -418,12 +446,32 @@ __device__ __forceinline__ uint32_t hkey_hash(uint64_t k, uint32_t mask)
     return (uint32_t)k & mask;
 }
 
-__device__ __forceinline__ bool chain_dirty(const uint32_t *__restrict__ stamp, const uint32_t *__restrict__ parent,
-                                            uint32_t leaf, uint32_t since)
+// a brick on the parent chain (at most 16 deep) of any of the bricks ll[]
+// updated after `since`; the chains advance together, so each level's loads
+// are in flight at once
+__device__ __forceinline__ bool chains_dirty(const uint32_t *__restrict__ stamp, const uint32_t *__restrict__ parent,
+                                             const uint32_t ll[8], uint32_t since)
 {
-    for (int d = 0; d < 16 && leaf < FF_BRANCHINIT; d++) {
-        if (stamp[leaf] > since) return true;
-        leaf = parent[leaf];
+    uint32_t cur[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++) cur[k] = ll[k];
+    for (int d = 0; d < 16; d++) {
+        uint32_t st[8], pa[8];
+#pragma unroll
+        for (int k = 0; k < 8; k++)
+            if (cur[k] < FF_BRANCHINIT) {
+                st[k] = stamp[cur[k]];
+                pa[k] = parent[cur[k]];
+            }
+        bool any = false;
+#pragma unroll
+        for (int k = 0; k < 8; k++)
+            if (cur[k] < FF_BRANCHINIT) {
+                if (st[k] > since) return true;
+                cur[k] = pa[k];
+                any = true;
+            }
+        if (!any) return false;
     }
     return false;
 }
@@ -747,9 +795,12 @@ __device__ __forceinline__ void mark_body(CellRec *cells, const CellRec *__restr
             bool clean = found >= 0;
             if (clean) {
                 const CellRec &p = prev[found];
-                for (int k = 0; k < 8; k++) clean = clean && (p.ll[k] == c.ll[k]);
-                for (int k = 0; k < 8 && clean; k++)
-                    if (c.ll[k] < FF_BRANCHINIT && chain_dirty(stamp, parent, c.ll[k], since)) clean = false;
+                uint32_t cl[8];
+                for (int k = 0; k < 8; k++) {
+                    cl[k] = c.ll[k];
+                    clean = clean && (p.ll[k] == cl[k]);
+                }
+                if (clean && chains_dirty(stamp, parent, cl, since)) clean = false;
                 if (clean) {
                     c.v0 = p.v0; c.nv = p.nv; c.f0 = p.f0; c.nf = p.nf;
                 }
@@ -1572,7 +1623,7 @@ bool mesh_launch_pre(ff_ctx *c, bool timed)
     const uint32_t nBlocks = c->nBranchesUsed / 16;
     k_terminal<<<(nBlocks * 8 + 255) / 256, 256, 0, c->stream>>>(c->d_tree, s->d_binfo, nBlocks, c->nLeavesTotal, c->n,
                                                                  s->d_term, s->termCap, s->d_mc);
-    k_cells<<<2048, 256, 0, c->stream>>>(c->d_tree, c->nLeavesTotal, c->n, s->d_term, s->termCap, s->d_mc,
+    k_cells<<<1024, 512, 0, c->stream>>>(c->d_tree, c->nLeavesTotal, c->n, s->d_term, s->termCap, s->d_mc,
                                          s->d_cells[nxt], s->cellCap);
     const MapRO m = s->m;
     k_mark<<<1024, 256, 0, c->stream>>>(s->d_cells[nxt], s->d_cells[s->cur], s->d_hkeys, s->d_hval, s->hmask,
@@ -1975,7 +2026,7 @@ int ff_update_meshes_batch(ff_ctx *const *ctxs, int n)
         const MeshBatchStream *S = ls->d_mbs;
         k_mesh_begin_b<<<dim3(1, 1, na), 64, 0, st>>>(S);
         k_terminal_b<<<dim3(maxT, 1, na), 256, 0, st>>>(S);
-        k_cells_b<<<dim3(batch_grid(2048, na, 64), 1, na), 256, 0, st>>>(S);
+        k_cells_b<<<dim3(batch_grid(1024, na, 32), 1, na), 512, 0, st>>>(S);
         k_mark_b<<<dim3(batch_grid(1024, na, 32), 1, na), 256, 0, st>>>(S);
         k_sample_b<<<dim3(batch_grid(4096, na, 64), 1, na), 256, 0, st>>>(S);
         k_mc_prefix<<<1, 64, 0, st>>>(ls->d_ms, na, ls->d_msPrefix);
