@@ -130,92 +130,69 @@ __global__ void k_bbox_first(const uint16_t *__restrict__ depth, int W, int H, f
 }
 
 // v_k = (int)((((qx_k[x] + qy_k[y]) * D) * scaling) + t_k), RN, no contraction
-// (traversal_wrapper_functions.cpp:613-630)
-__device__ __forceinline__ void voxel_of(const float *__restrict__ qx, const float *__restrict__ qy, int W, int H,
-                                         int x, int y, float D, float scaling, const float t[3], int v[3])
+// (traversal_wrapper_functions.cpp:613-630), with the reference's per-frame
+// ray tables qx_k = R_k0 * pxp[x], qy_k = R_k1 * pyp[y] + R_k2
+// (geometryfusion_mipmap_cpu.cpp:1765-1790) formed in place: the same float
+// operations, so the same values
+// (px = pxp[x], py = pyp[y])
+__device__ __forceinline__ void voxel_of(const CamF &p, float px, float py, float D, float scaling, int v[3])
 {
 #pragma unroll
-    for (int k = 0; k < 3; k++)
-        v[k] = cvt_x86(__fadd_rn(__fmul_rn(__fmul_rn(__fadd_rn(qx[k * W + x], qy[k * H + y]), D), scaling), t[k]));
-}
-
-// ---------------------------------------------------------------------------
-// k_frame_begin: the frame's ray tables qx = R * pxp (per column) and
-// qy = R * pyp + R_z (per row), as the reference forms them in float
-// (geometryfusion_mipmap_cpu.cpp:1765-1790), and the frame counters reset
-// ---------------------------------------------------------------------------
-__device__ __forceinline__ void frame_begin_body(const float *__restrict__ pxp, const float *__restrict__ pyp, int W,
-                                                 int H, const FrameDev *__restrict__ fd, float *__restrict__ qx,
-                                                 float *__restrict__ qy, FrameCounters *fc)
-{
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    const CamF &p = fd->p;
-    const uint32_t n = fd->n;
-    if (i < W) {
-        const float v = pxp[i];
-        qx[i] = __fmul_rn(p.r[0], v);
-        qx[W + i] = __fmul_rn(p.r[3], v);
-        qx[2 * W + i] = __fmul_rn(p.r[6], v);
-    }
-    if (i < H) {
-        const float v = pyp[i];
-        qy[i] = __fadd_rn(__fmul_rn(p.r[1], v), p.r[2]);
-        qy[H + i] = __fadd_rn(__fmul_rn(p.r[4], v), p.r[5]);
-        qy[2 * H + i] = __fadd_rn(__fmul_rn(p.r[7], v), p.r[8]);
-    }
-    if (i == 0) {
-        FrameCounters z;
-        z.nQueued = z.listCount = z.newB = z.newL = 0u;
-        for (int k = 0; k < 3; k++) { z.bmin[k] = 0; z.bmax[k] = (int)n - 1; }
-        z.flags = z.nProcessed = z.sdfRan = 0u;
-        *fc = z;
+    for (int k = 0; k < 3; k++) {
+        const float qx = __fmul_rn(p.r[3 * k], px);                            // R * pxp
+        const float qy = __fadd_rn(__fmul_rn(p.r[3 * k + 1], py), p.r[3 * k + 2]);  // R * pyp + R_z
+        v[k] = cvt_x86(__fadd_rn(__fmul_rn(__fmul_rn(__fadd_rn(qx, qy), D), scaling), p.t[k]));
     }
 }
 
-__global__ void k_frame_begin(const float *__restrict__ pxp, const float *__restrict__ pyp, int W, int H,
-                              const FrameDev *__restrict__ fd, float *__restrict__ qx, float *__restrict__ qy,
-                              FrameCounters *fc)
+// the frame counters' reset (bounds start as the whole volume)
+__device__ __forceinline__ void frame_counters_reset(FrameCounters *fc, uint32_t n)
 {
-    frame_begin_body(pxp, pyp, W, H, fd, qx, qy, fc);
+    FrameCounters z;
+    z.nQueued = z.listCount = z.newB = z.newL = 0u;
+    for (int k = 0; k < 3; k++) { z.bmin[k] = 0; z.bmax[k] = (int)n - 1; }
+    z.flags = z.nProcessed = z.sdfRan = 0u;
+    *fc = z;
 }
 
-// the batched forms of the allocation kernels: stream blockIdx.z of S, grid
-// x (and y) sized for the largest stream; blocks outside a stream's range
-// return at once
-__global__ void k_frame_begin_b(const AllocStream *__restrict__ S)
+__global__ void k_frame_begin(const FrameDev *__restrict__ fd, FrameCounters *fc)
 {
-    const AllocStream &a = S[blockIdx.z];
-    frame_begin_body(a.pxp, a.pyp, a.W, a.H, a.fd, a.qx, a.qy, a.fc);
+    if (threadIdx.x == 0) frame_counters_reset(fc, fd->n);
 }
 
 // ---------------------------------------------------------------------------
 // k_planes: per 64-pixel row chunk, ballot the three relations into bit
 // planes: plane[(y*words + w)*3 + {0: valid, 1: key == left key, 2: key == up key}]
 // ---------------------------------------------------------------------------
+// (block (0, 0) also resets the frame counters: no kernel before the
+// descent reads them)
 __device__ __forceinline__ void planes_body(const FrameDev *__restrict__ fd, int W, int H, int words,
-                                            const float *__restrict__ qx, const float *__restrict__ qy,
+                                            const float *__restrict__ pxp, const float *__restrict__ pyp,
                                             uint32_t *__restrict__ planes, uint32_t *__restrict__ cntB,
-                                            uint32_t *__restrict__ cntL, uint32_t *__restrict__ firstD)
+                                            uint32_t *__restrict__ cntL, uint32_t *__restrict__ firstD,
+                                            FrameCounters *fc)
 {
     const int lane = threadIdx.x, y = blockIdx.y, x = blockIdx.x * 64 + lane;
     const uint16_t *__restrict__ depth = fd->depth;
     const float scaling = fd->scaling, maxcam = fd->maxcam;
-    const float t[3] = {fd->p.t[0], fd->p.t[1], fd->p.t[2]};
+    const CamF &p = fd->p;
+    if (blockIdx.x == 0 && y == 0 && lane == 0) frame_counters_reset(fc, fd->n);
     bool valid = false, el = false, eu = false;
     if (x < W) {
         const float D = (float)depth[y * W + x];
         const float z = __fmul_rn(D, scaling);
         valid = z > 0.0f && z < maxcam;
+        const float px = pxp[x], py = pyp[y];
         int v[3];
-        voxel_of(qx, qy, W, H, x, y, D, scaling, t, v);
+        voxel_of(p, px, py, D, scaling, v);
         if (x > 0) {
             int u[3];
-            voxel_of(qx, qy, W, H, x - 1, y, (float)depth[y * W + x - 1], scaling, t, u);
+            voxel_of(p, pxp[x - 1], py, (float)depth[y * W + x - 1], scaling, u);
             el = ((u[0] & -8) == (v[0] & -8)) && ((u[1] & -8) == (v[1] & -8)) && ((u[2] & -8) == (v[2] & -8));
         }
         if (y > 0) {
             int u[3];
-            voxel_of(qx, qy, W, H, x, y - 1, (float)depth[(y - 1) * W + x], scaling, t, u);
+            voxel_of(p, px, pyp[y - 1], (float)depth[(y - 1) * W + x], scaling, u);
             eu = ((u[0] & -8) == (v[0] & -8)) && ((u[1] & -8) == (v[1] & -8)) && ((u[2] & -8) == (v[2] & -8));
         }
         const size_t i = (size_t)y * W + x;
@@ -237,18 +214,22 @@ __device__ __forceinline__ void planes_body(const FrameDev *__restrict__ fd, int
 }
 
 __global__ __launch_bounds__(64) void k_planes(const FrameDev *__restrict__ fd, int W, int H, int words,
-                                               const float *__restrict__ qx, const float *__restrict__ qy,
+                                               const float *__restrict__ pxp, const float *__restrict__ pyp,
                                                uint32_t *__restrict__ planes, uint32_t *__restrict__ cntB,
-                                               uint32_t *__restrict__ cntL, uint32_t *__restrict__ firstD)
+                                               uint32_t *__restrict__ cntL, uint32_t *__restrict__ firstD,
+                                               FrameCounters *fc)
 {
-    planes_body(fd, W, H, words, qx, qy, planes, cntB, cntL, firstD);
+    planes_body(fd, W, H, words, pxp, pyp, planes, cntB, cntL, firstD, fc);
 }
 
+// the batched forms of the allocation kernels: stream blockIdx.z of S, grid
+// x (and y) sized for the largest stream; blocks outside a stream's range
+// return at once
 __global__ __launch_bounds__(64) void k_planes_b(const AllocStream *__restrict__ S)
 {
     const AllocStream &a = S[blockIdx.z];
     if ((int)blockIdx.y >= a.H || (int)blockIdx.x * 64 >= a.W) return;
-    planes_body(a.fd, a.W, a.H, a.words, a.qx, a.qy, a.planes, a.cntB, a.cntL, a.firstD);
+    planes_body(a.fd, a.W, a.H, a.words, a.pxp, a.pyp, a.planes, a.cntB, a.cntL, a.firstD, a.fc);
 }
 
 // ---------------------------------------------------------------------------
@@ -371,17 +352,8 @@ __global__ __launch_bounds__(FF_DD_THREADS) void k_dedup_b(const AllocStream *__
 // k_descend: queryPointDepthSingle_func_subtree (traversal_functions.cpp:308-395)
 // read-only against the tree as it was before this frame.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ void queue_leaf(const MapView &m, uint32_t leaf, uint32_t stamp, FrameCounters *fc)
-{
-    if (m.stamp[leaf] == stamp) return;
-    if (atomicExch(&m.stamp[leaf], stamp) != stamp) {
-        const uint32_t q = atomicAdd(&fc->nQueued, 1u);
-        m.queue[q] = leaf;
-    }
-}
-
 __device__ __forceinline__ void descend_body(const FrameDev *__restrict__ fd, int W, int H, int words,
-                                             const float *__restrict__ qx, const float *__restrict__ qy,
+                                             const float *__restrict__ pxp, const float *__restrict__ pyp,
                                              const uint32_t *__restrict__ pbits, const MapView &m, const HashView &hv,
                                              uint32_t *__restrict__ pl0, FrameCounters *fc)
 {
@@ -395,12 +367,11 @@ __device__ __forceinline__ void descend_body(const FrameDev *__restrict__ fd, in
     const uint64_t pb = __ballot(proc);
     if (pb == 0) return;  // wave-uniform
     const uint32_t pix = (uint32_t)(y * W + x);
-    const float t[3] = {fd->p.t[0], fd->p.t[1], fd->p.t[2]};
     int v[3] = {0, 0, 0};
     float D = 0.0f;
     if (proc) {
         D = (float)depth[pix];
-        voxel_of(qx, qy, W, H, x, y, D, scaling, t, v);
+        voxel_of(fd->p, pxp[x], pyp[y], D, scaling, v);
     }
     // bounding-box tracking (traversal_wrapper_functions.cpp:642-676), one
     // atomic per wave and axis
@@ -437,65 +408,104 @@ __device__ __forceinline__ void descend_body(const FrameDev *__restrict__ fd, in
         if (lane == l0) leader = true;
         todo &= ~same;
     }
-    if (!leader) return;
-    uint32_t ox = 0, oy = 0, oz = 0, half = n >> 1, branch = 0, depthLvl = 1;
-    uint32_t pl = FF_BRANCHINIT, pld = 0;
-    bool missing = false;
-    uint32_t parentIdx = 0;  // tree index of the branch holding the current octant
-    while ((half >> 1) >= target) {
+    // The walk is a chain of dependent tree loads; the bricks on the path
+    // are queued after it, all at once (stamp loads, then the stamp
+    // exchanges in flight together, then one append per wave), instead of
+    // adding three round trips per brick to the chain.  The side n is at
+    // most 2^15 (16-bit brick coordinates), so the walk has at most 14
+    // steps before the target level.
+    constexpr int LV = 16;
+    uint32_t lv[LV];  // bricks on the path (FF_BRANCHINIT: none at that level)
+#pragma unroll
+    for (int k = 0; k < LV; k++) lv[k] = FF_BRANCHINIT;
+    if (leader) {
+        uint32_t ox = 0, oy = 0, oz = 0, half = n >> 1, branch = 0, depthLvl = 1;
+        uint32_t pl = FF_BRANCHINIT, pld = 0;
+        bool missing = false, walking = true;
+#pragma unroll
+        for (int k = 0; k < LV - 1; k++) {
+            walking = walking && (half >> 1) >= target;
+            if (walking) {
+                const uint32_t co = (X >= ox + half) * 1u + (Y >= oy + half) * 2u + (Z >= oz + half) * 4u;
+                const uint32_t nx = ox + half * (co & 1u), ny = oy + half * ((co >> 1) & 1u),
+                               nz = oz + half * ((co >> 2) & 1u);
+                if (!missing) {
+                    const uint32_t ptr = branch + co + 8;
+                    const uint32_t child = m.tree[ptr];
+                    const uint32_t leaf = m.tree[ptr - 8];
+                    if (leaf < m.nLeavesTotal) {
+                        pl = leaf;
+                        pld = depthLvl;
+                        lv[k] = leaf;
+                    }
+                    if (child == FF_BRANCHINIT) {
+                        missing = true;
+                        hash_insert_min(hv, node_key(0, depthLvl, nx, ny, nz), pix, branch, fc);
+                    }
+                    branch = child;
+                } else {
+                    hash_insert_min(hv, node_key(0, depthLvl, nx, ny, nz), pix, FF_BRANCHINIT, fc);
+                }
+                ox = nx; oy = ny; oz = nz;
+                half >>= 1;
+                depthLvl++;
+            }
+        }
         const uint32_t co = (X >= ox + half) * 1u + (Y >= oy + half) * 2u + (Z >= oz + half) * 4u;
         const uint32_t nx = ox + half * (co & 1u), ny = oy + half * ((co >> 1) & 1u), nz = oz + half * ((co >> 2) & 1u);
         if (!missing) {
-            const uint32_t ptr = branch + co + 8;
-            const uint32_t child = m.tree[ptr];
-            const uint32_t leaf = m.tree[ptr - 8];
-            if (leaf < m.nLeavesTotal) {
-                pl = leaf;
-                pld = depthLvl;
-                queue_leaf(m, leaf, stamp, fc);
-            }
-            if (child == FF_BRANCHINIT) {
-                missing = true;
-                hash_insert_min(hv, node_key(0, depthLvl, nx, ny, nz), pix, branch, fc);
-            } else {
-                parentIdx = child;
-            }
-            branch = child;
+            const uint32_t leaf = m.tree[branch + co];
+            if (leaf >= FF_BRANCHINIT)
+                hash_insert_min(hv, node_key(1, depthLvl, nx, ny, nz), pix, branch, fc);
+            else if (leaf < m.nLeavesTotal)
+                lv[LV - 1] = leaf;
         } else {
-            hash_insert_min(hv, node_key(0, depthLvl, nx, ny, nz), pix, FF_BRANCHINIT, fc);
+            hash_insert_min(hv, node_key(1, depthLvl, nx, ny, nz), pix, FF_BRANCHINIT, fc);
         }
-        ox = nx; oy = ny; oz = nz;
-        half >>= 1;
-        depthLvl++;
+        pl0[2 * pix] = pl;
+        pl0[2 * pix + 1] = pld;
     }
-    const uint32_t co = (X >= ox + half) * 1u + (Y >= oy + half) * 2u + (Z >= oz + half) * 4u;
-    const uint32_t nx = ox + half * (co & 1u), ny = oy + half * ((co >> 1) & 1u), nz = oz + half * ((co >> 2) & 1u);
-    if (!missing) {
-        const uint32_t leaf = m.tree[branch + co];
-        if (leaf >= FF_BRANCHINIT)
-            hash_insert_min(hv, node_key(1, depthLvl, nx, ny, nz), pix, branch, fc);
-        else if (leaf < m.nLeavesTotal)
-            queue_leaf(m, leaf, stamp, fc);
-    } else {
-        hash_insert_min(hv, node_key(1, depthLvl, nx, ny, nz), pix, FF_BRANCHINIT, fc);
+    // a brick joins the queue once per frame: its stamp is exchanged for the
+    // frame's, and the lane that saw the old stamp appends it
+    uint32_t st[LV];
+#pragma unroll
+    for (int k = 0; k < LV; k++) st[k] = lv[k] < m.nLeavesTotal ? m.stamp[lv[k]] : stamp;
+#pragma unroll
+    for (int k = 0; k < LV; k++)
+        if (st[k] != stamp) st[k] = atomicExch(&m.stamp[lv[k]], stamp);
+    uint32_t fresh = 0;
+#pragma unroll
+    for (int k = 0; k < LV; k++)
+        if (st[k] != stamp) fresh |= 1u << k;
+    const uint32_t cnt = (uint32_t)__popc(fresh);
+    uint32_t incl = cnt;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t t = __shfl_up(incl, o);
+        if (lane >= o) incl += t;
     }
-    (void)parentIdx;
-    pl0[2 * pix] = pl;
-    pl0[2 * pix + 1] = pld;
+    const uint32_t total = __shfl(incl, 63);
+    uint32_t base = 0;
+    if (lane == 63 && total) base = atomicAdd(&fc->nQueued, total);
+    base = __shfl(base, 63);
+    uint32_t q = base + incl - cnt;
+#pragma unroll
+    for (int k = 0; k < LV; k++)
+        if (fresh & (1u << k)) m.queue[q++] = lv[k];
 }
 
-__global__ void k_descend(const FrameDev *__restrict__ fd, int W, int H, int words, const float *__restrict__ qx,
-                          const float *__restrict__ qy, const uint32_t *__restrict__ pbits, MapView m, HashView hv,
+__global__ void k_descend(const FrameDev *__restrict__ fd, int W, int H, int words, const float *__restrict__ pxp,
+                          const float *__restrict__ pyp, const uint32_t *__restrict__ pbits, MapView m, HashView hv,
                           uint32_t *__restrict__ pl0, FrameCounters *fc)
 {
-    descend_body(fd, W, H, words, qx, qy, pbits, m, hv, pl0, fc);
+    descend_body(fd, W, H, words, pxp, pyp, pbits, m, hv, pl0, fc);
 }
 
 __global__ void k_descend_b(const AllocStream *__restrict__ S)
 {
     const AllocStream &a = S[blockIdx.z];
     if ((int)blockIdx.y >= a.H || (int)(blockIdx.x * blockDim.x) >= a.W) return;
-    descend_body(a.fd, a.W, a.H, a.words, a.qx, a.qy, a.pbits, a.m, a.hv, a.pl0, a.fc);
+    descend_body(a.fd, a.W, a.H, a.words, a.pxp, a.pyp, a.pbits, a.m, a.hv, a.pl0, a.fc);
 }
 
 // per new node: its creating pixel counts it
@@ -734,43 +744,47 @@ __global__ void k_link_b(const AllocStream *__restrict__ S)
     link_body(a.hv, a.m, a.fc, a.pl0, a.fd, a.fc);
 }
 
-__device__ __forceinline__ void hash_clear_body(const HashView &hv, const FrameCounters *fc)
+// The end of the allocation phase, per new node: the branch's geometry for
+// the mesher (origin, half size) and the node-table slot's reset; then the
+// frame's new branches and bricks join the map's used counts (a frame whose
+// node table overflowed commits nothing; the host reports it).  k_link,
+// whose lookups read other nodes' slots, has completed.
+__device__ __forceinline__ void alloc_end_body(const HashView &hv, const FrameCounters *__restrict__ fc,
+                                               MapCounters *mapc, ushort4 *binfo, uint32_t binfoBlocks,
+                                               const FrameDev *__restrict__ fd)
 {
-    const uint32_t n = min(fc->listCount, hv.listCap);
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const uint32_t n = fd->n;
+    const uint32_t cnt = min(fc->listCount, hv.listCap);
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < cnt; i += gridDim.x * blockDim.x) {
         const uint32_t s = hv.list[i];
+        const uint64_t k = hv.keys[s];
+        if (!((k >> 53) & 1u)) {
+            const uint32_t d = (uint32_t)(k >> 48) & 31u;
+            const uint32_t idx = hv.assigned[s];
+            if (idx / 16 < binfoBlocks)
+                binfo[idx / 16] = make_ushort4((unsigned short)((k >> 32) & 0xffffu),
+                                               (unsigned short)((k >> 16) & 0xffffu), (unsigned short)(k & 0xffffu),
+                                               (unsigned short)((n >> d) >> 1));
+        }
         hv.keys[s] = FF_HEMPTY;
         hv.minpix[s] = 0xffffffffu;
     }
-}
-
-__global__ void k_hash_clear(HashView hv, const FrameCounters *fc) { hash_clear_body(hv, fc); }
-
-__global__ void k_hash_clear_b(const AllocStream *__restrict__ S)
-{
-    const AllocStream &a = S[blockIdx.z];
-    hash_clear_body(a.hv, a.fc);
-}
-
-// the frame's new branches and bricks join the map's used counts (a frame
-// whose node table overflowed commits nothing; the host reports it)
-__device__ __forceinline__ void commit_body(const FrameCounters *__restrict__ fc, MapCounters *mapc, uint32_t listCap)
-{
-    if (threadIdx.x == 0 && !(fc->flags & FF_FLAG_HASH_FULL) && fc->listCount <= listCap) {
+    if (blockIdx.x == 0 && threadIdx.x == 0 && !(fc->flags & FF_FLAG_HASH_FULL) && fc->listCount <= hv.listCap) {
         mapc->nBranchesUsed += 16u * fc->newB;
         mapc->nLeavesUsed += fc->newL;
     }
 }
 
-__global__ void k_commit(const FrameCounters *__restrict__ fc, MapCounters *mapc, uint32_t listCap)
+__global__ void k_alloc_end(HashView hv, const FrameCounters *__restrict__ fc, MapCounters *mapc, ushort4 *binfo,
+                            uint32_t binfoBlocks, const FrameDev *__restrict__ fd)
 {
-    commit_body(fc, mapc, listCap);
+    alloc_end_body(hv, fc, mapc, binfo, binfoBlocks, fd);
 }
 
-__global__ void k_commit_b(const AllocStream *__restrict__ S)
+__global__ void k_alloc_end_b(const AllocStream *__restrict__ S)
 {
     const AllocStream &a = S[blockIdx.z];
-    commit_body(a.fc, a.mapc, a.hv.listCap);
+    alloc_end_body(a.hv, a.fc, a.mapc, a.binfo, a.binfoCap, a.fd);
 }
 
 // ---------------------------------------------------------------------------
@@ -1184,8 +1198,6 @@ int ensure_image(ff_ctx *c, int W, int H)
     c->freeFrameBuffers();
     HIPCHK(hipMalloc(&c->d_depth, npix * 2));
     HIPCHK(hipMalloc(&c->d_bgr, npix * 3));
-    HIPCHK(hipMalloc(&c->d_qx, sizeof(float) * 3 * W));
-    HIPCHK(hipMalloc(&c->d_qy, sizeof(float) * 3 * H));
     HIPCHK(hipMalloc(&c->d_pxp, sizeof(float) * W));
     HIPCHK(hipMalloc(&c->d_pyp, sizeof(float) * H));
     // bit planes padded to whole bands of the de-dup ring
@@ -1315,9 +1327,9 @@ using namespace ff;
 
 void ff_ctx::freeFrameBuffers()
 {
-    hipFree(d_depth); hipFree(d_bgr); hipFree(d_qx); hipFree(d_qy); hipFree(d_pxp); hipFree(d_pyp); hipFree(d_planes); hipFree(d_pbits);
+    hipFree(d_depth); hipFree(d_bgr); hipFree(d_pxp); hipFree(d_pyp); hipFree(d_planes); hipFree(d_pbits);
     hipFree(d_cntB); hipFree(d_cntL); hipFree(d_firstD); hipFree(d_pl0); hipFree(d_pref); hipFree(d_bsum);
-    d_depth = nullptr; d_bgr = nullptr; d_qx = d_qy = d_pxp = d_pyp = nullptr; d_planes = d_pbits = nullptr;
+    d_depth = nullptr; d_bgr = nullptr; d_pxp = d_pyp = nullptr; d_planes = d_pbits = nullptr;
     d_cntB = d_cntL = d_firstD = d_pl0 = nullptr; d_pref = d_bsum = nullptr;
     capW = capH = 0;
 }
@@ -1432,10 +1444,8 @@ void launch_alloc(ff_ctx *c, int W, int H)
     const int words = (W + 31) / 32;
     const uint32_t npix = (uint32_t)W * H;
     const FrameDev *fd = c->d_fd;
-    k_frame_begin<<<(std::max(W, H) + 255) / 256, 256, 0, c->stream>>>(c->d_pxp, c->d_pyp, W, H, fd, c->d_qx,
-                                                                       c->d_qy, c->d_fc);
-    k_planes<<<dim3((W + 63) / 64, H), 64, 0, c->stream>>>(fd, W, H, words, c->d_qx, c->d_qy, c->d_planes, c->d_cntB,
-                                                          c->d_cntL, c->d_firstD);
+    k_planes<<<dim3((W + 63) / 64, H), 64, 0, c->stream>>>(fd, W, H, words, c->d_pxp, c->d_pyp, c->d_planes, c->d_cntB,
+                                                          c->d_cntL, c->d_firstD, c->d_fc);
     if (dedup_rows(words) == 32)
         k_dedup<32><<<1, FF_DD_THREADS, dedup_lds_bytes(words), c->stream>>>(c->d_planes, H, words, c->d_pbits,
                                                                              c->d_fc);
@@ -1443,7 +1453,7 @@ void launch_alloc(ff_ctx *c, int W, int H)
         k_dedup<16><<<1, FF_DD_THREADS, dedup_lds_bytes(words), c->stream>>>(c->d_planes, H, words, c->d_pbits,
                                                                              c->d_fc);
     const MapView mv = c->mapView();
-    k_descend<<<dim3((W + 127) / 128, H), 128, 0, c->stream>>>(fd, W, H, words, c->d_qx, c->d_qy, c->d_pbits, mv,
+    k_descend<<<dim3((W + 127) / 128, H), 128, 0, c->stream>>>(fd, W, H, words, c->d_pxp, c->d_pyp, c->d_pbits, mv,
                                                                c->hv, c->d_pl0, c->d_fc);
     k_resolve<<<256, 256, 0, c->stream>>>(c->hv, c->d_fc, c->d_cntB, c->d_cntL, c->d_firstD);
     const uint32_t nb = (npix + 4095) / 4096;
@@ -1451,9 +1461,10 @@ void launch_alloc(ff_ctx *c, int W, int H)
     k_scan_b<<<1, 1024, 0, c->stream>>>(c->d_bsum, nb, c->d_fc);
     k_assign<<<256, 256, 0, c->stream>>>(c->hv, c->d_fc, c->d_pref, c->d_bsum, c->d_firstD, c->d_mapc);
     k_link<<<256, 256, 0, c->stream>>>(c->hv, mv, c->d_fc, c->d_pl0, fd, c->d_fc);
-    mesh_record_branches(c);
-    k_hash_clear<<<256, 256, 0, c->stream>>>(c->hv, c->d_fc);
-    k_commit<<<1, 64, 0, c->stream>>>(c->d_fc, c->d_mapc, c->hv.listCap);
+    ushort4 *binfo;
+    uint32_t binfoBlocks;
+    mesh_binfo(c, &binfo, &binfoBlocks);
+    k_alloc_end<<<256, 256, 0, c->stream>>>(c->hv, c->d_fc, c->d_mapc, binfo, binfoBlocks, fd);
 }
 
 AllocStream alloc_stream(ff_ctx *c, int W, int H)
@@ -1461,7 +1472,6 @@ AllocStream alloc_stream(ff_ctx *c, int W, int H)
     AllocStream a;
     a.fd = c->d_fd;
     a.pxp = c->d_pxp; a.pyp = c->d_pyp;
-    a.qx = c->d_qx; a.qy = c->d_qy;
     a.fc = c->d_fc;
     a.planes = c->d_planes; a.cntB = c->d_cntB; a.cntL = c->d_cntL; a.firstD = c->d_firstD;
     a.pbits = c->d_pbits; a.pl0 = c->d_pl0;
@@ -1497,7 +1507,6 @@ void launch_alloc_batch(hipStream_t st, const AllocStream *d_as, const AllocStre
         }
     }
     const int G = 64;  // blocks per stream of the node-list kernels
-    k_frame_begin_b<<<dim3((std::max(maxW, maxH) + 255) / 256, 1, n), 256, 0, st>>>(d_as);
     k_planes_b<<<dim3((maxW + 63) / 64, maxH, n), 64, 0, st>>>(d_as);
     if (r32) k_dedup_b<32><<<dim3(1, 1, n), FF_DD_THREADS, lds32, st>>>(d_as);
     if (r16) k_dedup_b<16><<<dim3(1, 1, n), FF_DD_THREADS, lds16, st>>>(d_as);
@@ -1507,9 +1516,7 @@ void launch_alloc_batch(hipStream_t st, const AllocStream *d_as, const AllocStre
     k_scan_b_b<<<dim3(1, 1, n), 1024, 0, st>>>(d_as);
     k_assign_b<<<dim3(G, 1, n), 256, 0, st>>>(d_as);
     k_link_b<<<dim3(G, 1, n), 256, 0, st>>>(d_as);
-    launch_set_binfo_batch(st, d_as, n, G);
-    k_hash_clear_b<<<dim3(G, 1, n), 256, 0, st>>>(d_as);
-    k_commit_b<<<dim3(1, 1, n), 64, 0, st>>>(d_as);
+    k_alloc_end_b<<<dim3(G, 1, n), 256, 0, st>>>(d_as);
 }
 
 // the SDF update waits for the host only when the node table overflowed or
@@ -1650,8 +1657,7 @@ int frame_prepare(ff_ctx *c, const uint16_t *depth, const uint8_t *bgr, int W, i
 // the device work of a frame without a volume (an empty first frame)
 void launch_empty(ff_ctx *c, int W, int H)
 {
-    k_frame_begin<<<(std::max(W, H) + 255) / 256, 256, 0, c->stream>>>(c->d_pxp, c->d_pyp, W, H, c->d_fd, c->d_qx,
-                                                                       c->d_qy, c->d_fc);
+    k_frame_begin<<<1, 64, 0, c->stream>>>(c->d_fd, c->d_fc);
     hipMemsetAsync(c->d_pbits, 0, pbits_bytes(H), c->stream);
     hipMemcpyAsync(c->h_fc, c->d_fc, sizeof(FrameCounters), hipMemcpyDeviceToHost, c->stream);
 }

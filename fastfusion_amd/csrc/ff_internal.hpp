@@ -100,12 +100,11 @@ struct SdfStream {
 struct AllocStream {
     const FrameDev *fd;
     const float *pxp, *pyp;
-    float *qx, *qy;
     FrameCounters *fc;
     uint32_t *planes, *cntB, *cntL, *firstD, *pbits, *pl0;
     uint64_t *pref, *bsum;
     MapCounters *mapc;
-    ushort4 *binfo;  // mesh-side branch geometry (k_set_binfo)
+    ushort4 *binfo;  // mesh-side branch geometry (k_alloc_end)
     MapView m;
     HashView hv;
     int W, H, words;
@@ -130,7 +129,7 @@ struct ff_ctx {
     uint32_t nLeavesTotal = 0, nBranchesTotal = 0, nLeavesUsed = 0, nBranchesUsed = 16, nQueued = 0;
     uint32_t frames = 0, grows = 0, stampCounter = 0;
     int imgW = 0, imgH = 0, capW = 0, capH = 0;
-    std::vector<float> pxp, pyp, h_qx, h_qy;
+    std::vector<float> pxp, pyp;
     hipStream_t stream = nullptr;
     // map arrays
     uint32_t *d_tree = nullptr, *d_treeTmp = nullptr, *d_parent = nullptr, *d_stamp = nullptr, *d_queue = nullptr;
@@ -140,7 +139,6 @@ struct ff_ctx {
     // frame buffers
     uint16_t *d_depth = nullptr;
     uint8_t *d_bgr = nullptr;
-    float *d_qx = nullptr, *d_qy = nullptr;
     float *d_pxp = nullptr, *d_pyp = nullptr;  // per-column / per-row ray tables of the current image size
     uint32_t *d_planes = nullptr, *d_pbits = nullptr, *d_cntB = nullptr, *d_cntL = nullptr, *d_firstD = nullptr,
              *d_pl0 = nullptr;
@@ -208,9 +206,9 @@ void mesh_free(ff_ctx *c);
 void mesh_on_initial_volume(ff_ctx *c);
 void mesh_on_grow(ff_ctx *c);
 void mesh_after_frame(ff_ctx *c);
-void mesh_record_branches(ff_ctx *c);
+// the mesher's branch-geometry table (origin and half size per branch
+// block), filled by the allocation phase
 void mesh_binfo(const ff_ctx *c, ushort4 **binfo, uint32_t *cap);
-void launch_set_binfo_batch(hipStream_t st, const AllocStream *d_as, int n, int grid);
 // the batched allocation phase of n streams (descriptors in device memory
 // d_as, host copy h_as for the grid sizes)
 AllocStream alloc_stream(ff_ctx *c, int W, int H);

@@ -849,9 +849,21 @@ __global__ void k_mark_b(const MeshBatchStream *__restrict__ S)
               a.cellCap, a.m, a.tiles, a.tileCap);
 }
 
-__device__ __forceinline__ void hash_build_body(CellRec *cells, const MeshCounters *mc, uint32_t cap, uint64_t *hkeys,
-                                                uint32_t *hval, uint32_t hmask)
+// arena (when not null): block 0 also writes the totals of the arena
+// counters into mc->pad (vertices, faces allocated since the last compaction)
+__device__ __forceinline__ void hash_build_body(CellRec *cells, MeshCounters *mc, uint32_t cap, uint64_t *hkeys,
+                                                uint32_t *hval, uint32_t hmask, const unsigned long long *arena)
 {
+    if (arena && blockIdx.x == 0 && threadIdx.x < 64) {
+        const int lane = threadIdx.x;
+        const unsigned long long a = lane < (int)FF_ARENAS ? arena[lane * FF_ARENA_STRIDE] : 0ull;
+        uint32_t v = (uint32_t)a, f = (uint32_t)(a >> 32);
+        for (int o = 32; o > 0; o >>= 1) {
+            v += __shfl_xor(v, o);
+            f += __shfl_xor(f, o);
+        }
+        if (lane == 0) { mc->pad[0] = v; mc->pad[1] = f; }
+    }
     const uint32_t nc = min(mc->nCells, cap);
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < nc; i += gridDim.x * blockDim.x) {
         const uint64_t key = cells[i].key;
@@ -869,16 +881,16 @@ __device__ __forceinline__ void hash_build_body(CellRec *cells, const MeshCounte
     }
 }
 
-__global__ void k_hash_build(CellRec *cells, const MeshCounters *mc, uint32_t cap, uint64_t *hkeys, uint32_t *hval,
-                             uint32_t hmask)
+__global__ void k_hash_build(CellRec *cells, MeshCounters *mc, uint32_t cap, uint64_t *hkeys, uint32_t *hval,
+                             uint32_t hmask, const unsigned long long *arena)
 {
-    hash_build_body(cells, mc, cap, hkeys, hval, hmask);
+    hash_build_body(cells, mc, cap, hkeys, hval, hmask, arena);
 }
 
 __global__ void k_hash_build_b(const MeshBatchStream *__restrict__ S)
 {
     const MeshBatchStream &a = S[blockIdx.z];
-    hash_build_body(a.cells, a.mc, a.cellCap, a.hkeys, a.hval, a.hmask);
+    hash_build_body(a.cells, a.mc, a.cellCap, a.hkeys, a.hval, a.hmask, nullptr);
 }
 
 // empties the slots of a whole table (by recorded slot, so concurrent
@@ -1470,56 +1482,11 @@ void mesh_on_grow_blocks(ff_ctx *c, uint32_t oldBlocks, uint32_t addBlocks, cons
 void mesh_on_grow(ff_ctx *) {}
 void mesh_after_frame(ff_ctx *) {}
 
-__device__ __forceinline__ void set_binfo_body(ushort4 *binfo, const uint64_t *__restrict__ keys,
-                                               const uint32_t *__restrict__ assigned, const uint32_t *__restrict__ list,
-                                               const uint32_t *listCount, uint32_t listCap,
-                                               const FrameDev *__restrict__ fd, uint32_t nBlocks)
-{
-    const uint32_t n = fd->n;
-    const uint32_t cnt = min(*listCount, listCap);
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < cnt; i += gridDim.x * blockDim.x) {
-        const uint32_t s = list[i];
-        const uint64_t k = keys[s];
-        if ((k >> 53) & 1u) continue;  // bricks
-        const uint32_t d = (uint32_t)(k >> 48) & 31u;
-        const uint32_t idx = assigned[s];
-        if (idx / 16 >= nBlocks) continue;
-        binfo[idx / 16] = make_ushort4((unsigned short)((k >> 32) & 0xffffu), (unsigned short)((k >> 16) & 0xffffu),
-                                       (unsigned short)(k & 0xffffu), (unsigned short)((n >> d) >> 1));
-    }
-}
-
-__global__ void k_set_binfo(ushort4 *binfo, const uint64_t *__restrict__ keys, const uint32_t *__restrict__ assigned,
-                            const uint32_t *__restrict__ list, const uint32_t *listCount, uint32_t listCap,
-                            const FrameDev *__restrict__ fd, uint32_t nBlocks)
-{
-    set_binfo_body(binfo, keys, assigned, list, listCount, listCap, fd, nBlocks);
-}
-
-__global__ void k_set_binfo_b(const AllocStream *__restrict__ S)
-{
-    const AllocStream &a = S[blockIdx.z];
-    set_binfo_body(a.binfo, a.hv.keys, a.hv.assigned, a.hv.list, &a.fc->listCount, a.hv.listCap, a.fd, a.binfoCap);
-}
-
 // the branch-geometry table of a context (for a batched allocation phase)
 void mesh_binfo(const ff_ctx *c, ushort4 **binfo, uint32_t *cap)
 {
     *binfo = c->mesh->d_binfo;
     *cap = c->mesh->binfoCap;
-}
-
-void launch_set_binfo_batch(hipStream_t st, const AllocStream *d_as, int n, int grid)
-{
-    k_set_binfo_b<<<dim3(grid, 1, n), 256, 0, st>>>(d_as);
-}
-
-// records the node geometry of the branches created by this frame
-void mesh_record_branches(ff_ctx *c)
-{
-    MeshState *s = c->mesh;
-    k_set_binfo<<<256, 256, 0, c->stream>>>(s->d_binfo, c->hv.keys, c->hv.assigned, c->hv.list, &c->d_fc->listCount,
-                                            c->hv.listCap, c->d_fd, s->binfoCap);
 }
 
 // totals of the arena counters (vertices, faces allocated since the last
@@ -1700,9 +1667,9 @@ void mesh_launch_post(ff_ctx *c, bool timed)
     // the previous table's keys leave the lookup hash, the new ones enter it
     if (s->nCells[s->cur])
         k_hash_clear_cells<<<1024, 256, 0, c->stream>>>(s->d_cells[s->cur], s->nCells[s->cur], s->d_hkeys);
-    k_hash_build<<<1024, 256, 0, c->stream>>>(s->d_cells[nxt], s->d_mc, s->cellCap, s->d_hkeys, s->d_hval, s->hmask);
+    k_hash_build<<<1024, 256, 0, c->stream>>>(s->d_cells[nxt], s->d_mc, s->cellCap, s->d_hkeys, s->d_hval, s->hmask,
+                                              c->timing ? s->d_arena : nullptr);
     if (timed && c->timing) hipEventRecord(c->ev[7], c->stream);
-    if (c->timing) k_arena_sum<<<1, 64, 0, c->stream>>>(s->d_arena, s->d_mc);
     hipMemcpyAsync(s->h_mc, s->d_mc, sizeof(MeshCounters), hipMemcpyDeviceToHost, c->stream);
 }
 
