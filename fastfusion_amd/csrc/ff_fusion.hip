@@ -168,9 +168,7 @@ __global__ void k_frame_begin(const FrameDev *__restrict__ fd, FrameCounters *fc
 // descent reads them)
 __device__ __forceinline__ void planes_body(const FrameDev *__restrict__ fd, int W, int H, int words,
                                             const float *__restrict__ pxp, const float *__restrict__ pyp,
-                                            uint32_t *__restrict__ planes, uint32_t *__restrict__ cntB,
-                                            uint32_t *__restrict__ cntL, uint32_t *__restrict__ firstD,
-                                            FrameCounters *fc)
+                                            uint32_t *__restrict__ planes, FrameCounters *fc)
 {
     const int lane = threadIdx.x, y = blockIdx.y, x = blockIdx.x * 64 + lane;
     const uint16_t *__restrict__ depth = fd->depth;
@@ -195,10 +193,6 @@ __device__ __forceinline__ void planes_body(const FrameDev *__restrict__ fd, int
             voxel_of(p, px, pyp[y - 1], (float)depth[(y - 1) * W + x], scaling, u);
             eu = ((u[0] & -8) == (v[0] & -8)) && ((u[1] & -8) == (v[1] & -8)) && ((u[2] & -8) == (v[2] & -8));
         }
-        const size_t i = (size_t)y * W + x;
-        cntB[i] = 0;
-        cntL[i] = 0;
-        firstD[i] = 0xffu;
     }
     const uint64_t bv = __ballot(valid), bl = __ballot(el), bu = __ballot(eu);
     if (lane < 2) {
@@ -215,11 +209,9 @@ __device__ __forceinline__ void planes_body(const FrameDev *__restrict__ fd, int
 
 __global__ __launch_bounds__(64) void k_planes(const FrameDev *__restrict__ fd, int W, int H, int words,
                                                const float *__restrict__ pxp, const float *__restrict__ pyp,
-                                               uint32_t *__restrict__ planes, uint32_t *__restrict__ cntB,
-                                               uint32_t *__restrict__ cntL, uint32_t *__restrict__ firstD,
-                                               FrameCounters *fc)
+                                               uint32_t *__restrict__ planes, FrameCounters *fc)
 {
-    planes_body(fd, W, H, words, pxp, pyp, planes, cntB, cntL, firstD, fc);
+    planes_body(fd, W, H, words, pxp, pyp, planes, fc);
 }
 
 // the batched forms of the allocation kernels: stream blockIdx.z of S, grid
@@ -229,7 +221,7 @@ __global__ __launch_bounds__(64) void k_planes_b(const AllocStream *__restrict__
 {
     const AllocStream &a = S[blockIdx.z];
     if ((int)blockIdx.y >= a.H || (int)blockIdx.x * 64 >= a.W) return;
-    planes_body(a.fd, a.W, a.H, a.words, a.pxp, a.pyp, a.planes, a.cntB, a.cntL, a.firstD, a.fc);
+    planes_body(a.fd, a.W, a.H, a.words, a.pxp, a.pyp, a.planes, a.fc);
 }
 
 // ---------------------------------------------------------------------------
@@ -508,6 +500,88 @@ __global__ void k_descend_b(const AllocStream *__restrict__ S)
     descend_body(a.fd, a.W, a.H, a.words, a.pxp, a.pyp, a.pbits, a.m, a.hv, a.pl0, a.fc);
 }
 
+// ---------------------------------------------------------------------------
+// Indices of the frame's new nodes.  The reference creates nodes pixel by
+// pixel in raster order, each pixel's missing branches from the top down,
+// then its brick; branch and brick indices are handed out in that order.
+// So a new branch's index is nBranchesUsed + 16 x (its rank among the new
+// branches by (creating pixel, depth)) and a new brick's is nLeavesUsed + its
+// rank among the new bricks by creating pixel.  k_order sorts up to
+// FF_ORDER_MAX nodes in one block (bitonic, in LDS); a frame with more (a
+// first frame, a large jump) is flagged and ordered by the host through the
+// per-pixel count scan below (k_resolve, k_scan_a/b, k_assign).
+// ---------------------------------------------------------------------------
+#define FF_ORDER_MAX 8192   // nodes ranked in LDS (64 KiB of keys)
+#define FF_ORDER_THREADS 1024
+
+__device__ __forceinline__ void order_body(const HashView &hv, FrameCounters *fc, const MapCounters *__restrict__ mapc)
+{
+    // key: type (63) | creating pixel (29..50, < 2^22) | depth (24..28) | list
+    // index (0..12): sorted ascending, branches come first in (pixel,
+    // depth) order, then bricks in pixel order
+    __shared__ uint64_t key[FF_ORDER_MAX];
+    __shared__ uint32_t nBr;
+    const uint32_t n = min(fc->listCount, hv.listCap);
+    if (n > FF_ORDER_MAX) {
+        if (threadIdx.x == 0) atomicOr(&fc->flags, FF_FLAG_ORDER_BIG);
+        return;
+    }
+    const uint32_t t = threadIdx.x;
+    uint32_t N = 2;
+    while (N < n) N <<= 1;
+    if (t == 0) nBr = 0;
+    __syncthreads();
+    uint32_t myBr = 0;
+    for (uint32_t i = t; i < N; i += FF_ORDER_THREADS) {
+        uint64_t k = ~0ull;  // padding sorts last
+        if (i < n) {
+            const uint32_t s = hv.list[i];
+            const uint64_t hk = hv.keys[s];
+            const uint64_t type = (hk >> 53) & 1ull;
+            k = (type << 63) | ((uint64_t)hv.minpix[s] << 29) | (((hk >> 48) & 31ull) << 24) | (uint64_t)i;
+            myBr += type ? 0u : 1u;
+        }
+        key[i] = k;
+    }
+    if (myBr) atomicAdd(&nBr, myBr);
+    __syncthreads();
+    // bitonic sort of the N keys
+    for (uint32_t k = 2; k <= N; k <<= 1)
+        for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+            for (uint32_t p = t; p < N / 2; p += FF_ORDER_THREADS) {
+                const uint32_t i = ((p & ~(j - 1u)) << 1) | (p & (j - 1u)), l = i + j;
+                const uint64_t a = key[i], b = key[l];
+                if ((a > b) == ((i & k) == 0u)) {
+                    key[i] = b;
+                    key[l] = a;
+                }
+            }
+            __syncthreads();
+        }
+    const uint32_t nb = nBr;
+    for (uint32_t r = t; r < n; r += FF_ORDER_THREADS) {
+        const uint64_t k = key[r];
+        const uint32_t s = hv.list[(uint32_t)k & 0x1fffu];
+        hv.assigned[s] = (k >> 63) ? mapc->nLeavesUsed + (r - nb) : mapc->nBranchesUsed + 16u * r;
+    }
+    if (t == 0) {
+        fc->newB = nb;
+        fc->newL = n - nb;
+    }
+}
+
+__global__ __launch_bounds__(FF_ORDER_THREADS) void k_order(HashView hv, FrameCounters *fc,
+                                                        const MapCounters *__restrict__ mapc)
+{
+    order_body(hv, fc, mapc);
+}
+
+__global__ __launch_bounds__(FF_ORDER_THREADS) void k_order_b(const AllocStream *__restrict__ S)
+{
+    const AllocStream &a = S[blockIdx.z];
+    order_body(a.hv, a.fc, a.mapc);
+}
+
 // per new node: its creating pixel counts it
 __device__ __forceinline__ void resolve_body(const HashView &hv, const FrameCounters *fc, uint32_t *__restrict__ cntB,
                                              uint32_t *__restrict__ cntL, uint32_t *__restrict__ firstD)
@@ -533,11 +607,6 @@ __global__ void k_resolve(HashView hv, const FrameCounters *fc, uint32_t *__rest
     resolve_body(hv, fc, cntB, cntL, firstD);
 }
 
-__global__ void k_resolve_b(const AllocStream *__restrict__ S)
-{
-    const AllocStream &a = S[blockIdx.z];
-    resolve_body(a.hv, a.fc, a.cntB, a.cntL, a.firstD);
-}
 
 // ---------------------------------------------------------------------------
 // exclusive scan of (cntB, cntL) over pixels, 4096 pixels per block
@@ -601,12 +670,6 @@ __global__ __launch_bounds__(1024) void k_scan_a(const uint32_t *__restrict__ cn
     scan_a_body(cntB, cntL, n, pref, bsum);
 }
 
-__global__ __launch_bounds__(1024) void k_scan_a_b(const AllocStream *__restrict__ S)
-{
-    const AllocStream &a = S[blockIdx.z];
-    if (blockIdx.x >= a.nb) return;
-    scan_a_body(a.cntB, a.cntL, a.npix, a.pref, a.bsum);
-}
 
 __device__ __forceinline__ void scan_b_body(uint64_t *__restrict__ bsum, uint32_t nb, FrameCounters *fc)
 {
@@ -635,11 +698,6 @@ __global__ __launch_bounds__(1024) void k_scan_b(uint64_t *__restrict__ bsum, ui
     scan_b_body(bsum, nb, fc);
 }
 
-__global__ __launch_bounds__(1024) void k_scan_b_b(const AllocStream *__restrict__ S)
-{
-    const AllocStream &a = S[blockIdx.z];
-    scan_b_body(a.bsum, a.nb, a.fc);
-}
 
 __device__ __forceinline__ void assign_body(const HashView &hv, const FrameCounters *fc,
                                             const uint64_t *__restrict__ pref, const uint64_t *__restrict__ bsum,
@@ -667,11 +725,6 @@ __global__ void k_assign(HashView hv, const FrameCounters *fc, const uint64_t *_
     assign_body(hv, fc, pref, bsum, firstD, mapc);
 }
 
-__global__ void k_assign_b(const AllocStream *__restrict__ S)
-{
-    const AllocStream &a = S[blockIdx.z];
-    assign_body(a.hv, a.fc, a.pref, a.bsum, a.firstD, a.mapc);
-}
 
 // parent branch's tree index of a node created this frame
 __device__ __forceinline__ uint32_t parent_branch(const HashView &hv, uint32_t aux, uint32_t d, uint32_t x,
@@ -684,10 +737,12 @@ __device__ __forceinline__ uint32_t parent_branch(const HashView &hv, uint32_t a
     return s >= 0 ? hv.assigned[s] : FF_BRANCHINIT;
 }
 
+// (slow: the host's ordering pass; otherwise a frame flagged for it skips)
 __device__ __forceinline__ void link_body(const HashView &hv, const MapView &m, const FrameCounters *fc,
                                           const uint32_t *__restrict__ pl0, const FrameDev *__restrict__ fd,
-                                          FrameCounters *fcw)
+                                          FrameCounters *fcw, bool slow)
 {
+    if (!slow && (fc->flags & FF_FLAG_ORDER_BIG)) return;
     const uint32_t n = fd->n, stamp = fd->stamp;
     const uint32_t cnt = min(fc->listCount, hv.listCap);
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < cnt; i += gridDim.x * blockDim.x) {
@@ -733,15 +788,15 @@ __device__ __forceinline__ void link_body(const HashView &hv, const MapView &m, 
 }
 
 __global__ void k_link(HashView hv, MapView m, const FrameCounters *fc, const uint32_t *__restrict__ pl0,
-                       const FrameDev *__restrict__ fd, FrameCounters *fcw)
+                       const FrameDev *__restrict__ fd, FrameCounters *fcw, int slow)
 {
-    link_body(hv, m, fc, pl0, fd, fcw);
+    link_body(hv, m, fc, pl0, fd, fcw, slow != 0);
 }
 
 __global__ void k_link_b(const AllocStream *__restrict__ S)
 {
     const AllocStream &a = S[blockIdx.z];
-    link_body(a.hv, a.m, a.fc, a.pl0, a.fd, a.fc);
+    link_body(a.hv, a.m, a.fc, a.pl0, a.fd, a.fc, false);
 }
 
 // The end of the allocation phase, per new node: the branch's geometry for
@@ -751,8 +806,9 @@ __global__ void k_link_b(const AllocStream *__restrict__ S)
 // whose lookups read other nodes' slots, has completed.
 __device__ __forceinline__ void alloc_end_body(const HashView &hv, const FrameCounters *__restrict__ fc,
                                                MapCounters *mapc, ushort4 *binfo, uint32_t binfoBlocks,
-                                               const FrameDev *__restrict__ fd)
+                                               const FrameDev *__restrict__ fd, bool slow)
 {
+    if (!slow && (fc->flags & FF_FLAG_ORDER_BIG)) return;
     const uint32_t n = fd->n;
     const uint32_t cnt = min(fc->listCount, hv.listCap);
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < cnt; i += gridDim.x * blockDim.x) {
@@ -776,15 +832,15 @@ __device__ __forceinline__ void alloc_end_body(const HashView &hv, const FrameCo
 }
 
 __global__ void k_alloc_end(HashView hv, const FrameCounters *__restrict__ fc, MapCounters *mapc, ushort4 *binfo,
-                            uint32_t binfoBlocks, const FrameDev *__restrict__ fd)
+                            uint32_t binfoBlocks, const FrameDev *__restrict__ fd, int slow)
 {
-    alloc_end_body(hv, fc, mapc, binfo, binfoBlocks, fd);
+    alloc_end_body(hv, fc, mapc, binfo, binfoBlocks, fd, slow != 0);
 }
 
 __global__ void k_alloc_end_b(const AllocStream *__restrict__ S)
 {
     const AllocStream &a = S[blockIdx.z];
-    alloc_end_body(a.hv, a.fc, a.mapc, a.binfo, a.binfoCap, a.fd);
+    alloc_end_body(a.hv, a.fc, a.mapc, a.binfo, a.binfoCap, a.fd, false);
 }
 
 // ---------------------------------------------------------------------------
@@ -998,7 +1054,7 @@ __device__ __forceinline__ bool sdf_gate_open(const FrameCounters *fc, const Map
                                               const SdfGate &g)
 {
     if (g.force) return true;
-    if ((fc->flags & FF_FLAG_HASH_FULL) || fc->listCount > g.listCap) return false;
+    if ((fc->flags & (FF_FLAG_HASH_FULL | FF_FLAG_ORDER_BIG)) || fc->listCount > g.listCap) return false;
     const bool out = fc->bmin[0] < 0 || fc->bmin[1] < 0 || fc->bmin[2] < 0 || fc->bmax[0] >= (int)n ||
                      fc->bmax[1] >= (int)n || fc->bmax[2] >= (int)n;
     const bool grows = mapc->nLeavesUsed < g.leavesTotal && mapc->nBranchesUsed < g.branchesTotal;
@@ -1442,10 +1498,8 @@ namespace ff {
 void launch_alloc(ff_ctx *c, int W, int H)
 {
     const int words = (W + 31) / 32;
-    const uint32_t npix = (uint32_t)W * H;
     const FrameDev *fd = c->d_fd;
-    k_planes<<<dim3((W + 63) / 64, H), 64, 0, c->stream>>>(fd, W, H, words, c->d_pxp, c->d_pyp, c->d_planes, c->d_cntB,
-                                                          c->d_cntL, c->d_firstD, c->d_fc);
+    k_planes<<<dim3((W + 63) / 64, H), 64, 0, c->stream>>>(fd, W, H, words, c->d_pxp, c->d_pyp, c->d_planes, c->d_fc);
     if (dedup_rows(words) == 32)
         k_dedup<32><<<1, FF_DD_THREADS, dedup_lds_bytes(words), c->stream>>>(c->d_planes, H, words, c->d_pbits,
                                                                              c->d_fc);
@@ -1455,16 +1509,38 @@ void launch_alloc(ff_ctx *c, int W, int H)
     const MapView mv = c->mapView();
     k_descend<<<dim3((W + 127) / 128, H), 128, 0, c->stream>>>(fd, W, H, words, c->d_pxp, c->d_pyp, c->d_pbits, mv,
                                                                c->hv, c->d_pl0, c->d_fc);
+    k_order<<<1, FF_ORDER_THREADS, 0, c->stream>>>(c->hv, c->d_fc, c->d_mapc);
+    k_link<<<256, 256, 0, c->stream>>>(c->hv, mv, c->d_fc, c->d_pl0, fd, c->d_fc, 0);
+    ushort4 *binfo;
+    uint32_t binfoBlocks;
+    mesh_binfo(c, &binfo, &binfoBlocks);
+    k_alloc_end<<<256, 256, 0, c->stream>>>(c->hv, c->d_fc, c->d_mapc, binfo, binfoBlocks, fd, 0);
+}
+
+// A frame with more new nodes than k_order ranks (flagged; its link, commit
+// and SDF update stood down): order them through the per-pixel count scan,
+// then link and commit, and bring the counters back.
+static int order_big_frame(ff_ctx *c, int W, int H)
+{
+    const uint32_t npix = (uint32_t)W * H;
+    const FrameDev *fd = c->d_fd;
+    HIPCHK(hipMemsetAsync(c->d_cntB, 0, sizeof(uint32_t) * npix, c->stream));
+    HIPCHK(hipMemsetAsync(c->d_cntL, 0, sizeof(uint32_t) * npix, c->stream));
+    HIPCHK(hipMemsetAsync(c->d_firstD, 0xff, sizeof(uint32_t) * npix, c->stream));
     k_resolve<<<256, 256, 0, c->stream>>>(c->hv, c->d_fc, c->d_cntB, c->d_cntL, c->d_firstD);
     const uint32_t nb = (npix + 4095) / 4096;
     k_scan_a<<<nb, 1024, 0, c->stream>>>(c->d_cntB, c->d_cntL, npix, c->d_pref, c->d_bsum);
     k_scan_b<<<1, 1024, 0, c->stream>>>(c->d_bsum, nb, c->d_fc);
     k_assign<<<256, 256, 0, c->stream>>>(c->hv, c->d_fc, c->d_pref, c->d_bsum, c->d_firstD, c->d_mapc);
-    k_link<<<256, 256, 0, c->stream>>>(c->hv, mv, c->d_fc, c->d_pl0, fd, c->d_fc);
+    k_link<<<256, 256, 0, c->stream>>>(c->hv, c->mapView(), c->d_fc, c->d_pl0, fd, c->d_fc, 1);
     ushort4 *binfo;
     uint32_t binfoBlocks;
     mesh_binfo(c, &binfo, &binfoBlocks);
-    k_alloc_end<<<256, 256, 0, c->stream>>>(c->hv, c->d_fc, c->d_mapc, binfo, binfoBlocks, fd);
+    k_alloc_end<<<256, 256, 0, c->stream>>>(c->hv, c->d_fc, c->d_mapc, binfo, binfoBlocks, fd, 1);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(c->h_fc, c->d_fc, sizeof(FrameCounters), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    return FF_OK;
 }
 
 AllocStream alloc_stream(ff_ctx *c, int W, int H)
@@ -1481,8 +1557,6 @@ AllocStream alloc_stream(ff_ctx *c, int W, int H)
     a.m = c->mapView();
     a.hv = c->hv;
     a.W = W; a.H = H; a.words = (W + 31) / 32;
-    a.npix = (uint32_t)W * H;
-    a.nb = (a.npix + 4095) / 4096;
     return a;
 }
 
@@ -1490,14 +1564,12 @@ AllocStream alloc_stream(ff_ctx *c, int W, int H)
 void launch_alloc_batch(hipStream_t st, const AllocStream *d_as, const AllocStream *h_as, int n)
 {
     int maxW = 1, maxH = 1;
-    uint32_t maxNb = 1;
     bool r32 = false, r16 = false;
     size_t lds32 = 0, lds16 = 0;
     for (int k = 0; k < n; k++) {
         const AllocStream &a = h_as[k];
         maxW = std::max(maxW, a.W);
         maxH = std::max(maxH, a.H);
-        maxNb = std::max(maxNb, a.nb);
         if (dedup_rows(a.words) == 32) {
             r32 = true;
             lds32 = std::max(lds32, dedup_lds_bytes(a.words));
@@ -1511,10 +1583,7 @@ void launch_alloc_batch(hipStream_t st, const AllocStream *d_as, const AllocStre
     if (r32) k_dedup_b<32><<<dim3(1, 1, n), FF_DD_THREADS, lds32, st>>>(d_as);
     if (r16) k_dedup_b<16><<<dim3(1, 1, n), FF_DD_THREADS, lds16, st>>>(d_as);
     k_descend_b<<<dim3((maxW + 127) / 128, maxH, n), 128, 0, st>>>(d_as);
-    k_resolve_b<<<dim3(G, 1, n), 256, 0, st>>>(d_as);
-    k_scan_a_b<<<dim3(maxNb, 1, n), 1024, 0, st>>>(d_as);
-    k_scan_b_b<<<dim3(1, 1, n), 1024, 0, st>>>(d_as);
-    k_assign_b<<<dim3(G, 1, n), 256, 0, st>>>(d_as);
+    k_order_b<<<dim3(1, 1, n), FF_ORDER_THREADS, 0, st>>>(d_as);
     k_link_b<<<dim3(G, 1, n), 256, 0, st>>>(d_as);
     k_alloc_end_b<<<dim3(G, 1, n), 256, 0, st>>>(d_as);
 }
@@ -1677,6 +1746,10 @@ int frame_finish(ff_ctx *c, int W, int H, bool timed)
             c->phaseMs[1] += ms;
             c->phaseCalls[1]++;
         }
+    }
+    if (hf->flags & FF_FLAG_ORDER_BIG) {
+        const int r = order_big_frame(c, W, H);
+        if (r) return r;
     }
     if (hf->flags & FF_FLAG_HASH_FULL) {
         c->err = "per-frame node table overflow";

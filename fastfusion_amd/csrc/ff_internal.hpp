@@ -11,6 +11,7 @@
 #define FF_HEMPTY 0xffffffffffffffffull
 #define FF_FLAG_HASH_FULL 1u
 #define FF_FLAG_POOL_FULL 2u
+#define FF_FLAG_ORDER_BIG 4u  // more new nodes than k_order ranks: the host orders them by the per-pixel scan
 
 // processed-pixel bitmap: 32 pixels per word, rows FF_PB_STRIDE words apart,
 // FF_PB_PAD rows of padding above the image and 2 FF_PB_PAD below it (the
@@ -108,7 +109,7 @@ struct AllocStream {
     MapView m;
     HashView hv;
     int W, H, words;
-    uint32_t npix, nb, binfoCap;
+    uint32_t binfoCap;
 };
 
 struct MeshState;
