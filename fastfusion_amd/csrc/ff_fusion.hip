@@ -400,90 +400,51 @@ __device__ __forceinline__ void descend_body(const FrameDev *__restrict__ fd, in
         if (lane == l0) leader = true;
         todo &= ~same;
     }
-    // The walk is a chain of dependent tree loads; the bricks on the path
-    // are queued after it, all at once (stamp loads, then the stamp
-    // exchanges in flight together, then one append per wave), instead of
-    // adding three round trips per brick to the chain.  The side n is at
-    // most 2^15 (16-bit brick coordinates), so the walk has at most 14
-    // steps before the target level.
-    constexpr int LV = 16;
-    uint32_t lv[LV];  // bricks on the path (FF_BRANCHINIT: none at that level)
-#pragma unroll
-    for (int k = 0; k < LV; k++) lv[k] = FF_BRANCHINIT;
-    if (leader) {
-        uint32_t ox = 0, oy = 0, oz = 0, half = n >> 1, branch = 0, depthLvl = 1;
-        uint32_t pl = FF_BRANCHINIT, pld = 0;
-        bool missing = false, walking = true;
-#pragma unroll
-        for (int k = 0; k < LV - 1; k++) {
-            walking = walking && (half >> 1) >= target;
-            if (walking) {
-                const uint32_t co = (X >= ox + half) * 1u + (Y >= oy + half) * 2u + (Z >= oz + half) * 4u;
-                const uint32_t nx = ox + half * (co & 1u), ny = oy + half * ((co >> 1) & 1u),
-                               nz = oz + half * ((co >> 2) & 1u);
-                if (!missing) {
-                    const uint32_t ptr = branch + co + 8;
-                    const uint32_t child = m.tree[ptr];
-                    const uint32_t leaf = m.tree[ptr - 8];
-                    if (leaf < m.nLeavesTotal) {
-                        pl = leaf;
-                        pld = depthLvl;
-                        lv[k] = leaf;
-                    }
-                    if (child == FF_BRANCHINIT) {
-                        missing = true;
-                        hash_insert_min(hv, node_key(0, depthLvl, nx, ny, nz), pix, branch, fc);
-                    }
-                    branch = child;
-                } else {
-                    hash_insert_min(hv, node_key(0, depthLvl, nx, ny, nz), pix, FF_BRANCHINIT, fc);
-                }
-                ox = nx; oy = ny; oz = nz;
-                half >>= 1;
-                depthLvl++;
-            }
-        }
+    // Bricks on the path take the frame's stamp (plain stores: no result is
+    // awaited, and the walk stays a chain of tree loads only); k_queue then
+    // collects the stamped bricks.  Queueing here with atomics cost more than
+    // the walk: every pixel's path shares the coarse bricks.
+    if (!leader) return;
+    uint32_t ox = 0, oy = 0, oz = 0, half = n >> 1, branch = 0, depthLvl = 1;
+    uint32_t pl = FF_BRANCHINIT, pld = 0;
+    bool missing = false;
+    while ((half >> 1) >= target) {
         const uint32_t co = (X >= ox + half) * 1u + (Y >= oy + half) * 2u + (Z >= oz + half) * 4u;
         const uint32_t nx = ox + half * (co & 1u), ny = oy + half * ((co >> 1) & 1u), nz = oz + half * ((co >> 2) & 1u);
         if (!missing) {
-            const uint32_t leaf = m.tree[branch + co];
-            if (leaf >= FF_BRANCHINIT)
-                hash_insert_min(hv, node_key(1, depthLvl, nx, ny, nz), pix, branch, fc);
-            else if (leaf < m.nLeavesTotal)
-                lv[LV - 1] = leaf;
+            const uint32_t ptr = branch + co + 8;
+            const uint32_t child = m.tree[ptr];
+            const uint32_t leaf = m.tree[ptr - 8];
+            if (leaf < m.nLeavesTotal) {
+                pl = leaf;
+                pld = depthLvl;
+                m.stamp[leaf] = stamp;
+            }
+            if (child == FF_BRANCHINIT) {
+                missing = true;
+                hash_insert_min(hv, node_key(0, depthLvl, nx, ny, nz), pix, branch, fc);
+            }
+            branch = child;
         } else {
-            hash_insert_min(hv, node_key(1, depthLvl, nx, ny, nz), pix, FF_BRANCHINIT, fc);
+            hash_insert_min(hv, node_key(0, depthLvl, nx, ny, nz), pix, FF_BRANCHINIT, fc);
         }
-        pl0[2 * pix] = pl;
-        pl0[2 * pix + 1] = pld;
+        ox = nx; oy = ny; oz = nz;
+        half >>= 1;
+        depthLvl++;
     }
-    // a brick joins the queue once per frame: its stamp is exchanged for the
-    // frame's, and the lane that saw the old stamp appends it
-    uint32_t st[LV];
-#pragma unroll
-    for (int k = 0; k < LV; k++) st[k] = lv[k] < m.nLeavesTotal ? m.stamp[lv[k]] : stamp;
-#pragma unroll
-    for (int k = 0; k < LV; k++)
-        if (st[k] != stamp) st[k] = atomicExch(&m.stamp[lv[k]], stamp);
-    uint32_t fresh = 0;
-#pragma unroll
-    for (int k = 0; k < LV; k++)
-        if (st[k] != stamp) fresh |= 1u << k;
-    const uint32_t cnt = (uint32_t)__popc(fresh);
-    uint32_t incl = cnt;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        const uint32_t t = __shfl_up(incl, o);
-        if (lane >= o) incl += t;
+    const uint32_t co = (X >= ox + half) * 1u + (Y >= oy + half) * 2u + (Z >= oz + half) * 4u;
+    const uint32_t nx = ox + half * (co & 1u), ny = oy + half * ((co >> 1) & 1u), nz = oz + half * ((co >> 2) & 1u);
+    if (!missing) {
+        const uint32_t leaf = m.tree[branch + co];
+        if (leaf >= FF_BRANCHINIT)
+            hash_insert_min(hv, node_key(1, depthLvl, nx, ny, nz), pix, branch, fc);
+        else if (leaf < m.nLeavesTotal)
+            m.stamp[leaf] = stamp;
+    } else {
+        hash_insert_min(hv, node_key(1, depthLvl, nx, ny, nz), pix, FF_BRANCHINIT, fc);
     }
-    const uint32_t total = __shfl(incl, 63);
-    uint32_t base = 0;
-    if (lane == 63 && total) base = atomicAdd(&fc->nQueued, total);
-    base = __shfl(base, 63);
-    uint32_t q = base + incl - cnt;
-#pragma unroll
-    for (int k = 0; k < LV; k++)
-        if (fresh & (1u << k)) m.queue[q++] = lv[k];
+    pl0[2 * pix] = pl;
+    pl0[2 * pix + 1] = pld;
 }
 
 __global__ void k_descend(const FrameDev *__restrict__ fd, int W, int H, int words, const float *__restrict__ pxp,
@@ -781,9 +742,7 @@ __device__ __forceinline__ void link_body(const HashView &hv, const MapView &m, 
         m.leafinfo[idx] = make_ushort4((unsigned short)x, (unsigned short)y, (unsigned short)z,
                                        (unsigned short)(size / FF_BL));
         m.parent[idx] = pl;
-        m.stamp[idx] = stamp;
-        const uint32_t q = atomicAdd(&fcw->nQueued, 1u);
-        m.queue[q] = idx;
+        m.stamp[idx] = stamp;  // queued by k_queue
     }
 }
 
@@ -841,6 +800,82 @@ __global__ void k_alloc_end_b(const AllocStream *__restrict__ S)
 {
     const AllocStream &a = S[blockIdx.z];
     alloc_end_body(a.hv, a.fc, a.mapc, a.binfo, a.binfoCap, a.fd, false);
+}
+
+// ---------------------------------------------------------------------------
+// k_queue: the frame's queue (_leafNumberSurface) is every brick carrying the
+// frame's stamp -- those on a processed pixel's path (k_descend) and the new
+// ones (k_link).  A block takes FF_QUEUE_SPAN bricks at a time and appends
+// them with one atomic.  The reference queues bricks in traversal order; the
+// SDF update does not depend on the order.
+// ---------------------------------------------------------------------------
+#define FF_QUEUE_THREADS 1024
+#define FF_QUEUE_SPAN (8 * FF_QUEUE_THREADS)
+
+// (the used count can pass the capacity when the pools ran out; bricks
+// exist only below nLeavesTotal)
+__device__ __forceinline__ void queue_body(const uint32_t *__restrict__ stamp, const MapCounters *__restrict__ mapc,
+                                           uint32_t nLeavesTotal, const FrameDev *__restrict__ fd,
+                                           uint32_t *__restrict__ queue, FrameCounters *fc, bool slow)
+{
+    __shared__ uint32_t wsum[FF_QUEUE_THREADS / 64 + 1];
+    if (!slow && (fc->flags & FF_FLAG_ORDER_BIG)) return;
+    const uint32_t nb = min(mapc->nLeavesUsed, nLeavesTotal), st = fd->stamp;
+    const uint32_t t = threadIdx.x, lane = t & 63u, w = t >> 6;
+    for (uint32_t base = blockIdx.x * FF_QUEUE_SPAN; base < nb; base += gridDim.x * FF_QUEUE_SPAN) {
+        uint32_t mask = 0;
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            const uint32_t i = base + (uint32_t)k * FF_QUEUE_THREADS + t;
+            if (i < nb && stamp[i] == st) mask |= 1u << k;
+        }
+        // block-exclusive prefix of the per-thread counts
+        const uint32_t c = (uint32_t)__popc(mask);
+        uint32_t incl = c;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t u = __shfl_up(incl, o);
+            if (lane >= (uint32_t)o) incl += u;
+        }
+        if (lane == 63) wsum[w] = incl;
+        __syncthreads();
+        if (t == 0) {
+            uint32_t tot = 0;
+            for (uint32_t k = 0; k < FF_QUEUE_THREADS / 64; k++) {
+                const uint32_t x = wsum[k];
+                wsum[k] = tot;
+                tot += x;
+            }
+            wsum[FF_QUEUE_THREADS / 64] = tot ? atomicAdd(&fc->nQueued, tot) : 0u;
+        }
+        __syncthreads();
+        uint32_t q = wsum[FF_QUEUE_THREADS / 64] + wsum[w] + incl - c;
+#pragma unroll
+        for (int k = 0; k < 8; k++)
+            if (mask & (1u << k)) queue[q++] = base + (uint32_t)k * FF_QUEUE_THREADS + t;
+        __syncthreads();  // wsum is reused
+    }
+}
+
+__global__ __launch_bounds__(FF_QUEUE_THREADS) void k_queue(const uint32_t *__restrict__ stamp,
+                                                            const MapCounters *__restrict__ mapc, uint32_t nLeavesTotal,
+                                                            const FrameDev *__restrict__ fd, uint32_t *__restrict__ queue,
+                                                            FrameCounters *fc, int slow)
+{
+    queue_body(stamp, mapc, nLeavesTotal, fd, queue, fc, slow != 0);
+}
+
+__global__ __launch_bounds__(FF_QUEUE_THREADS) void k_queue_b(const AllocStream *__restrict__ S)
+{
+    const AllocStream &a = S[blockIdx.z];
+    queue_body(a.m.stamp, a.mapc, a.m.nLeavesTotal, a.fd, a.m.queue, a.fc, false);
+}
+
+// blocks of k_queue for a map of nLeavesUsed bricks (its grid-stride loop
+// covers any count)
+static inline uint32_t queue_grid(uint32_t nLeavesUsed)
+{
+    return std::min<uint32_t>(256u, nLeavesUsed / FF_QUEUE_SPAN + 2u);
 }
 
 // ---------------------------------------------------------------------------
@@ -1515,6 +1550,8 @@ void launch_alloc(ff_ctx *c, int W, int H)
     uint32_t binfoBlocks;
     mesh_binfo(c, &binfo, &binfoBlocks);
     k_alloc_end<<<256, 256, 0, c->stream>>>(c->hv, c->d_fc, c->d_mapc, binfo, binfoBlocks, fd, 0);
+    k_queue<<<queue_grid(c->nLeavesUsed), FF_QUEUE_THREADS, 0, c->stream>>>(c->d_stamp, c->d_mapc, c->nLeavesTotal,
+                                                                            fd, c->d_queue, c->d_fc, 0);
 }
 
 // A frame with more new nodes than k_order ranks (flagged; its link, commit
@@ -1537,6 +1574,8 @@ static int order_big_frame(ff_ctx *c, int W, int H)
     uint32_t binfoBlocks;
     mesh_binfo(c, &binfo, &binfoBlocks);
     k_alloc_end<<<256, 256, 0, c->stream>>>(c->hv, c->d_fc, c->d_mapc, binfo, binfoBlocks, fd, 1);
+    k_queue<<<queue_grid(c->nLeavesUsed + c->hv.listCap), FF_QUEUE_THREADS, 0, c->stream>>>(
+        c->d_stamp, c->d_mapc, c->nLeavesTotal, fd, c->d_queue, c->d_fc, 1);
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(c->h_fc, c->d_fc, sizeof(FrameCounters), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
@@ -1557,6 +1596,7 @@ AllocStream alloc_stream(ff_ctx *c, int W, int H)
     a.m = c->mapView();
     a.hv = c->hv;
     a.W = W; a.H = H; a.words = (W + 31) / 32;
+    a.nLeavesUsed = c->nLeavesUsed;
     return a;
 }
 
@@ -1564,12 +1604,14 @@ AllocStream alloc_stream(ff_ctx *c, int W, int H)
 void launch_alloc_batch(hipStream_t st, const AllocStream *d_as, const AllocStream *h_as, int n)
 {
     int maxW = 1, maxH = 1;
+    uint32_t maxLeaves = 0;
     bool r32 = false, r16 = false;
     size_t lds32 = 0, lds16 = 0;
     for (int k = 0; k < n; k++) {
         const AllocStream &a = h_as[k];
         maxW = std::max(maxW, a.W);
         maxH = std::max(maxH, a.H);
+        maxLeaves = std::max(maxLeaves, a.nLeavesUsed);
         if (dedup_rows(a.words) == 32) {
             r32 = true;
             lds32 = std::max(lds32, dedup_lds_bytes(a.words));
@@ -1586,6 +1628,7 @@ void launch_alloc_batch(hipStream_t st, const AllocStream *d_as, const AllocStre
     k_order_b<<<dim3(1, 1, n), FF_ORDER_THREADS, 0, st>>>(d_as);
     k_link_b<<<dim3(G, 1, n), 256, 0, st>>>(d_as);
     k_alloc_end_b<<<dim3(G, 1, n), 256, 0, st>>>(d_as);
+    k_queue_b<<<dim3(queue_grid(maxLeaves), 1, n), FF_QUEUE_THREADS, 0, st>>>(d_as);
 }
 
 // the SDF update waits for the host only when the node table overflowed or

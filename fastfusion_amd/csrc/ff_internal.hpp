@@ -110,6 +110,7 @@ struct AllocStream {
     HashView hv;
     int W, H, words;
     uint32_t binfoCap;
+    uint32_t nLeavesUsed;  // host's count before the frame (k_queue's grid)
 };
 
 struct MeshState;
