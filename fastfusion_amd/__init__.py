@@ -70,6 +70,9 @@ ABI = {
     "ff_set_timing": (ctypes.c_int, [_P, ctypes.c_int]),
     "ff_get_timing": (ctypes.c_int, [_P, _P, _P, _P, _P]),
     "ff_reset_timing": (ctypes.c_int, [_P]),
+    "ff_map_description": (ctypes.c_int, [_P, ctypes.c_char_p, _SZ]),
+    "ff_save_map": (ctypes.c_int, [_P, ctypes.c_char_p]),
+    "ff_load_map": (ctypes.c_int, [_P, ctypes.c_char_p, ctypes.POINTER(ctypes.c_float), ctypes.c_uint32]),
     "ff_set_learned_weighting": (ctypes.c_int, [_P, _P, _SZ]),
     "ff_copy_confidence": (ctypes.c_int, [_P, _P, _SZ]),
     "ff_trainer_create": (_P, [ctypes.c_int, _SZ, _P, _SZ, ctypes.c_uint64]),
@@ -136,24 +139,64 @@ class MeshInterleaved:
         self.colors = colors
         self.faces = faces
 
-    def writePLY(self, path):
-        """Binary little-endian PLY, like MeshInterleaved::writePLY
-        (src/fusion/mesh_interleaved.cpp:139-200)."""
-        nv, nf = len(self.vertices), len(self.faces)
-        with open(path, "wb") as f:
-            hdr = ("ply\nformat binary_little_endian 1.0\n"
-                   f"element vertex {nv}\nproperty float x\nproperty float y\nproperty float z\n"
-                   "property uchar red\nproperty uchar green\nproperty uchar blue\n"
-                   f"element face {nf}\nproperty list uchar uint vertex_indices\nend_header\n")
-            f.write(hdr.encode())
-            v = np.zeros(nv, dtype=[("p", "<f4", 3), ("c", "u1", 3)])
-            v["p"] = self.vertices
-            v["c"] = self.colors
-            f.write(v.tobytes())
-            fa = np.zeros(nf, dtype=[("n", "u1"), ("i", "<u4", 3)])
-            fa["n"] = 3
-            fa["i"] = self.faces
-            f.write(fa.tobytes())
+    def writePLY(self, filename, binary=True):
+        """MeshInterleaved::writePLY (src/fusion/mesh_interleaved.cpp:139-200):
+        writes filename + ".ply", binary little-endian or ASCII, with the
+        reference's header lines."""
+        v, c, f = self.vertices, self.colors, self.faces
+        nv, nf = len(v), len(f)
+        hdr = "ply" + ("\nformat binary_little_endian 1.0" if binary else "\nformat ascii 1.0")
+        hdr += f"\nelement vertex {nv}\nproperty float32 x\nproperty float32 y\nproperty float32 z"
+        if len(c):
+            hdr += "\nproperty uchar red\nproperty uchar green\nproperty uchar blue"
+        if nf:
+            hdr += f"\nelement face {nf}\nproperty list uint8 int32 vertex_indices"
+        hdr += "\nend_header"
+        with open(filename + ".ply", "wb") as out:
+            if binary:
+                out.write((hdr + "\n").encode())
+                rec = [("p", "<f4", 3)] + ([("c", "u1", 3)] if len(c) else [])
+                vv = np.zeros(nv, dtype=rec)
+                vv["p"] = v
+                if len(c):
+                    vv["c"] = c
+                out.write(vv.tobytes())
+                ff = np.zeros(nf, dtype=[("n", "u1"), ("i", "<u4", 3)])
+                ff["n"] = 3
+                ff["i"] = f
+                out.write(ff.tobytes())
+            else:
+                lines = [hdr]
+                for i in range(nv):
+                    line = f"{_g(v[i][0])} {_g(v[i][1])} {_g(v[i][2])}"
+                    if len(c):
+                        line += f" {int(c[i][0])} {int(c[i][1])} {int(c[i][2])}"
+                    lines.append(line)
+                for t in f:
+                    lines.append(f"3 {int(t[0])} {int(t[1])} {int(t[2])}")
+                out.write("\n".join(lines).encode())
+
+    def writeOBJ(self, filename):
+        """MeshInterleaved::writeOBJ (src/fusion/mesh_interleaved.cpp:19-137)
+        for a mesh without normals and textures: filename + ".obj", vertices
+        with their colors as floats in [0, 1], 1-based faces."""
+        v, c, f = self.vertices, self.colors, self.faces
+        lines = ["", "#Written from Volume"]
+        cf = c.astype(np.float32) / np.float32(255.0) if len(c) else None
+        for i in range(len(v)):
+            line = f"v {_g(v[i][0])} {_g(v[i][1])} {_g(v[i][2])}"
+            if cf is not None:
+                line += f" {_g(cf[i][0])} {_g(cf[i][1])} {_g(cf[i][2])}"
+            lines.append(line)
+        for t in f:
+            lines.append(f"f {int(t[0]) + 1} {int(t[1]) + 1} {int(t[2]) + 1}")
+        with open(filename + ".obj", "wb") as out:
+            out.write("\n".join(lines).encode())
+
+
+def _g(x):
+    """A float as C++ streams print it by default (printf %g, 6 digits)."""
+    return "%g" % float(x)
 
 
 class FusionMipMapGPU:
@@ -308,6 +351,38 @@ class FusionMipMapGPU:
                 for i, n in enumerate(names)}
 
     # -- learned confidence (no counterpart in the reference)
+    def getDescription(self):
+        """FusionMipMapCPU::getDescription (geometryfusion_mipmap_cpu.cpp:1128-1133):
+        offset, scale and side as the map's file stem."""
+        buf = ctypes.create_string_buffer(300)
+        self._check(self._lib.ff_map_description(self._ctx, buf, 300))
+        return buf.value.decode()
+
+    def saveMap(self, path):
+        """The map in the reference's raw format (copyCPU2HDD) at `path`."""
+        self._check(self._lib.ff_save_map(self._ctx, os.fsencode(path)))
+
+    def loadMap(self, path, offset, n):
+        """Restores a map written in the reference's raw format into this
+        context (same scale and color setting), with its volume offset and side."""
+        off = (ctypes.c_float * 3)(*[float(x) for x in offset])
+        self._check(self._lib.ff_load_map(self._ctx, os.fsencode(path), off, int(n)))
+
+    def copyCPU2HDD(self, directory="resource"):
+        """copyCPU2HDD (geometryfusion_mipmap_cpu.cpp:1135-1157): writes
+        <directory>/<getDescription()>.dat; returns its path."""
+        path = os.path.join(directory, self.getDescription() + ".dat")
+        self.saveMap(path)
+        return path
+
+    def copyHDD2CPU(self, path):
+        """copyHDD2CPU (geometryfusion_mipmap_cpu.cpp:1160-1184) for a file
+        named by getDescription(): the volume is read from the name."""
+        stem = os.path.splitext(os.path.basename(path))[0].split("_")
+        if len(stem) != 5:
+            raise ValueError(f"{path}: expected <x>_<y>_<z>_<scale>_<n>.dat")
+        self.loadMap(path, [float(stem[0]), float(stem[1]), float(stem[2])], int(stem[4]))
+
     def setLearnedWeighting(self, params):
         """Turns learned per-pixel confidence on (params: MLP_NPARAMS floats)
         or off (None: the plain method)."""

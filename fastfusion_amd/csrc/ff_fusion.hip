@@ -1327,6 +1327,14 @@ void drop_frame_graphs(ff_ctx *c)
 }
 
 // writes the host's used counts to the device copy (after a host-side change)
+// tree slots [from, capacity) back to empty
+void fill_tree_tail(ff_ctx *c, uint32_t from)
+{
+    const uint32_t cnt = c->nBranchesTotal - from;
+    k_fill_u32<<<(cnt + 255) / 256, 256, 0, c->stream>>>(c->d_tree + from, FF_BRANCHINIT, cnt);
+    hipStreamSynchronize(c->stream);
+}
+
 void sync_map_counters(ff_ctx *c)
 {
     c->h_mapc->nBranchesUsed = c->nBranchesUsed;

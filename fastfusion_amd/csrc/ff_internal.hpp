@@ -206,6 +206,8 @@ void batch_free(ff_ctx *c);
 int mesh_init(ff_ctx *c);
 void mesh_free(ff_ctx *c);
 void mesh_on_initial_volume(ff_ctx *c);
+int mesh_on_load(ff_ctx *c, const std::vector<ushort4> &binfo);
+void fill_tree_tail(ff_ctx *c, uint32_t from);
 void mesh_on_grow(ff_ctx *c);
 void mesh_after_frame(ff_ctx *c);
 // the mesher's branch-geometry table (origin and half size per branch

@@ -1460,6 +1460,22 @@ void mesh_on_initial_volume(ff_ctx *c)
     s->haveVolume = true;
 }
 
+// a restored map: its branch geometry, and no previous cell table (every
+// cell is meshed by the next update)
+int mesh_on_load(ff_ctx *c, const std::vector<ushort4> &binfo)
+{
+    MeshState *s = c->mesh;
+    const size_t nb = std::min<size_t>(binfo.size(), s->binfoCap);
+    if (nb) MCHK(hipMemcpyAsync(s->d_binfo, binfo.data(), sizeof(ushort4) * nb, hipMemcpyHostToDevice, c->stream));
+    MCHK(hipMemsetAsync(s->d_hkeys, 0xff, sizeof(uint64_t) * (s->hmask + 1), c->stream));
+    MCHK(hipStreamSynchronize(c->stream));
+    s->nCells[0] = s->nCells[1] = 0;
+    s->cur = 0;
+    s->lastMeshStamp = 0;
+    s->haveVolume = true;
+    return FF_OK;
+}
+
 // grow: branch blocks moved by addBlocks, origins shifted; the new root
 // chain's node origins and octant sizes are given by the caller
 void mesh_on_grow_blocks(ff_ctx *c, uint32_t oldBlocks, uint32_t addBlocks, const uint32_t addv[3],

@@ -138,6 +138,22 @@ int ff_reset_timing(ff_ctx *ctx);
  * time or order work around it. */
 void *ff_stream(ff_ctx *ctx);
 
+/* ---- Map dump and restore ----------------------------------------------
+ * FusionMipMapCPU::getDescription / copyCPU2HDD / copyHDD2CPU
+ * (geometryfusion_mipmap_cpu.cpp:1128-1184): the reference's raw format
+ * (counts, tree, brick positions and scales, distances, weights, colors as
+ * R, G, B per voxel), followed by a block the reference does not write
+ * ("FFPARENT", count, the bricks' parents).  The reference names the file
+ * resource/<description>.dat and restores into an object whose volume
+ * already matches; ff_load_map takes the volume (offset, side n) instead
+ * and needs a context created with the map's scale and color setting and
+ * large enough pools.  Without the parents block, a brick's parent is the
+ * deepest brick above it in the tree.  After a restore, the next
+ * ff_update_meshes meshes every cell. */
+int ff_map_description(const ff_ctx *ctx, char *buf, size_t n);
+int ff_save_map(ff_ctx *ctx, const char *path);
+int ff_load_map(ff_ctx *ctx, const char *path, const float offset[3], uint32_t n);
+
 /* ---- Learned per-pixel confidence -------------------------------------
  * No counterpart in the reference, whose per-sample weight is the fixed
  * linear fall-off of getDistanceWeight_AoS_variables_h

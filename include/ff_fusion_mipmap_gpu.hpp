@@ -16,6 +16,7 @@
 // carries CameraInfo's rotation, translation (camera -> world) and intrinsics.
 #pragma once
 #include <cstdint>
+#include <fstream>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -46,6 +47,58 @@ struct Mesh {                // MeshInterleaved (src/fusion/mesh_interleaved.hpp
     std::vector<float> vertices;   // 3 per vertex
     std::vector<uint8_t> colors;   // R, G, B per vertex
     std::vector<uint32_t> faces;   // 3 per triangle
+
+    // MeshInterleaved::writePLY (src/fusion/mesh_interleaved.cpp:139-200):
+    // filename + ".ply", the reference's header lines
+    bool writePLY(const std::string &filename, bool binary = true) const
+    {
+        std::ofstream file((filename + ".ply").c_str(), std::ios::out | std::ios::binary);
+        if (!file.is_open()) return false;
+        const size_t nv = vertices.size() / 3, nf = faces.size() / 3;
+        file << "ply" << (binary ? "\nformat binary_little_endian 1.0" : "\nformat ascii 1.0");
+        file << "\nelement vertex " << nv << "\nproperty float32 x\nproperty float32 y\nproperty float32 z";
+        if (!colors.empty()) file << "\nproperty uchar red\nproperty uchar green\nproperty uchar blue";
+        if (nf) file << "\nelement face " << nf << "\nproperty list uint8 int32 vertex_indices";
+        file << "\nend_header";
+        if (binary) file << "\n";
+        for (size_t i = 0; i < nv; i++) {
+            if (binary) file.write(reinterpret_cast<const char *>(&vertices[3 * i]), 3 * sizeof(float));
+            else file << "\n" << vertices[3 * i] << " " << vertices[3 * i + 1] << " " << vertices[3 * i + 2];
+            if (!colors.empty()) {
+                if (binary) file.write(reinterpret_cast<const char *>(&colors[3 * i]), 3);
+                else file << " " << (int)colors[3 * i] << " " << (int)colors[3 * i + 1] << " " << (int)colors[3 * i + 2];
+            }
+        }
+        const unsigned char three = 3;
+        for (size_t i = 0; i < nf; i++) {
+            if (binary) {
+                file.write(reinterpret_cast<const char *>(&three), 1);
+                file.write(reinterpret_cast<const char *>(&faces[3 * i]), 3 * sizeof(uint32_t));
+            } else {
+                file << "\n" << 3 << " " << faces[3 * i] << " " << faces[3 * i + 1] << " " << faces[3 * i + 2];
+            }
+        }
+        return bool(file);
+    }
+
+    // MeshInterleaved::writeOBJ (src/fusion/mesh_interleaved.cpp:19-137) for a
+    // mesh without normals or textures: filename + ".obj"
+    bool writeOBJ(const std::string &filename) const
+    {
+        std::ofstream file((filename + ".obj").c_str(), std::ios::out | std::ios::binary);
+        if (!file.is_open()) return false;
+        file << "\n#Written from Volume";
+        const size_t nv = vertices.size() / 3, nf = faces.size() / 3;
+        for (size_t i = 0; i < nv; i++) {
+            file << "\nv " << vertices[3 * i] << " " << vertices[3 * i + 1] << " " << vertices[3 * i + 2];
+            if (!colors.empty())
+                file << " " << (float)colors[3 * i] / 255.0f << " " << (float)colors[3 * i + 1] / 255.0f << " "
+                     << (float)colors[3 * i + 2] / 255.0f;
+        }
+        for (size_t i = 0; i < nf; i++)
+            file << "\nf " << faces[3 * i] + 1 << " " << faces[3 * i + 1] + 1 << " " << faces[3 * i + 2] + 1;
+        return bool(file);
+    }
 };
 
 class FusionMipMapGPU {
@@ -100,6 +153,22 @@ public:
     }
 
     void setIncrementalMeshing(bool on) { check(ff_set_incremental_meshing(ctx_, on ? 1 : 0)); }
+
+    // std::string getDescription(), bool copyCPU2HDD(), bool copyHDD2CPU()
+    // (geometryfusion_mipmap_cpu.cpp:1128-1184): resource/<description>.dat
+    std::string getDescription()
+    {
+        char buf[300];
+        check(ff_map_description(ctx_, buf, sizeof buf));
+        return buf;
+    }
+    bool copyCPU2HDD() { return ff_save_map(ctx_, ("resource/" + getDescription() + ".dat").c_str()) == FF_OK; }
+    // the reference restores into an object whose volume already matches
+    // the file: here the volume is given
+    bool copyHDD2CPU(const std::string &path, const float offset[3], uint32_t n)
+    {
+        return ff_load_map(ctx_, path.c_str(), offset, n) == FF_OK;
+    }
     void setThreadMeshing(bool) {}  // meshing is a device pass; there is no meshing thread
     void setDepthChecks(int) {}     // the reference's depth checks are commented out (:1627-1700)
 
