@@ -70,6 +70,11 @@ ABI = {
     "ff_set_timing": (ctypes.c_int, [_P, ctypes.c_int]),
     "ff_get_timing": (ctypes.c_int, [_P, _P, _P, _P, _P]),
     "ff_reset_timing": (ctypes.c_int, [_P]),
+    "ff_tum_read": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, _P, _SZ]),
+    "ff_png_size": (ctypes.c_int, [ctypes.c_char_p, _P, _P, _P, _P]),
+    "ff_png_read_depth": (ctypes.c_int, [ctypes.c_char_p, _P, _SZ]),
+    "ff_png_read_bgr": (ctypes.c_int, [ctypes.c_char_p, _P, _SZ]),
+    "ff_input_last_error": (ctypes.c_char_p, []),
     "ff_map_description": (ctypes.c_int, [_P, ctypes.c_char_p, _SZ]),
     "ff_save_map": (ctypes.c_int, [_P, ctypes.c_char_p]),
     "ff_load_map": (ctypes.c_int, [_P, ctypes.c_char_p, ctypes.POINTER(ctypes.c_float), ctypes.c_uint32]),

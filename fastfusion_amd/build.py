@@ -20,6 +20,7 @@ HIP_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
 ]
 HIP_SOURCES = ["ff_fusion.hip", "ff_mesh.hip", "ff_learned.hip", "ff_batch.hip", "ff_mapio.hip"]
+HOST_SOURCES = ["ff_input.cpp"]  # host-only parts of libffgpu.so (zlib for PNG)
 
 
 def _run(cmd):
@@ -34,8 +35,13 @@ def build_gpu():
         obj = os.path.join(LIB, src.replace(".hip", ".o"))
         _run(["hipcc"] + HIP_FLAGS + ["-c", os.path.join(CSRC, src), "-o", obj])
         objs.append(obj)
+    for src in HOST_SOURCES:
+        obj = os.path.join(LIB, src.replace(".cpp", ".o"))
+        _run(["g++", "-O2", "-std=c++17", "-fPIC", "-I", os.path.join(os.path.dirname(HERE), "include"), "-c",
+              os.path.join(CSRC, src), "-o", obj])
+        objs.append(obj)
     out = os.path.join(LIB, "libffgpu.so")
-    _run(["hipcc", "--offload-arch=gfx950", "-shared", "-fPIC", "-o", out] + objs)
+    _run(["hipcc", "--offload-arch=gfx950", "-shared", "-fPIC", "-o", out] + objs + ["-lz"])
     return out
 
 

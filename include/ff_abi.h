@@ -154,6 +154,29 @@ int ff_map_description(const ff_ctx *ctx, char *buf, size_t n);
 int ff_save_map(ff_ctx *ctx, const char *path);
 int ff_load_map(ff_ctx *ctx, const char *path, const float offset[3], uint32_t n);
 
+/* ---- Input side: TUM association files and PNG images ------------------
+ * No member of FusionMipMapCPU: the reference's viewer reads them
+ * (onlinefusionviewer_main.cpp:266-346, onlinefusionviewer.cpp:934-935).
+ * Host functions; they need no GPU and no context. */
+typedef struct ff_tum_frame {
+    double R[9], t[3]; /* camera -> world, composed across files like the reference */
+    char depth[512];   /* association file's directory + the listed depth image */
+    char rgb[512];     /* the same for the color image ("" without color) */
+} ff_tum_frame;
+/* Lines "stamp tx ty tz qx qy qz qw stamp depth [stamp rgb]" of n_files
+ * association files, each file's poses composed with the last pose of the
+ * previous file.  Writes at most cap frames; returns the number of frames
+ * (possibly above cap), or a negative error. */
+int ff_tum_read(const char *const *assoc_paths, int n_files, int use_color, ff_tum_frame *out, size_t cap);
+/* PNG images as cv::imread reads them: ff_png_read_depth the samples of a
+ * single-channel 8- or 16-bit image unchanged (IMREAD_UNCHANGED);
+ * ff_png_read_bgr any 8/16-bit gray, RGB, gray+alpha, RGBA or palette image
+ * as 8-bit B, G, R (IMREAD_COLOR).  Non-interlaced images only. */
+int ff_png_size(const char *path, int *width, int *height, int *channels, int *bitdepth);
+int ff_png_read_depth(const char *path, uint16_t *out, size_t n);
+int ff_png_read_bgr(const char *path, uint8_t *out, size_t n);
+const char *ff_input_last_error(void);
+
 /* ---- Learned per-pixel confidence -------------------------------------
  * No counterpart in the reference, whose per-sample weight is the fixed
  * linear fall-off of getDistanceWeight_AoS_variables_h
