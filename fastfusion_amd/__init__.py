@@ -78,6 +78,19 @@ ABI = {
     "ff_map_description": (ctypes.c_int, [_P, ctypes.c_char_p, _SZ]),
     "ff_save_map": (ctypes.c_int, [_P, ctypes.c_char_p]),
     "ff_load_map": (ctypes.c_int, [_P, ctypes.c_char_p, ctypes.POINTER(ctypes.c_float), ctypes.c_uint32]),
+    "ff_loop_create": (_P, [_P, ctypes.c_uint32]),
+    "ff_loop_destroy": (None, [_P]),
+    "ff_loop_last_error": (ctypes.c_char_p, [_P]),
+    "ff_loop_reset": (ctypes.c_int, [_P]),
+    "ff_loop_delete_frame": (ctypes.c_int, [_P, _P, _P, _P, _P, ctypes.c_int, ctypes.c_int, _DP, _DP, _DP,
+                                            _P, _SZ, ctypes.POINTER(_SZ)]),
+    "ff_loop_delete_frame_device": (ctypes.c_int, [_P, _P, _P, _P, _P, ctypes.c_int, ctypes.c_int, _DP, _DP,
+                                                   _DP, _P, _SZ, ctypes.POINTER(_SZ)]),
+    "ff_loop_mapped": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_uint32)]),
+    "ff_loop_copy_map": (ctypes.c_int, [_P, _P, _SZ]),
+    "ff_loop_copy_bricks": (ctypes.c_int, [_P, _P, _SZ, _P, _P, _P]),
+    "ff_loop_set_timing": (ctypes.c_int, [_P, ctypes.c_int]),
+    "ff_loop_get_timing": (ctypes.c_int, [_P, _P, _P, _P, _P]),
     "ff_set_learned_weighting": (ctypes.c_int, [_P, _P, _SZ]),
     "ff_copy_confidence": (ctypes.c_int, [_P, _P, _SZ]),
     "ff_trainer_create": (_P, [ctypes.c_int, _SZ, _P, _SZ, ctypes.c_uint64]),

@@ -19,7 +19,7 @@ HIP_FLAGS = [
     # packed FP32 VALU ops stay off (plain VALU ops only)
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
 ]
-HIP_SOURCES = ["ff_fusion.hip", "ff_mesh.hip", "ff_learned.hip", "ff_batch.hip", "ff_mapio.hip"]
+HIP_SOURCES = ["ff_fusion.hip", "ff_mesh.hip", "ff_learned.hip", "ff_batch.hip", "ff_mapio.hip", "ff_loop.hip"]
 HOST_SOURCES = ["ff_input.cpp"]  # host-only parts of libffgpu.so (zlib for PNG)
 
 

@@ -20,6 +20,7 @@
 //                with the reference's round-toward-zero + RCPPS arithmetic
 #include "ff_device.hpp"
 #include "ff_internal.hpp"
+#include "ff_sdf.hpp"
 #include "rcp_table.h"
 
 #include <algorithm>
@@ -919,44 +920,6 @@ __global__ void k_shift_leaves(ushort4 *info, uint32_t n, uint32_t ax, uint32_t 
 // rebuilds its row origin with the same sequential adds as the reference
 // (z steps, then y steps) and adds k * d1j like the AVX lane k.
 // ---------------------------------------------------------------------------
-// The per-row part of update8AddLoopAVXSingleInteger: the row origin of
-// brick row (zr, yr) accumulated by repeated adds in the reference's order,
-// the column step d1, the truncations and the fall-off reciprocal.
-struct SdfRow {
-    float R[3], d1[3], td, tw, rfall;
-};
-
-__device__ __forceinline__ SdfRow sdf_row_setup(const ushort4 info, int zr, int yr, const SdfParams &P,
-                                                const uint32_t *tab)
-{
-    const float EPS = 0.005f;
-    SdfRow w;
-    const float ls = (float)info.w;
-    const float fls = rz_mul(ls, P.scale);
-    const float ofx = (float)info.x, ofy = (float)info.y, ofz = (float)info.z;
-    float d2[3], d3[3];
-#pragma unroll
-    for (int r = 0; r < 3; r++) {
-        w.R[r] = rz_add(rz_mul(rz_add(rz_add(rz_mul(P.m[4 * r], ofx), rz_mul(P.m[4 * r + 1], ofy)),
-                                      rz_mul(P.m[4 * r + 2], ofz)),
-                               P.scale),
-                        P.m[4 * r + 3]);
-        w.d1[r] = rz_mul(P.m[4 * r], fls);
-        d2[r] = rz_mul(P.m[4 * r + 1], fls);
-        d3[r] = rz_mul(P.m[4 * r + 2], fls);
-    }
-    w.td = rz_mul(P.dT, ls);
-    w.tw = rz_mul(rz_mul(3.0f, P.dT), ls);
-    for (int i = 0; i < zr; i++) {
-        w.R[0] = rz_add(w.R[0], d3[0]); w.R[1] = rz_add(w.R[1], d3[1]); w.R[2] = rz_add(w.R[2], d3[2]);
-    }
-    for (int i = 0; i < yr; i++) {
-        w.R[0] = rz_add(w.R[0], d2[0]); w.R[1] = rz_add(w.R[1], d2[1]); w.R[2] = rz_add(w.R[2], d2[2]);
-    }
-    w.rfall = rcp_x86(rz_sub(w.tw, EPS), tab);
-    return w;
-}
-
 // one voxel (column k of its row): old (dA, wA, cA) -> new (dN, wN, cN)
 __device__ __forceinline__ void sdf_point(const SdfRow &w, int k, float dA, float wA, const uint16_t cA[3],
                                           const uint16_t *__restrict__ depth, const uint8_t *__restrict__ bgr,
@@ -1275,6 +1238,11 @@ static CamF make_pinv(const ff_ctx *c, const double R[9], const double t[3], con
         q.t[i] = (float)(((Rt[i * 3 + 0] * o0 + Rt[i * 3 + 1] * o1) + Rt[i * 3 + 2] * o2) + tt[i]);
     q.fx = (float)K[0]; q.fy = (float)K[1]; q.cx = (float)K[2]; q.cy = (float)K[3];
     return q;
+}
+
+CamF camera_inverse(const ff_ctx *c, const double R[9], const double t[3], const double K[4])
+{
+    return make_pinv(c, R, t, K);
 }
 
 int ensure_image(ff_ctx *c, int W, int H)

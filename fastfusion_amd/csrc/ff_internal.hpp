@@ -195,6 +195,9 @@ void launch_sdf(ff_ctx *c, int W, int H);
 void launch_empty(ff_ctx *c, int W, int H);
 int frame_finish(ff_ctx *c, int W, int H, bool timed);
 int ensure_image(ff_ctx *c, int W, int H);
+// addMap's pInv: voxel -> camera with the map offset folded in
+// (geometryfusion_mipmap_cpu.cpp:1771-1793)
+CamF camera_inverse(const ff_ctx *c, const double R[9], const double t[3], const double K[4]);
 // batched SDF update over the concatenated brick lists of n streams
 // (ff_fusion.hip): per-stream gates and counts into prefix[0..n], then one
 // launch; threads per brick tpb (512 or 256), bricks per workgroup bpw

@@ -154,6 +154,57 @@ int ff_map_description(const ff_ctx *ctx, char *buf, size_t n);
 int ff_save_map(ff_ctx *ctx, const char *path);
 int ff_load_map(ff_ctx *ctx, const char *path, const float offset[3], uint32_t n);
 
+/* ---- Loop-closure de-integration ---------------------------------------
+ * The reference's loop-closure partner map (the FusionMipMapCPU(
+ * FusionMipMapCPU*, nLeavesTotalLoop) constructor,
+ * geometryfusion_mipmap_cpu.cpp:645-717): a dense store of `max_leaves` slots
+ * (0: MAX_LOOP_CLOSURE_LEAVES = 100000, definitions.h:124), each with 512
+ * float distances, 512 float weights (_weightsLoop) and, when the map fuses
+ * color, 512 colors; _leafMapLoop gives each brick of the map its slot.
+ * The partner reads the brick positions, scales and volume offset of `ctx`
+ * as they are at each call (the reference copies them into the partner,
+ * :3912-3932); `ctx` must outlive it.  Colors are copied as R, G, B planes
+ * of 512 per slot. */
+typedef struct ff_loop ff_loop;
+ff_loop *ff_loop_create(ff_ctx *ctx, uint32_t max_leaves);
+void ff_loop_destroy(ff_loop *lp);
+const char *ff_loop_last_error(const ff_loop *lp);
+/* updateArraysForLoopClosure's reset (geometryfusion_mipmap_cpu.cpp:3927-3931):
+ * no slot handed out, every slot cleared. */
+int ff_loop_reset(ff_loop *lp);
+/* One frame of deleteOldLoopPoses (geometryfusion_mipmap_cpu.cpp:3990-4019):
+ * updateWrapper_deleteloop (loopclosure.cpp:232-311) over the frame's
+ * recorded brick queue with update8AddLoopSSESingleDeleteloop
+ * (loopclosure.cpp:15-230).  depth: W x H floats in metres, NaN where there
+ * is no measurement (IMAGEINFINITE); red, green, blue: W x H byte planes
+ * (required when the map fuses color, else ignored); R, t: CameraInfo
+ * camera -> world; K = {fx, fy, cx, cy}; queue: brick indices of the map,
+ * in the order they are handed slots.  *applied (may be NULL) gets the
+ * number of queue entries applied.  Returns FF_OK, or FF_E_POOL_FULL when a
+ * brick found no free slot: the entries before it were applied and the rest
+ * not (the reference returns false there); FF_E_ARG (nothing applied) when
+ * an entry names a brick beyond the map's used bricks. */
+int ff_loop_delete_frame(ff_loop *lp, const float *depth, const uint8_t *red, const uint8_t *green,
+                         const uint8_t *blue, int width, int height, const double R[9], const double t[3],
+                         const double K[4], const uint32_t *queue, size_t n_queue, size_t *applied);
+/* Same, with every array already in device memory (dense rows). */
+int ff_loop_delete_frame_device(ff_loop *lp, const float *depth_dev, const uint8_t *red_dev,
+                                const uint8_t *green_dev, const uint8_t *blue_dev, int width, int height,
+                                const double R[9], const double t[3], const double K[4],
+                                const uint32_t *queue_dev, size_t n_queue, size_t *applied);
+/* _nLeavesMapped and the slot capacity. */
+int ff_loop_mapped(ff_loop *lp, uint32_t *n_mapped, uint32_t *capacity);
+/* _leafMapLoop for bricks [0, n): the slot, or 4294967294 (BRANCHINIT). */
+int ff_loop_copy_map(ff_loop *lp, uint32_t *out, size_t n);
+/* The partner's _distance, _weightsLoop and _color of the given slots. */
+int ff_loop_copy_bricks(ff_loop *lp, const uint32_t *slots, size_t n, float *dist, float *weight,
+                        uint16_t *color);
+/* GPU timing of the de-integration kernel (no counterpart in the
+ * reference): summed milliseconds, calls, algorithmic bytes and applied
+ * queue entries. */
+int ff_loop_set_timing(ff_loop *lp, int on);
+int ff_loop_get_timing(ff_loop *lp, double *ms, uint64_t *calls, double *bytes, uint64_t *units);
+
 /* ---- Input side: TUM association files and PNG images ------------------
  * No member of FusionMipMapCPU: the reference's viewer reads them
  * (onlinefusionviewer_main.cpp:266-346, onlinefusionviewer.cpp:934-935).

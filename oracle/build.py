@@ -18,7 +18,7 @@ CFLAGS = ["-O2", "-fPIC", "-shared", "-std=c11", "-frounding-math", "-ffp-contra
 def build():
     os.makedirs(LIB, exist_ok=True)
     out = os.path.join(LIB, "libfforacle.so")
-    srcs = [os.path.join(HERE, f) for f in ("ff_oracle.c", "ff_oracle_mesh.c") if os.path.exists(os.path.join(HERE, f))]
+    srcs = [os.path.join(HERE, f) for f in ("ff_oracle.c", "ff_oracle_mesh.c", "ff_oracle_loop.c") if os.path.exists(os.path.join(HERE, f))]
     cmd = ["gcc"] + CFLAGS + srcs + ["-o", out, "-lm"]
     print("+", " ".join(cmd), flush=True)
     subprocess.check_call(cmd)

@@ -135,6 +135,17 @@ static campams make_pinv(const ffo_ctx *c, const double R[9], const double t[3],
     return q;
 }
 
+void ffo_camera_inverse(const ffo_ctx *c, const double R[9], const double t[3], const double K[4],
+                        float m[12], float Kf[4])
+{
+    campams q = make_pinv(c, R, t, K);
+    for (int r = 0; r < 3; r++) {
+        m[4 * r] = q.r[3 * r]; m[4 * r + 1] = q.r[3 * r + 1]; m[4 * r + 2] = q.r[3 * r + 2];
+        m[4 * r + 3] = q.t[r];
+    }
+    Kf[0] = q.fx; Kf[1] = q.fy; Kf[2] = q.cx; Kf[3] = q.cy;
+}
+
 /* computeBoundingboxIntCPU, u16 overload, geometryfusion_mipmap_cpu.cpp:1270-1303 */
 static void bbox_first(const campams *p, const uint16_t *depth, float scaling, float maxcam,
                        int w, int h, int bw, int mn[3], int mx[3])
@@ -536,3 +547,4 @@ const uint8_t *ffo_processed(const ffo_ctx *c) { return c->processed; }
 int ffo_grow_count(const ffo_ctx *c) { return c->grows; }
 uint32_t ffo_leaves_total(const ffo_ctx *c) { return c->nLeavesTotal; }
 float ffo_scale(const ffo_ctx *c) { return c->scale; }
+float ffo_threshold(const ffo_ctx *c) { return c->dT; }

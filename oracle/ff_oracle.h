@@ -87,6 +87,36 @@ void ffo_update_brick(const uint16_t *depth, float scaling, float maxcam, const 
                       float distance_threshold, const uint16_t pos[3], uint16_t leaf_scale,
                       float *dist, float *weight, uint16_t *color);
 
+float ffo_threshold(const ffo_ctx *c);
+/* addMap's pInv (geometryfusion_mipmap_cpu.cpp:1771-1793) as 12 floats
+ * (row r: R'_r0, R'_r1, R'_r2, t''_r) and the intrinsics fx, fy, cx, cy. */
+void ffo_camera_inverse(const ffo_ctx *c, const double R[9], const double t[3], const double K[4],
+                        float m[12], float Kf[4]);
+
+/* Loop-closure de-integration (ff_oracle_loop.c, src/fusion/loopclosure.cpp):
+ * a partner map of `total` slots for bricks numbered below map_n. */
+typedef struct ffo_loop ffo_loop;
+ffo_loop *ffo_loop_create(uint32_t map_n, uint32_t total, int use_color);
+void ffo_loop_destroy(ffo_loop *L);
+void ffo_loop_reset(ffo_loop *L);
+/* Subtracts one frame (depth in metres, NaN = no measurement; color planes or
+ * NULL) from the queued bricks of the main map c.  Returns 1, or 0 when the
+ * slots ran out; *applied = queue entries updated. */
+int ffo_loop_delete(ffo_loop *L, const ffo_ctx *c, const float *depth, const uint8_t *red,
+                    const uint8_t *green, const uint8_t *blue, int w, int h, const double R[9],
+                    const double t[3], const double K[4], const uint32_t *queue, uint32_t n,
+                    uint32_t *applied);
+/* One brick through the de-integration kernel, in place (color: R, G, B per voxel). */
+void ffo_delete_brick(const float *depth, const uint8_t *red, const uint8_t *green, const uint8_t *blue,
+                      int w, int h, const float m[12], const float K[4], float scale,
+                      float distance_threshold, const uint16_t pos[3], uint16_t leaf_scale,
+                      float *dist, float *weight, uint16_t *color);
+uint32_t ffo_loop_mapped(const ffo_loop *L);
+const uint32_t *ffo_loop_map(const ffo_loop *L);
+const float *ffo_loop_distance(const ffo_loop *L);
+const float *ffo_loop_weights(const ffo_loop *L);
+const uint16_t *ffo_loop_color(const ffo_loop *L);
+
 #ifdef __cplusplus
 }
 #endif

@@ -46,6 +46,7 @@ def load():
                                      ctypes.c_float, ctypes.c_float, P, ctypes.c_uint16, P, P, P]
     lib.ffo_set_confidence.argtypes = [P, P]
     _mesh_sigs(lib)
+    _loop_sigs(lib)
     _lib = lib
     return lib
 
@@ -61,6 +62,27 @@ def _mesh_sigs(lib):
     lib.ffo_mesh_update.argtypes = [P]
     lib.ffo_mesh_update_faces.restype = ctypes.c_size_t
     lib.ffo_mesh_update_faces.argtypes = []
+
+
+def _loop_sigs(lib):
+    P, U32, I = ctypes.c_void_p, ctypes.c_uint32, ctypes.c_int
+    D = ctypes.POINTER(ctypes.c_double)
+    lib.ffo_loop_create.restype = P
+    lib.ffo_loop_create.argtypes = [U32, U32, I]
+    lib.ffo_loop_destroy.argtypes = [P]
+    lib.ffo_loop_reset.argtypes = [P]
+    lib.ffo_loop_delete.restype = I
+    lib.ffo_loop_delete.argtypes = [P, P, P, P, P, P, I, I, D, D, D, P, U32, P]
+    lib.ffo_delete_brick.argtypes = [P, P, P, P, I, I, P, P, ctypes.c_float, ctypes.c_float, P, ctypes.c_uint16,
+                                     P, P, P]
+    lib.ffo_camera_inverse.argtypes = [P, D, D, D, P, P]
+    lib.ffo_loop_mapped.restype = U32
+    lib.ffo_loop_mapped.argtypes = [P]
+    for name in ("ffo_loop_map", "ffo_loop_distance", "ffo_loop_weights", "ffo_loop_color"):
+        getattr(lib, name).restype = P
+        getattr(lib, name).argtypes = [P]
+    lib.ffo_leaves_total.restype = U32
+    lib.ffo_leaves_total.argtypes = [P]
 
 
 def _vp(a):
@@ -191,3 +213,74 @@ def update_brick(depth, bgr, scaling, maxcam, m12, K, scale, threshold, pos, lea
     load().ffo_update_brick(_vp(depth), float(scaling), float(maxcam), _vp(bgr), w, h, _vp(m), _vp(Kf),
                             float(scale), float(threshold), _vp(p), int(leaf_scale), _vp(dist), _vp(weight),
                             _vp(color))
+
+
+class LoopOracle:
+    """The loop-closure partner map (ff_oracle_loop.c): `total` slots for the
+    bricks of the oracle map `main`, handed out in queue order."""
+
+    def __init__(self, main, total):
+        self.lib = load()
+        self.main = main
+        self.map_n = self.lib.ffo_leaves_total(main.h)
+        self.total = int(total)
+        self.h = self.lib.ffo_loop_create(self.map_n, self.total, int(bool(main.color)))
+        self.color = main.color
+
+    def __del__(self):
+        try:
+            self.lib.ffo_loop_destroy(self.h)
+        except Exception:
+            pass
+
+    def reset(self):
+        self.lib.ffo_loop_reset(self.h)
+
+    def delete_frame(self, depth_m, rgb_planes, R, t, K, queue):
+        """depth_m: H x W float32 metres (NaN = no measurement); rgb_planes:
+        (red, green, blue) H x W uint8 or None.  Returns (ok, applied)."""
+        d = np.ascontiguousarray(depth_m, np.float32)
+        h, w = d.shape
+        pl = [np.ascontiguousarray(p, np.uint8) for p in rgb_planes] if rgb_planes is not None else [None] * 3
+        q = np.ascontiguousarray(queue, np.uint32)
+        applied = np.zeros(1, np.uint32)
+        ok = self.lib.ffo_loop_delete(self.h, self.main.h, _vp(d), _vp(pl[0]), _vp(pl[1]), _vp(pl[2]), w, h,
+                                      _dp(R), _dp(t), _dp(K), _vp(q), len(q), _vp(applied))
+        return bool(ok), int(applied[0])
+
+    def mapped(self):
+        return self.lib.ffo_loop_mapped(self.h)
+
+    def slot_map(self):
+        return _arr(self.lib.ffo_loop_map(self.h), np.uint32, self.map_n)
+
+    def bricks(self, slots):
+        n = self.total
+        idx = np.asarray(slots, np.int64)
+        d = _arr(self.lib.ffo_loop_distance(self.h), np.float32, 512 * n).reshape(n, 512)[idx]
+        w = _arr(self.lib.ffo_loop_weights(self.h), np.float32, 512 * n).reshape(n, 512)[idx]
+        c = None
+        if self.color:
+            c = _arr(self.lib.ffo_loop_color(self.h), np.uint16, 3 * 512 * n).reshape(n, 512, 3)
+            c = c[idx].transpose(0, 2, 1)
+        return d, w, c
+
+
+def camera_inverse(main, R, t, K):
+    """addMap's pInv for the oracle map `main`: (m (12,), K (4,)) float32."""
+    m = np.zeros(12, np.float32)
+    Kf = np.zeros(4, np.float32)
+    load().ffo_camera_inverse(main.h, _dp(R), _dp(t), _dp(K), _vp(m), _vp(Kf))
+    return m, Kf
+
+
+def delete_brick(depth_m, planes, m12, K, scale, threshold, pos, leaf_scale, dist, weight, color):
+    """One brick through the oracle's de-integration kernel, in place; color
+    is (512, 3) R, G, B per voxel or None."""
+    d = np.ascontiguousarray(depth_m, np.float32)
+    h, w = d.shape
+    pl = [np.ascontiguousarray(p, np.uint8) for p in planes] if planes is not None else [None] * 3
+    load().ffo_delete_brick(_vp(d), _vp(pl[0]), _vp(pl[1]), _vp(pl[2]), w, h,
+                            _vp(np.ascontiguousarray(m12, np.float32)), _vp(np.ascontiguousarray(K, np.float32)),
+                            float(scale), float(threshold), _vp(np.ascontiguousarray(pos, np.uint16)),
+                            int(leaf_scale), _vp(dist), _vp(weight), _vp(color) if color is not None else None)

@@ -1,0 +1,250 @@
+"""CPU checks that pin the oracle's loop-closure de-integration
+(oracle/ff_oracle_loop.c; update8AddLoopSSESingleDeleteloop and
+updateWrapper_deleteloop, loopclosure.cpp:15-311) without running the
+reference (its translation unit needs OpenCV and Boost headers this image does
+not have): a fronto-parallel plane in closed form, the exact relation to the
+integration kernel (pinned in test_cpu_oracle.py) on exactly representable
+geometry, the cancellation of the fall-off weight when a frame is added and
+then subtracted, which is exact only in the SSE kernel's first group of four
+lanes, and the slot hand-out of the wrapper."""
+import numpy as np
+import pytest
+
+from fastfusion_amd import loopclosure, synth
+from oracle import pyoracle
+from oracle.pyoracle import LoopOracle, Oracle
+
+EPS = 0.005
+
+
+def _plane(z, w=64, h=48):
+    return np.full((h, w), z, np.float32)
+
+
+def _planes(h=48, w=64, rgb=(30, 20, 10)):
+    return [np.full((h, w), c, np.uint8) for c in rgb]
+
+
+def _fresh():
+    return np.zeros(512, np.float32), np.zeros(512, np.float32), np.zeros((512, 3), np.uint16)
+
+
+def _geometry(pos, scale, ls=1):
+    idx = np.arange(512)
+    return ((pos[0] + ls * (idx & 7)) * scale, (pos[1] + ls * ((idx >> 3) & 7)) * scale,
+            (pos[2] + ls * (idx >> 6)) * scale)
+
+
+def test_plane_closed_form():
+    """Identity pose, plane at Z: ray SDF |p| (1 - Z / p_z); weight +1 in front
+    (d < 5 mm), -(tw - d) / (tw - 5 mm) on the fall-off (the sign the
+    reference reverses), 0 beyond; distance and color averaged with the
+    weight-sum guard rcp(w + (w <= 0)).  A brick of scale 2 spans front,
+    fall-off and beyond; its thresholds are td = tau * 2, tw = 3 tau * 2."""
+    Z, scale, tau, ls = 1.0, 0.01, 0.01, 2
+    Kc = np.float32([60.0, 60.0, 31.5, 23.5])
+    m = np.float32([1, 0, 0, -0.04, 0, 1, 0, -0.04, 0, 0, 1, 0.0])
+    pos = np.uint16([0, 0, 96])  # z from 0.96 to 1.10 m
+    d, w, c = _fresh()
+    pyoracle.delete_brick(_plane(Z), _planes(), m, Kc, scale, tau, pos, ls, d, w, c)
+    px, py, pz = _geometry(pos, scale, ls)
+    px, py = px - 0.04, py - 0.04
+    L = np.sqrt(px * px + py * py + pz * pz)
+    dI = L * (1.0 - Z / pz)
+    tol = 3.7e-4 * L / pz * Z + 1e-6
+    td, tw = tau * ls, 3 * tau * ls
+    g = (tw - dI) / (tw - EPS)
+    front, ramp, beyond = dI < EPS - 2 * tol, (dI > EPS + 2 * tol) & (dI < tw - 2 * tol), dI > tw + 2 * tol
+    assert front.sum() > 50 and ramp.sum() > 50 and beyond.sum() > 50
+    assert np.all(w[front] == 1.0) and np.all(w[beyond] == 0.0)
+    assert np.all(np.abs(w[ramp] + g[ramp]) <= 2 * tol[ramp] / (tw - EPS) + 1e-3)
+    assert np.all(np.abs(d[front] - np.clip(dI, -td, td)[front]) <= tol[front])
+    want = -np.clip(dI, -td, td) * g / (1 - g)
+    rel = np.abs(d[ramp] - want[ramp]) / np.maximum(np.abs(want[ramp]), 1e-6)
+    assert np.all(rel <= 2e-2)
+    assert np.all(d[beyond] == 0.0)
+    # colors: R, G, B planes times 256, no clamp; negative averages wrap
+    # through the 32-bit truncation to 16 bits
+    assert np.all(np.abs(c[front].astype(int) - [7680, 5120, 2560]) <= 4)
+    ramp_c = (-np.array([30, 20, 10]) * 256.0 * (g / (1 - g))[ramp, None])
+    got = c[ramp].astype(np.int64)
+    got = np.where(got > 32767, got - 65536, got)
+    assert np.all(np.abs(got - ramp_c) <= np.abs(ramp_c) * 2e-2 + 2)
+
+
+def _exact_setup():
+    # power-of-two voxel size and integer-voxel offsets: every position sum
+    # is exact, so both kernels see the same points whatever their lane order
+    scale, tau = 2.0 ** -6, 0.03
+    Kc = np.float32([40.0, 40.0, 15.5, 11.5])
+    m = np.float32([1, 0, 0, -0.25, 0, 1, 0, -0.1875, 0, 0, 1, 0.0])
+    pos = np.uint16([4, 6, 60])
+    rng = np.random.default_rng(3)
+    D = rng.integers(3800, 4300, (24, 32)).astype(np.uint16)  # in units of 2^-12 m
+    depth_m = D.astype(np.float32) * np.float32(2.0 ** -12)
+    bgr = rng.integers(0, 256, (24, 32, 3)).astype(np.uint8)
+    planes = [bgr[:, :, 2], bgr[:, :, 1], bgr[:, :, 0]]
+    return scale, tau, Kc, m, pos, D, depth_m, bgr, planes
+
+
+def test_exact_relation_to_the_integration_kernel():
+    """Same points and the same depth (u16 times 2^-12 vs float): outside the
+    fall-off the de-integration kernel writes exactly what the integration
+    kernel writes; on the fall-off its weight is exactly the negative."""
+    scale, tau, Kc, m, pos, D, depth_m, bgr, planes = _exact_setup()
+    da, wa, ca = _fresh()
+    pyoracle.update_brick(D, bgr, 2.0 ** -12, 1e9, m, Kc, scale, tau, pos, 1, da, wa, ca.reshape(-1))
+    dd, wd, cd = _fresh()
+    pyoracle.delete_brick(depth_m, planes, m, Kc, scale, tau, pos, 1, dd, wd, cd)
+    ramp = (wa > 0) & (wa < 1)
+    assert ramp.sum() > 20 and (wa == 1).sum() > 20
+    same = ~ramp
+    assert np.array_equal(dd[same].view(np.uint32), da[same].view(np.uint32))
+    assert np.array_equal(wd[same].view(np.uint32), wa[same].view(np.uint32))
+    assert np.array_equal(cd[same], ca[same])
+    assert np.array_equal(wd[ramp], -wa[ramp])
+
+
+def test_add_then_delete_cancels_in_the_first_lane_group():
+    """A frame added by the integration kernel and subtracted by the
+    de-integration kernel at a general pose: where both evaluate the same
+    point, the fall-off weights cancel exactly (f + (-f)) and front weights
+    add to 2.  The SSE kernel starts lanes 4-7 from the row origin plus
+    4 d11 while the AVX kernel adds k d11 to the origin, so in lanes 4-7
+    the two kernels round the point differently and the cancellation is
+    inexact on some voxels."""
+    scale, tau = 0.01, 0.03
+    Kc = np.float32([70.0, 70.0, 39.5, 29.5])
+    a = 0.23
+    Rm = np.array([[np.cos(a), 0, np.sin(a)], [0, 1, 0], [-np.sin(a), 0, np.cos(a)]], np.float32)
+    rng = np.random.default_rng(5)
+    D = (4096 * (1.05 + 0.02 * rng.standard_normal((60, 80)))).astype(np.uint16)
+    depth_m = D.astype(np.float32) * np.float32(2.0 ** -12)
+    bgr = np.zeros((60, 80, 3), np.uint8)
+    planes = [bgr[:, :, 2], bgr[:, :, 1], bgr[:, :, 0]]
+    exact_lo, inexact_hi, ramp_hi = True, 0, 0
+    for trial in range(12):
+        t = np.float32([-0.07 + 0.011 * trial, -0.05, -0.01 * trial])
+        m = np.concatenate([Rm, t[:, None]], 1).astype(np.float32).ravel()
+        pos = np.uint16([trial, 3, 110])
+        d, w, c = _fresh()
+        pyoracle.update_brick(D, bgr, 2.0 ** -12, 1e9, m, Kc, scale, tau, pos, 1, d, w, c.reshape(-1))
+        w0 = w.copy()
+        pyoracle.delete_brick(depth_m, planes, m, Kc, scale, tau, pos, 1, d, w, c)
+        lane = np.arange(512) & 7
+        ramp = (w0 > 0) & (w0 < 1)
+        front = w0 == 1
+        lo, hi = lane < 4, lane >= 4
+        exact_lo &= bool(np.all(w[ramp & lo] == 0.0)) and bool(np.all(w[front & lo] == 2.0))
+        # lanes 4-7: the same up to the rounding of the point
+        assert np.all(np.abs(w[ramp & hi]) <= 2e-3)
+        inexact_hi += int(np.count_nonzero(w[ramp & hi]))
+        ramp_hi += int((ramp & hi).sum())
+    assert exact_lo
+    assert ramp_hi > 100 and inexact_hi > 0
+
+
+def test_nan_depth_is_masked():
+    scale, tau, Kc, m, pos, D, depth_m, bgr, planes = _exact_setup()
+    d, w, c = _fresh()
+    d[:] = 0.25
+    w[:] = 2.0
+    c[:] = 1000
+    nan = np.full_like(depth_m, np.nan)
+    pyoracle.delete_brick(nan, planes, m, Kc, scale, tau, pos, 1, d, w, c)
+    # weight unchanged; distance and color re-averaged with rcp(2) * 2
+    assert np.all(w == 2.0)
+    r2 = pyoracle.rcp([2.0])[0]
+    assert np.allclose(d, 0.25 * 2 * r2, rtol=1e-6) and np.all(np.abs(c.astype(int) - 1000) <= 1)
+
+
+def _map(frames=3):
+    sc = synth.Scene.room_scene(9, (4.0, 4.0, 3.0), 3, 3, 0.1, 0.4)
+    R, t = sc.walk(frames + 1, 10, 0.5, 3.0, 0.02, 2.0)
+    K = (262.5, 262.5, 159.5, 119.5)
+    o = Oracle(0.01, 0.03, True, 1 << 16, 1 << 20)
+    fr = []
+    for i in range(frames + 1):
+        d, bgr = sc.render(R[i].reshape(3, 3), t[i], K, 320, 240, seed=70 + i)
+        fr.append((d, bgr, R[i], t[i]))
+    for d, bgr, Ri, ti in fr[:frames]:
+        o.add_map(d, Ri, ti, K, bgr, 1 / 5000.0, 4.0)
+    return o, fr, K
+
+
+def test_wrapper_hands_out_slots_in_queue_order():
+    o, fr, K = _map()
+    d, bgr, Ri, ti = fr[-1]
+    q = o.queue()
+    rng = np.random.default_rng(0)
+    q = q[rng.permutation(len(q))]
+    lp = LoopOracle(o, 1 << 14)
+    ok, applied = lp.delete_frame(loopclosure.depth_metres(d), loopclosure.planes(bgr), Ri, ti, K, q)
+    assert ok and applied == len(q) and lp.mapped() == len(np.unique(q))
+    mp = lp.slot_map()
+    _, first = np.unique(q, return_index=True)
+    order = q[np.sort(first)]
+    assert np.array_equal(mp[order], np.arange(len(order)))
+    rest = np.setdiff1d(np.arange(len(mp)), order)
+    assert np.all(mp[rest] == 4294967294)
+    # each slot holds what one kernel call on a fresh brick gives
+    m, Kf = pyoracle.camera_inverse(o, Ri, ti, K)
+    pos, scl, _ = o.leaves()
+    dm = loopclosure.depth_metres(d)
+    for b in order[:: max(1, len(order) // 25)]:
+        dd, ww, cc = _fresh()
+        pyoracle.delete_brick(dm, loopclosure.planes(bgr), m, Kf, 0.01, 0.03, pos[b], scl[b], dd, ww, cc)
+        gd, gw, gc = lp.bricks([mp[b]])
+        assert np.array_equal(gd[0].view(np.uint32), dd.view(np.uint32))
+        assert np.array_equal(gw[0].view(np.uint32), ww.view(np.uint32))
+        assert np.array_equal(gc[0], cc.T)
+
+
+def test_wrapper_duplicates_and_slot_exhaustion():
+    o, fr, K = _map()
+    d, bgr, Ri, ti = fr[-1]
+    dm, pl = loopclosure.depth_metres(d), loopclosure.planes(bgr)
+    q = o.queue()[:40]
+    # a brick queued twice is updated twice, from its own first result
+    lp = LoopOracle(o, 64)
+    ok, applied = lp.delete_frame(dm, pl, Ri, ti, K, np.concatenate([q[:5], q[:1]]))
+    assert ok and applied == 6 and lp.mapped() == 5
+    once = LoopOracle(o, 64)
+    once.delete_frame(dm, pl, Ri, ti, K, q[:1])
+    once.delete_frame(dm, pl, Ri, ti, K, q[:1])
+    assert np.array_equal(lp.bricks([0])[1], once.bricks([0])[1])
+    ok, applied = lp.delete_frame(dm, pl, Ri, ti, K, np.array([q[10], q[11], q[0], q[12], q[1]], np.uint32))
+    assert ok and applied == 5 and lp.mapped() == 8
+    assert np.array_equal(lp.slot_map()[[q[10], q[11], q[12]]], [5, 6, 7])
+    # 8 slots, 6 taken: the third new brick of the next queue finds none; the
+    # call stops there, before a brick that already has a slot
+    full = LoopOracle(o, 8)
+    full.delete_frame(dm, pl, Ri, ti, K, q[:6])
+    w_before = full.bricks([0])[1].copy()
+    ok, applied = full.delete_frame(dm, pl, Ri, ti, K, np.array([q[10], q[11], q[12], q[0]], np.uint32))
+    assert not ok and applied == 2 and full.mapped() == 8
+    assert full.slot_map()[q[12]] == 4294967294
+    assert np.array_equal(full.bricks([0])[1], w_before)  # q[0] after the stop: untouched
+
+
+def test_pose_order_is_the_reference_posemap():
+    """startLoopClosure's std::multimap<float, size_t> keyed by minus the
+    squared translation change (geometryfusion_mipmap_cpu.cpp:4045-4054):
+    largest change first, ties in pose order, float keys."""
+    old = np.array([[0, 0, 0], [1, 0, 0], [0, 2, 0], [0, 0, 0.5], [3, 3, 3]], np.float64)
+    new = old + np.array([[0, 0, 0], [0.1, 0, 0], [0, 0.3, 0], [0.3, 0, 0], [0, 0, 0]])
+    assert loopclosure.pose_order(new, old) == [2, 3, 1, 0, 4]
+    assert loopclosure.pose_order(new, old, max_poses=2) == [2, 3]
+    with pytest.raises(ValueError):
+        loopclosure.pose_order(new[:3], old)
+
+
+def test_depth_metres_is_the_reference_conversion():
+    """convertTo(CV_32FC1, 1/5000) then zero -> IMAGEINFINITE (quiet NaN),
+    geometryfusion_mipmap_cpu.cpp:3950-3951."""
+    d = np.array([[0, 1, 5000, 65535]], np.uint16)
+    m = loopclosure.depth_metres(d)
+    assert np.isnan(m[0, 0]) and m.dtype == np.float32
+    # one float product per pixel: 5000 * float(1/5000) rounds to 1 exactly
+    assert m[0, 2] == 1.0
+    assert np.array_equal(m[0, 1:], (d[0, 1:].astype(np.float32) * np.float32(1 / 5000.0)))
