@@ -62,6 +62,146 @@ def cpu_baseline(depth, bgr, R, t, cfg, budget_s):
                       f"{' + updateMeshes' if has_mesh else ''} on 1 host core, {dt:.1f} s"}
 
 
+LOOP_METRIC = "de-integrated frames/sec (loop closure) at 640x480, 5mm"
+
+
+def loop_pose(R, t, k):
+    """The corrected pose of frame k after a loop closure: a rotation of
+    0.1 mrad per frame about z and a drift of 0.2 mm per frame."""
+    a = 1e-4 * k
+    Rz = np.array([[np.cos(a), -np.sin(a), 0], [np.sin(a), np.cos(a), 0], [0, 0, 1]])
+    return Rz @ np.asarray(R, np.float64).reshape(3, 3), np.asarray(t, np.float64) + 2e-4 * k * np.ones(3)
+
+
+def loop_cpu_baseline(depth, bgr, R, t, cfg, budget_s):
+    """The oracle's de-integration (scalar port, 1 core): the first frames of
+    the same walk fused into an oracle map (untimed), then subtracted at their
+    corrected poses until budget_s seconds are used."""
+    from fastfusion_amd import loopclosure, synth
+    from oracle.pyoracle import LoopOracle, Oracle
+    o = Oracle(cfg["scale"], cfg["threshold"], True, max_leaves=1 << 20, max_branches=1 << 24)
+    queues = []
+    t0 = time.perf_counter()
+    while len(queues) < len(depth) and time.perf_counter() - t0 < budget_s / 3:
+        i = len(queues)
+        o.add_map(depth[i], R[i], t[i], synth.K_DEFAULT, bgr[i], 1 / 5000.0, cfg["maxcam"])
+        queues.append(np.sort(o.queue()))
+    lo = LoopOracle(o, o.counts()["n_leaves_used"])
+    n = bricks = 0
+    t0 = time.perf_counter()
+    while n < len(queues) and time.perf_counter() - t0 < budget_s:
+        R2, t2 = loop_pose(R[n], t[n], n)
+        lo.delete_frame(loopclosure.depth_metres(depth[n]), loopclosure.planes(bgr[n]), R2, t2, synth.K_DEFAULT,
+                        queues[n])
+        bricks += len(queues[n])
+        n += 1
+    dt = time.perf_counter() - t0
+    return {"value": n / dt, "unit": "frames/s", "bricks_per_s": bricks / dt, "cores": 1, "kind": "port",
+            "sample": f"first {n} frames of the same 640x480 5 mm walk ({bricks} queued bricks) subtracted by "
+                      f"the oracle's updateWrapper_deleteloop on 1 host core, {dt:.1f} s"}
+
+
+def run_loop(args):
+    """Loop-closure de-integration: the walk's frames are fused (untimed) with
+    their brick queues recorded, then subtracted from a partner map at
+    corrected poses, in the reference's pose order, with the depth (metres),
+    color planes and queues resident in HBM."""
+    from fastfusion_amd import FusionMipMapGPU, hiprt, loopclosure, synth
+    n_frames = args.warmup + args.steps
+    sc, R, t, cfg = synth.config_c2(n_frames=max(n_frames, 1))
+    depth, bgr = render_frames(sc, R, t, n_frames, cfg)
+    w, h = cfg["w"], cfg["h"]
+    K = synth.K_DEFAULT
+    fz = FusionMipMapGPU(0, 0, 0, cfg["scale"], cfg["threshold"], 0, True, max_leaves=1 << 21,
+                         max_branches=1 << 25)
+    d_depth = hiprt.DeviceBuffer.from_array(depth)
+    d_bgr = hiprt.DeviceBuffer.from_array(bgr)
+    fb, cb = w * h * 2, w * h * 3
+    queues = []
+    for i in range(n_frames):
+        fz.addMapDevice(d_depth.addr(i * fb), d_bgr.addr(i * cb), w, h, R[i], t[i], K, 1 / 5000.0, cfg["maxcam"])
+        queues.append(fz.queue())
+        if i % 200 == 199:
+            print(f"[bench] fused {i + 1} frames", file=sys.stderr, flush=True)
+    del d_depth, d_bgr
+    counts = fz.counts()
+    lp = loopclosure.LoopClosure(fz, counts["n_leaves_used"])
+    dm = loopclosure.depth_metres(depth)
+    pl = np.ascontiguousarray(np.stack([bgr[..., 2], bgr[..., 1], bgr[..., 0]], 1))  # (n, 3, h, w)
+    qoff = np.concatenate([[0], np.cumsum([len(q) for q in queues])])
+    d_dm = hiprt.DeviceBuffer.from_array(dm)
+    d_pl = hiprt.DeviceBuffer.from_array(pl)
+    d_q = hiprt.DeviceBuffer.from_array(np.concatenate(queues).astype(np.uint32))
+    moved = [loop_pose(R[i], t[i], i) for i in range(n_frames)]
+    order = loopclosure.pose_order([m[1] for m in moved], t[:n_frames])
+    npix = w * h
+
+    def frame(p):
+        R2, t2 = moved[p]
+        return (d_dm.addr(p * npix * 4), d_pl.addr(p * 3 * npix), d_pl.addr((p * 3 + 1) * npix),
+                d_pl.addr((p * 3 + 2) * npix), w, h, R2, t2, K, d_q.addr(int(qoff[p]) * 4),
+                int(qoff[p + 1] - qoff[p]))
+
+    # deleteOldLoopPoses visits at most MAX_NUMBER_POSES_PER_LOOP poses per
+    # loop closure: one submission per such group of frames, prepared ahead
+    G = loopclosure.MAX_NUMBER_POSES_PER_LOOP
+
+    def groups(poses):
+        return [lp.prepare_frames([frame(p) for p in poses[g:g + G]]) for g in range(0, len(poses), G)]
+
+    def submit(prepared):
+        done = 0
+        for sub in prepared:
+            ok, applied = lp.delete_frames(sub)
+            assert ok
+            done += int(applied.sum())
+        return done
+
+    warm, timed = groups(order[:args.warmup]), groups(order[args.warmup:n_frames])
+    submit(warm)
+    hiprt.synchronize()
+    lp.set_timing(True)
+    t0 = time.perf_counter()
+    bricks = submit(timed)
+    hiprt.synchronize()
+    dt = time.perf_counter() - t0
+    ms, calls, by, units = lp.timing()
+    k_ms = ms / max(calls, 1)
+    k_bytes = by / max(calls, 1)
+    achieved = k_bytes / (k_ms * 1e-3) / 1e9 if k_ms > 0 else 0.0
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = loop_cpu_baseline(depth, bgr, R, t, cfg, args.cpu_seconds)
+    line = {
+        "metric": LOOP_METRIC,
+        "value": round(args.steps / dt, 2),
+        "unit": "frames/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(dt * 1e3 / args.steps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic: seeded analytic room ray-cast on the host; float depth (metres, NaN where 0), "
+                "R/G/B planes and brick queues resident in HBM",
+        "config": {"workload": "loop-closure de-integration of the 640x480, 5 mm walk's frames at corrected "
+                               "poses, in the reference's pose order, 50 frames per submission, into a partner "
+                               "map sized to the map",
+                   "frames": n_frames, "resolution": [w, h], "voxel_m": cfg["scale"],
+                   "truncation_m": cfg["threshold"]},
+        "bricks_per_s": round(bricks / dt, 1),
+        "bricks_per_frame": round(bricks / args.steps, 1),
+        "map": {"bricks": counts["n_leaves_used"], "partner_slots_used": lp.mapped()},
+        "roofline": {"kernel": "k_loop_delete", "bound": "hbm", "achieved": round(achieved, 2),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": None, "avg_launch_us": round(k_ms * 1e3, 3), "bytes_per_launch": round(k_bytes)},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+
+
 def stream_configs(n):
     """The batched-streams workload: stream i (seed 100 + i) has a room of
     2-8 m per horizontal side and 2.5-3.5 m height with 3-20 primitives, a
@@ -188,6 +328,8 @@ def main():
                     help="batched-streams workload with this many streams (24 in the reference workload)")
     ap.add_argument("--frames", type=int, default=300, help="frames per stream (batched-streams workload)")
     ap.add_argument("--warmup-ticks", type=int, default=80, help="untimed ticks (autotuning) of the streams workload")
+    ap.add_argument("--loop", action="store_true",
+                    help="loop-closure de-integration workload (subtract the walk's frames from a partner map)")
     ap.add_argument("--calib-copy", type=int, default=0,
                     help="MiB copied device-to-device once before the run (HBM counter calibration)")
     args = ap.parse_args()
@@ -205,6 +347,8 @@ def main():
         del src, dst
     if args.streams:
         return run_streams(args)
+    if args.loop:
+        return run_loop(args)
 
     from fastfusion_amd import FusionMipMapGPU, hiprt, synth
 

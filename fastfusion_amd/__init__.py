@@ -40,6 +40,13 @@ class _Frame(ctypes.Structure):
                 ("K", ctypes.c_double * 4), ("scaling", ctypes.c_float), ("maxcam", ctypes.c_float)]
 
 
+class _LoopFrame(ctypes.Structure):
+    _fields_ = [("depth", ctypes.c_void_p), ("red", ctypes.c_void_p), ("green", ctypes.c_void_p),
+                ("blue", ctypes.c_void_p), ("width", ctypes.c_int), ("height", ctypes.c_int),
+                ("R", ctypes.c_double * 9), ("t", ctypes.c_double * 3), ("K", ctypes.c_double * 4),
+                ("queue", ctypes.c_void_p), ("n_queue", ctypes.c_size_t)]
+
+
 # name -> (restype, argtypes); exactly the symbols of include/ff_abi.h
 _P = ctypes.c_void_p
 _SZ = ctypes.c_size_t
@@ -86,6 +93,7 @@ ABI = {
                                             _P, _SZ, ctypes.POINTER(_SZ)]),
     "ff_loop_delete_frame_device": (ctypes.c_int, [_P, _P, _P, _P, _P, ctypes.c_int, ctypes.c_int, _DP, _DP,
                                                    _DP, _P, _SZ, ctypes.POINTER(_SZ)]),
+    "ff_loop_delete_frames": (ctypes.c_int, [_P, _P, ctypes.c_int, _P]),
     "ff_loop_mapped": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_uint32)]),
     "ff_loop_copy_map": (ctypes.c_int, [_P, _P, _SZ]),
     "ff_loop_copy_bricks": (ctypes.c_int, [_P, _P, _SZ, _P, _P, _P]),

@@ -12,13 +12,17 @@
 // has none (stopping when none is left), and runs
 // update8AddLoopSSESingleDeleteloop (loopclosure.cpp:15-230) on it.
 //
-// Here the partner lives in HBM next to the main map, and one frame is four
-// launches on the map's stream:
-//   k_loop_claim   per queue entry: the first position of each brick
-//   k_loop_flags   per 1024 entries: how many take a new slot
-//   k_loop_assign  slots in queue order, the position where the hand-out
-//                  stops when the store is full, and per brick the number of
-//                  entries to apply (a brick queued twice is updated twice)
+// Here the partner lives in HBM next to the main map, with the slot count
+// in device memory, so that any number of frames (deleteOldLoopPoses' loop,
+// :3940-4022) go out as one submission with one synchronisation at the end.
+// Per frame, on the map's stream:
+//   k_loop_map1    queues up to 8192 entries: one workgroup claims each
+//                  brick's first entry, hands out slots in queue order, finds
+//                  where the hand-out stops when the store is full and counts
+//                  per brick the entries to apply (a brick queued twice is
+//                  updated twice)
+//   k_loop_claim / k_loop_flags / k_loop_assign
+//                  the same for longer queues, over many workgroups
 //   k_loop_delete  a 512-thread block per brick, one voxel per thread: the
 //                  de-integration update in the hardware round-toward-zero
 //                  mode with the RCPPS table, repeated per entry
@@ -36,11 +40,17 @@ namespace ff {
 
 __constant__ uint32_t c_rcp_loop[2048];
 
+// the slot count (_nLeavesMapped), kept in device memory across frames
+struct LoopState {
+    uint32_t nMapped, pad[3];
+};
+
+// per frame
 struct LoopCounters {
-    uint32_t nMapped;      // slots handed out before the call
-    uint32_t nMappedNext;  // after it
-    uint32_t failpos;      // first queue entry not applied (the queue length when all were)
-    uint32_t bad;          // a queue entry names a brick beyond the map's used bricks
+    uint32_t failpos;      // first queue entry not applied (set to the queue length before the frame)
+    uint32_t bad;          // a queue entry names a brick beyond the map's used bricks (0 before)
+    uint32_t nMappedNext;  // the slot count after the frame
+    uint32_t pad;
 };
 
 // device pointers and sizes the loop kernels share
@@ -56,8 +66,16 @@ struct LoopArgs {
 };
 
 #define FF_LOOP_TPB 1024
-#define FF_LOOP_GRID 8192  // blocks of k_loop_delete; each loops over queue entries
+#define FF_LOOP_SMALL 8192  // queues up to this length are mapped by one workgroup
+#define FF_LOOP_GRID 8192   // blocks of k_loop_delete; each loops over queue entries
 #define FF_UNCLAIMED 0xffffffffu
+
+// a claim made by another thread of the same kernel: read at L2, not from a
+// possibly stale L1 line
+__device__ __forceinline__ uint32_t load_claim(const uint32_t *p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
 
 __global__ void k_loop_claim(const uint32_t *__restrict__ queue, uint32_t n, LoopArgs A, LoopCounters *lc)
 {
@@ -75,15 +93,15 @@ __global__ void k_loop_claim(const uint32_t *__restrict__ queue, uint32_t n, Loo
 // reference's test threadOffset >= _nLeavesMapped, loopclosure.cpp:278)
 __device__ __forceinline__ bool loop_takes_slot(uint32_t i, uint32_t b, const LoopArgs &A, uint32_t nMapped)
 {
-    return b < A.nLeavesUsed && A.first[b] == i && A.map[b] >= nMapped;
+    return b < A.nLeavesUsed && load_claim(&A.first[b]) == i && A.map[b] >= nMapped;
 }
 
 __global__ __launch_bounds__(FF_LOOP_TPB) void k_loop_flags(const uint32_t *__restrict__ queue, uint32_t n,
-                                                            LoopArgs A, const LoopCounters *lc,
+                                                            LoopArgs A, const LoopState *st,
                                                             uint32_t *__restrict__ bsum)
 {
     const uint32_t i = blockIdx.x * FF_LOOP_TPB + threadIdx.x;
-    const uint32_t nMapped = lc->nMapped;
+    const uint32_t nMapped = st->nMapped;
     const bool nw = i < n && loop_takes_slot(i, queue[i], A, nMapped);
     const int cnt = __syncthreads_count(nw);
     if (threadIdx.x == 0) bsum[blockIdx.x] = (uint32_t)cnt;
@@ -102,11 +120,11 @@ __device__ __forceinline__ uint32_t block_sum_1024(uint32_t v, uint32_t *red)
 }
 
 __global__ __launch_bounds__(FF_LOOP_TPB) void k_loop_assign(const uint32_t *__restrict__ queue, uint32_t n,
-                                                             LoopArgs A, LoopCounters *lc,
+                                                             LoopArgs A, const LoopState *st, LoopCounters *lc,
                                                              const uint32_t *__restrict__ bsum, uint32_t nb)
 {
     __shared__ uint32_t red[FF_LOOP_TPB / 64], wsum[FF_LOOP_TPB / 64];
-    const uint32_t nMapped = lc->nMapped;
+    const uint32_t nMapped = st->nMapped;
     if (lc->bad) {  // the call is rejected: nothing is mapped or applied
         if (blockIdx.x == 0 && threadIdx.x == 0) lc->nMappedNext = nMapped;
         return;
@@ -147,6 +165,76 @@ __global__ __launch_bounds__(FF_LOOP_TPB) void k_loop_assign(const uint32_t *__r
         }
     }
     if (blockIdx.x == 0 && threadIdx.x == 0) lc->nMappedNext = nMapped + min(all, avail);
+}
+
+// The three kernels above in one workgroup, for queues of at most
+// FF_LOOP_SMALL entries: claims, then the hand-out chunk by chunk with a
+// running count of the slots taken.
+__global__ __launch_bounds__(FF_LOOP_TPB) void k_loop_map1(const uint32_t *__restrict__ queue, uint32_t n,
+                                                           LoopArgs A, const LoopState *st, LoopCounters *lc)
+{
+    __shared__ uint32_t wsum[FF_LOOP_TPB / 64];
+    __shared__ uint32_t s_bad, s_fail;
+    const uint32_t nMapped = st->nMapped;
+    if (threadIdx.x == 0) {
+        s_bad = 0u;
+        s_fail = n;
+    }
+    __syncthreads();  // flags initialised
+    for (uint32_t i = threadIdx.x; i < n; i += FF_LOOP_TPB) {
+        const uint32_t b = queue[i];
+        if (b >= A.nLeavesUsed)
+            s_bad = 1u;
+        else
+            atomicMin(&A.first[b], i);
+    }
+    __threadfence();  // the claims are performed before any thread reads them
+    __syncthreads();
+    if (s_bad) {  // the frame is rejected; k_loop_delete releases the claims
+        if (threadIdx.x == 0) {
+            lc->bad = 1u;
+            lc->failpos = 0u;
+            lc->nMappedNext = nMapped;
+        }
+        return;
+    }
+    const uint32_t avail = A.total - nMapped;
+    const uint32_t lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    uint32_t running = 0;
+    for (uint32_t base = 0; base < n; base += FF_LOOP_TPB) {
+        const uint32_t i = base + threadIdx.x;
+        uint32_t b = 0;
+        bool nw = false;
+        if (i < n) {
+            b = queue[i];
+            nw = loop_takes_slot(i, b, A, nMapped);
+        }
+        const uint64_t bal = __ballot(nw);
+        if (lane == 0) wsum[wid] = (uint32_t)__popcll(bal);
+        __syncthreads();  // wave counts of this chunk written
+        uint32_t woff = 0, tot = 0;
+#pragma unroll
+        for (uint32_t k = 0; k < FF_LOOP_TPB / 64; k++) {
+            const uint32_t v = wsum[k];
+            woff += (k < wid) ? v : 0u;
+            tot += v;
+        }
+        const uint32_t P = running + woff + (uint32_t)__popcll(bal & ((1ull << lane) - 1ull));
+        if (i < n) {
+            if (P < avail || (P == avail && !nw)) {
+                if (nw) A.map[b] = nMapped + P;
+                atomicAdd(&A.count[b], 1u);
+            } else if (nw && P == avail) {
+                s_fail = i;  // the one entry that found the store full
+            }
+        }
+        running += tot;
+        __syncthreads();  // wsum read by every thread before the next chunk
+    }
+    if (threadIdx.x == 0) {
+        lc->failpos = s_fail;
+        lc->nMappedNext = nMapped + min(running, avail);
+    }
 }
 
 // One voxel of update8AddLoopSSESingleDeleteloop (loopclosure.cpp:56-225):
@@ -204,7 +292,7 @@ __device__ __forceinline__ void loop_point(const SdfRow &w, int x, float &d, flo
 }
 
 __global__ __launch_bounds__(512) void k_loop_delete(const uint32_t *__restrict__ queue, uint32_t n, LoopArgs A,
-                                                     SdfParams P, LoopCounters *lc)
+                                                     SdfParams P, LoopState *st, const LoopCounters *lc)
 {
     rz_mode_on();  // first: every float operation below rounds toward zero
     __shared__ uint32_t tab[2048];
@@ -212,7 +300,7 @@ __global__ __launch_bounds__(512) void k_loop_delete(const uint32_t *__restrict_
     const uint32_t failpos = lc->failpos;
     const bool bad = lc->bad != 0u;
     __syncthreads();  // rcp table ready; counters read
-    if (blockIdx.x == 0 && threadIdx.x == 0) lc->nMapped = lc->nMappedNext;
+    if (blockIdx.x == 0 && threadIdx.x == 0) st->nMapped = lc->nMappedNext;
     const int v = threadIdx.x;
     for (uint32_t e = blockIdx.x; e < n; e += gridDim.x) {
         const uint32_t b = queue[e];
@@ -259,6 +347,8 @@ __global__ void k_loop_fill(uint32_t *p, size_t n, uint32_t v)
 
 using namespace ff;
 
+#include <vector>
+
 struct ff_loop {
     ff_ctx *ctx = nullptr;
     uint32_t total = 0, mapN = 0, nMapped = 0;
@@ -267,9 +357,10 @@ struct ff_loop {
     float *d_dist = nullptr, *d_wgt = nullptr, *d_depth = nullptr;
     uint16_t *d_col = nullptr;
     uint8_t *d_planes = nullptr;
-    size_t bsumCap = 0, queueCap = 0, imgCap = 0;
-    LoopCounters *d_lc = nullptr, *h_lc = nullptr;
-    hipEvent_t ev[2] = {nullptr, nullptr};
+    size_t bsumCap = 0, queueCap = 0, imgCap = 0, lcCap = 0;
+    LoopState *d_st = nullptr, *h_st = nullptr;
+    LoopCounters *d_lc = nullptr, *h_lc = nullptr;  // one per frame of a submission
+    std::vector<hipEvent_t> ev;                      // two per timed frame
     bool timing = false;
     double ms = 0, bytes = 0;
     uint64_t calls = 0, units = 0;
@@ -292,9 +383,10 @@ static void loop_free(ff_loop *lp)
 {
     hipFree(lp->d_map); hipFree(lp->d_first); hipFree(lp->d_count); hipFree(lp->d_bsum);
     hipFree(lp->d_queue); hipFree(lp->d_dist); hipFree(lp->d_wgt); hipFree(lp->d_col);
-    hipFree(lp->d_depth); hipFree(lp->d_planes); hipFree(lp->d_lc);
+    hipFree(lp->d_depth); hipFree(lp->d_planes); hipFree(lp->d_lc); hipFree(lp->d_st);
     if (lp->h_lc) hipHostFree(lp->h_lc);
-    for (hipEvent_t &e : lp->ev)
+    if (lp->h_st) hipHostFree(lp->h_st);
+    for (hipEvent_t e : lp->ev)
         if (e) hipEventDestroy(e);
 }
 
@@ -321,93 +413,153 @@ static int loop_clear(ff_loop *lp)
     if ((rc = loop_fill(lp, reinterpret_cast<uint32_t *>(lp->d_dist), v, 0u)) != FF_OK) return rc;
     if ((rc = loop_fill(lp, reinterpret_cast<uint32_t *>(lp->d_wgt), v, 0u)) != FF_OK) return rc;
     if (lp->d_col && (rc = loop_fill(lp, reinterpret_cast<uint32_t *>(lp->d_col), v * 3 / 2, 0u)) != FF_OK) return rc;
+    if ((rc = loop_fill(lp, reinterpret_cast<uint32_t *>(lp->d_st), sizeof(LoopState) / 4, 0u)) != FF_OK) return rc;
     LCHK(hipStreamSynchronize(st));
     lp->nMapped = 0;
     return FF_OK;
 }
 
-static int loop_delete(ff_loop *lp, const float *depth, const uint8_t *red, const uint8_t *green,
-                       const uint8_t *blue, int W, int H, const double R[9], const double t[3], const double K[4],
-                       const uint32_t *queue, size_t n, size_t *applied)
+static int frame_check(ff_loop *lp, const ff_loop_frame &f, int k)
+{
+    const char *why = nullptr;
+    if (f.width <= 0 || f.height <= 0 || !f.depth || (f.n_queue && !f.queue))
+        why = "invalid argument";
+    else if (lp->color && (!f.red || !f.green || !f.blue))
+        why = "the map fuses color: red, green and blue planes are required";
+    else if (f.n_queue > 0x7fffffffu)
+        why = "queue longer than 2^31 entries";
+    if (!why) return FF_OK;
+    lp->err = std::string("frame ") + std::to_string(k) + ": " + why;
+    return FF_E_ARG;
+}
+
+// deleteOldLoopPoses' loop over nf frames (geometryfusion_mipmap_cpu.cpp:
+// 3940-4022) as one submission on the map's stream, then one
+// synchronisation; inputs in device memory
+static int loop_submit(ff_loop *lp, const ff_loop_frame *fr, int nf, size_t *applied)
 {
     ff_ctx *c = lp->ctx;
-    if (applied) *applied = 0;
-    if (W <= 0 || H <= 0 || !depth || !R || !t || !K || (n && !queue)) {
-        lp->err = "invalid argument";
-        return FF_E_ARG;
+    for (int k = 0; k < nf; k++) {
+        if (applied) applied[k] = 0;
+        const int rc = frame_check(lp, fr[k], k);
+        if (rc != FF_OK) return rc;
     }
-    if (lp->color && (!red || !green || !blue)) {
-        lp->err = "the map fuses color: red, green and blue planes are required";
-        return FF_E_ARG;
-    }
-    if (n > 0x7fffffffu) {
-        lp->err = "queue longer than 2^31 entries";
-        return FF_E_ARG;
-    }
-    if (!n) return FF_OK;
+    if (nf <= 0) return FF_OK;
     hipStream_t st = c->stream;
-    const uint32_t nb = (uint32_t)((n + FF_LOOP_TPB - 1) / FF_LOOP_TPB);
-    if (nb > lp->bsumCap) {
+    if ((size_t)nf > lp->lcCap) {
+        hipFree(lp->d_lc);
+        if (lp->h_lc) hipHostFree(lp->h_lc);
+        lp->d_lc = nullptr; lp->h_lc = nullptr; lp->lcCap = 0;
+        LCHK(hipMalloc(&lp->d_lc, sizeof(LoopCounters) * nf));
+        LCHK(hipHostMalloc(&lp->h_lc, sizeof(LoopCounters) * nf, hipHostMallocDefault));
+        lp->lcCap = nf;
+    }
+    uint32_t nbMax = 0;
+    for (int k = 0; k < nf; k++) {
+        const size_t n = fr[k].n_queue;
+        lp->h_lc[k] = LoopCounters{(uint32_t)n, 0u, 0u, 0u};
+        if (n > FF_LOOP_SMALL) nbMax = std::max(nbMax, (uint32_t)((n + FF_LOOP_TPB - 1) / FF_LOOP_TPB));
+    }
+    if (nbMax > lp->bsumCap) {
         hipFree(lp->d_bsum);
-        lp->d_bsum = nullptr;
-        LCHK(hipMalloc(&lp->d_bsum, sizeof(uint32_t) * nb));
-        lp->bsumCap = nb;
+        lp->d_bsum = nullptr; lp->bsumCap = 0;
+        LCHK(hipMalloc(&lp->d_bsum, sizeof(uint32_t) * nbMax));
+        lp->bsumCap = nbMax;
     }
-    const CamF q = camera_inverse(c, R, t, K);
-    SdfParams P{};
-    for (int r = 0; r < 3; r++) {
-        P.m[4 * r] = q.r[3 * r];
-        P.m[4 * r + 1] = q.r[3 * r + 1];
-        P.m[4 * r + 2] = q.r[3 * r + 2];
-        P.m[4 * r + 3] = q.t[r];
+    if (lp->timing && lp->ev.size() < 2 * (size_t)nf) {
+        const size_t have = lp->ev.size();
+        lp->ev.resize(2 * (size_t)nf, nullptr);
+        for (size_t i = have; i < lp->ev.size(); i++) LCHK(hipEventCreate(&lp->ev[i]));
     }
-    P.fx = q.fx; P.fy = q.fy; P.cx = q.cx; P.cy = q.cy;
-    P.scale = c->scale;
-    P.dT = c->dT;
-    P.useColor = lp->color ? 1 : 0;
-    LoopArgs A{};
-    A.leafinfo = c->d_leafinfo;
-    A.map = lp->d_map; A.first = lp->d_first; A.count = lp->d_count;
-    A.dist = lp->d_dist; A.wgt = lp->d_wgt; A.col = lp->d_col;
-    A.depth = depth; A.red = red; A.green = green; A.blue = blue;
-    A.W = W; A.H = H;
-    A.nLeavesUsed = c->nLeavesUsed;
-    A.total = lp->total;
-    LoopCounters *h = lp->h_lc;
-    h->nMapped = h->nMappedNext = lp->nMapped;
-    h->failpos = (uint32_t)n;
-    h->bad = 0;
-    LCHK(hipMemcpyAsync(lp->d_lc, h, sizeof(LoopCounters), hipMemcpyHostToDevice, st));
-    const uint32_t nn = (uint32_t)n;
-    k_loop_claim<<<(nn + 255) / 256, 256, 0, st>>>(queue, nn, A, lp->d_lc);
-    k_loop_flags<<<nb, FF_LOOP_TPB, 0, st>>>(queue, nn, A, lp->d_lc, lp->d_bsum);
-    k_loop_assign<<<nb, FF_LOOP_TPB, 0, st>>>(queue, nn, A, lp->d_lc, lp->d_bsum, nb);
-    if (lp->timing) LCHK(hipEventRecord(lp->ev[0], st));
-    k_loop_delete<<<std::min<uint32_t>(nn, FF_LOOP_GRID), 512, 0, st>>>(queue, nn, A, P, lp->d_lc);
-    if (lp->timing) LCHK(hipEventRecord(lp->ev[1], st));
-    LCHK(hipGetLastError());
-    LCHK(hipMemcpyAsync(h, lp->d_lc, sizeof(LoopCounters), hipMemcpyDeviceToHost, st));
+    LCHK(hipMemcpyAsync(lp->d_lc, lp->h_lc, sizeof(LoopCounters) * nf, hipMemcpyHostToDevice, st));
+    for (int k = 0; k < nf; k++) {
+        const ff_loop_frame &f = fr[k];
+        const uint32_t n = (uint32_t)f.n_queue;
+        if (!n) continue;
+        const CamF q = camera_inverse(c, f.R, f.t, f.K);
+        SdfParams P{};
+        for (int r = 0; r < 3; r++) {
+            P.m[4 * r] = q.r[3 * r];
+            P.m[4 * r + 1] = q.r[3 * r + 1];
+            P.m[4 * r + 2] = q.r[3 * r + 2];
+            P.m[4 * r + 3] = q.t[r];
+        }
+        P.fx = q.fx; P.fy = q.fy; P.cx = q.cx; P.cy = q.cy;
+        P.scale = c->scale;
+        P.dT = c->dT;
+        P.useColor = lp->color ? 1 : 0;
+        LoopArgs A{};
+        A.leafinfo = c->d_leafinfo;
+        A.map = lp->d_map; A.first = lp->d_first; A.count = lp->d_count;
+        A.dist = lp->d_dist; A.wgt = lp->d_wgt; A.col = lp->d_col;
+        A.depth = f.depth; A.red = f.red; A.green = f.green; A.blue = f.blue;
+        A.W = f.width; A.H = f.height;
+        A.nLeavesUsed = c->nLeavesUsed;
+        A.total = lp->total;
+        LoopCounters *lc = lp->d_lc + k;
+        if (n <= FF_LOOP_SMALL) {
+            k_loop_map1<<<1, FF_LOOP_TPB, 0, st>>>(f.queue, n, A, lp->d_st, lc);
+        } else {
+            const uint32_t nb = (n + FF_LOOP_TPB - 1) / FF_LOOP_TPB;
+            k_loop_claim<<<(n + 255) / 256, 256, 0, st>>>(f.queue, n, A, lc);
+            k_loop_flags<<<nb, FF_LOOP_TPB, 0, st>>>(f.queue, n, A, lp->d_st, lp->d_bsum);
+            k_loop_assign<<<nb, FF_LOOP_TPB, 0, st>>>(f.queue, n, A, lp->d_st, lc, lp->d_bsum, nb);
+        }
+        if (lp->timing) LCHK(hipEventRecord(lp->ev[2 * k], st));
+        k_loop_delete<<<std::min<uint32_t>(n, FF_LOOP_GRID), 512, 0, st>>>(f.queue, n, A, P, lp->d_st, lc);
+        if (lp->timing) LCHK(hipEventRecord(lp->ev[2 * k + 1], st));
+        LCHK(hipGetLastError());
+    }
+    LCHK(hipMemcpyAsync(lp->h_lc, lp->d_lc, sizeof(LoopCounters) * nf, hipMemcpyDeviceToHost, st));
+    LCHK(hipMemcpyAsync(lp->h_st, lp->d_st, sizeof(LoopState), hipMemcpyDeviceToHost, st));
     LCHK(hipStreamSynchronize(st));
-    if (h->bad) {
-        lp->err = "a queue entry names a brick beyond the map's used bricks";
+    lp->nMapped = lp->h_st->nMapped;
+    int bad = -1, full = -1;
+    for (int k = 0; k < nf; k++) {
+        const LoopCounters &h = lp->h_lc[k];
+        const size_t n = fr[k].n_queue;
+        if (h.bad) {
+            if (bad < 0) bad = k;
+            continue;
+        }
+        if (applied) applied[k] = h.failpos;
+        if (h.failpos < n && full < 0) full = k;
+        if (lp->timing && n) {
+            float ms = 0;
+            LCHK(hipEventElapsedTime(&ms, lp->ev[2 * k], lp->ev[2 * k + 1]));
+            lp->ms += ms;
+            lp->calls++;
+            lp->units += h.failpos;
+            lp->bytes += (double)h.failpos * (lp->color ? 14336.0 : 8192.0) +
+                         (double)fr[k].width * fr[k].height * (lp->color ? 7.0 : 4.0);
+        }
+    }
+    if (bad >= 0) {
+        lp->err = "frame " + std::to_string(bad) + ": a queue entry names a brick beyond the map's used bricks";
         return FF_E_ARG;
     }
-    lp->nMapped = h->nMapped;
-    if (applied) *applied = h->failpos;
-    if (lp->timing) {
-        float ms = 0;
-        LCHK(hipEventElapsedTime(&ms, lp->ev[0], lp->ev[1]));
-        lp->ms += ms;
-        lp->calls++;
-        lp->units += h->failpos;
-        lp->bytes += (double)h->failpos * (lp->color ? 14336.0 : 8192.0) + (double)W * H * (lp->color ? 7.0 : 4.0);
-    }
-    if (h->failpos < n) {
-        lp->err = "the loop-closure map's " + std::to_string(lp->total) + " slots are taken: applied the first " +
-                  std::to_string(h->failpos) + " of " + std::to_string(n) + " queue entries";
+    if (full >= 0) {
+        lp->err = "the loop-closure map's " + std::to_string(lp->total) + " slots are taken: frame " +
+                  std::to_string(full) + " applied the first " + std::to_string(lp->h_lc[full].failpos) + " of " +
+                  std::to_string(fr[full].n_queue) + " queue entries";
         return FF_E_POOL_FULL;
     }
     return FF_OK;
+}
+
+static ff_loop_frame one_frame(const float *depth, const uint8_t *red, const uint8_t *green, const uint8_t *blue,
+                               int W, int H, const double R[9], const double t[3], const double K[4],
+                               const uint32_t *queue, size_t n)
+{
+    ff_loop_frame f{};
+    f.depth = depth; f.red = red; f.green = green; f.blue = blue;
+    f.width = W; f.height = H;
+    if (R) std::copy(R, R + 9, f.R);
+    if (t) std::copy(t, t + 3, f.t);
+    if (K) std::copy(K, K + 4, f.K);
+    f.queue = queue;
+    f.n_queue = n;
+    return f;
 }
 
 extern "C" {
@@ -428,9 +580,8 @@ ff_loop *ff_loop_create(ff_ctx *ctx, uint32_t max_leaves)
               hipMalloc(&lp->d_dist, sizeof(float) * v) == hipSuccess &&
               hipMalloc(&lp->d_wgt, sizeof(float) * v) == hipSuccess &&
               (!lp->color || hipMalloc(&lp->d_col, sizeof(uint16_t) * v * 3) == hipSuccess) &&
-              hipMalloc(&lp->d_lc, sizeof(LoopCounters)) == hipSuccess &&
-              hipHostMalloc(&lp->h_lc, sizeof(LoopCounters), hipHostMallocDefault) == hipSuccess &&
-              hipEventCreate(&lp->ev[0]) == hipSuccess && hipEventCreate(&lp->ev[1]) == hipSuccess &&
+              hipMalloc(&lp->d_st, sizeof(LoopState)) == hipSuccess &&
+              hipHostMalloc(&lp->h_st, sizeof(LoopState), hipHostMallocDefault) == hipSuccess &&
               hipMemcpyToSymbol(HIP_SYMBOL(c_rcp_loop), FF_RCP_TABLE, sizeof(FF_RCP_TABLE), 0,
                                 hipMemcpyHostToDevice) == hipSuccess;
     if (!ok || loop_clear(lp) != FF_OK) {
@@ -464,8 +615,11 @@ int ff_loop_delete_frame(ff_loop *lp, const float *depth, const uint8_t *red, co
 {
     if (!lp) return FF_E_ARG;
     if (applied) *applied = 0;
-    if (width <= 0 || height <= 0 || !depth || (n_queue && !queue)) {
-        lp->err = "invalid argument";
+    ff_loop_frame f = one_frame(depth, red, green, blue, width, height, R, t, K, queue, n_queue);
+    int rc = frame_check(lp, f, 0);
+    if (rc != FF_OK) return rc;
+    if (!R || !t || !K) {
+        lp->err = "frame 0: invalid argument";
         return FF_E_ARG;
     }
     hipStream_t st = lp->ctx->stream;
@@ -485,14 +639,15 @@ int ff_loop_delete_frame(ff_loop *lp, const float *depth, const uint8_t *red, co
     }
     LCHK(hipMemcpyAsync(lp->d_depth, depth, sizeof(float) * npix, hipMemcpyHostToDevice, st));
     const uint8_t *pl[3] = {red, green, blue};
-    uint8_t *dp[3] = {nullptr, nullptr, nullptr};
+    const uint8_t *dp[3] = {nullptr, nullptr, nullptr};
     for (int k = 0; k < 3; k++)
-        if (pl[k]) {
+        if (pl[k] && lp->color) {
             dp[k] = lp->d_planes + k * npix;
-            LCHK(hipMemcpyAsync(dp[k], pl[k], npix, hipMemcpyHostToDevice, st));
+            LCHK(hipMemcpyAsync(lp->d_planes + k * npix, pl[k], npix, hipMemcpyHostToDevice, st));
         }
     if (n_queue) LCHK(hipMemcpyAsync(lp->d_queue, queue, sizeof(uint32_t) * n_queue, hipMemcpyHostToDevice, st));
-    return loop_delete(lp, lp->d_depth, dp[0], dp[1], dp[2], width, height, R, t, K, lp->d_queue, n_queue, applied);
+    f = one_frame(lp->d_depth, dp[0], dp[1], dp[2], width, height, R, t, K, lp->d_queue, n_queue);
+    return loop_submit(lp, &f, 1, applied);
 }
 
 int ff_loop_delete_frame_device(ff_loop *lp, const float *depth_dev, const uint8_t *red_dev,
@@ -501,8 +656,23 @@ int ff_loop_delete_frame_device(ff_loop *lp, const float *depth_dev, const uint8
                                 const uint32_t *queue_dev, size_t n_queue, size_t *applied)
 {
     if (!lp) return FF_E_ARG;
-    return loop_delete(lp, depth_dev, red_dev, green_dev, blue_dev, width, height, R, t, K, queue_dev, n_queue,
-                       applied);
+    if (!R || !t || !K) {
+        lp->err = "frame 0: invalid argument";
+        return FF_E_ARG;
+    }
+    const ff_loop_frame f = one_frame(depth_dev, red_dev, green_dev, blue_dev, width, height, R, t, K, queue_dev,
+                                      n_queue);
+    return loop_submit(lp, &f, 1, applied);
+}
+
+int ff_loop_delete_frames(ff_loop *lp, const ff_loop_frame *frames, int n_frames, size_t *applied)
+{
+    if (!lp) return FF_E_ARG;
+    if (n_frames < 0 || (n_frames && !frames)) {
+        lp->err = "invalid argument";
+        return FF_E_ARG;
+    }
+    return loop_submit(lp, frames, n_frames, applied);
 }
 
 int ff_loop_mapped(ff_loop *lp, uint32_t *n_mapped, uint32_t *capacity)

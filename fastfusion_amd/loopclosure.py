@@ -22,7 +22,7 @@ import ctypes
 
 import numpy as np
 
-from . import FFError, _SZ, _dptr, _ptr, load_library
+from . import FFError, _LoopFrame, _SZ, _dptr, _ptr, load_library
 
 MAX_LOOP_CLOSURE_LEAVES = 100000  # definitions.h:124
 MAX_NUMBER_POSES_PER_LOOP = 50    # definitions.h:125
@@ -141,6 +141,36 @@ class LoopClosure:
             return False, applied.value
         self._check(rc)
         return True, applied.value
+
+    @staticmethod
+    def prepare_frames(frames):
+        """The ff_loop_frame array of frames = [(depth_ptr, red_ptr, green_ptr,
+        blue_ptr, w, h, R, t, K, queue_ptr, n_queue)] (device pointers), for
+        delete_frames."""
+        n = len(frames)
+        arr = (_LoopFrame * max(n, 1))()
+        for k, (dp, rp, gp, bp, w, h, R, t, K, qp, nq) in enumerate(frames):
+            f = arr[k]
+            f.depth, f.red, f.green, f.blue = dp, rp or None, gp or None, bp or None
+            f.width, f.height = int(w), int(h)
+            f.R[:] = [float(x) for x in np.asarray(R, np.float64).ravel()]
+            f.t[:] = [float(x) for x in np.asarray(t, np.float64).ravel()]
+            f.K[:] = [float(x) for x in np.asarray(K, np.float64).ravel()]
+            f.queue, f.n_queue = qp or None, int(nq)
+        return arr, n
+
+    def delete_frames(self, frames):
+        """deleteOldLoopPoses' loop as one submission (ff_loop_delete_frames)
+        over a list as for prepare_frames, or its result.  Returns (ok,
+        applied per frame); a frame naming an invalid brick raises after the
+        others ran."""
+        arr, n = frames if isinstance(frames, tuple) else self.prepare_frames(frames)
+        applied = np.zeros(max(n, 1), np.uintp)
+        rc = self._lib.ff_loop_delete_frames(self._h, ctypes.cast(arr, ctypes.c_void_p), n, _ptr(applied))
+        if rc == FF_E_POOL_FULL:
+            return False, applied[:n].astype(np.int64)
+        self._check(rc)
+        return True, applied[:n].astype(np.int64)
 
     def mapped(self):
         """_nLeavesMapped."""

@@ -192,6 +192,21 @@ int ff_loop_delete_frame_device(ff_loop *lp, const float *depth_dev, const uint8
                                 const uint8_t *green_dev, const uint8_t *blue_dev, int width, int height,
                                 const double R[9], const double t[3], const double K[4],
                                 const uint32_t *queue_dev, size_t n_queue, size_t *applied);
+/* deleteOldLoopPoses' loop (geometryfusion_mipmap_cpu.cpp:3940-4022) as one
+ * submission: frames[0..n_frames) applied in order, each as
+ * ff_loop_delete_frame_device, with one synchronisation at the end.
+ * applied[k] (may be NULL) gets frame k's applied entries.  A frame naming
+ * a brick beyond the map's used bricks is skipped (FF_E_ARG after the
+ * others are applied); FF_E_POOL_FULL when a frame stopped early. */
+typedef struct ff_loop_frame {
+    const float *depth;                /* device, W x H metres, NaN = no measurement */
+    const uint8_t *red, *green, *blue; /* device planes, or NULL without color */
+    int width, height;
+    double R[9], t[3], K[4];           /* camera -> world, intrinsics fx fy cx cy */
+    const uint32_t *queue;             /* device, n_queue brick indices */
+    size_t n_queue;
+} ff_loop_frame;
+int ff_loop_delete_frames(ff_loop *lp, const ff_loop_frame *frames, int n_frames, size_t *applied);
 /* _nLeavesMapped and the slot capacity. */
 int ff_loop_mapped(ff_loop *lp, uint32_t *n_mapped, uint32_t *capacity);
 /* _leafMapLoop for bricks [0, n): the slot, or 4294967294 (BRANCHINIT). */

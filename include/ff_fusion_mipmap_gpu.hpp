@@ -18,6 +18,7 @@
 #include <cstdint>
 #include <fstream>
 #include <stdexcept>
+#include <algorithm>
 #include <string>
 #include <vector>
 
@@ -174,6 +175,18 @@ public:
 
     ff_ctx *handle() { return ctx_; }
 
+    // the bricks the last addMap updated (_leafNumberSurface, ascending), as
+    // the reference records them per frame for loop closure (_leafQueues,
+    // geometryfusion_mipmap_cpu.cpp:2165-2167)
+    std::vector<uint32_t> lastQueue()
+    {
+        ff_counts c{};
+        check(ff_get_counts(ctx_, &c));
+        std::vector<uint32_t> q(c.n_queued);
+        if (c.n_queued) q.resize((size_t)check(ff_copy_queue(ctx_, q.data(), q.size())));
+        return q;
+    }
+
 private:
     int check(int rc)
     {
@@ -181,6 +194,87 @@ private:
         return rc;
     }
     ff_ctx *ctx_ = nullptr;
+};
+
+// The loop-closure partner map (FusionMipMapCPU(FusionMipMapCPU*,
+// nLeavesTotalLoop), geometryfusion_mipmap_cpu.cpp:645-717) of a
+// FusionMipMapGPU, which must outlive it.
+class LoopClosureGPU {
+public:
+    explicit LoopClosureGPU(FusionMipMapGPU &fusion, uint32_t nLeavesTotalLoop = 100000)
+    {
+        lp_ = ff_loop_create(fusion.handle(), nLeavesTotalLoop);
+        if (!lp_) throw std::runtime_error("ff_loop_create failed");
+    }
+    ~LoopClosureGPU() { ff_loop_destroy(lp_); }
+    LoopClosureGPU(const LoopClosureGPU &) = delete;
+    LoopClosureGPU &operator=(const LoopClosureGPU &) = delete;
+
+    // updateArraysForLoopClosure's reset (:3927-3931)
+    void reset() { check(ff_loop_reset(lp_)); }
+
+    // updateWrapper_deleteloop (loopclosure.cpp:232-311) for one frame:
+    // depth in metres (NaN: no measurement), R, G, B planes (NULL without
+    // color), the frame's recorded queue.  Returns false when the slots ran
+    // out (the first *applied entries were applied), like the reference.
+    bool deleteFrame(const float *depth, const uint8_t *red, const uint8_t *green, const uint8_t *blue, int width,
+                     int height, const CameraPose &cam, const std::vector<uint32_t> &queue, size_t *applied = nullptr)
+    {
+        const double K[4] = {cam.fx, cam.fy, cam.cx, cam.cy};
+        const int rc = ff_loop_delete_frame(lp_, depth, red, green, blue, width, height, cam.R, cam.t, K,
+                                            queue.data(), queue.size(), applied);
+        if (rc == FF_E_POOL_FULL) return false;
+        check(rc);
+        return true;
+    }
+
+    // deleteOldLoopPoses' loop (:3940-4022) over frames in device memory,
+    // one submission
+    bool deleteFrames(const std::vector<ff_loop_frame> &frames, std::vector<size_t> *applied = nullptr)
+    {
+        std::vector<size_t> a(frames.size());
+        const int rc = ff_loop_delete_frames(lp_, frames.data(), (int)frames.size(), a.data());
+        if (applied) *applied = a;
+        if (rc == FF_E_POOL_FULL) return false;
+        check(rc);
+        return true;
+    }
+
+    uint32_t leavesMapped()
+    {
+        uint32_t n = 0;
+        check(ff_loop_mapped(lp_, &n, nullptr));
+        return n;
+    }
+
+    // startLoopClosure's pose map (geometryfusion_mipmap_cpu.cpp:4045-4054):
+    // poses ordered by minus the squared translation change (double sum,
+    // float key), equal keys in pose order, at most maxPoses of them
+    static std::vector<size_t> poseOrder(const std::vector<CameraPose> &loop, const std::vector<CameraPose> &fused,
+                                         size_t maxPoses = 50)
+    {
+        std::vector<std::pair<float, size_t>> key;
+        for (size_t i = 0; i < loop.size() && i < fused.size(); i++) {
+            const double dx = loop[i].t[0] - fused[i].t[0], dy = loop[i].t[1] - fused[i].t[1],
+                         dz = loop[i].t[2] - fused[i].t[2];
+            key.emplace_back((float)(-((dx * dx + dy * dy) + dz * dz)), i);
+        }
+        std::stable_sort(key.begin(), key.end(),
+                         [](const std::pair<float, size_t> &a, const std::pair<float, size_t> &b) {
+                             return a.first < b.first;
+                         });
+        std::vector<size_t> order;
+        for (size_t i = 0; i < key.size() && i < maxPoses; i++) order.push_back(key[i].second);
+        return order;
+    }
+
+private:
+    int check(int rc)
+    {
+        if (rc < 0) throw std::runtime_error(std::string("fastfusion_amd: ") + ff_loop_last_error(lp_));
+        return rc;
+    }
+    ff_loop *lp_ = nullptr;
 };
 
 }  // namespace ff

@@ -248,3 +248,28 @@ def test_depth_metres_is_the_reference_conversion():
     # one float product per pixel: 5000 * float(1/5000) rounds to 1 exactly
     assert m[0, 2] == 1.0
     assert np.array_equal(m[0, 1:], (d[0, 1:].astype(np.float32) * np.float32(1 / 5000.0)))
+
+
+def test_cpp_pose_order_matches_python(tmp_path):
+    import os
+    import subprocess
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    rng = np.random.default_rng(8)
+    old = rng.normal(size=(40, 3))
+    new = old + rng.normal(scale=0.05, size=(40, 3)) * (rng.random((40, 1)) < 0.7)
+    new[5] = old[5] + [0.25, 0, 0]   # equal keys keep pose order
+    new[9] = old[9] + [0, 0.25, 0]
+    src = tmp_path / "order.cpp"
+    src.write_text(
+        '#include "ff_fusion_mipmap_gpu.hpp"\n#include <cstdio>\n'
+        "int main() { int n; if (scanf(\"%d\", &n) != 1) return 1;\n"
+        "  std::vector<ff::CameraPose> a(n), b(n);\n"
+        "  for (int i = 0; i < n; i++) for (int k = 0; k < 3; k++) if (scanf(\"%lf\", &a[i].t[k]) != 1) return 1;\n"
+        "  for (int i = 0; i < n; i++) for (int k = 0; k < 3; k++) if (scanf(\"%lf\", &b[i].t[k]) != 1) return 1;\n"
+        "  for (size_t p : ff::LoopClosureGPU::poseOrder(a, b, 25)) printf(\"%zu \", p);\n  return 0; }\n")
+    exe = tmp_path / "order"
+    subprocess.check_call(["g++", "-std=c++17", "-O1", "-I", os.path.join(root, "include"), str(src), "-o", str(exe)])
+    txt = f"{len(old)}\n" + "\n".join(" ".join(repr(float(x)) for x in r) for r in new) + "\n" + \
+        "\n".join(" ".join(repr(float(x)) for x in r) for r in old) + "\n"
+    out = subprocess.run([str(exe)], input=txt, capture_output=True, text=True, check=True).stdout.split()
+    assert [int(x) for x in out] == loopclosure.pose_order(new, old, 25)

@@ -159,3 +159,38 @@ def test_delete_old_poses_follows_the_pose_order():
         d, bgr, _, _, q = frames[p]
         lo.delete_frame(loopclosure.depth_metres(d), loopclosure.planes(bgr), new[p][0], new[p][1], K2, q)
     _same(lg, lo, "deleteOldLoopPoses", n)
+
+
+def test_one_submission_equals_frame_by_frame():
+    print("loop: 6 frames in one submission vs one by one, a rejected frame, slots running out", flush=True)
+    K2 = (262.5, 262.5, 159.5, 119.5)
+    gpu, ora, frames = _fused(6, w=320, h=240, K_=K2)
+    n = gpu.counts()["n_leaves_used"]
+    union = np.unique(np.concatenate([f[4] for f in frames]))
+    total = len(union) * 3 // 4
+    la, lb, lo = LoopClosure(gpu, total), LoopClosure(gpu, total), LoopOracle(ora, total)
+    keep, sub = [], []
+    for k, (d, bgr, R, t, q) in enumerate(frames):
+        R2, t2 = _moved(R, t, k)
+        dm = loopclosure.depth_metres(d)
+        pl = loopclosure.planes(bgr)
+        if k == 2:  # a brick beyond the map: this frame is skipped, the others run
+            q = np.concatenate([q[:5], [n + 3], q[5:]]).astype(np.uint32)
+        else:
+            lo.delete_frame(dm, pl, R2, t2, K2, q)
+        bufs = [hiprt.DeviceBuffer.from_array(a) for a in (dm, pl[0], pl[1], pl[2], q)]
+        keep.append(bufs)
+        sub.append((bufs[0].addr(), bufs[1].addr(), bufs[2].addr(), bufs[3].addr(), 320, 240, R2, t2, K2,
+                    bufs[4].addr(), len(q)))
+    with pytest.raises(FFError, match="frame 2"):
+        la.delete_frames(sub)
+    stopped = False
+    for k, f in enumerate(sub):
+        if k == 2:
+            with pytest.raises(FFError, match="beyond the map"):
+                lb.delete_frame_device(*f)
+            continue
+        stopped |= not lb.delete_frame_device(*f)[0]
+    assert stopped
+    _same(la, lo, "one submission", n)
+    _same(lb, lo, "frame by frame", n)
