@@ -163,7 +163,8 @@ __global__ void k_frame_begin(const FrameDev *__restrict__ fd, FrameCounters *fc
 
 // ---------------------------------------------------------------------------
 // k_planes: per 64-pixel row chunk, ballot the three relations into bit
-// planes: plane[(y*words + w)*3 + {0: valid, 1: key == left key, 2: key == up key}]
+// planes: one uint4 per (row y, word w) at y * words + w holding
+// {valid, key == left key, key == up key, 0}
 // ---------------------------------------------------------------------------
 // (block (0, 0) also resets the frame counters: no kernel before the
 // descent reads them)
@@ -200,10 +201,8 @@ __device__ __forceinline__ void planes_body(const FrameDev *__restrict__ fd, int
         const int w = blockIdx.x * 2 + lane;
         if (w < words) {
             const int sh = lane * 32;
-            uint32_t *o = planes + ((size_t)y * words + w) * 3;
-            o[0] = (uint32_t)(bv >> sh);
-            o[1] = (uint32_t)(bl >> sh);
-            o[2] = (uint32_t)(bu >> sh);
+            reinterpret_cast<uint4 *>(planes)[(size_t)y * words + w] =
+                make_uint4((uint32_t)(bv >> sh), (uint32_t)(bl >> sh), (uint32_t)(bu >> sh), 0u);
         }
     }
 }
@@ -232,113 +231,144 @@ __global__ __launch_bounds__(64) void k_planes_b(const AllocStream *__restrict__
 // result for the same row arrives from lane l-1's previous step through a
 // one-lane DPP shift; rows of a lane are carried in a register.  Within a
 // lane the horizontal chain is solved bit-parallel: maximal runs linked by
-// El alternate 1,0,1,... from their start.  The other waves stream the bit
-// planes through an LDS ring of 8 bands of R rows, three bands ahead of the
-// sweep, with one block barrier per band.  Rows outside the image read ring
-// rows of other bands: their valid bits are masked, so they yield P = 0.
-// pbits rows are FF_PB_STRIDE words apart with FF_PB_PAD rows of padding
-// above and below, so every step stores unconditionally.
+// El alternate 1,0,1,... from their start, so P is the run bits whose
+// distance from the run's start is even.  One wave does every step, so a
+// step costs its instruction count: the other waves stream the bit planes
+// into an LDS ring laid out by diagonal (row + word), where step s of every
+// lane is ring row s at the lane's column, read with constant offsets; rows
+// outside the image are written as zeros, so no step tests bounds.  Four
+// bands of R diagonals, written three bands ahead of the sweep, one block
+// barrier per band.  pbits rows are FF_PB_STRIDE words apart with
+// FF_PB_PAD rows of padding above and below, so every step stores
+// unconditionally.
 // ---------------------------------------------------------------------------
-#define FF_DD_NB 8
+#define FF_DD_NB 4
+#define FF_DD_R 32
 #define FF_DD_THREADS 1024
 
-// R = 32 while one uint4 per loader thread covers a band (24 words <= 960)
-__host__ __device__ static inline int dedup_rows(int words) { return words <= 40 ? 32 : 16; }
-size_t dedup_lds_bytes(int words) { return (size_t)FF_DD_NB * dedup_rows(words) * words * 3 * sizeof(uint32_t); }
-int dedup_plane_rows(int H) { return (H + 31) / 32 * 32; }
+// ring columns: 20, 40 or 64 words (images up to 640, 1280, 2048 pixels wide)
+__host__ __device__ static inline int dedup_wmax(int words) { return words <= 20 ? 20 : (words <= 40 ? 40 : 64); }
+size_t dedup_lds_bytes(int words) { return (size_t)FF_DD_NB * FF_DD_R * dedup_wmax(words) * sizeof(uint4); }
+int dedup_plane_rows(int H) { return H; }
 
-__device__ __forceinline__ uint32_t lane_shr1(uint32_t v)
+// the ring entries one loader thread moves per band
+template <int WMAX>
+struct DedupLoad {
+    static constexpr int PER_BAND = FF_DD_R * WMAX;
+    static constexpr int PT = (PER_BAND + FF_DD_THREADS - 65) / (FF_DD_THREADS - 64);
+    uint4 v[PT];
+};
+
+// diagonal band `band` of the planes: entry e = kk * WMAX + w holds the
+// (valid, El, Eu) words of row band * R + kk - w, column word w, or zeros
+template <int WMAX>
+__device__ __forceinline__ void dedup_fetch(DedupLoad<WMAX> &L, const uint4 *__restrict__ planes, int H, int words,
+                                            int band, int lt)
 {
-    // wave_shr:1 -- lane l receives lane l-1's value, lane 0 keeps 0
-    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x138, 0xf, 0xf, false);
+#pragma unroll
+    for (int j = 0; j < DedupLoad<WMAX>::PT; j++) {
+        const int e = lt + j * (FF_DD_THREADS - 64);
+        const int kk = e / WMAX, w = e - kk * WMAX;
+        const int row = band * FF_DD_R + kk - w;
+        L.v[j] = make_uint4(0, 0, 0, 0);
+        if (e < DedupLoad<WMAX>::PER_BAND && w < words && row >= 0 && row < H)
+            L.v[j] = planes[(size_t)row * words + w];
+    }
 }
 
-template <int R>
-__device__ __forceinline__ void dedup_body(uint32_t *ring, const uint32_t *__restrict__ planes, int H, int words,
+template <int WMAX>
+__device__ __forceinline__ void dedup_put(const DedupLoad<WMAX> &L, uint4 *ring, int band, int lt)
+{
+    constexpr int RING = FF_DD_NB * FF_DD_R;
+#pragma unroll
+    for (int j = 0; j < DedupLoad<WMAX>::PT; j++) {
+        const int e = lt + j * (FF_DD_THREADS - 64);
+        if (e < DedupLoad<WMAX>::PER_BAND) {
+            const int kk = e / WMAX, w = e - kk * WMAX;
+            ring[((band * FF_DD_R + kk) & (RING - 1)) * WMAX + w] = L.v[j];
+        }
+    }
+}
+
+template <int WMAX>
+__device__ __forceinline__ void dedup_body(uint4 *ring, const uint32_t *__restrict__ planes32, int H, int words,
                                            uint32_t *__restrict__ pbits, FrameCounters *fc)
 {
-    constexpr int RING = FF_DD_NB * R;  // rows, a power of two
+    constexpr int R = FF_DD_R, RING = FF_DD_NB * R;
+    constexpr uint32_t EVEN = 0x55555555u;
+    const uint4 *planes = reinterpret_cast<const uint4 *>(planes32);
     const int tid = threadIdx.x;
     const bool compute = tid < 64;
-    const int rowDw = words * 3;
-    const int bandV = R * rowDw / 4;  // uint4 per band (R * 3 * words is a multiple of 4)
-    const int nBands = (H + R - 1) / R;
     const int nsteps = H + words - 1;
     const int niter = (nsteps + R - 1) / R;
-    const uint4 *src = reinterpret_cast<const uint4 *>(planes);
-    uint4 *ringV = reinterpret_cast<uint4 *>(ring);
-    const int lt = tid - 64, nl = (int)blockDim.x - 64;  // loader index and count (nl >= bandV)
-    uint4 pre = make_uint4(0, 0, 0, 0);
+    const int lt = tid - 64;
+    DedupLoad<WMAX> pre;
     if (!compute) {
-        for (int b = 0; b < 3 && b < nBands; b++)
-            for (int i = lt; i < bandV; i += nl) ringV[b * bandV + i] = src[(size_t)b * bandV + i];
-        if (3 < nBands && lt < bandV) pre = src[(size_t)3 * bandV + lt];
+        for (int b = 0; b < 3; b++) {
+            dedup_fetch<WMAX>(pre, planes, H, words, b, lt);
+            dedup_put<WMAX>(pre, ring, b, lt);
+        }
+        dedup_fetch<WMAX>(pre, planes, H, words, 3, lt);
     }
-    __syncthreads();
+    __syncthreads();  // the first three bands are in the ring
     const int l = tid & 63;
-    const bool lane_ok = l < words;
-    const int lw = lane_ok ? l : words - 1;
-    const uint32_t Hl = lane_ok ? (uint32_t)H : 0u;
+    const int lw = l < WMAX ? l : WMAX - 1;  // lanes past the ring's columns read a column they ignore
     uint32_t *out = pbits + (size_t)(FF_PB_PAD - l) * FF_PB_STRIDE + l;  // row 0 - l: the lane's row at step 0
-    uint32_t pprev = 0u, pout = 0u, nproc = 0u;
+    uint32_t pprev = 0u, pin = 0u, nproc = 0u;
     for (int it = 0; it < niter; it++) {
         if (compute) {
-            // the band's plane words first, so the LDS latency is paid once
-            uint32_t rva[R], rel[R], reu[R];
-            const int r0 = it * R - l;
+            const uint4 *q = ring + ((it * R) & (RING - 1)) * WMAX + lw;
+            uint4 rq[R];
 #pragma unroll
-            for (int k = 0; k < R; k++) {
-                const uint32_t *r = ring + ((r0 + k) & (RING - 1)) * rowDw + lw * 3;
-                rva[k] = r[0]; rel[k] = r[1]; reu[k] = r[2];
-            }
+            for (int k = 0; k < R; k++) rq[k] = q[k * WMAX];
             uint32_t *o = out + (size_t)it * R * FF_PB_STRIDE;
 #pragma unroll
             for (int k = 0; k < R; k++) {
-                const uint32_t va = (uint32_t)(r0 + k) < Hl ? rva[k] : 0u, el = rel[k], eu = reu[k];
-                const uint32_t pin = lane_shr1(pout);
+                const uint32_t va = rq[k].x, el = rq[k].y, eu = rq[k].z;
                 const uint32_t a = va & ~(eu & pprev);
-                const uint32_t virt = a & el & pin & 1u;          // bit 0 continues a chain whose last was processed
-                const uint32_t cont = (a & el & (a << 1)) | virt;  // bit j continues from j-1
-                const uint32_t S = a & ~cont;                      // chain starts
-                const uint32_t ev = S & 0x55555555u, od = S & 0xaaaaaaaau;
-                const uint32_t fe = ev | (((cont + (ev << 1)) ^ cont) & cont);
-                const uint32_t fo = od | (((cont + ((od << 1) | virt)) ^ cont) & cont);
-                const uint32_t P = a & ((fe & 0x55555555u) | (fo & 0xaaaaaaaau));
+                // link j <- j-1 inside the word (a at j-1) and bit 0 <- the
+                // left lane's bit 31 (linked only if that pixel was processed)
+                const uint32_t a1 = __builtin_amdgcn_alignbit(a, pin, 31);
+                const uint32_t cont = a & el & a1;
+                const uint32_t ev = a & ~cont & EVEN;                  // runs starting at even bits
+                const uint32_t t = (ev << 1) + cont;                   // carries run through each such run
+                const uint32_t E = (~t & cont) | ev;                   // the bits of runs with an even start
+                const uint32_t P = a & ~(E ^ EVEN);                    // even distance from the run's start
                 o[k * FF_PB_STRIDE] = P;
                 nproc += (uint32_t)__popc(P);
                 pprev = P;
-                pout = P >> 31;
+                pin = (uint32_t)__builtin_amdgcn_mov_dpp((int)P, 0x138, 0xf, 0xf, true);  // wave_shr:1, lane 0 gets 0
             }
         } else {
-            // band it + 3 into the slot of band it - 5; band it + 4 into registers
-            const int wb = it + 3, lb = it + 4;
-            if (wb < nBands && lt < bandV) ringV[(wb % FF_DD_NB) * bandV + lt] = pre;
-            if (lb < nBands && lt < bandV) pre = src[(size_t)lb * bandV + lt];
+            // band it + 3 into the slot band it - 1 was read from; band it + 4 into registers
+            dedup_put<WMAX>(pre, ring, it + 3, lt);
+            dedup_fetch<WMAX>(pre, planes, H, words, it + 4, lt);
         }
         __syncthreads();
     }
     if (compute) {
+        if (l >= words) nproc = 0u;
         for (int o = 32; o > 0; o >>= 1) nproc += __shfl_xor(nproc, o);
         if (l == 0) fc->nProcessed = nproc;
     }
 }
 
-template <int R>
+template <int WMAX>
 __global__ __launch_bounds__(FF_DD_THREADS) void k_dedup(const uint32_t *__restrict__ planes, int H, int words,
                                                          uint32_t *__restrict__ pbits, FrameCounters *fc)
 {
-    extern __shared__ uint32_t ring[];
-    dedup_body<R>(ring, planes, H, words, pbits, fc);
+    extern __shared__ uint4 ring[];
+    dedup_body<WMAX>(ring, planes, H, words, pbits, fc);
 }
 
-// one block per stream whose row count per band is R
-template <int R>
+// one block per stream whose ring width is WMAX
+template <int WMAX>
 __global__ __launch_bounds__(FF_DD_THREADS) void k_dedup_b(const AllocStream *__restrict__ S)
 {
-    extern __shared__ uint32_t ring[];
+    extern __shared__ uint4 ring[];
     const AllocStream &a = S[blockIdx.z];
-    if (dedup_rows(a.words) != R) return;
-    dedup_body<R>(ring, a.planes, a.H, a.words, a.pbits, a.fc);
+    if (dedup_wmax(a.words) != WMAX) return;
+    dedup_body<WMAX>(ring, a.planes, a.H, a.words, a.pbits, a.fc);
 }
 
 // ---------------------------------------------------------------------------
@@ -1260,14 +1290,18 @@ int ensure_image(ff_ctx *c, int W, int H)
     HIPCHK(hipMalloc(&c->d_pxp, sizeof(float) * W));
     HIPCHK(hipMalloc(&c->d_pyp, sizeof(float) * H));
     // bit planes padded to whole bands of the de-dup ring
-    HIPCHK(hipMalloc(&c->d_planes, sizeof(uint32_t) * 3 * words * (size_t)dedup_plane_rows(H)));
-    HIPCHK(hipFuncSetAttribute((const void *)k_dedup<32>, hipFuncAttributeMaxDynamicSharedMemorySize,
+    HIPCHK(hipMalloc(&c->d_planes, sizeof(uint4) * words * (size_t)dedup_plane_rows(H)));
+    HIPCHK(hipFuncSetAttribute((const void *)k_dedup<20>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)dedup_lds_bytes(20)));
+    HIPCHK(hipFuncSetAttribute((const void *)k_dedup<40>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                (int)dedup_lds_bytes(40)));
-    HIPCHK(hipFuncSetAttribute((const void *)k_dedup<16>, hipFuncAttributeMaxDynamicSharedMemorySize,
+    HIPCHK(hipFuncSetAttribute((const void *)k_dedup<64>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                (int)dedup_lds_bytes(64)));
-    HIPCHK(hipFuncSetAttribute((const void *)k_dedup_b<32>, hipFuncAttributeMaxDynamicSharedMemorySize,
+    HIPCHK(hipFuncSetAttribute((const void *)k_dedup_b<20>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)dedup_lds_bytes(20)));
+    HIPCHK(hipFuncSetAttribute((const void *)k_dedup_b<40>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                (int)dedup_lds_bytes(40)));
-    HIPCHK(hipFuncSetAttribute((const void *)k_dedup_b<16>, hipFuncAttributeMaxDynamicSharedMemorySize,
+    HIPCHK(hipFuncSetAttribute((const void *)k_dedup_b<64>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                (int)dedup_lds_bytes(64)));
     HIPCHK(hipMalloc(&c->d_pbits, pbits_bytes(H)));
     HIPCHK(hipMalloc(&c->d_cntB, sizeof(uint32_t) * npix));
@@ -1511,12 +1545,12 @@ void launch_alloc(ff_ctx *c, int W, int H)
     const int words = (W + 31) / 32;
     const FrameDev *fd = c->d_fd;
     k_planes<<<dim3((W + 63) / 64, H), 64, 0, c->stream>>>(fd, W, H, words, c->d_pxp, c->d_pyp, c->d_planes, c->d_fc);
-    if (dedup_rows(words) == 32)
-        k_dedup<32><<<1, FF_DD_THREADS, dedup_lds_bytes(words), c->stream>>>(c->d_planes, H, words, c->d_pbits,
-                                                                             c->d_fc);
-    else
-        k_dedup<16><<<1, FF_DD_THREADS, dedup_lds_bytes(words), c->stream>>>(c->d_planes, H, words, c->d_pbits,
-                                                                             c->d_fc);
+    const size_t lds = dedup_lds_bytes(words);
+    switch (dedup_wmax(words)) {
+    case 20: k_dedup<20><<<1, FF_DD_THREADS, lds, c->stream>>>(c->d_planes, H, words, c->d_pbits, c->d_fc); break;
+    case 40: k_dedup<40><<<1, FF_DD_THREADS, lds, c->stream>>>(c->d_planes, H, words, c->d_pbits, c->d_fc); break;
+    default: k_dedup<64><<<1, FF_DD_THREADS, lds, c->stream>>>(c->d_planes, H, words, c->d_pbits, c->d_fc); break;
+    }
     const MapView mv = c->mapView();
     k_descend<<<dim3((W + 127) / 128, H), 128, 0, c->stream>>>(fd, W, H, words, c->d_pxp, c->d_pyp, c->d_pbits, mv,
                                                                c->hv, c->d_pl0, c->d_fc);
@@ -1581,25 +1615,20 @@ void launch_alloc_batch(hipStream_t st, const AllocStream *d_as, const AllocStre
 {
     int maxW = 1, maxH = 1;
     uint32_t maxLeaves = 0;
-    bool r32 = false, r16 = false;
-    size_t lds32 = 0, lds16 = 0;
+    bool cls[3] = {false, false, false};
     for (int k = 0; k < n; k++) {
         const AllocStream &a = h_as[k];
         maxW = std::max(maxW, a.W);
         maxH = std::max(maxH, a.H);
         maxLeaves = std::max(maxLeaves, a.nLeavesUsed);
-        if (dedup_rows(a.words) == 32) {
-            r32 = true;
-            lds32 = std::max(lds32, dedup_lds_bytes(a.words));
-        } else {
-            r16 = true;
-            lds16 = std::max(lds16, dedup_lds_bytes(a.words));
-        }
+        const int wm = dedup_wmax(a.words);
+        cls[wm == 20 ? 0 : (wm == 40 ? 1 : 2)] = true;
     }
     const int G = 64;  // blocks per stream of the node-list kernels
     k_planes_b<<<dim3((maxW + 63) / 64, maxH, n), 64, 0, st>>>(d_as);
-    if (r32) k_dedup_b<32><<<dim3(1, 1, n), FF_DD_THREADS, lds32, st>>>(d_as);
-    if (r16) k_dedup_b<16><<<dim3(1, 1, n), FF_DD_THREADS, lds16, st>>>(d_as);
+    if (cls[0]) k_dedup_b<20><<<dim3(1, 1, n), FF_DD_THREADS, dedup_lds_bytes(20), st>>>(d_as);
+    if (cls[1]) k_dedup_b<40><<<dim3(1, 1, n), FF_DD_THREADS, dedup_lds_bytes(40), st>>>(d_as);
+    if (cls[2]) k_dedup_b<64><<<dim3(1, 1, n), FF_DD_THREADS, dedup_lds_bytes(64), st>>>(d_as);
     k_descend_b<<<dim3((maxW + 127) / 128, maxH, n), 128, 0, st>>>(d_as);
     k_order_b<<<dim3(1, 1, n), FF_ORDER_THREADS, 0, st>>>(d_as);
     k_link_b<<<dim3(G, 1, n), 256, 0, st>>>(d_as);
