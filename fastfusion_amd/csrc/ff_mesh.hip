@@ -160,6 +160,11 @@ struct MeshState {
     int msCap = 0;
     uint64_t lastVAlloc = 0, lastFAlloc = 0;  // arena totals after the previous update (timing)
     bool haveVolume = false;
+    // the tree the current table was built for (used branch slots, bricks,
+    // grows): while it is unchanged the partition is, and an update re-marks
+    // the current table in place instead of building a new one
+    uint32_t builtNB = 0, builtNL = 0, builtGrows = 0;
+    bool reuse = false;  // the update in flight re-marks the current table
 };
 
 // ---------------------------------------------------------------------------
@@ -849,6 +854,55 @@ __global__ void k_mark_b(const MeshBatchStream *__restrict__ S)
               a.cellCap, a.m, a.tiles, a.tileCap);
 }
 
+// The table of the previous update re-marked in place, when the tree has not
+// changed since it was built: every cell then exists with the same bricks,
+// so it is clean exactly when none of its bricks' parent chains was updated
+// (k_mark's test with the previous table's own record).  A dirty cell's mesh
+// range is cleared first, like the fresh record k_cells writes: k_mc fills
+// it unless the new mesh is empty.
+__global__ void k_mark_same(CellRec *cells, const uint32_t *__restrict__ stamp, const uint32_t *__restrict__ parent,
+                            uint32_t since, MeshCounters *mc, CellGeom *geom, uint32_t cap, MapRO m, uint32_t *tiles,
+                            uint32_t tileCap)
+{
+    const uint32_t nc = min(mc->nCells, cap);
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t ib = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); ib < nc; ib += stride) {
+        const uint32_t i = ib + lane;
+        bool isDirty = false;
+        CellGeom g;
+        uint32_t nt = 0;
+        if (i < nc) {
+            CellRec &c = cells[i];
+            uint32_t ll[8];
+            for (int k = 0; k < 8; k++) ll[k] = c.ll[k];
+            isDirty = chains_dirty(stamp, parent, ll, since);
+            if (isDirty) {
+                c.v0 = c.nv = c.f0 = c.nf = 0u;
+                cell_geom(m, c.key, ll, g);
+                g.cell = i;
+                g.pad = 0;
+                nt = ((uint32_t)geom_total(g) + 63u) >> 6;
+            }
+        }
+        const uint32_t d = wave_append(&mc->nDirty, isDirty);
+        uint32_t ntot;
+        const uint32_t ex = wave_excl_scan_u32(nt, &ntot);
+        uint32_t tb = 0;
+        if (ntot) {
+            if (lane == 0) tb = atomicAdd(&mc->nTiles, ntot);
+            tb = __shfl(tb, 0);
+        }
+        if (isDirty) {
+            const uint32_t t0 = tb + ex;
+            const bool fits = t0 + nt <= tileCap;
+            g.tile0 = nt == 0 ? FF_EMPTY_CELL : (fits ? t0 : FF_NO_TILE);
+            for (uint32_t k = 0; k < nt && t0 + k < tileCap; k++) tiles[t0 + k] = fits ? ((d << 3) | k) : FF_NO_TILE;
+            geom[d] = g;
+        }
+    }
+}
+
 // arena (when not null): block 0 also writes the totals of the arena
 // counters into mc->pad (vertices, faces allocated since the last compaction)
 __device__ __forceinline__ void hash_build_body(CellRec *cells, MeshCounters *mc, uint32_t cap, uint64_t *hkeys,
@@ -1529,6 +1583,18 @@ __global__ void k_mesh_begin(MeshCounters *mc)
     }
 }
 
+// the counters of an update that re-marks the current table of nCells cells
+__global__ void k_mesh_begin_same(MeshCounters *mc, uint32_t nCells)
+{
+    if (threadIdx.x == 0) {
+        MeshCounters z;
+        z.nTerm = z.nDirty = z.flags = z.nTiles = 0u;
+        z.nCells = nCells;
+        z.pad[0] = z.pad[1] = z.pad[2] = 0u;
+        *mc = z;
+    }
+}
+
 __global__ void k_mesh_begin_b(const MeshBatchStream *__restrict__ S)
 {
     if (threadIdx.x == 0) {
@@ -1582,11 +1648,13 @@ static Pools pools(const MeshState *s)
 // post -- lookup hash, counters' copy back; finish -- after the device work.
 // host part of pre: the table to build and the map view (false: nothing
 // to mesh)
-static bool mesh_pre_host(ff_ctx *c)
+static bool mesh_pre_host(ff_ctx *c, bool allowReuse)
 {
     MeshState *s = c->mesh;
     if (!s->haveVolume || !c->n) return false;
-    s->nxt = 1 - s->cur;
+    s->reuse = allowReuse && s->nCells[s->cur] > 0 && s->builtNB == c->nBranchesUsed &&
+               s->builtNL == c->nLeavesUsed && s->builtGrows == c->grows;
+    s->nxt = s->reuse ? s->cur : 1 - s->cur;
     MapRO m;
     m.info = c->d_leafinfo; m.parent = c->d_parent; m.dist = c->d_dist; m.wgt = c->d_wgt; m.col = c->d_col;
     for (int k = 0; k < 3; k++) m.off[k] = c->offset[k];
@@ -1599,9 +1667,20 @@ static bool mesh_pre_host(ff_ctx *c)
 bool mesh_launch_pre(ff_ctx *c, bool timed)
 {
     MeshState *s = c->mesh;
-    if (!mesh_pre_host(c)) return false;
+    if (!mesh_pre_host(c, true)) return false;
     if (timed && c->timing) hipEventRecord(c->ev[6], c->stream);
     const int nxt = s->nxt;
+    if (s->reuse) {
+        const MapRO m = s->m;
+        const uint32_t nc = std::min(s->nCells[s->cur], s->cellCap);
+        k_mesh_begin_same<<<1, 64, 0, c->stream>>>(s->d_mc, nc);
+        k_mark_same<<<std::max(1u, std::min(4096u, (nc + 255) / 256)), 256, 0, c->stream>>>(
+            s->d_cells[nxt], c->d_stamp, c->d_parent, s->lastMeshStamp, s->d_mc, s->d_geom, s->cellCap, m, s->d_tiles,
+            s->tileCap);
+        k_sample<<<4096, 256, 0, c->stream>>>(s->d_geom, s->d_cells[nxt], s->d_tiles, s->d_mc, s->tileCap, m,
+                                              s->d_sbuf);
+        return true;
+    }
     k_mesh_begin<<<1, 64, 0, c->stream>>>(s->d_mc);
     const uint32_t nBlocks = c->nBranchesUsed / 16;
     k_terminal<<<(nBlocks * 8 + 255) / 256, 256, 0, c->stream>>>(c->d_tree, s->d_binfo, nBlocks, c->nLeavesTotal, c->n,
@@ -1680,11 +1759,16 @@ void mesh_launch_post(ff_ctx *c, bool timed)
 {
     MeshState *s = c->mesh;
     const int nxt = s->nxt;
-    // the previous table's keys leave the lookup hash, the new ones enter it
-    if (s->nCells[s->cur])
-        k_hash_clear_cells<<<1024, 256, 0, c->stream>>>(s->d_cells[s->cur], s->nCells[s->cur], s->d_hkeys);
-    k_hash_build<<<1024, 256, 0, c->stream>>>(s->d_cells[nxt], s->d_mc, s->cellCap, s->d_hkeys, s->d_hval, s->hmask,
-                                              c->timing ? s->d_arena : nullptr);
+    if (s->reuse) {
+        // same table, same keys: the lookup hash stands
+        if (c->timing) k_arena_sum<<<1, 64, 0, c->stream>>>(s->d_arena, s->d_mc);
+    } else {
+        // the previous table's keys leave the lookup hash, the new ones enter it
+        if (s->nCells[s->cur])
+            k_hash_clear_cells<<<1024, 256, 0, c->stream>>>(s->d_cells[s->cur], s->nCells[s->cur], s->d_hkeys);
+        k_hash_build<<<1024, 256, 0, c->stream>>>(s->d_cells[nxt], s->d_mc, s->cellCap, s->d_hkeys, s->d_hval,
+                                                  s->hmask, c->timing ? s->d_arena : nullptr);
+    }
     if (timed && c->timing) hipEventRecord(c->ev[7], c->stream);
     hipMemcpyAsync(s->h_mc, s->d_mc, sizeof(MeshCounters), hipMemcpyDeviceToHost, c->stream);
 }
@@ -1735,6 +1819,11 @@ int mesh_finish(ff_ctx *c, bool timed)
     s->cur = nxt;
     s->nCells[nxt] = std::min(h.nCells, s->cellCap);
     s->lastMeshStamp = c->stampCounter;
+    if (!s->reuse) {
+        s->builtNB = c->nBranchesUsed;
+        s->builtNL = c->nLeavesUsed;
+        s->builtGrows = c->grows;
+    }
     c->phaseUnits[2] += h.nDirty;
     // algorithmic bytes: the 512 source voxels (14 B) of each re-meshed cell,
     // 15 B per vertex and 12 B per triangle written (vertex and face counts
@@ -1995,7 +2084,7 @@ int ff_update_meshes_batch(ff_ctx *const *ctxs, int n)
     }
     std::vector<int> act;
     for (int i = 0; i < n; i++)
-        if (ctxs[i]->incremental && mesh_pre_host(ctxs[i])) act.push_back(i);
+        if (ctxs[i]->incremental && mesh_pre_host(ctxs[i], false)) act.push_back(i);
     const int na = (int)act.size();
     if (na) {
         // one upload of all descriptors, one launch per kernel, one copy back

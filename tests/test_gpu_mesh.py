@@ -90,3 +90,30 @@ def test_mesh_sample_buffer_overflow():
     ncell, nface = run_mesh(sc, R, t, list(range(0, 6)), cfg, (262.5, 262.5, 159.5, 119.5), check_every=1,
                             sample_tiles=8)
     assert nface > 1000
+
+
+def test_mesh_unchanged_tree_remarks_in_place():
+    """Frames that create no brick or branch leave the cell partition as it
+    was: the update then re-marks the previous table in place.  Re-fusing
+    frames already seen guarantees such updates; the meshes must still equal
+    the oracle's from-scratch meshes, including cells whose new mesh is
+    empty."""
+    sc, R, t, cfg = synth.config_c2(n_frames=60)
+    cfg = dict(cfg, w=320, h=240)
+    K = (262.5, 262.5, 159.5, 119.5)
+    gpu, ora = make_pair(cfg["scale"], cfg["threshold"])
+    frames = list(range(0, 6)) + [5, 5, 3, 4, 5, 2]
+    unchanged = 0
+    prev = None
+    for k, i in enumerate(frames):
+        d, bgr = sc.render(R[i].reshape(3, 3), t[i], K, 320, 240, noise=cfg["noise"], dropout=cfg["dropout"],
+                           seed=1000 + i + (100 * k if k >= 6 else 0))
+        assert gpu.addMap(d, R[i], t[i], K, bgr, 1 / 5000.0, cfg["maxcam"]) == \
+            ora.add_map(d, R[i], t[i], K, bgr, 1 / 5000.0, cfg["maxcam"])
+        c = gpu.counts()
+        key = (c["n_leaves_used"], c["n_branches_used"], c["grows"])
+        unchanged += key == prev
+        prev = key
+        gpu.updateMeshes()
+        compare_mesh(gpu, ora, cfg["scale"], f"step {k} (frame {i})")
+    assert unchanged >= 3
