@@ -359,8 +359,9 @@ class FusionMipMapGPU:
         """Replay each frame's launches from captured HIP graphs, or launch directly (default)."""
         self._check(self._lib.ff_set_graph_capture(self._ctx, int(bool(on))))
 
-    def set_timing(self, on=True):
-        self._check(self._lib.ff_set_timing(self._ctx, int(bool(on))))
+    def set_timing(self, on=True, sdf_only=False):
+        """HIP-event timing of every phase, or (sdf_only) of the SDF update alone."""
+        self._check(self._lib.ff_set_timing(self._ctx, (2 if sdf_only else 1) if on else 0))
 
     def reset_timing(self):
         self._check(self._lib.ff_reset_timing(self._ctx))

@@ -207,6 +207,12 @@ int ff_add_map_batch(ff_ctx *const *ctxs, const ff_frame *frames, int n)
     }
     ff_ctx *lead = ctxs[0];
     hipSetDevice(lead->device);
+    // a context's own stream may still hold its last mesh update: the
+    // batch runs on the lead's stream
+    for (int i = 0; i < n; i++) {
+        const int rc = mesh_complete(ctxs[i]);
+        if (rc != FF_OK) return rc;
+    }
     BatchState *b = batch_for(ctxs, n);
     if (!b) {
         lead->err = "ff_add_map_batch: out of device memory";

@@ -1685,10 +1685,10 @@ static int run_frame(ff_ctx *c, int W, int H)
         }
     }
     const bool g = c->useGraphs && c->frameExec[0];
-    if (c->timing) hipEventRecord(c->ev[0], c->stream);
+    if (c->timingAll) hipEventRecord(c->ev[0], c->stream);
     if (g) HIPCHK(hipGraphLaunch(c->frameExec[0], c->stream));
     else launch_alloc(c, W, H);
-    if (c->timing) hipEventRecord(c->ev[1], c->stream);
+    if (c->timingAll) hipEventRecord(c->ev[1], c->stream);
     // the SDF phase's events bracket k_sdf alone when launched directly
     if (!g && c->learned) learned_launch(c, W, H);
     if (c->timing) hipEventRecord(c->ev[2], c->stream);
@@ -1786,7 +1786,7 @@ int frame_finish(ff_ctx *c, int W, int H, bool timed)
     FrameCounters *hf = c->h_fc;
     if (timed && c->timing && c->n) {
         float ms = 0.0f;
-        if (hipEventElapsedTime(&ms, c->ev[0], c->ev[1]) == hipSuccess) {
+        if (c->timingAll && hipEventElapsedTime(&ms, c->ev[0], c->ev[1]) == hipSuccess) {
             c->phaseMs[0] += ms;
             c->phaseCalls[0]++;
         }
@@ -1924,16 +1924,21 @@ int ff_set_graph_capture(ff_ctx *c, int on)
 
 int ff_set_timing(ff_ctx *c, int on)
 {
-    if (!c) return FF_E_ARG;
+    if (!c || on < 0 || on > 2) return FF_E_ARG;
+    const int rc = mesh_complete(c);
+    if (rc != FF_OK) return rc;
     if (on && !c->ev[0])
         for (int i = 0; i < 8; i++) HIPCHK(hipEventCreate(&c->ev[i]));
     c->timing = on != 0;
+    c->timingAll = on == 1;
     return FF_OK;
 }
 
 int ff_get_timing(ff_ctx *c, double *ms, uint64_t *calls, double *bytes, uint64_t *units)
 {
     if (!c) return FF_E_ARG;
+    const int rc = mesh_complete(c);
+    if (rc != FF_OK) return rc;
     HIPCHK(hipStreamSynchronize(c->stream));
     for (int i = 0; i < 3; i++) {
         if (ms) ms[i] = c->phaseMs[i];
@@ -1947,6 +1952,8 @@ int ff_get_timing(ff_ctx *c, double *ms, uint64_t *calls, double *bytes, uint64_
 int ff_reset_timing(ff_ctx *c)
 {
     if (!c) return FF_E_ARG;
+    const int rc = mesh_complete(c);  // its counts belong before the reset
+    if (rc != FF_OK) return rc;
     for (int i = 0; i < 3; i++) {
         c->phaseMs[i] = 0.0;
         c->phaseCalls[i] = 0;

@@ -160,7 +160,8 @@ struct ff_ctx {
     ff::BatchState *batch = nullptr;      // batched ticks led by this context
     std::string err;
     // per-phase timing (0 allocation, 1 SDF update, 2 meshing)
-    bool timing = false;
+    bool timing = false;     // events around the SDF update
+    bool timingAll = false;  // and around the allocation and meshing phases
     hipEvent_t ev[8] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     double phaseMs[3] = {0, 0, 0}, phaseBytes[3] = {0, 0, 0};
     uint64_t phaseCalls[3] = {0, 0, 0}, phaseUnits[3] = {0, 0, 0};
@@ -207,6 +208,9 @@ void launch_gather_fc(hipStream_t st, const SdfStream *d_ss, int n, FrameCounter
 void batch_free(ff_ctx *c);
 // meshing module hooks (ff_mesh.hip)
 int mesh_init(ff_ctx *c);
+// the host part of an enqueued mesh update (a no-op when none is pending):
+// every mesh query and every change of the map's structure calls it first
+int mesh_complete(ff_ctx *c);
 void mesh_free(ff_ctx *c);
 void mesh_on_initial_volume(ff_ctx *c);
 int mesh_on_load(ff_ctx *c, const std::vector<ushort4> &binfo);

@@ -165,6 +165,13 @@ struct MeshState {
     // the current table in place instead of building a new one
     uint32_t builtNB = 0, builtNL = 0, builtGrows = 0;
     bool reuse = false;  // the update in flight re-marks the current table
+    // an update whose device work is enqueued and whose host part
+    // (mesh_finish) runs at the next mesh call, query or state change
+    bool pending = false, pendingTimed = false;
+    bool inPlaceDone = false;  // its in-place marching cubes ran on the device
+    // the map the update in flight meshes: the frame stamp and the tree
+    // (frames fused before its host part runs must not be attributed to it)
+    uint32_t launchStamp = 0, launchNB = 0, launchNL = 0, launchGrows = 0;
 };
 
 // ---------------------------------------------------------------------------
@@ -1298,11 +1305,13 @@ __device__ void mc_cell(WaveLds &L, int lane, CellRec *cells, const CellGeom &g,
 
 template <bool InPlace>
 __global__ __launch_bounds__(256) void k_mc(CellRec *cells, const CellGeom *__restrict__ geom, MeshCounters *mc,
-                                            const uint4 *__restrict__ sbuf, MapRO m, Pools P)
+                                            const uint4 *__restrict__ sbuf, MapRO m, Pools P, uint32_t tileCap)
 {
     __shared__ WaveLds lds[4];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     WaveLds &L = lds[wave];
+    // the in-place pass has work only when some cell's samples did not fit
+    if (InPlace && mc->nTiles <= tileCap) return;
     const uint32_t nd = mc->nDirty;
     for (uint32_t base = blockIdx.x * 4; base < nd; base += gridDim.x * 4) {
         // waves of a block run independently: each owns lds[wave]
@@ -1508,6 +1517,7 @@ void mesh_free(ff_ctx *c)
 void mesh_on_initial_volume(ff_ctx *c)
 {
     MeshState *s = c->mesh;
+    mesh_complete(c);
     const ushort4 root = make_ushort4(0, 0, 0, (unsigned short)(c->n >> 1));
     hipMemcpyAsync(s->d_binfo, &root, sizeof root, hipMemcpyHostToDevice, c->stream);
     hipStreamSynchronize(c->stream);
@@ -1519,6 +1529,7 @@ void mesh_on_initial_volume(ff_ctx *c)
 int mesh_on_load(ff_ctx *c, const std::vector<ushort4> &binfo)
 {
     MeshState *s = c->mesh;
+    mesh_complete(c);
     const size_t nb = std::min<size_t>(binfo.size(), s->binfoCap);
     if (nb) MCHK(hipMemcpyAsync(s->d_binfo, binfo.data(), sizeof(ushort4) * nb, hipMemcpyHostToDevice, c->stream));
     MCHK(hipMemsetAsync(s->d_hkeys, 0xff, sizeof(uint64_t) * (s->hmask + 1), c->stream));
@@ -1536,6 +1547,7 @@ void mesh_on_grow_blocks(ff_ctx *c, uint32_t oldBlocks, uint32_t addBlocks, cons
                          const std::vector<ushort4> &chain)
 {
     MeshState *s = c->mesh;
+    mesh_complete(c);  // its table is the one the grow discards
     ushort4 *tmp = nullptr;
     if (hipMalloc(&tmp, sizeof(ushort4) * (oldBlocks + 1)) != hipSuccess) return;
     hipMemcpyAsync(tmp, s->d_binfo, sizeof(ushort4) * oldBlocks, hipMemcpyDeviceToDevice, c->stream);
@@ -1654,6 +1666,10 @@ static bool mesh_pre_host(ff_ctx *c, bool allowReuse)
     if (!s->haveVolume || !c->n) return false;
     s->reuse = allowReuse && s->nCells[s->cur] > 0 && s->builtNB == c->nBranchesUsed &&
                s->builtNL == c->nLeavesUsed && s->builtGrows == c->grows;
+    s->launchStamp = c->stampCounter;
+    s->launchNB = c->nBranchesUsed;
+    s->launchNL = c->nLeavesUsed;
+    s->launchGrows = c->grows;
     s->nxt = s->reuse ? s->cur : 1 - s->cur;
     MapRO m;
     m.info = c->d_leafinfo; m.parent = c->d_parent; m.dist = c->d_dist; m.wgt = c->d_wgt; m.col = c->d_col;
@@ -1668,7 +1684,7 @@ bool mesh_launch_pre(ff_ctx *c, bool timed)
 {
     MeshState *s = c->mesh;
     if (!mesh_pre_host(c, true)) return false;
-    if (timed && c->timing) hipEventRecord(c->ev[6], c->stream);
+    if (timed && c->timingAll) hipEventRecord(c->ev[6], c->stream);
     const int nxt = s->nxt;
     if (s->reuse) {
         const MapRO m = s->m;
@@ -1725,7 +1741,7 @@ static MeshBatchStream mesh_batch_stream(ff_ctx *c)
     a.tileCap = s->tileCap;
     a.sbuf = s->d_sbuf;
     a.arena = s->d_arena;
-    a.arenaSum = c->timing ? 1 : 0;
+    a.arenaSum = c->timingAll ? 1 : 0;
     return a;
 }
 
@@ -1752,7 +1768,19 @@ McStream mesh_stream(ff_ctx *c)
 void mesh_launch_mc(ff_ctx *c)
 {
     MeshState *s = c->mesh;
-    k_mc<false><<<2048, 256, 0, c->stream>>>(s->d_cells[s->nxt], s->d_geom, s->d_mc, s->d_sbuf, s->m, pools(s));
+    k_mc<false><<<2048, 256, 0, c->stream>>>(s->d_cells[s->nxt], s->d_geom, s->d_mc, s->d_sbuf, s->m, pools(s),
+                                             s->tileCap);
+}
+
+// the cells whose samples did not fit the buffer, meshed in place on the
+// device (every block returns at once when all fitted), so that the update's
+// meshes never wait for the host
+static void mesh_launch_inplace(ff_ctx *c)
+{
+    MeshState *s = c->mesh;
+    k_mc<true><<<2048, 256, 0, c->stream>>>(s->d_cells[s->nxt], s->d_geom, s->d_mc, s->d_sbuf, s->m, pools(s),
+                                            s->tileCap);
+    s->inPlaceDone = true;
 }
 
 void mesh_launch_post(ff_ctx *c, bool timed)
@@ -1761,15 +1789,15 @@ void mesh_launch_post(ff_ctx *c, bool timed)
     const int nxt = s->nxt;
     if (s->reuse) {
         // same table, same keys: the lookup hash stands
-        if (c->timing) k_arena_sum<<<1, 64, 0, c->stream>>>(s->d_arena, s->d_mc);
+        if (c->timingAll) k_arena_sum<<<1, 64, 0, c->stream>>>(s->d_arena, s->d_mc);
     } else {
         // the previous table's keys leave the lookup hash, the new ones enter it
         if (s->nCells[s->cur])
             k_hash_clear_cells<<<1024, 256, 0, c->stream>>>(s->d_cells[s->cur], s->nCells[s->cur], s->d_hkeys);
         k_hash_build<<<1024, 256, 0, c->stream>>>(s->d_cells[nxt], s->d_mc, s->cellCap, s->d_hkeys, s->d_hval,
-                                                  s->hmask, c->timing ? s->d_arena : nullptr);
+                                                  s->hmask, c->timingAll ? s->d_arena : nullptr);
     }
-    if (timed && c->timing) hipEventRecord(c->ev[7], c->stream);
+    if (timed && c->timingAll) hipEventRecord(c->ev[7], c->stream);
     hipMemcpyAsync(s->h_mc, s->d_mc, sizeof(MeshCounters), hipMemcpyDeviceToHost, c->stream);
 }
 
@@ -1780,19 +1808,24 @@ int mesh_finish(ff_ctx *c, bool timed)
     const int nxt = s->nxt;
     const MapRO m = s->m;
     const Pools P = pools(s);
-    if (timed && c->timing) {
+    if (timed && c->timingAll) {
         float ms = 0.0f;
         if (hipEventElapsedTime(&ms, c->ev[6], c->ev[7]) == hipSuccess) {
             c->phaseMs[2] += ms;
             c->phaseCalls[2]++;
         }
     }
+    const bool inPlaceDone = s->inPlaceDone;
+    s->inPlaceDone = false;
     if (s->h_mc->nTiles > s->tileCap && !(s->h_mc->flags & FF_CELL_OVERFLOW)) {
-        // some cells' samples did not fit the buffer: mesh them in place,
-        // then grow the buffer for the next updates
-        k_mc<true><<<2048, 256, 0, c->stream>>>(s->d_cells[nxt], s->d_geom, s->d_mc, s->d_sbuf, m, P);
-        MCHK(hipMemcpyAsync(s->h_mc, s->d_mc, sizeof(MeshCounters), hipMemcpyDeviceToHost, c->stream));
-        MCHK(hipStreamSynchronize(c->stream));
+        // some cells' samples did not fit the buffer: mesh them in place
+        // (unless the update did on the device), then grow the buffer for
+        // the next updates
+        if (!inPlaceDone) {
+            k_mc<true><<<2048, 256, 0, c->stream>>>(s->d_cells[nxt], s->d_geom, s->d_mc, s->d_sbuf, m, P, s->tileCap);
+            MCHK(hipMemcpyAsync(s->h_mc, s->d_mc, sizeof(MeshCounters), hipMemcpyDeviceToHost, c->stream));
+            MCHK(hipStreamSynchronize(c->stream));
+        }
         uint32_t cap = s->tileCap;
         while (cap < s->h_mc->nTiles && cap < FF_TILE_CAP_MAX) cap <<= 1;
         if (cap > s->tileCap) {
@@ -1818,17 +1851,17 @@ int mesh_finish(ff_ctx *c, bool timed)
     }
     s->cur = nxt;
     s->nCells[nxt] = std::min(h.nCells, s->cellCap);
-    s->lastMeshStamp = c->stampCounter;
+    s->lastMeshStamp = s->launchStamp;
     if (!s->reuse) {
-        s->builtNB = c->nBranchesUsed;
-        s->builtNL = c->nLeavesUsed;
-        s->builtGrows = c->grows;
+        s->builtNB = s->launchNB;
+        s->builtNL = s->launchNL;
+        s->builtGrows = s->launchGrows;
     }
     c->phaseUnits[2] += h.nDirty;
     // algorithmic bytes: the 512 source voxels (14 B) of each re-meshed cell,
     // 15 B per vertex and 12 B per triangle written (vertex and face counts
     // from the arena totals, which only grow between compactions)
-    if (c->timing) {
+    if (c->timingAll) {
         const uint64_t va = h.pad[0], fa = h.pad[1];
         const uint64_t dv = va >= s->lastVAlloc ? va - s->lastVAlloc : va, df = fa >= s->lastFAlloc ? fa - s->lastFAlloc : fa;
         c->phaseBytes[2] += (double)h.nDirty * 512.0 * 14.0 + 15.0 * (double)dv + 12.0 * (double)df;
@@ -1850,12 +1883,12 @@ int mesh_finish(ff_ctx *c, bool timed)
         dst.v = nv;
         dst.f = nf;
         k_compact<<<1024, 256, 0, c->stream>>>(s->d_cells[nxt], s->d_mc, s->cellCap, s->d_vpool, s->d_fpool, dst);
-        if (c->timing) {
+        if (c->timingAll) {
             k_arena_sum<<<1, 64, 0, c->stream>>>(s->d_arena, s->d_mc);
             MCHK(hipMemcpyAsync(s->h_mc, s->d_mc, sizeof(MeshCounters), hipMemcpyDeviceToHost, c->stream));
         }
         MCHK(hipStreamSynchronize(c->stream));
-        if (c->timing) {
+        if (c->timingAll) {
             s->lastVAlloc = s->h_mc->pad[0];
             s->lastFAlloc = s->h_mc->pad[1];
         }
@@ -1872,20 +1905,38 @@ int mesh_finish(ff_ctx *c, bool timed)
     return (int)nDirty;
 }
 
+// the host part of an enqueued update, once its device work completed
+int mesh_complete(ff_ctx *c)
+{
+    MeshState *s = c->mesh;
+    if (!s || !s->pending) return FF_OK;
+    s->pending = false;
+    MCHK(hipStreamSynchronize(c->stream));
+    const int r = mesh_finish(c, s->pendingTimed);
+    return r < 0 ? r : FF_OK;
+}
+
+// enqueues the update; the previous one's host part runs first
 static int update_meshes(ff_ctx *c)
 {
-    if (!mesh_launch_pre(c, true)) return 0;
+    const int rc = mesh_complete(c);
+    if (rc != FF_OK) return rc;
+    if (!mesh_launch_pre(c, true)) return FF_OK;
     mesh_launch_mc(c);
+    mesh_launch_inplace(c);
     mesh_launch_post(c, true);
     MCHK(hipGetLastError());
-    MCHK(hipStreamSynchronize(c->stream));
-    return mesh_finish(c, true);
+    c->mesh->pending = true;
+    c->mesh->pendingTimed = true;
+    return FF_OK;
 }
 
 // host copy of the current cell table, sorted by key
 static int host_cells(ff_ctx *c, std::vector<CellRec> &out)
 {
     MeshState *s = c->mesh;
+    const int rc = mesh_complete(c);
+    if (rc != FF_OK) return rc;
     out.resize(std::min(s->nCells[s->cur], s->cellCap));
     MCHK(hipStreamSynchronize(c->stream));
     if (!out.empty())
@@ -1984,6 +2035,8 @@ int ff_copy_mesh(ff_ctx *c, float *xyz, uint8_t *rgb, uint32_t *faces)
 int ff_cell_count(ff_ctx *c, size_t *n)
 {
     if (!c || !n) return FF_E_ARG;
+    const int rc = mesh_complete(c);
+    if (rc != FF_OK) return rc;
     *n = c->mesh->nCells[c->mesh->cur];
     return FF_OK;
 }
@@ -2012,6 +2065,8 @@ int ff_copy_cells(ff_ctx *c, int32_t *desc5, uint32_t *nv, uint32_t *nf)
 int ff_set_incremental_meshing(ff_ctx *c, int on)
 {
     if (!c) return FF_E_ARG;
+    const int rc = mesh_complete(c);
+    if (rc != FF_OK) return rc;
     c->incremental = on != 0;
     return FF_OK;
 }
@@ -2024,6 +2079,8 @@ int ff_set_mesh_sample_tiles(ff_ctx *c, uint32_t tiles)
     }
     hipSetDevice(c->device);
     MeshState *s = c->mesh;
+    const int rc = mesh_complete(c);
+    if (rc != FF_OK) return rc;
     MCHK(hipStreamSynchronize(c->stream));
     uint32_t *nt = nullptr;
     uint4 *nb = nullptr;
@@ -2051,6 +2108,10 @@ int ff_update_meshes_batch(ff_ctx *const *ctxs, int n)
     }
     ff_ctx *lead = ctxs[0];
     hipSetDevice(lead->device);
+    for (int i = 0; i < n; i++) {
+        const int rc = mesh_complete(ctxs[i]);
+        if (rc != FF_OK) return rc;
+    }
     MeshState *ls = lead->mesh;
     if (ls->msCap < n) {
         hipFree(ls->d_mdesc); hipHostFree(ls->h_mdesc); hipFree(ls->d_msPrefix);

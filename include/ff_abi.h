@@ -74,9 +74,13 @@ int ff_add_map_device(ff_ctx *ctx, const uint16_t *depth_dev, const uint8_t *bgr
                       int height, const double R[9], const double t[3], const double K[4],
                       float scaling, float maxcam);
 
-/* FusionMipMapCPU::updateMeshes (geometryfusion_mipmap_cpu.cpp:2884-2954), run
- * synchronously: re-meshes the cells touched by the frames since the last
- * call.  Returns the number of cells re-meshed or FF_E*. */
+/* FusionMipMapCPU::updateMeshes (geometryfusion_mipmap_cpu.cpp:2884-2954):
+ * re-meshes the cells touched by the frames since the last call.  The work
+ * is enqueued on the context's stream and the call returns (the reference's
+ * updateMeshes returns true; its meshing thread also runs behind the
+ * caller); every mesh query below waits for it and sees its result.  Its
+ * host bookkeeping runs at the next mesh call or query, which also returns
+ * its errors (FF_E_POOL_FULL: cell table overflow).  Returns FF_OK or FF_E*. */
 int ff_update_meshes(ff_ctx *ctx);
 
 /* getMeshInterleavedMarchingCubes (geometryfusion_mipmap_cpu.cpp:3359-3374):
@@ -120,8 +124,10 @@ int ff_copy_processed(ff_ctx *ctx, uint8_t *out, size_t n);
 
 /* Per-phase GPU timing (no counterpart in the reference, which prints
  * cv::getTickCount averages, geometryfusion_mipmap_cpu.cpp:979-991).
- * When on, each frame records HIP events around its phases on the context
- * stream: 0 = allocation kernels, 1 = SDF update kernel, 2 = meshing.
+ * ff_set_timing(ctx, 1) records HIP events around each frame's phases on
+ * the context stream: 0 = allocation kernels, 1 = SDF update kernel,
+ * 2 = meshing; ff_set_timing(ctx, 2) only around the SDF update (the least
+ * event traffic: each event record costs launch time); 0 turns it off.
  * ff_get_timing returns per phase the summed milliseconds, the number of
  * timed launches, the summed algorithmic bytes and work units (processed
  * pixels, queued bricks, re-meshed cells); arrays have 3 entries. */
