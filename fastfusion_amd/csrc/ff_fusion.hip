@@ -1455,6 +1455,13 @@ ff_ctx *ff_create(const ff_params *prm)
         delete c;
         return nullptr;
     }
+    if (hipStreamCreate(&c->side) != hipSuccess || hipEventCreateWithFlags(&c->evFront, hipEventDisableTiming) != hipSuccess) {
+        // the front then runs on the main stream
+        if (c->side) hipStreamDestroy(c->side);
+        c->side = nullptr;
+        c->evFront = nullptr;
+        (void)hipGetLastError();
+    }
     const size_t NL = c->nLeavesTotal;
     bool ok = hipMalloc(&c->d_tree, sizeof(uint32_t) * c->nBranchesTotal) == hipSuccess &&
               hipMalloc(&c->d_treeTmp, sizeof(uint32_t) * c->nBranchesTotal) == hipSuccess &&
@@ -1524,6 +1531,11 @@ void ff_destroy(ff_ctx *c)
     hipFree(c->hv.keys); hipFree(c->hv.minpix); hipFree(c->hv.aux); hipFree(c->hv.assigned); hipFree(c->hv.list);
     for (int i = 0; i < 8; i++)
         if (c->ev[i]) hipEventDestroy(c->ev[i]);
+    if (c->side) {
+        hipStreamSynchronize(c->side);
+        hipStreamDestroy(c->side);
+    }
+    if (c->evFront) hipEventDestroy(c->evFront);
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;
 }
@@ -1540,17 +1552,28 @@ namespace ff {
 // it can be captured once and replayed as graphs: the allocation (ending
 // with the commit of the used counts), then the learned confidence when on,
 // the gated SDF update and the counters' copy back.
+void launch_front(ff_ctx *c, int W, int H, hipStream_t st)
+{
+    const int words = (W + 31) / 32;
+    k_planes<<<dim3((W + 63) / 64, H), 64, 0, st>>>(c->d_fd, W, H, words, c->d_pxp, c->d_pyp, c->d_planes, c->d_fc);
+    const size_t lds = dedup_lds_bytes(words);
+    switch (dedup_wmax(words)) {
+    case 20: k_dedup<20><<<1, FF_DD_THREADS, lds, st>>>(c->d_planes, H, words, c->d_pbits, c->d_fc); break;
+    case 40: k_dedup<40><<<1, FF_DD_THREADS, lds, st>>>(c->d_planes, H, words, c->d_pbits, c->d_fc); break;
+    default: k_dedup<64><<<1, FF_DD_THREADS, lds, st>>>(c->d_planes, H, words, c->d_pbits, c->d_fc); break;
+    }
+}
+
 void launch_alloc(ff_ctx *c, int W, int H)
+{
+    launch_front(c, W, H, c->stream);
+    launch_back(c, W, H);
+}
+
+void launch_back(ff_ctx *c, int W, int H)
 {
     const int words = (W + 31) / 32;
     const FrameDev *fd = c->d_fd;
-    k_planes<<<dim3((W + 63) / 64, H), 64, 0, c->stream>>>(fd, W, H, words, c->d_pxp, c->d_pyp, c->d_planes, c->d_fc);
-    const size_t lds = dedup_lds_bytes(words);
-    switch (dedup_wmax(words)) {
-    case 20: k_dedup<20><<<1, FF_DD_THREADS, lds, c->stream>>>(c->d_planes, H, words, c->d_pbits, c->d_fc); break;
-    case 40: k_dedup<40><<<1, FF_DD_THREADS, lds, c->stream>>>(c->d_planes, H, words, c->d_pbits, c->d_fc); break;
-    default: k_dedup<64><<<1, FF_DD_THREADS, lds, c->stream>>>(c->d_planes, H, words, c->d_pbits, c->d_fc); break;
-    }
     const MapView mv = c->mapView();
     k_descend<<<dim3((W + 127) / 128, H), 128, 0, c->stream>>>(fd, W, H, words, c->d_pxp, c->d_pyp, c->d_pbits, mv,
                                                                c->hv, c->d_pl0, c->d_fc);
@@ -1670,8 +1693,33 @@ static bool capture(ff_ctx *c, void (*fn)(ff_ctx *, int, int), int W, int H, hip
 
 // the frame's two launch sequences, from graphs captured for this
 // configuration (timing events are recorded between them)
-static int run_frame(ff_ctx *c, int W, int H)
+// whether a frame's front may run on the side stream: direct launches (a
+// captured frame replays on one stream), no per-phase events (they would
+// time the overlap), an existing volume and image tables of this size (the
+// first frame and a new size prepare on the main stream)
+static bool front_overlap(const ff_ctx *c, int W, int H)
 {
+    return c->side && c->evFront && !c->useGraphs && !c->timingAll && c->n && c->imgW == W && c->imgH == H;
+}
+
+static int run_frame(ff_ctx *c, int W, int H, bool overlap)
+{
+    if (overlap) {
+        // the front beside the meshing still in flight on the main stream;
+        // the descent waits for it
+        HIPCHK(hipMemcpyAsync(c->d_fd, c->h_fd, sizeof(FrameDev), hipMemcpyHostToDevice, c->side));
+        launch_front(c, W, H, c->side);
+        HIPCHK(hipEventRecord(c->evFront, c->side));
+        HIPCHK(hipStreamWaitEvent(c->stream, c->evFront, 0));
+        launch_back(c, W, H);
+        if (c->learned) learned_launch(c, W, H);
+        if (c->timing) hipEventRecord(c->ev[2], c->stream);
+        launch_sdf_only(c, W, H);
+        if (c->timing) hipEventRecord(c->ev[3], c->stream);
+        HIPCHK(hipMemcpyAsync(c->h_fc, c->d_fc, sizeof(FrameCounters), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(hipGetLastError());
+        return FF_OK;
+    }
     const uint64_t key = ((uint64_t)W << 48) ^ ((uint64_t)H << 32) ^ ((uint64_t)c->bufGen << 1) ^
                          (uint64_t)(c->learned != nullptr);
     if (c->useGraphs && (!c->frameExec[0] || c->frameKey != key)) {
@@ -1859,12 +1907,12 @@ int frame_finish(ff_ctx *c, int W, int H, bool timed)
 }
 
 static int add_map_impl(ff_ctx *c, const uint16_t *depth, const uint8_t *bgr, int W, int H, const double R[9],
-                        const double t[3], const double K[4], float scaling, float maxcam)
+                        const double t[3], const double K[4], float scaling, float maxcam, bool overlap)
 {
-    int rc = frame_prepare(c, depth, bgr, W, H, R, t, K, scaling, maxcam);
+    int rc = frame_prepare(c, depth, bgr, W, H, R, t, K, scaling, maxcam, !overlap);
     if (rc) return rc;
     if (c->n) {
-        rc = run_frame(c, W, H);
+        rc = run_frame(c, W, H, overlap);
         if (rc) return rc;
     } else {
         launch_empty(c, W, H);
@@ -1890,12 +1938,14 @@ int ff_add_map(ff_ctx *c, const uint16_t *depth, size_t depth_stride, const uint
     if (rc) return rc;
     if (!depth_stride) depth_stride = (size_t)W * 2;
     if (!bgr_stride) bgr_stride = (size_t)W * 3;
+    // the images go up on the stream that first reads them
+    const bool overlap = front_overlap(c, W, H);
+    hipStream_t us = overlap ? c->side : c->stream;
     HIPCHK(hipMemcpy2DAsync(c->d_depth, (size_t)W * 2, depth, depth_stride, (size_t)W * 2, H, hipMemcpyHostToDevice,
-                            c->stream));
+                            us));
     if (c->useColor)
-        HIPCHK(hipMemcpy2DAsync(c->d_bgr, (size_t)W * 3, bgr, bgr_stride, (size_t)W * 3, H, hipMemcpyHostToDevice,
-                                c->stream));
-    return add_map_impl(c, c->d_depth, c->useColor ? c->d_bgr : nullptr, W, H, R, t, K, scaling, maxcam);
+        HIPCHK(hipMemcpy2DAsync(c->d_bgr, (size_t)W * 3, bgr, bgr_stride, (size_t)W * 3, H, hipMemcpyHostToDevice, us));
+    return add_map_impl(c, c->d_depth, c->useColor ? c->d_bgr : nullptr, W, H, R, t, K, scaling, maxcam, overlap);
 }
 
 int ff_add_map_device(ff_ctx *c, const uint16_t *depth_dev, const uint8_t *bgr_dev, int W, int H, const double R[9],
@@ -1910,7 +1960,8 @@ int ff_add_map_device(ff_ctx *c, const uint16_t *depth_dev, const uint8_t *bgr_d
     int rc = ensure_image(c, W, H);
     if (rc) return rc;
     // read in place: the call returns after the frame's kernels completed
-    return add_map_impl(c, depth_dev, c->useColor ? bgr_dev : nullptr, W, H, R, t, K, scaling, maxcam);
+    return add_map_impl(c, depth_dev, c->useColor ? bgr_dev : nullptr, W, H, R, t, K, scaling, maxcam,
+                        front_overlap(c, W, H));
 }
 
 int ff_set_graph_capture(ff_ctx *c, int on)

@@ -133,6 +133,10 @@ struct ff_ctx {
     int imgW = 0, imgH = 0, capW = 0, capH = 0;
     std::vector<float> pxp, pyp;
     hipStream_t stream = nullptr;
+    // the frame's front (bit planes and de-dup) runs on `side`, beside the
+    // previous frame's meshing on `stream`; evFront joins it before the descent
+    hipStream_t side = nullptr;
+    hipEvent_t evFront = nullptr;
     // map arrays
     uint32_t *d_tree = nullptr, *d_treeTmp = nullptr, *d_parent = nullptr, *d_stamp = nullptr, *d_queue = nullptr;
     ushort4 *d_leafinfo = nullptr;
@@ -192,6 +196,11 @@ void drop_frame_graphs(ff_ctx *c);
 int frame_prepare(ff_ctx *c, const uint16_t *depth, const uint8_t *bgr, int W, int H, const double R[9],
                   const double t[3], const double K[4], float scaling, float maxcam, bool upload = true);
 void launch_alloc(ff_ctx *c, int W, int H);
+// launch_alloc in two parts: the front reads the frame and writes only the
+// bit planes, the processed-pixel bitmap and the frame counters (nothing of
+// the map, nothing the meshing reads); the back descends and allocates
+void launch_front(ff_ctx *c, int W, int H, hipStream_t st);
+void launch_back(ff_ctx *c, int W, int H);
 void launch_sdf(ff_ctx *c, int W, int H);
 void launch_empty(ff_ctx *c, int W, int H);
 int frame_finish(ff_ctx *c, int W, int H, bool timed);
