@@ -168,7 +168,6 @@ struct MeshState {
     // an update whose device work is enqueued and whose host part
     // (mesh_finish) runs at the next mesh call, query or state change
     bool pending = false, pendingTimed = false;
-    bool inPlaceDone = false;  // its in-place marching cubes ran on the device
     // the map the update in flight meshes: the frame stamp and the tree
     // (frames fused before its host part runs must not be attributed to it)
     uint32_t launchStamp = 0, launchNB = 0, launchNL = 0, launchGrows = 0;
@@ -1096,20 +1095,19 @@ __device__ __forceinline__ void put_sample(WaveLds &L, int lg0, int lg1, int s, 
 // LDS, cube classification, edge numbering, vertices and faces.  di is the
 // cell's dirty index (it picks the pool arena); act false for a wave without
 // a cell (it still takes part in the wave-wide steps).
-// InPlace = false: cells whose samples k_sample wrote to the buffer;
-// InPlace = true: cells that did not fit the buffer, sampled by the wave itself
-template <bool InPlace>
+// Cells whose samples k_sample wrote to the buffer read them from there;
+// cells that did not fit the buffer are sampled by the wave itself.
 __device__ void mc_cell(WaveLds &L, int lane, CellRec *cells, const CellGeom &g, MeshCounters *mc,
                         const uint4 *__restrict__ sbuf, const MapRO &m, const Pools &P, uint32_t di, bool act)
 {
     // the wave's cell is uniform: skip cells of the other pass (empty
     // cells keep the zero counts k_cells gave them)
-    if (InPlace ? g.tile0 != FF_NO_TILE : g.tile0 >= FF_EMPTY_CELL) return;
+    if (g.tile0 == FF_EMPTY_CELL) return;
     const int type = g.type;
     const int n[3] = {g.n[0], g.n[1], g.n[2]};
     const int total = n[0] * n[1] * n[2];
     const int lg0 = __ffs(max(n[0], 1)) - 1, lg1 = __ffs(max(n[1], 1)) - 1;
-    if (!InPlace) {
+    if (g.tile0 != FF_NO_TILE) {
         const uint4 *src = sbuf + (size_t)g.tile0 * 64u;
         for (int s = lane; s < total; s += 64) {
             const uint4 v = src[s];
@@ -1303,15 +1301,12 @@ __device__ void mc_cell(WaveLds &L, int lane, CellRec *cells, const CellGeom &g,
     wave_sync();
 }
 
-template <bool InPlace>
 __global__ __launch_bounds__(256) void k_mc(CellRec *cells, const CellGeom *__restrict__ geom, MeshCounters *mc,
-                                            const uint4 *__restrict__ sbuf, MapRO m, Pools P, uint32_t tileCap)
+                                            const uint4 *__restrict__ sbuf, MapRO m, Pools P)
 {
     __shared__ WaveLds lds[4];
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     WaveLds &L = lds[wave];
-    // the in-place pass has work only when some cell's samples did not fit
-    if (InPlace && mc->nTiles <= tileCap) return;
     const uint32_t nd = mc->nDirty;
     for (uint32_t base = blockIdx.x * 4; base < nd; base += gridDim.x * 4) {
         // waves of a block run independently: each owns lds[wave]
@@ -1326,7 +1321,7 @@ __global__ __launch_bounds__(256) void k_mc(CellRec *cells, const CellGeom *__re
             g.o[0] = g.o[1] = g.o[2] = 0;
             g.size = g.sizeMin = g.stretch = 1;
         }
-        mc_cell<InPlace>(L, lane, cells, g, mc, sbuf, m, P, di, act);
+        mc_cell(L, lane, cells, g, mc, sbuf, m, P, di, act);
     }
 }
 
@@ -1382,7 +1377,7 @@ __global__ __launch_bounds__(256) void k_mc_batch(const McStream *__restrict__ m
             g.o[0] = g.o[1] = g.o[2] = 0;
             g.size = g.sizeMin = g.stretch = 1;
         }
-        mc_cell<false>(L, lane, S.cells, g, S.mc, S.sbuf, S.m, S.P, di, act);
+        mc_cell(L, lane, S.cells, g, S.mc, S.sbuf, S.m, S.P, di, act);
     }
 }
 
@@ -1768,19 +1763,7 @@ McStream mesh_stream(ff_ctx *c)
 void mesh_launch_mc(ff_ctx *c)
 {
     MeshState *s = c->mesh;
-    k_mc<false><<<2048, 256, 0, c->stream>>>(s->d_cells[s->nxt], s->d_geom, s->d_mc, s->d_sbuf, s->m, pools(s),
-                                             s->tileCap);
-}
-
-// the cells whose samples did not fit the buffer, meshed in place on the
-// device (every block returns at once when all fitted), so that the update's
-// meshes never wait for the host
-static void mesh_launch_inplace(ff_ctx *c)
-{
-    MeshState *s = c->mesh;
-    k_mc<true><<<2048, 256, 0, c->stream>>>(s->d_cells[s->nxt], s->d_geom, s->d_mc, s->d_sbuf, s->m, pools(s),
-                                            s->tileCap);
-    s->inPlaceDone = true;
+    k_mc<<<2048, 256, 0, c->stream>>>(s->d_cells[s->nxt], s->d_geom, s->d_mc, s->d_sbuf, s->m, pools(s));
 }
 
 void mesh_launch_post(ff_ctx *c, bool timed)
@@ -1806,8 +1789,6 @@ int mesh_finish(ff_ctx *c, bool timed)
 {
     MeshState *s = c->mesh;
     const int nxt = s->nxt;
-    const MapRO m = s->m;
-    const Pools P = pools(s);
     if (timed && c->timingAll) {
         float ms = 0.0f;
         if (hipEventElapsedTime(&ms, c->ev[6], c->ev[7]) == hipSuccess) {
@@ -1815,17 +1796,9 @@ int mesh_finish(ff_ctx *c, bool timed)
             c->phaseCalls[2]++;
         }
     }
-    const bool inPlaceDone = s->inPlaceDone;
-    s->inPlaceDone = false;
     if (s->h_mc->nTiles > s->tileCap && !(s->h_mc->flags & FF_CELL_OVERFLOW)) {
-        // some cells' samples did not fit the buffer: mesh them in place
-        // (unless the update did on the device), then grow the buffer for
-        // the next updates
-        if (!inPlaceDone) {
-            k_mc<true><<<2048, 256, 0, c->stream>>>(s->d_cells[nxt], s->d_geom, s->d_mc, s->d_sbuf, m, P, s->tileCap);
-            MCHK(hipMemcpyAsync(s->h_mc, s->d_mc, sizeof(MeshCounters), hipMemcpyDeviceToHost, c->stream));
-            MCHK(hipStreamSynchronize(c->stream));
-        }
+        // some cells' samples did not fit the buffer (the marching cubes
+        // sampled them in place): grow it for the next updates
         uint32_t cap = s->tileCap;
         while (cap < s->h_mc->nTiles && cap < FF_TILE_CAP_MAX) cap <<= 1;
         if (cap > s->tileCap) {
@@ -1923,7 +1896,6 @@ static int update_meshes(ff_ctx *c)
     if (rc != FF_OK) return rc;
     if (!mesh_launch_pre(c, true)) return FF_OK;
     mesh_launch_mc(c);
-    mesh_launch_inplace(c);
     mesh_launch_post(c, true);
     MCHK(hipGetLastError());
     c->mesh->pending = true;
