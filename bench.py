@@ -27,8 +27,8 @@ HBM_PEAK_GBS = 8000.0
 # HBM bytes per k_sdf launch over the timed frames of the default command
 # (950 frames after 50), from rocprofv3 FETCH_SIZE and WRITE_SIZE passes of
 # that command, each calibrated against a 256 MiB copy
-# (profiles/hbm_traffic_640x480_5mm.md): 8 056 KiB read + 3 314 KiB written
-SDF_TRAFFIC_DEFAULT = 11_643_085
+# (profiles/hbm_traffic_640x480_5mm.md): 7 964 KiB read + 3 314 KiB written
+SDF_TRAFFIC_DEFAULT = 11_548_610
 
 
 def render_frames(sc, R, t, n, cfg):
