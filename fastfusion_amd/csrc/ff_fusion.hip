@@ -1077,16 +1077,37 @@ __device__ __forceinline__ void sdf_row(uint32_t leaf, int row, const MapView &m
 
 #define FF_SDF_GRID 1024  // blocks of the SDF update; each loops over queued bricks
 
-// the map's used counts are the committed ones (after this frame)
-__device__ __forceinline__ bool sdf_gate_open(const FrameCounters *fc, const MapCounters *mapc, uint32_t n,
-                                              const SdfGate &g)
+// A frame's counters and the map's used counts, each field of which the
+// gate below tests: read as whole 16-byte words in flight together (field by
+// field behind the tests' branches they were eight dependent round trips).
+struct GateView {
+    uint4 w[4];  // FrameCounters
+    uint4 mc;    // MapCounters
+};
+
+__device__ __forceinline__ GateView gate_load(const FrameCounters *fc, const MapCounters *mapc)
 {
-    if (g.force) return true;
-    if ((fc->flags & (FF_FLAG_HASH_FULL | FF_FLAG_ORDER_BIG)) || fc->listCount > g.listCap) return false;
-    const bool out = fc->bmin[0] < 0 || fc->bmin[1] < 0 || fc->bmin[2] < 0 || fc->bmax[0] >= (int)n ||
-                     fc->bmax[1] >= (int)n || fc->bmax[2] >= (int)n;
-    const bool grows = mapc->nLeavesUsed < g.leavesTotal && mapc->nBranchesUsed < g.branchesTotal;
-    return !(out && grows);
+    GateView v;
+    const uint4 *p = reinterpret_cast<const uint4 *>(fc);
+#pragma unroll
+    for (int k = 0; k < 4; k++) v.w[k] = p[k];
+    v.mc = *reinterpret_cast<const uint4 *>(mapc);
+    return v;
+}
+
+// the map's used counts are the committed ones (after this frame); the
+// words of FrameCounters: w[0] = nQueued, listCount, newB, newL;
+// w[1] = bmin[0..2], bmax[0]; w[2] = bmax[1..2], flags, nProcessed
+__device__ __forceinline__ bool sdf_gate_open(const GateView &v, uint32_t n, const SdfGate &g)
+{
+    const uint32_t listCount = v.w[0].y, flags = v.w[2].z;
+    const int bmin0 = (int)v.w[1].x, bmin1 = (int)v.w[1].y, bmin2 = (int)v.w[1].z;
+    const int bmax0 = (int)v.w[1].w, bmax1 = (int)v.w[2].x, bmax2 = (int)v.w[2].y;
+    const bool held = ((flags & (FF_FLAG_HASH_FULL | FF_FLAG_ORDER_BIG)) != 0u) | (listCount > g.listCap);
+    const bool out = (bmin0 < 0) | (bmin1 < 0) | (bmin2 < 0) | (bmax0 >= (int)n) | (bmax1 >= (int)n) |
+                     (bmax2 >= (int)n);
+    const bool grows = (v.mc.y < g.leavesTotal) & (v.mc.x < g.branchesTotal);  // nLeavesUsed, nBranchesUsed
+    return g.force || !(held | (out & grows));
 }
 
 __global__ __launch_bounds__(512) void k_sdf(const uint32_t *__restrict__ queue, MapView m,
@@ -1094,17 +1115,27 @@ __global__ __launch_bounds__(512) void k_sdf(const uint32_t *__restrict__ queue,
                                              const MapCounters *__restrict__ mapc, SdfGate gate)
 {
     rz_mode_on();  // first: every float operation below rounds toward zero
+    // the counters, the rcp table's entries and the first queue slot are
+    // all requested before anything waits on them
+    const GateView gv = gate_load(fc, mapc);
+    uint32_t tv[2048 / 512];
+#pragma unroll
+    for (int k = 0; k < 2048 / 512; k++) tv[k] = c_rcp_table[threadIdx.x + 512 * k];
+    const uint32_t leaf0 = blockIdx.x < m.nLeavesTotal ? queue[blockIdx.x] : 0u;
     // blocks loop over the queued bricks; the count comes from the device
-    if (!sdf_gate_open(fc, mapc, fd->n, gate)) return;
+    if (!sdf_gate_open(gv, fd->n, gate)) return;
     const uint16_t *__restrict__ depth = fd->depth;
     const uint8_t *__restrict__ bgr = fd->bgr;
     const SdfParams P = fd->sp;
-    const uint32_t nQ = fc->nQueued;
+    const uint32_t nQ = gv.w[0].x;
     if (blockIdx.x == 0 && threadIdx.x == 0) fc->sdfRan = 1u;
     if (blockIdx.x >= nQ) return;
     __shared__ uint32_t tab[2048];
-    for (int i = threadIdx.x; i < 2048; i += blockDim.x) tab[i] = c_rcp_table[i];
+#pragma unroll
+    for (int k = 0; k < 2048 / 512; k++) tab[threadIdx.x + 512 * k] = tv[k];
     __syncthreads();  // rcp table ready
+    sdf_voxel(leaf0, threadIdx.x, m, depth, bgr, W, H, P, tab);
+    for (uint32_t b = blockIdx.x + gridDim.x; b < nQ; b += gridDim.x)
     for (uint32_t b = blockIdx.x; b < nQ; b += gridDim.x)
         sdf_voxel(queue[b], threadIdx.x, m, depth, bgr, W, H, P, tab);
 }
@@ -1120,8 +1151,9 @@ __global__ void k_batch_prefix(const SdfStream *__restrict__ ss, int n, uint32_t
     uint32_t cnt = 0;
     if (lane < n) {
         const SdfStream &S = ss[lane];
-        const bool open = sdf_gate_open(S.fc, S.mapc, S.fd->n, S.gate);
-        cnt = open ? S.fc->nQueued : 0u;
+        const GateView gv = gate_load(S.fc, S.mapc);
+        const bool open = sdf_gate_open(gv, S.fd->n, S.gate);
+        cnt = open ? gv.w[0].x : 0u;
         S.fc->sdfRan = open ? 1u : 0u;
     }
     uint32_t x = cnt;

@@ -32,7 +32,9 @@ struct FrameCounters {
     int bmin[3], bmax[3];
     uint32_t flags, nProcessed;
     uint32_t sdfRan;  // set by the SDF update when it ran for the frame
+    uint32_t pad[3];  // 64 bytes: a kernel reads them as four 16-byte words
 };
+static_assert(sizeof(FrameCounters) == 64, "FrameCounters is read as four 16-byte words");
 
 // what the SDF update checks on the device before running: a frame whose
 // node table overflowed, or that must grow the volume first, waits for the host
@@ -84,6 +86,7 @@ struct FrameDev {
 struct MapCounters {
     uint32_t nBranchesUsed, nLeavesUsed, pad[2];
 };
+static_assert(sizeof(MapCounters) == 16, "MapCounters is read as one 16-byte word");
 
 // one stream of a batched SDF launch
 struct SdfStream {
