@@ -1041,15 +1041,10 @@ __device__ __forceinline__ uint32_t cinterp(const uint16_t c1[3], const uint16_t
     return (uint32_t)(r[0] / 256) | ((uint32_t)(r[1] / 256) << 8) | ((uint32_t)(r[2] / 256) << 16);
 }
 
-// Bourke corners and edges
-__constant__ int8_t c_corner[8][3] = {{0, 0, 0}, {1, 0, 0}, {1, 1, 0}, {0, 1, 0}, {0, 0, 1}, {1, 0, 1}, {1, 1, 1}, {0, 1, 1}};
-__constant__ int8_t c_edge[12][4] = {{0, 0, 0, 0}, {1, 0, 0, 1}, {0, 1, 0, 0}, {0, 0, 0, 1}, {0, 0, 1, 0}, {1, 0, 1, 1},
-                                     {0, 1, 1, 0}, {0, 0, 1, 1}, {0, 0, 0, 2}, {1, 0, 0, 2}, {1, 1, 0, 2}, {0, 1, 0, 2}};
-__constant__ int8_t c_edgec[12][2] = {{0, 1}, {1, 2}, {2, 3}, {3, 0}, {4, 5}, {5, 6}, {6, 7}, {7, 4}, {0, 4}, {1, 5}, {2, 6}, {3, 7}};
-
 // edge e of a cube as an offset of used-edge slots from the cube's first
-// sample: 3 (dz * 64 + dy * 8 + dx) + axis (c_edge), 8 bits per edge, in
-// registers rather than a table
+// sample: 3 (dz * 64 + dy * 8 + dx) + axis, where (dx, dy, dz) is the edge's
+// lower end and axis its direction (Bourke edges e0-e11), 8 bits per edge,
+// in registers rather than a table
 __device__ __forceinline__ int edge_slot_off(int e)
 {
     constexpr uint32_t E0 = 0u | (4u << 8) | (24u << 16) | (1u << 24);       // e0..e3
@@ -1069,7 +1064,10 @@ __device__ __forceinline__ uint32_t div_small(uint32_t q, uint32_t m) { return (
 struct WaveLds {
     float d[FF_BRICK];
     uint16_t c[3][FF_BRICK];
-    uint8_t sf[FF_BRICK];  // bit 0: distance < 0, bit 1: zero weight
+    // sample flags as bit rows: byte (l * FF_BL + j) holds the row of 8
+    // samples (., j, l), bit i for sample i; neg: distance < 0, zero: zero
+    // weight (words 0-15 neg, 16-31 zero)
+    uint32_t flags[32];
     uint32_t used[48];     // 1536 edge slots: (sample * 3 + axis)
     uint32_t wpref[48];    // vertex index of the first slot of each word
     uint16_t cube[344];    // cube index | triangle count << 8
@@ -1088,7 +1086,28 @@ __device__ __forceinline__ void put_sample(WaveLds &L, int lg0, int lg1, int s, 
     L.c[0][id] = (uint16_t)(c01 & 0xffffu);
     L.c[1][id] = (uint16_t)(c01 >> 16);
     L.c[2][id] = (uint16_t)c2;
-    L.sf[id] = (uint8_t)((d < 0.0f ? 1u : 0u) | (w == 0.0f ? 2u : 0u));
+    if (d < 0.0f) atomicOr(&L.flags[id >> 5], 1u << (id & 31));
+    if (w == 0.0f) atomicOr(&L.flags[16 + (id >> 5)], 1u << (id & 31));
+}
+
+// byte r of the neg (zero = false) or zero-weight (zero = true) bit rows
+__device__ __forceinline__ uint32_t flag_row(const WaveLds &L, bool zero, int r)
+{
+    return reinterpret_cast<const uint8_t *>(L.flags)[(zero ? 64 : 0) + r];
+}
+
+// Bourke corner k of a cube, coordinate a: x = bit 0 ^ bit 1, y = bit 1,
+// z = bit 2
+__device__ __forceinline__ int corner_c(int k, int a) { return a == 0 ? (k ^ (k >> 1)) & 1 : (k >> a) & 1; }
+// sample index of corner k from the cube's first sample
+__device__ __forceinline__ int corner_id(int k) { return (corner_c(k, 2) * FF_BL + corner_c(k, 1)) * FF_BL + corner_c(k, 0); }
+
+// the two corners of Bourke edge e (0-11), 4 bits each in a constant
+__device__ __forceinline__ int edge_corner(int e, int end)
+{
+    constexpr uint64_t EA = 0x321076543210ull;  // first corners of e11..e0, 4 bits each
+    constexpr uint64_t EB = 0x765447650321ull;  // second corners
+    return (int)(((end ? EB : EA) >> (4 * e)) & 15u);
 }
 
 // One dirty cell, by one wave (lane 0..63) with its LDS slice: samples into
@@ -1097,7 +1116,7 @@ __device__ __forceinline__ void put_sample(WaveLds &L, int lg0, int lg1, int s, 
 // a cell (it still takes part in the wave-wide steps).
 // Cells whose samples k_sample wrote to the buffer read them from there;
 // cells that did not fit the buffer are sampled by the wave itself.
-__device__ void mc_cell(WaveLds &L, int lane, CellRec *cells, const CellGeom &g, MeshCounters *mc,
+__device__ void mc_cell(WaveLds &L, const uint64_t *tri, int lane, CellRec *cells, const CellGeom &g, MeshCounters *mc,
                         const uint4 *__restrict__ sbuf, const MapRO &m, const Pools &P, uint32_t di, bool act)
 {
     // the wave's cell is uniform: skip cells of the other pass (empty
@@ -1107,12 +1126,22 @@ __device__ void mc_cell(WaveLds &L, int lane, CellRec *cells, const CellGeom &g,
     const int n[3] = {g.n[0], g.n[1], g.n[2]};
     const int total = n[0] * n[1] * n[2];
     const int lg0 = __ffs(max(n[0], 1)) - 1, lg1 = __ffs(max(n[1], 1)) - 1;
+    if (lane < 32) L.flags[lane] = 0u;
+    wave_sync();  // the samples OR their flags in
     if (g.tile0 != FF_NO_TILE) {
+        // 1 <= total <= 512 (a cell with tiles has samples; n <= FF_BL per
+        // axis): every lane's loads are in flight together before the first
+        // LDS store waits on one; lanes past the end re-read the last sample
+        // so that no load sits behind a branch
         const uint4 *src = sbuf + (size_t)g.tile0 * 64u;
-        for (int s = lane; s < total; s += 64) {
-            const uint4 v = src[s];
-            put_sample(L, lg0, lg1, s, __uint_as_float(v.x), __uint_as_float(v.y), v.z, v.w);
-        }
+        uint4 v[FF_BRICK / 64];
+#pragma unroll
+        for (int k = 0; k < FF_BRICK / 64; k++) v[k] = src[min(lane + 64 * k, total - 1)];
+#pragma unroll
+        for (int k = 0; k < FF_BRICK / 64; k++)
+            if (lane + 64 * k < total)
+                put_sample(L, lg0, lg1, lane + 64 * k, __uint_as_float(v[k].x), __uint_as_float(v[k].y), v[k].z,
+                           v[k].w);
     } else {
         const uint32_t *ll = cells[g.cell].ll;
         for (int s = lane; s < total; s += 64) {
@@ -1134,18 +1163,17 @@ __device__ void mc_cell(WaveLds &L, int lane, CellRec *cells, const CellGeom &g,
         int ci = 0;
         bool veto = false;
         for (int k = 0; k < 8; k++) {
-            const int id = (c_corner[k][2] * FF_BL + c_corner[k][1]) * FF_BL + c_corner[k][0];
+            const int r = corner_c(k, 2) * FF_BL + corner_c(k, 1), x = corner_c(k, 0);
             if (total) {
-                const uint32_t f = L.sf[id];
-                if (f & 2u) veto = true;
-                if (f & 1u) ci |= 1 << k;
+                if ((flag_row(L, true, r) >> x) & 1u) veto = true;
+                if ((flag_row(L, false, r) >> x) & 1u) ci |= 1 << k;
             }
         }
         float basev[3];
         const float sx = m.scale * (float)stretch;
         for (int a = 0; a < 3; a++) basev[a] = m.off[a] + (float)geom_coord(g, a, 0) * m.scale;
         if (veto || !total) ci = 0;
-        const uint64_t T = c_tri64[ci];
+        const uint64_t T = tri[ci];
         const int ntri = tri_count(T);
         bool keep = false;
         float V[3][3];
@@ -1153,13 +1181,12 @@ __device__ void mc_cell(WaveLds &L, int lane, CellRec *cells, const CellGeom &g,
         if (lane < ntri) {
             for (int q = 0; q < 3; q++) {
                 const int e = tri_edge(T, 3 * lane + q);
-                const int a = c_edgec[e][0], b = c_edgec[e][1];
-                const int ia = (c_corner[a][2] * FF_BL + c_corner[a][1]) * FF_BL + c_corner[a][0];
-                const int ib = (c_corner[b][2] * FF_BL + c_corner[b][1]) * FF_BL + c_corner[b][0];
+                const int a = edge_corner(e, 0), b = edge_corner(e, 1);
+                const int ia = corner_id(a), ib = corner_id(b);
                 float Pa[3], Pb[3];
                 for (int k = 0; k < 3; k++) {
-                    Pa[k] = c_corner[a][k] ? basev[k] + 1.0f * sx : basev[k] + 0.0f * sx;
-                    Pb[k] = c_corner[b][k] ? basev[k] + 1.0f * sx : basev[k] + 0.0f * sx;
+                    Pa[k] = corner_c(a, k) ? basev[k] + 1.0f * sx : basev[k] + 0.0f * sx;
+                    Pb[k] = corner_c(b, k) ? basev[k] + 1.0f * sx : basev[k] + 0.0f * sx;
                 }
                 vinterp(Pa, Pb, L.d[ia], L.d[ib], V[q]);
                 const uint16_t ca[3] = {L.c[0][ia], L.c[1][ia], L.c[2][ia]};
@@ -1204,16 +1231,17 @@ __device__ void mc_cell(WaveLds &L, int lane, CellRec *cells, const CellGeom &g,
         const int jl = (int)div_small((uint32_t)q, m0), l = (int)div_small((uint32_t)q, m01);
         const int i = q - jl * nc0, j = jl - l * nc1;
         const int b0 = (l * FF_BL + j) * FF_BL + i;
-        int ci = 0;
-        uint32_t zw = 0;
-#pragma unroll
-        for (int k = 0; k < 8; k++) {
-            const uint32_t f = L.sf[b0 + (c_corner[k][2] * FF_BL + c_corner[k][1]) * FF_BL + c_corner[k][0]];
-            zw |= f;
-            ci |= (int)(f & 1u) << k;
-        }
-        if (zw & 2u) ci = 0;
-        const uint64_t T = ci ? c_tri64[ci] : 0ull;
+        // the cube's corners are bits i, i + 1 of rows r, r + 1 (y + 1),
+        // r + 8 (z + 1) and r + 9
+        const int r = l * FF_BL + j;
+        const uint32_t n00 = flag_row(L, false, r) >> i, n10 = flag_row(L, false, r + 1) >> i;
+        const uint32_t n01 = flag_row(L, false, r + FF_BL) >> i, n11 = flag_row(L, false, r + FF_BL + 1) >> i;
+        const uint32_t zw = (flag_row(L, true, r) | flag_row(L, true, r + 1) | flag_row(L, true, r + FF_BL) |
+                             flag_row(L, true, r + FF_BL + 1)) >> i;
+        int ci = (int)((n00 & 1u) | (n00 & 2u) | ((n10 & 2u) << 1) | ((n10 & 1u) << 3) | ((n01 & 1u) << 4) |
+                       ((n01 & 2u) << 4) | ((n11 & 2u) << 5) | ((n11 & 1u) << 7));
+        if (zw & 3u) ci = 0;
+        const uint64_t T = tri[ci];
         const int nt = tri_count(T);
         L.cube[q] = (uint16_t)(ci | (nt << 8));
         for (int t = 0; t < 3 * nt; t++) {
@@ -1282,7 +1310,7 @@ __device__ void mc_cell(WaveLds &L, int lane, CellRec *cells, const CellGeom &g,
             const int cq = L.cube[q];
             const int nt = cq >> 8;
             if (!nt) continue;
-            const uint64_t T = c_tri64[cq & 255];
+            const uint64_t T = tri[cq & 255];
             const int jl = (int)div_small((uint32_t)q, m0), l = (int)div_small((uint32_t)q, m01);
             const int i = q - jl * nc0, j = jl - l * nc1;
             const int b0 = (l * FF_BL + j) * FF_BL + i;
@@ -1301,10 +1329,13 @@ __device__ void mc_cell(WaveLds &L, int lane, CellRec *cells, const CellGeom &g,
     wave_sync();
 }
 
-__global__ __launch_bounds__(256) void k_mc(CellRec *cells, const CellGeom *__restrict__ geom, MeshCounters *mc,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_mc(CellRec *cells, const CellGeom *__restrict__ geom, MeshCounters *mc,
                                             const uint4 *__restrict__ sbuf, MapRO m, Pools P)
 {
     __shared__ WaveLds lds[4];
+    __shared__ uint64_t tri[256];
+    tri[threadIdx.x] = c_tri64[threadIdx.x];  // 256 threads
+    __syncthreads();
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     WaveLds &L = lds[wave];
     const uint32_t nd = mc->nDirty;
@@ -1321,7 +1352,7 @@ __global__ __launch_bounds__(256) void k_mc(CellRec *cells, const CellGeom *__re
             g.o[0] = g.o[1] = g.o[2] = 0;
             g.size = g.sizeMin = g.stretch = 1;
         }
-        mc_cell(L, lane, cells, g, mc, sbuf, m, P, di, act);
+        mc_cell(L, tri, lane, cells, g, mc, sbuf, m, P, di, act);
     }
 }
 
@@ -1350,10 +1381,13 @@ __global__ void k_mc_prefix(const McStream *__restrict__ ms, int n, uint32_t *__
 }
 
 // the dirty cells of n streams as one worklist, one wave per cell
-__global__ __launch_bounds__(256) void k_mc_batch(const McStream *__restrict__ ms, int n,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_mc_batch(const McStream *__restrict__ ms, int n,
                                                   const uint32_t *__restrict__ prefix)
 {
     __shared__ WaveLds lds[4];
+    __shared__ uint64_t tri[256];
+    tri[threadIdx.x] = c_tri64[threadIdx.x];  // 256 threads
+    __syncthreads();
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     WaveLds &L = lds[wave];
     const uint32_t total = prefix[n];
@@ -1377,7 +1411,7 @@ __global__ __launch_bounds__(256) void k_mc_batch(const McStream *__restrict__ m
             g.o[0] = g.o[1] = g.o[2] = 0;
             g.size = g.sizeMin = g.stretch = 1;
         }
-        mc_cell(L, lane, S.cells, g, S.mc, S.sbuf, S.m, S.P, di, act);
+        mc_cell(L, tri, lane, S.cells, g, S.mc, S.sbuf, S.m, S.P, di, act);
     }
 }
 
