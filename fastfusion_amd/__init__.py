@@ -242,6 +242,20 @@ class FusionMipMapGPU:
         if not self._ctx:
             raise FFError("ff_create failed (no HIP device or allocation failure)")
         self.color = bool(color)
+        # a frame's R (9), t (3) and K (4), staged in one buffer the library
+        # reads during the call (far cheaper per frame than converting each
+        # argument to a fresh array)
+        self._pose = (ctypes.c_double * 16)()
+        self._posev = np.frombuffer(self._pose, np.float64)
+        a = ctypes.addressof(self._pose)
+        self._pR, self._pt, self._pK = (ctypes.cast(a + 8 * o, _DP) for o in (0, 9, 12))
+
+    def _pose_args(self, R, t, K):
+        v = self._posev
+        v[0:9] = np.ravel(R)
+        v[9:12] = np.ravel(t)
+        v[12:16] = np.ravel(K)
+        return self._pR, self._pt, self._pK
 
     def close(self):
         if getattr(self, "_ctx", None):
@@ -267,13 +281,14 @@ class FusionMipMapGPU:
             rgb = np.ascontiguousarray(rgb, dtype=np.uint8)
             if rgb.shape != (h, w, 3):
                 raise FFError("rgb must be (h, w, 3) BGR8 like depth")
-        rc = self._lib.ff_add_map(self._ctx, _ptr(depth), w * 2, _ptr(rgb), w * 3, w, h, _dptr(R), _dptr(t),
-                                  _dptr(K), float(scaling), float(maxcamdistance))
+        pR, pt, pK = self._pose_args(R, t, K)
+        rc = self._lib.ff_add_map(self._ctx, _ptr(depth), w * 2, _ptr(rgb), w * 3, w, h, pR, pt, pK, float(scaling),
+                                  float(maxcamdistance))
         return self._check(rc)
 
     def addMapDevice(self, depth_ptr, rgb_ptr, w, h, R, t, K, scaling=1.0 / 5000.0, maxcamdistance=10.0):
-        rc = self._lib.ff_add_map_device(self._ctx, ctypes.c_void_p(depth_ptr), ctypes.c_void_p(rgb_ptr),
-                                         int(w), int(h), _dptr(R), _dptr(t), _dptr(K), float(scaling),
+        pR, pt, pK = self._pose_args(R, t, K)
+        rc = self._lib.ff_add_map_device(self._ctx, depth_ptr, rgb_ptr, int(w), int(h), pR, pt, pK, float(scaling),
                                          float(maxcamdistance))
         return self._check(rc)
 

@@ -168,15 +168,15 @@ __global__ void k_frame_begin(const FrameDev *__restrict__ fd, FrameCounters *fc
 // ---------------------------------------------------------------------------
 // (block (0, 0) also resets the frame counters: no kernel before the
 // descent reads them)
-__device__ __forceinline__ void planes_body(const FrameDev *__restrict__ fd, int W, int H, int words,
+__device__ __forceinline__ void planes_body(const FrameDev &fd, int W, int H, int words,
                                             const float *__restrict__ pxp, const float *__restrict__ pyp,
                                             uint32_t *__restrict__ planes, FrameCounters *fc)
 {
     const int lane = threadIdx.x, y = blockIdx.y, x = blockIdx.x * 64 + lane;
-    const uint16_t *__restrict__ depth = fd->depth;
-    const float scaling = fd->scaling, maxcam = fd->maxcam;
-    const CamF &p = fd->p;
-    if (blockIdx.x == 0 && y == 0 && lane == 0) frame_counters_reset(fc, fd->n);
+    const uint16_t *__restrict__ depth = fd.depth;
+    const float scaling = fd.scaling, maxcam = fd.maxcam;
+    const CamF &p = fd.p;
+    if (blockIdx.x == 0 && y == 0 && lane == 0) frame_counters_reset(fc, fd.n);
     bool valid = false, el = false, eu = false;
     if (x < W) {
         const float D = (float)depth[y * W + x];
@@ -211,7 +211,18 @@ __global__ __launch_bounds__(64) void k_planes(const FrameDev *__restrict__ fd, 
                                                const float *__restrict__ pxp, const float *__restrict__ pyp,
                                                uint32_t *__restrict__ planes, FrameCounters *fc)
 {
-    planes_body(fd, W, H, words, pxp, pyp, planes, fc);
+    planes_body(*fd, W, H, words, pxp, pyp, planes, fc);
+}
+
+// The same with the frame's parameters as a kernel argument: the first
+// thread also stores them where the frame's later kernels read them, in
+// place of a separate upload.
+__global__ __launch_bounds__(64) void k_planes_v(const FrameDev f, FrameDev *fdOut, int W, int H, int words,
+                                                 const float *__restrict__ pxp, const float *__restrict__ pyp,
+                                                 uint32_t *__restrict__ planes, FrameCounters *fc)
+{
+    if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) *fdOut = f;
+    planes_body(f, W, H, words, pxp, pyp, planes, fc);
 }
 
 // the batched forms of the allocation kernels: stream blockIdx.z of S, grid
@@ -221,7 +232,7 @@ __global__ __launch_bounds__(64) void k_planes_b(const AllocStream *__restrict__
 {
     const AllocStream &a = S[blockIdx.z];
     if ((int)blockIdx.y >= a.H || (int)blockIdx.x * 64 >= a.W) return;
-    planes_body(a.fd, a.W, a.H, a.words, a.pxp, a.pyp, a.planes, a.fc);
+    planes_body(*a.fd, a.W, a.H, a.words, a.pxp, a.pyp, a.planes, a.fc);
 }
 
 // ---------------------------------------------------------------------------
@@ -1112,7 +1123,8 @@ __device__ __forceinline__ bool sdf_gate_open(const GateView &v, uint32_t n, con
 
 __global__ __launch_bounds__(512) void k_sdf(const uint32_t *__restrict__ queue, MapView m,
                                              const FrameDev *__restrict__ fd, int W, int H, FrameCounters *fc,
-                                             const MapCounters *__restrict__ mapc, SdfGate gate)
+                                             const MapCounters *__restrict__ mapc, SdfGate gate,
+                                             FrameCounters *hostFc, const uint4 *carrySrc, uint4 *carryDst)
 {
     rz_mode_on();  // first: every float operation below rounds toward zero
     // the counters, the rcp table's entries and the first queue slot are
@@ -1123,7 +1135,20 @@ __global__ __launch_bounds__(512) void k_sdf(const uint32_t *__restrict__ queue,
     for (int k = 0; k < 2048 / 512; k++) tv[k] = c_rcp_table[threadIdx.x + 512 * k];
     const uint32_t leaf0 = blockIdx.x < m.nLeavesTotal ? queue[blockIdx.x] : 0u;
     // blocks loop over the queued bricks; the count comes from the device
-    if (!sdf_gate_open(gv, fd->n, gate)) return;
+    const bool open = sdf_gate_open(gv, fd->n, gate);
+    if (hostFc && blockIdx.x == 0 && threadIdx.x == 0) {
+        // the frame's final counters to host memory (the host reads them
+        // once the stream is done), with whether this update ran
+        uint4 *h = reinterpret_cast<uint4 *>(hostFc);
+        h[0] = gv.w[0]; h[1] = gv.w[1]; h[2] = gv.w[2];
+        h[3] = make_uint4(open ? 1u : 0u, 0u, 0u, 0u);
+    }
+    if (carrySrc && blockIdx.x == 0 && threadIdx.x < 2) {
+        // the previous mesh update's counters (its kernels precede this one
+        // on the stream) to host memory, in place of a separate copy
+        carryDst[threadIdx.x] = carrySrc[threadIdx.x];
+    }
+    if (!open) return;
     const uint16_t *__restrict__ depth = fd->depth;
     const uint8_t *__restrict__ bgr = fd->bgr;
     const SdfParams P = fd->sp;
@@ -1487,11 +1512,14 @@ ff_ctx *ff_create(const ff_params *prm)
         delete c;
         return nullptr;
     }
-    if (hipStreamCreate(&c->side) != hipSuccess || hipEventCreateWithFlags(&c->evFront, hipEventDisableTiming) != hipSuccess) {
-        // the front then runs on the main stream
+    if (hipStreamCreate(&c->side) != hipSuccess ||
+        hipEventCreateWithFlags(&c->evFront, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c->evMark, hipEventDisableTiming) != hipSuccess) {
+        // the allocation phase then runs on the main stream
         if (c->side) hipStreamDestroy(c->side);
+        if (c->evFront) hipEventDestroy(c->evFront);
         c->side = nullptr;
-        c->evFront = nullptr;
+        c->evFront = c->evMark = nullptr;
         (void)hipGetLastError();
     }
     const size_t NL = c->nLeavesTotal;
@@ -1504,7 +1532,9 @@ ff_ctx *ff_create(const ff_params *prm)
               hipMalloc(&c->d_dist, sizeof(float) * NL * FF_BRICK) == hipSuccess &&
               hipMalloc(&c->d_wgt, sizeof(float) * NL * FF_BRICK) == hipSuccess &&
               hipMalloc(&c->d_fc, sizeof(FrameCounters)) == hipSuccess &&
-              hipHostMalloc(&c->h_fc, sizeof(FrameCounters), hipHostMallocDefault) == hipSuccess &&
+              hipHostMalloc(&c->h_fc, sizeof(FrameCounters), hipHostMallocMapped | hipHostMallocCoherent) ==
+                  hipSuccess &&
+              hipHostGetDevicePointer(reinterpret_cast<void **>(&c->d_hfc), c->h_fc, 0) == hipSuccess &&
               hipMalloc(&c->d_fd, sizeof(FrameDev)) == hipSuccess &&
               hipHostMalloc(&c->h_fd, sizeof(FrameDev), hipHostMallocDefault) == hipSuccess &&
               hipMalloc(&c->d_mapc, sizeof(MapCounters)) == hipSuccess &&
@@ -1568,6 +1598,7 @@ void ff_destroy(ff_ctx *c)
         hipStreamDestroy(c->side);
     }
     if (c->evFront) hipEventDestroy(c->evFront);
+    if (c->evMark) hipEventDestroy(c->evMark);
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;
 }
@@ -1584,10 +1615,15 @@ namespace ff {
 // it can be captured once and replayed as graphs: the allocation (ending
 // with the commit of the used counts), then the learned confidence when on,
 // the gated SDF update and the counters' copy back.
-void launch_front(ff_ctx *c, int W, int H, hipStream_t st)
+void launch_front(ff_ctx *c, int W, int H, hipStream_t st, bool carry)
 {
     const int words = (W + 31) / 32;
-    k_planes<<<dim3((W + 63) / 64, H), 64, 0, st>>>(c->d_fd, W, H, words, c->d_pxp, c->d_pyp, c->d_planes, c->d_fc);
+    if (carry)
+        k_planes_v<<<dim3((W + 63) / 64, H), 64, 0, st>>>(*c->h_fd, c->d_fd, W, H, words, c->d_pxp, c->d_pyp,
+                                                          c->d_planes, c->d_fc);
+    else
+        k_planes<<<dim3((W + 63) / 64, H), 64, 0, st>>>(c->d_fd, W, H, words, c->d_pxp, c->d_pyp, c->d_planes,
+                                                        c->d_fc);
     const size_t lds = dedup_lds_bytes(words);
     switch (dedup_wmax(words)) {
     case 20: k_dedup<20><<<1, FF_DD_THREADS, lds, st>>>(c->d_planes, H, words, c->d_pbits, c->d_fc); break;
@@ -1598,25 +1634,25 @@ void launch_front(ff_ctx *c, int W, int H, hipStream_t st)
 
 void launch_alloc(ff_ctx *c, int W, int H)
 {
-    launch_front(c, W, H, c->stream);
-    launch_back(c, W, H);
+    launch_front(c, W, H, c->stream, false);
+    launch_back(c, W, H, c->stream);
 }
 
-void launch_back(ff_ctx *c, int W, int H)
+void launch_back(ff_ctx *c, int W, int H, hipStream_t st)
 {
     const int words = (W + 31) / 32;
     const FrameDev *fd = c->d_fd;
     const MapView mv = c->mapView();
-    k_descend<<<dim3((W + 127) / 128, H), 128, 0, c->stream>>>(fd, W, H, words, c->d_pxp, c->d_pyp, c->d_pbits, mv,
-                                                               c->hv, c->d_pl0, c->d_fc);
-    k_order<<<1, FF_ORDER_THREADS, 0, c->stream>>>(c->hv, c->d_fc, c->d_mapc);
-    k_link<<<256, 256, 0, c->stream>>>(c->hv, mv, c->d_fc, c->d_pl0, fd, c->d_fc, 0);
+    k_descend<<<dim3((W + 127) / 128, H), 128, 0, st>>>(fd, W, H, words, c->d_pxp, c->d_pyp, c->d_pbits, mv, c->hv,
+                                                       c->d_pl0, c->d_fc);
+    k_order<<<1, FF_ORDER_THREADS, 0, st>>>(c->hv, c->d_fc, c->d_mapc);
+    k_link<<<256, 256, 0, st>>>(c->hv, mv, c->d_fc, c->d_pl0, fd, c->d_fc, 0);
     ushort4 *binfo;
     uint32_t binfoBlocks;
     mesh_binfo(c, &binfo, &binfoBlocks);
-    k_alloc_end<<<256, 256, 0, c->stream>>>(c->hv, c->d_fc, c->d_mapc, binfo, binfoBlocks, fd, 0);
-    k_queue<<<queue_grid(c->nLeavesUsed), FF_QUEUE_THREADS, 0, c->stream>>>(c->d_stamp, c->d_mapc, c->nLeavesTotal,
-                                                                            fd, c->d_queue, c->d_fc, 0);
+    k_alloc_end<<<256, 256, 0, st>>>(c->hv, c->d_fc, c->d_mapc, binfo, binfoBlocks, fd, 0);
+    k_queue<<<queue_grid(c->nLeavesUsed), FF_QUEUE_THREADS, 0, st>>>(c->d_stamp, c->d_mapc, c->nLeavesTotal, fd,
+                                                                     c->d_queue, c->d_fc, 0);
 }
 
 // A frame with more new nodes than k_order ranks (flagged; its link, commit
@@ -1692,15 +1728,20 @@ void launch_alloc_batch(hipStream_t st, const AllocStream *d_as, const AllocStre
 }
 
 // the SDF update waits for the host only when the node table overflowed or
-// the volume must grow first
-static void launch_sdf_only(ff_ctx *c, int W, int H)
+// the volume must grow first; carry: a direct launch, which also takes the
+// previous mesh update's counters up
+static void launch_sdf_only(ff_ctx *c, int W, int H, bool carry)
 {
     SdfGate gate;
     gate.listCap = c->hv.listCap;
     gate.leavesTotal = c->nLeavesTotal;
     gate.branchesTotal = c->nBranchesTotal;
     gate.force = 0;
-    k_sdf<<<FF_SDF_GRID, 512, 0, c->stream>>>(c->d_queue, c->mapView(), c->d_fd, W, H, c->d_fc, c->d_mapc, gate);
+    const uint4 *cs = carry ? static_cast<const uint4 *>(c->carrySrc) : nullptr;
+    uint4 *cd = carry ? static_cast<uint4 *>(c->carryDst) : nullptr;
+    k_sdf<<<FF_SDF_GRID, 512, 0, c->stream>>>(c->d_queue, c->mapView(), c->d_fd, W, H, c->d_fc, c->d_mapc, gate,
+                                              c->d_hfc, cs, cd);
+    if (cs) c->carrySrc = c->carryDst = nullptr;
 }
 
 // (the learned confidence first when on; the counters' copy back follows
@@ -1708,7 +1749,7 @@ static void launch_sdf_only(ff_ctx *c, int W, int H)
 void launch_sdf(ff_ctx *c, int W, int H)
 {
     if (c->learned) learned_launch(c, W, H);
-    launch_sdf_only(c, W, H);
+    launch_sdf_only(c, W, H, false);
 }
 
 static bool capture(ff_ctx *c, void (*fn)(ff_ctx *, int, int), int W, int H, hipGraph_t *g, hipGraphExec_t *x)
@@ -1725,30 +1766,33 @@ static bool capture(ff_ctx *c, void (*fn)(ff_ctx *, int, int), int W, int H, hip
 
 // the frame's two launch sequences, from graphs captured for this
 // configuration (timing events are recorded between them)
-// whether a frame's front may run on the side stream: direct launches (a
-// captured frame replays on one stream), no per-phase events (they would
-// time the overlap), an existing volume and image tables of this size (the
-// first frame and a new size prepare on the main stream)
+// whether a frame's allocation phase may run on the side stream: direct
+// launches (a captured frame replays on one stream), no per-phase events
+// (they would time the overlap), an existing volume and image tables of
+// this size (the first frame and a new size prepare on the main stream)
 static bool front_overlap(const ff_ctx *c, int W, int H)
 {
-    return c->side && c->evFront && !c->useGraphs && !c->timingAll && c->n && c->imgW == W && c->imgH == H;
+    return c->side && c->evFront && c->evMark && !c->useGraphs && !c->timingAll && c->n && c->imgW == W &&
+           c->imgH == H;
 }
 
 static int run_frame(ff_ctx *c, int W, int H, bool overlap)
 {
     if (overlap) {
-        // the front beside the meshing still in flight on the main stream;
-        // the descent waits for it
-        HIPCHK(hipMemcpyAsync(c->d_fd, c->h_fd, sizeof(FrameDev), hipMemcpyHostToDevice, c->side));
-        launch_front(c, W, H, c->side);
+        // the allocation phase beside the mesh update still in flight on the
+        // main stream: the front at once, the descent once the update has
+        // passed over the tree and the stamps; the SDF update (after the
+        // update's marching cubes on the main stream) waits for the phase
+        launch_front(c, W, H, c->side, true);  // carries the frame's parameters up
+        if (c->markPending) HIPCHK(hipStreamWaitEvent(c->side, c->evMark, 0));
+        c->markPending = false;
+        launch_back(c, W, H, c->side);
         HIPCHK(hipEventRecord(c->evFront, c->side));
         HIPCHK(hipStreamWaitEvent(c->stream, c->evFront, 0));
-        launch_back(c, W, H);
         if (c->learned) learned_launch(c, W, H);
         if (c->timing) hipEventRecord(c->ev[2], c->stream);
-        launch_sdf_only(c, W, H);
+        launch_sdf_only(c, W, H, true);  // which leaves the counters in h_fc
         if (c->timing) hipEventRecord(c->ev[3], c->stream);
-        HIPCHK(hipMemcpyAsync(c->h_fc, c->d_fc, sizeof(FrameCounters), hipMemcpyDeviceToHost, c->stream));
         HIPCHK(hipGetLastError());
         return FF_OK;
     }
@@ -1773,9 +1817,8 @@ static int run_frame(ff_ctx *c, int W, int H, bool overlap)
     if (!g && c->learned) learned_launch(c, W, H);
     if (c->timing) hipEventRecord(c->ev[2], c->stream);
     if (g) HIPCHK(hipGraphLaunch(c->frameExec[1], c->stream));
-    else launch_sdf_only(c, W, H);
+    else launch_sdf_only(c, W, H, true);  // which leaves the counters in h_fc
     if (c->timing) hipEventRecord(c->ev[3], c->stream);
-    HIPCHK(hipMemcpyAsync(c->h_fc, c->d_fc, sizeof(FrameCounters), hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipGetLastError());
     return FF_OK;
 }
@@ -1912,7 +1955,8 @@ int frame_finish(ff_ctx *c, int W, int H, bool timed)
         gate.branchesTotal = c->nBranchesTotal;
         gate.force = 1;
         if (c->timing) hipEventRecord(c->ev[4], c->stream);
-        k_sdf<<<FF_SDF_GRID, 512, 0, c->stream>>>(c->d_queue, c->mapView(), c->d_fd, W, H, c->d_fc, c->d_mapc, gate);
+        k_sdf<<<FF_SDF_GRID, 512, 0, c->stream>>>(c->d_queue, c->mapView(), c->d_fd, W, H, c->d_fc, c->d_mapc, gate,
+                                                  nullptr, nullptr, nullptr);
         if (c->timing) hipEventRecord(c->ev[5], c->stream);
         HIPCHK(hipGetLastError());
         HIPCHK(hipStreamSynchronize(c->stream));

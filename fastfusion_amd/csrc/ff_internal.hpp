@@ -136,10 +136,15 @@ struct ff_ctx {
     int imgW = 0, imgH = 0, capW = 0, capH = 0;
     std::vector<float> pxp, pyp;
     hipStream_t stream = nullptr;
-    // the frame's front (bit planes and de-dup) runs on `side`, beside the
-    // previous frame's meshing on `stream`; evFront joins it before the descent
+    // the frame's allocation phase (bit planes, de-dup, descent, node
+    // commit, queue) runs on `side`, beside the previous mesh update on
+    // `stream`: the descent waits for evMark, recorded after the update's
+    // passes over the tree and the stamps (its sampling and marching cubes
+    // read only bricks that already exist, which the allocation leaves
+    // alone); evFront joins the phase before the SDF update
     hipStream_t side = nullptr;
-    hipEvent_t evFront = nullptr;
+    hipEvent_t evFront = nullptr, evMark = nullptr;
+    bool markPending = false;  // evMark recorded since the last frame
     // map arrays
     uint32_t *d_tree = nullptr, *d_treeTmp = nullptr, *d_parent = nullptr, *d_stamp = nullptr, *d_queue = nullptr;
     ushort4 *d_leafinfo = nullptr;
@@ -153,6 +158,11 @@ struct ff_ctx {
              *d_pl0 = nullptr;
     uint64_t *d_pref = nullptr, *d_bsum = nullptr;
     ff::FrameCounters *d_fc = nullptr, *h_fc = nullptr;
+    ff::FrameCounters *d_hfc = nullptr;  // h_fc (mapped host memory) as the device addresses it
+    // 32 bytes the next directly launched SDF update copies from device to
+    // (mapped) host memory: the enqueued mesh update's counters
+    const void *carrySrc = nullptr;
+    void *carryDst = nullptr;
     ff::FrameDev *d_fd = nullptr, *h_fd = nullptr;     // h_fd: pinned staging
     ff::MapCounters *d_mapc = nullptr, *h_mapc = nullptr;
     // the captured per-frame launches: [0] allocation, [1] confidence + SDF
@@ -202,8 +212,8 @@ void launch_alloc(ff_ctx *c, int W, int H);
 // launch_alloc in two parts: the front reads the frame and writes only the
 // bit planes, the processed-pixel bitmap and the frame counters (nothing of
 // the map, nothing the meshing reads); the back descends and allocates
-void launch_front(ff_ctx *c, int W, int H, hipStream_t st);
-void launch_back(ff_ctx *c, int W, int H);
+void launch_front(ff_ctx *c, int W, int H, hipStream_t st, bool carry);
+void launch_back(ff_ctx *c, int W, int H, hipStream_t st);
 void launch_sdf(ff_ctx *c, int W, int H);
 void launch_empty(ff_ctx *c, int W, int H);
 int frame_finish(ff_ctx *c, int W, int H, bool timed);

@@ -43,6 +43,7 @@ struct CellRec {
 struct MeshCounters {
     uint32_t nTerm, nCells, nDirty, flags, nTiles, pad[3];
 };
+static_assert(sizeof(MeshCounters) == 32, "the SDF update carries MeshCounters up as two 16-byte words");
 
 #define FF_MESH_OVERFLOW 1u
 #define FF_CELL_OVERFLOW 2u
@@ -147,6 +148,7 @@ struct MeshState {
     uint32_t vCap = 0, fCap = 0;  // FF_ARENAS arenas of vCap / FF_ARENAS, fCap / FF_ARENAS
     unsigned long long *d_arena = nullptr;
     MeshCounters *d_mc = nullptr, *h_mc = nullptr;
+    MeshCounters *d_hmc = nullptr;  // h_mc (mapped host memory) as the device addresses it
     uint32_t lastMeshStamp = 0;
     int nxt = 1;   // table being built by the update in flight
     MapRO m{};     // its map view
@@ -1521,7 +1523,8 @@ int mesh_init(ff_ctx *c)
     MCHK(hipMalloc(&s->d_arena, sizeof(unsigned long long) * FF_ARENAS * FF_ARENA_STRIDE));
     MCHK(hipMemset(s->d_arena, 0, sizeof(unsigned long long) * FF_ARENAS * FF_ARENA_STRIDE));
     MCHK(hipMalloc(&s->d_mc, sizeof(MeshCounters)));
-    MCHK(hipHostMalloc(&s->h_mc, sizeof(MeshCounters), hipHostMallocDefault));
+    MCHK(hipHostMalloc(&s->h_mc, sizeof(MeshCounters), hipHostMallocMapped | hipHostMallocCoherent));
+    MCHK(hipHostGetDevicePointer(reinterpret_cast<void **>(&s->d_hmc), s->h_mc, 0));
     memset(s->h_mc, 0, sizeof(MeshCounters));
     MCHK(hipMemset(s->d_mc, 0, sizeof(MeshCounters)));
     return FF_OK;
@@ -1815,7 +1818,14 @@ void mesh_launch_post(ff_ctx *c, bool timed)
                                                   s->hmask, c->timingAll ? s->d_arena : nullptr);
     }
     if (timed && c->timingAll) hipEventRecord(c->ev[7], c->stream);
-    hipMemcpyAsync(s->h_mc, s->d_mc, sizeof(MeshCounters), hipMemcpyDeviceToHost, c->stream);
+    if (c->timingAll) {
+        hipMemcpyAsync(s->h_mc, s->d_mc, sizeof(MeshCounters), hipMemcpyDeviceToHost, c->stream);
+    } else {
+        // the next SDF update on this stream carries the counters up (the
+        // host part copies them itself if none was launched by then)
+        c->carrySrc = s->d_mc;
+        c->carryDst = s->d_hmc;
+    }
 }
 
 // after the device work of pre / mc / post completed
@@ -1919,6 +1929,10 @@ int mesh_complete(ff_ctx *c)
     if (!s || !s->pending) return FF_OK;
     s->pending = false;
     MCHK(hipStreamSynchronize(c->stream));
+    if (c->carrySrc) {
+        MCHK(hipMemcpy(s->h_mc, s->d_mc, sizeof(MeshCounters), hipMemcpyDeviceToHost));
+        c->carrySrc = c->carryDst = nullptr;
+    }
     const int r = mesh_finish(c, s->pendingTimed);
     return r < 0 ? r : FF_OK;
 }
