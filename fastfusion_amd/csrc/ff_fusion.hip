@@ -921,6 +921,36 @@ static inline uint32_t queue_grid(uint32_t nLeavesUsed)
 }
 
 // ---------------------------------------------------------------------------
+// k_alloc_tail: k_order, k_link, k_alloc_end and (queue) k_queue as the
+// passes of one block, each behind a block barrier instead of a launch.
+// Their work is the frame's new nodes (at most FF_ORDER_MAX) and, for the
+// queue, a map of at most FF_TAIL_SPANS spans of bricks.  Within the block
+// plain stores are seen by the later passes; the ORDER_BIG flag is set by an
+// atomic, so it is read with an atomic load.
+// ---------------------------------------------------------------------------
+#define FF_TAIL_SPANS 4
+
+__global__ __launch_bounds__(FF_ORDER_THREADS) void k_alloc_tail(HashView hv, MapView m, FrameCounters *fc,
+                                                             const uint32_t *__restrict__ pl0,
+                                                             const FrameDev *__restrict__ fd, MapCounters *mapc,
+                                                             ushort4 *binfo, uint32_t binfoBlocks, int queue)
+{
+    __shared__ uint32_t big;
+    order_body(hv, fc, mapc);
+    __syncthreads();
+    if (threadIdx.x == 0)
+        big = __hip_atomic_load(&fc->flags, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & FF_FLAG_ORDER_BIG;
+    __syncthreads();
+    if (big) return;  // the host orders the frame
+    link_body(hv, m, fc, pl0, fd, fc, true);
+    __syncthreads();  // every lookup of the node table is done before its slots are reset
+    alloc_end_body(hv, fc, mapc, binfo, binfoBlocks, fd, true);
+    if (!queue) return;
+    __syncthreads();  // the used counts are committed
+    queue_body(m.stamp, mapc, m.nLeavesTotal, fd, m.queue, fc, true);
+}
+
+// ---------------------------------------------------------------------------
 // grow (geometryfusion_mipmap_cpu.cpp:3071-3319): shift child-branch pointers,
 // move the tree, shift brick positions
 // ---------------------------------------------------------------------------
@@ -1645,14 +1675,17 @@ void launch_back(ff_ctx *c, int W, int H, hipStream_t st)
     const MapView mv = c->mapView();
     k_descend<<<dim3((W + 127) / 128, H), 128, 0, st>>>(fd, W, H, words, c->d_pxp, c->d_pyp, c->d_pbits, mv, c->hv,
                                                        c->d_pl0, c->d_fc);
-    k_order<<<1, FF_ORDER_THREADS, 0, st>>>(c->hv, c->d_fc, c->d_mapc);
-    k_link<<<256, 256, 0, st>>>(c->hv, mv, c->d_fc, c->d_pl0, fd, c->d_fc, 0);
     ushort4 *binfo;
     uint32_t binfoBlocks;
     mesh_binfo(c, &binfo, &binfoBlocks);
-    k_alloc_end<<<256, 256, 0, st>>>(c->hv, c->d_fc, c->d_mapc, binfo, binfoBlocks, fd, 0);
-    k_queue<<<queue_grid(c->nLeavesUsed), FF_QUEUE_THREADS, 0, st>>>(c->d_stamp, c->d_mapc, c->nLeavesTotal, fd,
-                                                                     c->d_queue, c->d_fc, 0);
+    // the tail in one block; the queue pass with it while the map (with
+    // this frame's new bricks) is a few spans
+    const bool small = (uint64_t)c->nLeavesUsed + FF_ORDER_MAX <= (uint64_t)FF_TAIL_SPANS * FF_QUEUE_SPAN;
+    k_alloc_tail<<<1, FF_ORDER_THREADS, 0, st>>>(c->hv, mv, c->d_fc, c->d_pl0, fd, c->d_mapc, binfo, binfoBlocks,
+                                                 small ? 1 : 0);
+    if (!small)
+        k_queue<<<queue_grid(c->nLeavesUsed), FF_QUEUE_THREADS, 0, st>>>(c->d_stamp, c->d_mapc, c->nLeavesTotal, fd,
+                                                                         c->d_queue, c->d_fc, 0);
 }
 
 // A frame with more new nodes than k_order ranks (flagged; its link, commit
