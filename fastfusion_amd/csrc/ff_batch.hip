@@ -162,7 +162,7 @@ static void launch_tick(ff_ctx *lead, int, int)
     const int nd = (int)P.dev.size();
     launch_alloc_batch(P.st, b->dd.as, b->hd.as, nd);
     for (int i : P.dev)
-        if (b->ctxs[i]->learned) learned_launch(b->ctxs[i], P.W[i], P.H[i]);
+        if (b->ctxs[i]->learned) learned_launch(b->ctxs[i], P.W[i], P.H[i], b->ctxs[i]->stream);
     // events only in direct launches (lead != nullptr): the batched SDF time
     launch_batch_prefix(P.st, b->d_ss, nd, b->d_prefix);
     if (lead && lead->timing) hipEventRecord(lead->ev[2], P.st);

@@ -1543,13 +1543,13 @@ ff_ctx *ff_create(const ff_params *prm)
         return nullptr;
     }
     if (hipStreamCreate(&c->side) != hipSuccess ||
-        hipEventCreateWithFlags(&c->evFront, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&c->evMark, hipEventDisableTiming) != hipSuccess) {
-        // the allocation phase then runs on the main stream
+        hipEventCreateWithFlags(&c->evMark, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c->evMc, hipEventDisableTiming) != hipSuccess) {
+        // the frame then runs on the main stream
         if (c->side) hipStreamDestroy(c->side);
-        if (c->evFront) hipEventDestroy(c->evFront);
+        if (c->evMark) hipEventDestroy(c->evMark);
         c->side = nullptr;
-        c->evFront = c->evMark = nullptr;
+        c->evMark = c->evMc = nullptr;
         (void)hipGetLastError();
     }
     const size_t NL = c->nLeavesTotal;
@@ -1627,8 +1627,8 @@ void ff_destroy(ff_ctx *c)
         hipStreamSynchronize(c->side);
         hipStreamDestroy(c->side);
     }
-    if (c->evFront) hipEventDestroy(c->evFront);
     if (c->evMark) hipEventDestroy(c->evMark);
+    if (c->evMc) hipEventDestroy(c->evMc);
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;
 }
@@ -1763,7 +1763,7 @@ void launch_alloc_batch(hipStream_t st, const AllocStream *d_as, const AllocStre
 // the SDF update waits for the host only when the node table overflowed or
 // the volume must grow first; carry: a direct launch, which also takes the
 // previous mesh update's counters up
-static void launch_sdf_only(ff_ctx *c, int W, int H, bool carry)
+static void launch_sdf_only(ff_ctx *c, int W, int H, bool carry, hipStream_t st)
 {
     SdfGate gate;
     gate.listCap = c->hv.listCap;
@@ -1772,8 +1772,8 @@ static void launch_sdf_only(ff_ctx *c, int W, int H, bool carry)
     gate.force = 0;
     const uint4 *cs = carry ? static_cast<const uint4 *>(c->carrySrc) : nullptr;
     uint4 *cd = carry ? static_cast<uint4 *>(c->carryDst) : nullptr;
-    k_sdf<<<FF_SDF_GRID, 512, 0, c->stream>>>(c->d_queue, c->mapView(), c->d_fd, W, H, c->d_fc, c->d_mapc, gate,
-                                              c->d_hfc, cs, cd);
+    k_sdf<<<FF_SDF_GRID, 512, 0, st>>>(c->d_queue, c->mapView(), c->d_fd, W, H, c->d_fc, c->d_mapc, gate, c->d_hfc,
+                                       cs, cd);
     if (cs) c->carrySrc = c->carryDst = nullptr;
 }
 
@@ -1781,8 +1781,8 @@ static void launch_sdf_only(ff_ctx *c, int W, int H, bool carry)
 // outside, after the timing event)
 void launch_sdf(ff_ctx *c, int W, int H)
 {
-    if (c->learned) learned_launch(c, W, H);
-    launch_sdf_only(c, W, H, false);
+    if (c->learned) learned_launch(c, W, H, c->stream);
+    launch_sdf_only(c, W, H, false, c->stream);
 }
 
 static bool capture(ff_ctx *c, void (*fn)(ff_ctx *, int, int), int W, int H, hipGraph_t *g, hipGraphExec_t *x)
@@ -1805,27 +1805,28 @@ static bool capture(ff_ctx *c, void (*fn)(ff_ctx *, int, int), int W, int H, hip
 // this size (the first frame and a new size prepare on the main stream)
 static bool front_overlap(const ff_ctx *c, int W, int H)
 {
-    return c->side && c->evFront && c->evMark && !c->useGraphs && !c->timingAll && c->n && c->imgW == W &&
+    return c->side && c->evMark && c->evMc && !c->useGraphs && !c->timingAll && c->n && c->imgW == W &&
            c->imgH == H;
 }
 
 static int run_frame(ff_ctx *c, int W, int H, bool overlap)
 {
     if (overlap) {
-        // the allocation phase beside the mesh update still in flight on the
-        // main stream: the front at once, the descent once the update has
-        // passed over the tree and the stamps; the SDF update (after the
-        // update's marching cubes on the main stream) waits for the phase
+        // the frame on the side stream, beside the mesh update still in
+        // flight on the main stream: the front at once, the descent once the
+        // update has passed over the tree and the stamps, the SDF update once
+        // its marching cubes (which read the bricks) are done; the update's
+        // hash turnover may still run beside it
         launch_front(c, W, H, c->side, true);  // carries the frame's parameters up
         if (c->markPending) HIPCHK(hipStreamWaitEvent(c->side, c->evMark, 0));
         c->markPending = false;
         launch_back(c, W, H, c->side);
-        HIPCHK(hipEventRecord(c->evFront, c->side));
-        HIPCHK(hipStreamWaitEvent(c->stream, c->evFront, 0));
-        if (c->learned) learned_launch(c, W, H);
-        if (c->timing) hipEventRecord(c->ev[2], c->stream);
-        launch_sdf_only(c, W, H, true);  // which leaves the counters in h_fc
-        if (c->timing) hipEventRecord(c->ev[3], c->stream);
+        if (c->learned) learned_launch(c, W, H, c->side);
+        if (c->mcPending) HIPCHK(hipStreamWaitEvent(c->side, c->evMc, 0));
+        c->mcPending = false;
+        if (c->timing) hipEventRecord(c->ev[2], c->side);
+        launch_sdf_only(c, W, H, true, c->side);  // which leaves the counters in h_fc
+        if (c->timing) hipEventRecord(c->ev[3], c->side);
         HIPCHK(hipGetLastError());
         return FF_OK;
     }
@@ -1847,10 +1848,10 @@ static int run_frame(ff_ctx *c, int W, int H, bool overlap)
     else launch_alloc(c, W, H);
     if (c->timingAll) hipEventRecord(c->ev[1], c->stream);
     // the SDF phase's events bracket k_sdf alone when launched directly
-    if (!g && c->learned) learned_launch(c, W, H);
+    if (!g && c->learned) learned_launch(c, W, H, c->stream);
     if (c->timing) hipEventRecord(c->ev[2], c->stream);
     if (g) HIPCHK(hipGraphLaunch(c->frameExec[1], c->stream));
-    else launch_sdf_only(c, W, H, true);  // which leaves the counters in h_fc
+    else launch_sdf_only(c, W, H, true, c->stream);  // which leaves the counters in h_fc
     if (c->timing) hipEventRecord(c->ev[3], c->stream);
     HIPCHK(hipGetLastError());
     return FF_OK;
@@ -2026,7 +2027,9 @@ static int add_map_impl(ff_ctx *c, const uint16_t *depth, const uint8_t *bgr, in
     } else {
         launch_empty(c, W, H);
     }
-    HIPCHK(hipStreamSynchronize(c->stream));
+    // (an overlapped frame ran on the side stream; work enqueued on the main
+    // stream from here on starts after it)
+    HIPCHK(hipStreamSynchronize(overlap && c->n ? c->side : c->stream));
     return frame_finish(c, W, H, true);
 }
 

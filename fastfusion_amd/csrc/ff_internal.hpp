@@ -136,15 +136,15 @@ struct ff_ctx {
     int imgW = 0, imgH = 0, capW = 0, capH = 0;
     std::vector<float> pxp, pyp;
     hipStream_t stream = nullptr;
-    // the frame's allocation phase (bit planes, de-dup, descent, node
-    // commit, queue) runs on `side`, beside the previous mesh update on
-    // `stream`: the descent waits for evMark, recorded after the update's
-    // passes over the tree and the stamps (its sampling and marching cubes
-    // read only bricks that already exist, which the allocation leaves
-    // alone); evFront joins the phase before the SDF update
+    // a frame (bit planes, de-dup, descent, node commit, queue, SDF update)
+    // runs on `side`, beside the previous mesh update on `stream`: the
+    // descent waits for evMark, recorded after the update's passes over the
+    // tree and the stamps (its sampling and marching cubes read only bricks
+    // that already exist, which the allocation leaves alone), the SDF update
+    // for evMc, recorded after its marching cubes
     hipStream_t side = nullptr;
-    hipEvent_t evFront = nullptr, evMark = nullptr;
-    bool markPending = false;  // evMark recorded since the last frame
+    hipEvent_t evMark = nullptr, evMc = nullptr;
+    bool markPending = false, mcPending = false;  // recorded since the last frame
     // map arrays
     uint32_t *d_tree = nullptr, *d_treeTmp = nullptr, *d_parent = nullptr, *d_stamp = nullptr, *d_queue = nullptr;
     ushort4 *d_leafinfo = nullptr;
@@ -199,7 +199,7 @@ namespace ff {
 // frame (allocated on first use), and the launch filling it from c->d_fd
 void learned_free(ff_ctx *c);
 const float *learned_prepare(ff_ctx *c, int W, int H);
-void learned_launch(ff_ctx *c, int W, int H);
+void learned_launch(ff_ctx *c, int W, int H, hipStream_t st);
 void sync_map_counters(ff_ctx *c);
 void drop_frame_graphs(ff_ctx *c);
 // a frame in phases (ff_fusion.hip): host preparation and parameter upload,

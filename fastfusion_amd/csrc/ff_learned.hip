@@ -675,12 +675,12 @@ const float *learned_prepare(ff_ctx *c, int W, int H)
 }
 
 // per-pixel confidence of the frame described by c->d_fd
-void learned_launch(ff_ctx *c, int W, int H)
+void learned_launch(ff_ctx *c, int W, int H, hipStream_t st)
 {
     LearnedState *L = c->learned;
     const int tiles = (W + 15) / 16 * H;
     const int blocks = std::min(2048, (tiles + 3) / 4);
-    k_confidence<<<blocks, 256, 0, c->stream>>>(c->d_fd, W, H, L->w.frag, L->w.bias, L->w.w4, L->conf);
+    k_confidence<<<blocks, 256, 0, st>>>(c->d_fd, W, H, L->w.frag, L->w.bias, L->w.w4, L->conf);
 }
 
 }  // namespace ff

@@ -170,6 +170,7 @@ struct MeshState {
     // an update whose device work is enqueued and whose host part
     // (mesh_finish) runs at the next mesh call, query or state change
     bool pending = false, pendingTimed = false;
+    bool carryRequested = false;  // the update's counters left for the next SDF update to take up
     // the map the update in flight meshes: the frame stamp and the tree
     // (frames fused before its host part runs must not be attributed to it)
     uint32_t launchStamp = 0, launchNB = 0, launchNL = 0, launchGrows = 0;
@@ -1801,6 +1802,15 @@ void mesh_launch_mc(ff_ctx *c)
 {
     MeshState *s = c->mesh;
     k_mc<<<2048, 256, 0, c->stream>>>(s->d_cells[s->nxt], s->d_geom, s->d_mc, s->d_sbuf, s->m, pools(s));
+    // the bricks are read: the next frame's SDF update may start
+    if (c->evMc) {
+        if (hipEventRecord(c->evMc, c->stream) == hipSuccess) {
+            c->mcPending = true;
+        } else {
+            (void)hipGetLastError();
+            hipStreamSynchronize(c->stream);
+        }
+    }
 }
 
 void mesh_launch_post(ff_ctx *c, bool timed)
@@ -1821,10 +1831,11 @@ void mesh_launch_post(ff_ctx *c, bool timed)
     if (c->timingAll) {
         hipMemcpyAsync(s->h_mc, s->d_mc, sizeof(MeshCounters), hipMemcpyDeviceToHost, c->stream);
     } else {
-        // the next SDF update on this stream carries the counters up (the
-        // host part copies them itself if none was launched by then)
+        // the next SDF update carries the counters up (the host part
+        // copies them itself if none was launched by then)
         c->carrySrc = s->d_mc;
         c->carryDst = s->d_hmc;
+        s->carryRequested = true;
     }
 }
 
@@ -1928,10 +1939,18 @@ int mesh_complete(ff_ctx *c)
     MeshState *s = c->mesh;
     if (!s || !s->pending) return FF_OK;
     s->pending = false;
-    MCHK(hipStreamSynchronize(c->stream));
-    if (c->carrySrc) {
-        MCHK(hipMemcpy(s->h_mc, s->d_mc, sizeof(MeshCounters), hipMemcpyDeviceToHost));
-        c->carrySrc = c->carryDst = nullptr;
+    // counters taken up by a frame's SDF update, which follows the update's
+    // marching cubes and had completed when its frame returned: the host
+    // part needs no wait (later work on the stream stays ordered after the
+    // update's hash turnover)
+    const bool carried = s->carryRequested && !c->carrySrc;
+    s->carryRequested = false;
+    if (!carried) {
+        MCHK(hipStreamSynchronize(c->stream));
+        if (c->carrySrc) {
+            MCHK(hipMemcpy(s->h_mc, s->d_mc, sizeof(MeshCounters), hipMemcpyDeviceToHost));
+            c->carrySrc = c->carryDst = nullptr;
+        }
     }
     const int r = mesh_finish(c, s->pendingTimed);
     return r < 0 ? r : FF_OK;
