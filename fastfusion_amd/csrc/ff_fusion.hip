@@ -1154,7 +1154,7 @@ __device__ __forceinline__ bool sdf_gate_open(const GateView &v, uint32_t n, con
 __global__ __launch_bounds__(512) void k_sdf(const uint32_t *__restrict__ queue, MapView m,
                                              const FrameDev *__restrict__ fd, int W, int H, FrameCounters *fc,
                                              const MapCounters *__restrict__ mapc, SdfGate gate,
-                                             FrameCounters *hostFc, uint4 *carrySrc, uint4 *carryDst)
+                                             FrameCounters *hostFc, const uint4 *carrySrc, uint4 *carryDst)
 {
     rz_mode_on();  // first: every float operation below rounds toward zero
     // the counters, the rcp table's entries and the first queue slot are
@@ -1174,11 +1174,9 @@ __global__ __launch_bounds__(512) void k_sdf(const uint32_t *__restrict__ queue,
         h[3] = make_uint4(open ? 1u : 0u, 0u, 0u, 0u);
     }
     if (carrySrc && blockIdx.x == 0 && threadIdx.x < 2) {
-        // the previous mesh update's counters (its marching cubes precede
-        // this update) to host memory, in place of a separate copy, and
-        // zeros for the next update to count from
+        // the previous mesh update's counters (its kernels precede this one
+        // on the stream) to host memory, in place of a separate copy
         carryDst[threadIdx.x] = carrySrc[threadIdx.x];
-        carrySrc[threadIdx.x] = make_uint4(0u, 0u, 0u, 0u);
     }
     if (!open) return;
     const uint16_t *__restrict__ depth = fd->depth;
@@ -1772,7 +1770,7 @@ static void launch_sdf_only(ff_ctx *c, int W, int H, bool carry, hipStream_t st)
     gate.leavesTotal = c->nLeavesTotal;
     gate.branchesTotal = c->nBranchesTotal;
     gate.force = 0;
-    uint4 *cs = carry ? static_cast<uint4 *>(c->carrySrc) : nullptr;
+    const uint4 *cs = carry ? static_cast<const uint4 *>(c->carrySrc) : nullptr;
     uint4 *cd = carry ? static_cast<uint4 *>(c->carryDst) : nullptr;
     k_sdf<<<FF_SDF_GRID, 512, 0, st>>>(c->d_queue, c->mapView(), c->d_fd, W, H, c->d_fc, c->d_mapc, gate, c->d_hfc,
                                        cs, cd);

@@ -160,9 +160,8 @@ struct ff_ctx {
     ff::FrameCounters *d_fc = nullptr, *h_fc = nullptr;
     ff::FrameCounters *d_hfc = nullptr;  // h_fc (mapped host memory) as the device addresses it
     // 32 bytes the next directly launched SDF update copies from device to
-    // (mapped) host memory and then zeroes: the enqueued mesh update's
-    // counters
-    void *carrySrc = nullptr;
+    // (mapped) host memory: the enqueued mesh update's counters
+    const void *carrySrc = nullptr;
     void *carryDst = nullptr;
     ff::FrameDev *d_fd = nullptr, *h_fd = nullptr;     // h_fd: pinned staging
     ff::MapCounters *d_mapc = nullptr, *h_mapc = nullptr;

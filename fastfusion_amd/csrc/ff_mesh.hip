@@ -171,10 +171,6 @@ struct MeshState {
     // (mesh_finish) runs at the next mesh call, query or state change
     bool pending = false, pendingTimed = false;
     bool carryRequested = false;  // the update's counters left for the next SDF update to take up
-    // the device counters are zero (taken up and cleared by an SDF update),
-    // so the next update needs no clearing launch; beginZero: the update in
-    // flight started from such counters
-    bool countersZero = false, beginZero = false;
     // the map the update in flight meshes: the frame stamp and the tree
     // (frames fused before its host part runs must not be attributed to it)
     uint32_t launchStamp = 0, launchNB = 0, launchNL = 0, launchGrows = 0;
@@ -873,13 +869,11 @@ __global__ void k_mark_b(const MeshBatchStream *__restrict__ S)
 // (k_mark's test with the previous table's own record).  A dirty cell's mesh
 // range is cleared first, like the fresh record k_cells writes: k_mc fills
 // it unless the new mesh is empty.
-// (the counters start from zero; block 0 records the table's nCells)
 __global__ void k_mark_same(CellRec *cells, const uint32_t *__restrict__ stamp, const uint32_t *__restrict__ parent,
-                            uint32_t since, MeshCounters *mc, uint32_t nCells, CellGeom *geom, uint32_t cap, MapRO m,
-                            uint32_t *tiles, uint32_t tileCap)
+                            uint32_t since, MeshCounters *mc, CellGeom *geom, uint32_t cap, MapRO m, uint32_t *tiles,
+                            uint32_t tileCap)
 {
-    if (blockIdx.x == 0 && threadIdx.x == 0) mc->nCells = nCells;
-    const uint32_t nc = min(nCells, cap);
+    const uint32_t nc = min(mc->nCells, cap);
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t stride = gridDim.x * blockDim.x;
     for (uint32_t ib = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); ib < nc; ib += stride) {
@@ -1634,6 +1628,18 @@ __global__ void k_mesh_begin(MeshCounters *mc)
     }
 }
 
+// the counters of an update that re-marks the current table of nCells cells
+__global__ void k_mesh_begin_same(MeshCounters *mc, uint32_t nCells)
+{
+    if (threadIdx.x == 0) {
+        MeshCounters z;
+        z.nTerm = z.nDirty = z.flags = z.nTiles = 0u;
+        z.nCells = nCells;
+        z.pad[0] = z.pad[1] = z.pad[2] = 0u;
+        *mc = z;
+    }
+}
+
 __global__ void k_mesh_begin_b(const MeshBatchStream *__restrict__ S)
 {
     if (threadIdx.x == 0) {
@@ -1693,8 +1699,6 @@ static bool mesh_pre_host(ff_ctx *c, bool allowReuse)
     if (!s->haveVolume || !c->n) return false;
     s->reuse = allowReuse && s->nCells[s->cur] > 0 && s->builtNB == c->nBranchesUsed &&
                s->builtNL == c->nLeavesUsed && s->builtGrows == c->grows;
-    s->beginZero = s->countersZero;
-    s->countersZero = false;
     s->launchStamp = c->stampCounter;
     s->launchNB = c->nBranchesUsed;
     s->launchNL = c->nLeavesUsed;
@@ -1718,15 +1722,15 @@ bool mesh_launch_pre(ff_ctx *c, bool timed)
     if (s->reuse) {
         const MapRO m = s->m;
         const uint32_t nc = std::min(s->nCells[s->cur], s->cellCap);
-        if (!s->beginZero) k_mesh_begin<<<1, 64, 0, c->stream>>>(s->d_mc);
+        k_mesh_begin_same<<<1, 64, 0, c->stream>>>(s->d_mc, nc);
         k_mark_same<<<std::max(1u, std::min(4096u, (nc + 255) / 256)), 256, 0, c->stream>>>(
-            s->d_cells[nxt], c->d_stamp, c->d_parent, s->lastMeshStamp, s->d_mc, nc, s->d_geom, s->cellCap, m,
-            s->d_tiles, s->tileCap);
+            s->d_cells[nxt], c->d_stamp, c->d_parent, s->lastMeshStamp, s->d_mc, s->d_geom, s->cellCap, m, s->d_tiles,
+            s->tileCap);
         k_sample<<<4096, 256, 0, c->stream>>>(s->d_geom, s->d_cells[nxt], s->d_tiles, s->d_mc, s->tileCap, m,
                                               s->d_sbuf);
         return true;
     }
-    if (!s->beginZero) k_mesh_begin<<<1, 64, 0, c->stream>>>(s->d_mc);
+    k_mesh_begin<<<1, 64, 0, c->stream>>>(s->d_mc);
     const uint32_t nBlocks = c->nBranchesUsed / 16;
     k_terminal<<<(nBlocks * 8 + 255) / 256, 256, 0, c->stream>>>(c->d_tree, s->d_binfo, nBlocks, c->nLeavesTotal, c->n,
                                                                  s->d_term, s->termCap, s->d_mc);
@@ -1941,7 +1945,6 @@ int mesh_complete(ff_ctx *c)
     // update's hash turnover)
     const bool carried = s->carryRequested && !c->carrySrc;
     s->carryRequested = false;
-    s->countersZero = carried;
     if (!carried) {
         MCHK(hipStreamSynchronize(c->stream));
         if (c->carrySrc) {
