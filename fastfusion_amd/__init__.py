@@ -66,6 +66,8 @@ ABI = {
     "ff_copy_cells": (ctypes.c_int, [_P, _P, _P, _P]),
     "ff_set_incremental_meshing": (ctypes.c_int, [_P, ctypes.c_int]),
     "ff_set_mesh_sample_tiles": (ctypes.c_int, [_P, ctypes.c_uint32]),
+    "ff_set_mesh_pool": (ctypes.c_int, [_P, ctypes.c_uint32]),
+    "ff_mesh_pool_info": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_uint32)]),
     "ff_get_counts": (ctypes.c_int, [_P, ctypes.POINTER(_Counts)]),
     "ff_copy_tree": (ctypes.c_int, [_P, _P, _SZ]),
     "ff_copy_leaves": (ctypes.c_int, [_P, _P, _P, _P, _SZ]),
@@ -303,6 +305,17 @@ class FusionMipMapGPU:
         """Capacity of the meshing sample buffer in 64-sample tiles (no
         counterpart in the reference; cells beyond it are meshed in place)."""
         self._check(self._lib.ff_set_mesh_sample_tiles(self._ctx, int(tiles)))
+
+    def setMeshPool(self, vertices):
+        """Capacity of the mesh pools in vertices and triangles (no
+        counterpart in the reference; the next update meshes every cell)."""
+        self._check(self._lib.ff_set_mesh_pool(self._ctx, int(vertices)))
+
+    def meshPoolInfo(self):
+        """(pool capacity, compactions so far)."""
+        v, n = ctypes.c_uint32(), ctypes.c_uint32()
+        self._check(self._lib.ff_mesh_pool_info(self._ctx, ctypes.byref(v), ctypes.byref(n)))
+        return v.value, n.value
 
     # -- getMeshInterleavedMarchingCubes, :3359-3374
     def getMeshInterleavedMarchingCubes(self):

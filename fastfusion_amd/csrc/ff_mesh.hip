@@ -146,6 +146,7 @@ struct MeshState {
     float4 *d_vpool = nullptr;    // x, y, z, rgb packed
     uint32_t *d_fpool = nullptr;  // 3 local indices per face
     uint32_t vCap = 0, fCap = 0;  // FF_ARENAS arenas of vCap / FF_ARENAS, fCap / FF_ARENAS
+    uint32_t compactions = 0;     // live-geometry compactions so far
     unsigned long long *d_arena = nullptr;
     MeshCounters *d_mc = nullptr, *h_mc = nullptr;
     MeshCounters *d_hmc = nullptr;  // h_mc (mapped host memory) as the device addresses it
@@ -1924,6 +1925,7 @@ int mesh_finish(ff_ctx *c, bool timed)
         hipFree(s->d_fpool);
         s->d_vpool = nv;
         s->d_fpool = nf;
+        s->compactions++;
         if (flags & FF_MESH_OVERFLOW) {
             // cells that did not fit were stored empty; mesh everything again next time
             s->nCells[nxt] = 0;
@@ -2133,6 +2135,48 @@ int ff_set_mesh_sample_tiles(ff_ctx *c, uint32_t tiles)
     s->d_tiles = nt;
     s->d_sbuf = nb;
     s->tileCap = tiles;
+    return FF_OK;
+}
+
+int ff_set_mesh_pool(ff_ctx *c, uint32_t vertices)
+{
+    if (!c || !c->mesh || vertices < 64u * FF_ARENAS) {
+        if (c) c->err = "ff_set_mesh_pool: at least 4096 vertices";
+        return FF_E_ARG;
+    }
+    hipSetDevice(c->device);
+    MeshState *s = c->mesh;
+    const int rc = mesh_complete(c);
+    if (rc != FF_OK) return rc;
+    MCHK(hipStreamSynchronize(c->stream));
+    const uint32_t cap = vertices - vertices % FF_ARENAS;
+    float4 *nv = nullptr;
+    uint32_t *nf = nullptr;
+    MCHK(hipMalloc(&nv, sizeof(float4) * cap));
+    if (!mok(c, hipMalloc(&nf, sizeof(uint32_t) * 3 * (size_t)cap), "hipMalloc(face pool)")) {
+        hipFree(nv);
+        return FF_E_DEVICE;
+    }
+    hipFree(s->d_vpool);
+    hipFree(s->d_fpool);
+    s->d_vpool = nv;
+    s->d_fpool = nf;
+    s->vCap = s->fCap = cap;
+    MCHK(hipMemset(s->d_arena, 0, sizeof(unsigned long long) * FF_ARENAS * FF_ARENA_STRIDE));
+    // the geometry of the old pools is gone: the next update meshes every cell
+    s->nCells[s->cur] = 0;
+    MCHK(hipMemset(s->d_hkeys, 0xff, sizeof(uint64_t) * (s->hmask + 1)));
+    s->lastVAlloc = s->lastFAlloc = 0;
+    return FF_OK;
+}
+
+int ff_mesh_pool_info(ff_ctx *c, uint32_t *vertices, uint32_t *compactions)
+{
+    if (!c || !c->mesh) return FF_E_ARG;
+    const int rc = mesh_complete(c);
+    if (rc != FF_OK) return rc;
+    if (vertices) *vertices = c->mesh->vCap;
+    if (compactions) *compactions = c->mesh->compactions;
     return FF_OK;
 }
 

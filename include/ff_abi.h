@@ -103,9 +103,21 @@ int ff_set_incremental_meshing(ff_ctx *ctx, int on);
 /* Device memory knob with no counterpart in the reference (whose meshing
  * samples cells on the host): the capacity, in tiles of 64 samples (1 KiB),
  * of the buffer that holds the samples of the cells re-meshed by one
- * updateMeshes.  Cells beyond it are sampled in place by a second meshing
- * pass and the buffer then grows; the default is sized from max_leaves. */
+ * updateMeshes.  Cells beyond it are sampled in place by the marching cubes
+ * pass itself and the buffer then grows; the default is sized from
+ * max_leaves. */
 int ff_set_mesh_sample_tiles(ff_ctx *ctx, uint32_t tiles);
+
+/* Device memory knob with no counterpart in the reference (whose cells own
+ * their meshes on the heap): the capacity, in vertices and in triangles, of
+ * the pools the cells' meshes live in (at least 4096; rounded down to a
+ * multiple of 64).  When the space allocated since the last compaction
+ * passes half of it, an update compacts the live geometry into fresh pools.
+ * Setting it drops the current geometry: the next updateMeshes meshes every
+ * cell.  The default is sized from max_leaves. */
+int ff_set_mesh_pool(ff_ctx *ctx, uint32_t vertices);
+/* The pool capacity and the number of compactions so far. */
+int ff_mesh_pool_info(ff_ctx *ctx, uint32_t *vertices, uint32_t *compactions);
 
 /* Map state, for inspection and parity (the reference's protected members
  * _tree, _leafPos, _leafScale, _leafParent, _distance, _weights, _color). */

@@ -32,10 +32,14 @@ def compare_mesh(gpu, ora, voxel, tag):
     return len(desc), len(m.faces)
 
 
-def run_mesh(sc, R, t, frames, cfg, K, check_every=1, max_leaves=1 << 17, sample_tiles=None):
+def run_mesh(sc, R, t, frames, cfg, K, check_every=1, max_leaves=1 << 17, sample_tiles=None, pool=None, out_gpu=None):
     gpu, ora = make_pair(cfg["scale"], cfg["threshold"], max_leaves=max_leaves)
     if sample_tiles is not None:
         gpu.setMeshSampleTiles(sample_tiles)
+    if pool is not None:
+        gpu.setMeshPool(pool)
+    if out_gpu is not None:
+        out_gpu.append(gpu)
     w, h = cfg["w"], cfg["h"]
     out = None
     for k, i in enumerate(frames):
@@ -117,3 +121,25 @@ def test_mesh_unchanged_tree_remarks_in_place():
         gpu.updateMeshes()
         compare_mesh(gpu, ora, cfg["scale"], f"step {k} (frame {i})")
     assert unchanged >= 3
+
+
+def test_mesh_compaction_behind_overlapped_frames():
+    """Mesh pools a few times the live mesh, so that updates compact the live
+    geometry into fresh pools, compared only after the last frame: every
+    update but the last then finishes (its host part, with the compaction)
+    after the next frame took its counters up, without waiting for the
+    update's own stream.  The meshes must still equal the oracle's."""
+    sc, R, t, cfg = synth.config_c1(n_frames=200)
+    cfg = dict(cfg, w=320, h=240)
+    K = (262.5, 262.5, 159.5, 119.5)
+    frames = list(range(0, 24, 2))
+    ref = []
+    run_mesh(sc, R, t, frames, cfg, K, check_every=len(frames), out_gpu=ref)
+    live = len(ref[0].getMeshInterleavedMarchingCubes().vertices)
+    assert live > 2000
+    pool = 64 * (4 * live // 64 + 1)
+    print(f"mesh compaction: {live} live vertices, pools of {pool}", flush=True)
+    got = []
+    run_mesh(sc, R, t, frames, cfg, K, check_every=len(frames), pool=pool, out_gpu=got)
+    cap, compactions = got[0].meshPoolInfo()
+    assert cap == pool and compactions >= 1, f"pools of {cap} compacted {compactions} times"
