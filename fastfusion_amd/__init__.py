@@ -66,7 +66,7 @@ ABI = {
     "ff_copy_cells": (ctypes.c_int, [_P, _P, _P, _P]),
     "ff_set_incremental_meshing": (ctypes.c_int, [_P, ctypes.c_int]),
     "ff_set_mesh_sample_tiles": (ctypes.c_int, [_P, ctypes.c_uint32]),
-    "ff_set_mesh_pool": (ctypes.c_int, [_P, ctypes.c_uint32]),
+    "ff_set_mesh_pool": (ctypes.c_int, [_P, ctypes.c_uint32, ctypes.c_uint32]),
     "ff_mesh_pool_info": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_uint32), ctypes.POINTER(ctypes.c_uint32)]),
     "ff_get_counts": (ctypes.c_int, [_P, ctypes.POINTER(_Counts)]),
     "ff_copy_tree": (ctypes.c_int, [_P, _P, _SZ]),
@@ -306,10 +306,12 @@ class FusionMipMapGPU:
         counterpart in the reference; cells beyond it are meshed in place)."""
         self._check(self._lib.ff_set_mesh_sample_tiles(self._ctx, int(tiles)))
 
-    def setMeshPool(self, vertices):
-        """Capacity of the mesh pools in vertices and triangles (no
-        counterpart in the reference; the next update meshes every cell)."""
-        self._check(self._lib.ff_set_mesh_pool(self._ctx, int(vertices)))
+    def setMeshPool(self, vertices, compact_percent=50):
+        """Capacity of the mesh pools in vertices and triangles, and the
+        share of an arena whose use since the last compaction makes an update
+        compact (no counterpart in the reference; the next update meshes
+        every cell)."""
+        self._check(self._lib.ff_set_mesh_pool(self._ctx, int(vertices), int(compact_percent)))
 
     def meshPoolInfo(self):
         """(pool capacity, compactions so far)."""

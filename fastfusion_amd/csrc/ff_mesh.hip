@@ -61,6 +61,7 @@ struct Pools {
     uint32_t *f;                  // 3 cell-local indices per face
     unsigned long long *arena;    // FF_ARENAS counters, FF_ARENA_STRIDE apart
     uint32_t vA, fA;              // capacity of one arena
+    uint32_t vH, fH;              // use of one arena that asks for a compaction
 };
 
 // reserves nv vertices and nf faces in arena k; false when they do not fit
@@ -75,7 +76,7 @@ __device__ __forceinline__ bool pool_alloc(const Pools &P, uint32_t k, uint32_t 
         atomicOr(&mc->flags, FF_MESH_OVERFLOW);
         return false;
     }
-    const uint32_t vh = P.vA / 2, fh = P.fA / 2;
+    const uint32_t vh = P.vH, fh = P.fH;
     if ((vo <= vh && vo + nv > vh) || (fo <= fh && fo + nf > fh)) atomicOr(&mc->flags, FF_MESH_HALF);
     v0 = k * P.vA + vo;
     f0 = k * P.fA + fo;
@@ -87,6 +88,7 @@ __device__ __forceinline__ bool pool_alloc(const Pools &P, uint32_t k, uint32_t 
 __constant__ uint64_t c_tri64[256];
 
 #define FF_TILE_CAP_MAX (1u << 22)  // 4 GiB of samples
+#define FF_POOL_MAX (1u << 29)      // vertices (8 GiB) and triangles (6 GiB) the pools grow to
 
 // read-only view of the map for sampling
 struct MapRO {
@@ -130,6 +132,7 @@ struct MeshState {
     ushort4 *d_binfo = nullptr;  // per branch block: origin of the node, octant size
     uint32_t binfoCap = 0;
     uint4 *d_term = nullptr;     // x, y, z, (size << 0) ; .w = covering leaf, size packed in x high bits
+    uint4 *d_jump = nullptr;     // FF_JUMP_CELLS descent entries of the current tree
     uint32_t *d_termSize = nullptr;
     uint32_t termCap = 0;
     CellRec *d_cells[2] = {nullptr, nullptr};
@@ -147,6 +150,7 @@ struct MeshState {
     uint32_t *d_fpool = nullptr;  // 3 local indices per face
     uint32_t vCap = 0, fCap = 0;  // FF_ARENAS arenas of vCap / FF_ARENAS, fCap / FF_ARENAS
     uint32_t compactions = 0;     // live-geometry compactions so far
+    uint32_t compactPercent = 50; // an arena used this far since the last compaction asks for one
     unsigned long long *d_arena = nullptr;
     MeshCounters *d_mc = nullptr, *h_mc = nullptr;
     MeshCounters *d_hmc = nullptr;  // h_mc (mapped host memory) as the device addresses it
@@ -189,6 +193,71 @@ __device__ uint32_t locate(const uint32_t *__restrict__ tree, uint32_t nLeavesTo
     if (x < 0 || y < 0 || z < 0 || x >= (int)n || y >= (int)n || z >= (int)n) return 0u;
     uint32_t half = n >> 1, branch = 0;
     int ox = 0, oy = 0, oz = 0;
+    for (;;) {
+        uint32_t co = 0;
+        if (x >= ox + (int)half) { co |= 1u; ox += half; }
+        if (y >= oy + (int)half) { co |= 2u; oy += half; }
+        if (z >= oz + (int)half) { co |= 4u; oz += half; }
+        const uint32_t l = tree[branch + co];
+        if (l < nLeavesTotal) *leaf = l;
+        const uint32_t child = tree[branch + co + 8];
+        if (child == FF_BRANCHINIT || half <= FF_BL) {
+            org[0] = ox; org[1] = oy; org[2] = oz;
+            return half;
+        }
+        branch = child;
+        half >>= 1;
+    }
+}
+
+// Jump table for descents: the volume as a 16 x 16 x 16 grid of cells of side
+// n / 16 (n >= 16, a power of two), each with what the first four levels of
+// a descent to any of its voxels give: the branch to go on from (half size
+// n / 32) and the deepest brick so far, or, for a descent that ends within
+// them, no branch and the terminal node's half size.  Nodes of those levels
+// are aligned to their size, so every voxel of a cell takes the same octants.
+#define FF_JUMP_N 16
+#define FF_JUMP_CELLS (FF_JUMP_N * FF_JUMP_N * FF_JUMP_N)
+
+__device__ uint4 jump_entry(const uint32_t *__restrict__ tree, uint32_t nLeavesTotal, uint32_t n, uint32_t cell)
+{
+    const uint32_t cs = n / FF_JUMP_N;
+    const int x = (int)((cell % FF_JUMP_N) * cs), y = (int)(((cell / FF_JUMP_N) % FF_JUMP_N) * cs),
+              z = (int)((cell / (FF_JUMP_N * FF_JUMP_N)) * cs);
+    uint32_t leaf = FF_BRANCHINIT, half = n >> 1, branch = 0;
+    int ox = 0, oy = 0, oz = 0;
+    for (int lv = 0; lv < 4; lv++) {
+        uint32_t co = 0;
+        if (x >= ox + (int)half) { co |= 1u; ox += half; }
+        if (y >= oy + (int)half) { co |= 2u; oy += half; }
+        if (z >= oz + (int)half) { co |= 4u; oz += half; }
+        const uint32_t l = tree[branch + co];
+        if (l < nLeavesTotal) leaf = l;
+        const uint32_t child = tree[branch + co + 8];
+        if (child == FF_BRANCHINIT || half <= FF_BL) return make_uint4(FF_BRANCHINIT, leaf, half, 0u);
+        branch = child;
+        half >>= 1;
+    }
+    return make_uint4(branch, leaf, 0u, 0u);
+}
+
+// locate() starting from the jump table: the same result, four dependent
+// tree loads fewer
+__device__ uint32_t locate_jump(const uint4 *__restrict__ jump, const uint32_t *__restrict__ tree,
+                                uint32_t nLeavesTotal, uint32_t n, int x, int y, int z, int org[3], uint32_t *leaf)
+{
+    *leaf = FF_BRANCHINIT;
+    if (x < 0 || y < 0 || z < 0 || x >= (int)n || y >= (int)n || z >= (int)n) return 0u;
+    const uint32_t cs = n / FF_JUMP_N;
+    const uint4 e = jump[((uint32_t)z / cs * FF_JUMP_N + (uint32_t)y / cs) * FF_JUMP_N + (uint32_t)x / cs];
+    *leaf = e.y;
+    if (e.x == FF_BRANCHINIT) {
+        const int m = ~((int)e.z - 1);
+        org[0] = x & m; org[1] = y & m; org[2] = z & m;
+        return e.z;
+    }
+    uint32_t half = n >> 5, branch = e.x;
+    int ox = x & ~((int)cs - 1), oy = y & ~((int)cs - 1), oz = z & ~((int)cs - 1);
     for (;;) {
         uint32_t co = 0;
         if (x >= ox + (int)half) { co |= 1u; ox += half; }
@@ -286,10 +355,17 @@ __device__ __forceinline__ void terminal_body(const uint32_t *__restrict__ tree,
     }
 }
 
+// (blocks past the terminal candidates fill the jump table, 256 cells each)
 __global__ void k_terminal(const uint32_t *__restrict__ tree, const ushort4 *__restrict__ binfo, uint32_t nBlocks,
                            uint32_t nLeavesTotal, uint32_t n, uint4 *__restrict__ term, uint32_t termCap,
-                           MeshCounters *mc)
+                           MeshCounters *mc, uint4 *__restrict__ jump)
 {
+    const uint32_t tb = (nBlocks * 8u + blockDim.x - 1u) / blockDim.x;
+    if (blockIdx.x >= tb) {
+        const uint32_t cell = (blockIdx.x - tb) * blockDim.x + threadIdx.x;
+        if (cell < FF_JUMP_CELLS) jump[cell] = jump_entry(tree, nLeavesTotal, n, cell);
+        return;
+    }
     terminal_body(tree, binfo, nBlocks, nLeavesTotal, n, term, termCap, mc);
 }
 
@@ -312,7 +388,8 @@ __global__ void k_terminal_b(const MeshBatchStream *__restrict__ S)
 // ownership from them; the block appends its cells with one atomic.
 __device__ __forceinline__ int opt_code(int ox, int oy, int oz) { return ox + 4 * oy + 16 * oz; }
 
-__device__ __forceinline__ void cells_body(const uint32_t *__restrict__ tree, uint32_t nLeavesTotal, uint32_t n,
+__device__ __forceinline__ void cells_body(const uint4 *__restrict__ jump, const uint32_t *__restrict__ tree,
+                                           uint32_t nLeavesTotal, uint32_t n,
                                            const uint4 *__restrict__ term, uint32_t termCap, MeshCounters *mc,
                                            CellRec *cells, uint32_t cap)
 {
@@ -338,7 +415,8 @@ __device__ __forceinline__ void cells_body(const uint32_t *__restrict__ tree, ui
                         q[k] = op[k] == 0 ? p[k] - 1 : (op[k] == 1 ? p[k] : (op[k] == 2 ? p[k] + sz - 1 : p[k] + sz));
                     int org[3];
                     uint32_t lf;
-                    r.x = locate(tree, nLeavesTotal, n, q[0], q[1], q[2], org, &lf);
+                    r.x = jump ? locate_jump(jump, tree, nLeavesTotal, n, q[0], q[1], q[2], org, &lf)
+                               : locate(tree, nLeavesTotal, n, q[0], q[1], q[2], org, &lf);
                     r.y = lf;
                 }
             }
@@ -437,15 +515,15 @@ __device__ __forceinline__ void cells_body(const uint32_t *__restrict__ tree, ui
 
 __global__ __launch_bounds__(512) void k_cells(const uint32_t *__restrict__ tree, uint32_t nLeavesTotal, uint32_t n,
                                                const uint4 *__restrict__ term, uint32_t termCap, MeshCounters *mc,
-                                               CellRec *cells, uint32_t cap)
+                                               CellRec *cells, uint32_t cap, const uint4 *__restrict__ jump)
 {
-    cells_body(tree, nLeavesTotal, n, term, termCap, mc, cells, cap);
+    cells_body(jump, tree, nLeavesTotal, n, term, termCap, mc, cells, cap);
 }
 
 __global__ __launch_bounds__(512) void k_cells_b(const MeshBatchStream *__restrict__ S)
 {
     const MeshBatchStream &a = S[blockIdx.z];
-    cells_body(a.tree, a.nLeavesTotal, a.n, a.term, a.termCap, a.mc, a.cells, a.cellCap);
+    cells_body(nullptr, a.tree, a.nLeavesTotal, a.n, a.term, a.termCap, a.mc, a.cells, a.cellCap);
 }
 
 // ---------------------------------------------------------------------------
@@ -1505,6 +1583,7 @@ int mesh_init(ff_ctx *c)
     MCHK(hipMemset(s->d_binfo, 0, sizeof(ushort4) * s->binfoCap));
     s->termCap = std::max<uint32_t>(1u << 16, std::min<uint32_t>(c->nLeavesTotal * 2u, 1u << 24));
     MCHK(hipMalloc(&s->d_term, sizeof(uint4) * s->termCap));
+    MCHK(hipMalloc(&s->d_jump, sizeof(uint4) * FF_JUMP_CELLS));
     s->cellCap = std::max<uint32_t>(1u << 18, std::min<uint32_t>(c->nLeavesTotal * 16u, 1u << 25));
     for (int k = 0; k < 2; k++) MCHK(hipMalloc(&s->d_cells[k], sizeof(CellRec) * s->cellCap));
     MCHK(hipMalloc(&s->d_geom, sizeof(CellGeom) * s->cellCap));
@@ -1536,7 +1615,7 @@ void mesh_free(ff_ctx *c)
 {
     MeshState *s = c->mesh;
     if (!s) return;
-    hipFree(s->d_binfo); hipFree(s->d_term);
+    hipFree(s->d_binfo); hipFree(s->d_term); hipFree(s->d_jump);
     hipFree(s->d_cells[0]); hipFree(s->d_cells[1]);
     hipFree(s->d_geom); hipFree(s->d_tiles); hipFree(s->d_sbuf);
     hipFree(s->d_hkeys); hipFree(s->d_hval); hipFree(s->d_vpool); hipFree(s->d_fpool);
@@ -1686,6 +1765,8 @@ static Pools pools(const MeshState *s)
     P.arena = s->d_arena;
     P.vA = s->vCap / FF_ARENAS;
     P.fA = s->fCap / FF_ARENAS;
+    P.vH = (uint32_t)((uint64_t)P.vA * s->compactPercent / 100u);
+    P.fH = (uint32_t)((uint64_t)P.fA * s->compactPercent / 100u);
     return P;
 }
 
@@ -1733,10 +1814,11 @@ bool mesh_launch_pre(ff_ctx *c, bool timed)
     }
     k_mesh_begin<<<1, 64, 0, c->stream>>>(s->d_mc);
     const uint32_t nBlocks = c->nBranchesUsed / 16;
-    k_terminal<<<(nBlocks * 8 + 255) / 256, 256, 0, c->stream>>>(c->d_tree, s->d_binfo, nBlocks, c->nLeavesTotal, c->n,
-                                                                 s->d_term, s->termCap, s->d_mc);
+    // (plus FF_JUMP_CELLS / 256 blocks that fill the jump table)
+    k_terminal<<<(nBlocks * 8 + 255) / 256 + FF_JUMP_CELLS / 256, 256, 0, c->stream>>>(
+        c->d_tree, s->d_binfo, nBlocks, c->nLeavesTotal, c->n, s->d_term, s->termCap, s->d_mc, s->d_jump);
     k_cells<<<1024, 512, 0, c->stream>>>(c->d_tree, c->nLeavesTotal, c->n, s->d_term, s->termCap, s->d_mc,
-                                         s->d_cells[nxt], s->cellCap);
+                                         s->d_cells[nxt], s->cellCap, s->d_jump);
     const MapRO m = s->m;
     k_mark<<<1024, 256, 0, c->stream>>>(s->d_cells[nxt], s->d_cells[s->cur], s->d_hkeys, s->d_hval, s->hmask,
                                         s->nCells[s->cur] > 0, c->d_stamp, c->d_parent, s->lastMeshStamp, s->d_mc,
@@ -1911,12 +1993,16 @@ int mesh_finish(ff_ctx *c, bool timed)
         Pools dst = pools(s);
         dst.v = nv;
         dst.f = nf;
+        // the compaction's own overflow shows in the flags word, cleared first
+        MCHK(hipMemsetAsync(&s->d_mc->flags, 0, sizeof(uint32_t), c->stream));
         k_compact<<<1024, 256, 0, c->stream>>>(s->d_cells[nxt], s->d_mc, s->cellCap, s->d_vpool, s->d_fpool, dst);
         if (c->timingAll) {
             k_arena_sum<<<1, 64, 0, c->stream>>>(s->d_arena, s->d_mc);
             MCHK(hipMemcpyAsync(s->h_mc, s->d_mc, sizeof(MeshCounters), hipMemcpyDeviceToHost, c->stream));
         }
         MCHK(hipStreamSynchronize(c->stream));
+        uint32_t cflags = 0;
+        MCHK(hipMemcpy(&cflags, &s->d_mc->flags, sizeof cflags, hipMemcpyDeviceToHost));
         if (c->timingAll) {
             s->lastVAlloc = s->h_mc->pad[0];
             s->lastFAlloc = s->h_mc->pad[1];
@@ -1926,7 +2012,29 @@ int mesh_finish(ff_ctx *c, bool timed)
         s->d_vpool = nv;
         s->d_fpool = nf;
         s->compactions++;
-        if (flags & FF_MESH_OVERFLOW) {
+        const bool lost = (cflags & FF_MESH_OVERFLOW) != 0;
+        if ((lost || (flags & FF_MESH_OVERFLOW)) && s->vCap < FF_POOL_MAX) {
+            // the update's meshes or the live geometry itself did not fit an
+            // arena: twice the pools (empty: the next update meshes every cell,
+            // which also needs room beside the live geometry)
+            const uint32_t cap = (uint32_t)std::min<uint64_t>((uint64_t)s->vCap * 2u, FF_POOL_MAX);
+            float4 *gv = nullptr;
+            uint32_t *gf = nullptr;
+            if (hipMalloc(&gv, sizeof(float4) * cap) == hipSuccess &&
+                hipMalloc(&gf, sizeof(uint32_t) * 3 * (size_t)cap) == hipSuccess) {
+                hipFree(s->d_vpool);
+                hipFree(s->d_fpool);
+                s->d_vpool = gv;
+                s->d_fpool = gf;
+                s->vCap = s->fCap = cap;
+                MCHK(hipMemset(s->d_arena, 0, sizeof(unsigned long long) * FF_ARENAS * FF_ARENA_STRIDE));
+                s->lastVAlloc = s->lastFAlloc = 0;
+            } else {
+                hipFree(gv);
+                (void)hipGetLastError();
+            }
+        }
+        if ((flags & FF_MESH_OVERFLOW) || lost) {
             // cells that did not fit were stored empty; mesh everything again next time
             s->nCells[nxt] = 0;
             MCHK(hipMemsetAsync(s->d_hkeys, 0xff, sizeof(uint64_t) * (s->hmask + 1), c->stream));
@@ -2138,10 +2246,10 @@ int ff_set_mesh_sample_tiles(ff_ctx *c, uint32_t tiles)
     return FF_OK;
 }
 
-int ff_set_mesh_pool(ff_ctx *c, uint32_t vertices)
+int ff_set_mesh_pool(ff_ctx *c, uint32_t vertices, uint32_t compact_percent)
 {
-    if (!c || !c->mesh || vertices < 64u * FF_ARENAS) {
-        if (c) c->err = "ff_set_mesh_pool: at least 4096 vertices";
+    if (!c || !c->mesh || vertices < 64u * FF_ARENAS || compact_percent < 1 || compact_percent > 100) {
+        if (c) c->err = "ff_set_mesh_pool: at least 4096 vertices, compact_percent in [1, 100]";
         return FF_E_ARG;
     }
     hipSetDevice(c->device);
@@ -2162,6 +2270,7 @@ int ff_set_mesh_pool(ff_ctx *c, uint32_t vertices)
     s->d_vpool = nv;
     s->d_fpool = nf;
     s->vCap = s->fCap = cap;
+    s->compactPercent = compact_percent;
     MCHK(hipMemset(s->d_arena, 0, sizeof(unsigned long long) * FF_ARENAS * FF_ARENA_STRIDE));
     // the geometry of the old pools is gone: the next update meshes every cell
     s->nCells[s->cur] = 0;

@@ -111,11 +111,12 @@ int ff_set_mesh_sample_tiles(ff_ctx *ctx, uint32_t tiles);
 /* Device memory knob with no counterpart in the reference (whose cells own
  * their meshes on the heap): the capacity, in vertices and in triangles, of
  * the pools the cells' meshes live in (at least 4096; rounded down to a
- * multiple of 64).  When the space allocated since the last compaction
- * passes half of it, an update compacts the live geometry into fresh pools.
- * Setting it drops the current geometry: the next updateMeshes meshes every
- * cell.  The default is sized from max_leaves. */
-int ff_set_mesh_pool(ff_ctx *ctx, uint32_t vertices);
+ * multiple of 64; split into 64 arenas), and compact_percent (1-100,
+ * default 50): when an arena's space allocated since the last compaction
+ * passes that share of it, the update compacts the live geometry into fresh
+ * pools.  Setting it drops the current geometry: the next updateMeshes
+ * meshes every cell.  The default capacity is sized from max_leaves. */
+int ff_set_mesh_pool(ff_ctx *ctx, uint32_t vertices, uint32_t compact_percent);
 /* The pool capacity and the number of compactions so far. */
 int ff_mesh_pool_info(ff_ctx *ctx, uint32_t *vertices, uint32_t *compactions);
 

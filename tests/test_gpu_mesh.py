@@ -37,7 +37,7 @@ def run_mesh(sc, R, t, frames, cfg, K, check_every=1, max_leaves=1 << 17, sample
     if sample_tiles is not None:
         gpu.setMeshSampleTiles(sample_tiles)
     if pool is not None:
-        gpu.setMeshPool(pool)
+        gpu.setMeshPool(*pool)
     if out_gpu is not None:
         out_gpu.append(gpu)
     w, h = cfg["w"], cfg["h"]
@@ -124,22 +124,43 @@ def test_mesh_unchanged_tree_remarks_in_place():
 
 
 def test_mesh_compaction_behind_overlapped_frames():
-    """Mesh pools a few times the live mesh, so that updates compact the live
-    geometry into fresh pools, compared only after the last frame: every
-    update but the last then finishes (its host part, with the compaction)
-    after the next frame took its counters up, without waiting for the
-    update's own stream.  The meshes must still equal the oracle's."""
+    """Updates that compact the live geometry into fresh pools (arenas ask
+    for a compaction once 1 % of them is used), compared only after the last
+    frame: every update but the last then finishes (its host part, with the
+    compaction) after the next frame took its counters up, without waiting
+    for the update's own stream.  The meshes must still equal the oracle's."""
     sc, R, t, cfg = synth.config_c1(n_frames=200)
     cfg = dict(cfg, w=320, h=240)
     K = (262.5, 262.5, 159.5, 119.5)
     frames = list(range(0, 24, 2))
-    ref = []
-    run_mesh(sc, R, t, frames, cfg, K, check_every=len(frames), out_gpu=ref)
-    live = len(ref[0].getMeshInterleavedMarchingCubes().vertices)
-    assert live > 2000
-    pool = 64 * (4 * live // 64 + 1)
-    print(f"mesh compaction: {live} live vertices, pools of {pool}", flush=True)
     got = []
-    run_mesh(sc, R, t, frames, cfg, K, check_every=len(frames), pool=pool, out_gpu=got)
+    run_mesh(sc, R, t, frames, cfg, K, check_every=len(frames), pool=(1 << 18, 1), out_gpu=got)
     cap, compactions = got[0].meshPoolInfo()
-    assert cap == pool and compactions >= 1, f"pools of {cap} compacted {compactions} times"
+    print(f"mesh compaction: pools of {cap}, {compactions} compactions", flush=True)
+    assert cap >= 1 << 18 and compactions >= 1  # (an update that overflowed doubles the pools)
+
+
+def test_mesh_pools_too_small_grow():
+    """Pools of 4 096 vertices, far below the live mesh: updates overflow,
+    their compactions lose geometry, so the pools double and the next update
+    meshes every cell again; once an update fits, the meshes equal the
+    oracle's."""
+    sc, R, t, cfg = synth.config_c1(n_frames=200)
+    cfg = dict(cfg, w=320, h=240)
+    K = (262.5, 262.5, 159.5, 119.5)
+    gpu, ora = make_pair(cfg["scale"], cfg["threshold"], max_leaves=1 << 17)
+    gpu.setMeshPool(4096, 50)
+    for i in range(0, 12, 2):
+        d, bgr = sc.render(R[i].reshape(3, 3), t[i], K, cfg["w"], cfg["h"], noise=1.0, dropout=0.005, seed=1000 + i)
+        assert gpu.addMap(d, R[i], t[i], K, bgr, 1 / 5000.0, cfg["maxcam"]) == \
+            ora.add_map(d, R[i], t[i], K, bgr, 1 / 5000.0, cfg["maxcam"])
+        gpu.updateMeshes()
+    caps = [gpu.meshPoolInfo()[0]]
+    for _ in range(10):  # no new frame: an update after an overflow meshes every cell
+        gpu.updateMeshes()
+        caps.append(gpu.meshPoolInfo()[0])
+        if len(caps) >= 3 and caps[-1] == caps[-2] == caps[-3]:
+            break
+    print(f"mesh pools: {caps}", flush=True)
+    assert caps[-1] > 4096 and caps[-1] == caps[-2]
+    compare_mesh(gpu, ora, cfg["scale"], "after growth")
