@@ -369,8 +369,8 @@ def main():
         from fastfusion_amd import learned
         fz.setLearnedWeighting(learned.load_weights())
     # warm-up frames carry events around every phase (the phase breakdown);
-    # the timed frames only around k_sdf (the roofline), since every event
-    # record costs launch time
+    # the timed frames only around k_sdf on one frame in 8 (the roofline),
+    # recorded by its dispatch, since every event costs launch time
     fz.set_timing(True)
     K = synth.K_DEFAULT
     fb, cb = w * h * 2, w * h * 3
@@ -386,7 +386,7 @@ def main():
     hiprt.synchronize()
     tm_warm = fz.timing()
     fz.reset_timing()
-    fz.set_timing(True, sdf_only=True)
+    fz.set_timing(True, sdf_only=True, every=8)
     nq_sum = 0
     t0 = time.perf_counter()
     for i in range(args.warmup, n_frames):

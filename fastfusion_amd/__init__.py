@@ -77,6 +77,7 @@ ABI = {
     "ff_stream": (_P, [_P]),
     "ff_set_graph_capture": (ctypes.c_int, [_P, ctypes.c_int]),
     "ff_set_timing": (ctypes.c_int, [_P, ctypes.c_int]),
+    "ff_set_timing_every": (ctypes.c_int, [_P, ctypes.c_uint32]),
     "ff_get_timing": (ctypes.c_int, [_P, _P, _P, _P, _P]),
     "ff_reset_timing": (ctypes.c_int, [_P]),
     "ff_tum_read": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, _P, _SZ]),
@@ -389,9 +390,11 @@ class FusionMipMapGPU:
         """Replay each frame's launches from captured HIP graphs, or launch directly (default)."""
         self._check(self._lib.ff_set_graph_capture(self._ctx, int(bool(on))))
 
-    def set_timing(self, on=True, sdf_only=False):
-        """HIP-event timing of every phase, or (sdf_only) of the SDF update alone."""
+    def set_timing(self, on=True, sdf_only=False, every=1):
+        """HIP-event timing of every phase, or (sdf_only) of the SDF update
+        alone; the SDF update on one frame in `every`."""
         self._check(self._lib.ff_set_timing(self._ctx, (2 if sdf_only else 1) if on else 0))
+        self._check(self._lib.ff_set_timing_every(self._ctx, int(every)))
 
     def reset_timing(self):
         self._check(self._lib.ff_reset_timing(self._ctx))

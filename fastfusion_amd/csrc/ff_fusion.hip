@@ -23,6 +23,8 @@
 #include "ff_sdf.hpp"
 #include "rcp_table.h"
 
+#include <hip/hip_ext.h>
+
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
@@ -1762,7 +1764,8 @@ void launch_alloc_batch(hipStream_t st, const AllocStream *d_as, const AllocStre
 
 // the SDF update waits for the host only when the node table overflowed or
 // the volume must grow first; carry: a direct launch, which also takes the
-// previous mesh update's counters up
+// previous mesh update's counters up and, with timing on, has the dispatch
+// itself record the SDF phase's events (they then time the kernel alone)
 static void launch_sdf_only(ff_ctx *c, int W, int H, bool carry, hipStream_t st)
 {
     SdfGate gate;
@@ -1772,8 +1775,16 @@ static void launch_sdf_only(ff_ctx *c, int W, int H, bool carry, hipStream_t st)
     gate.force = 0;
     const uint4 *cs = carry ? static_cast<const uint4 *>(c->carrySrc) : nullptr;
     uint4 *cd = carry ? static_cast<uint4 *>(c->carryDst) : nullptr;
-    k_sdf<<<FF_SDF_GRID, 512, 0, st>>>(c->d_queue, c->mapView(), c->d_fd, W, H, c->d_fc, c->d_mapc, gate, c->d_hfc,
-                                       cs, cd);
+    // (timed on one frame in timeEvery)
+    const bool timedNow = carry && c->timing && c->frames % c->timeEvery == 0;
+    if (carry) c->sdfTimed = timedNow;
+    if (timedNow)
+        hipExtLaunchKernelGGL(k_sdf, dim3(FF_SDF_GRID), dim3(512), 0, st, c->ev[2], c->ev[3], 0,
+                              (const uint32_t *)c->d_queue, c->mapView(), (const FrameDev *)c->d_fd, W, H, c->d_fc,
+                              (const MapCounters *)c->d_mapc, gate, c->d_hfc, cs, cd);
+    else
+        k_sdf<<<FF_SDF_GRID, 512, 0, st>>>(c->d_queue, c->mapView(), c->d_fd, W, H, c->d_fc, c->d_mapc, gate,
+                                           c->d_hfc, cs, cd);
     if (cs) c->carrySrc = c->carryDst = nullptr;
 }
 
@@ -1824,9 +1835,7 @@ static int run_frame(ff_ctx *c, int W, int H, bool overlap)
         if (c->learned) learned_launch(c, W, H, c->side);
         if (c->mcPending) HIPCHK(hipStreamWaitEvent(c->side, c->evMc, 0));
         c->mcPending = false;
-        if (c->timing) hipEventRecord(c->ev[2], c->side);
         launch_sdf_only(c, W, H, true, c->side);  // which leaves the counters in h_fc
-        if (c->timing) hipEventRecord(c->ev[3], c->side);
         HIPCHK(hipGetLastError());
         return FF_OK;
     }
@@ -1849,10 +1858,14 @@ static int run_frame(ff_ctx *c, int W, int H, bool overlap)
     if (c->timingAll) hipEventRecord(c->ev[1], c->stream);
     // the SDF phase's events bracket k_sdf alone when launched directly
     if (!g && c->learned) learned_launch(c, W, H, c->stream);
-    if (c->timing) hipEventRecord(c->ev[2], c->stream);
-    if (g) HIPCHK(hipGraphLaunch(c->frameExec[1], c->stream));
-    else launch_sdf_only(c, W, H, true, c->stream);  // which leaves the counters in h_fc
-    if (c->timing) hipEventRecord(c->ev[3], c->stream);
+    if (g) {
+        if (c->timing) hipEventRecord(c->ev[2], c->stream);
+        HIPCHK(hipGraphLaunch(c->frameExec[1], c->stream));
+        if (c->timing) hipEventRecord(c->ev[3], c->stream);
+        c->sdfTimed = c->timing;
+    } else {
+        launch_sdf_only(c, W, H, true, c->stream);  // which leaves the counters in h_fc
+    }
     HIPCHK(hipGetLastError());
     return FF_OK;
 }
@@ -1938,6 +1951,13 @@ void launch_empty(ff_ctx *c, int W, int H)
 
 // Host part after the frame's device work completed and c->h_fc holds its
 // counters: errors, used counts, grow with the deferred SDF update, bytes.
+// algorithmic bytes of one SDF update: 512 voxels x 14 B (8 B without
+// color) read and written per queued brick, and the BGR image
+static double sdf_bytes(const ff_ctx *c, uint32_t nQueued, int W, int H)
+{
+    return (double)nQueued * (c->useColor ? 14336.0 : 8192.0) + (c->useColor ? 3.0 * W * H : 0.0);
+}
+
 int frame_finish(ff_ctx *c, int W, int H, bool timed)
 {
     FrameCounters *hf = c->h_fc;
@@ -1947,11 +1967,14 @@ int frame_finish(ff_ctx *c, int W, int H, bool timed)
             c->phaseMs[0] += ms;
             c->phaseCalls[0]++;
         }
-        if (hf->sdfRan && hipEventElapsedTime(&ms, c->ev[2], c->ev[3]) == hipSuccess) {
+        if (c->sdfTimed && hf->sdfRan && hipEventElapsedTime(&ms, c->ev[2], c->ev[3]) == hipSuccess) {
             c->phaseMs[1] += ms;
             c->phaseCalls[1]++;
+            c->phaseBytes[1] += sdf_bytes(c, hf->nQueued, W, H);
+            c->phaseUnits[1] += hf->nQueued;
         }
     }
+    c->sdfTimed = false;
     if (hf->flags & FF_FLAG_ORDER_BIG) {
         const int r = order_big_frame(c, W, H);
         if (r) return r;
@@ -1998,16 +2021,16 @@ int frame_finish(ff_ctx *c, int W, int H, bool timed)
         if (c->timing && hipEventElapsedTime(&ms, c->ev[4], c->ev[5]) == hipSuccess) {
             c->phaseMs[1] += ms;
             c->phaseCalls[1]++;
+            c->phaseBytes[1] += sdf_bytes(c, c->nQueued, W, H);
+            c->phaseUnits[1] += c->nQueued;
         }
     }
-    // algorithmic bytes: images read once, 16 B per processed pixel, 512 voxels
-    // x 14 B (8 B without color) read and written per queued brick (batched
-    // ticks account their SDF launch on the lead context instead)
+    // algorithmic bytes of the allocation: images read once, 16 B per
+    // processed pixel (the SDF update's are added with its timed launches;
+    // batched ticks account their SDF launch on the lead context instead)
     if (timed) {
         c->phaseBytes[0] += (double)W * H * 2 + 16.0 * hf->nProcessed;
-        c->phaseBytes[1] += (double)c->nQueued * (c->useColor ? 14336.0 : 8192.0) + (c->useColor ? 3.0 * W * H : 0.0);
         c->phaseUnits[0] += hf->nProcessed;
-        c->phaseUnits[1] += c->nQueued;
     }
 
     if (c->nLeavesUsed > c->nLeavesTotal || c->nBranchesUsed > c->nBranchesTotal) c->incremental = false;
@@ -2094,6 +2117,13 @@ int ff_set_timing(ff_ctx *c, int on)
         for (int i = 0; i < 8; i++) HIPCHK(hipEventCreate(&c->ev[i]));
     c->timing = on != 0;
     c->timingAll = on == 1;
+    return FF_OK;
+}
+
+int ff_set_timing_every(ff_ctx *c, uint32_t n)
+{
+    if (!c || n == 0) return FF_E_ARG;
+    c->timeEvery = n;
     return FF_OK;
 }
 

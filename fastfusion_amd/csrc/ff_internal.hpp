@@ -171,6 +171,8 @@ struct ff_ctx {
     uint64_t frameKey = 0;     // what the captured graph depends on
     uint32_t bufGen = 0;       // bumped when buffers a graph refers to are replaced
     bool useGraphs = false;    // direct launches measure faster for one stream
+    uint32_t timeEvery = 1;    // the SDF update is timed on one frame in timeEvery
+    bool sdfTimed = false;     // the frame in flight records the SDF phase's events
     ff::HashView hv{};
     ff::MeshState *mesh = nullptr;
     ff::LearnedState *learned = nullptr;  // learned confidence (off: nullptr)

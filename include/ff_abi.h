@@ -150,6 +150,9 @@ int ff_copy_processed(ff_ctx *ctx, uint8_t *out, size_t n);
  * directly. */
 int ff_set_graph_capture(ff_ctx *ctx, int on);
 int ff_set_timing(ff_ctx *ctx, int on);
+/* Times the SDF update on one frame in n (default 1): its events are
+ * recorded by the kernel's own dispatch, which costs launch time. */
+int ff_set_timing_every(ff_ctx *ctx, uint32_t n);
 int ff_get_timing(ff_ctx *ctx, double *ms, uint64_t *calls, double *bytes, uint64_t *units);
 int ff_reset_timing(ff_ctx *ctx);
 
