@@ -33,6 +33,8 @@
 #include "ff_sdf.hpp"
 #include "rcp_table.h"
 
+#include <hip/hip_ext.h>
+
 #include <algorithm>
 #include <string>
 
@@ -68,6 +70,7 @@ struct LoopArgs {
 #define FF_LOOP_TPB 1024
 #define FF_LOOP_SMALL 8192  // queues up to this length are mapped by one workgroup
 #define FF_LOOP_GRID 8192   // blocks of k_loop_delete; each loops over queue entries
+#define FF_LOOP_TIME_EVERY 8  // with timing on, frames of a submission timed: one in 8
 #define FF_UNCLAIMED 0xffffffffu
 
 // a claim made by another thread of the same kernel: read at L2, not from a
@@ -505,9 +508,14 @@ static int loop_submit(ff_loop *lp, const ff_loop_frame *fr, int nf, size_t *app
             k_loop_flags<<<nb, FF_LOOP_TPB, 0, st>>>(f.queue, n, A, lp->d_st, lp->d_bsum);
             k_loop_assign<<<nb, FF_LOOP_TPB, 0, st>>>(f.queue, n, A, lp->d_st, lc, lp->d_bsum, nb);
         }
-        if (lp->timing) LCHK(hipEventRecord(lp->ev[2 * k], st));
-        k_loop_delete<<<std::min<uint32_t>(n, FF_LOOP_GRID), 512, 0, st>>>(f.queue, n, A, P, lp->d_st, lc);
-        if (lp->timing) LCHK(hipEventRecord(lp->ev[2 * k + 1], st));
+        // timed frames (one in FF_LOOP_TIME_EVERY): the dispatch itself
+        // records the events, so they time the kernel alone
+        const uint32_t grid = std::min<uint32_t>(n, FF_LOOP_GRID);
+        if (lp->timing && k % FF_LOOP_TIME_EVERY == 0)
+            hipExtLaunchKernelGGL(k_loop_delete, dim3(grid), dim3(512), 0, st, lp->ev[2 * k], lp->ev[2 * k + 1], 0,
+                                  (const uint32_t *)f.queue, (uint32_t)n, A, P, lp->d_st, (const LoopCounters *)lc);
+        else
+            k_loop_delete<<<grid, 512, 0, st>>>(f.queue, n, A, P, lp->d_st, lc);
         LCHK(hipGetLastError());
     }
     LCHK(hipMemcpyAsync(lp->h_lc, lp->d_lc, sizeof(LoopCounters) * nf, hipMemcpyDeviceToHost, st));
@@ -524,7 +532,7 @@ static int loop_submit(ff_loop *lp, const ff_loop_frame *fr, int nf, size_t *app
         }
         if (applied) applied[k] = h.failpos;
         if (h.failpos < n && full < 0) full = k;
-        if (lp->timing && n) {
+        if (lp->timing && n && k % FF_LOOP_TIME_EVERY == 0) {
             float ms = 0;
             LCHK(hipEventElapsedTime(&ms, lp->ev[2 * k], lp->ev[2 * k + 1]));
             lp->ms += ms;
