@@ -121,7 +121,7 @@ struct MeshBatchStream {
     uint32_t since;
     struct CellGeom *geom;
     MapRO m;
-    uint32_t *tiles;
+    struct TileRec *tiles;
     uint32_t tileCap;
     uint4 *sbuf;
     const unsigned long long *arena;
@@ -140,7 +140,7 @@ struct MeshState {
     uint32_t nCells[2] = {0, 0};
     int cur = 0;
     struct CellGeom *d_geom = nullptr;  // per dirty cell
-    uint32_t *d_tiles = nullptr;        // sample tile -> (dirty cell, tile in cell)
+    struct TileRec *d_tiles = nullptr;  // per sample tile: its cell's geometry and bricks
     uint4 *d_sbuf = nullptr;            // 64 samples per tile: d, w, R | G << 16, B
     uint32_t tileCap = 0;
     uint64_t *d_hkeys = nullptr;  // previous table: key -> index
@@ -747,6 +747,31 @@ struct CellGeom {
 #define FF_NO_TILE 0xffffffffu     // samples did not fit the buffer: sampled in place
 #define FF_EMPTY_CELL 0xfffffffeu  // the cell meshes to nothing
 
+// A 64-sample tile of a dirty cell, self-contained so that the sampling pass
+// reads one record per tile (no cell or geometry lookups behind it): the
+// cell's geometry, with g.pad = the tile's index within the cell and
+// g.tile0 = FF_NO_TILE for a tile of a cell sampled in place, and its bricks.
+struct TileRec {
+    CellGeom g;  // 28 B
+    uint32_t spare;
+    uint32_t ll[8];
+};
+static_assert(sizeof(TileRec) == 64, "a tile record is four 16-byte words");
+
+// the nt tile records of a dirty cell from tile t0 (those within the buffer)
+__device__ __forceinline__ void put_tiles(TileRec *tiles, uint32_t tileCap, uint32_t t0, uint32_t nt, const CellGeom &g,
+                                          const uint32_t ll[8])
+{
+    for (uint32_t k = 0; k < nt && t0 + k < tileCap; k++) {
+        TileRec r;
+        r.g = g;
+        r.g.pad = k;
+        r.spare = 0;
+        for (int j = 0; j < 8; j++) r.ll[j] = ll[j];
+        tiles[t0 + k] = r;
+    }
+}
+
 __device__ __forceinline__ bool axis_full(int type, int k)
 {
     return type == 0 || (type <= 3 ? k != type - 1 : (type <= 6 ? k == type - 4 : false));
@@ -862,7 +887,7 @@ __device__ __forceinline__ void mark_body(CellRec *cells, const CellRec *__restr
                                           const uint64_t *__restrict__ hkeys, const uint32_t *__restrict__ hval,
                                           uint32_t hmask, bool havePrev, const uint32_t *__restrict__ stamp,
                                           const uint32_t *__restrict__ parent, uint32_t since, MeshCounters *mc,
-                                          CellGeom *geom, uint32_t cap, const MapRO &m, uint32_t *tiles,
+                                          CellGeom *geom, uint32_t cap, const MapRO &m, TileRec *tiles,
                                           uint32_t tileCap)
 {
     const uint32_t nc = min(mc->nCells, cap);
@@ -872,6 +897,7 @@ __device__ __forceinline__ void mark_body(CellRec *cells, const CellRec *__restr
         const uint32_t i = ib + lane;
         bool isDirty = false;
         CellGeom g;
+        uint32_t ll[8];
         uint32_t nt = 0;
         if (i < nc) {
             CellRec &c = cells[i];
@@ -900,7 +926,6 @@ __device__ __forceinline__ void mark_body(CellRec *cells, const CellRec *__restr
             }
             isDirty = !clean;
             if (isDirty) {
-                uint32_t ll[8];
                 for (int k = 0; k < 8; k++) ll[k] = c.ll[k];
                 cell_geom(m, c.key, ll, g);
                 g.cell = i;
@@ -921,7 +946,7 @@ __device__ __forceinline__ void mark_body(CellRec *cells, const CellRec *__restr
             const uint32_t t0 = tb + ex;
             const bool fits = t0 + nt <= tileCap;
             g.tile0 = nt == 0 ? FF_EMPTY_CELL : (fits ? t0 : FF_NO_TILE);
-            for (uint32_t k = 0; k < nt && t0 + k < tileCap; k++) tiles[t0 + k] = fits ? ((d << 3) | k) : FF_NO_TILE;
+            put_tiles(tiles, tileCap, t0, nt, g, ll);
             geom[d] = g;
         }
     }
@@ -930,7 +955,7 @@ __device__ __forceinline__ void mark_body(CellRec *cells, const CellRec *__restr
 __global__ void k_mark(CellRec *cells, const CellRec *__restrict__ prev, const uint64_t *__restrict__ hkeys,
                        const uint32_t *__restrict__ hval, uint32_t hmask, bool havePrev,
                        const uint32_t *__restrict__ stamp, const uint32_t *__restrict__ parent, uint32_t since,
-                       MeshCounters *mc, CellGeom *geom, uint32_t cap, MapRO m, uint32_t *tiles, uint32_t tileCap)
+                       MeshCounters *mc, CellGeom *geom, uint32_t cap, MapRO m, TileRec *tiles, uint32_t tileCap)
 {
     mark_body(cells, prev, hkeys, hval, hmask, havePrev, stamp, parent, since, mc, geom, cap, m, tiles, tileCap);
 }
@@ -949,7 +974,7 @@ __global__ void k_mark_b(const MeshBatchStream *__restrict__ S)
 // range is cleared first, like the fresh record k_cells writes: k_mc fills
 // it unless the new mesh is empty.
 __global__ void k_mark_same(CellRec *cells, const uint32_t *__restrict__ stamp, const uint32_t *__restrict__ parent,
-                            uint32_t since, MeshCounters *mc, CellGeom *geom, uint32_t cap, MapRO m, uint32_t *tiles,
+                            uint32_t since, MeshCounters *mc, CellGeom *geom, uint32_t cap, MapRO m, TileRec *tiles,
                             uint32_t tileCap)
 {
     const uint32_t nc = min(mc->nCells, cap);
@@ -959,10 +984,10 @@ __global__ void k_mark_same(CellRec *cells, const uint32_t *__restrict__ stamp, 
         const uint32_t i = ib + lane;
         bool isDirty = false;
         CellGeom g;
+        uint32_t ll[8];
         uint32_t nt = 0;
         if (i < nc) {
             CellRec &c = cells[i];
-            uint32_t ll[8];
             for (int k = 0; k < 8; k++) ll[k] = c.ll[k];
             isDirty = chains_dirty(stamp, parent, ll, since);
             if (isDirty) {
@@ -985,7 +1010,7 @@ __global__ void k_mark_same(CellRec *cells, const uint32_t *__restrict__ stamp, 
             const uint32_t t0 = tb + ex;
             const bool fits = t0 + nt <= tileCap;
             g.tile0 = nt == 0 ? FF_EMPTY_CELL : (fits ? t0 : FF_NO_TILE);
-            for (uint32_t k = 0; k < nt && t0 + k < tileCap; k++) tiles[t0 + k] = fits ? ((d << 3) | k) : FF_NO_TILE;
+            put_tiles(tiles, tileCap, t0, nt, g, ll);
             geom[d] = g;
         }
     }
@@ -1062,22 +1087,24 @@ __global__ void k_hash_clear_cells_b(const MeshBatchStream *__restrict__ S)
 // k_sample: one thread per sample of the dirty cells' 64-sample tiles
 // ---------------------------------------------------------------------------
 __device__ __forceinline__ void sample_body(const CellGeom *__restrict__ geom, const CellRec *__restrict__ cells,
-                                            const uint32_t *__restrict__ tiles, const MeshCounters *mc,
+                                            const TileRec *__restrict__ tiles, const MeshCounters *mc,
                                             uint32_t tileCap, const MapRO &m, uint4 *__restrict__ sbuf)
 {
     const uint32_t nt = min(mc->nTiles, tileCap);
     for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < nt * 64u; t += gridDim.x * blockDim.x) {
-        const uint32_t e = tiles[t >> 6];
-        if (e >= FF_EMPTY_CELL) continue;
-        const CellGeom g = geom[e >> 3];
-        const int s = (int)((e & 7u) * 64u + (t & 63u));
+        // one record: the geometry and the bricks (read where they lie, by
+        // the index the sample picks) come in together
+        const TileRec *r = tiles + (t >> 6);
+        const CellGeom g = r->g;
+        if (g.tile0 == FF_NO_TILE) continue;
+        const int s = (int)(g.pad * 64u + (t & 63u));
         if (s >= geom_total(g)) continue;
-        sbuf[t] = pack_sample(sample_cell(m, g, cells[g.cell].ll, s));
+        sbuf[t] = pack_sample(sample_cell(m, g, r->ll, s));
     }
 }
 
 __global__ __launch_bounds__(256) void k_sample(const CellGeom *__restrict__ geom, const CellRec *__restrict__ cells,
-                                                const uint32_t *__restrict__ tiles, const MeshCounters *mc,
+                                                const TileRec *__restrict__ tiles, const MeshCounters *mc,
                                                 uint32_t tileCap, MapRO m, uint4 *__restrict__ sbuf)
 {
     sample_body(geom, cells, tiles, mc, tileCap, m, sbuf);
@@ -1588,7 +1615,7 @@ int mesh_init(ff_ctx *c)
     for (int k = 0; k < 2; k++) MCHK(hipMalloc(&s->d_cells[k], sizeof(CellRec) * s->cellCap));
     MCHK(hipMalloc(&s->d_geom, sizeof(CellGeom) * s->cellCap));
     s->tileCap = std::max<uint32_t>(1u << 16, std::min<uint32_t>(c->nLeavesTotal * 8u, 1u << 19));
-    MCHK(hipMalloc(&s->d_tiles, sizeof(uint32_t) * s->tileCap));
+    MCHK(hipMalloc(&s->d_tiles, sizeof(TileRec) * s->tileCap));
     MCHK(hipMalloc(&s->d_sbuf, sizeof(uint4) * 64 * (size_t)s->tileCap));
     uint32_t hcap = 1;
     while (hcap < 2 * s->cellCap) hcap <<= 1;
@@ -1940,9 +1967,9 @@ int mesh_finish(ff_ctx *c, bool timed)
         uint32_t cap = s->tileCap;
         while (cap < s->h_mc->nTiles && cap < FF_TILE_CAP_MAX) cap <<= 1;
         if (cap > s->tileCap) {
-            uint32_t *nt = nullptr;
+            TileRec *nt = nullptr;
             uint4 *nb = nullptr;
-            if (hipMalloc(&nt, sizeof(uint32_t) * cap) == hipSuccess &&
+            if (hipMalloc(&nt, sizeof(TileRec) * cap) == hipSuccess &&
                 hipMalloc(&nb, sizeof(uint4) * 64 * (size_t)cap) == hipSuccess) {
                 hipFree(s->d_tiles);
                 hipFree(s->d_sbuf);
@@ -2231,9 +2258,9 @@ int ff_set_mesh_sample_tiles(ff_ctx *c, uint32_t tiles)
     const int rc = mesh_complete(c);
     if (rc != FF_OK) return rc;
     MCHK(hipStreamSynchronize(c->stream));
-    uint32_t *nt = nullptr;
+    TileRec *nt = nullptr;
     uint4 *nb = nullptr;
-    MCHK(hipMalloc(&nt, sizeof(uint32_t) * tiles));
+    MCHK(hipMalloc(&nt, sizeof(TileRec) * tiles));
     if (!mok(c, hipMalloc(&nb, sizeof(uint4) * 64 * (size_t)tiles), "hipMalloc(sample buffer)")) {
         hipFree(nt);
         return FF_E_DEVICE;
