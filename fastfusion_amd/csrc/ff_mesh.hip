@@ -758,6 +758,28 @@ struct TileRec {
 };
 static_assert(sizeof(TileRec) == 64, "a tile record is four 16-byte words");
 
+// The geometry slot of a dirty cell (a wave-wide call; mc->nDirty counts
+// them all).  split: cells of more than two tiles (interior cells) take
+// slots from the front of the array and the others from the back
+// (mc->pad[2], mc->pad[1] count them), so that the marching cubes, which
+// takes dirty cells in slot order, starts the long cells first; otherwise
+// slots in append order.
+__device__ __forceinline__ uint32_t dirty_slot(MeshCounters *mc, bool isDirty, uint32_t nt, uint32_t cap, bool split)
+{
+    const uint32_t d = wave_append(&mc->nDirty, isDirty);
+    if (!split) return d;
+    const bool big = isDirty && nt > 2u;
+    const uint32_t b = wave_append(&mc->pad[2], big);
+    const uint32_t sm = wave_append(&mc->pad[1], isDirty && !big);
+    return big ? b : cap - 1u - sm;
+}
+
+// the dirty cell of marching-cubes index di (slots as dirty_slot gives them)
+__device__ __forceinline__ uint32_t dirty_geom(uint32_t di, uint32_t nBig, uint32_t cap, bool split)
+{
+    return !split || di < nBig ? di : cap - 1u - (di - nBig);
+}
+
 // the nt tile records of a dirty cell from tile t0 (those within the buffer)
 __device__ __forceinline__ void put_tiles(TileRec *tiles, uint32_t tileCap, uint32_t t0, uint32_t nt, const CellGeom &g,
                                           const uint32_t ll[8])
@@ -888,7 +910,7 @@ __device__ __forceinline__ void mark_body(CellRec *cells, const CellRec *__restr
                                           uint32_t hmask, bool havePrev, const uint32_t *__restrict__ stamp,
                                           const uint32_t *__restrict__ parent, uint32_t since, MeshCounters *mc,
                                           CellGeom *geom, uint32_t cap, const MapRO &m, TileRec *tiles,
-                                          uint32_t tileCap)
+                                          uint32_t tileCap, bool split)
 {
     const uint32_t nc = min(mc->nCells, cap);
     const uint32_t lane = threadIdx.x & 63u;
@@ -933,7 +955,7 @@ __device__ __forceinline__ void mark_body(CellRec *cells, const CellRec *__restr
                 nt = ((uint32_t)geom_total(g) + 63u) >> 6;
             }
         }
-        const uint32_t d = wave_append(&mc->nDirty, isDirty);
+        const uint32_t d = dirty_slot(mc, isDirty, nt, cap, split);
         // sample tiles: one atomic per wave for the wave's total
         uint32_t ntot;
         const uint32_t ex = wave_excl_scan_u32(nt, &ntot);
@@ -957,14 +979,14 @@ __global__ void k_mark(CellRec *cells, const CellRec *__restrict__ prev, const u
                        const uint32_t *__restrict__ stamp, const uint32_t *__restrict__ parent, uint32_t since,
                        MeshCounters *mc, CellGeom *geom, uint32_t cap, MapRO m, TileRec *tiles, uint32_t tileCap)
 {
-    mark_body(cells, prev, hkeys, hval, hmask, havePrev, stamp, parent, since, mc, geom, cap, m, tiles, tileCap);
+    mark_body(cells, prev, hkeys, hval, hmask, havePrev, stamp, parent, since, mc, geom, cap, m, tiles, tileCap, true);
 }
 
 __global__ void k_mark_b(const MeshBatchStream *__restrict__ S)
 {
     const MeshBatchStream &a = S[blockIdx.z];
     mark_body(a.cells, a.prev, a.hkeys, a.hval, a.hmask, a.havePrev != 0, a.stamp, a.parent, a.since, a.mc, a.geom,
-              a.cellCap, a.m, a.tiles, a.tileCap);
+              a.cellCap, a.m, a.tiles, a.tileCap, false);
 }
 
 // The table of the previous update re-marked in place, when the tree has not
@@ -977,6 +999,7 @@ __global__ void k_mark_same(CellRec *cells, const uint32_t *__restrict__ stamp, 
                             uint32_t since, MeshCounters *mc, CellGeom *geom, uint32_t cap, MapRO m, TileRec *tiles,
                             uint32_t tileCap)
 {
+    const bool split = true;  // long cells first (dirty_slot)
     const uint32_t nc = min(mc->nCells, cap);
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t stride = gridDim.x * blockDim.x;
@@ -998,7 +1021,7 @@ __global__ void k_mark_same(CellRec *cells, const uint32_t *__restrict__ stamp, 
                 nt = ((uint32_t)geom_total(g) + 63u) >> 6;
             }
         }
-        const uint32_t d = wave_append(&mc->nDirty, isDirty);
+        const uint32_t d = dirty_slot(mc, isDirty, nt, cap, split);
         uint32_t ntot;
         const uint32_t ex = wave_excl_scan_u32(nt, &ntot);
         uint32_t tb = 0;
@@ -1438,8 +1461,10 @@ __device__ void mc_cell(WaveLds &L, const uint64_t *tri, int lane, CellRec *cell
     wave_sync();
 }
 
+// (dirty cells in slot order as k_mark / k_mark_same split them: the long
+// ones first)
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_mc(CellRec *cells, const CellGeom *__restrict__ geom, MeshCounters *mc,
-                                            const uint4 *__restrict__ sbuf, MapRO m, Pools P)
+                                            const uint4 *__restrict__ sbuf, MapRO m, Pools P, uint32_t geomCap)
 {
     __shared__ WaveLds lds[4];
     __shared__ uint64_t tri[256];
@@ -1447,14 +1472,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
     __syncthreads();
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     WaveLds &L = lds[wave];
-    const uint32_t nd = mc->nDirty;
+    const uint32_t nd = mc->nDirty, nBig = mc->pad[2];
     for (uint32_t base = blockIdx.x * 4; base < nd; base += gridDim.x * 4) {
         // waves of a block run independently: each owns lds[wave]
         const uint32_t di = base + wave;
         const bool act = di < nd;
         CellGeom g;
         if (act) {
-            g = geom[di];
+            g = geom[dirty_geom(di, nBig, geomCap, true)];
         } else {
             g.cell = 0; g.tile0 = FF_EMPTY_CELL; g.type = 0;
             g.n[0] = g.n[1] = g.n[2] = 0;
@@ -1911,7 +1936,7 @@ McStream mesh_stream(ff_ctx *c)
 void mesh_launch_mc(ff_ctx *c)
 {
     MeshState *s = c->mesh;
-    k_mc<<<2048, 256, 0, c->stream>>>(s->d_cells[s->nxt], s->d_geom, s->d_mc, s->d_sbuf, s->m, pools(s));
+    k_mc<<<2048, 256, 0, c->stream>>>(s->d_cells[s->nxt], s->d_geom, s->d_mc, s->d_sbuf, s->m, pools(s), s->cellCap);
     // the bricks are read: the next frame's SDF update may start
     if (c->evMc) {
         if (hipEventRecord(c->evMc, c->stream) == hipSuccess) {
