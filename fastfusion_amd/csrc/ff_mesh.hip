@@ -152,7 +152,15 @@ struct MeshState {
     uint32_t compactions = 0;     // live-geometry compactions so far
     uint32_t compactPercent = 50; // an arena used this far since the last compaction asks for one
     unsigned long long *d_arena = nullptr;
-    MeshCounters *d_mc = nullptr, *h_mc = nullptr;
+    MeshCounters *d_mc = nullptr, *h_mc = nullptr;  // d_mc: the counters of the update in flight
+    // two sets of device counters, alternating between single updates: the
+    // first kernel of an update clears the set the next update counts in
+    // (its last readers -- the previous update's kernels, compaction and
+    // the SDF update that took its counters up -- are done by then), so
+    // that no clearing launch is needed
+    MeshCounters *d_mcBuf[2] = {nullptr, nullptr};
+    int mcIdx = 0;
+    bool mcNextZero = false, mcZeroNow = false;
     MeshCounters *d_hmc = nullptr;  // h_mc (mapped host memory) as the device addresses it
     uint32_t lastMeshStamp = 0;
     int nxt = 1;   // table being built by the update in flight
@@ -358,8 +366,9 @@ __device__ __forceinline__ void terminal_body(const uint32_t *__restrict__ tree,
 // (blocks past the terminal candidates fill the jump table, 256 cells each)
 __global__ void k_terminal(const uint32_t *__restrict__ tree, const ushort4 *__restrict__ binfo, uint32_t nBlocks,
                            uint32_t nLeavesTotal, uint32_t n, uint4 *__restrict__ term, uint32_t termCap,
-                           MeshCounters *mc, uint4 *__restrict__ jump)
+                           MeshCounters *mc, uint4 *__restrict__ jump, MeshCounters *clearNext)
 {
+    if (blockIdx.x == 0 && threadIdx.x == 0) *clearNext = MeshCounters{};  // the next update's counters
     const uint32_t tb = (nBlocks * 8u + blockDim.x - 1u) / blockDim.x;
     if (blockIdx.x >= tb) {
         const uint32_t cell = (blockIdx.x - tb) * blockDim.x + threadIdx.x;
@@ -996,11 +1005,17 @@ __global__ void k_mark_b(const MeshBatchStream *__restrict__ S)
 // range is cleared first, like the fresh record k_cells writes: k_mc fills
 // it unless the new mesh is empty.
 __global__ void k_mark_same(CellRec *cells, const uint32_t *__restrict__ stamp, const uint32_t *__restrict__ parent,
-                            uint32_t since, MeshCounters *mc, CellGeom *geom, uint32_t cap, MapRO m, TileRec *tiles,
-                            uint32_t tileCap)
+                            uint32_t since, MeshCounters *mc, uint32_t nCells, MeshCounters *clearNext, CellGeom *geom,
+                            uint32_t cap, MapRO m, TileRec *tiles, uint32_t tileCap)
 {
     const bool split = true;  // long cells first (dirty_slot)
-    const uint32_t nc = min(mc->nCells, cap);
+    // the counters start at zero; block 0 records the table's size and
+    // clears the next update's counters
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        mc->nCells = nCells;
+        *clearNext = MeshCounters{};
+    }
+    const uint32_t nc = min(nCells, cap);
     const uint32_t lane = threadIdx.x & 63u;
     const uint32_t stride = gridDim.x * blockDim.x;
     for (uint32_t ib = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); ib < nc; ib += stride) {
@@ -1655,11 +1670,15 @@ int mesh_init(ff_ctx *c)
     MCHK(hipMalloc(&s->d_fpool, sizeof(uint32_t) * 3 * (size_t)s->fCap));
     MCHK(hipMalloc(&s->d_arena, sizeof(unsigned long long) * FF_ARENAS * FF_ARENA_STRIDE));
     MCHK(hipMemset(s->d_arena, 0, sizeof(unsigned long long) * FF_ARENAS * FF_ARENA_STRIDE));
-    MCHK(hipMalloc(&s->d_mc, sizeof(MeshCounters)));
+    for (int k = 0; k < 2; k++) {
+        MCHK(hipMalloc(&s->d_mcBuf[k], sizeof(MeshCounters)));
+        MCHK(hipMemset(s->d_mcBuf[k], 0, sizeof(MeshCounters)));
+    }
+    s->d_mc = s->d_mcBuf[0];
     MCHK(hipHostMalloc(&s->h_mc, sizeof(MeshCounters), hipHostMallocMapped | hipHostMallocCoherent));
     MCHK(hipHostGetDevicePointer(reinterpret_cast<void **>(&s->d_hmc), s->h_mc, 0));
     memset(s->h_mc, 0, sizeof(MeshCounters));
-    MCHK(hipMemset(s->d_mc, 0, sizeof(MeshCounters)));
+
     return FF_OK;
 }
 
@@ -1674,7 +1693,7 @@ void mesh_free(ff_ctx *c)
     hipFree(s->d_arena);
     hipFree(s->d_mdesc); hipHostFree(s->h_mdesc); hipFree(s->d_msPrefix);
     hipFree(s->d_mcs); hipHostFree(s->h_mcs);
-    hipFree(s->d_mc); hipHostFree(s->h_mc);
+    hipFree(s->d_mcBuf[0]); hipFree(s->d_mcBuf[1]); hipHostFree(s->h_mc);
     delete s;
     c->mesh = nullptr;
 }
@@ -1825,6 +1844,17 @@ static Pools pools(const MeshState *s)
 // update_meshes in phases (so that a batch can share the marching-cubes
 // launch): pre -- cells, dirty marks, samples; the marching cubes (mc);
 // post -- lookup hash, counters' copy back; finish -- after the device work.
+// the counter set the update after this one counts in
+static inline MeshCounters *mc_other(const MeshState *s) { return s->d_mcBuf[1 - s->mcIdx]; }
+
+// after a single update's first kernel (which cleared the other set): the
+// next update counts in that set
+static inline void mc_flip(MeshState *s)
+{
+    s->mcIdx ^= 1;
+    s->mcNextZero = true;
+}
+
 // host part of pre: the table to build and the map view (false: nothing
 // to mesh)
 static bool mesh_pre_host(ff_ctx *c, bool allowReuse)
@@ -1833,6 +1863,12 @@ static bool mesh_pre_host(ff_ctx *c, bool allowReuse)
     if (!s->haveVolume || !c->n) return false;
     s->reuse = allowReuse && s->nCells[s->cur] > 0 && s->builtNB == c->nBranchesUsed &&
                s->builtNL == c->nLeavesUsed && s->builtGrows == c->grows;
+    // this update's counter set (zero when the previous single update's
+    // first kernel cleared it); a batched update counts in it too and
+    // clears no other set
+    s->d_mc = s->d_mcBuf[s->mcIdx];
+    s->mcZeroNow = s->mcNextZero;
+    s->mcNextZero = false;
     s->launchStamp = c->stampCounter;
     s->launchNB = c->nBranchesUsed;
     s->launchNL = c->nLeavesUsed;
@@ -1856,19 +1892,21 @@ bool mesh_launch_pre(ff_ctx *c, bool timed)
     if (s->reuse) {
         const MapRO m = s->m;
         const uint32_t nc = std::min(s->nCells[s->cur], s->cellCap);
-        k_mesh_begin_same<<<1, 64, 0, c->stream>>>(s->d_mc, nc);
+        if (!s->mcZeroNow) k_mesh_begin<<<1, 64, 0, c->stream>>>(s->d_mc);
         k_mark_same<<<std::max(1u, std::min(4096u, (nc + 255) / 256)), 256, 0, c->stream>>>(
-            s->d_cells[nxt], c->d_stamp, c->d_parent, s->lastMeshStamp, s->d_mc, s->d_geom, s->cellCap, m, s->d_tiles,
-            s->tileCap);
+            s->d_cells[nxt], c->d_stamp, c->d_parent, s->lastMeshStamp, s->d_mc, nc, mc_other(s), s->d_geom,
+            s->cellCap, m, s->d_tiles, s->tileCap);
         k_sample<<<4096, 256, 0, c->stream>>>(s->d_geom, s->d_cells[nxt], s->d_tiles, s->d_mc, s->tileCap, m,
                                               s->d_sbuf);
+        mc_flip(s);
         return true;
     }
-    k_mesh_begin<<<1, 64, 0, c->stream>>>(s->d_mc);
+    if (!s->mcZeroNow) k_mesh_begin<<<1, 64, 0, c->stream>>>(s->d_mc);
     const uint32_t nBlocks = c->nBranchesUsed / 16;
     // (plus FF_JUMP_CELLS / 256 blocks that fill the jump table)
     k_terminal<<<(nBlocks * 8 + 255) / 256 + FF_JUMP_CELLS / 256, 256, 0, c->stream>>>(
-        c->d_tree, s->d_binfo, nBlocks, c->nLeavesTotal, c->n, s->d_term, s->termCap, s->d_mc, s->d_jump);
+        c->d_tree, s->d_binfo, nBlocks, c->nLeavesTotal, c->n, s->d_term, s->termCap, s->d_mc, s->d_jump,
+        mc_other(s));
     k_cells<<<1024, 512, 0, c->stream>>>(c->d_tree, c->nLeavesTotal, c->n, s->d_term, s->termCap, s->d_mc,
                                          s->d_cells[nxt], s->cellCap, s->d_jump);
     const MapRO m = s->m;
@@ -1876,7 +1914,8 @@ bool mesh_launch_pre(ff_ctx *c, bool timed)
                                         s->nCells[s->cur] > 0, c->d_stamp, c->d_parent, s->lastMeshStamp, s->d_mc,
                                         s->d_geom, s->cellCap, m, s->d_tiles, s->tileCap);
     k_sample<<<4096, 256, 0, c->stream>>>(s->d_geom, s->d_cells[nxt], s->d_tiles, s->d_mc, s->tileCap, m, s->d_sbuf);
-    return true;
+    mc_flip(s);
+        return true;
 }
 
 // the descriptor of a context in a batched update (after mesh_pre_host)
