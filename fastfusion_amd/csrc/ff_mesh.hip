@@ -143,8 +143,14 @@ struct MeshState {
     struct TileRec *d_tiles = nullptr;  // per sample tile: its cell's geometry and bricks
     uint4 *d_sbuf = nullptr;            // 64 samples per tile: d, w, R | G << 16, B
     uint32_t tileCap = 0;
-    uint64_t *d_hkeys = nullptr;  // previous table: key -> index
-    uint32_t *d_hval = nullptr;
+    // cell key -> index of the current table, in one of two hashes: an
+    // update that builds a new table empties the current hash of the old
+    // keys and fills the other with the new ones in one pass (k_hash_turnover)
+    uint64_t *d_hkeysBuf[2] = {nullptr, nullptr};
+    uint32_t *d_hvalBuf[2] = {nullptr, nullptr};
+    int hIdx = 0;
+    uint64_t *d_hkeys = nullptr;  // = d_hkeysBuf[hIdx]
+    uint32_t *d_hval = nullptr;   // = d_hvalBuf[hIdx]
     uint32_t hmask = 0;
     float4 *d_vpool = nullptr;    // x, y, z, rgb packed
     uint32_t *d_fpool = nullptr;  // 3 local indices per face
@@ -188,6 +194,16 @@ struct MeshState {
     // (frames fused before its host part runs must not be attributed to it)
     uint32_t launchStamp = 0, launchNB = 0, launchNL = 0, launchGrows = 0;
 };
+
+// both cell hashes emptied (every cell is meshed again by the next update)
+static hipError_t hash_reset(MeshState *s, hipStream_t st)
+{
+    hipError_t e = hipSuccess;
+    for (int k = 0; k < 2 && e == hipSuccess; k++)
+        e = hipMemsetAsync(s->d_hkeysBuf[k], 0xff, sizeof(uint64_t) * (s->hmask + 1), st);
+    return e;
+}
+
 
 // ---------------------------------------------------------------------------
 // tree queries
@@ -1086,11 +1102,6 @@ __device__ __forceinline__ void hash_build_body(CellRec *cells, MeshCounters *mc
     }
 }
 
-__global__ void k_hash_build(CellRec *cells, MeshCounters *mc, uint32_t cap, uint64_t *hkeys, uint32_t *hval,
-                             uint32_t hmask, const unsigned long long *arena)
-{
-    hash_build_body(cells, mc, cap, hkeys, hval, hmask, arena);
-}
 
 __global__ void k_hash_build_b(const MeshBatchStream *__restrict__ S)
 {
@@ -1108,9 +1119,16 @@ __device__ __forceinline__ void hash_clear_cells_body(const CellRec *__restrict_
     }
 }
 
-__global__ void k_hash_clear_cells(const CellRec *__restrict__ cells, uint32_t nc, uint64_t *hkeys)
+
+// the turnover of an update that built a new table: the previous table's
+// slots emptied in its hash, the new table's keys into the other (empty)
+// hash; the two parts touch different memory, so one pass does both
+__global__ void k_hash_turnover(const CellRec *__restrict__ prev, uint32_t nPrev, uint64_t *prevKeys, CellRec *cells,
+                                MeshCounters *mc, uint32_t cap, uint64_t *hkeys, uint32_t *hval, uint32_t hmask,
+                                const unsigned long long *arena)
 {
-    hash_clear_cells_body(cells, nc, hkeys);
+    hash_clear_cells_body(prev, nPrev, prevKeys);
+    hash_build_body(cells, mc, cap, hkeys, hval, hmask, arena);
 }
 
 // (the previous table of each stream)
@@ -1660,9 +1678,14 @@ int mesh_init(ff_ctx *c)
     uint32_t hcap = 1;
     while (hcap < 2 * s->cellCap) hcap <<= 1;
     s->hmask = hcap - 1;
-    MCHK(hipMalloc(&s->d_hkeys, sizeof(uint64_t) * hcap));
-    MCHK(hipMemset(s->d_hkeys, 0xff, sizeof(uint64_t) * hcap));
-    MCHK(hipMalloc(&s->d_hval, sizeof(uint32_t) * hcap));
+    for (int k = 0; k < 2; k++) {
+        MCHK(hipMalloc(&s->d_hkeysBuf[k], sizeof(uint64_t) * hcap));
+        MCHK(hipMemset(s->d_hkeysBuf[k], 0xff, sizeof(uint64_t) * hcap));
+        MCHK(hipMalloc(&s->d_hvalBuf[k], sizeof(uint32_t) * hcap));
+    }
+    s->hIdx = 0;
+    s->d_hkeys = s->d_hkeysBuf[0];
+    s->d_hval = s->d_hvalBuf[0];
     s->vCap = std::max<uint32_t>(1u << 22, std::min<uint32_t>(c->nLeavesTotal * 64u, 1u << 28));
     s->vCap -= s->vCap % FF_ARENAS;
     s->fCap = s->vCap;
@@ -1689,7 +1712,8 @@ void mesh_free(ff_ctx *c)
     hipFree(s->d_binfo); hipFree(s->d_term); hipFree(s->d_jump);
     hipFree(s->d_cells[0]); hipFree(s->d_cells[1]);
     hipFree(s->d_geom); hipFree(s->d_tiles); hipFree(s->d_sbuf);
-    hipFree(s->d_hkeys); hipFree(s->d_hval); hipFree(s->d_vpool); hipFree(s->d_fpool);
+    for (int k = 0; k < 2; k++) { hipFree(s->d_hkeysBuf[k]); hipFree(s->d_hvalBuf[k]); }
+    hipFree(s->d_vpool); hipFree(s->d_fpool);
     hipFree(s->d_arena);
     hipFree(s->d_mdesc); hipHostFree(s->h_mdesc); hipFree(s->d_msPrefix);
     hipFree(s->d_mcs); hipHostFree(s->h_mcs);
@@ -1716,7 +1740,7 @@ int mesh_on_load(ff_ctx *c, const std::vector<ushort4> &binfo)
     mesh_complete(c);
     const size_t nb = std::min<size_t>(binfo.size(), s->binfoCap);
     if (nb) MCHK(hipMemcpyAsync(s->d_binfo, binfo.data(), sizeof(ushort4) * nb, hipMemcpyHostToDevice, c->stream));
-    MCHK(hipMemsetAsync(s->d_hkeys, 0xff, sizeof(uint64_t) * (s->hmask + 1), c->stream));
+    MCHK(hash_reset(s, c->stream));
     MCHK(hipStreamSynchronize(c->stream));
     s->nCells[0] = s->nCells[1] = 0;
     s->cur = 0;
@@ -1742,7 +1766,7 @@ void mesh_on_grow_blocks(ff_ctx *c, uint32_t oldBlocks, uint32_t addBlocks, cons
     hipFree(tmp);
     // every cell moved: drop the previous table so all cells re-mesh
     s->nCells[s->cur] = 0;
-    hipMemsetAsync(s->d_hkeys, 0xff, sizeof(uint64_t) * (s->hmask + 1), c->stream);
+    (void)hash_reset(s, c->stream);
 }
 
 void mesh_on_grow(ff_ctx *) {}
@@ -1996,10 +2020,13 @@ void mesh_launch_post(ff_ctx *c, bool timed)
         if (c->timingAll) k_arena_sum<<<1, 64, 0, c->stream>>>(s->d_arena, s->d_mc);
     } else {
         // the previous table's keys leave the lookup hash, the new ones enter it
-        if (s->nCells[s->cur])
-            k_hash_clear_cells<<<1024, 256, 0, c->stream>>>(s->d_cells[s->cur], s->nCells[s->cur], s->d_hkeys);
-        k_hash_build<<<1024, 256, 0, c->stream>>>(s->d_cells[nxt], s->d_mc, s->cellCap, s->d_hkeys, s->d_hval,
-                                                  s->hmask, c->timingAll ? s->d_arena : nullptr);
+        const int b = 1 - s->hIdx;
+        k_hash_turnover<<<1024, 256, 0, c->stream>>>(s->d_cells[s->cur], s->nCells[s->cur], s->d_hkeys,
+                                                     s->d_cells[nxt], s->d_mc, s->cellCap, s->d_hkeysBuf[b],
+                                                     s->d_hvalBuf[b], s->hmask, c->timingAll ? s->d_arena : nullptr);
+        s->hIdx = b;  // the next update looks the new table up there
+        s->d_hkeys = s->d_hkeysBuf[b];
+        s->d_hval = s->d_hvalBuf[b];
     }
     if (timed && c->timingAll) hipEventRecord(c->ev[7], c->stream);
     if (c->timingAll) {
@@ -2128,7 +2155,7 @@ int mesh_finish(ff_ctx *c, bool timed)
         if ((flags & FF_MESH_OVERFLOW) || lost) {
             // cells that did not fit were stored empty; mesh everything again next time
             s->nCells[nxt] = 0;
-            MCHK(hipMemsetAsync(s->d_hkeys, 0xff, sizeof(uint64_t) * (s->hmask + 1), c->stream));
+            MCHK(hash_reset(s, c->stream));
         }
     }
     return (int)nDirty;
@@ -2365,7 +2392,8 @@ int ff_set_mesh_pool(ff_ctx *c, uint32_t vertices, uint32_t compact_percent)
     MCHK(hipMemset(s->d_arena, 0, sizeof(unsigned long long) * FF_ARENAS * FF_ARENA_STRIDE));
     // the geometry of the old pools is gone: the next update meshes every cell
     s->nCells[s->cur] = 0;
-    MCHK(hipMemset(s->d_hkeys, 0xff, sizeof(uint64_t) * (s->hmask + 1)));
+    MCHK(hash_reset(s, c->stream));
+    MCHK(hipStreamSynchronize(c->stream));
     s->lastVAlloc = s->lastFAlloc = 0;
     return FF_OK;
 }
