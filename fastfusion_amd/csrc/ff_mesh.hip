@@ -793,10 +793,20 @@ __device__ __forceinline__ uint32_t dirty_slot(MeshCounters *mc, bool isDirty, u
 {
     const uint32_t d = wave_append(&mc->nDirty, isDirty);
     if (!split) return d;
+    // both runs with one 64-bit atomic: pad[1] (low half) the short cells,
+    // pad[2] (high half) the long ones
     const bool big = isDirty && nt > 2u;
-    const uint32_t b = wave_append(&mc->pad[2], big);
-    const uint32_t sm = wave_append(&mc->pad[1], isDirty && !big);
-    return big ? b : cap - 1u - sm;
+    const uint64_t mb = __ballot(big), ms = __ballot(isDirty && !big);
+    if (!(mb | ms)) return 0xffffffffu;
+    const int lane = threadIdx.x & 63;
+    const int leader = __ffsll((long long)(mb | ms)) - 1;
+    unsigned long long base = 0;
+    if (lane == leader)
+        base = atomicAdd(reinterpret_cast<unsigned long long *>(&mc->pad[1]),
+                         ((unsigned long long)__popcll(mb) << 32) | (unsigned long long)__popcll(ms));
+    const uint32_t lo = __shfl((uint32_t)base, leader), hi = __shfl((uint32_t)(base >> 32), leader);
+    const uint64_t below = (1ull << lane) - 1ull;
+    return big ? hi + (uint32_t)__popcll(mb & below) : cap - 1u - (lo + (uint32_t)__popcll(ms & below));
 }
 
 // the dirty cell of marching-cubes index di (slots as dirty_slot gives them)
