@@ -748,6 +748,9 @@ __device__ __forceinline__ void link_body(const HashView &hv, const MapView &m, 
                                           FrameCounters *fcw, bool slow)
 {
     if (!slow && (fc->flags & FF_FLAG_ORDER_BIG)) return;
+    // a frame whose node table or node list overflowed has an incomplete
+    // node set: nothing of it enters the tree (nothing is committed either)
+    if ((fc->flags & FF_FLAG_HASH_FULL) || fc->listCount > hv.listCap) return;
     const uint32_t n = fd->n, stamp = fd->stamp;
     const uint32_t cnt = min(fc->listCount, hv.listCap);
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < cnt; i += gridDim.x * blockDim.x) {
@@ -1193,7 +1196,6 @@ __global__ __launch_bounds__(512) void k_sdf(const uint32_t *__restrict__ queue,
     __syncthreads();  // rcp table ready
     sdf_voxel(leaf0, threadIdx.x, m, depth, bgr, W, H, P, tab);
     for (uint32_t b = blockIdx.x + gridDim.x; b < nQ; b += gridDim.x)
-    for (uint32_t b = blockIdx.x; b < nQ; b += gridDim.x)
         sdf_voxel(queue[b], threadIdx.x, m, depth, bgr, W, H, P, tab);
 }
 
@@ -1876,6 +1878,10 @@ static int run_frame(ff_ctx *c, int W, int H, bool overlap)
 int frame_prepare(ff_ctx *c, const uint16_t *depth, const uint8_t *bgr, int W, int H, const double R[9],
                   const double t[3], const double K[4], float scaling, float maxcam, bool upload)
 {
+    if (c->broken) {
+        c->err = "a frame overflowed the per-frame node table: the context is unusable";
+        return FF_E_STATE;
+    }
     CamF p = make_p(c, R, t, K);
     FrameCounters *hf = c->h_fc;
     const uint32_t stamp = ++c->stampCounter;
@@ -1979,12 +1985,12 @@ int frame_finish(ff_ctx *c, int W, int H, bool timed)
         const int r = order_big_frame(c, W, H);
         if (r) return r;
     }
-    if (hf->flags & FF_FLAG_HASH_FULL) {
-        c->err = "per-frame node table overflow";
-        return FF_E_POOL_FULL;
-    }
-    if (hf->listCount > c->hv.listCap) {
-        c->err = "per-frame node list overflow";
+    if ((hf->flags & FF_FLAG_HASH_FULL) || hf->listCount > c->hv.listCap) {
+        // the frame's nodes were dropped, but node-table entries beyond the
+        // list may remain: the context takes no further frames
+        c->broken = true;
+        c->err = (hf->flags & FF_FLAG_HASH_FULL) ? "per-frame node table overflow: the context is unusable"
+                                                 : "per-frame node list overflow: the context is unusable";
         return FF_E_POOL_FULL;
     }
     // the host mirror of the counts k_commit added on the device

@@ -126,6 +126,7 @@ struct ff_ctx {
     // parameters
     float scale = 0, dT = 0;
     bool useColor = true, incremental = true;
+    bool broken = false;  // a frame overflowed the per-frame node table: no further frames
     int device = 0;
     uint16_t bandwidth = 0;
     // map scalars (host mirror)

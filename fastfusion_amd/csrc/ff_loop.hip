@@ -497,7 +497,9 @@ static int loop_submit(ff_loop *lp, const ff_loop_frame *fr, int nf, size_t *app
         A.dist = lp->d_dist; A.wgt = lp->d_wgt; A.col = lp->d_col;
         A.depth = f.depth; A.red = f.red; A.green = f.green; A.blue = f.blue;
         A.W = f.width; A.H = f.height;
-        A.nLeavesUsed = c->nLeavesUsed;
+        // (the used count passes the capacity when the pool ran out; the
+        // slot map and the per-brick arrays have mapN entries)
+        A.nLeavesUsed = std::min(c->nLeavesUsed, lp->mapN);
         A.total = lp->total;
         LoopCounters *lc = lp->d_lc + k;
         if (n <= FF_LOOP_SMALL) {

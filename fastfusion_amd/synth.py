@@ -126,3 +126,37 @@ def config_c2(n_frames=1000, seed_scene=11, seed_traj=12, w=640, h=480):
     sc = Scene.room_scene(seed_scene, (4.0, 4.0, 3.0), 6, 4, 0.1, 0.5)
     R, t = sc.walk(n_frames, seed_traj, 0.5, 3.0, 0.02, 2.0)
     return sc, R, t, dict(scale=0.005, threshold=0.015, maxcam=4.0, w=w, h=h, dropout=0.005, noise=1.0)
+
+
+def config_c5(n_frames=3000, seed=3000, w=1280, h=960):
+    """Large map: a 12x12x3 m floor of 3x3 rooms (interior walls with door
+    gaps) and 60 primitives, a seeded random walk 0.5-3 m from surfaces in
+    2 cm / 2 degree steps, 1280x960 depth (fx = fy = 1050); voxel 2.5 mm,
+    4-voxel truncation (10 mm), max depth 4 m."""
+    sc = Scene.floor_scene(seed, (12.0, 12.0, 3.0), (3, 3), 60)
+    R, t = sc.walk(n_frames, seed + 1, 0.5, 3.0, 0.02, 2.0)
+    f = 525.0 * w / 640.0
+    K = (f, f, (w - 1) / 2.0, (h - 1) / 2.0)
+    return sc, R, t, dict(scale=0.0025, threshold=0.010, maxcam=4.0, w=w, h=h, dropout=0.005, noise=1.0, K=K)
+
+
+def stream_configs(n):
+    """The batched-streams workload: stream i (seed 100 + i) has a room of
+    2-8 m per horizontal side and 2.5-3.5 m height with 3-20 primitives, a
+    depth resolution from {320x240, 640x480, 1280x960} (fx = fy = 525 w / 640),
+    4-10 mm voxels, 3-voxel truncation, a random walk 0.5-3 m from surfaces
+    (walk seed = stream seed + 7), frame i rendered with noise seed
+    seed * 10007 + i."""
+    out = []
+    for i in range(n):
+        rng = np.random.default_rng(100 + i)
+        size = (float(rng.uniform(2, 8)), float(rng.uniform(2, 8)), float(rng.uniform(2.5, 3.5)))
+        nprim = int(rng.integers(3, 21))
+        nb = int(rng.integers(0, nprim + 1))
+        w, h = [(320, 240), (640, 480), (1280, 960)][int(rng.integers(0, 3))]
+        f = 525.0 * w / 640.0
+        vox = float(rng.uniform(0.004, 0.010))
+        sc = Scene.room_scene(100 + i, size, nb, nprim - nb, 0.1, 0.5)
+        out.append(dict(scene=sc, size=size, prims=nprim, w=w, h=h, K=(f, f, (w - 1) / 2.0, (h - 1) / 2.0),
+                        scale=vox, threshold=3 * vox, seed=100 + i, maxcam=4.0))
+    return out

@@ -43,13 +43,25 @@ def compare_state(gpu, ora, w, h, check_bricks=True, tag=""):
     return cg
 
 
-def run_pair(sc, R, t, frames, cfg, gpu, ora, seed0=1000, check_every=1, check_bricks=True):
+def run_pair(sc, R, t, frames, cfg, gpu, ora, seed0=1000, check_every=1, check_bricks=True, learned=False,
+             on_frame=None):
+    """Fuses the given frames into both maps (the camera is cfg["K"], else
+    this module's K) and compares their state every check_every frames and
+    after the last.  learned: the GPU's per-pixel confidence of each frame is
+    handed to the oracle.  Returns the queued count of every frame."""
     w, h = cfg["w"], cfg["h"]
+    Kc = cfg.get("K", K)
+    queued = []
     for k, i in enumerate(frames):
-        d, bgr = sc.render(R[i].reshape(3, 3), t[i], K, w, h, noise=cfg.get("noise", 1.0),
+        d, bgr = sc.render(R[i].reshape(3, 3), t[i], Kc, w, h, noise=cfg.get("noise", 1.0),
                            dropout=cfg.get("dropout", 0.005), seed=seed0 + i)
-        ng = gpu.addMap(d, R[i], t[i], K, bgr, 1 / 5000.0, cfg["maxcam"])
-        no = ora.add_map(d, R[i], t[i], K, bgr, 1 / 5000.0, cfg["maxcam"])
+        ng = gpu.addMap(d, R[i], t[i], Kc, bgr, 1 / 5000.0, cfg["maxcam"])
+        conf = gpu.confidence(w, h) if learned else None
+        no = ora.add_map(d, R[i], t[i], Kc, bgr, 1 / 5000.0, cfg["maxcam"], conf=conf)
         assert ng == no, f"frame {i}: gpu queued {ng}, oracle {no}"
+        queued.append(ng)
         if (k % check_every) == 0 or k == len(frames) - 1:
             compare_state(gpu, ora, w, h, check_bricks, tag=f"frame {i}")
+        if on_frame is not None:
+            on_frame(k, i)
+    return queued
