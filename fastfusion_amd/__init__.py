@@ -59,6 +59,8 @@ ABI = {
                                   ctypes.c_float, ctypes.c_float]),
     "ff_add_map_device": (ctypes.c_int, [_P, _P, _P, ctypes.c_int, ctypes.c_int, _DP, _DP, _DP,
                                          ctypes.c_float, ctypes.c_float]),
+    "ff_add_map_device_noalloc": (ctypes.c_int, [_P, _P, _P, ctypes.c_int, ctypes.c_int, _DP, _DP, _DP,
+                                                 ctypes.c_float, ctypes.c_float]),
     "ff_update_meshes": (ctypes.c_int, [_P]),
     "ff_mesh_sizes": (ctypes.c_int, [_P, ctypes.POINTER(_SZ), ctypes.POINTER(_SZ)]),
     "ff_copy_mesh": (ctypes.c_int, [_P, _P, _P, _P]),
@@ -293,6 +295,15 @@ class FusionMipMapGPU:
         pR, pt, pK = self._pose_args(R, t, K)
         rc = self._lib.ff_add_map_device(self._ctx, depth_ptr, rgb_ptr, int(w), int(h), pR, pt, pK, float(scaling),
                                          float(maxcamdistance))
+        return self._check(rc)
+
+    def addMapDeviceNoAlloc(self, depth_ptr, rgb_ptr, w, h, R, t, K, scaling=1.0 / 5000.0, maxcamdistance=10.0):
+        """addMapDevice through ff_add_map_device_noalloc: the frame's
+        image-sized buffers from the library's per-device scratch array, no
+        device allocation (no counterpart in the reference)."""
+        pR, pt, pK = self._pose_args(R, t, K)
+        rc = self._lib.ff_add_map_device_noalloc(self._ctx, depth_ptr, rgb_ptr, int(w), int(h), pR, pt, pK,
+                                                 float(scaling), float(maxcamdistance))
         return self._check(rc)
 
     # -- updateMeshes, :2884-2954

@@ -29,6 +29,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 using namespace ff;
@@ -1366,10 +1367,92 @@ CamF camera_inverse(const ff_ctx *c, const double R[9], const double t[3], const
     return make_pinv(c, R, t, K);
 }
 
+// the dynamic LDS of the de-dup sweeps (their ring of diagonal bands)
+static hipError_t dedup_attributes()
+{
+    hipError_t e = hipSuccess;
+#define FF_DD_ATTR(K, WM)                                                                                  \
+    if (e == hipSuccess)                                                                                   \
+        e = hipFuncSetAttribute((const void *)K<WM>, hipFuncAttributeMaxDynamicSharedMemorySize,         \
+                                (int)dedup_lds_bytes(WM));
+    FF_DD_ATTR(k_dedup, 20) FF_DD_ATTR(k_dedup, 40) FF_DD_ATTR(k_dedup, 64)
+    FF_DD_ATTR(k_dedup_b, 20) FF_DD_ATTR(k_dedup_b, 40) FF_DD_ATTR(k_dedup_b, 64)
+#undef FF_DD_ATTR
+    return e;
+}
+
+// ---------------------------------------------------------------------------
+// The per-device scratch array of ff_add_map_device_noalloc: a frame's
+// image-sized buffers (bit planes, processed bitmap, the count scan's
+// per-pixel arrays, the per-pixel parent record, the confidences) carved
+// from one file-scope array instead of the context's own allocations.  In
+// host code `scratch` names a host-side stand-in; its device address on
+// each device comes from hipGetSymbolAddress, taken once per device with
+// that device current (at the first context created on it).
+// ---------------------------------------------------------------------------
+#define FF_SCRATCH_BYTES (40u << 20)  // 40 MiB: images up to 1280 x 960 (about 32 B per pixel)
+#define FF_MAX_DEVICES 64
+static __device__ __align__(256) unsigned char scratch[FF_SCRATCH_BYTES];
+static void *g_scratch[FF_MAX_DEVICES];
+static std::mutex g_scratch_mu;
+
+static hipError_t scratch_register(int device)
+{
+    if (device < 0 || device >= FF_MAX_DEVICES) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch[device]) return hipSuccess;
+    void *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));  // the caller made `device` current
+    if (e == hipSuccess) g_scratch[device] = ptr;
+    return e;
+}
+
+static void *scratch_ptr(int device)
+{
+    if (device < 0 || device >= FF_MAX_DEVICES) return nullptr;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    return g_scratch[device];
+}
+
+// carves a W x H frame's buffers from the scratch array at base; returns the
+// bytes used (more than FF_SCRATCH_BYTES: the image does not fit)
+struct ScratchLayout {
+    uint32_t *planes, *pbits, *cntB, *cntL, *firstD, *pl0;
+    uint64_t *pref, *bsum;
+    float *conf;
+};
+static size_t scratch_layout(unsigned char *base, int W, int H, ScratchLayout *L)
+{
+    const size_t npix = (size_t)W * H;
+    const size_t words = (size_t)(W + 31) / 32;
+    const size_t sizes[9] = {sizeof(uint4) * words * (size_t)dedup_plane_rows(H), pbits_bytes(H),
+                             sizeof(uint32_t) * npix, sizeof(uint32_t) * npix, sizeof(uint32_t) * npix,
+                             sizeof(uint32_t) * 2 * npix, sizeof(uint64_t) * npix,
+                             sizeof(uint64_t) * ((npix + 4095) / 4096 + 1), sizeof(float) * npix};
+    unsigned char *p[9];
+    size_t off = 0;
+    for (int k = 0; k < 9; k++) {
+        p[k] = base + off;
+        off += (sizes[k] + 255) & ~(size_t)255;
+    }
+    if (L) {
+        L->planes = reinterpret_cast<uint32_t *>(p[0]);
+        L->pbits = reinterpret_cast<uint32_t *>(p[1]);
+        L->cntB = reinterpret_cast<uint32_t *>(p[2]);
+        L->cntL = reinterpret_cast<uint32_t *>(p[3]);
+        L->firstD = reinterpret_cast<uint32_t *>(p[4]);
+        L->pl0 = reinterpret_cast<uint32_t *>(p[5]);
+        L->pref = reinterpret_cast<uint64_t *>(p[6]);
+        L->bsum = reinterpret_cast<uint64_t *>(p[7]);
+        L->conf = reinterpret_cast<float *>(p[8]);
+    }
+    return off;
+}
+
 int ensure_image(ff_ctx *c, int W, int H)
 {
-    if (W <= 0 || H <= 0 || W > 2048) {
-        c->err = "image width must be in [1, 2048]";
+    if (W <= 0 || H <= 0 || W > FF_MAX_W || H > FF_MAX_H) {
+        c->err = "image width and height must be in [1, 2048]";
         return FF_E_ARG;
     }
     if (c->capW >= W && c->capH >= H) return FF_OK;
@@ -1378,22 +1461,7 @@ int ensure_image(ff_ctx *c, int W, int H)
     c->freeFrameBuffers();
     HIPCHK(hipMalloc(&c->d_depth, npix * 2));
     HIPCHK(hipMalloc(&c->d_bgr, npix * 3));
-    HIPCHK(hipMalloc(&c->d_pxp, sizeof(float) * W));
-    HIPCHK(hipMalloc(&c->d_pyp, sizeof(float) * H));
-    // bit planes padded to whole bands of the de-dup ring
     HIPCHK(hipMalloc(&c->d_planes, sizeof(uint4) * words * (size_t)dedup_plane_rows(H)));
-    HIPCHK(hipFuncSetAttribute((const void *)k_dedup<20>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                               (int)dedup_lds_bytes(20)));
-    HIPCHK(hipFuncSetAttribute((const void *)k_dedup<40>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                               (int)dedup_lds_bytes(40)));
-    HIPCHK(hipFuncSetAttribute((const void *)k_dedup<64>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                               (int)dedup_lds_bytes(64)));
-    HIPCHK(hipFuncSetAttribute((const void *)k_dedup_b<20>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                               (int)dedup_lds_bytes(20)));
-    HIPCHK(hipFuncSetAttribute((const void *)k_dedup_b<40>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                               (int)dedup_lds_bytes(40)));
-    HIPCHK(hipFuncSetAttribute((const void *)k_dedup_b<64>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                               (int)dedup_lds_bytes(64)));
     HIPCHK(hipMalloc(&c->d_pbits, pbits_bytes(H)));
     HIPCHK(hipMalloc(&c->d_cntB, sizeof(uint32_t) * npix));
     HIPCHK(hipMalloc(&c->d_cntL, sizeof(uint32_t) * npix));
@@ -1519,9 +1587,9 @@ using namespace ff;
 
 void ff_ctx::freeFrameBuffers()
 {
-    hipFree(d_depth); hipFree(d_bgr); hipFree(d_pxp); hipFree(d_pyp); hipFree(d_planes); hipFree(d_pbits);
+    hipFree(d_depth); hipFree(d_bgr); hipFree(d_planes); hipFree(d_pbits);
     hipFree(d_cntB); hipFree(d_cntL); hipFree(d_firstD); hipFree(d_pl0); hipFree(d_pref); hipFree(d_bsum);
-    d_depth = nullptr; d_bgr = nullptr; d_pxp = d_pyp = nullptr; d_planes = d_pbits = nullptr;
+    d_depth = nullptr; d_bgr = nullptr; d_planes = d_pbits = nullptr;
     d_cntB = d_cntL = d_firstD = d_pl0 = nullptr; d_pref = d_bsum = nullptr;
     capW = capH = 0;
 }
@@ -1572,7 +1640,10 @@ ff_ctx *ff_create(const ff_params *prm)
               hipMalloc(&c->d_fd, sizeof(FrameDev)) == hipSuccess &&
               hipHostMalloc(&c->h_fd, sizeof(FrameDev), hipHostMallocDefault) == hipSuccess &&
               hipMalloc(&c->d_mapc, sizeof(MapCounters)) == hipSuccess &&
-              hipHostMalloc(&c->h_mapc, sizeof(MapCounters), hipHostMallocDefault) == hipSuccess;
+              hipHostMalloc(&c->h_mapc, sizeof(MapCounters), hipHostMallocDefault) == hipSuccess &&
+              hipMalloc(&c->d_pxp, sizeof(float) * FF_MAX_W) == hipSuccess &&
+              hipMalloc(&c->d_pyp, sizeof(float) * FF_MAX_H) == hipSuccess &&
+              dedup_attributes() == hipSuccess && scratch_register(c->device) == hipSuccess;
     if (ok && c->useColor) ok = hipMalloc(&c->d_col, sizeof(uint16_t) * NL * FF_BRICK * 3) == hipSuccess;
     const uint32_t hcap = 1u << 22;
     ok = ok && hipMalloc(&c->hv.keys, sizeof(uint64_t) * hcap) == hipSuccess &&
@@ -1623,6 +1694,7 @@ void ff_destroy(ff_ctx *c)
     hipFree(c->d_fc); hipHostFree(c->h_fc);
     hipFree(c->d_fd); hipHostFree(c->h_fd);
     hipFree(c->d_mapc); hipHostFree(c->h_mapc);
+    hipFree(c->d_pxp); hipFree(c->d_pyp);
     drop_frame_graphs(c);
     hipFree(c->hv.keys); hipFree(c->hv.minpix); hipFree(c->hv.aux); hipFree(c->hv.assigned); hipFree(c->hv.list);
     for (int i = 0; i < 8; i++)
@@ -1728,6 +1800,7 @@ AllocStream alloc_stream(ff_ctx *c, int W, int H)
     a.fc = c->d_fc;
     a.planes = c->d_planes; a.cntB = c->d_cntB; a.cntL = c->d_cntL; a.firstD = c->d_firstD;
     a.pbits = c->d_pbits; a.pl0 = c->d_pl0;
+    c->lastPbits = c->d_pbits;
     a.pref = c->d_pref; a.bsum = c->d_bsum;
     a.mapc = c->d_mapc;
     mesh_binfo(c, &a.binfo, &a.binfoCap);
@@ -1843,7 +1916,8 @@ static int run_frame(ff_ctx *c, int W, int H, bool overlap)
     }
     const uint64_t key = ((uint64_t)W << 48) ^ ((uint64_t)H << 32) ^ ((uint64_t)c->bufGen << 1) ^
                          (uint64_t)(c->learned != nullptr);
-    if (c->useGraphs && (!c->frameExec[0] || c->frameKey != key)) {
+    const bool graphs = c->useGraphs && !c->directOnly;
+    if (graphs && (!c->frameExec[0] || c->frameKey != key)) {
         drop_frame_graphs(c);
         if (capture(c, launch_alloc, W, H, &c->frameGraph[0], &c->frameExec[0]) &&
             capture(c, launch_sdf, W, H, &c->frameGraph[1], &c->frameExec[1])) {
@@ -1853,7 +1927,7 @@ static int run_frame(ff_ctx *c, int W, int H, bool overlap)
             c->useGraphs = false;
         }
     }
-    const bool g = c->useGraphs && c->frameExec[0];
+    const bool g = graphs && c->useGraphs && c->frameExec[0];
     if (c->timingAll) hipEventRecord(c->ev[0], c->stream);
     if (g) HIPCHK(hipGraphLaunch(c->frameExec[0], c->stream));
     else launch_alloc(c, W, H);
@@ -1933,6 +2007,7 @@ int frame_prepare(ff_ctx *c, const uint16_t *depth, const uint8_t *bgr, int W, i
         f.sp.conf = learned_prepare(c, W, H);
         if (!f.sp.conf) return FF_E_DEVICE;
     }
+    c->lastConf = f.sp.conf;
     f.depth = depth;
     f.bgr = bgr;
     f.scaling = scaling;
@@ -2050,6 +2125,7 @@ static int add_map_impl(ff_ctx *c, const uint16_t *depth, const uint8_t *bgr, in
 {
     int rc = frame_prepare(c, depth, bgr, W, H, R, t, K, scaling, maxcam, !overlap);
     if (rc) return rc;
+    c->lastPbits = c->d_pbits;
     if (c->n) {
         rc = run_frame(c, W, H, overlap);
         if (rc) return rc;
@@ -2103,6 +2179,51 @@ int ff_add_map_device(ff_ctx *c, const uint16_t *depth_dev, const uint8_t *bgr_d
     // read in place: the call returns after the frame's kernels completed
     return add_map_impl(c, depth_dev, c->useColor ? bgr_dev : nullptr, W, H, R, t, K, scaling, maxcam,
                         front_overlap(c, W, H));
+}
+
+int ff_add_map_device_noalloc(ff_ctx *c, const uint16_t *depth_dev, const uint8_t *bgr_dev, int W, int H,
+                              const double R[9], const double t[3], const double K[4], float scaling, float maxcam)
+{
+    if (!c || !depth_dev || !R || !t || !K) return FF_E_ARG;
+    if (c->useColor && !bgr_dev) {
+        c->err = "color is enabled but no bgr image was given";
+        return FF_E_ARG;
+    }
+    if (W <= 0 || H <= 0 || W > FF_MAX_W || H > FF_MAX_H) {
+        c->err = "image width and height must be in [1, 2048]";
+        return FF_E_ARG;
+    }
+    ScratchLayout L;
+    if (scratch_layout(nullptr, W, H, nullptr) > FF_SCRATCH_BYTES) {
+        c->err = "image too large for the allocation-free entry point (its buffers must fit 40 MiB: up to "
+                 "1280 x 960 pixels)";
+        return FF_E_ARG;
+    }
+    hipSetDevice(c->device);
+    unsigned char *base = static_cast<unsigned char *>(scratch_ptr(c->device));
+    if (!base) {
+        c->err = "no scratch array address for this device";
+        return FF_E_STATE;
+    }
+    scratch_layout(base, W, H, &L);
+    // the frame runs on the scratch array's buffers in place of the
+    // context's (which keep their contents), with direct launches (a
+    // captured graph refers to the context's buffers)
+    uint32_t *sv[6] = {c->d_planes, c->d_pbits, c->d_cntB, c->d_cntL, c->d_firstD, c->d_pl0};
+    uint64_t *sv64[2] = {c->d_pref, c->d_bsum};
+    c->d_planes = L.planes; c->d_pbits = L.pbits; c->d_cntB = L.cntB; c->d_cntL = L.cntL;
+    c->d_firstD = L.firstD; c->d_pl0 = L.pl0; c->d_pref = L.pref; c->d_bsum = L.bsum;
+    float *confSave = nullptr;
+    size_t confCapSave = 0;
+    learned_swap_conf(c, L.conf, (size_t)W * H, &confSave, &confCapSave);
+    c->directOnly = true;
+    const int rc = add_map_impl(c, depth_dev, c->useColor ? bgr_dev : nullptr, W, H, R, t, K, scaling, maxcam,
+                                front_overlap(c, W, H));
+    c->directOnly = false;
+    learned_swap_conf(c, confSave, confCapSave, nullptr, nullptr);
+    c->d_planes = sv[0]; c->d_pbits = sv[1]; c->d_cntB = sv[2]; c->d_cntL = sv[3]; c->d_firstD = sv[4];
+    c->d_pl0 = sv[5]; c->d_pref = sv64[0]; c->d_bsum = sv64[1];
+    return rc;
 }
 
 int ff_set_graph_capture(ff_ctx *c, int on)
@@ -2229,8 +2350,9 @@ int ff_copy_processed(ff_ctx *c, uint8_t *out, size_t n)
     HIPCHK(hipStreamSynchronize(c->stream));
     const int W = c->imgW, H = c->imgH, words = (W + 31) / 32;
     (void)words;
+    if (!c->lastPbits) return FF_E_STATE;
     std::vector<uint32_t> bits((size_t)(H + 2 * FF_PB_PAD) * FF_PB_STRIDE);
-    HIPCHK(hipMemcpy(bits.data(), c->d_pbits, bits.size() * 4, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(bits.data(), c->lastPbits, bits.size() * 4, hipMemcpyDeviceToHost));
     for (int y = 0; y < H; y++)
         for (int x = 0; x < W; x++)
             out[(size_t)y * W + x] = (bits[(size_t)(y + FF_PB_PAD) * FF_PB_STRIDE + (x >> 5)] >> (x & 31)) & 1u;

@@ -18,6 +18,10 @@
 // de-dup sweep's skewed lanes store up to H + 158 rows past the top)
 #define FF_PB_STRIDE 64
 #define FF_PB_PAD 64
+// image size limits (the de-dup ring has 64 words of columns; the ray
+// tables are allocated with the context at these sizes)
+#define FF_MAX_W 2048
+#define FF_MAX_H 2048
 inline size_t pbits_bytes(int H) { return sizeof(uint32_t) * FF_PB_STRIDE * (size_t)(H + 3 * FF_PB_PAD); }
 
 namespace ff {
@@ -154,7 +158,13 @@ struct ff_ctx {
     // frame buffers
     uint16_t *d_depth = nullptr;
     uint8_t *d_bgr = nullptr;
-    float *d_pxp = nullptr, *d_pyp = nullptr;  // per-column / per-row ray tables of the current image size
+    float *d_pxp = nullptr, *d_pyp = nullptr;  // per-column / per-row ray tables (FF_MAX_W / FF_MAX_H entries)
+    // where the last frame left its processed-pixel bitmap and confidences
+    // (the context's buffers, or the device's scratch array after
+    // ff_add_map_device_noalloc)
+    const uint32_t *lastPbits = nullptr;
+    const float *lastConf = nullptr;
+    bool directOnly = false;  // the frame in flight launches directly (no graph capture or replay)
     uint32_t *d_planes = nullptr, *d_pbits = nullptr, *d_cntB = nullptr, *d_cntL = nullptr, *d_firstD = nullptr,
              *d_pl0 = nullptr;
     uint64_t *d_pref = nullptr, *d_bsum = nullptr;
@@ -203,6 +213,7 @@ namespace ff {
 void learned_free(ff_ctx *c);
 const float *learned_prepare(ff_ctx *c, int W, int H);
 void learned_launch(ff_ctx *c, int W, int H, hipStream_t st);
+void learned_swap_conf(ff_ctx *c, float *buf, size_t cap, float **old, size_t *oldCap);
 void sync_map_counters(ff_ctx *c);
 void drop_frame_graphs(ff_ctx *c);
 // a frame in phases (ff_fusion.hip): host preparation and parameter upload,
@@ -234,8 +245,10 @@ void batch_free(ff_ctx *c);
 // meshing module hooks (ff_mesh.hip)
 int mesh_init(ff_ctx *c);
 // the host part of an enqueued mesh update (a no-op when none is pending):
-// every mesh query and every change of the map's structure calls it first
-int mesh_complete(ff_ctx *c);
+// every mesh query and every change of the map's structure calls it first.
+// maintain = false (inside a frame) leaves its allocating part (buffer
+// growth, pool compaction) to the next call with maintain = true
+int mesh_complete(ff_ctx *c, bool maintain = true);
 void mesh_free(ff_ctx *c);
 void mesh_on_initial_volume(ff_ctx *c);
 int mesh_on_load(ff_ctx *c, const std::vector<ushort4> &binfo);

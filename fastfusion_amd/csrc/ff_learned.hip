@@ -655,6 +655,7 @@ void learned_free(ff_ctx *c)
     hipFree(c->learned->conf);
     delete c->learned;
     c->learned = nullptr;
+    c->lastConf = nullptr;
 }
 
 // the confidence buffer for a W x H frame, or nullptr on failure
@@ -672,6 +673,18 @@ const float *learned_prepare(ff_ctx *c, int W, int H)
         c->bufGen++;
     }
     return L->conf;
+}
+
+// the confidence buffer replaced by (buf, cap) for one frame; the previous
+// pair goes to (*old, *oldCap) when those are given (no-op with learned off)
+void learned_swap_conf(ff_ctx *c, float *buf, size_t cap, float **old, size_t *oldCap)
+{
+    LearnedState *L = c->learned;
+    if (!L) return;
+    if (old) *old = L->conf;
+    if (oldCap) *oldCap = L->confCap;
+    L->conf = buf;
+    L->confCap = cap;
 }
 
 // per-pixel confidence of the frame described by c->d_fd
@@ -815,14 +828,14 @@ int ff_set_learned_weighting(ff_ctx *c, const float *params, size_t n)
 int ff_copy_confidence(ff_ctx *c, float *out, size_t n)
 {
     if (!c || !out) return FF_E_ARG;
-    if (!c->learned || !c->learned->conf) {
+    if (!c->learned || !c->lastConf) {
         c->err = "learned weighting is off";
         return FF_E_STATE;
     }
     const size_t np = (size_t)c->imgW * c->imgH;
     if (n < np) return FF_E_ARG;
     if (!lok(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize")) return FF_E_DEVICE;
-    if (!lok(c, hipMemcpy(out, c->learned->conf, sizeof(float) * np, hipMemcpyDeviceToHost), "hipMemcpy")) return FF_E_DEVICE;
+    if (!lok(c, hipMemcpy(out, c->lastConf, sizeof(float) * np, hipMemcpyDeviceToHost), "hipMemcpy")) return FF_E_DEVICE;
     return FF_OK;
 }
 

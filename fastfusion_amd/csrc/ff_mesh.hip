@@ -193,6 +193,13 @@ struct MeshState {
     // the map the update in flight meshes: the frame stamp and the tree
     // (frames fused before its host part runs must not be attributed to it)
     uint32_t launchStamp = 0, launchNB = 0, launchNL = 0, launchGrows = 0;
+    // the allocating part of an update's host part (sample-buffer growth,
+    // compaction into fresh pools, pool growth), left for the next mesh call
+    // or query when the host part ran inside a frame (a grow): frames
+    // allocate no device memory
+    bool maintPending = false;
+    uint32_t maintFlags = 0, maintTiles = 0;
+    int maintTable = 0;
 };
 
 // both cell hashes emptied (every cell is meshed again by the next update)
@@ -1735,7 +1742,7 @@ void mesh_free(ff_ctx *c)
 void mesh_on_initial_volume(ff_ctx *c)
 {
     MeshState *s = c->mesh;
-    mesh_complete(c);
+    mesh_complete(c, false);  // (inside a frame: no allocation)
     const ushort4 root = make_ushort4(0, 0, 0, (unsigned short)(c->n >> 1));
     hipMemcpyAsync(s->d_binfo, &root, sizeof root, hipMemcpyHostToDevice, c->stream);
     hipStreamSynchronize(c->stream);
@@ -1765,15 +1772,15 @@ void mesh_on_grow_blocks(ff_ctx *c, uint32_t oldBlocks, uint32_t addBlocks, cons
                          const std::vector<ushort4> &chain)
 {
     MeshState *s = c->mesh;
-    mesh_complete(c);  // its table is the one the grow discards
-    ushort4 *tmp = nullptr;
-    if (hipMalloc(&tmp, sizeof(ushort4) * (oldBlocks + 1)) != hipSuccess) return;
+    mesh_complete(c, false);  // its table is the one the grow discards (inside a frame: no allocation)
+    // the old geometry through the tree's scratch copy, free once the tree
+    // has moved (4 B per tree slot, 0.5 B per slot needed here)
+    ushort4 *tmp = reinterpret_cast<ushort4 *>(c->d_treeTmp);
     hipMemcpyAsync(tmp, s->d_binfo, sizeof(ushort4) * oldBlocks, hipMemcpyDeviceToDevice, c->stream);
     k_binfo_shift<<<(oldBlocks + 255) / 256, 256, 0, c->stream>>>(tmp, s->d_binfo + addBlocks, oldBlocks, addv[0],
                                                                   addv[1], addv[2]);
     hipMemcpyAsync(s->d_binfo, chain.data(), sizeof(ushort4) * chain.size(), hipMemcpyHostToDevice, c->stream);
     hipStreamSynchronize(c->stream);
-    hipFree(tmp);
     // every cell moved: drop the previous table so all cells re-mesh
     s->nCells[s->cur] = 0;
     (void)hash_reset(s, c->stream);
@@ -2050,28 +2057,25 @@ void mesh_launch_post(ff_ctx *c, bool timed)
     }
 }
 
-// after the device work of pre / mc / post completed
-int mesh_finish(ff_ctx *c, bool timed)
+// the allocating part of an update's host part (see maintPending)
+static int mesh_maintain(ff_ctx *c)
 {
     MeshState *s = c->mesh;
-    const int nxt = s->nxt;
-    if (timed && c->timingAll) {
-        float ms = 0.0f;
-        if (hipEventElapsedTime(&ms, c->ev[6], c->ev[7]) == hipSuccess) {
-            c->phaseMs[2] += ms;
-            c->phaseCalls[2]++;
-        }
-    }
-    if (s->h_mc->nTiles > s->tileCap && !(s->h_mc->flags & FF_CELL_OVERFLOW)) {
+    if (!s->maintPending) return FF_OK;
+    s->maintPending = false;
+    const int nxt = s->maintTable;
+    const uint32_t flags = s->maintFlags;
+    if (s->maintTiles > s->tileCap) {
         // some cells' samples did not fit the buffer (the marching cubes
         // sampled them in place): grow it for the next updates
         uint32_t cap = s->tileCap;
-        while (cap < s->h_mc->nTiles && cap < FF_TILE_CAP_MAX) cap <<= 1;
+        while (cap < s->maintTiles && cap < FF_TILE_CAP_MAX) cap <<= 1;
         if (cap > s->tileCap) {
             TileRec *nt = nullptr;
             uint4 *nb = nullptr;
             if (hipMalloc(&nt, sizeof(TileRec) * cap) == hipSuccess &&
                 hipMalloc(&nb, sizeof(uint4) * 64 * (size_t)cap) == hipSuccess) {
+                MCHK(hipStreamSynchronize(c->stream));  // no update in flight reads the old buffer
                 hipFree(s->d_tiles);
                 hipFree(s->d_sbuf);
                 s->d_tiles = nt;
@@ -2083,31 +2087,6 @@ int mesh_finish(ff_ctx *c, bool timed)
             }
         }
     }
-    const MeshCounters &h = *s->h_mc;
-    if (h.flags & FF_CELL_OVERFLOW) {
-        c->err = "mesh cell table overflow";
-        return FF_E_POOL_FULL;
-    }
-    s->cur = nxt;
-    s->nCells[nxt] = std::min(h.nCells, s->cellCap);
-    s->lastMeshStamp = s->launchStamp;
-    if (!s->reuse) {
-        s->builtNB = s->launchNB;
-        s->builtNL = s->launchNL;
-        s->builtGrows = s->launchGrows;
-    }
-    c->phaseUnits[2] += h.nDirty;
-    // algorithmic bytes: the 512 source voxels (14 B) of each re-meshed cell,
-    // 15 B per vertex and 12 B per triangle written (vertex and face counts
-    // from the arena totals, which only grow between compactions)
-    if (c->timingAll) {
-        const uint64_t va = h.pad[0], fa = h.pad[1];
-        const uint64_t dv = va >= s->lastVAlloc ? va - s->lastVAlloc : va, df = fa >= s->lastFAlloc ? fa - s->lastFAlloc : fa;
-        c->phaseBytes[2] += (double)h.nDirty * 512.0 * 14.0 + 15.0 * (double)dv + 12.0 * (double)df;
-        s->lastVAlloc = va;
-        s->lastFAlloc = fa;
-    }
-    const uint32_t flags = h.flags, nDirty = h.nDirty;
     if (flags & (FF_MESH_OVERFLOW | FF_MESH_HALF)) {
         // compact live geometry into fresh pools (cell k to arena k mod FF_ARENAS)
         float4 *nv = nullptr;
@@ -2168,14 +2147,62 @@ int mesh_finish(ff_ctx *c, bool timed)
             MCHK(hash_reset(s, c->stream));
         }
     }
+    return FF_OK;
+}
+
+// after the device work of pre / mc / post completed
+int mesh_finish(ff_ctx *c, bool timed)
+{
+    MeshState *s = c->mesh;
+    const int nxt = s->nxt;
+    if (timed && c->timingAll) {
+        float ms = 0.0f;
+        if (hipEventElapsedTime(&ms, c->ev[6], c->ev[7]) == hipSuccess) {
+            c->phaseMs[2] += ms;
+            c->phaseCalls[2]++;
+        }
+    }
+    const MeshCounters &h = *s->h_mc;
+    if (h.flags & FF_CELL_OVERFLOW) {
+        c->err = "mesh cell table overflow";
+        return FF_E_POOL_FULL;
+    }
+    s->cur = nxt;
+    s->nCells[nxt] = std::min(h.nCells, s->cellCap);
+    s->lastMeshStamp = s->launchStamp;
+    if (!s->reuse) {
+        s->builtNB = s->launchNB;
+        s->builtNL = s->launchNL;
+        s->builtGrows = s->launchGrows;
+    }
+    c->phaseUnits[2] += h.nDirty;
+    // algorithmic bytes: the 512 source voxels (14 B) of each re-meshed cell,
+    // 15 B per vertex and 12 B per triangle written (vertex and face counts
+    // from the arena totals, which only grow between compactions)
+    if (c->timingAll) {
+        const uint64_t va = h.pad[0], fa = h.pad[1];
+        const uint64_t dv = va >= s->lastVAlloc ? va - s->lastVAlloc : va, df = fa >= s->lastFAlloc ? fa - s->lastFAlloc : fa;
+        c->phaseBytes[2] += (double)h.nDirty * 512.0 * 14.0 + 15.0 * (double)dv + 12.0 * (double)df;
+        s->lastVAlloc = va;
+        s->lastFAlloc = fa;
+    }
+    const uint32_t nDirty = h.nDirty;
+    const bool tilesShort = h.nTiles > s->tileCap;
+    if (tilesShort || (h.flags & (FF_MESH_OVERFLOW | FF_MESH_HALF))) {
+        s->maintPending = true;
+        s->maintFlags = h.flags & (FF_MESH_OVERFLOW | FF_MESH_HALF);
+        s->maintTiles = tilesShort ? h.nTiles : 0u;
+        s->maintTable = nxt;
+    }
     return (int)nDirty;
 }
 
 // the host part of an enqueued update, once its device work completed
-int mesh_complete(ff_ctx *c)
+int mesh_complete(ff_ctx *c, bool maintain)
 {
     MeshState *s = c->mesh;
-    if (!s || !s->pending) return FF_OK;
+    if (!s) return FF_OK;
+    if (!s->pending) return maintain ? mesh_maintain(c) : FF_OK;
     s->pending = false;
     // counters taken up by a frame's SDF update, which follows the update's
     // marching cubes and had completed when its frame returned: the host
@@ -2191,7 +2218,8 @@ int mesh_complete(ff_ctx *c)
         }
     }
     const int r = mesh_finish(c, s->pendingTimed);
-    return r < 0 ? r : FF_OK;
+    if (r < 0) return r;
+    return maintain ? mesh_maintain(c) : FF_OK;
 }
 
 // enqueues the update; the previous one's host part runs first

@@ -74,6 +74,23 @@ int ff_add_map_device(ff_ctx *ctx, const uint16_t *depth_dev, const uint8_t *bgr
                       int height, const double R[9], const double t[3], const double K[4],
                       float scaling, float maxcam);
 
+/* No counterpart in the reference: ff_add_map_device for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  The frame's image-sized buffers (bit
+ * planes, processed bitmap, count-scan arrays, per-pixel parent records,
+ * confidences) come from one per-device array of 40 MiB that the library
+ * holds for the life of the process, instead of the context's own buffers;
+ * the call allocates nothing (a mesh pool compaction or buffer growth due
+ * from a previous updateMeshes waits for the next ff_update_meshes).  Images
+ * whose buffers exceed the array (above 1280 x 960 pixels) give FF_E_ARG.
+ * Results equal ff_add_map_device's.  Every call on a device shares that
+ * device's array: calls that overlap in time on one device (from several
+ * threads, on any contexts) race.  ff_copy_processed and ff_copy_confidence
+ * read the array after such a call, until the next call on the device. */
+int ff_add_map_device_noalloc(ff_ctx *ctx, const uint16_t *depth_dev, const uint8_t *bgr_dev, int width,
+                              int height, const double R[9], const double t[3], const double K[4],
+                              float scaling, float maxcam);
+
 /* FusionMipMapCPU::updateMeshes (geometryfusion_mipmap_cpu.cpp:2884-2954):
  * re-meshes the cells touched by the frames since the last call.  The work
  * is enqueued on the context's stream and the call returns (the reference's
