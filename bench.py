@@ -24,11 +24,6 @@ sys.path.insert(0, ROOT)
 
 METRIC = "fused+meshed frames/sec and HBM GB/s at 640x480, 5mm; RMS surface err vs GT"
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per k_sdf launch over the timed frames of the default command
-# (950 frames after 50), from rocprofv3 FETCH_SIZE and WRITE_SIZE passes of
-# that command, each calibrated against a 256 MiB copy
-# (profiles/hbm_traffic_640x480_5mm.md): 7 964 KiB read + 3 314 KiB written
-SDF_TRAFFIC_DEFAULT = 11_548_610
 
 
 def render_frames(sc, R, t, n, cfg):
@@ -202,34 +197,14 @@ def run_loop(args):
     print(json.dumps(line), flush=True)
 
 
-def stream_configs(n):
-    """The batched-streams workload: stream i (seed 100 + i) has a room of
-    2-8 m per horizontal side and 2.5-3.5 m height with 3-20 primitives, a
-    depth resolution from {320x240, 640x480, 1280x960} (fx = fy = 525 w / 640),
-    4-10 mm voxels, 3-voxel truncation, a random walk 0.5-3 m from surfaces."""
-    from fastfusion_amd import synth
-    out = []
-    for i in range(n):
-        rng = np.random.default_rng(100 + i)
-        size = (float(rng.uniform(2, 8)), float(rng.uniform(2, 8)), float(rng.uniform(2.5, 3.5)))
-        nprim = int(rng.integers(3, 21))
-        nb = int(rng.integers(0, nprim + 1))
-        w, h = [(320, 240), (640, 480), (1280, 960)][int(rng.integers(0, 3))]
-        f = 525.0 * w / 640.0
-        vox = float(rng.uniform(0.004, 0.010))
-        sc = synth.Scene.room_scene(100 + i, size, nb, nprim - nb, 0.1, 0.5)
-        out.append(dict(scene=sc, size=size, prims=nprim, w=w, h=h, K=(f, f, (w - 1) / 2.0, (h - 1) / 2.0),
-                        scale=vox, threshold=3 * vox, seed=100 + i))
-    return out
-
-
 def run_streams(args):
     """Batched streams: every stream advances one frame per tick (one SDF and
     one marching-cubes launch for all of them); aggregate frames/s and the
     tick latency distribution."""
     from fastfusion_amd import Batch, FusionMipMapGPU, hiprt
     n, F = args.streams, args.frames
-    cfgs = stream_configs(n)
+    from fastfusion_amd import synth
+    cfgs = synth.stream_configs(n)
     t_r = time.perf_counter()
     dev, frames_t = [], []
     for k, c in enumerate(cfgs):
@@ -332,10 +307,13 @@ def main():
                     help="loop-closure de-integration workload (subtract the walk's frames from a partner map)")
     ap.add_argument("--calib-copy", type=int, default=0,
                     help="MiB copied device-to-device once before the run (HBM counter calibration)")
+    ap.add_argument("--pmc-summary", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
+                    help="summarise the counter_collection.csv of a FETCH_SIZE and a WRITE_SIZE rocprofv3 run of "
+                         "this script (with --calib-copy 256) for the timed frames given by --steps / --warmup")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported in this build "
-              "(independent map streams shard by stream, a single map by Morton range: see DESIGN.md)",
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported: every run here uses one "
+              "GPU (independent map streams would shard by stream, a single map by Morton range of its bricks)",
               file=sys.stderr)
         sys.exit(2)
     if args.calib_copy:
@@ -345,102 +323,228 @@ def main():
         hiprt.check(hiprt.hip().hipMemcpy(dst.ptr, src.ptr, nb, hiprt.D2D), "calibration copy")
         hiprt.synchronize()
         del src, dst
+    if args.pmc_summary:
+        return pmc_summary(args)
     if args.streams:
         return run_streams(args)
     if args.loop:
         return run_loop(args)
 
+    return run_walk(args)
+
+
+def pmc_summary(args):
+    """HBM traffic per frame and per kernel launch of the timed frames, from
+    rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs of this script (KiB per
+    dispatch), each scaled by what the counter reads for the 256 MiB
+    device-to-device copy that --calib-copy 256 runs first.  Dispatches are
+    grouped into frames by the frame's first kernel (k_planes / k_planes_v:
+    a frame's kernels, then its mesh update's, are enqueued in that order);
+    the timed frames are those of the timed run (the first pass over the
+    walk), i.e. frames [n_walk - steps, n_walk), up to the gather of the
+    final mesh that follows them."""
+    import csv
+    import collections
+
+    def load(path):
+        rows = []
+        with open(path) as f:
+            for r in csv.DictReader(f):
+                rows.append((int(r["Dispatch_Id"]), r["Kernel_Name"], float(r["Counter_Value"])))
+        rows.sort()
+        return rows
+
+    n_walk = max(1000, args.warmup + args.steps)
+    first, last = n_walk - args.steps, n_walk
+    out = {}
+    for tag, path in (("fetch", args.pmc_summary[0]), ("write", args.pmc_summary[1])):
+        rows = load(path)
+        calib = next(v for _, k, v in rows if "copyBuffer" in k)  # the 256 MiB copy comes first
+        scale = 256.0 * 1024.0 / calib  # KiB the copy moved / KiB the counter read
+        frame = -1
+        per = collections.defaultdict(list)
+        total = 0.0
+        done = False
+        for _, k, v in rows:
+            name = k.split("(")[0].replace("void ", "").replace("ff::", "").split("<")[0].strip()
+            if name in ("k_planes", "k_planes_v"):
+                frame += 1
+            # the final mesh's gather ends the timed run (the replay follows)
+            if frame >= first and name == "k_gather":
+                done = True
+            if first <= frame < last and not done:
+                b = v * 1024.0 * scale
+                per[name].append(b)
+                total += b
+        out[tag] = {"scale": scale, "frame": total / args.steps,
+                    "kernels": {k: (sum(v) / len(v), len(v)) for k, v in per.items()}}
+    f, w = out["fetch"], out["write"]
+    names = sorted(set(f["kernels"]) | set(w["kernels"]), key=lambda k: -(f["kernels"].get(k, (0, 0))[0] * f["kernels"].get(k, (0, 0))[1]))
+    print(f"calibration: FETCH_SIZE x{f['scale']:.4f}, WRITE_SIZE x{w['scale']:.4f} (256 MiB device copy)")
+    print("| kernel | launches | read B / launch | written B / launch |")
+    print("|---|---|---|---|")
+    for k in names:
+        rb, n = f["kernels"].get(k, (0.0, 0))
+        wb, _ = w["kernels"].get(k, (0.0, 0))
+        print(f"| {k} | {n} | {rb:,.0f} | {wb:,.0f} |")
+    key = "learned" if args.learned else "plain"
+    ks = {k: f["kernels"].get(k, (0.0, 0))[0] + w["kernels"].get(k, (0.0, 0))[0] for k in ("k_sdf", "k_mc")}
+    print(f"per frame: read {f['frame']:,.0f} B, written {w['frame']:,.0f} B")
+    print(f"bench-traffic: {key} frame={f['frame'] + w['frame']:.0f} k_sdf={ks['k_sdf']:.0f} k_mc={ks['k_mc']:.0f}")
+
+
+def read_traffic(path, key):
+    """HBM traffic figures of a committed counter profile: the line
+    `bench-traffic: <key> frame=B k_sdf=B k_mc=B` of the summary (bytes per
+    frame and per launch, from calibrated FETCH_SIZE + WRITE_SIZE)."""
+    try:
+        for line in open(path):
+            line = line.strip().strip("`")
+            if line.startswith("bench-traffic:"):
+                parts = line.split()
+                if len(parts) > 1 and parts[1] == key:
+                    return {k: float(v) for k, v in (x.split("=") for x in parts[2:])}
+    except OSError:
+        pass
+    return {}
+
+
+def fuse_walk(fz, d_depth, d_bgr, R, t, K, cfg, frames):
+    """addMap + updateMeshes over the given frames (device-resident images);
+    returns the summed queued bricks."""
+    w, h = cfg["w"], cfg["h"]
+    fb, cb = w * h * 2, w * h * 3
+    nq = 0
+    for i in frames:
+        nq += fz.addMapDevice(d_depth.addr(i * fb), d_bgr.addr(i * cb), w, h, R[i], t[i], K, 1 / 5000.0,
+                              cfg["maxcam"])
+        fz.updateMeshes()
+    return nq
+
+
+def kernel_entry(name, ms, calls, bytes_per_launch, traffic):
+    avg = ms / max(calls, 1)
+    ach = bytes_per_launch / (avg * 1e-3) / 1e9 if avg > 0 else 0.0
+    return {"kernel": name, "bound": "hbm", "achieved": round(ach, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic, "avg_launch_us": round(avg * 1e3, 3),
+            "timed_launches": int(calls), "bytes_per_launch": round(bytes_per_launch)}
+
+
+def run_walk(args):
+    """The 640x480, 5 mm walk.  Frames before the warm-up and timed ones are
+    fused untimed first, so that the timed frames are always the walk's last
+    `steps` frames (its steady state) whatever --steps is.  The algorithmic
+    bytes of the timed frames come from a replay of the same frames on a
+    second map with every phase evented (the map is deterministic: the
+    replay's counts equal the timed run's, which is checked)."""
     from fastfusion_amd import FusionMipMapGPU, hiprt, synth
 
-    n_frames = args.warmup + args.steps
-    sc, R, t, cfg = synth.config_c2(n_frames=max(n_frames, 1))
+    n_walk = max(1000, args.warmup + args.steps)
+    prefuse = n_walk - args.warmup - args.steps
+    sc, R, t, cfg = synth.config_c2(n_frames=n_walk)
     t_r = time.perf_counter()
-    depth, bgr = render_frames(sc, R, t, n_frames, cfg)
-    print(f"[bench] rendered {n_frames} frames in {time.perf_counter() - t_r:.1f} s", file=sys.stderr, flush=True)
-
+    depth, bgr = render_frames(sc, R, t, n_walk, cfg)
+    print(f"[bench] rendered {n_walk} frames in {time.perf_counter() - t_r:.1f} s", file=sys.stderr, flush=True)
     w, h = cfg["w"], cfg["h"]
+    K = synth.K_DEFAULT
     d_depth = hiprt.DeviceBuffer.from_array(depth)
     d_bgr = hiprt.DeviceBuffer.from_array(bgr)
-    fz = FusionMipMapGPU(0, 0, 0, cfg["scale"], cfg["threshold"], 0, True, max_leaves=1 << 21,
-                         max_branches=1 << 25)
-    if args.graphs:
-        fz.setGraphCapture(True)
-    if args.learned:
-        from fastfusion_amd import learned
-        fz.setLearnedWeighting(learned.load_weights())
-    # warm-up frames carry events around every phase (the phase breakdown);
-    # the timed frames only around k_sdf on one frame in 8 (the roofline),
-    # recorded by its dispatch, since every event costs launch time
-    fz.set_timing(True)
-    K = synth.K_DEFAULT
-    fb, cb = w * h * 2, w * h * 3
 
-    def step(i):
-        nq = fz.addMapDevice(d_depth.addr(i * fb), d_bgr.addr(i * cb), w, h, R[i], t[i], K, 1 / 5000.0,
-                             cfg["maxcam"])
-        fz.updateMeshes()
-        return nq
+    def make():
+        fz = FusionMipMapGPU(0, 0, 0, cfg["scale"], cfg["threshold"], 0, True, max_leaves=1 << 21,
+                             max_branches=1 << 25)
+        if args.graphs:
+            fz.setGraphCapture(True)
+        if args.learned:
+            from fastfusion_amd import learned
+            fz.setLearnedWeighting(learned.load_weights())
+        return fz
 
-    for i in range(args.warmup):
-        step(i)
+    untimed = range(0, prefuse + args.warmup)
+    timed = range(prefuse + args.warmup, n_walk)
+    # the timed run: k_sdf's and k_mc's dispatches record events on one
+    # frame in 8 (each event costs launch time)
+    fz = make()
+    fuse_walk(fz, d_depth, d_bgr, R, t, K, cfg, untimed)
     hiprt.synchronize()
-    tm_warm = fz.timing()
-    fz.reset_timing()
     fz.set_timing(True, sdf_only=True, every=8)
-    nq_sum = 0
     t0 = time.perf_counter()
-    for i in range(args.warmup, n_frames):
-        nq_sum += step(i)
+    nq_sum = fuse_walk(fz, d_depth, d_bgr, R, t, K, cfg, timed)
     hiprt.synchronize()
     dt = time.perf_counter() - t0
     tm = fz.timing()
+    mc = fz.mc_timing()
     counts = fz.counts()
-    fps = args.steps / dt
-    ms_step = dt * 1e3 / args.steps
-
-    sdf = tm["sdf"]
-    sdf_ms = sdf["ms"] / max(sdf["calls"], 1)
-    sdf_bytes = sdf["bytes"] / max(sdf["calls"], 1)
-    achieved = sdf_bytes / (sdf_ms * 1e-3) / 1e9 if sdf_ms > 0 else 0.0
-    nw = max(args.warmup, 1)
-    warm_ms = sum(v["ms"] for v in tm_warm.values()) / nw
-    warm_bytes = sum(v["bytes"] for v in tm_warm.values()) / nw
-    print(f"[bench] timed {json.dumps(tm)} warm-up {json.dumps(tm_warm)}", file=sys.stderr, flush=True)
-
-    # surface error of the final mesh against the analytic scene (GT)
+    print(f"[bench] timed {args.steps} frames in {dt:.4f} s", file=sys.stderr, flush=True)
+    # surface error of the final mesh against the analytic scene (GT); its
+    # gather kernel also ends the timed run in a counter profile
     mesh = fz.getMeshInterleavedMarchingCubes()
     err = sc.sdf(mesh.vertices) if len(mesh.vertices) else np.zeros(1, np.float32)
     rms = float(np.sqrt(np.mean(err.astype(np.float64) ** 2)))
 
+    # the replay: every phase evented over the same timed frames
+    fr = make()
+    fuse_walk(fr, d_depth, d_bgr, R, t, K, cfg, untimed)
+    hiprt.synchronize()
+    fr.set_timing(True)
+    nq_replay = fuse_walk(fr, d_depth, d_bgr, R, t, K, cfg, timed)
+    hiprt.synchronize()
+    rp = fr.timing()
+    rmc = fr.mc_timing()
+    rc = fr.counts()
+    same = nq_replay == nq_sum and all(rc[k] == counts[k] for k in ("n_leaves_used", "n_branches_used", "grows"))
+    if not same:
+        raise SystemExit("[bench] the replay's map differs from the timed run's")
+    del fr
+
+    # whole frame (addMap + updateMeshes): algorithmic bytes of the timed frames
+    B = sum(v["bytes"] for v in rp.values())
+    achieved = B / dt / 1e9
+    tr = read_traffic(os.path.join(ROOT, "profiles", "hbm_traffic_640x480_5mm.md"),
+                      "learned" if args.learned else "plain")
+    sdf = tm["sdf"]
+    sdf_bpl = sdf["bytes"] / max(sdf["calls"], 1)
+    mc_bpl = rmc["bytes"] / max(rmc["byte_calls"], 1)
+    kernels = {
+        "k_sdf": kernel_entry("k_sdf", sdf["ms"], sdf["calls"], sdf_bpl, tr.get("k_sdf")),
+        "k_mc": kernel_entry("k_mc", mc["ms"], mc["calls"], mc_bpl, tr.get("k_mc")),
+    }
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(depth, bgr, R, t, cfg, args.cpu_seconds)
-    traffic = (SDF_TRAFFIC_DEFAULT if (args.steps, args.warmup) == (950, 50) and not args.learned else None)
+    nst = max(args.steps, 1)
     line = {
         "metric": METRIC,
-        "value": round(fps, 2),
+        "value": round(args.steps / dt, 2),
         "unit": "frames/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms_step, 4),
+        "ms_per_step": round(dt * 1e3 / nst, 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic: seeded analytic room ray-cast on the host, u16 depth + BGR8, resident in HBM",
         "config": {"workload": ("learned confidence" if args.learned else "plain method") +
-                               ", 640x480, 5 mm voxels, 15 mm truncation, 4 m max depth, "
-                               "1000-frame seeded random walk (room seed 11, walk seed 12)",
-                   "frames": n_frames, "resolution": [w, h], "voxel_m": cfg["scale"],
-                   "truncation_m": cfg["threshold"], "maxcam_m": cfg["maxcam"]},
-        "hbm_gbs_algorithmic_warmup": round(warm_bytes / (warm_ms * 1e-3) / 1e9, 2) if warm_ms > 0 else None,
-        "bricks_per_frame": round(nq_sum / args.steps, 1),
+                               f", 640x480, 5 mm voxels, 15 mm truncation, 4 m max depth, {n_walk}-frame seeded "
+                               "random walk (room seed 11, walk seed 12); timed: its last frames",
+                   "frames": n_walk, "untimed_before_warmup": prefuse, "timed_frames": [timed.start, timed.stop],
+                   "resolution": [w, h], "voxel_m": cfg["scale"], "truncation_m": cfg["threshold"],
+                   "maxcam_m": cfg["maxcam"]},
+        "roofline": {"kernel": "whole frame (addMap + updateMeshes, every kernel)", "bound": "hbm",
+                     "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": tr.get("frame"),
+                     "algorithmic_bytes_per_frame": round(B / nst),
+                     "bytes_split_per_frame": {"img_alloc": round(rp["alloc"]["bytes"] / nst),
+                                               "sdf": round(rp["sdf"]["bytes"] / nst),
+                                               "mesh": round(rp["mesh"]["bytes"] / nst)},
+                     "kernels": kernels},
+        "bricks_per_frame": round(nq_sum / nst, 1),
+        "processed_px_per_frame": round(rp["alloc"]["units"] / nst, 1),
+        "remeshed_cells_per_frame": round(rp["mesh"]["units"] / nst, 1),
         "map": {"bricks": counts["n_leaves_used"], "side": counts["side"]},
-        "roofline": {"kernel": "k_sdf", "bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "avg_launch_us": round(sdf_ms * 1e3, 3), "bytes_per_launch": round(sdf_bytes)},
-        "phases_ms_per_frame_warmup": {k: round(v["ms"] / nw, 4) for k, v in tm_warm.items()},
+        "phases_ms_per_frame_replay": {k: round(v["ms"] / max(v["calls"], 1), 4) for k, v in rp.items()},
         "rms_surface_err_m": round(rms, 6),
         "rms_surface_err_voxels": round(rms / cfg["scale"], 4),
         "mesh": {"vertices": int(len(mesh.vertices)), "triangles": int(len(mesh.faces))},

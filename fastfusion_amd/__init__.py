@@ -82,6 +82,7 @@ ABI = {
     "ff_set_timing_every": (ctypes.c_int, [_P, ctypes.c_uint32]),
     "ff_get_timing": (ctypes.c_int, [_P, _P, _P, _P, _P]),
     "ff_reset_timing": (ctypes.c_int, [_P]),
+    "ff_get_mc_timing": (ctypes.c_int, [_P, _P, _P, _P, _P, _P]),
     "ff_tum_read": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, _P, _SZ]),
     "ff_png_size": (ctypes.c_int, [ctypes.c_char_p, _P, _P, _P, _P]),
     "ff_png_read_depth": (ctypes.c_int, [ctypes.c_char_p, _P, _SZ]),
@@ -420,6 +421,17 @@ class FusionMipMapGPU:
         names = ("alloc", "sdf", "mesh")
         return {n: {"ms": float(ms[i]), "calls": int(calls[i]), "bytes": float(by[i]), "units": int(un[i])}
                 for i, n in enumerate(names)}
+
+    def mc_timing(self):
+        """The marching-cubes kernel alone: summed ms and timed launches
+        (events its dispatch records), and with all phases timed the summed
+        algorithmic bytes over `byte_calls` updates and the re-meshed cells."""
+        ms, by = ctypes.c_double(), ctypes.c_double()
+        calls, bcalls, cells = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64()
+        self._check(self._lib.ff_get_mc_timing(self._ctx, ctypes.byref(ms), ctypes.byref(calls), ctypes.byref(by),
+                                               ctypes.byref(bcalls), ctypes.byref(cells)))
+        return {"ms": ms.value, "calls": calls.value, "bytes": by.value, "byte_calls": bcalls.value,
+                "cells": cells.value}
 
     # -- learned confidence (no counterpart in the reference)
     def getDescription(self):

@@ -1697,7 +1697,7 @@ void ff_destroy(ff_ctx *c)
     hipFree(c->d_pxp); hipFree(c->d_pyp);
     drop_frame_graphs(c);
     hipFree(c->hv.keys); hipFree(c->hv.minpix); hipFree(c->hv.aux); hipFree(c->hv.assigned); hipFree(c->hv.list);
-    for (int i = 0; i < 8; i++)
+    for (int i = 0; i < 10; i++)
         if (c->ev[i]) hipEventDestroy(c->ev[i]);
     if (c->side) {
         hipStreamSynchronize(c->side);
@@ -2241,7 +2241,7 @@ int ff_set_timing(ff_ctx *c, int on)
     const int rc = mesh_complete(c);
     if (rc != FF_OK) return rc;
     if (on && !c->ev[0])
-        for (int i = 0; i < 8; i++) HIPCHK(hipEventCreate(&c->ev[i]));
+        for (int i = 0; i < 10; i++) HIPCHK(hipEventCreate(&c->ev[i]));
     c->timing = on != 0;
     c->timingAll = on == 1;
     return FF_OK;
@@ -2280,6 +2280,21 @@ int ff_reset_timing(ff_ctx *c)
         c->phaseBytes[i] = 0.0;
         c->phaseUnits[i] = 0;
     }
+    c->mcMs = c->mcBytes = 0.0;
+    c->mcCalls = c->mcByteCalls = c->mcUnits = 0;
+    return FF_OK;
+}
+
+int ff_get_mc_timing(ff_ctx *c, double *ms, uint64_t *calls, double *bytes, uint64_t *byte_calls, uint64_t *cells)
+{
+    if (!c) return FF_E_ARG;
+    const int rc = mesh_complete(c);
+    if (rc != FF_OK) return rc;
+    if (ms) *ms = c->mcMs;
+    if (calls) *calls = c->mcCalls;
+    if (bytes) *bytes = c->mcBytes;
+    if (byte_calls) *byte_calls = c->mcByteCalls;
+    if (cells) *cells = c->mcUnits;
     return FF_OK;
 }
 

@@ -192,9 +192,15 @@ struct ff_ctx {
     // per-phase timing (0 allocation, 1 SDF update, 2 meshing)
     bool timing = false;     // events around the SDF update
     bool timingAll = false;  // and around the allocation and meshing phases
-    hipEvent_t ev[8] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+    hipEvent_t ev[10] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
     double phaseMs[3] = {0, 0, 0}, phaseBytes[3] = {0, 0, 0};
     uint64_t phaseCalls[3] = {0, 0, 0}, phaseUnits[3] = {0, 0, 0};
+    // the marching-cubes kernel alone: its dispatch records ev[8], ev[9] on
+    // one update in timeEvery (durations); with all phases timed, its
+    // algorithmic bytes and re-meshed cells on every update
+    uint64_t updates = 0;
+    double mcMs = 0, mcBytes = 0;
+    uint64_t mcCalls = 0, mcByteCalls = 0, mcUnits = 0;
 
     ff::MapView mapView() const
     {

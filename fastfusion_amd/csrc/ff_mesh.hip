@@ -26,6 +26,7 @@
 #include "ff_internal.hpp"
 #include "mc_tables.h"
 
+#include <hip/hip_ext.h>
 #include <algorithm>
 #include <cstring>
 #include <vector>
@@ -186,6 +187,7 @@ struct MeshState {
     // the current table in place instead of building a new one
     uint32_t builtNB = 0, builtNL = 0, builtGrows = 0;
     bool reuse = false;  // the update in flight re-marks the current table
+    bool mcTimed = false;  // the update in flight records k_mc's events (ev[8], ev[9])
     // an update whose device work is enqueued and whose host part
     // (mesh_finish) runs at the next mesh call, query or state change
     bool pending = false, pendingTimed = false;
@@ -2016,7 +2018,15 @@ McStream mesh_stream(ff_ctx *c)
 void mesh_launch_mc(ff_ctx *c)
 {
     MeshState *s = c->mesh;
-    k_mc<<<2048, 256, 0, c->stream>>>(s->d_cells[s->nxt], s->d_geom, s->d_mc, s->d_sbuf, s->m, pools(s), s->cellCap);
+    // (timed on one update in timeEvery, by events its own dispatch records)
+    s->mcTimed = c->timing && c->ev[8] && (c->updates++ % c->timeEvery) == 0;
+    if (s->mcTimed)
+        hipExtLaunchKernelGGL(k_mc, dim3(2048), dim3(256), 0, c->stream, c->ev[8], c->ev[9], 0,
+                              (CellRec *)s->d_cells[s->nxt], (const CellGeom *)s->d_geom, s->d_mc,
+                              (const uint4 *)s->d_sbuf, s->m, pools(s), s->cellCap);
+    else
+        k_mc<<<2048, 256, 0, c->stream>>>(s->d_cells[s->nxt], s->d_geom, s->d_mc, s->d_sbuf, s->m, pools(s),
+                                          s->cellCap);
     // the bricks are read: the next frame's SDF update may start
     if (c->evMc) {
         if (hipEventRecord(c->evMc, c->stream) == hipSuccess) {
@@ -2176,6 +2186,14 @@ int mesh_finish(ff_ctx *c, bool timed)
         s->builtGrows = s->launchGrows;
     }
     c->phaseUnits[2] += h.nDirty;
+    if (s->mcTimed) {
+        float ms = 0.0f;
+        if (hipEventElapsedTime(&ms, c->ev[8], c->ev[9]) == hipSuccess) {
+            c->mcMs += ms;
+            c->mcCalls++;
+        }
+        s->mcTimed = false;
+    }
     // algorithmic bytes: the 512 source voxels (14 B) of each re-meshed cell,
     // 15 B per vertex and 12 B per triangle written (vertex and face counts
     // from the arena totals, which only grow between compactions)
@@ -2183,6 +2201,11 @@ int mesh_finish(ff_ctx *c, bool timed)
         const uint64_t va = h.pad[0], fa = h.pad[1];
         const uint64_t dv = va >= s->lastVAlloc ? va - s->lastVAlloc : va, df = fa >= s->lastFAlloc ? fa - s->lastFAlloc : fa;
         c->phaseBytes[2] += (double)h.nDirty * 512.0 * 14.0 + 15.0 * (double)dv + 12.0 * (double)df;
+        // k_mc alone: the re-meshed cells' samples it consumes (one 14 B
+        // voxel record per sample, 64 per tile) and the geometry it writes
+        c->mcBytes += 14.0 * 64.0 * (double)h.nTiles + 15.0 * (double)dv + 12.0 * (double)df;
+        c->mcByteCalls++;
+        c->mcUnits += h.nDirty;
         s->lastVAlloc = va;
         s->lastFAlloc = fa;
     }

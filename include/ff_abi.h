@@ -171,6 +171,15 @@ int ff_set_timing(ff_ctx *ctx, int on);
  * recorded by the kernel's own dispatch, which costs launch time. */
 int ff_set_timing_every(ff_ctx *ctx, uint32_t n);
 int ff_get_timing(ff_ctx *ctx, double *ms, uint64_t *calls, double *bytes, uint64_t *units);
+/* The marching-cubes kernel alone (no counterpart in the reference): with
+ * timing on, its dispatch records events on one mesh update in n (the
+ * ff_set_timing_every count): summed milliseconds and timed launches; with
+ * all phases timed (ff_set_timing(ctx, 1)) also, on every update, its
+ * algorithmic bytes (14 B per consumed cell sample, 15 B per vertex and 12 B
+ * per triangle written), the number of updates counted and of re-meshed
+ * cells. */
+int ff_get_mc_timing(ff_ctx *ctx, double *ms, uint64_t *calls, double *bytes, uint64_t *byte_calls,
+                     uint64_t *cells);
 int ff_reset_timing(ff_ctx *ctx);
 
 /* Device stream the context launches on (a hipStream_t), for callers that
