@@ -62,9 +62,12 @@ ABI = {
     "ff_add_map_device_noalloc": (ctypes.c_int, [_P, _P, _P, ctypes.c_int, ctypes.c_int, _DP, _DP, _DP,
                                                  ctypes.c_float, ctypes.c_float]),
     "ff_update_meshes": (ctypes.c_int, [_P]),
+    "ff_mesh_full": (ctypes.c_int, [_P]),
     "ff_mesh_sizes": (ctypes.c_int, [_P, ctypes.POINTER(_SZ), ctypes.POINTER(_SZ)]),
     "ff_copy_mesh": (ctypes.c_int, [_P, _P, _P, _P]),
     "ff_cell_count": (ctypes.c_int, [_P, ctypes.POINTER(_SZ)]),
+    "ff_cell_update_sizes": (ctypes.c_int, [_P, ctypes.POINTER(_SZ), ctypes.POINTER(_SZ), ctypes.POINTER(_SZ)]),
+    "ff_copy_cell_update": (ctypes.c_int, [_P, _P, _P, _P, _P, _P, _P]),
     "ff_copy_cells": (ctypes.c_int, [_P, _P, _P, _P]),
     "ff_set_incremental_meshing": (ctypes.c_int, [_P, ctypes.c_int]),
     "ff_set_mesh_sample_tiles": (ctypes.c_int, [_P, ctypes.c_uint32]),
@@ -311,6 +314,11 @@ class FusionMipMapGPU:
     def updateMeshes(self):
         return self._check(self._lib.ff_update_meshes(self._ctx))
 
+    def meshFull(self):
+        """Every cell meshed from scratch now (getMeshRecursive's counterpart,
+        geometryfusion_mipmap_cpu.cpp:3658-3674); returns when complete."""
+        return self._check(self._lib.ff_mesh_full(self._ctx))
+
     def setIncrementalMeshing(self, on=True):
         self._check(self._lib.ff_set_incremental_meshing(self._ctx, int(bool(on))))
 
@@ -351,6 +359,22 @@ class FusionMipMapGPU:
         nf = np.zeros(n.value, np.uint32)
         self._check(self._lib.ff_copy_cells(self._ctx, _ptr(d), _ptr(nv), _ptr(nf)))
         return d, nv, nf
+
+    def getMeshCellsUpdate(self):
+        """getMeshCellsUpdate (geometryfusion_mipmap_cpu.cpp:2989-3047): the
+        cells re-meshed by the last updateMeshes, each with its own mesh."""
+        nc, nv, nf = _SZ(), _SZ(), _SZ()
+        self._check(self._lib.ff_cell_update_sizes(self._ctx, ctypes.byref(nc), ctypes.byref(nv), ctypes.byref(nf)))
+        u = CellUpdate()
+        u.desc = np.zeros((nc.value, 5), np.int32)
+        u.n_vertices = np.zeros(nc.value, np.uint32)
+        u.n_faces = np.zeros(nc.value, np.uint32)
+        u.vertices = np.zeros((nv.value, 3), np.float32)
+        u.colors = np.zeros((nv.value, 3), np.uint8)
+        u.faces = np.zeros((nf.value, 3), np.uint32)
+        self._check(self._lib.ff_copy_cell_update(self._ctx, _ptr(u.desc), _ptr(u.n_vertices), _ptr(u.n_faces),
+                                                  _ptr(u.vertices), _ptr(u.colors), _ptr(u.faces)))
+        return u
 
     # -- state export
     def counts(self):
@@ -556,6 +580,21 @@ class Trainer:
         out = np.zeros(MLP_NPARAMS, np.float32)
         self._check(self._lib.ff_trainer_params(self._h, _ptr(out), out.size))
         return out
+
+
+class CellUpdate:
+    """CellUpdate (geometryfusion_mipmap_cpu.hpp:40-59): per re-meshed cell its
+    descriptor {type, ox, oy, oz, size} and its own mesh.  Arrays hold the
+    cells' meshes concatenated; `cells()` yields (desc, vertices, colors,
+    indices) per cell with indices local to the cell, like
+    InterleavedMeshCell."""
+
+    def cells(self):
+        v0 = np.concatenate([[0], np.cumsum(self.n_vertices)]).astype(np.int64)
+        f0 = np.concatenate([[0], np.cumsum(self.n_faces)]).astype(np.int64)
+        for i in range(len(self.desc)):
+            yield (self.desc[i], self.vertices[v0[i]:v0[i + 1]], self.colors[v0[i]:v0[i + 1]],
+                   self.faces[f0[i]:f0[i + 1]])
 
 
 class Batch:

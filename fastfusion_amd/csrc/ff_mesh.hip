@@ -120,6 +120,7 @@ struct MeshBatchStream {
     int havePrev;
     const uint32_t *stamp, *parent;
     uint32_t since;
+    uint32_t updateId;  // recorded in the cells this update re-meshes
     struct CellGeom *geom;
     MapRO m;
     struct TileRec *tiles;
@@ -188,6 +189,10 @@ struct MeshState {
     uint32_t builtNB = 0, builtNL = 0, builtGrows = 0;
     bool reuse = false;  // the update in flight re-marks the current table
     bool mcTimed = false;  // the update in flight records k_mc's events (ev[8], ev[9])
+    // updates numbered from 1: the cells an update re-meshes carry its
+    // number (CellRec::pad[0]); lastUpdateId is the last completed one
+    uint32_t updateSeq = 0, launchId = 0, lastUpdateId = 0;
+    bool fullNext = false;  // the next update meshes every cell from scratch (ff_mesh_full)
     // an update whose device work is enqueued and whose host part
     // (mesh_finish) runs at the next mesh call, query or state change
     bool pending = false, pendingTimed = false;
@@ -954,7 +959,7 @@ __device__ __forceinline__ void mark_body(CellRec *cells, const CellRec *__restr
                                           uint32_t hmask, bool havePrev, const uint32_t *__restrict__ stamp,
                                           const uint32_t *__restrict__ parent, uint32_t since, MeshCounters *mc,
                                           CellGeom *geom, uint32_t cap, const MapRO &m, TileRec *tiles,
-                                          uint32_t tileCap, bool split)
+                                          uint32_t tileCap, bool split, uint32_t updateId)
 {
     const uint32_t nc = min(mc->nCells, cap);
     const uint32_t lane = threadIdx.x & 63u;
@@ -988,10 +993,12 @@ __device__ __forceinline__ void mark_body(CellRec *cells, const CellRec *__restr
                 if (clean && chains_dirty(stamp, parent, cl, since)) clean = false;
                 if (clean) {
                     c.v0 = p.v0; c.nv = p.nv; c.f0 = p.f0; c.nf = p.nf;
+                    c.pad[0] = p.pad[0];
                 }
             }
             isDirty = !clean;
             if (isDirty) {
+                c.pad[0] = updateId;  // the update that meshed it (CellUpdate)
                 for (int k = 0; k < 8; k++) ll[k] = c.ll[k];
                 cell_geom(m, c.key, ll, g);
                 g.cell = i;
@@ -1021,16 +1028,18 @@ __device__ __forceinline__ void mark_body(CellRec *cells, const CellRec *__restr
 __global__ void k_mark(CellRec *cells, const CellRec *__restrict__ prev, const uint64_t *__restrict__ hkeys,
                        const uint32_t *__restrict__ hval, uint32_t hmask, bool havePrev,
                        const uint32_t *__restrict__ stamp, const uint32_t *__restrict__ parent, uint32_t since,
-                       MeshCounters *mc, CellGeom *geom, uint32_t cap, MapRO m, TileRec *tiles, uint32_t tileCap)
+                       MeshCounters *mc, CellGeom *geom, uint32_t cap, MapRO m, TileRec *tiles, uint32_t tileCap,
+                       uint32_t updateId)
 {
-    mark_body(cells, prev, hkeys, hval, hmask, havePrev, stamp, parent, since, mc, geom, cap, m, tiles, tileCap, true);
+    mark_body(cells, prev, hkeys, hval, hmask, havePrev, stamp, parent, since, mc, geom, cap, m, tiles, tileCap, true,
+              updateId);
 }
 
 __global__ void k_mark_b(const MeshBatchStream *__restrict__ S)
 {
     const MeshBatchStream &a = S[blockIdx.z];
     mark_body(a.cells, a.prev, a.hkeys, a.hval, a.hmask, a.havePrev != 0, a.stamp, a.parent, a.since, a.mc, a.geom,
-              a.cellCap, a.m, a.tiles, a.tileCap, false);
+              a.cellCap, a.m, a.tiles, a.tileCap, false, a.updateId);
 }
 
 // The table of the previous update re-marked in place, when the tree has not
@@ -1041,7 +1050,7 @@ __global__ void k_mark_b(const MeshBatchStream *__restrict__ S)
 // it unless the new mesh is empty.
 __global__ void k_mark_same(CellRec *cells, const uint32_t *__restrict__ stamp, const uint32_t *__restrict__ parent,
                             uint32_t since, MeshCounters *mc, uint32_t nCells, MeshCounters *clearNext, CellGeom *geom,
-                            uint32_t cap, MapRO m, TileRec *tiles, uint32_t tileCap)
+                            uint32_t cap, MapRO m, TileRec *tiles, uint32_t tileCap, uint32_t updateId)
 {
     const bool split = true;  // long cells first (dirty_slot)
     // the counters start at zero; block 0 records the table's size and
@@ -1065,6 +1074,7 @@ __global__ void k_mark_same(CellRec *cells, const uint32_t *__restrict__ stamp, 
             isDirty = chains_dirty(stamp, parent, ll, since);
             if (isDirty) {
                 c.v0 = c.nv = c.f0 = c.nf = 0u;
+                c.pad[0] = updateId;
                 cell_geom(m, c.key, ll, g);
                 g.cell = i;
                 g.pad = 0;
@@ -1629,14 +1639,15 @@ __global__ void k_compact(CellRec *cells, MeshCounters *mc, uint32_t cap, const 
 // fo[k] with indices made global
 __global__ void k_gather(const uint4 *__restrict__ segs, const uint2 *__restrict__ outOff, uint32_t n,
                          const float4 *__restrict__ vsrc, const uint32_t *__restrict__ fsrc, float4 *__restrict__ vdst,
-                         uint32_t *__restrict__ fdst)
+                         uint32_t *__restrict__ fdst, uint32_t localFaces)
 {
     const int lane = threadIdx.x & 63;
     for (uint32_t c = (blockIdx.x * blockDim.x + threadIdx.x) >> 6; c < n; c += (gridDim.x * blockDim.x) >> 6) {
         const uint4 sg = segs[c];  // v0, nv, f0, nf
         const uint2 oo = outOff[c];
         for (uint32_t i = lane; i < sg.y; i += 64) vdst[oo.x + i] = vsrc[sg.x + i];
-        for (uint32_t i = lane; i < 3 * sg.w; i += 64) fdst[3 * (size_t)oo.y + i] = fsrc[3 * (size_t)sg.z + i] + oo.x;
+        const uint32_t fo = localFaces ? 0u : oo.x;  // face indices local to the cell, or into the whole mesh
+        for (uint32_t i = lane; i < 3 * sg.w; i += 64) fdst[3 * (size_t)oo.y + i] = fsrc[3 * (size_t)sg.z + i] + fo;
     }
 }
 
@@ -1904,7 +1915,7 @@ static bool mesh_pre_host(ff_ctx *c, bool allowReuse)
 {
     MeshState *s = c->mesh;
     if (!s->haveVolume || !c->n) return false;
-    s->reuse = allowReuse && s->nCells[s->cur] > 0 && s->builtNB == c->nBranchesUsed &&
+    s->reuse = allowReuse && !s->fullNext && s->nCells[s->cur] > 0 && s->builtNB == c->nBranchesUsed &&
                s->builtNL == c->nLeavesUsed && s->builtGrows == c->grows;
     // this update's counter set (zero when the previous single update's
     // first kernel cleared it); a batched update counts in it too and
@@ -1913,6 +1924,7 @@ static bool mesh_pre_host(ff_ctx *c, bool allowReuse)
     s->mcZeroNow = s->mcNextZero;
     s->mcNextZero = false;
     s->launchStamp = c->stampCounter;
+    s->launchId = ++s->updateSeq;
     s->launchNB = c->nBranchesUsed;
     s->launchNL = c->nLeavesUsed;
     s->launchGrows = c->grows;
@@ -1938,7 +1950,7 @@ bool mesh_launch_pre(ff_ctx *c, bool timed)
         if (!s->mcZeroNow) k_mesh_begin<<<1, 64, 0, c->stream>>>(s->d_mc);
         k_mark_same<<<std::max(1u, std::min(4096u, (nc + 255) / 256)), 256, 0, c->stream>>>(
             s->d_cells[nxt], c->d_stamp, c->d_parent, s->lastMeshStamp, s->d_mc, nc, mc_other(s), s->d_geom,
-            s->cellCap, m, s->d_tiles, s->tileCap);
+            s->cellCap, m, s->d_tiles, s->tileCap, s->launchId);
         k_sample<<<4096, 256, 0, c->stream>>>(s->d_geom, s->d_cells[nxt], s->d_tiles, s->d_mc, s->tileCap, m,
                                               s->d_sbuf);
         mc_flip(s);
@@ -1954,8 +1966,9 @@ bool mesh_launch_pre(ff_ctx *c, bool timed)
                                          s->d_cells[nxt], s->cellCap, s->d_jump);
     const MapRO m = s->m;
     k_mark<<<1024, 256, 0, c->stream>>>(s->d_cells[nxt], s->d_cells[s->cur], s->d_hkeys, s->d_hval, s->hmask,
-                                        s->nCells[s->cur] > 0, c->d_stamp, c->d_parent, s->lastMeshStamp, s->d_mc,
-                                        s->d_geom, s->cellCap, m, s->d_tiles, s->tileCap);
+                                        s->nCells[s->cur] > 0 && !s->fullNext, c->d_stamp, c->d_parent,
+                                        s->lastMeshStamp, s->d_mc,
+                                        s->d_geom, s->cellCap, m, s->d_tiles, s->tileCap, s->launchId);
     k_sample<<<4096, 256, 0, c->stream>>>(s->d_geom, s->d_cells[nxt], s->d_tiles, s->d_mc, s->tileCap, m, s->d_sbuf);
     mc_flip(s);
         return true;
@@ -1981,10 +1994,11 @@ static MeshBatchStream mesh_batch_stream(ff_ctx *c)
     a.hkeys = s->d_hkeys;
     a.hval = s->d_hval;
     a.hmask = s->hmask;
-    a.havePrev = s->nCells[s->cur] > 0;
+    a.havePrev = s->nCells[s->cur] > 0 && !s->fullNext;
     a.stamp = c->d_stamp;
     a.parent = c->d_parent;
     a.since = s->lastMeshStamp;
+    a.updateId = s->launchId;
     a.geom = s->d_geom;
     a.m = s->m;
     a.tiles = s->d_tiles;
@@ -2179,6 +2193,7 @@ int mesh_finish(ff_ctx *c, bool timed)
     }
     s->cur = nxt;
     s->nCells[nxt] = std::min(h.nCells, s->cellCap);
+    s->lastUpdateId = s->launchId;
     s->lastMeshStamp = s->launchStamp;
     if (!s->reuse) {
         s->builtNB = s->launchNB;
@@ -2273,41 +2288,14 @@ static int host_cells(ff_ctx *c, std::vector<CellRec> &out)
     return FF_OK;
 }
 
-}  // namespace ff
-
-using namespace ff;
-
-extern "C" {
-
-int ff_update_meshes(ff_ctx *c)
+// the meshes of the given cells (in that order) into host buffers: vertices
+// (xyz, rgb) concatenated, faces indexing the concatenation or (local) each
+// cell's own vertices
+static int gather_cells(ff_ctx *c, const std::vector<CellRec> &cells, bool local, float *xyz, uint8_t *rgb,
+                        uint32_t *faces)
 {
-    if (!c) return FF_E_ARG;
-    hipSetDevice(c->device);
-    if (!c->incremental) return 0;
-    return update_meshes(c);
-}
-
-int ff_mesh_sizes(ff_ctx *c, size_t *nv, size_t *nf)
-{
-    if (!c) return FF_E_ARG;
-    std::vector<CellRec> cells;
-    int rc = host_cells(c, cells);
-    if (rc) return rc;
-    size_t v = 0, f = 0;
-    for (const auto &r : cells) { v += r.nv; f += r.nf; }
-    if (nv) *nv = v;
-    if (nf) *nf = f;
-    return FF_OK;
-}
-
-int ff_copy_mesh(ff_ctx *c, float *xyz, uint8_t *rgb, uint32_t *faces)
-{
-    if (!c) return FF_E_ARG;
     MeshState *s = c->mesh;
-    std::vector<CellRec> cells;
-    int rc = host_cells(c, cells);
-    if (rc) return rc;
-    // device gather of the cells' segments (in key order) into dense buffers
+    // device gather of the cells' segments into dense buffers
     std::vector<uint4> segs(cells.size());
     std::vector<uint2> offs(cells.size());
     size_t vt = 0, ft = 0;
@@ -2319,7 +2307,7 @@ int ff_copy_mesh(ff_ctx *c, float *xyz, uint8_t *rgb, uint32_t *faces)
         ft += r.nf;
     }
     if (vt >= 0xffffffffull || ft >= 0xffffffffull) {
-        c->err = "ff_copy_mesh: mesh larger than 2^32 vertices";
+        c->err = "mesh larger than 2^32 vertices";
         return FF_E_ARG;
     }
     std::vector<float4> vp(vt);
@@ -2336,7 +2324,8 @@ int ff_copy_mesh(ff_ctx *c, float *xyz, uint8_t *rgb, uint32_t *faces)
         if (e == hipSuccess) e = hipMemcpyAsync(d_segs, segs.data(), sizeof(uint4) * segs.size(), hipMemcpyHostToDevice, c->stream);
         if (e == hipSuccess) e = hipMemcpyAsync(d_offs, offs.data(), sizeof(uint2) * offs.size(), hipMemcpyHostToDevice, c->stream);
         if (e == hipSuccess) {
-            k_gather<<<1024, 256, 0, c->stream>>>(d_segs, d_offs, (uint32_t)cells.size(), s->d_vpool, s->d_fpool, d_v, d_f);
+            k_gather<<<1024, 256, 0, c->stream>>>(d_segs, d_offs, (uint32_t)cells.size(), s->d_vpool, s->d_fpool, d_v, d_f,
+                                                  local ? 1u : 0u);
             e = hipGetLastError();
         }
         if (e == hipSuccess && vt) e = hipMemcpyAsync(vp.data(), d_v, sizeof(float4) * vt, hipMemcpyDeviceToHost, c->stream);
@@ -2358,6 +2347,108 @@ int ff_copy_mesh(ff_ctx *c, float *xyz, uint8_t *rgb, uint32_t *faces)
     }
     if (faces) memcpy(faces, fp.data(), sizeof(uint32_t) * fp.size());
     return FF_OK;
+}
+
+// the cells the last completed update re-meshed, in key order
+static int update_cells(ff_ctx *c, std::vector<CellRec> &out)
+{
+    std::vector<CellRec> all;
+    const int rc = host_cells(c, all);
+    if (rc) return rc;
+    out.clear();
+    const uint32_t id = c->mesh->lastUpdateId;
+    if (!id) return FF_OK;
+    for (const CellRec &r : all)
+        if (r.pad[0] == id) out.push_back(r);
+    return FF_OK;
+}
+
+static void cell_desc(const CellRec &r, int32_t *d)
+{
+    d[0] = (int32_t)(r.key & 7u);
+    d[1] = (int32_t)((r.key >> 48) & 0xffffu);
+    d[2] = (int32_t)((r.key >> 32) & 0xffffu);
+    d[3] = (int32_t)((r.key >> 16) & 0xffffu);
+    d[4] = 1 << ((r.key >> 3) & 31u);
+}
+
+}  // namespace ff
+
+using namespace ff;
+
+extern "C" {
+
+int ff_update_meshes(ff_ctx *c)
+{
+    if (!c) return FF_E_ARG;
+    hipSetDevice(c->device);
+    if (!c->incremental) return 0;
+    return update_meshes(c);
+}
+
+int ff_mesh_full(ff_ctx *c)
+{
+    if (!c) return FF_E_ARG;
+    hipSetDevice(c->device);
+    int rc = mesh_complete(c);
+    if (rc != FF_OK) return rc;
+    MeshState *s = c->mesh;
+    s->fullNext = true;
+    rc = update_meshes(c);
+    s->fullNext = false;
+    if (rc != FF_OK) return rc;
+    return mesh_complete(c);
+}
+
+int ff_mesh_sizes(ff_ctx *c, size_t *nv, size_t *nf)
+{
+    if (!c) return FF_E_ARG;
+    std::vector<CellRec> cells;
+    int rc = host_cells(c, cells);
+    if (rc) return rc;
+    size_t v = 0, f = 0;
+    for (const auto &r : cells) { v += r.nv; f += r.nf; }
+    if (nv) *nv = v;
+    if (nf) *nf = f;
+    return FF_OK;
+}
+
+int ff_copy_mesh(ff_ctx *c, float *xyz, uint8_t *rgb, uint32_t *faces)
+{
+    if (!c) return FF_E_ARG;
+    std::vector<CellRec> cells;
+    int rc = host_cells(c, cells);
+    if (rc) return rc;
+    return gather_cells(c, cells, false, xyz, rgb, faces);
+}
+
+int ff_cell_update_sizes(ff_ctx *c, size_t *n_cells, size_t *n_vertices, size_t *n_faces)
+{
+    if (!c) return FF_E_ARG;
+    std::vector<CellRec> cells;
+    const int rc = update_cells(c, cells);
+    if (rc) return rc;
+    size_t v = 0, f = 0;
+    for (const CellRec &r : cells) { v += r.nv; f += r.nf; }
+    if (n_cells) *n_cells = cells.size();
+    if (n_vertices) *n_vertices = v;
+    if (n_faces) *n_faces = f;
+    return FF_OK;
+}
+
+int ff_copy_cell_update(ff_ctx *c, int32_t *desc5, uint32_t *nv, uint32_t *nf, float *xyz, uint8_t *rgb,
+                        uint32_t *faces)
+{
+    if (!c) return FF_E_ARG;
+    std::vector<CellRec> cells;
+    const int rc = update_cells(c, cells);
+    if (rc) return rc;
+    for (size_t i = 0; i < cells.size(); i++) {
+        if (desc5) cell_desc(cells[i], desc5 + 5 * i);
+        if (nv) nv[i] = cells[i].nv;
+        if (nf) nf[i] = cells[i].nf;
+    }
+    return gather_cells(c, cells, true, xyz, rgb, faces);
 }
 
 int ff_cell_count(ff_ctx *c, size_t *n)

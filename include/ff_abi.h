@@ -100,6 +100,14 @@ int ff_add_map_device_noalloc(ff_ctx *ctx, const uint16_t *depth_dev, const uint
  * its errors (FF_E_POOL_FULL: cell table overflow).  Returns FF_OK or FF_E*. */
 int ff_update_meshes(ff_ctx *ctx);
 
+/* The whole map meshed from scratch, the counterpart of the reference's
+ * non-incremental getMeshRecursive (geometryfusion_mipmap_cpu.cpp:3658-3674):
+ * every cell is re-meshed with the same per-cell marching cubes, whatever
+ * changed, and the call returns when the meshes are complete; ff_mesh_sizes
+ * and ff_copy_mesh then return the full mesh, which equals the union of the
+ * incremental cell meshes of an up-to-date map. */
+int ff_mesh_full(ff_ctx *ctx);
+
 /* getMeshInterleavedMarchingCubes (geometryfusion_mipmap_cpu.cpp:3359-3374):
  * the MeshInterleaved layout -- vertices 3 x f32, colors 3 x u8 (R, G, B),
  * faces 3 x u32 per triangle.  ff_mesh_sizes returns the counts; ff_copy_mesh
@@ -113,6 +121,20 @@ int ff_copy_mesh(ff_ctx *ctx, float *xyz, uint8_t *rgb, uint32_t *faces);
  * in the order their geometry appears in ff_copy_mesh. */
 int ff_cell_count(ff_ctx *ctx, size_t *n_cells);
 int ff_copy_cells(ff_ctx *ctx, int32_t *desc5, uint32_t *n_vertices, uint32_t *n_faces);
+
+/* getMeshCellsUpdate (geometryfusion_mipmap_cpu.cpp:2989-3047; CellUpdate /
+ * InterleavedMeshCell, geometryfusion_mipmap_cpu.hpp:40-59): the cells the
+ * last completed ff_update_meshes re-meshed, each with its own mesh
+ * (vertices, colors and triangle indices local to the cell), empty meshes
+ * included.  The reference names a cell by its index in its cell array;
+ * here a cell is named by its descriptor {type, ox, oy, oz, size}, which
+ * stays the same across updates.  ff_cell_update_sizes gives the number of
+ * cells and their summed vertex and face counts; ff_copy_cell_update fills
+ * per cell the descriptor and counts, and the meshes concatenated in that
+ * order (any pointer may be NULL).  Cells are in key order. */
+int ff_cell_update_sizes(ff_ctx *ctx, size_t *n_cells, size_t *n_vertices, size_t *n_faces);
+int ff_copy_cell_update(ff_ctx *ctx, int32_t *desc5, uint32_t *n_vertices, uint32_t *n_faces, float *xyz,
+                        uint8_t *rgb, uint32_t *faces);
 
 /* setIncrementalMeshing (geometryfusion_mipmap_cpu.hpp:131). */
 int ff_set_incremental_meshing(ff_ctx *ctx, int on);

@@ -99,6 +99,7 @@ void ffo_destroy(ffo_ctx *c)
     free(c->tree); free(c->leafPos); free(c->leafScale); free(c->leafParent);
     free(c->dist); free(c->wgt); free(c->col); free(c->queueIndexOfLeaf); free(c->queue);
     free(c->pxp); free(c->pyp); free(c->processed);
+    ffo_mesh_forget(c);
     free(c);
 }
 

@@ -743,9 +743,37 @@ typedef struct {
     uint32_t nleaves_cap;
     uint32_t epoch;
     meshbuf all;
+    cellvec upd; /* the cells the last update re-meshed (getMeshCellsUpdate) */
+    const ffo_ctx *owner; /* the context this state belongs to */
+    uint32_t last_epoch;  /* epoch of the last update */
 } inc_state;
 
 static inc_state g_inc;
+
+/* the incremental state is kept for one context at a time: a call for
+ * another context starts it afresh */
+static void inc_reset(const ffo_ctx *c)
+{
+    for (size_t i = 0; i < g_inc.cap; i++) {
+        free(g_inc.tab[i].xyz); free(g_inc.tab[i].rgb); free(g_inc.tab[i].faces);
+    }
+    free(g_inc.tab);
+    free(g_inc.last_queue_frame);
+    free(g_inc.all.xyz); free(g_inc.all.rgb); free(g_inc.all.faces);
+    free(g_inc.upd.cells);
+    memset(&g_inc, 0, sizeof g_inc);
+    g_inc.owner = c;
+}
+
+static void inc_own(const ffo_ctx *c)
+{
+    if (g_inc.owner != c) inc_reset(c);
+}
+
+void ffo_mesh_forget(const ffo_ctx *c)
+{
+    if (g_inc.owner == c) inc_reset(NULL);
+}
 
 static size_t inc_find(uint64_t key)
 {
@@ -777,6 +805,7 @@ static int chain_changed(const mapview *m, uint32_t leaf, uint32_t since)
 /* Records the bricks queued by the last add_map (call after each frame). */
 void ffo_mesh_note_frame(ffo_ctx *c)
 {
+    inc_own(c);
     uint32_t cap = ffo_leaves_total(c);
     if (g_inc.nleaves_cap != cap) {
         free(g_inc.last_queue_frame);
@@ -791,7 +820,8 @@ void ffo_mesh_note_frame(ffo_ctx *c)
 /* Incremental updateMeshes; returns the number of cells re-meshed. */
 int ffo_mesh_update(ffo_ctx *c)
 {
-    static uint32_t last_epoch = 0;
+    inc_own(c);
+    uint32_t last_epoch = g_inc.last_epoch;
     mapview m;
     make_view(c, &m);
     if (!m.n) return 0;
@@ -809,6 +839,7 @@ int ffo_mesh_update(ffo_ctx *c)
     while (2 * all.n > g_inc.cap) inc_rehash(2 * g_inc.cap);
     int remeshed = 0;
     meshbuf tmp = {0};
+    g_inc.upd.n = 0;
     for (size_t i = 0; i < all.n; i++) {
         const cell_t *cl = &all.cells[i];
         uint64_t key = cell_key(cl);
@@ -831,6 +862,11 @@ int ffo_mesh_update(ffo_ctx *c)
         e->nv = (uint32_t)tmp.nv;
         e->nf = (uint32_t)tmp.nf;
         remeshed++;
+        if (g_inc.upd.n == g_inc.upd.cap) {
+            g_inc.upd.cap = g_inc.upd.cap ? 2 * g_inc.upd.cap : 1024;
+            g_inc.upd.cells = (cell_t *)realloc(g_inc.upd.cells, g_inc.upd.cap * sizeof(cell_t));
+        }
+        g_inc.upd.cells[g_inc.upd.n++] = *cl;
     }
     free(tmp.xyz); free(tmp.rgb); free(tmp.faces);
     /* concatenate every current cell's mesh (the reference's per-update copy) */
@@ -843,8 +879,42 @@ int ffo_mesh_update(ffo_ctx *c)
             mb_face(&g_inc.all, base + e->faces[3 * f], base + e->faces[3 * f + 1], base + e->faces[3 * f + 2]);
     }
     free(all.cells);
-    last_epoch = g_inc.epoch;
+    g_inc.last_epoch = g_inc.epoch;
     return remeshed;
 }
 
 size_t ffo_mesh_update_faces(void) { return g_inc.all.nf; }
+
+/* CellUpdate (geometryfusion_mipmap_cpu.cpp:2961-3047): the cells the last
+ * ffo_mesh_update re-meshed, in key order, each with its own mesh (faces
+ * local to the cell).  out = {cells, vertices, faces}. */
+void ffo_mesh_update_sizes(size_t out[3])
+{
+    qsort(g_inc.upd.cells, g_inc.upd.n, sizeof(cell_t), cmp_cells);
+    out[0] = g_inc.upd.n;
+    out[1] = out[2] = 0;
+    for (size_t i = 0; i < g_inc.upd.n; i++) {
+        const centry *e = &g_inc.tab[inc_find(cell_key(&g_inc.upd.cells[i]))];
+        out[1] += e->nv;
+        out[2] += e->nf;
+    }
+}
+
+void ffo_mesh_update_copy(int32_t *desc13, uint32_t *nv, uint32_t *nf, float *xyz, uint8_t *rgb, uint32_t *faces)
+{
+    size_t v = 0, f = 0;
+    for (size_t i = 0; i < g_inc.upd.n; i++) {
+        const cell_t *cl = &g_inc.upd.cells[i];
+        const centry *e = &g_inc.tab[inc_find(cell_key(cl))];
+        int32_t *d = desc13 + 13 * i;
+        d[0] = cl->type; d[1] = cl->o[0]; d[2] = cl->o[1]; d[3] = cl->o[2]; d[4] = cl->size;
+        for (int k = 0; k < 8; k++) d[5 + k] = (int32_t)cl->ll[k];
+        nv[i] = e->nv;
+        nf[i] = e->nf;
+        memcpy(xyz + 3 * v, e->xyz, e->nv * 3 * sizeof(float));
+        memcpy(rgb + 3 * v, e->rgb, e->nv * 3);
+        memcpy(faces + 3 * f, e->faces, e->nf * 3 * sizeof(uint32_t));
+        v += e->nv;
+        f += e->nf;
+    }
+}

@@ -62,6 +62,8 @@ def _mesh_sigs(lib):
     lib.ffo_mesh_update.argtypes = [P]
     lib.ffo_mesh_update_faces.restype = ctypes.c_size_t
     lib.ffo_mesh_update_faces.argtypes = []
+    lib.ffo_mesh_update_sizes.argtypes = [P]
+    lib.ffo_mesh_update_copy.argtypes = [P, P, P, P, P, P]
 
 
 def _loop_sigs(lib):
@@ -166,6 +168,21 @@ class Oracle:
         """Incremental re-mesh of the cells affected since the last call plus the
         concatenation of all cell meshes (CPU-baseline timing path)."""
         return self.lib.ffo_mesh_update(self.h)
+
+    def cell_update(self):
+        """The cells the last update_meshes re-meshed (CellUpdate), in key
+        order: (desc (n, 13), nv, nf, xyz, rgb, faces local per cell)."""
+        cnt = np.zeros(3, np.uint64)
+        self.lib.ffo_mesh_update_sizes(_vp(cnt))
+        nc, nv, nf = (int(v) for v in cnt)
+        desc = np.zeros((nc, 13), np.int32)
+        cv = np.zeros(nc, np.uint32)
+        cf = np.zeros(nc, np.uint32)
+        xyz = np.zeros((nv, 3), np.float32)
+        rgb = np.zeros((nv, 3), np.uint8)
+        faces = np.zeros((nf, 3), np.uint32)
+        self.lib.ffo_mesh_update_copy(_vp(desc), _vp(cv), _vp(cf), _vp(xyz), _vp(rgb), _vp(faces))
+        return desc, cv, cf, xyz, rgb, faces
 
     def mesh_all(self):
         """Meshes every meshable cell from scratch.  Returns (desc (n, 13): type,

@@ -164,3 +164,60 @@ def test_mesh_pools_too_small_grow():
     print(f"mesh pools: {caps}", flush=True)
     assert caps[-1] > 4096 and caps[-1] == caps[-2]
     compare_mesh(gpu, ora, cfg["scale"], "after growth")
+
+
+def test_cell_update_matches_oracle_incremental():
+    """getMeshCellsUpdate (geometryfusion_mipmap_cpu.cpp:2989-3047): after
+    each frame and updateMeshes, the cells the GPU re-meshed are the cells
+    the oracle's incremental mesher re-meshed, in key order, each with the
+    same mesh (faces local to the cell)."""
+    sc, R, t, cfg = synth.config_c1(n_frames=200)
+    K = (262.5, 262.5, 159.5, 119.5)
+    gpu, ora = make_pair(cfg["scale"], cfg["threshold"])
+    sizes = []
+    for k, i in enumerate([0, 3, 6, 9, 9, 12, 40]):
+        d, bgr = sc.render(R[i].reshape(3, 3), t[i], K, 320, 240, noise=cfg["noise"], dropout=cfg["dropout"],
+                           seed=1000 + i + 100 * k)
+        assert gpu.addMap(d, R[i], t[i], K, bgr, 1 / 5000.0, cfg["maxcam"]) == \
+            ora.add_map(d, R[i], t[i], K, bgr, 1 / 5000.0, cfg["maxcam"])
+        gpu.updateMeshes()
+        ora.update_meshes()
+        u = gpu.getMeshCellsUpdate()
+        desc, nv, nf, xyz, rgb, faces = ora.cell_update()
+        tag = f"update {k} (frame {i})"
+        assert np.array_equal(u.desc, desc[:, :5]), f"{tag}: {len(u.desc)} gpu cells vs {len(desc)} oracle cells"
+        assert np.array_equal(u.n_vertices, nv) and np.array_equal(u.n_faces, nf), tag
+        err = np.abs(u.vertices.astype(np.float64) - xyz).max() if len(xyz) else 0.0
+        assert err <= 1e-4 * cfg["scale"], f"{tag}: vertex error {err}"
+        assert np.array_equal(u.faces, faces) and np.array_equal(u.colors, rgb), tag
+        for dsc, v, c, f in u.cells():
+            assert len(f) == 0 or f.max() < len(v)
+        sizes.append(len(u.desc))
+    assert min(sizes) > 0 and max(sizes) > 100
+    # fewer cells than the map has: the update is incremental
+    assert sizes[-2] < len(gpu.getMeshCells()[0])
+
+
+def test_full_mesh_equals_incremental_union():
+    """The from-scratch mesh of the whole map (ff_mesh_full, the counterpart
+    of getMeshRecursive) equals the union of the incremental cell meshes,
+    bit for bit, and the oracle's from-scratch meshes."""
+    sc, R, t, cfg = synth.config_c2(n_frames=60)
+    cfg = dict(cfg, w=320, h=240)
+    K = (262.5, 262.5, 159.5, 119.5)
+    gpu, ora = make_pair(cfg["scale"], cfg["threshold"])
+    for i in range(0, 24, 2):
+        d, bgr = sc.render(R[i].reshape(3, 3), t[i], K, 320, 240, noise=cfg["noise"], dropout=cfg["dropout"],
+                           seed=1000 + i)
+        gpu.addMap(d, R[i], t[i], K, bgr, 1 / 5000.0, cfg["maxcam"])
+        ora.add_map(d, R[i], t[i], K, bgr, 1 / 5000.0, cfg["maxcam"])
+        gpu.updateMeshes()
+    inc = gpu.getMeshInterleavedMarchingCubes()
+    gpu.meshFull()
+    full = gpu.getMeshInterleavedMarchingCubes()
+    assert len(full.faces) > 1000
+    assert np.array_equal(inc.vertices.view(np.uint32), full.vertices.view(np.uint32))
+    assert np.array_equal(inc.colors, full.colors) and np.array_equal(inc.faces, full.faces)
+    # every cell was re-meshed by the full pass
+    assert len(gpu.getMeshCellsUpdate().desc) == len(gpu.getMeshCells()[0])
+    compare_mesh(gpu, ora, cfg["scale"], "full")
