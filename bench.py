@@ -37,24 +37,24 @@ def render_frames(sc, R, t, n, cfg):
     return depth, bgr
 
 
-def cpu_baseline(depth, bgr, R, t, cfg, budget_s):
+def cpu_baseline(frame, n_max, R, t, K, cfg, budget_s, what):
     """The oracle (a scalar port of the reference path, 1 core) on the first
-    frames of the same workload, bounded by budget_s seconds."""
-    from fastfusion_amd import synth
+    frames of the same workload, bounded by budget_s seconds of oracle time
+    (frame(i) gives the host images; producing them is not timed)."""
     from oracle.pyoracle import Oracle
     o = Oracle(cfg["scale"], cfg["threshold"], True, max_leaves=1 << 20, max_branches=1 << 24)
-    has_mesh = hasattr(o, "update_meshes")
     n = 0
-    t0 = time.perf_counter()
-    while n < len(depth) and time.perf_counter() - t0 < budget_s:
-        o.add_map(depth[n], R[n], t[n], synth.K_DEFAULT, bgr[n], 1 / 5000.0, cfg["maxcam"])
-        if has_mesh:
-            o.update_meshes()
+    dt = 0.0
+    while n < n_max and dt < budget_s:
+        d, bgr = frame(n)
+        t0 = time.perf_counter()
+        o.add_map(d, R[n], t[n], K, bgr, 1 / 5000.0, cfg["maxcam"])
+        o.update_meshes()
+        dt += time.perf_counter() - t0
         n += 1
-    dt = time.perf_counter() - t0
     return {"value": n / dt, "unit": "frames/s", "cores": 1, "kind": "port",
-            "sample": f"first {n} frames of the same 640x480 5 mm sequence, oracle addMap"
-                      f"{' + updateMeshes' if has_mesh else ''} on 1 host core, {dt:.1f} s"}
+            "sample": f"first {n} frames of the same {what} sequence, oracle addMap + updateMeshes (plain "
+                      f"weighting) on 1 host core, {dt:.1f} s"}
 
 
 LOOP_METRIC = "de-integrated frames/sec (loop closure) at 640x480, 5mm"
@@ -292,8 +292,12 @@ def run_streams(args):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=950)
-    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--steps", type=int, default=None, help="timed frames (default 950; 2950 with --large)")
+    ap.add_argument("--warmup", type=int, default=None, help="untimed frames before them (default 50)")
+    ap.add_argument("--seq-frames", type=int, default=None,
+                    help="frames of the sequence (default 1000; 3000 with --large); the timed ones are its last")
+    ap.add_argument("--large", action="store_true",
+                    help="large-map workload: 1280x960, 2.5 mm floor, learned weighting, 3000 frames")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--graphs", action="store_true", help="replay each frame from captured HIP graphs")
@@ -311,6 +315,10 @@ def main():
                     help="summarise the counter_collection.csv of a FETCH_SIZE and a WRITE_SIZE rocprofv3 run of "
                          "this script (with --calib-copy 256) for the timed frames given by --steps / --warmup")
     args = ap.parse_args()
+    if args.steps is None:
+        args.steps = 2950 if args.large else 950
+    if args.warmup is None:
+        args.warmup = 50
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported: every run here uses one "
               "GPU (independent map streams would shard by stream, a single map by Morton range of its bricks)",
@@ -409,19 +417,6 @@ def read_traffic(path, key):
     return {}
 
 
-def fuse_walk(fz, d_depth, d_bgr, R, t, K, cfg, frames):
-    """addMap + updateMeshes over the given frames (device-resident images);
-    returns the summed queued bricks."""
-    w, h = cfg["w"], cfg["h"]
-    fb, cb = w * h * 2, w * h * 3
-    nq = 0
-    for i in frames:
-        nq += fz.addMapDevice(d_depth.addr(i * fb), d_bgr.addr(i * cb), w, h, R[i], t[i], K, 1 / 5000.0,
-                              cfg["maxcam"])
-        fz.updateMeshes()
-    return nq
-
-
 def kernel_entry(name, ms, calls, bytes_per_launch, traffic):
     avg = ms / max(calls, 1)
     ach = bytes_per_launch / (avg * 1e-3) / 1e9 if avg > 0 else 0.0
@@ -431,45 +426,87 @@ def kernel_entry(name, ms, calls, bytes_per_launch, traffic):
 
 
 def run_walk(args):
-    """The 640x480, 5 mm walk.  Frames before the warm-up and timed ones are
-    fused untimed first, so that the timed frames are always the walk's last
-    `steps` frames (its steady state) whatever --steps is.  The algorithmic
-    bytes of the timed frames come from a replay of the same frames on a
-    second map with every phase evented (the map is deterministic: the
-    replay's counts equal the timed run's, which is checked)."""
-    from fastfusion_amd import FusionMipMapGPU, hiprt, synth
+    """The 640x480, 5 mm walk (or, with --large, the 1280x960, 2.5 mm floor
+    with learned weighting).  Frames before the warm-up and timed ones are
+    fused untimed first, so that the timed frames are always the sequence's
+    last `steps` frames (its steady state) whatever --steps is.  The
+    algorithmic bytes of the timed frames come from a replay of the same
+    frames on a second map with every phase evented (the map is
+    deterministic: the replay's counts equal the timed run's, which is
+    checked).  The frames are rendered on the host in chunks and kept in HBM."""
+    from fastfusion_amd import FusionMipMapGPU, hiprt, learned, synth
 
-    n_walk = max(1000, args.warmup + args.steps)
-    prefuse = n_walk - args.warmup - args.steps
-    sc, R, t, cfg = synth.config_c2(n_frames=n_walk)
-    t_r = time.perf_counter()
-    depth, bgr = render_frames(sc, R, t, n_walk, cfg)
-    print(f"[bench] rendered {n_walk} frames in {time.perf_counter() - t_r:.1f} s", file=sys.stderr, flush=True)
+    large = args.large
+    n_seq = max(args.seq_frames or (3000 if large else 1000), args.warmup + args.steps)
+    prefuse = n_seq - args.warmup - args.steps
+    if large:
+        sc, R, t, cfg = synth.config_c5(n_frames=n_seq)
+        K = cfg["K"]
+        what = "1280x960 2.5 mm floor"
+    else:
+        sc, R, t, cfg = synth.config_c2(n_frames=n_seq)
+        K = synth.K_DEFAULT
+        what = "640x480 5 mm"
     w, h = cfg["w"], cfg["h"]
-    K = synth.K_DEFAULT
-    d_depth = hiprt.DeviceBuffer.from_array(depth)
-    d_bgr = hiprt.DeviceBuffer.from_array(bgr)
+    fb, cb = w * h * 2, w * h * 3
+    d_depth = hiprt.DeviceBuffer(fb * n_seq)
+    d_bgr = hiprt.DeviceBuffer(cb * n_seq)
+
+    def frame(i):
+        return sc.render(R[i].reshape(3, 3), t[i], K, w, h, noise=cfg["noise"], dropout=cfg["dropout"],
+                         seed=1000 + i)
+
+    t_r = time.perf_counter()
+    chunk = 100
+    for c0 in range(0, n_seq, chunk):
+        n = min(chunk, n_seq - c0)
+        dd = np.zeros((n, h, w), np.uint16)
+        bb = np.zeros((n, h, w, 3), np.uint8)
+        for k in range(n):
+            dd[k], bb[k] = frame(c0 + k)
+        hiprt.copy_to_device(d_depth.addr(c0 * fb), dd)
+        hiprt.copy_to_device(d_bgr.addr(c0 * cb), bb)
+        if large:
+            print(f"[bench] rendered {c0 + n}/{n_seq} frames ({time.perf_counter() - t_r:.0f} s)", file=sys.stderr,
+                  flush=True)
+    print(f"[bench] rendered {n_seq} frames in {time.perf_counter() - t_r:.1f} s", file=sys.stderr, flush=True)
+    use_learned = args.learned or large
 
     def make():
-        fz = FusionMipMapGPU(0, 0, 0, cfg["scale"], cfg["threshold"], 0, True, max_leaves=1 << 21,
-                             max_branches=1 << 25)
+        if large:
+            # 8 Mi bricks (57 GiB of TSDF) and 2^27 tree slots: sized for one
+            # MI355X's 288 GB beside the frames and the mesh pools
+            fz = FusionMipMapGPU(0, 0, 0, cfg["scale"], cfg["threshold"], 0, True, max_leaves=1 << 23,
+                                 max_branches=1 << 27)
+        else:
+            fz = FusionMipMapGPU(0, 0, 0, cfg["scale"], cfg["threshold"], 0, True, max_leaves=1 << 21,
+                                 max_branches=1 << 25)
         if args.graphs:
             fz.setGraphCapture(True)
-        if args.learned:
-            from fastfusion_amd import learned
+        if use_learned:
             fz.setLearnedWeighting(learned.load_weights())
         return fz
 
+    def fuse(fz, frames, tag):
+        nq = 0
+        for k, i in enumerate(frames):
+            nq += fz.addMapDevice(d_depth.addr(i * fb), d_bgr.addr(i * cb), w, h, R[i], t[i], K, 1 / 5000.0,
+                                  cfg["maxcam"])
+            fz.updateMeshes()
+            if large and k % 500 == 499:
+                print(f"[bench] {tag}: {k + 1} frames", file=sys.stderr, flush=True)
+        return nq
+
     untimed = range(0, prefuse + args.warmup)
-    timed = range(prefuse + args.warmup, n_walk)
+    timed = range(prefuse + args.warmup, n_seq)
     # the timed run: k_sdf's and k_mc's dispatches record events on one
     # frame in 8 (each event costs launch time)
     fz = make()
-    fuse_walk(fz, d_depth, d_bgr, R, t, K, cfg, untimed)
+    fuse(fz, untimed, "untimed")
     hiprt.synchronize()
     fz.set_timing(True, sdf_only=True, every=8)
     t0 = time.perf_counter()
-    nq_sum = fuse_walk(fz, d_depth, d_bgr, R, t, K, cfg, timed)
+    nq_sum = fuse(fz, timed, "timed")
     hiprt.synchronize()
     dt = time.perf_counter() - t0
     tm = fz.timing()
@@ -478,16 +515,31 @@ def run_walk(args):
     print(f"[bench] timed {args.steps} frames in {dt:.4f} s", file=sys.stderr, flush=True)
     # surface error of the final mesh against the analytic scene (GT); its
     # gather kernel also ends the timed run in a counter profile
+    t1 = time.perf_counter()
     mesh = fz.getMeshInterleavedMarchingCubes()
+    t_copy_inc = time.perf_counter() - t1
     err = sc.sdf(mesh.vertices) if len(mesh.vertices) else np.zeros(1, np.float32)
     rms = float(np.sqrt(np.mean(err.astype(np.float64) ** 2)))
+    # the final full extraction (getMeshRecursive's counterpart), timed apart
+    hiprt.synchronize()
+    t1 = time.perf_counter()
+    fz.meshFull()
+    t_full = time.perf_counter() - t1
+    t1 = time.perf_counter()
+    full = fz.getMeshInterleavedMarchingCubes()
+    t_copy = time.perf_counter() - t1
+    same_full = (len(full.faces) == len(mesh.faces) and
+                 np.array_equal(full.vertices.view(np.uint32), mesh.vertices.view(np.uint32)))
+    del full
+    n_cells = len(fz.getMeshCells()[0])
+    del fz
 
     # the replay: every phase evented over the same timed frames
     fr = make()
-    fuse_walk(fr, d_depth, d_bgr, R, t, K, cfg, untimed)
+    fuse(fr, untimed, "replay untimed")
     hiprt.synchronize()
     fr.set_timing(True)
-    nq_replay = fuse_walk(fr, d_depth, d_bgr, R, t, K, cfg, timed)
+    nq_replay = fuse(fr, timed, "replay")
     hiprt.synchronize()
     rp = fr.timing()
     rmc = fr.mc_timing()
@@ -500,8 +552,8 @@ def run_walk(args):
     # whole frame (addMap + updateMeshes): algorithmic bytes of the timed frames
     B = sum(v["bytes"] for v in rp.values())
     achieved = B / dt / 1e9
-    tr = read_traffic(os.path.join(ROOT, "profiles", "hbm_traffic_640x480_5mm.md"),
-                      "learned" if args.learned else "plain")
+    tr = read_traffic(os.path.join(ROOT, "profiles", "hbm_traffic_1280x960_2.5mm_floor.md" if large else
+                                   "hbm_traffic_640x480_5mm.md"), "learned" if use_learned else "plain")
     sdf = tm["sdf"]
     sdf_bpl = sdf["bytes"] / max(sdf["calls"], 1)
     mc_bpl = rmc["bytes"] / max(rmc["byte_calls"], 1)
@@ -511,10 +563,21 @@ def run_walk(args):
     }
     cpu = None
     if not args.no_cpu_baseline:
-        cpu = cpu_baseline(depth, bgr, R, t, cfg, args.cpu_seconds)
+        cpu = cpu_baseline(frame, n_seq, R, t, K, cfg, args.cpu_seconds, what)
     nst = max(args.steps, 1)
+    if large:
+        metric = ("large map: fused+meshed frames/sec and HBM GB/s at 1280x960, 2.5mm, learned weighting; "
+                  "RMS surface err vs GT; final full extraction time")
+        workload = (f"large map: 12x12x3 m floor of 3x3 rooms, 60 primitives (seed 3000), {n_seq}-frame seeded "
+                    "random walk, 1280x960 (f = 1050), 2.5 mm voxels, 10 mm truncation, 4 m max depth, learned "
+                    "confidence on; timed: its last frames")
+    else:
+        metric = METRIC
+        workload = (("learned confidence" if args.learned else "plain method") +
+                    f", 640x480, 5 mm voxels, 15 mm truncation, 4 m max depth, {n_seq}-frame seeded random walk "
+                    "(room seed 11, walk seed 12); timed: its last frames")
     line = {
-        "metric": METRIC,
+        "metric": metric,
         "value": round(args.steps / dt, 2),
         "unit": "frames/s",
         "n_gpus": 1,
@@ -525,13 +588,11 @@ def run_walk(args):
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32",
-        "data": "synthetic: seeded analytic room ray-cast on the host, u16 depth + BGR8, resident in HBM",
-        "config": {"workload": ("learned confidence" if args.learned else "plain method") +
-                               f", 640x480, 5 mm voxels, 15 mm truncation, 4 m max depth, {n_walk}-frame seeded "
-                               "random walk (room seed 11, walk seed 12); timed: its last frames",
-                   "frames": n_walk, "untimed_before_warmup": prefuse, "timed_frames": [timed.start, timed.stop],
+        "data": "synthetic: seeded analytic scene ray-cast on the host, u16 depth + BGR8, resident in HBM",
+        "config": {"workload": workload,
+                   "frames": n_seq, "untimed_before_warmup": prefuse, "timed_frames": [timed.start, timed.stop],
                    "resolution": [w, h], "voxel_m": cfg["scale"], "truncation_m": cfg["threshold"],
-                   "maxcam_m": cfg["maxcam"]},
+                   "maxcam_m": cfg["maxcam"], "learned": use_learned},
         "roofline": {"kernel": "whole frame (addMap + updateMeshes, every kernel)", "bound": "hbm",
                      "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": tr.get("frame"),
@@ -543,8 +604,11 @@ def run_walk(args):
         "bricks_per_frame": round(nq_sum / nst, 1),
         "processed_px_per_frame": round(rp["alloc"]["units"] / nst, 1),
         "remeshed_cells_per_frame": round(rp["mesh"]["units"] / nst, 1),
-        "map": {"bricks": counts["n_leaves_used"], "side": counts["side"]},
+        "map": {"bricks": counts["n_leaves_used"], "side": counts["side"], "cells": n_cells,
+                "tree_slots": counts["n_branches_used"]},
         "phases_ms_per_frame_replay": {k: round(v["ms"] / max(v["calls"], 1), 4) for k, v in rp.items()},
+        "full_extraction": {"ms": round(t_full * 1e3, 2), "copy_to_host_ms": round(t_copy * 1e3, 2),
+                            "equals_incremental": bool(same_full), "incremental_copy_ms": round(t_copy_inc * 1e3, 2)},
         "rms_surface_err_m": round(rms, 6),
         "rms_surface_err_voxels": round(rms / cfg["scale"], 4),
         "mesh": {"vertices": int(len(mesh.vertices)), "triangles": int(len(mesh.faces))},

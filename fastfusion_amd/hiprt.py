@@ -75,5 +75,11 @@ class DeviceBuffer:
             pass
 
 
+def copy_to_device(addr, a):
+    """Copies the host array a to device address addr (synchronous)."""
+    a = np.ascontiguousarray(a)
+    check(hip().hipMemcpy(ctypes.c_void_p(addr), a.ctypes.data_as(ctypes.c_void_p), a.nbytes, H2D), "hipMemcpy H2D")
+
+
 def synchronize():
     check(hip().hipDeviceSynchronize(), "hipDeviceSynchronize")
