@@ -316,13 +316,17 @@ __device__ __forceinline__ void dedup_body(uint4 *ring, const uint32_t *__restri
     const int nsteps = H + words - 1;
     const int niter = (nsteps + R - 1) / R;
     const int lt = tid - 64;
-    DedupLoad<WMAX> pre;
+    // the loaders keep two bands in registers: band b is fetched two
+    // iterations before it goes into the ring, so a fetch has two
+    // iterations of the sweep to arrive
+    DedupLoad<WMAX> pre[2];
     if (!compute) {
         for (int b = 0; b < 3; b++) {
-            dedup_fetch<WMAX>(pre, planes, H, words, b, lt);
-            dedup_put<WMAX>(pre, ring, b, lt);
+            dedup_fetch<WMAX>(pre[0], planes, H, words, b, lt);
+            dedup_put<WMAX>(pre[0], ring, b, lt);
         }
-        dedup_fetch<WMAX>(pre, planes, H, words, 3, lt);
+        dedup_fetch<WMAX>(pre[1], planes, H, words, 3, lt);
+        dedup_fetch<WMAX>(pre[0], planes, H, words, 4, lt);
     }
     __syncthreads();  // the first three bands are in the ring
     const int l = tid & 63;
@@ -354,9 +358,15 @@ __device__ __forceinline__ void dedup_body(uint4 *ring, const uint32_t *__restri
                 pin = (uint32_t)__builtin_amdgcn_mov_dpp((int)P, 0x138, 0xf, 0xf, true);  // wave_shr:1, lane 0 gets 0
             }
         } else {
-            // band it + 3 into the slot band it - 1 was read from; band it + 4 into registers
-            dedup_put<WMAX>(pre, ring, it + 3, lt);
-            dedup_fetch<WMAX>(pre, planes, H, words, it + 4, lt);
+            // band it + 3 into the slot band it - 1 was read from; band it + 5
+            // into the registers it came from
+            if ((it + 3) & 1) {
+                dedup_put<WMAX>(pre[1], ring, it + 3, lt);
+                dedup_fetch<WMAX>(pre[1], planes, H, words, it + 5, lt);
+            } else {
+                dedup_put<WMAX>(pre[0], ring, it + 3, lt);
+                dedup_fetch<WMAX>(pre[0], planes, H, words, it + 5, lt);
+            }
         }
         __syncthreads();
     }
@@ -1169,7 +1179,6 @@ __global__ __launch_bounds__(512) void k_sdf(const uint32_t *__restrict__ queue,
     uint32_t tv[2048 / 512];
 #pragma unroll
     for (int k = 0; k < 2048 / 512; k++) tv[k] = c_rcp_table[threadIdx.x + 512 * k];
-    const uint32_t leaf0 = blockIdx.x < m.nLeavesTotal ? queue[blockIdx.x] : 0u;
     // blocks loop over the queued bricks; the count comes from the device
     const bool open = sdf_gate_open(gv, fd->n, gate);
     if (hostFc && blockIdx.x == 0 && threadIdx.x == 0) {
@@ -1190,14 +1199,21 @@ __global__ __launch_bounds__(512) void k_sdf(const uint32_t *__restrict__ queue,
     const SdfParams P = fd->sp;
     const uint32_t nQ = gv.w[0].x;
     if (blockIdx.x == 0 && threadIdx.x == 0) fc->sdfRan = 1u;
-    if (blockIdx.x >= nQ) return;
+    // Workgroups go to the 8 XCDs in turn (block b on XCD b % 8); XCD x
+    // takes the x-th eighth of the queue.  The queue is in brick order, and
+    // bricks made together lie near each other, so each XCD's L2 holds the
+    // image lines of its own part of the view instead of all of it.
+    const uint32_t x = blockIdx.x & 7u, per = gridDim.x >> 3;
+    const uint32_t lo = (uint32_t)(((uint64_t)nQ * x) >> 3), hi = (uint32_t)(((uint64_t)nQ * (x + 1)) >> 3);
+    uint32_t b = lo + (blockIdx.x >> 3);
+    if (b >= hi) return;
+    const uint32_t leaf0 = queue[b];
     __shared__ uint32_t tab[2048];
 #pragma unroll
     for (int k = 0; k < 2048 / 512; k++) tab[threadIdx.x + 512 * k] = tv[k];
     __syncthreads();  // rcp table ready
     sdf_voxel(leaf0, threadIdx.x, m, depth, bgr, W, H, P, tab);
-    for (uint32_t b = blockIdx.x + gridDim.x; b < nQ; b += gridDim.x)
-        sdf_voxel(queue[b], threadIdx.x, m, depth, bgr, W, H, P, tab);
+    for (b += per; b < hi; b += per) sdf_voxel(queue[b], threadIdx.x, m, depth, bgr, W, H, P, tab);
 }
 
 // ---------------------------------------------------------------------------
@@ -1624,6 +1640,14 @@ ff_ctx *ff_create(const ff_params *prm)
         c->evMark = c->evMc = nullptr;
         (void)hipGetLastError();
     }
+    if (c->side && (hipStreamCreate(&c->aux) != hipSuccess ||
+                    hipEventCreateWithFlags(&c->evConf, hipEventDisableTiming) != hipSuccess)) {
+        // the confidence then runs on the side stream with the frame
+        if (c->aux) hipStreamDestroy(c->aux);
+        c->aux = nullptr;
+        c->evConf = nullptr;
+        (void)hipGetLastError();
+    }
     const size_t NL = c->nLeavesTotal;
     bool ok = hipMalloc(&c->d_tree, sizeof(uint32_t) * c->nBranchesTotal) == hipSuccess &&
               hipMalloc(&c->d_treeTmp, sizeof(uint32_t) * c->nBranchesTotal) == hipSuccess &&
@@ -1705,6 +1729,11 @@ void ff_destroy(ff_ctx *c)
     }
     if (c->evMark) hipEventDestroy(c->evMark);
     if (c->evMc) hipEventDestroy(c->evMc);
+    if (c->aux) {
+        hipStreamSynchronize(c->aux);
+        hipStreamDestroy(c->aux);
+    }
+    if (c->evConf) hipEventDestroy(c->evConf);
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;
 }
@@ -1903,11 +1932,26 @@ static int run_frame(ff_ctx *c, int W, int H, bool overlap)
         // update has passed over the tree and the stamps, the SDF update once
         // its marching cubes (which read the bricks) are done; the update's
         // hash turnover may still run beside it
+        // the learned confidence reads only the image: it starts at once on
+        // the third stream, beside the de-dup and the descent
+        const bool confAside = c->learned && c->aux && c->evConf;
+        if (confAside) {
+            if (c->uploadOnSide) {
+                // the frame's host images go up on the side stream first
+                HIPCHK(hipEventRecord(c->evConf, c->side));
+                HIPCHK(hipStreamWaitEvent(c->aux, c->evConf, 0));
+            }
+            learned_launch_direct(c, W, H, c->aux);
+            HIPCHK(hipEventRecord(c->evConf, c->aux));
+        }
         launch_front(c, W, H, c->side, true);  // carries the frame's parameters up
         if (c->markPending) HIPCHK(hipStreamWaitEvent(c->side, c->evMark, 0));
         c->markPending = false;
         launch_back(c, W, H, c->side);
-        if (c->learned) learned_launch(c, W, H, c->side);
+        if (c->learned) {
+            if (confAside) HIPCHK(hipStreamWaitEvent(c->side, c->evConf, 0));
+            else learned_launch(c, W, H, c->side);
+        }
         if (c->mcPending) HIPCHK(hipStreamWaitEvent(c->side, c->evMc, 0));
         c->mcPending = false;
         launch_sdf_only(c, W, H, true, c->side);  // which leaves the counters in h_fc
@@ -2158,11 +2202,14 @@ int ff_add_map(ff_ctx *c, const uint16_t *depth, size_t depth_stride, const uint
     // the images go up on the stream that first reads them
     const bool overlap = front_overlap(c, W, H);
     hipStream_t us = overlap ? c->side : c->stream;
+    c->uploadOnSide = overlap;
     HIPCHK(hipMemcpy2DAsync(c->d_depth, (size_t)W * 2, depth, depth_stride, (size_t)W * 2, H, hipMemcpyHostToDevice,
                             us));
     if (c->useColor)
         HIPCHK(hipMemcpy2DAsync(c->d_bgr, (size_t)W * 3, bgr, bgr_stride, (size_t)W * 3, H, hipMemcpyHostToDevice, us));
-    return add_map_impl(c, c->d_depth, c->useColor ? c->d_bgr : nullptr, W, H, R, t, K, scaling, maxcam, overlap);
+    rc = add_map_impl(c, c->d_depth, c->useColor ? c->d_bgr : nullptr, W, H, R, t, K, scaling, maxcam, overlap);
+    c->uploadOnSide = false;
+    return rc;
 }
 
 int ff_add_map_device(ff_ctx *c, const uint16_t *depth_dev, const uint8_t *bgr_dev, int W, int H, const double R[9],

@@ -149,6 +149,11 @@ struct ff_ctx {
     // for evMc, recorded after its marching cubes
     hipStream_t side = nullptr;
     hipEvent_t evMark = nullptr, evMc = nullptr;
+    // the learned confidence of an overlapped frame runs on `aux`, beside the
+    // frame's de-dup and descent; its SDF update waits for evConf
+    hipStream_t aux = nullptr;
+    hipEvent_t evConf = nullptr;
+    bool uploadOnSide = false;  // the frame in flight had its host images copied up on `side`
     bool markPending = false, mcPending = false;  // recorded since the last frame
     // map arrays
     uint32_t *d_tree = nullptr, *d_treeTmp = nullptr, *d_parent = nullptr, *d_stamp = nullptr, *d_queue = nullptr;
@@ -219,6 +224,7 @@ namespace ff {
 void learned_free(ff_ctx *c);
 const float *learned_prepare(ff_ctx *c, int W, int H);
 void learned_launch(ff_ctx *c, int W, int H, hipStream_t st);
+void learned_launch_direct(ff_ctx *c, int W, int H, hipStream_t st);
 void learned_swap_conf(ff_ctx *c, float *buf, size_t cap, float **old, size_t *oldCap);
 void sync_map_counters(ff_ctx *c);
 void drop_frame_graphs(ff_ctx *c);

@@ -262,12 +262,10 @@ __device__ float feature(const uint16_t *__restrict__ depth, int W, int H, int x
 #define CONF_WIN_W 20
 #define CONF_WIN 100
 
-__global__ __launch_bounds__(256) void k_confidence(const FrameDev *__restrict__ fd, int W, int H,
-                                                    const bf16x8 *__restrict__ frag, const float *__restrict__ bias,
-                                                    const float *__restrict__ w4, float *__restrict__ conf)
+__device__ __forceinline__ void confidence_body(const uint16_t *__restrict__ depth, const FeatCam K, int W, int H,
+                                                const bf16x8 *__restrict__ frag, const float *__restrict__ bias,
+                                                const float *__restrict__ w4, float *__restrict__ conf)
 {
-    const uint16_t *__restrict__ depth = fd->depth;
-    const FeatCam K{fd->sp.fx, fd->sp.fy, fd->sp.cx, fd->sp.cy, fd->scaling, fd->maxcam, fd->ifx, fd->ify};
     __shared__ float sb[3 * 64 + 1], sw4[64];
     __shared__ float win[4][CONF_WIN + 4];
     for (int i = threadIdx.x; i < 3 * 64 + 1; i += blockDim.x) sb[i] = bias[i];
@@ -311,6 +309,24 @@ __global__ __launch_bounds__(256) void k_confidence(const FrameDev *__restrict__
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
     }
+}
+
+// the frame's values from the device frame block (a captured frame replays it)
+__global__ __launch_bounds__(256) void k_confidence(const FrameDev *__restrict__ fd, int W, int H,
+                                                    const bf16x8 *__restrict__ frag, const float *__restrict__ bias,
+                                                    const float *__restrict__ w4, float *__restrict__ conf)
+{
+    const FeatCam K{fd->sp.fx, fd->sp.fy, fd->sp.cx, fd->sp.cy, fd->scaling, fd->maxcam, fd->ifx, fd->ify};
+    confidence_body(fd->depth, K, W, H, frag, bias, w4, conf);
+}
+
+// the frame's values as arguments (direct launches beside the frame's
+// allocation, before its frame block is on the device)
+__global__ __launch_bounds__(256) void k_confidence_v(const uint16_t *__restrict__ depth, FeatCam K, int W, int H,
+                                                      const bf16x8 *__restrict__ frag, const float *__restrict__ bias,
+                                                      const float *__restrict__ w4, float *__restrict__ conf)
+{
+    confidence_body(depth, K, W, H, frag, bias, w4, conf);
 }
 
 // ---------------------------------------------------------------------------
@@ -694,6 +710,17 @@ void learned_launch(ff_ctx *c, int W, int H, hipStream_t st)
     const int tiles = (W + 15) / 16 * H;
     const int blocks = std::min(2048, (tiles + 3) / 4);
     k_confidence<<<blocks, 256, 0, st>>>(c->d_fd, W, H, L->w.frag, L->w.bias, L->w.w4, L->conf);
+}
+
+// the same with the frame's values from the host frame block (c->h_fd)
+void learned_launch_direct(ff_ctx *c, int W, int H, hipStream_t st)
+{
+    LearnedState *L = c->learned;
+    const FrameDev &f = *c->h_fd;
+    const FeatCam K{f.sp.fx, f.sp.fy, f.sp.cx, f.sp.cy, f.scaling, f.maxcam, f.ifx, f.ify};
+    const int tiles = (W + 15) / 16 * H;
+    const int blocks = std::min(2048, (tiles + 3) / 4);
+    k_confidence_v<<<blocks, 256, 0, st>>>(f.depth, K, W, H, L->w.frag, L->w.bias, L->w.w4, L->conf);
 }
 
 }  // namespace ff

@@ -24,14 +24,19 @@ def _frame(seed=3, w=320, h=240):
     return sc, R, t, K, fr
 
 
-def test_confidence_matches_oracle():
+@pytest.mark.parametrize("frame", [0, 1, 3])
+def test_confidence_matches_oracle(frame):
+    """Per-pixel confidences of the first frame (run with the frame) and of
+    later ones (run beside the frame's allocation, after its images are up)."""
     sc, R, t, K, fr = _frame()
     p = L.init_params(11)
     p[L.OFF["b4"]] = 0.3
     fz = FusionMipMapGPU(0, 0, 0, 0.01, 0.03, 0, True, max_leaves=1 << 16)
     fz.setLearnedWeighting(p)
-    d, bgr = fr[0]
-    fz.addMap(d, R[0], t[0], K, bgr, 1 / 5000.0, 4.0)
+    for i in range(frame + 1):
+        d, bgr = fr[i]
+        fz.addMap(d, R[i], t[i], K, bgr, 1 / 5000.0, 4.0)
+        fz.updateMeshes()
     c_gpu = fz.confidence(d.shape[1], d.shape[0])
     F, valid = L.features(d, K, 1 / 5000.0, 4.0)
     c_ora = np.zeros(d.shape, np.float32)
