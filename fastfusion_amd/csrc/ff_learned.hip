@@ -259,52 +259,132 @@ __device__ float feature(const uint16_t *__restrict__ depth, int W, int H, int x
 // staged in the wave's LDS slice, every lane of a pixel (one per 8-feature
 // group) computes the same incidence, and feature selection is branch-free.
 // ---------------------------------------------------------------------------
-#define CONF_WIN_W 20
-#define CONF_WIN 100
+// A wave takes 64 pixels of a row per step: the 5 x 68 depth window they
+// need is staged in the wave's LDS slice; each lane computes its own pixel's
+// centre depth, validity, reciprocal and incidence once, and then, for each
+// of the four 16-pixel MFMA tiles, its 8 features of one pixel of the tile
+// (feature selection branch-free).  The network runs once per tile.
+#define CONF_WIN_W 68
+#define CONF_WIN (5 * CONF_WIN_W)
+
+// inference forward of mlp_forward (same operations, same order): the
+// activations go straight to the next layer's bf16 operand, and only the
+// confidence is kept
+__device__ __forceinline__ float mlp_infer(const MlpW &W, const bf16x8 &bx, const float *__restrict__ bias,
+                                           const float *__restrict__ w4, int g)
+{
+    bf16x8 b[2];
+    f32x4 acc[4];
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+        acc[t] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+        acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(W.a1[t], bx, acc[t], 0, 0, 0);
+    }
+#pragma unroll
+    for (int s = 0; s < 2; s++)
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+            const int t = 2 * s + (q >> 2), r = q & 3;
+            b[s][q] = (__bf16)relu(acc[t][r] + bias[16 * t + 4 * g + r]);
+        }
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+        acc[t] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int s = 0; s < 2; s++)
+            acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(W.a2[t][s], b[s], acc[t], 0, 0, 0);
+    }
+#pragma unroll
+    for (int s = 0; s < 2; s++)
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+            const int t = 2 * s + (q >> 2), r = q & 3;
+            b[s][q] = (__bf16)relu(acc[t][r] + bias[64 + 16 * t + 4 * g + r]);
+        }
+    float part = 0.0f;
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+        f32x4 a = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int s = 0; s < 2; s++) a = __builtin_amdgcn_mfma_f32_16x16x32_bf16(W.a3[t][s], b[s], a, 0, 0, 0);
+#pragma unroll
+        for (int r = 0; r < 4; r++)
+            part += w4[16 * t + 4 * g + r] * (float)(__bf16)relu(a[r] + bias[128 + 16 * t + 4 * g + r]);
+    }
+    part += __shfl_xor(part, 16);
+    part += __shfl_xor(part, 32);
+    const float o = part + bias[192];
+    return 1.0f / (1.0f + expf(-o));
+}
+
+struct ConfWave {
+    float z[CONF_WIN];
+    float zc[64], rc[64], inc[64];
+};
 
 __device__ __forceinline__ void confidence_body(const uint16_t *__restrict__ depth, const FeatCam K, int W, int H,
                                                 const bf16x8 *__restrict__ frag, const float *__restrict__ bias,
                                                 const float *__restrict__ w4, float *__restrict__ conf)
 {
     __shared__ float sb[3 * 64 + 1], sw4[64];
-    __shared__ float win[4][CONF_WIN + 4];
+    __shared__ ConfWave cw[4];
     for (int i = threadIdx.x; i < 3 * 64 + 1; i += blockDim.x) sb[i] = bias[i];
     for (int i = threadIdx.x; i < 64; i += blockDim.x) sw4[i] = w4[i];
     __syncthreads();
     const int lane = threadIdx.x & 63, g = lane >> 4, p = lane & 15;
-    float *zt = win[threadIdx.x >> 6];
+    ConfWave &C = cw[threadIdx.x >> 6];
     MlpW Wt;
     load_forward(Wt, frag, lane);
-    const int tilesPerRow = (W + 15) / 16;
-    const int nTiles = tilesPerRow * H;
+    const int segsPerRow = (W + 63) / 64;
+    const int nSegs = segsPerRow * H;
     const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6, nWaves = (gridDim.x * blockDim.x) >> 6;
-    for (int tile = wave; tile < nTiles; tile += nWaves) {
-        const int y = tile / tilesPerRow, x0 = (tile % tilesPerRow) * 16, x = x0 + p;
-        // window rows y-2..y+2, columns x0-2..x0+17
-        for (int i = lane; i < CONF_WIN; i += 64)
-            zt[i] = depth_at(depth, W, H, x0 - 2 + i % CONF_WIN_W, y - 2 + i / CONF_WIN_W, K.scaling);
+    for (int seg = wave; seg < nSegs; seg += nWaves) {
+        const int y = seg / segsPerRow, x0 = (seg % segsPerRow) * 64;
+        // window rows y-2..y+2, columns x0-2..x0+65
+#pragma unroll
+        for (int k = 0; k < (CONF_WIN + 63) / 64; k++) {
+            const int i = lane + 64 * k;
+            if (i < CONF_WIN) C.z[i] = depth_at(depth, W, H, x0 - 2 + i % CONF_WIN_W, y - 2 + i / CONF_WIN_W, K.scaling);
+        }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        const bool inImg = x < W;
-        const float zc = zt[2 * CONF_WIN_W + p + 2];
-        const bool valid = inImg && zc > 0.0f && zc < K.maxcam;
-        const float rc = __frcp_rn(valid ? zc : 1.0f);
-        const float inc = incidence_cos(zt[2 * CONF_WIN_W + p + 1], zt[2 * CONF_WIN_W + p + 3],
-                                        zt[CONF_WIN_W + p + 2], zt[3 * CONF_WIN_W + p + 2], zc, x, y, K);
-        bf16x8 bx;
-#pragma unroll
-        for (int q = 0; q < 8; q++) {
-            const int k = 8 * g + q, kk = k < 25 ? k : 0;
-            const float fp = patch_feature(zt[(kk / 5) * CONF_WIN_W + p + kk % 5], rc);
-            const float f = k < 25 ? fp : (k == 25 ? __fmul_rn(zc, 0.25f) : (k == 26 ? inc : 0.0f));
-            bx[q] = (__bf16)(valid ? f : 0.0f);
+        {
+            // this lane's own pixel x0 + lane
+            const int x = x0 + lane;
+            const float zc = C.z[2 * CONF_WIN_W + lane + 2];
+            const bool valid = x < W && zc > 0.0f && zc < K.maxcam;
+            C.zc[lane] = valid ? zc : -1.0f;  // -1: not a valid pixel
+            C.rc[lane] = __frcp_rn(valid ? zc : 1.0f);
+            C.inc[lane] = incidence_cos(C.z[2 * CONF_WIN_W + lane + 1], C.z[2 * CONF_WIN_W + lane + 3],
+                                        C.z[CONF_WIN_W + lane + 2], C.z[3 * CONF_WIN_W + lane + 2], zc, x, y, K);
         }
-        Fwd F;
-        mlp_forward(Wt, bx, sb, sw4, lane, F);
-        // a pixel whose predicted residual is beyond every truncation band
-        // (confidence under the gate) is not fused at all
-        if (g == 0 && inImg) conf[(size_t)y * W + x] = (valid && F.c >= FF_MLP_GATE) ? F.c : 0.0f;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll 1
+        for (int j = 0; j < 4; j++) {
+            const int px = 16 * j + p;  // the tile's pixel of this lane
+            const float zc = C.zc[px], rc = C.rc[px], inc = C.inc[px];
+            const bool valid = zc >= 0.0f;
+            bf16x8 bx;
+#pragma unroll
+            for (int q = 0; q < 8; q++) {
+                const int k = 8 * g + q, kk = k < 25 ? k : 0;
+                const float fp = patch_feature(C.z[(kk / 5) * CONF_WIN_W + px + kk % 5], rc);
+                const float f = k < 25 ? fp : (k == 25 ? __fmul_rn(zc, 0.25f) : (k == 26 ? inc : 0.0f));
+                bx[q] = (__bf16)(valid ? f : 0.0f);
+            }
+            // the biases are re-read from LDS for every tile (held in
+            // registers across tiles they would cost the occupancy)
+            const float *bsb = sb, *bsw = sw4;
+            asm volatile("" : "+v"(bsb), "+v"(bsw));
+            const float cc = mlp_infer(Wt, bx, bsb, bsw, g);
+            // a pixel whose predicted residual is beyond every truncation band
+            // (confidence under the gate) is not fused at all
+            const int x = x0 + px;
+            if (g == 0 && x < W) conf[(size_t)y * W + x] = (valid && cc >= FF_MLP_GATE) ? cc : 0.0f;
+        }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -312,7 +392,7 @@ __device__ __forceinline__ void confidence_body(const uint16_t *__restrict__ dep
 }
 
 // the frame's values from the device frame block (a captured frame replays it)
-__global__ __launch_bounds__(256) void k_confidence(const FrameDev *__restrict__ fd, int W, int H,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_confidence(const FrameDev *__restrict__ fd, int W, int H,
                                                     const bf16x8 *__restrict__ frag, const float *__restrict__ bias,
                                                     const float *__restrict__ w4, float *__restrict__ conf)
 {
@@ -322,7 +402,7 @@ __global__ __launch_bounds__(256) void k_confidence(const FrameDev *__restrict__
 
 // the frame's values as arguments (direct launches beside the frame's
 // allocation, before its frame block is on the device)
-__global__ __launch_bounds__(256) void k_confidence_v(const uint16_t *__restrict__ depth, FeatCam K, int W, int H,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k_confidence_v(const uint16_t *__restrict__ depth, FeatCam K, int W, int H,
                                                       const bf16x8 *__restrict__ frag, const float *__restrict__ bias,
                                                       const float *__restrict__ w4, float *__restrict__ conf)
 {
@@ -707,8 +787,8 @@ void learned_swap_conf(ff_ctx *c, float *buf, size_t cap, float **old, size_t *o
 void learned_launch(ff_ctx *c, int W, int H, hipStream_t st)
 {
     LearnedState *L = c->learned;
-    const int tiles = (W + 15) / 16 * H;
-    const int blocks = std::min(2048, (tiles + 3) / 4);
+    const int segs = (W + 63) / 64 * H;
+    const int blocks = std::min(2048, (segs + 3) / 4);
     k_confidence<<<blocks, 256, 0, st>>>(c->d_fd, W, H, L->w.frag, L->w.bias, L->w.w4, L->conf);
 }
 
@@ -718,8 +798,8 @@ void learned_launch_direct(ff_ctx *c, int W, int H, hipStream_t st)
     LearnedState *L = c->learned;
     const FrameDev &f = *c->h_fd;
     const FeatCam K{f.sp.fx, f.sp.fy, f.sp.cx, f.sp.cy, f.scaling, f.maxcam, f.ifx, f.ify};
-    const int tiles = (W + 15) / 16 * H;
-    const int blocks = std::min(2048, (tiles + 3) / 4);
+    const int segs = (W + 63) / 64 * H;
+    const int blocks = std::min(2048, (segs + 3) / 4);
     k_confidence_v<<<blocks, 256, 0, st>>>(f.depth, K, W, H, L->w.frag, L->w.bias, L->w.w4, L->conf);
 }
 
