@@ -530,12 +530,13 @@ __global__ void k_descend_b(const AllocStream *__restrict__ S)
 #define FF_ORDER_MAX 8192   // nodes ranked in LDS (64 KiB of keys)
 #define FF_ORDER_THREADS 1024
 
-__device__ __forceinline__ void order_body(const HashView &hv, FrameCounters *fc, const MapCounters *__restrict__ mapc)
+// key: FF_ORDER_MAX words of LDS
+__device__ __forceinline__ void order_body(const HashView &hv, FrameCounters *fc, const MapCounters *__restrict__ mapc,
+                                           uint64_t *key)
 {
     // key: type (63) | creating pixel (29..50, < 2^22) | depth (24..28) | list
     // index (0..12): sorted ascending, branches come first in (pixel,
     // depth) order, then bricks in pixel order
-    __shared__ uint64_t key[FF_ORDER_MAX];
     __shared__ uint32_t nBr;
     const uint32_t n = min(fc->listCount, hv.listCap);
     if (n > FF_ORDER_MAX) {
@@ -589,13 +590,15 @@ __device__ __forceinline__ void order_body(const HashView &hv, FrameCounters *fc
 __global__ __launch_bounds__(FF_ORDER_THREADS) void k_order(HashView hv, FrameCounters *fc,
                                                         const MapCounters *__restrict__ mapc)
 {
-    order_body(hv, fc, mapc);
+    __shared__ uint64_t key[FF_ORDER_MAX];
+    order_body(hv, fc, mapc, key);
 }
 
 __global__ __launch_bounds__(FF_ORDER_THREADS) void k_order_b(const AllocStream *__restrict__ S)
 {
     const AllocStream &a = S[blockIdx.z];
-    order_body(a.hv, a.fc, a.mapc);
+    __shared__ uint64_t key[FF_ORDER_MAX];
+    order_body(a.hv, a.fc, a.mapc, key);
 }
 
 // per new node: its creating pixel counts it
@@ -929,6 +932,66 @@ __global__ __launch_bounds__(FF_QUEUE_THREADS) void k_queue_b(const AllocStream 
     queue_body(a.m.stamp, a.mapc, a.m.nLeavesTotal, a.fd, a.m.queue, a.fc, false);
 }
 
+// The queue of a frame built by one block, ordered by the image row band
+// its bricks project to (a counting sort over FF_QBANDS bands of the image
+// height, by each brick centre's projection with the frame's camera): the
+// SDF update hands contiguous eighths of the queue to the 8 XCDs, which then
+// each read one horizontal strip of the image.  Order within a band is
+// whatever the block's atomics give; the update does not depend on it.
+#define FF_QBANDS 32
+
+__device__ __forceinline__ uint32_t queue_band(const ushort4 li, const FrameDev *__restrict__ fd, int H)
+{
+    const SdfParams &P = fd->sp;
+    const float h = 4.0f * (float)li.w;  // the brick's centre, in voxels
+    const float c[3] = {(float)li.x + h, (float)li.y + h, (float)li.z + h};
+    float q[3];
+#pragma unroll
+    for (int r = 0; r < 3; r++)
+        q[r] = (P.m[4 * r] * c[0] + P.m[4 * r + 1] * c[1] + P.m[4 * r + 2] * c[2]) * P.scale + P.m[4 * r + 3];
+    if (!(q[2] > 0.0f)) return 0u;
+    const float v = P.cy + P.fy * (q[1] / q[2]);
+    const int b = (int)(v * (float)FF_QBANDS / (float)H);
+    return (uint32_t)min(max(b, 0), FF_QBANDS - 1);
+}
+
+// the frame's queue (built by queue_body in brick order) reordered by band:
+// entries and bands staged in `stage` (at least FF_ORDER_MAX words of LDS),
+// a counting sort back into the queue; a queue longer than the stage stays
+// in brick order
+__device__ __forceinline__ void queue_band_sort(const ushort4 *__restrict__ info, const FrameDev *__restrict__ fd,
+                                                int H, uint32_t *queue, FrameCounters *fc, uint64_t *stage)
+{
+    __shared__ uint32_t cnt[FF_QBANDS];
+    // the count was taken up by an atomic and the entries stored by other
+    // waves: both read past the vector L1
+    const uint32_t n = __hip_atomic_load(&fc->nQueued, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const uint32_t t = threadIdx.x;
+    if (n > FF_ORDER_MAX) return;  // (uniform)
+    if (t < FF_QBANDS) cnt[t] = 0u;
+    __syncthreads();
+    for (uint32_t j = t; j < n; j += blockDim.x) {
+        const uint32_t i = __hip_atomic_load(&queue[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const uint32_t band = queue_band(info[i], fd, H);
+        stage[j] = ((uint64_t)band << 32) | i;
+        atomicAdd(&cnt[band], 1u);
+    }
+    __syncthreads();
+    if (t == 0) {
+        uint32_t tot = 0;
+        for (int b = 0; b < FF_QBANDS; b++) {
+            const uint32_t c = cnt[b];
+            cnt[b] = tot;
+            tot += c;
+        }
+    }
+    __syncthreads();
+    for (uint32_t j = t; j < n; j += blockDim.x) {
+        const uint64_t e = stage[j];
+        queue[atomicAdd(&cnt[(uint32_t)(e >> 32)], 1u)] = (uint32_t)e;
+    }
+}
+
 // blocks of k_queue for a map of nLeavesUsed bricks (its grid-stride loop
 // covers any count)
 static inline uint32_t queue_grid(uint32_t nLeavesUsed)
@@ -949,10 +1012,11 @@ static inline uint32_t queue_grid(uint32_t nLeavesUsed)
 __global__ __launch_bounds__(FF_ORDER_THREADS) void k_alloc_tail(HashView hv, MapView m, FrameCounters *fc,
                                                              const uint32_t *__restrict__ pl0,
                                                              const FrameDev *__restrict__ fd, MapCounters *mapc,
-                                                             ushort4 *binfo, uint32_t binfoBlocks, int queue)
+                                                             ushort4 *binfo, uint32_t binfoBlocks, int queue, int H)
 {
     __shared__ uint32_t big;
-    order_body(hv, fc, mapc);
+    __shared__ uint64_t key[FF_ORDER_MAX];  // the ordering's sort keys, then the queue's band sort
+    order_body(hv, fc, mapc, key);
     __syncthreads();
     if (threadIdx.x == 0)
         big = __hip_atomic_load(&fc->flags, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & FF_FLAG_ORDER_BIG;
@@ -964,6 +1028,8 @@ __global__ __launch_bounds__(FF_ORDER_THREADS) void k_alloc_tail(HashView hv, Ma
     if (!queue) return;
     __syncthreads();  // the used counts are committed
     queue_body(m.stamp, mapc, m.nLeavesTotal, fd, m.queue, fc, true);
+    __syncthreads();  // the queue and its count are written
+    queue_band_sort(m.leafinfo, fd, H, m.queue, fc, key);
 }
 
 // ---------------------------------------------------------------------------
@@ -1787,7 +1853,7 @@ void launch_back(ff_ctx *c, int W, int H, hipStream_t st)
     // this frame's new bricks) is a few spans
     const bool small = (uint64_t)c->nLeavesUsed + FF_ORDER_MAX <= (uint64_t)FF_TAIL_SPANS * FF_QUEUE_SPAN;
     k_alloc_tail<<<1, FF_ORDER_THREADS, 0, st>>>(c->hv, mv, c->d_fc, c->d_pl0, fd, c->d_mapc, binfo, binfoBlocks,
-                                                 small ? 1 : 0);
+                                                 small ? 1 : 0, H);
     if (!small)
         k_queue<<<queue_grid(c->nLeavesUsed), FF_QUEUE_THREADS, 0, st>>>(c->d_stamp, c->d_mapc, c->nLeavesTotal, fd,
                                                                          c->d_queue, c->d_fc, 0);
