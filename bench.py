@@ -353,6 +353,7 @@ def pmc_summary(args):
     final mesh that follows them."""
     import csv
     import collections
+    import re
 
     def load(path):
         rows = []
@@ -362,7 +363,7 @@ def pmc_summary(args):
         rows.sort()
         return rows
 
-    n_walk = max(1000, args.warmup + args.steps)
+    n_walk = max(args.seq_frames or (3000 if args.large else 1000), args.warmup + args.steps)
     first, last = n_walk - args.steps, n_walk
     out = {}
     for tag, path in (("fetch", args.pmc_summary[0]), ("write", args.pmc_summary[1])):
@@ -375,6 +376,9 @@ def pmc_summary(args):
         done = False
         for _, k, v in rows:
             name = k.split("(")[0].replace("void ", "").replace("ff::", "").split("<")[0].strip()
+            mg = re.match(r"_ZN2ff(\d+)", name)  # a name the profiler left mangled
+            if mg:
+                name = name[mg.end():mg.end() + int(mg.group(1))]
             if name in ("k_planes", "k_planes_v"):
                 frame += 1
             # the final mesh's gather ends the timed run (the replay follows)
@@ -395,7 +399,7 @@ def pmc_summary(args):
         rb, n = f["kernels"].get(k, (0.0, 0))
         wb, _ = w["kernels"].get(k, (0.0, 0))
         print(f"| {k} | {n} | {rb:,.0f} | {wb:,.0f} |")
-    key = "learned" if args.learned else "plain"
+    key = "learned" if args.learned or args.large else "plain"
     ks = {k: f["kernels"].get(k, (0.0, 0))[0] + w["kernels"].get(k, (0.0, 0))[0] for k in ("k_sdf", "k_mc")}
     print(f"per frame: read {f['frame']:,.0f} B, written {w['frame']:,.0f} B")
     print(f"bench-traffic: {key} frame={f['frame'] + w['frame']:.0f} k_sdf={ks['k_sdf']:.0f} k_mc={ks['k_mc']:.0f}")
