@@ -271,6 +271,7 @@ struct DedupLoad {
     static constexpr int PER_BAND = FF_DD_R * WMAX;
     static constexpr int PT = (PER_BAND + FF_DD_THREADS - 65) / (FF_DD_THREADS - 64);
     uint4 v[PT];
+    uint32_t in;  // bit j: entry j lies in the image (the others put zeros)
 };
 
 // diagonal band `band` of the planes: entry e = kk * WMAX + w holds the
@@ -279,14 +280,19 @@ template <int WMAX>
 __device__ __forceinline__ void dedup_fetch(DedupLoad<WMAX> &L, const uint4 *__restrict__ planes, int H, int words,
                                             int band, int lt)
 {
+    // every loader issues its PT loads unconditionally (an entry outside the
+    // image reads row 0, word 0 and is zeroed when it is put), so that the
+    // put of the older register set waits for its own loads alone: loads
+    // behind a branch make the wait before it cover every load in flight
+    L.in = 0u;
 #pragma unroll
     for (int j = 0; j < DedupLoad<WMAX>::PT; j++) {
         const int e = lt + j * (FF_DD_THREADS - 64);
         const int kk = e / WMAX, w = e - kk * WMAX;
         const int row = band * FF_DD_R + kk - w;
-        L.v[j] = make_uint4(0, 0, 0, 0);
-        if (e < DedupLoad<WMAX>::PER_BAND && w < words && row >= 0 && row < H)
-            L.v[j] = planes[(size_t)row * words + w];
+        const bool in = e < DedupLoad<WMAX>::PER_BAND && w < words && row >= 0 && row < H;
+        L.v[j] = planes[in ? (size_t)row * words + w : 0];
+        L.in |= (uint32_t)in << j;
     }
 }
 
@@ -299,7 +305,9 @@ __device__ __forceinline__ void dedup_put(const DedupLoad<WMAX> &L, uint4 *ring,
         const int e = lt + j * (FF_DD_THREADS - 64);
         if (e < DedupLoad<WMAX>::PER_BAND) {
             const int kk = e / WMAX, w = e - kk * WMAX;
-            ring[((band * FF_DD_R + kk) & (RING - 1)) * WMAX + w] = L.v[j];
+            const uint32_t m = 0u - ((L.in >> j) & 1u);  // all ones for an entry in the image
+            const uint4 v = L.v[j];
+            ring[((band * FF_DD_R + kk) & (RING - 1)) * WMAX + w] = make_uint4(v.x & m, v.y & m, v.z & m, v.w & m);
         }
     }
 }
@@ -312,7 +320,7 @@ __device__ __forceinline__ void dedup_body(uint4 *ring, const uint32_t *__restri
     constexpr uint32_t EVEN = 0x55555555u;
     const uint4 *planes = reinterpret_cast<const uint4 *>(planes32);
     const int tid = threadIdx.x;
-    const bool compute = tid < 64;
+    const bool compute = __builtin_amdgcn_readfirstlane(tid >> 6) == 0;  // wave 0 (a wave-uniform branch)
     const int nsteps = H + words - 1;
     const int niter = (nsteps + R - 1) / R;
     const int lt = tid - 64;
@@ -333,8 +341,22 @@ __device__ __forceinline__ void dedup_body(uint4 *ring, const uint32_t *__restri
     const int lw = l < WMAX ? l : WMAX - 1;  // lanes past the ring's columns read a column they ignore
     uint32_t *out = pbits + (size_t)(FF_PB_PAD - l) * FF_PB_STRIDE + l;  // row 0 - l: the lane's row at step 0
     uint32_t pprev = 0u, pin = 0u, nproc = 0u;
-    for (int it = 0; it < niter; it++) {
-        if (compute) {
+    // The computing wave and the loaders run separate loops with one barrier
+    // per iteration each, niter rounded up to even in both (the computing
+    // wave skips the extra one), so that the loaders' loop has no
+    // conditional half.  Kept apart and unconditional, each loop's memory
+    // waits count its own operations: the loaders wait for the older band's
+    // loads only, and a fetch has two iterations to arrive.  (In one shared
+    // loop the waits also covered the computing wave's stores and came out
+    // as waits for every load in flight: a memory round trip per iteration.)
+    const int niter2 = (niter + 1) & ~1;
+    if (compute) {
+        for (int it = 0; it < niter2; it++) {
+            if (it == niter) {
+                __syncthreads();
+                break;
+            }
+            // R diagonals
             const uint4 *q = ring + ((it * R) & (RING - 1)) * WMAX + lw;
             uint4 rq[R];
 #pragma unroll
@@ -357,18 +379,21 @@ __device__ __forceinline__ void dedup_body(uint4 *ring, const uint32_t *__restri
                 pprev = P;
                 pin = (uint32_t)__builtin_amdgcn_mov_dpp((int)P, 0x138, 0xf, 0xf, true);  // wave_shr:1, lane 0 gets 0
             }
-        } else {
-            // band it + 3 into the slot band it - 1 was read from; band it + 5
-            // into the registers it came from
-            if ((it + 3) & 1) {
-                dedup_put<WMAX>(pre[1], ring, it + 3, lt);
-                dedup_fetch<WMAX>(pre[1], planes, H, words, it + 5, lt);
-            } else {
-                dedup_put<WMAX>(pre[0], ring, it + 3, lt);
-                dedup_fetch<WMAX>(pre[0], planes, H, words, it + 5, lt);
-            }
+            __syncthreads();
         }
-        __syncthreads();
+    } else {
+        // band it + 3 into the ring (into the slot band it - 1 was read
+        // from), then band it + 5 fetched into the registers it came from;
+        // unrolled by two so that each register set keeps its registers
+        // (band it + 3 is in pre[1] for even it, in pre[0] for odd it)
+        for (int it = 0; it < niter2; it += 2) {
+            dedup_put<WMAX>(pre[1], ring, it + 3, lt);
+            dedup_fetch<WMAX>(pre[1], planes, H, words, it + 5, lt);
+            __syncthreads();
+            dedup_put<WMAX>(pre[0], ring, it + 4, lt);
+            dedup_fetch<WMAX>(pre[0], planes, H, words, it + 6, lt);
+            __syncthreads();
+        }
     }
     if (compute) {
         if (l >= words) nproc = 0u;
