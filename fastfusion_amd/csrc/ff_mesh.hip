@@ -12,11 +12,13 @@
 //                the lowest lastLeaf slot); lastLeaf[k] = the finest brick
 //                covering side k (located by descending the tree)
 //   k_mark       a cell keeps last update's mesh when it existed then with
-//                the same bricks and none of the bricks it can sample (its
-//                lastLeafs and their parent chains) changed since; a dirty
-//                cell gets its sampling grid and a run of 64-sample tiles
+//                the same bricks and none of the bricks its samples read
+//                (its lastLeafs and their parent chains up to the level the
+//                samples reached) changed since; a dirty cell gets its
+//                sampling grid and a run of 64-sample tiles
 //   k_sample     one thread per sample of every dirty cell (reference
-//                sampling rules per cell type, parent-chain fallback)
+//                sampling rules per cell type, parent-chain fallback); each
+//                tile raises its cell's chain depth
 //   k_mc         one wave per dirty cell: samples into LDS, cube
 //                classification, a bitmap prefix scan numbers the used edges
 //                in raster order, vertices (double VertexInterp) and faces
@@ -578,16 +580,19 @@ __device__ __forceinline__ uint32_t hkey_hash(uint64_t k, uint32_t mask)
     return (uint32_t)k & mask;
 }
 
-// a brick on the parent chain (at most 16 deep) of any of the bricks ll[]
-// updated after `since`; the chains advance together, so each level's loads
-// are in flight at once
+// a brick among the first `depth` (at most 16) bricks of the parent chain of
+// any of the bricks ll[] updated after `since`; the chains advance together,
+// so each level's loads are in flight at once.  depth is the deepest chain
+// level the cell's samples read when it was last meshed: a brick further up
+// was not read (every sample stopped at a brick with weight), and the bricks
+// below it are unchanged, so the samples still stop there.
 __device__ __forceinline__ bool chains_dirty(const uint32_t *__restrict__ stamp, const uint32_t *__restrict__ parent,
-                                             const uint32_t ll[8], uint32_t since)
+                                             const uint32_t ll[8], uint32_t since, uint32_t depth)
 {
     uint32_t cur[8];
 #pragma unroll
     for (int k = 0; k < 8; k++) cur[k] = ll[k];
-    for (int d = 0; d < 16; d++) {
+    for (int d = 0; d < (int)min(depth, 16u); d++) {
         uint32_t st[8], pa[8];
 #pragma unroll
         for (int k = 0; k < 8; k++)
@@ -615,6 +620,7 @@ __device__ __forceinline__ bool chains_dirty(const uint32_t *__restrict__ stamp,
 struct Sample {
     float d, w;
     uint16_t c[3];
+    uint16_t lv;  // bricks read along the parent chain (1: the lastLeaf alone)
 };
 
 // orders LDS traffic between the lanes of one wave (each wave owns its own
@@ -682,6 +688,7 @@ __device__ void sample_interior(const MapRO &m, uint32_t ll, const int X[3], Sam
             s.c[ch] = f2u16((b0 + b1) + (b2 + b3));
         }
     }
+    s.lv = (uint16_t)guard;
 }
 
 // wall: thin axis a pointwise, u (inner) and v (outer) bilinear (:700-760)
@@ -713,6 +720,7 @@ __device__ void sample_wall(const MapRO &m, uint32_t ll, int a, int u, int v, co
         for (int ch = 0; ch < 3; ch++)
             s.c[ch] = f2u16(rvI * ruI * cf[0][ch] + rvI * ru * cf[1][ch] + rv * ruI * cf[2][ch] + rv * ru * cf[3][ch]);
     }
+    s.lv = (uint16_t)guard;
 }
 
 // edge: axis e linear, weight = max of the two (:1800-1860)
@@ -737,6 +745,7 @@ __device__ void sample_edge(const MapRO &m, uint32_t ll, int e, const int X[3], 
 #pragma unroll
         for (int ch = 0; ch < 3; ch++) s.c[ch] = f2u16((1.0f - re) * c0[ch] + re * c1[ch]);
     }
+    s.lv = (uint16_t)guard;
 }
 
 // corner: pointwise (:2550-2575)
@@ -753,6 +762,7 @@ __device__ void sample_point(const MapRO &m, uint32_t ll, const int X[3], Sample
         s.d = d;
         s.c[0] = (uint16_t)c[0]; s.c[1] = (uint16_t)c[1]; s.c[2] = (uint16_t)c[2];
     }
+    s.lv = (uint16_t)guard;
 }
 
 __device__ __forceinline__ uint32_t wave_excl_scan_u32(uint32_t v, uint32_t *total)
@@ -932,6 +942,7 @@ __device__ Sample sample_cell(const MapRO &m, const CellGeom &g, const uint32_t 
             if (li.x == g.o[0] && li.y == g.o[1] && li.z == g.o[2] && w > 0.0f) {
                 sm.d = d; sm.w = w;
                 sm.c[0] = c0; sm.c[1] = c1; sm.c[2] = c2;
+                sm.lv = 1;
                 return sm;
             }
         }
@@ -990,10 +1001,11 @@ __device__ __forceinline__ void mark_body(CellRec *cells, const CellRec *__restr
                     cl[k] = c.ll[k];
                     clean = clean && (p.ll[k] == cl[k]);
                 }
-                if (clean && chains_dirty(stamp, parent, cl, since)) clean = false;
+                if (clean && chains_dirty(stamp, parent, cl, since, p.pad[1])) clean = false;
                 if (clean) {
                     c.v0 = p.v0; c.nv = p.nv; c.f0 = p.f0; c.nf = p.nf;
                     c.pad[0] = p.pad[0];
+                    c.pad[1] = p.pad[1];
                 }
             }
             isDirty = !clean;
@@ -1004,6 +1016,7 @@ __device__ __forceinline__ void mark_body(CellRec *cells, const CellRec *__restr
                 g.cell = i;
                 g.pad = 0;
                 nt = ((uint32_t)geom_total(g) + 63u) >> 6;
+                c.pad[1] = nt ? 1u : 0u;  // chain depth: raised by the sampling
             }
         }
         const uint32_t d = dirty_slot(mc, isDirty, nt, cap, split);
@@ -1071,7 +1084,7 @@ __global__ void k_mark_same(CellRec *cells, const uint32_t *__restrict__ stamp, 
         if (i < nc) {
             CellRec &c = cells[i];
             for (int k = 0; k < 8; k++) ll[k] = c.ll[k];
-            isDirty = chains_dirty(stamp, parent, ll, since);
+            isDirty = chains_dirty(stamp, parent, ll, since, c.pad[1]);
             if (isDirty) {
                 c.v0 = c.nv = c.f0 = c.nf = 0u;
                 c.pad[0] = updateId;
@@ -1079,6 +1092,7 @@ __global__ void k_mark_same(CellRec *cells, const uint32_t *__restrict__ stamp, 
                 g.cell = i;
                 g.pad = 0;
                 nt = ((uint32_t)geom_total(g) + 63u) >> 6;
+                c.pad[1] = nt ? 1u : 0u;  // chain depth: raised by the sampling
             }
         }
         const uint32_t d = dirty_slot(mc, isDirty, nt, cap, split);
@@ -1171,7 +1185,17 @@ __global__ void k_hash_clear_cells_b(const MeshBatchStream *__restrict__ S)
 // ---------------------------------------------------------------------------
 // k_sample: one thread per sample of the dirty cells' 64-sample tiles
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ void sample_body(const CellGeom *__restrict__ geom, const CellRec *__restrict__ cells,
+// the deepest parent-chain level any sample of a tile read, raised into
+// its cell's record (CellRec::pad[1], set to 1 by the marking)
+__device__ __forceinline__ void raise_depth(CellRec *cells, uint32_t cell, uint32_t lv)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) lv = max(lv, (uint32_t)__shfl_xor((int)lv, o));
+    if ((threadIdx.x & 63u) == 0u && lv > 1u) atomicMax(&cells[cell].pad[1], lv);
+}
+
+// (the grid's stride is a multiple of 64, so a wave's lanes are one tile)
+__device__ __forceinline__ void sample_body(const CellGeom *__restrict__ geom, CellRec *cells,
                                             const TileRec *__restrict__ tiles, const MeshCounters *mc,
                                             uint32_t tileCap, const MapRO &m, uint4 *__restrict__ sbuf)
 {
@@ -1181,14 +1205,19 @@ __device__ __forceinline__ void sample_body(const CellGeom *__restrict__ geom, c
         // the index the sample picks) come in together
         const TileRec *r = tiles + (t >> 6);
         const CellGeom g = r->g;
-        if (g.tile0 == FF_NO_TILE) continue;
+        if (g.tile0 == FF_NO_TILE) continue;  // the whole tile: its cell is sampled in place
         const int s = (int)(g.pad * 64u + (t & 63u));
-        if (s >= geom_total(g)) continue;
-        sbuf[t] = pack_sample(sample_cell(m, g, r->ll, s));
+        uint32_t lv = 0;
+        if (s < geom_total(g)) {
+            const Sample sm = sample_cell(m, g, r->ll, s);
+            sbuf[t] = pack_sample(sm);
+            lv = sm.lv;
+        }
+        raise_depth(cells, g.cell, lv);
     }
 }
 
-__global__ __launch_bounds__(256) void k_sample(const CellGeom *__restrict__ geom, const CellRec *__restrict__ cells,
+__global__ __launch_bounds__(256) void k_sample(const CellGeom *__restrict__ geom, CellRec *cells,
                                                 const TileRec *__restrict__ tiles, const MeshCounters *mc,
                                                 uint32_t tileCap, MapRO m, uint4 *__restrict__ sbuf)
 {
@@ -1338,10 +1367,14 @@ __device__ void mc_cell(WaveLds &L, const uint64_t *tri, int lane, CellRec *cell
                            v[k].w);
     } else {
         const uint32_t *ll = cells[g.cell].ll;
+        uint32_t lv = 0;
         for (int s = lane; s < total; s += 64) {
-            const uint4 v = pack_sample(sample_cell(m, g, ll, s));
+            const Sample sm = sample_cell(m, g, ll, s);
+            const uint4 v = pack_sample(sm);
             put_sample(L, lg0, lg1, s, __uint_as_float(v.x), __uint_as_float(v.y), v.z, v.w);
+            lv = max(lv, (uint32_t)sm.lv);
         }
+        if (act) raise_depth(cells, g.cell, lv);
     }
     if (lane < 48) L.used[lane] = 0u;
     wave_sync();

@@ -74,6 +74,8 @@ void ffo_mesh_copy(const ffo_ctx *c, int32_t *desc13, uint32_t *nv, uint32_t *nf
 void ffo_mesh_note_frame(ffo_ctx *c);
 int ffo_mesh_update(ffo_ctx *c);
 size_t ffo_mesh_update_faces(void);
+void ffo_mesh_update_all(size_t out[2], float *xyz, uint8_t *rgb, uint32_t *faces);
+void ffo_mesh_set_depth_cap(int cap);
 void ffo_mesh_update_sizes(size_t out[3]);
 /* drops the incremental state if it belongs to c (c is being destroyed) */
 void ffo_mesh_forget(const ffo_ctx *c);

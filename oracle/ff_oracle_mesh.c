@@ -41,6 +41,17 @@
 #define BL 8
 #define BRICK 512
 
+/* the deepest parent-chain level (1: the lastLeaf itself) any sample of the
+ * cell being meshed read; the incremental update keeps it per cell */
+static int g_sample_depth;
+/* the most chain levels the incremental update checks: 16 (all) except
+ * where a test plants a mistake (ffo_mesh_set_depth_cap) */
+static int g_depth_cap = 16;
+static void note_depth(int lv)
+{
+    if (lv > g_sample_depth) g_sample_depth = lv;
+}
+
 /* the context layout is private to ff_oracle.c; accessors are used instead */
 
 typedef struct {
@@ -269,7 +280,9 @@ static uint16_t f2u16(float v) { return (uint16_t)(int)v; }
 static void sample_interior(const mapview *m, uint32_t ll, const int X[3], sample_t *s)
 {
     s->w = 0.0f; s->d = 0.0f; s->c[0] = s->c[1] = s->c[2] = 0;
+    int lv = 0;
     for (uint32_t leaf = ll; leaf < FFO_BRANCHINIT && s->w <= 0.0f; leaf = m->par[leaf]) {
+        lv++;
         const uint16_t *lo = m->pos + 3 * (size_t)leaf;
         int ls = m->scl[leaf];
         int il[3], ih[3];
@@ -301,6 +314,7 @@ static void sample_interior(const mapview *m, uint32_t ll, const int X[3], sampl
             s->c[ch] = f2u16((b0 + b1) + (b2 + b3));
         }
     }
+    note_depth(lv);
 }
 
 /* wall sample: thin axis a at X[a] (no interpolation), full axes u (inner)
@@ -308,7 +322,9 @@ static void sample_interior(const mapview *m, uint32_t ll, const int X[3], sampl
 static void sample_wall(const mapview *m, uint32_t ll, int a, int u, int v, const int X[3], sample_t *s)
 {
     s->w = 0.0f; s->d = 0.0f; s->c[0] = s->c[1] = s->c[2] = 0;
+    int lv = 0;
     for (uint32_t leaf = ll; leaf < FFO_BRANCHINIT && s->w <= 0.0f; leaf = m->par[leaf]) {
+        lv++;
         const uint16_t *lo = m->pos + 3 * (size_t)leaf;
         int ls = m->scl[leaf];
         int la = clampi((X[a] - lo[a]) / ls);
@@ -335,6 +351,7 @@ static void sample_wall(const mapview *m, uint32_t ll, int a, int u, int v, cons
         for (int ch = 0; ch < 3; ch++)
             s->c[ch] = f2u16(rvI * ruI * cf[ch][0] + rvI * ru * cf[ch][1] + rv * ruI * cf[ch][2] + rv * ru * cf[ch][3]);
     }
+    note_depth(lv);
 }
 
 /* edge sample: full axis e interpolated linearly, weight = max of the two
@@ -342,7 +359,9 @@ static void sample_wall(const mapview *m, uint32_t ll, int a, int u, int v, cons
 static void sample_edge(const mapview *m, uint32_t ll, int e, const int X[3], sample_t *s)
 {
     s->w = 0.0f; s->d = 0.0f; s->c[0] = s->c[1] = s->c[2] = 0;
+    int lv = 0;
     for (uint32_t leaf = ll; leaf < FFO_BRANCHINIT && s->w <= 0.0f; leaf = m->par[leaf]) {
+        lv++;
         const uint16_t *lo = m->pos + 3 * (size_t)leaf;
         int ls = m->scl[leaf];
         int el, eh;
@@ -360,13 +379,16 @@ static void sample_edge(const mapview *m, uint32_t ll, int e, const int X[3], sa
         for (int ch = 0; ch < 3; ch++)
             s->c[ch] = f2u16((1.0f - re) * (float)m->col[q0 * 3 + ch] + re * (float)m->col[q1 * 3 + ch]);
     }
+    note_depth(lv);
 }
 
 /* corner sample: point-wise (mesh_interleaved_meshcell.cpp:2550-2575) */
 static void sample_point(const mapview *m, uint32_t ll, const int X[3], sample_t *s)
 {
     s->w = 0.0f; s->d = 0.0f; s->c[0] = s->c[1] = s->c[2] = 0;
+    int lv = 0;
     for (uint32_t leaf = ll; leaf < FFO_BRANCHINIT && s->w <= 0.0f; leaf = m->par[leaf]) {
+        lv++;
         const uint16_t *lo = m->pos + 3 * (size_t)leaf;
         int ls = m->scl[leaf];
         size_t q = (size_t)leaf * BRICK;
@@ -377,6 +399,7 @@ static void sample_point(const mapview *m, uint32_t ll, const int X[3], sample_t
         s->d = m->dist[q];
         for (int ch = 0; ch < 3; ch++) s->c[ch] = m->col[q * 3 + ch];
     }
+    note_depth(lv);
 }
 
 /* ---------------- interpolation (mesh.cpp:1266-1339) ---------------- */
@@ -721,9 +744,10 @@ void ffo_mesh_copy(const ffo_ctx *c, int32_t *desc, uint32_t *nv, uint32_t *nf, 
 }
 
 /* ---------------- incremental update (CPU baseline) ----------------
- * updateMeshes semantics for timing: re-mesh the cells whose bricks (their
- * lastLeafs or parent chains) were updated since the previous call, or that
- * are new or changed their lastLeafs; keep the other cells' meshes; then
+ * updateMeshes semantics for timing: re-mesh the cells whose samples read a
+ * brick (a lastLeaf, or a brick of its parent chain up to the level the
+ * samples reached) updated since the previous call, or that are new or
+ * changed their lastLeafs; keep the other cells' meshes; then
  * concatenate all cell meshes into one mesh like meshWrapperInterleaved
  * (geometryfusion_mipmap_cpu.cpp:2820-2880). */
 typedef struct {
@@ -734,6 +758,7 @@ typedef struct {
     uint32_t *faces;
     uint32_t nv, nf;
     int used;
+    int depth; /* the deepest parent-chain level its samples read (0: no samples) */
 } centry;
 
 typedef struct {
@@ -793,9 +818,11 @@ static void inc_rehash(size_t newcap)
     free(old);
 }
 
-static int chain_changed(const mapview *m, uint32_t leaf, uint32_t since)
+/* a brick among the first `depth` (at most 16) of the parent chain of leaf
+ * was queued after `since` */
+static int chain_changed(const mapview *m, uint32_t leaf, uint32_t since, int depth)
 {
-    for (int d = 0; d < 16 && leaf < FFO_BRANCHINIT; d++) {
+    for (int d = 0; d < depth && d < 16 && leaf < FFO_BRANCHINIT; d++) {
         if (g_inc.last_queue_frame[leaf] > since) return 1;
         leaf = m->par[leaf];
     }
@@ -845,12 +872,19 @@ int ffo_mesh_update(ffo_ctx *c)
         uint64_t key = cell_key(cl);
         size_t h = inc_find(key);
         centry *e = &g_inc.tab[h];
+        /* a kept cell re-meshes when a brick its samples read changed: one
+         * within the chain depth they reached (the samples stop at the same
+         * bricks while those are unchanged) */
         int dirty = !e->used || memcmp(e->ll, cl->ll, sizeof e->ll) != 0;
         for (int s = 0; s < 8 && !dirty; s++)
-            if (cl->ll[s] < FFO_BRANCHINIT && chain_changed(&m, cl->ll[s], last_epoch)) dirty = 1;
+            if (cl->ll[s] < FFO_BRANCHINIT &&
+                chain_changed(&m, cl->ll[s], last_epoch, e->depth < g_depth_cap ? e->depth : g_depth_cap))
+                dirty = 1;
         if (!dirty) continue;
         tmp.nv = tmp.nf = 0;
+        g_sample_depth = 0;
         mesh_cell(&m, cl, &tmp);
+        e->depth = g_sample_depth;
         if (!e->used) { e->used = 1; e->key = key; g_inc.n++; }
         memcpy(e->ll, cl->ll, sizeof e->ll);
         e->xyz = (float *)realloc(e->xyz, (tmp.nv ? tmp.nv : 1) * 3 * sizeof(float));
@@ -884,6 +918,21 @@ int ffo_mesh_update(ffo_ctx *c)
 }
 
 size_t ffo_mesh_update_faces(void) { return g_inc.all.nf; }
+
+/* the concatenated mesh of the last ffo_mesh_update: out = {vertices, faces};
+ * copied when the pointers are not null */
+void ffo_mesh_update_all(size_t out[2], float *xyz, uint8_t *rgb, uint32_t *faces)
+{
+    out[0] = g_inc.all.nv;
+    out[1] = g_inc.all.nf;
+    if (xyz) memcpy(xyz, g_inc.all.xyz, g_inc.all.nv * 3 * sizeof(float));
+    if (rgb) memcpy(rgb, g_inc.all.rgb, g_inc.all.nv * 3);
+    if (faces) memcpy(faces, g_inc.all.faces, g_inc.all.nf * 3 * sizeof(uint32_t));
+}
+
+/* Test hook: caps the chain depth the incremental update checks (a planted
+ * mistake for the tests that show the depth rule matters). */
+void ffo_mesh_set_depth_cap(int cap) { g_depth_cap = cap; }
 
 /* CellUpdate (geometryfusion_mipmap_cpu.cpp:2961-3047): the cells the last
  * ffo_mesh_update re-meshed, in key order, each with its own mesh (faces

@@ -63,6 +63,8 @@ def _mesh_sigs(lib):
     lib.ffo_mesh_update_faces.restype = ctypes.c_size_t
     lib.ffo_mesh_update_faces.argtypes = []
     lib.ffo_mesh_update_sizes.argtypes = [P]
+    lib.ffo_mesh_update_all.argtypes = [P, P, P, P]
+    lib.ffo_mesh_set_depth_cap.argtypes = [ctypes.c_int]
     lib.ffo_mesh_update_copy.argtypes = [P, P, P, P, P, P]
 
 
@@ -168,6 +170,17 @@ class Oracle:
         """Incremental re-mesh of the cells affected since the last call plus the
         concatenation of all cell meshes (CPU-baseline timing path)."""
         return self.lib.ffo_mesh_update(self.h)
+
+    def mesh_incremental(self):
+        """The concatenated mesh the last update_meshes produced: (xyz, rgb, faces)."""
+        cnt = np.zeros(2, np.uint64)
+        self.lib.ffo_mesh_update_all(_vp(cnt), None, None, None)
+        nv, nf = (int(v) for v in cnt)
+        xyz = np.zeros((nv, 3), np.float32)
+        rgb = np.zeros((nv, 3), np.uint8)
+        faces = np.zeros((nf, 3), np.uint32)
+        self.lib.ffo_mesh_update_all(_vp(cnt), _vp(xyz), _vp(rgb), _vp(faces))
+        return xyz, rgb, faces
 
     def cell_update(self):
         """The cells the last update_meshes re-meshed (CellUpdate), in key

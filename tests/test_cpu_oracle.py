@@ -321,11 +321,49 @@ def test_mesh_surface_near_analytic_sdf():
     assert np.percentile(sdf, 95) < 4 * cfg["scale"] * 4
 
 
-def test_incremental_mesh_matches_full():
-    sc, R, t, cfg = synth.config_c1(n_frames=200)
-    o = Oracle(cfg["scale"], cfg["threshold"], True, 1 << 17, 1 << 21)
-    for i in range(0, 30, 3):
-        _fuse(o, sc, R, t, [i])
-        o.update_meshes()
+def _triangles(xyz, rgb, faces):
+    """Triangles as sorted rows of their corners' (x, y, z bits, r, g, b)."""
+    v = np.concatenate([xyz.view(np.uint32).astype(np.int64), rgb.astype(np.int64)], axis=1)
+    t = v[faces.astype(np.int64)].reshape(len(faces), 18)
+    return t[np.lexsort(t.T[::-1])] if len(t) else t
+
+
+def _scratch_triangles(o):
     desc, nv, nf, xyz, rgb, faces = o.mesh_all()
-    assert o.lib.ffo_mesh_update_faces() == len(faces)
+    base = np.repeat(np.concatenate([[0], np.cumsum(nv)[:-1]]).astype(np.int64), nf)
+    return _triangles(xyz, rgb, faces.astype(np.int64) + base[:, None])
+
+
+def _incremental_walk(cap):
+    """Fuses 12 frames of the 640x480 walk (at 320x240, bricks of scales 1,
+    2 and 4) with an incremental update after each; per update, whether the
+    incremental mesh equals the from-scratch mesh, and the cells re-meshed."""
+    sc, R, t, cfg = synth.config_c2(n_frames=60)
+    o = Oracle(cfg["scale"], cfg["threshold"], True, 1 << 17, 1 << 21)
+    o.lib.ffo_mesh_set_depth_cap(cap)
+    same, remeshed = [], []
+    try:
+        for i in range(0, 24, 2):
+            _fuse(o, sc, R, t, [i])
+            remeshed.append(o.update_meshes())
+            same.append(np.array_equal(_triangles(*o.mesh_incremental()), _scratch_triangles(o)))
+    finally:
+        o.lib.ffo_mesh_set_depth_cap(16)
+    scales = np.unique(o.leaves()[1])
+    return same, remeshed, scales
+
+
+def test_incremental_mesh_equals_scratch():
+    """The incremental update re-meshes a kept cell only when a brick its
+    samples read changed (its lastLeafs, and their parent chains up to the
+    level its samples reached); after every update of a mixed-scale walk its
+    mesh equals the from-scratch mesh of every cell, triangle for triangle
+    and bit for bit.  With the check cut to the lastLeafs alone (a planted
+    mistake: the parent-chain fallback ignored) some update's mesh differs,
+    so the walk exercises the fallback."""
+    same, remeshed, scales = _incremental_walk(16)
+    assert len(scales) >= 3, scales
+    assert all(same), same
+    assert min(remeshed[1:]) > 0
+    same1, _, _ = _incremental_walk(1)
+    assert not all(same1)
