@@ -67,13 +67,21 @@ struct Pools {
     uint32_t vH, fH;              // use of one arena that asks for a compaction
 };
 
-// reserves nv vertices and nf faces in arena k; false when they do not fit
-__device__ __forceinline__ bool pool_alloc(const Pools &P, uint32_t k, uint32_t nv, uint32_t nf, uint32_t &v0,
-                                           uint32_t &f0, MeshCounters *mc)
+// reserves nv vertices and nf faces in arena k; false when they do not fit.
+// In two steps, so that a caller can work while the atomic returns:
+// pool_reserve issues it (old: the arena's use before), pool_place checks
+// the reservation and gives the first vertex and face.
+__device__ __forceinline__ unsigned long long pool_reserve(const Pools &P, uint32_t k, uint32_t nv, uint32_t nf)
+{
+    if (nv == 0 && nf == 0) return 0ull;
+    return atomicAdd(&P.arena[k * FF_ARENA_STRIDE], ((unsigned long long)nf << 32) | nv);
+}
+
+__device__ __forceinline__ bool pool_place(const Pools &P, uint32_t k, uint32_t nv, uint32_t nf,
+                                           unsigned long long old, uint32_t &v0, uint32_t &f0, MeshCounters *mc)
 {
     v0 = f0 = 0;
     if (nv == 0 && nf == 0) return true;
-    const unsigned long long old = atomicAdd(&P.arena[k * FF_ARENA_STRIDE], ((unsigned long long)nf << 32) | nv);
     const uint32_t vo = (uint32_t)old, fo = (uint32_t)(old >> 32);
     if (vo + nv > P.vA || fo + nf > P.fA) {
         atomicOr(&mc->flags, FF_MESH_OVERFLOW);
@@ -84,6 +92,12 @@ __device__ __forceinline__ bool pool_alloc(const Pools &P, uint32_t k, uint32_t 
     v0 = k * P.vA + vo;
     f0 = k * P.fA + fo;
     return true;
+}
+
+__device__ __forceinline__ bool pool_alloc(const Pools &P, uint32_t k, uint32_t nv, uint32_t nf, uint32_t &v0,
+                                           uint32_t &f0, MeshCounters *mc)
+{
+    return pool_place(P, k, nv, nf, pool_reserve(P, k, nv, nf), v0, f0, mc);
 }
 
 // per cube index: its triangles' edges, 4 bits each in order (bits 0-59),
@@ -362,9 +376,36 @@ __device__ __forceinline__ uint32_t block_append(uint32_t *counter, bool want)
     return want ? r : 0xffffffffu;
 }
 
+// A block-wide exclusive rank of `want` among the block's threads and the
+// block's total (every thread calls it; it holds two barriers).
+template <int NW>
+__device__ __forceinline__ uint32_t block_rank(bool want, uint32_t *total)
+{
+    __shared__ uint32_t wc[NW + 1];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const uint64_t m = __ballot(want);
+    if (lane == 0) wc[w] = (uint32_t)__popcll(m);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t t = 0;
+        for (int k = 0; k < NW; k++) {
+            const uint32_t c = wc[k];
+            wc[k] = t;
+            t += c;
+        }
+        wc[NW] = t;
+    }
+    __syncthreads();
+    const uint32_t r = wc[w] + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+    *total = wc[NW];
+    __syncthreads();  // wc is reused by the next call
+    return r;
+}
+
 // ---------------------------------------------------------------------------
 // k_terminal: covered terminal nodes
 // ---------------------------------------------------------------------------
+#define FF_TERM_THREADS 256  // block size of k_terminal (every thread of a block appends together)
 __device__ __forceinline__ void terminal_body(const uint32_t *__restrict__ tree, const ushort4 *__restrict__ binfo,
                                               uint32_t nBlocks, uint32_t nLeavesTotal, uint32_t n,
                                               uint4 *__restrict__ term, uint32_t termCap, MeshCounters *mc)
@@ -388,7 +429,7 @@ __device__ __forceinline__ void terminal_body(const uint32_t *__restrict__ tree,
             }
         }
     }
-    const uint32_t i = wave_append(&mc->nTerm, want);
+    const uint32_t i = block_append<FF_TERM_THREADS / 64>(&mc->nTerm, want);
     if (want) {
         if (i < termCap) term[i] = rec;
         else atomicOr(&mc->flags, FF_CELL_OVERFLOW);
@@ -396,7 +437,7 @@ __device__ __forceinline__ void terminal_body(const uint32_t *__restrict__ tree,
 }
 
 // (blocks past the terminal candidates fill the jump table, 256 cells each)
-__global__ void k_terminal(const uint32_t *__restrict__ tree, const ushort4 *__restrict__ binfo, uint32_t nBlocks,
+__global__ __launch_bounds__(FF_TERM_THREADS) void k_terminal(const uint32_t *__restrict__ tree, const ushort4 *__restrict__ binfo, uint32_t nBlocks,
                            uint32_t nLeavesTotal, uint32_t n, uint4 *__restrict__ term, uint32_t termCap,
                            MeshCounters *mc, uint4 *__restrict__ jump, MeshCounters *clearNext)
 {
@@ -411,7 +452,7 @@ __global__ void k_terminal(const uint32_t *__restrict__ tree, const ushort4 *__r
 }
 
 // the batched forms of the per-stream mesh kernels: stream blockIdx.z of S
-__global__ void k_terminal_b(const MeshBatchStream *__restrict__ S)
+__global__ __launch_bounds__(FF_TERM_THREADS) void k_terminal_b(const MeshBatchStream *__restrict__ S)
 {
     const MeshBatchStream &a = S[blockIdx.z];
     if (blockIdx.x * blockDim.x >= a.nBlocks * 8u) return;
@@ -429,14 +470,36 @@ __global__ void k_terminal_b(const MeshBatchStream *__restrict__ S)
 // ownership from them; the block appends its cells with one atomic.
 __device__ __forceinline__ int opt_code(int ox, int oy, int oz) { return ox + 4 * oy + 16 * oz; }
 
+#define FF_CELL_STAGE 256  // cells a k_cells block stages before appending (8 nodes emit at most 216)
+
 __device__ __forceinline__ void cells_body(const uint4 *__restrict__ jump, const uint32_t *__restrict__ tree,
                                            uint32_t nLeavesTotal, uint32_t n,
                                            const uint4 *__restrict__ term, uint32_t termCap, MeshCounters *mc,
                                            CellRec *cells, uint32_t cap)
 {
     __shared__ uint2 nb[8][64];  // (size, lastLeaf) per node and option point
+    // the block's cells are staged here and appended to the table together,
+    // one atomic per FF_CELL_STAGE cells (appends of every block to one
+    // counter serialise on its address)
+    __shared__ uint4 stage[FF_CELL_STAGE][4];
+    __shared__ uint32_t nstage, sbase;
     const uint32_t nt = min(mc->nTerm, termCap);
     const int tid = threadIdx.x;
+    if (tid == 0) nstage = 0u;
+    // (the first iteration's barriers order this before any use)
+    auto flush = [&]() {
+        if (tid == 0) sbase = nstage ? atomicAdd(&mc->nCells, nstage) : 0u;
+        __syncthreads();
+        const uint32_t ns = nstage, b = sbase;
+        for (uint32_t e = (uint32_t)tid; e < ns * 4u; e += blockDim.x) {
+            const uint32_t k = e >> 2;
+            if (b + k < cap) reinterpret_cast<uint4 *>(cells + b + k)[e & 3u] = stage[k][e & 3u];
+        }
+        if (tid == 0 && ns && b + ns > cap) atomicOr(&mc->flags, FF_CELL_OVERFLOW);
+        __syncthreads();  // stage is read before it is refilled
+        if (tid == 0) nstage = 0u;
+        __syncthreads();
+    };
     for (uint32_t n0 = blockIdx.x * 8u; n0 < nt; n0 += gridDim.x * 8u) {
         {
             const uint32_t node = n0 + (uint32_t)(tid >> 6);
@@ -536,22 +599,25 @@ __device__ __forceinline__ void cells_body(const uint4 *__restrict__ jump, const
             }
             // (its barriers also order this iteration's nb reads before the
             // next iteration's writes)
-            const uint32_t i = block_append<8>(&mc->nCells, emit);
+            uint32_t total;
+            const uint32_t rank = block_rank<8>(emit, &total);
+            if (nstage + total > FF_CELL_STAGE) flush();  // block-uniform
             if (emit) {
-                if (i >= cap) {
-                    atomicOr(&mc->flags, FF_CELL_OVERFLOW);
-                } else {
-                    CellRec r;
-                    r.key = cell_key(type, o[0], o[1], o[2], s);
-                    for (int k = 0; k < 8; k++) r.ll[k] = k < nll ? ll[k] : FF_BRANCHINIT;
-                    r.v0 = r.nv = r.f0 = r.nf = 0;
-                    r.hslot = 0xffffffffu;
-                    r.pad[0] = r.pad[1] = r.pad[2] = 0;
-                    cells[i] = r;
-                }
+                CellRec r;
+                r.key = cell_key(type, o[0], o[1], o[2], s);
+                for (int k = 0; k < 8; k++) r.ll[k] = k < nll ? ll[k] : FF_BRANCHINIT;
+                r.v0 = r.nv = r.f0 = r.nf = 0;
+                r.hslot = 0xffffffffu;
+                r.pad[0] = r.pad[1] = r.pad[2] = 0;
+                const uint4 *q = reinterpret_cast<const uint4 *>(&r);
+                for (int k = 0; k < 4; k++) stage[nstage + rank][k] = q[k];
             }
+            __syncthreads();  // every stage write and nstage read are done
+            if (tid == 0) nstage += total;
+            __syncthreads();
         }
     }
+    flush();
 }
 
 __global__ __launch_bounds__(512) void k_cells(const uint32_t *__restrict__ tree, uint32_t nLeavesTotal, uint32_t n,
@@ -807,33 +873,73 @@ struct TileRec {
 };
 static_assert(sizeof(TileRec) == 64, "a tile record is four 16-byte words");
 
-// The geometry slot of a dirty cell (a wave-wide call; mc->nDirty counts
-// them all).  split: cells of more than two tiles (interior cells) take
-// slots from the front of the array and the others from the back
-// (mc->pad[2], mc->pad[1] count them), so that the marching cubes, which
-// takes dirty cells in slot order, starts the long cells first; otherwise
-// slots in append order.
-__device__ __forceinline__ uint32_t dirty_slot(MeshCounters *mc, bool isDirty, uint32_t nt, uint32_t cap, bool split)
+// The marking's slots for a whole block (every thread of the block calls it
+// with its cell; it holds two barriers): the geometry slot of a dirty cell
+// and the first of its nt tiles.  mc->nDirty counts the dirty cells.  split:
+// cells of more than two tiles (interior cells) take geometry slots from
+// the front of the array and the others from the back (mc->pad[2] and
+// mc->pad[1] count them, taken together by one 64-bit atomic), so that the
+// marching cubes, which takes dirty cells in slot order, starts the long
+// cells first; otherwise slots in append order.  One atomic per counter
+// per block, not per wave: the counters are shared by every block of the
+// launch, and appends to one address serialise.
+#define FF_MARK_THREADS 256  // block size of the marking kernels
+
+struct MarkSlot {
+    uint32_t d, t0;
+};
+
+template <int NW>
+__device__ __forceinline__ MarkSlot block_mark_slots(MeshCounters *mc, bool isDirty, uint32_t nt, uint32_t cap,
+                                                     bool split)
 {
-    const uint32_t d = wave_append(&mc->nDirty, isDirty);
-    if (!split) return d;
-    // both runs with one 64-bit atomic: pad[1] (low half) the short cells,
-    // pad[2] (high half) the long ones
-    const bool big = isDirty && nt > 2u;
+    __shared__ uint32_t cnt[NW][3];  // per wave: long cells, short cells (or all dirty ones), tiles
+    __shared__ uint32_t base[NW][3];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const bool big = split && isDirty && nt > 2u;
     const uint64_t mb = __ballot(big), ms = __ballot(isDirty && !big);
-    if (!(mb | ms)) return 0xffffffffu;
-    const int lane = threadIdx.x & 63;
-    const int leader = __ffsll((long long)(mb | ms)) - 1;
-    unsigned long long base = 0;
-    if (lane == leader)
-        base = atomicAdd(reinterpret_cast<unsigned long long *>(&mc->pad[1]),
-                         ((unsigned long long)__popcll(mb) << 32) | (unsigned long long)__popcll(ms));
-    const uint32_t lo = __shfl((uint32_t)base, leader), hi = __shfl((uint32_t)(base >> 32), leader);
+    uint32_t ntot;
+    const uint32_t ex = wave_excl_scan_u32(nt, &ntot);
+    if (lane == 0) {
+        cnt[w][0] = (uint32_t)__popcll(mb);
+        cnt[w][1] = (uint32_t)__popcll(ms);
+        cnt[w][2] = ntot;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t tb = 0, ts = 0, tt = 0;
+        for (int k = 0; k < NW; k++) {
+            base[k][0] = tb; base[k][1] = ts; base[k][2] = tt;
+            tb += cnt[k][0]; ts += cnt[k][1]; tt += cnt[k][2];
+        }
+        uint32_t hi = 0, lo = 0, t = 0;
+        if (tb + ts) {
+            lo = atomicAdd(&mc->nDirty, tb + ts);  // append order when not split
+            if (split) {
+                const unsigned long long r = atomicAdd(reinterpret_cast<unsigned long long *>(&mc->pad[1]),
+                                                       ((unsigned long long)tb << 32) | (unsigned long long)ts);
+                lo = (uint32_t)r;
+                hi = (uint32_t)(r >> 32);
+            }
+        }
+        if (tt) t = atomicAdd(&mc->nTiles, tt);
+        for (int k = 0; k < NW; k++) {
+            base[k][0] += hi; base[k][1] += lo; base[k][2] += t;
+        }
+    }
+    __syncthreads();
     const uint64_t below = (1ull << lane) - 1ull;
-    return big ? hi + (uint32_t)__popcll(mb & below) : cap - 1u - (lo + (uint32_t)__popcll(ms & below));
+    MarkSlot r;
+    r.t0 = base[w][2] + ex;
+    if (!isDirty) r.d = 0xffffffffu;
+    else if (big) r.d = base[w][0] + (uint32_t)__popcll(mb & below);
+    else if (split) r.d = cap - 1u - (base[w][1] + (uint32_t)__popcll(ms & below));
+    else r.d = base[w][1] + (uint32_t)__popcll(ms & below);
+    __syncthreads();  // base is read before the next call writes it
+    return r;
 }
 
-// the dirty cell of marching-cubes index di (slots as dirty_slot gives them)
+// the dirty cell of marching-cubes index di (slots as block_mark_slots gives them)
 __device__ __forceinline__ uint32_t dirty_geom(uint32_t di, uint32_t nBig, uint32_t cap, bool split)
 {
     return !split || di < nBig ? di : cap - 1u - (di - nBig);
@@ -973,10 +1079,10 @@ __device__ __forceinline__ void mark_body(CellRec *cells, const CellRec *__restr
                                           uint32_t tileCap, bool split, uint32_t updateId)
 {
     const uint32_t nc = min(mc->nCells, cap);
-    const uint32_t lane = threadIdx.x & 63u;
     const uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t ib = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); ib < nc; ib += stride) {
-        const uint32_t i = ib + lane;
+    // block-uniform iterations (the slots are assigned per block)
+    for (uint32_t ib = blockIdx.x * blockDim.x; ib < nc; ib += stride) {
+        const uint32_t i = ib + threadIdx.x;
         bool isDirty = false;
         CellGeom g;
         uint32_t ll[8];
@@ -1019,17 +1125,10 @@ __device__ __forceinline__ void mark_body(CellRec *cells, const CellRec *__restr
                 c.pad[1] = nt ? 1u : 0u;  // chain depth: raised by the sampling
             }
         }
-        const uint32_t d = dirty_slot(mc, isDirty, nt, cap, split);
-        // sample tiles: one atomic per wave for the wave's total
-        uint32_t ntot;
-        const uint32_t ex = wave_excl_scan_u32(nt, &ntot);
-        uint32_t tb = 0;
-        if (ntot) {
-            if (lane == 0) tb = atomicAdd(&mc->nTiles, ntot);
-            tb = __shfl(tb, 0);
-        }
+        const MarkSlot sl = block_mark_slots<FF_MARK_THREADS / 64>(mc, isDirty, nt, cap, split);
+        const uint32_t d = sl.d;
         if (isDirty) {
-            const uint32_t t0 = tb + ex;
+            const uint32_t t0 = sl.t0;
             const bool fits = t0 + nt <= tileCap;
             g.tile0 = nt == 0 ? FF_EMPTY_CELL : (fits ? t0 : FF_NO_TILE);
             put_tiles(tiles, tileCap, t0, nt, g, ll);
@@ -1038,7 +1137,7 @@ __device__ __forceinline__ void mark_body(CellRec *cells, const CellRec *__restr
     }
 }
 
-__global__ void k_mark(CellRec *cells, const CellRec *__restrict__ prev, const uint64_t *__restrict__ hkeys,
+__global__ __launch_bounds__(FF_MARK_THREADS) void k_mark(CellRec *cells, const CellRec *__restrict__ prev, const uint64_t *__restrict__ hkeys,
                        const uint32_t *__restrict__ hval, uint32_t hmask, bool havePrev,
                        const uint32_t *__restrict__ stamp, const uint32_t *__restrict__ parent, uint32_t since,
                        MeshCounters *mc, CellGeom *geom, uint32_t cap, MapRO m, TileRec *tiles, uint32_t tileCap,
@@ -1048,7 +1147,7 @@ __global__ void k_mark(CellRec *cells, const CellRec *__restrict__ prev, const u
               updateId);
 }
 
-__global__ void k_mark_b(const MeshBatchStream *__restrict__ S)
+__global__ __launch_bounds__(FF_MARK_THREADS) void k_mark_b(const MeshBatchStream *__restrict__ S)
 {
     const MeshBatchStream &a = S[blockIdx.z];
     mark_body(a.cells, a.prev, a.hkeys, a.hval, a.hmask, a.havePrev != 0, a.stamp, a.parent, a.since, a.mc, a.geom,
@@ -1061,11 +1160,11 @@ __global__ void k_mark_b(const MeshBatchStream *__restrict__ S)
 // (k_mark's test with the previous table's own record).  A dirty cell's mesh
 // range is cleared first, like the fresh record k_cells writes: k_mc fills
 // it unless the new mesh is empty.
-__global__ void k_mark_same(CellRec *cells, const uint32_t *__restrict__ stamp, const uint32_t *__restrict__ parent,
+__global__ __launch_bounds__(FF_MARK_THREADS) void k_mark_same(CellRec *cells, const uint32_t *__restrict__ stamp, const uint32_t *__restrict__ parent,
                             uint32_t since, MeshCounters *mc, uint32_t nCells, MeshCounters *clearNext, CellGeom *geom,
                             uint32_t cap, MapRO m, TileRec *tiles, uint32_t tileCap, uint32_t updateId)
 {
-    const bool split = true;  // long cells first (dirty_slot)
+    const bool split = true;  // long cells first (block_mark_slots)
     // the counters start at zero; block 0 records the table's size and
     // clears the next update's counters
     if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -1073,10 +1172,10 @@ __global__ void k_mark_same(CellRec *cells, const uint32_t *__restrict__ stamp, 
         *clearNext = MeshCounters{};
     }
     const uint32_t nc = min(nCells, cap);
-    const uint32_t lane = threadIdx.x & 63u;
     const uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t ib = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); ib < nc; ib += stride) {
-        const uint32_t i = ib + lane;
+    // block-uniform iterations (the slots are assigned per block)
+    for (uint32_t ib = blockIdx.x * blockDim.x; ib < nc; ib += stride) {
+        const uint32_t i = ib + threadIdx.x;
         bool isDirty = false;
         CellGeom g;
         uint32_t ll[8];
@@ -1095,16 +1194,10 @@ __global__ void k_mark_same(CellRec *cells, const uint32_t *__restrict__ stamp, 
                 c.pad[1] = nt ? 1u : 0u;  // chain depth: raised by the sampling
             }
         }
-        const uint32_t d = dirty_slot(mc, isDirty, nt, cap, split);
-        uint32_t ntot;
-        const uint32_t ex = wave_excl_scan_u32(nt, &ntot);
-        uint32_t tb = 0;
-        if (ntot) {
-            if (lane == 0) tb = atomicAdd(&mc->nTiles, ntot);
-            tb = __shfl(tb, 0);
-        }
+        const MarkSlot sl = block_mark_slots<FF_MARK_THREADS / 64>(mc, isDirty, nt, cap, split);
+        const uint32_t d = sl.d;
         if (isDirty) {
-            const uint32_t t0 = tb + ex;
+            const uint32_t t0 = sl.t0;
             const bool fits = t0 + nt <= tileCap;
             g.tile0 = nt == 0 ? FF_EMPTY_CELL : (fits ? t0 : FF_NO_TILE);
             put_tiles(tiles, tileCap, t0, nt, g, ll);
@@ -1333,6 +1426,47 @@ __device__ __forceinline__ int edge_corner(int e, int end)
     return (int)(((end ? EB : EA) >> (4 * e)) & 15u);
 }
 
+// bits 0..7 of x moved to bits 0, 3, ..., 21
+__device__ __forceinline__ uint32_t spread3(uint32_t x)
+{
+    uint32_t r = 0;
+#pragma unroll
+    for (int i = 0; i < 8; i++) r |= ((x >> i) & 1u) << (3 * i);
+    return r;
+}
+
+// The used-edge bitmap (slot = sample * 3 + axis, axes x, y, z) from the
+// flag rows, lane r for sample row r = l * 8 + j: an edge is used by some
+// triangle exactly when its two samples differ in sign and a cube that
+// contains it has no zero-weight corner (each triangle-table row uses
+// exactly the cut edges of its cube; a cube with a zero-weight corner
+// emits nothing).  Row r's 24 slots are bits 24 r .. 24 r + 23; lane w
+// assembles word w from rows (32 w) / 24 and the next.
+__device__ __forceinline__ void used_edges(WaveLds &L, int lane, int nc0, int nc1, int nc2)
+{
+    const int r = lane, j = r & 7, l = r >> 3;
+    // cubes of row (j, l) with no zero-weight corner, bits i < nc0
+    uint32_t V = 0u;
+    if (j < nc1 && l < nc2) {
+        const uint32_t z = flag_row(L, true, r) | flag_row(L, true, r + 1) | flag_row(L, true, r + FF_BL) |
+                           flag_row(L, true, r + FF_BL + 1);
+        V = ~(z | (z >> 1)) & ((1u << nc0) - 1u);
+    }
+    const uint32_t Vy = __shfl_up(V, 1), Vz = __shfl_up(V, FF_BL), Vyz = __shfl_up(V, FF_BL + 1);
+    const uint32_t vj = j > 0 ? Vy : 0u, vl = l > 0 ? Vz : 0u, vjl = (j > 0 && l > 0) ? Vyz : 0u;
+    const uint32_t n0 = flag_row(L, false, r);
+    const uint32_t ny = j < FF_BL - 1 ? flag_row(L, false, r + 1) : 0u;
+    const uint32_t nz = l < FF_BL - 1 ? flag_row(L, false, r + FF_BL) : 0u;
+    const uint32_t ux = (n0 ^ (n0 >> 1)) & (V | vj | vl | vjl);
+    const uint32_t wy = V | vl, wz = V | vj;
+    const uint32_t uy = (n0 ^ ny) & (wy | (wy << 1));
+    const uint32_t uz = (n0 ^ nz) & (wz | (wz << 1));
+    const uint32_t chunk = spread3(ux & 0xffu) | (spread3(uy & 0xffu) << 1) | (spread3(uz & 0xffu) << 2);
+    const int w = lane < 48 ? lane : 47, r0 = (32 * w) / 24, o = 32 * w - 24 * r0;
+    const uint32_t a = (uint32_t)__shfl((int)chunk, r0), b = (uint32_t)__shfl((int)chunk, min(r0 + 1, 63));
+    if (lane < 48) L.used[lane] = (a >> o) | (b << (24 - o));
+}
+
 // One dirty cell, by one wave (lane 0..63) with its LDS slice: samples into
 // LDS, cube classification, edge numbering, vertices and faces.  di is the
 // cell's dirty index (it picks the pool arena); act false for a wave without
@@ -1360,11 +1494,33 @@ __device__ void mc_cell(WaveLds &L, const uint64_t *tri, int lane, CellRec *cell
         uint4 v[FF_BRICK / 64];
 #pragma unroll
         for (int k = 0; k < FF_BRICK / 64; k++) v[k] = src[min(lane + 64 * k, total - 1)];
+        if (total == FF_BRICK) {
+            // the full 8 x 8 x 8 grid: sample s is bit s of the flag rows, so
+            // the wave's 64 samples give two words of each by a ballot (in
+            // place of an LDS atomic per sample, 32 lanes on one word)
 #pragma unroll
-        for (int k = 0; k < FF_BRICK / 64; k++)
-            if (lane + 64 * k < total)
-                put_sample(L, lg0, lg1, lane + 64 * k, __uint_as_float(v[k].x), __uint_as_float(v[k].y), v[k].z,
-                           v[k].w);
+            for (int k = 0; k < FF_BRICK / 64; k++) {
+                const int id = lane + 64 * k;
+                const float d = __uint_as_float(v[k].x), w = __uint_as_float(v[k].y);
+                L.d[id] = d;
+                L.c[0][id] = (uint16_t)(v[k].z & 0xffffu);
+                L.c[1][id] = (uint16_t)(v[k].z >> 16);
+                L.c[2][id] = (uint16_t)v[k].w;
+                const uint64_t nb = __ballot(d < 0.0f), zb = __ballot(w == 0.0f);
+                if (lane == 0) {
+                    L.flags[2 * k] = (uint32_t)nb;
+                    L.flags[2 * k + 1] = (uint32_t)(nb >> 32);
+                    L.flags[16 + 2 * k] = (uint32_t)zb;
+                    L.flags[16 + 2 * k + 1] = (uint32_t)(zb >> 32);
+                }
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < FF_BRICK / 64; k++)
+                if (lane + 64 * k < total)
+                    put_sample(L, lg0, lg1, lane + 64 * k, __uint_as_float(v[k].x), __uint_as_float(v[k].y), v[k].z,
+                               v[k].w);
+        }
     } else {
         const uint32_t *ll = cells[g.cell].ll;
         uint32_t lv = 0;
@@ -1457,7 +1613,6 @@ __device__ void mc_cell(WaveLds &L, const uint64_t *tri, int lane, CellRec *cell
     for (int q = lane; q < ncubes; q += 64) {
         const int jl = (int)div_small((uint32_t)q, m0), l = (int)div_small((uint32_t)q, m01);
         const int i = q - jl * nc0, j = jl - l * nc1;
-        const int b0 = (l * FF_BL + j) * FF_BL + i;
         // the cube's corners are bits i, i + 1 of rows r, r + 1 (y + 1),
         // r + 8 (z + 1) and r + 9
         const int r = l * FF_BL + j;
@@ -1471,11 +1626,8 @@ __device__ void mc_cell(WaveLds &L, const uint64_t *tri, int lane, CellRec *cell
         const uint64_t T = tri[ci];
         const int nt = tri_count(T);
         L.cube[q] = (uint16_t)(ci | (nt << 8));
-        for (int t = 0; t < 3 * nt; t++) {
-            const int slot = b0 * 3 + edge_slot_off(tri_edge(T, t));
-            atomicOr(&L.used[slot >> 5], 1u << (slot & 31));
-        }
     }
+    used_edges(L, lane, nc0, nc1, nc2);
     wave_sync();
     // -- vertex numbering over the used-edge bitmap, and the slot list
     {
@@ -1507,31 +1659,40 @@ __device__ void mc_cell(WaveLds &L, const uint64_t *tri, int lane, CellRec *cell
         nf = tot;
     }
     wave_sync();
+    // vertex vi: the used edge L.list[vi] interpolated (VertexInterp)
+    auto vertex = [&](uint32_t vi) {
+        const int slot = L.list[vi], a = slot % 3, id = slot / 3;
+        const int gg[3] = {id & 7, (id >> 3) & 7, id >> 6};
+        const int id2 = id + (a == 0 ? 1 : (a == 1 ? FF_BL : FF_BL * FF_BL));
+        float p1[3], p2[3];
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+            const int ck = geom_coord(g, k, gg[k]);
+            p1[k] = m.off[k] + (float)ck * m.scale;
+            p2[k] = k == a ? m.off[k] + (float)(ck + elen[k]) * m.scale : p1[k];
+        }
+        float V[3];
+        vinterp(p1, p2, L.d[id], L.d[id2], V);
+        const uint16_t ca[3] = {L.c[0][id], L.c[1][id], L.c[2][id]};
+        const uint16_t cb[3] = {L.c[0][id2], L.c[1][id2], L.c[2][id2]};
+        const uint32_t rgb = cinterp(ca, cb, L.d[id], L.d[id2]);
+        return make_float4(V[0], V[1], V[2], __uint_as_float(rgb));
+    };
+    // the pool reservation is issued first and each lane's first vertex is
+    // computed while it returns
+    unsigned long long old = 0ull;
+    if (lane == 0) old = pool_reserve(P, di % FF_ARENAS, nv, nf);
+    float4 first = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    if ((uint32_t)lane < nv) first = vertex((uint32_t)lane);
     uint32_t v0 = 0, f0 = 0, ok = 1;
-    if (lane == 0) ok = pool_alloc(P, di % FF_ARENAS, nv, nf, v0, f0, mc) ? 1u : 0u;
+    if (lane == 0) ok = pool_place(P, di % FF_ARENAS, nv, nf, old, v0, f0, mc) ? 1u : 0u;
     v0 = __shfl(v0, 0);
     f0 = __shfl(f0, 0);
     const bool fits = __shfl(ok, 0) != 0;
     if (fits) {
         // vertices: one used edge per lane
-        for (uint32_t vi = lane; vi < nv; vi += 64) {
-            const int slot = L.list[vi], a = slot % 3, id = slot / 3;
-            const int gg[3] = {id & 7, (id >> 3) & 7, id >> 6};
-            const int id2 = id + (a == 0 ? 1 : (a == 1 ? FF_BL : FF_BL * FF_BL));
-            float p1[3], p2[3];
-#pragma unroll
-            for (int k = 0; k < 3; k++) {
-                const int ck = geom_coord(g, k, gg[k]);
-                p1[k] = m.off[k] + (float)ck * m.scale;
-                p2[k] = k == a ? m.off[k] + (float)(ck + elen[k]) * m.scale : p1[k];
-            }
-            float V[3];
-            vinterp(p1, p2, L.d[id], L.d[id2], V);
-            const uint16_t ca[3] = {L.c[0][id], L.c[1][id], L.c[2][id]};
-            const uint16_t cb[3] = {L.c[0][id2], L.c[1][id2], L.c[2][id2]};
-            const uint32_t rgb = cinterp(ca, cb, L.d[id], L.d[id2]);
-            P.v[v0 + vi] = make_float4(V[0], V[1], V[2], __uint_as_float(rgb));
-        }
+        if ((uint32_t)lane < nv) P.v[v0 + lane] = first;
+        for (uint32_t vi = lane + 64; vi < nv; vi += 64) P.v[v0 + vi] = vertex(vi);
         // faces
         for (int q = lane; q < ncubes; q += 64) {
             const int cq = L.cube[q];
@@ -1981,7 +2142,8 @@ bool mesh_launch_pre(ff_ctx *c, bool timed)
         const MapRO m = s->m;
         const uint32_t nc = std::min(s->nCells[s->cur], s->cellCap);
         if (!s->mcZeroNow) k_mesh_begin<<<1, 64, 0, c->stream>>>(s->d_mc);
-        k_mark_same<<<std::max(1u, std::min(4096u, (nc + 255) / 256)), 256, 0, c->stream>>>(
+        k_mark_same<<<std::max(1u, std::min(4096u, (nc + FF_MARK_THREADS - 1) / FF_MARK_THREADS)), FF_MARK_THREADS, 0,
+                      c->stream>>>(
             s->d_cells[nxt], c->d_stamp, c->d_parent, s->lastMeshStamp, s->d_mc, nc, mc_other(s), s->d_geom,
             s->cellCap, m, s->d_tiles, s->tileCap, s->launchId);
         k_sample<<<4096, 256, 0, c->stream>>>(s->d_geom, s->d_cells[nxt], s->d_tiles, s->d_mc, s->tileCap, m,
@@ -1992,13 +2154,14 @@ bool mesh_launch_pre(ff_ctx *c, bool timed)
     if (!s->mcZeroNow) k_mesh_begin<<<1, 64, 0, c->stream>>>(s->d_mc);
     const uint32_t nBlocks = c->nBranchesUsed / 16;
     // (plus FF_JUMP_CELLS / 256 blocks that fill the jump table)
-    k_terminal<<<(nBlocks * 8 + 255) / 256 + FF_JUMP_CELLS / 256, 256, 0, c->stream>>>(
+    k_terminal<<<(nBlocks * 8 + FF_TERM_THREADS - 1) / FF_TERM_THREADS + FF_JUMP_CELLS / FF_TERM_THREADS,
+                 FF_TERM_THREADS, 0, c->stream>>>(
         c->d_tree, s->d_binfo, nBlocks, c->nLeavesTotal, c->n, s->d_term, s->termCap, s->d_mc, s->d_jump,
         mc_other(s));
     k_cells<<<1024, 512, 0, c->stream>>>(c->d_tree, c->nLeavesTotal, c->n, s->d_term, s->termCap, s->d_mc,
                                          s->d_cells[nxt], s->cellCap, s->d_jump);
     const MapRO m = s->m;
-    k_mark<<<1024, 256, 0, c->stream>>>(s->d_cells[nxt], s->d_cells[s->cur], s->d_hkeys, s->d_hval, s->hmask,
+    k_mark<<<1024, FF_MARK_THREADS, 0, c->stream>>>(s->d_cells[nxt], s->d_cells[s->cur], s->d_hkeys, s->d_hval, s->hmask,
                                         s->nCells[s->cur] > 0 && !s->fullNext, c->d_stamp, c->d_parent,
                                         s->lastMeshStamp, s->d_mc,
                                         s->d_geom, s->cellCap, m, s->d_tiles, s->tileCap, s->launchId);
@@ -2649,14 +2812,14 @@ int ff_update_meshes_batch(ff_ctx *const *ctxs, int n)
         for (int k = 0; k < na; k++) {
             ls->h_mbs[k] = mesh_batch_stream(ctxs[act[k]]);
             ls->h_ms[k] = mesh_stream(ctxs[act[k]]);
-            maxT = std::max(maxT, (ls->h_mbs[k].nBlocks * 8u + 255u) / 256u);
+            maxT = std::max(maxT, (ls->h_mbs[k].nBlocks * 8u + FF_TERM_THREADS - 1u) / FF_TERM_THREADS);
         }
         hipMemcpyAsync(ls->d_mdesc, ls->h_mdesc, ls->mdescBytes, hipMemcpyHostToDevice, st);
         const MeshBatchStream *S = ls->d_mbs;
         k_mesh_begin_b<<<dim3(1, 1, na), 64, 0, st>>>(S);
-        k_terminal_b<<<dim3(maxT, 1, na), 256, 0, st>>>(S);
+        k_terminal_b<<<dim3(maxT, 1, na), FF_TERM_THREADS, 0, st>>>(S);
         k_cells_b<<<dim3(batch_grid(1024, na, 32), 1, na), 512, 0, st>>>(S);
-        k_mark_b<<<dim3(batch_grid(1024, na, 32), 1, na), 256, 0, st>>>(S);
+        k_mark_b<<<dim3(batch_grid(1024, na, 32), 1, na), FF_MARK_THREADS, 0, st>>>(S);
         k_sample_b<<<dim3(batch_grid(4096, na, 64), 1, na), 256, 0, st>>>(S);
         k_mc_prefix<<<1, 64, 0, st>>>(ls->d_ms, na, ls->d_msPrefix);
         k_mc_batch<<<2048, 256, 0, st>>>(ls->d_ms, na, ls->d_msPrefix);
