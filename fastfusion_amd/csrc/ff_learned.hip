@@ -61,7 +61,8 @@ static_assert(P_COUNT == FF_MLP_NPARAMS, "parameter layout");
 #define F_A3 12   // W3, 4 x 2
 #define F_A3T 20  // W3 transposed (backward), 4 x 2
 #define F_A2T 28  // W2 transposed (backward), 4 x 2
-#define F_COUNT 36
+#define F_A4 36   // w4 as row 0 of a 16 x 64 matrix (inference), 1 x 2
+#define F_COUNT 38
 
 __host__ __device__ __forceinline__ int unit_of(int s, int p)
 {
@@ -70,6 +71,9 @@ __host__ __device__ __forceinline__ int unit_of(int s, int p)
 }
 
 __device__ __forceinline__ float relu(float x) { return x > 0.0f ? x : 0.0f; }
+// the same for finite x, as an integer max of the bits (a float max of an
+// MFMA result would first be canonicalised: one more instruction)
+__device__ __forceinline__ float relu_i(float x) { return __int_as_float(max(__float_as_int(x), 0)); }
 
 // ---------------------------------------------------------------------------
 // weight packing: fp32 master -> bf16 fragments, biases, w4 (bf16-rounded)
@@ -88,6 +92,8 @@ __global__ void k_mlp_pack(const float *__restrict__ P, bf16x8 *__restrict__ fra
             if (f < F_A2) {
                 const int row = 16 * f + row16, k = p;
                 w = k < MLP_IN ? P[P_W1 + row * MLP_IN + k] : 0.0f;
+            } else if (f >= F_A4) {
+                w = row16 == 0 ? P[P_W4 + unit_of(f - F_A4, p)] : 0.0f;
             } else {
                 const int j = (f - F_A2) % 8, t = j / 2, s = j % 2, row = 16 * t + row16, u = unit_of(s, p);
                 if (f < F_A3) w = P[P_W2 + row * 64 + u];
@@ -267,29 +273,31 @@ __device__ float feature(const uint16_t *__restrict__ depth, int W, int H, int x
 #define CONF_WIN_W 68
 #define CONF_WIN (5 * CONF_WIN_W)
 
-// inference forward of mlp_forward (same operations, same order): the
-// activations go straight to the next layer's bf16 operand, and only the
-// confidence is kept
-__device__ __forceinline__ float mlp_infer(const MlpW &W, const bf16x8 &bx, const float *__restrict__ bias,
-                                           const float *__restrict__ w4, int g)
+// inference forward of mlp_forward: each layer's accumulator starts at its
+// bias (the MFMA adds the products to it), the activations go straight to
+// the next layer's bf16 operand, and the output layer (w4, bf16 as in
+// mlp_forward) is two more MFMAs with w4 as row 0 of the A matrix, so its
+// result is row 0 of the tile: lanes 0-15 (g = 0) hold their pixel's o.
+// Only the summation order differs from mlp_forward.
+__device__ __forceinline__ float mlp_infer(const MlpW &W, const bf16x8 (&a4)[2], const bf16x8 &bx,
+                                           const float *__restrict__ bias, int g)
 {
     bf16x8 b[2];
     f32x4 acc[4];
 #pragma unroll
     for (int t = 0; t < 4; t++) {
-        acc[t] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int r = 0; r < 4; r++) acc[t][r] = bias[16 * t + 4 * g + r];
         acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(W.a1[t], bx, acc[t], 0, 0, 0);
     }
 #pragma unroll
     for (int s = 0; s < 2; s++)
 #pragma unroll
-        for (int q = 0; q < 8; q++) {
-            const int t = 2 * s + (q >> 2), r = q & 3;
-            b[s][q] = (__bf16)relu(acc[t][r] + bias[16 * t + 4 * g + r]);
-        }
+        for (int q = 0; q < 8; q++) b[s][q] = (__bf16)relu_i(acc[2 * s + (q >> 2)][q & 3]);
 #pragma unroll
     for (int t = 0; t < 4; t++) {
-        acc[t] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int r = 0; r < 4; r++) acc[t][r] = bias[64 + 16 * t + 4 * g + r];
 #pragma unroll
         for (int s = 0; s < 2; s++)
             acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(W.a2[t][s], b[s], acc[t], 0, 0, 0);
@@ -297,24 +305,23 @@ __device__ __forceinline__ float mlp_infer(const MlpW &W, const bf16x8 &bx, cons
 #pragma unroll
     for (int s = 0; s < 2; s++)
 #pragma unroll
-        for (int q = 0; q < 8; q++) {
-            const int t = 2 * s + (q >> 2), r = q & 3;
-            b[s][q] = (__bf16)relu(acc[t][r] + bias[64 + 16 * t + 4 * g + r]);
-        }
-    float part = 0.0f;
+        for (int q = 0; q < 8; q++) b[s][q] = (__bf16)relu_i(acc[2 * s + (q >> 2)][q & 3]);
 #pragma unroll
     for (int t = 0; t < 4; t++) {
-        f32x4 a = {0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
-        for (int s = 0; s < 2; s++) a = __builtin_amdgcn_mfma_f32_16x16x32_bf16(W.a3[t][s], b[s], a, 0, 0, 0);
+        for (int r = 0; r < 4; r++) acc[t][r] = bias[128 + 16 * t + 4 * g + r];
 #pragma unroll
-        for (int r = 0; r < 4; r++)
-            part += w4[16 * t + 4 * g + r] * (float)(__bf16)relu(a[r] + bias[128 + 16 * t + 4 * g + r]);
+        for (int s = 0; s < 2; s++)
+            acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(W.a3[t][s], b[s], acc[t], 0, 0, 0);
     }
-    part += __shfl_xor(part, 16);
-    part += __shfl_xor(part, 32);
-    const float o = part + bias[192];
-    return 1.0f / (1.0f + expf(-o));
+#pragma unroll
+    for (int s = 0; s < 2; s++)
+#pragma unroll
+        for (int q = 0; q < 8; q++) b[s][q] = (__bf16)relu_i(acc[2 * s + (q >> 2)][q & 3]);
+    f32x4 out = {bias[192], 0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int s = 0; s < 2; s++) out = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a4[s], b[s], out, 0, 0, 0);
+    return 1.0f / (1.0f + expf(-out[0]));
 }
 
 struct ConfWave {
@@ -324,17 +331,17 @@ struct ConfWave {
 
 __device__ __forceinline__ void confidence_body(const uint16_t *__restrict__ depth, const FeatCam K, int W, int H,
                                                 const bf16x8 *__restrict__ frag, const float *__restrict__ bias,
-                                                const float *__restrict__ w4, float *__restrict__ conf)
+                                                float *__restrict__ conf)
 {
-    __shared__ float sb[3 * 64 + 1], sw4[64];
+    __shared__ float sb[3 * 64 + 1];
     __shared__ ConfWave cw[4];
     for (int i = threadIdx.x; i < 3 * 64 + 1; i += blockDim.x) sb[i] = bias[i];
-    for (int i = threadIdx.x; i < 64; i += blockDim.x) sw4[i] = w4[i];
     __syncthreads();
     const int lane = threadIdx.x & 63, g = lane >> 4, p = lane & 15;
     ConfWave &C = cw[threadIdx.x >> 6];
     MlpW Wt;
     load_forward(Wt, frag, lane);
+    const bf16x8 a4[2] = {frag[F_A4 * 64 + lane], frag[(F_A4 + 1) * 64 + lane]};
     const int segsPerRow = (W + 63) / 64;
     const int nSegs = segsPerRow * H;
     const int wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6, nWaves = (gridDim.x * blockDim.x) >> 6;
@@ -377,9 +384,9 @@ __device__ __forceinline__ void confidence_body(const uint16_t *__restrict__ dep
             }
             // the biases are re-read from LDS for every tile (held in
             // registers across tiles they would cost the occupancy)
-            const float *bsb = sb, *bsw = sw4;
-            asm volatile("" : "+v"(bsb), "+v"(bsw));
-            const float cc = mlp_infer(Wt, bx, bsb, bsw, g);
+            const float *bsb = sb;
+            asm volatile("" : "+v"(bsb));
+            const float cc = mlp_infer(Wt, a4, bx, bsb, g);
             // a pixel whose predicted residual is beyond every truncation band
             // (confidence under the gate) is not fused at all
             const int x = x0 + px;
@@ -397,7 +404,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
                                                     const float *__restrict__ w4, float *__restrict__ conf)
 {
     const FeatCam K{fd->sp.fx, fd->sp.fy, fd->sp.cx, fd->sp.cy, fd->scaling, fd->maxcam, fd->ifx, fd->ify};
-    confidence_body(fd->depth, K, W, H, frag, bias, w4, conf);
+    (void)w4;
+    confidence_body(fd->depth, K, W, H, frag, bias, conf);
 }
 
 // the frame's values as arguments (direct launches beside the frame's
@@ -406,7 +414,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) void k
                                                       const bf16x8 *__restrict__ frag, const float *__restrict__ bias,
                                                       const float *__restrict__ w4, float *__restrict__ conf)
 {
-    confidence_body(depth, K, W, H, frag, bias, w4, conf);
+    (void)w4;
+    confidence_body(depth, K, W, H, frag, bias, conf);
 }
 
 // ---------------------------------------------------------------------------
@@ -566,9 +575,9 @@ __global__ __launch_bounds__(256) void k_train_fb(Batch bt, const bf16x8 *__rest
                                                   const float *__restrict__ bias, const float *__restrict__ w4,
                                                   Acts A)
 {
-    __shared__ bf16x8 sf[F_COUNT * 64];
+    __shared__ bf16x8 sf[F_A4 * 64];  // the fragments the training pass uses
     __shared__ float sb[3 * 64 + 1], sw4[64], sl[4];
-    for (int i = threadIdx.x; i < F_COUNT * 64; i += blockDim.x) sf[i] = frag[i];
+    for (int i = threadIdx.x; i < F_A4 * 64; i += blockDim.x) sf[i] = frag[i];
     for (int i = threadIdx.x; i < 3 * 64 + 1; i += blockDim.x) sb[i] = bias[i];
     for (int i = threadIdx.x; i < 64; i += blockDim.x) sw4[i] = w4[i];
     __syncthreads();

@@ -338,6 +338,13 @@ __device__ __forceinline__ uint64_t cell_key(int type, int ox, int oy, int oz, i
 
 // Appends one element per participating lane with a single atomic per wave.
 // Every lane of the wave must call it (want may be false).
+// the set bits of m below this lane (mbcnt: no 64-bit lane mask is formed,
+// which a kernel would otherwise hoist and hold in two registers)
+__device__ __forceinline__ uint32_t lanes_below(uint64_t m)
+{
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
 __device__ __forceinline__ uint32_t wave_append(uint32_t *counter, bool want)
 {
     const uint64_t m = __ballot(want);
@@ -347,7 +354,7 @@ __device__ __forceinline__ uint32_t wave_append(uint32_t *counter, bool want)
     uint32_t base = 0;
     if (lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(m));
     base = __shfl(base, leader);
-    return want ? base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull)) : 0xffffffffu;
+    return want ? base + lanes_below(m) : 0xffffffffu;
 }
 
 // The same for a whole block (every thread calls it; it holds two barriers):
@@ -371,7 +378,7 @@ __device__ __forceinline__ uint32_t block_append(uint32_t *counter, bool want)
         wc[NW] = t ? atomicAdd(counter, t) : 0u;
     }
     __syncthreads();
-    const uint32_t r = wc[NW] + wc[w] + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+    const uint32_t r = wc[NW] + wc[w] + lanes_below(m);
     __syncthreads();  // wc is reused by the next call
     return want ? r : 0xffffffffu;
 }
@@ -396,7 +403,7 @@ __device__ __forceinline__ uint32_t block_rank(bool want, uint32_t *total)
         wc[NW] = t;
     }
     __syncthreads();
-    const uint32_t r = wc[w] + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
+    const uint32_t r = wc[w] + lanes_below(m);
     *total = wc[NW];
     __syncthreads();  // wc is reused by the next call
     return r;
@@ -928,13 +935,12 @@ __device__ __forceinline__ MarkSlot block_mark_slots(MeshCounters *mc, bool isDi
         }
     }
     __syncthreads();
-    const uint64_t below = (1ull << lane) - 1ull;
     MarkSlot r;
     r.t0 = base[w][2] + ex;
     if (!isDirty) r.d = 0xffffffffu;
-    else if (big) r.d = base[w][0] + (uint32_t)__popcll(mb & below);
-    else if (split) r.d = cap - 1u - (base[w][1] + (uint32_t)__popcll(ms & below));
-    else r.d = base[w][1] + (uint32_t)__popcll(ms & below);
+    else if (big) r.d = base[w][0] + lanes_below(mb);
+    else if (split) r.d = cap - 1u - (base[w][1] + lanes_below(ms));
+    else r.d = base[w][1] + lanes_below(ms);
     __syncthreads();  // base is read before the next call writes it
     return r;
 }
@@ -1583,7 +1589,7 @@ __device__ void mc_cell(WaveLds &L, const uint64_t *tri, int lane, CellRec *cell
             }
         }
         const uint64_t kb = __ballot(keep);
-        const uint32_t rank = __popcll(kb & ((1ull << lane) - 1ull));
+        const uint32_t rank = lanes_below(kb);
         nf = __popcll(kb);
         nv = 3 * nf;
         uint32_t v0 = 0, f0 = 0, ok = 1;
