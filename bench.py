@@ -508,6 +508,7 @@ def run_walk(args):
     fz = make()
     fuse(fz, untimed, "untimed")
     hiprt.synchronize()
+    fz.reset_timing()
     fz.set_timing(True, sdf_only=True, every=8)
     t0 = time.perf_counter()
     nq_sum = fuse(fz, timed, "timed")
@@ -542,6 +543,7 @@ def run_walk(args):
     fr = make()
     fuse(fr, untimed, "replay untimed")
     hiprt.synchronize()
+    fr.reset_timing()  # the counts cover the timed frames only
     fr.set_timing(True)
     nq_replay = fuse(fr, timed, "replay")
     hiprt.synchronize()

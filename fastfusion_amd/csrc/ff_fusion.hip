@@ -1722,13 +1722,11 @@ ff_ctx *ff_create(const ff_params *prm)
         return nullptr;
     }
     if (hipStreamCreate(&c->side) != hipSuccess ||
-        hipEventCreateWithFlags(&c->evMark, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&c->evMc, hipEventDisableTiming) != hipSuccess) {
         // the frame then runs on the main stream
         if (c->side) hipStreamDestroy(c->side);
-        if (c->evMark) hipEventDestroy(c->evMark);
         c->side = nullptr;
-        c->evMark = c->evMc = nullptr;
+        c->evMc = nullptr;
         (void)hipGetLastError();
     }
     if (c->side && (hipStreamCreate(&c->aux) != hipSuccess ||
@@ -1818,7 +1816,6 @@ void ff_destroy(ff_ctx *c)
         hipStreamSynchronize(c->side);
         hipStreamDestroy(c->side);
     }
-    if (c->evMark) hipEventDestroy(c->evMark);
     if (c->evMc) hipEventDestroy(c->evMc);
     if (c->aux) {
         hipStreamSynchronize(c->aux);
@@ -2011,7 +2008,7 @@ static bool capture(ff_ctx *c, void (*fn)(ff_ctx *, int, int), int W, int H, hip
 // this size (the first frame and a new size prepare on the main stream)
 static bool front_overlap(const ff_ctx *c, int W, int H)
 {
-    return c->side && c->evMark && c->evMc && !c->useGraphs && !c->timingAll && c->n && c->imgW == W &&
+    return c->side && c->evMc && !c->useGraphs && !c->timingAll && c->n && c->imgW == W &&
            c->imgH == H;
 }
 
@@ -2036,8 +2033,6 @@ static int run_frame(ff_ctx *c, int W, int H, bool overlap)
             HIPCHK(hipEventRecord(c->evConf, c->aux));
         }
         launch_front(c, W, H, c->side, true);  // carries the frame's parameters up
-        if (c->markPending) HIPCHK(hipStreamWaitEvent(c->side, c->evMark, 0));
-        c->markPending = false;
         launch_back(c, W, H, c->side);
         if (c->learned) {
             if (confAside) HIPCHK(hipStreamWaitEvent(c->side, c->evConf, 0));
@@ -2243,8 +2238,9 @@ int frame_finish(ff_ctx *c, int W, int H, bool timed)
     }
     // algorithmic bytes of the allocation: images read once, 16 B per
     // processed pixel (the SDF update's are added with its timed launches;
-    // batched ticks account their SDF launch on the lead context instead)
-    if (timed) {
+    // batched ticks account their SDF launch on the lead context instead);
+    // counted only while every phase is timed, like the phases' times
+    if (timed && c->timingAll) {
         c->phaseBytes[0] += (double)W * H * 2 + 16.0 * hf->nProcessed;
         c->phaseUnits[0] += hf->nProcessed;
     }

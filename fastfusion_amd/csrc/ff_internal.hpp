@@ -142,19 +142,21 @@ struct ff_ctx {
     std::vector<float> pxp, pyp;
     hipStream_t stream = nullptr;
     // a frame (bit planes, de-dup, descent, node commit, queue, SDF update)
-    // runs on `side`, beside the previous mesh update on `stream`: the
-    // descent waits for evMark, recorded after the update's passes over the
-    // tree and the stamps (its sampling and marching cubes read only bricks
-    // that already exist, which the allocation leaves alone), the SDF update
-    // for evMc, recorded after its marching cubes
+    // runs on `side`, beside the previous mesh update on `stream`, and only
+    // the SDF update waits: for evMc, recorded after the update's marching
+    // cubes (the last of its kernels to read bricks).  The descent and the
+    // allocation may run beside the update's partition and marking: the
+    // update reads the tree only through the bricks and branch slots counted
+    // when it was launched (a later frame's links point past them), and a
+    // brick the next frame stamps can only make a cell dirty, never clean
     hipStream_t side = nullptr;
-    hipEvent_t evMark = nullptr, evMc = nullptr;
+    hipEvent_t evMc = nullptr;
     // the learned confidence of an overlapped frame runs on `aux`, beside the
     // frame's de-dup and descent; its SDF update waits for evConf
     hipStream_t aux = nullptr;
     hipEvent_t evConf = nullptr;
     bool uploadOnSide = false;  // the frame in flight had its host images copied up on `side`
-    bool markPending = false, mcPending = false;  // recorded since the last frame
+    bool mcPending = false;  // evMc recorded since the last frame
     // map arrays
     uint32_t *d_tree = nullptr, *d_treeTmp = nullptr, *d_parent = nullptr, *d_stamp = nullptr, *d_queue = nullptr;
     ushort4 *d_leafinfo = nullptr;

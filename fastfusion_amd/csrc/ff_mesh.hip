@@ -123,7 +123,7 @@ struct MapRO {
 struct MeshBatchStream {
     const uint32_t *tree;
     const ushort4 *binfo;
-    uint32_t nBlocks, nLeavesTotal, n;
+    uint32_t nBlocks, nLeaves, nSlots, n;  // the tree as launched (bricks, branch slots)
     uint4 *term;
     uint32_t termCap;
     MeshCounters *mc;
@@ -238,10 +238,16 @@ static hipError_t hash_reset(MeshState *s, hipStream_t st)
 // ---------------------------------------------------------------------------
 // tree queries
 // ---------------------------------------------------------------------------
+// The mesh update reads the tree as it was when the update was launched:
+// bricks below nLeaves and branch slots below nSlots.  Indices are handed
+// out in increasing order and a slot, once linked, keeps its link, so a link
+// to a brick or branch at or past those counts was made by a later frame
+// (whose allocation may run beside the update) and reads as empty.
+//
 // Terminal node containing voxel (x, y, z): returns its size (0 outside the
 // volume), origin, and the deepest brick on the path.
-__device__ uint32_t locate(const uint32_t *__restrict__ tree, uint32_t nLeavesTotal, uint32_t n, int x, int y, int z,
-                           int org[3], uint32_t *leaf)
+__device__ uint32_t locate(const uint32_t *__restrict__ tree, uint32_t nLeaves, uint32_t nSlots, uint32_t n, int x,
+                           int y, int z, int org[3], uint32_t *leaf)
 {
     *leaf = FF_BRANCHINIT;
     if (x < 0 || y < 0 || z < 0 || x >= (int)n || y >= (int)n || z >= (int)n) return 0u;
@@ -253,9 +259,9 @@ __device__ uint32_t locate(const uint32_t *__restrict__ tree, uint32_t nLeavesTo
         if (y >= oy + (int)half) { co |= 2u; oy += half; }
         if (z >= oz + (int)half) { co |= 4u; oz += half; }
         const uint32_t l = tree[branch + co];
-        if (l < nLeavesTotal) *leaf = l;
+        if (l < nLeaves) *leaf = l;
         const uint32_t child = tree[branch + co + 8];
-        if (child == FF_BRANCHINIT || half <= FF_BL) {
+        if (child >= nSlots || half <= FF_BL) {
             org[0] = ox; org[1] = oy; org[2] = oz;
             return half;
         }
@@ -273,7 +279,8 @@ __device__ uint32_t locate(const uint32_t *__restrict__ tree, uint32_t nLeavesTo
 #define FF_JUMP_N 16
 #define FF_JUMP_CELLS (FF_JUMP_N * FF_JUMP_N * FF_JUMP_N)
 
-__device__ uint4 jump_entry(const uint32_t *__restrict__ tree, uint32_t nLeavesTotal, uint32_t n, uint32_t cell)
+__device__ uint4 jump_entry(const uint32_t *__restrict__ tree, uint32_t nLeaves, uint32_t nSlots, uint32_t n,
+                           uint32_t cell)
 {
     const uint32_t cs = n / FF_JUMP_N;
     const int x = (int)((cell % FF_JUMP_N) * cs), y = (int)(((cell / FF_JUMP_N) % FF_JUMP_N) * cs),
@@ -286,9 +293,9 @@ __device__ uint4 jump_entry(const uint32_t *__restrict__ tree, uint32_t nLeavesT
         if (y >= oy + (int)half) { co |= 2u; oy += half; }
         if (z >= oz + (int)half) { co |= 4u; oz += half; }
         const uint32_t l = tree[branch + co];
-        if (l < nLeavesTotal) leaf = l;
+        if (l < nLeaves) leaf = l;
         const uint32_t child = tree[branch + co + 8];
-        if (child == FF_BRANCHINIT || half <= FF_BL) return make_uint4(FF_BRANCHINIT, leaf, half, 0u);
+        if (child >= nSlots || half <= FF_BL) return make_uint4(FF_BRANCHINIT, leaf, half, 0u);
         branch = child;
         half >>= 1;
     }
@@ -298,7 +305,8 @@ __device__ uint4 jump_entry(const uint32_t *__restrict__ tree, uint32_t nLeavesT
 // locate() starting from the jump table: the same result, four dependent
 // tree loads fewer
 __device__ uint32_t locate_jump(const uint4 *__restrict__ jump, const uint32_t *__restrict__ tree,
-                                uint32_t nLeavesTotal, uint32_t n, int x, int y, int z, int org[3], uint32_t *leaf)
+                                uint32_t nLeaves, uint32_t nSlots, uint32_t n, int x, int y, int z, int org[3],
+                                uint32_t *leaf)
 {
     *leaf = FF_BRANCHINIT;
     if (x < 0 || y < 0 || z < 0 || x >= (int)n || y >= (int)n || z >= (int)n) return 0u;
@@ -318,9 +326,9 @@ __device__ uint32_t locate_jump(const uint4 *__restrict__ jump, const uint32_t *
         if (y >= oy + (int)half) { co |= 2u; oy += half; }
         if (z >= oz + (int)half) { co |= 4u; oz += half; }
         const uint32_t l = tree[branch + co];
-        if (l < nLeavesTotal) *leaf = l;
+        if (l < nLeaves) *leaf = l;
         const uint32_t child = tree[branch + co + 8];
-        if (child == FF_BRANCHINIT || half <= FF_BL) {
+        if (child >= nSlots || half <= FF_BL) {
             org[0] = ox; org[1] = oy; org[2] = oz;
             return half;
         }
@@ -414,7 +422,7 @@ __device__ __forceinline__ uint32_t block_rank(bool want, uint32_t *total)
 // ---------------------------------------------------------------------------
 #define FF_TERM_THREADS 256  // block size of k_terminal (every thread of a block appends together)
 __device__ __forceinline__ void terminal_body(const uint32_t *__restrict__ tree, const ushort4 *__restrict__ binfo,
-                                              uint32_t nBlocks, uint32_t nLeavesTotal, uint32_t n,
+                                              uint32_t nBlocks, uint32_t nLeaves, uint32_t nSlots, uint32_t n,
                                               uint4 *__restrict__ term, uint32_t termCap, MeshCounters *mc)
 {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -424,12 +432,12 @@ __device__ __forceinline__ void terminal_body(const uint32_t *__restrict__ tree,
         const uint32_t b = t >> 3, co = t & 7u;
         const ushort4 bi = binfo[b];
         const uint32_t half = bi.w;
-        if (tree[b * 16 + 8 + co] == FF_BRANCHINIT && half != 0) {
+        if (tree[b * 16 + 8 + co] >= nSlots && half != 0) {
             const int x = bi.x + (int)(half * (co & 1u)), y = bi.y + (int)(half * ((co >> 1) & 1u)),
                       z = bi.z + (int)(half * ((co >> 2) & 1u));
             int org[3];
             uint32_t lt;
-            const uint32_t sz = locate(tree, nLeavesTotal, n, x, y, z, org, &lt);
+            const uint32_t sz = locate(tree, nLeaves, nSlots, n, x, y, z, org, &lt);
             if (sz == half && lt != FF_BRANCHINIT) {
                 want = true;
                 rec = make_uint4((uint32_t)x | ((uint32_t)y << 16), (uint32_t)z, half, lt);
@@ -445,17 +453,17 @@ __device__ __forceinline__ void terminal_body(const uint32_t *__restrict__ tree,
 
 // (blocks past the terminal candidates fill the jump table, 256 cells each)
 __global__ __launch_bounds__(FF_TERM_THREADS) void k_terminal(const uint32_t *__restrict__ tree, const ushort4 *__restrict__ binfo, uint32_t nBlocks,
-                           uint32_t nLeavesTotal, uint32_t n, uint4 *__restrict__ term, uint32_t termCap,
+                           uint32_t nLeaves, uint32_t nSlots, uint32_t n, uint4 *__restrict__ term, uint32_t termCap,
                            MeshCounters *mc, uint4 *__restrict__ jump, MeshCounters *clearNext)
 {
     if (blockIdx.x == 0 && threadIdx.x == 0) *clearNext = MeshCounters{};  // the next update's counters
     const uint32_t tb = (nBlocks * 8u + blockDim.x - 1u) / blockDim.x;
     if (blockIdx.x >= tb) {
         const uint32_t cell = (blockIdx.x - tb) * blockDim.x + threadIdx.x;
-        if (cell < FF_JUMP_CELLS) jump[cell] = jump_entry(tree, nLeavesTotal, n, cell);
+        if (cell < FF_JUMP_CELLS) jump[cell] = jump_entry(tree, nLeaves, nSlots, n, cell);
         return;
     }
-    terminal_body(tree, binfo, nBlocks, nLeavesTotal, n, term, termCap, mc);
+    terminal_body(tree, binfo, nBlocks, nLeaves, nSlots, n, term, termCap, mc);
 }
 
 // the batched forms of the per-stream mesh kernels: stream blockIdx.z of S
@@ -463,7 +471,7 @@ __global__ __launch_bounds__(FF_TERM_THREADS) void k_terminal_b(const MeshBatchS
 {
     const MeshBatchStream &a = S[blockIdx.z];
     if (blockIdx.x * blockDim.x >= a.nBlocks * 8u) return;
-    terminal_body(a.tree, a.binfo, a.nBlocks, a.nLeavesTotal, a.n, a.term, a.termCap, a.mc);
+    terminal_body(a.tree, a.binfo, a.nBlocks, a.nLeaves, a.nSlots, a.n, a.term, a.termCap, a.mc);
 }
 
 // ---------------------------------------------------------------------------
@@ -480,7 +488,7 @@ __device__ __forceinline__ int opt_code(int ox, int oy, int oz) { return ox + 4 
 #define FF_CELL_STAGE 256  // cells a k_cells block stages before appending (8 nodes emit at most 216)
 
 __device__ __forceinline__ void cells_body(const uint4 *__restrict__ jump, const uint32_t *__restrict__ tree,
-                                           uint32_t nLeavesTotal, uint32_t n,
+                                           uint32_t nLeaves, uint32_t nSlots, uint32_t n,
                                            const uint4 *__restrict__ term, uint32_t termCap, MeshCounters *mc,
                                            CellRec *cells, uint32_t cap)
 {
@@ -526,8 +534,8 @@ __device__ __forceinline__ void cells_body(const uint4 *__restrict__ jump, const
                         q[k] = op[k] == 0 ? p[k] - 1 : (op[k] == 1 ? p[k] : (op[k] == 2 ? p[k] + sz - 1 : p[k] + sz));
                     int org[3];
                     uint32_t lf;
-                    r.x = jump ? locate_jump(jump, tree, nLeavesTotal, n, q[0], q[1], q[2], org, &lf)
-                               : locate(tree, nLeavesTotal, n, q[0], q[1], q[2], org, &lf);
+                    r.x = jump ? locate_jump(jump, tree, nLeaves, nSlots, n, q[0], q[1], q[2], org, &lf)
+                               : locate(tree, nLeaves, nSlots, n, q[0], q[1], q[2], org, &lf);
                     r.y = lf;
                 }
             }
@@ -627,17 +635,18 @@ __device__ __forceinline__ void cells_body(const uint4 *__restrict__ jump, const
     flush();
 }
 
-__global__ __launch_bounds__(512) void k_cells(const uint32_t *__restrict__ tree, uint32_t nLeavesTotal, uint32_t n,
+__global__ __launch_bounds__(512) void k_cells(const uint32_t *__restrict__ tree, uint32_t nLeaves, uint32_t nSlots,
+                                               uint32_t n,
                                                const uint4 *__restrict__ term, uint32_t termCap, MeshCounters *mc,
                                                CellRec *cells, uint32_t cap, const uint4 *__restrict__ jump)
 {
-    cells_body(jump, tree, nLeavesTotal, n, term, termCap, mc, cells, cap);
+    cells_body(jump, tree, nLeaves, nSlots, n, term, termCap, mc, cells, cap);
 }
 
 __global__ __launch_bounds__(512) void k_cells_b(const MeshBatchStream *__restrict__ S)
 {
     const MeshBatchStream &a = S[blockIdx.z];
-    cells_body(nullptr, a.tree, a.nLeavesTotal, a.n, a.term, a.termCap, a.mc, a.cells, a.cellCap);
+    cells_body(nullptr, a.tree, a.nLeaves, a.nSlots, a.n, a.term, a.termCap, a.mc, a.cells, a.cellCap);
 }
 
 // ---------------------------------------------------------------------------
@@ -2159,12 +2168,15 @@ bool mesh_launch_pre(ff_ctx *c, bool timed)
     }
     if (!s->mcZeroNow) k_mesh_begin<<<1, 64, 0, c->stream>>>(s->d_mc);
     const uint32_t nBlocks = c->nBranchesUsed / 16;
+    // the tree as of this update (the next frame's allocation may link new
+    // nodes while it runs)
+    const uint32_t nLeaves = std::min(c->nLeavesUsed, c->nLeavesTotal), nSlots = c->nBranchesUsed;
     // (plus FF_JUMP_CELLS / 256 blocks that fill the jump table)
     k_terminal<<<(nBlocks * 8 + FF_TERM_THREADS - 1) / FF_TERM_THREADS + FF_JUMP_CELLS / FF_TERM_THREADS,
                  FF_TERM_THREADS, 0, c->stream>>>(
-        c->d_tree, s->d_binfo, nBlocks, c->nLeavesTotal, c->n, s->d_term, s->termCap, s->d_mc, s->d_jump,
+        c->d_tree, s->d_binfo, nBlocks, nLeaves, nSlots, c->n, s->d_term, s->termCap, s->d_mc, s->d_jump,
         mc_other(s));
-    k_cells<<<1024, 512, 0, c->stream>>>(c->d_tree, c->nLeavesTotal, c->n, s->d_term, s->termCap, s->d_mc,
+    k_cells<<<1024, 512, 0, c->stream>>>(c->d_tree, nLeaves, nSlots, c->n, s->d_term, s->termCap, s->d_mc,
                                          s->d_cells[nxt], s->cellCap, s->d_jump);
     const MapRO m = s->m;
     k_mark<<<1024, FF_MARK_THREADS, 0, c->stream>>>(s->d_cells[nxt], s->d_cells[s->cur], s->d_hkeys, s->d_hval, s->hmask,
@@ -2184,7 +2196,8 @@ static MeshBatchStream mesh_batch_stream(ff_ctx *c)
     a.tree = c->d_tree;
     a.binfo = s->d_binfo;
     a.nBlocks = c->nBranchesUsed / 16;
-    a.nLeavesTotal = c->nLeavesTotal;
+    a.nLeaves = std::min(c->nLeavesUsed, c->nLeavesTotal);
+    a.nSlots = c->nBranchesUsed;
     a.n = c->n;
     a.term = s->d_term;
     a.termCap = s->termCap;
@@ -2402,7 +2415,7 @@ int mesh_finish(ff_ctx *c, bool timed)
         s->builtNL = s->launchNL;
         s->builtGrows = s->launchGrows;
     }
-    c->phaseUnits[2] += h.nDirty;
+    if (c->timingAll) c->phaseUnits[2] += h.nDirty;
     if (s->mcTimed) {
         float ms = 0.0f;
         if (hipEventElapsedTime(&ms, c->ev[8], c->ev[9]) == hipSuccess) {

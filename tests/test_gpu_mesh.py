@@ -140,6 +140,42 @@ def test_mesh_compaction_behind_overlapped_frames():
     assert cap >= 1 << 18 and compactions >= 1  # (an update that overflowed doubles the pools)
 
 
+def test_mesh_update_beside_next_frame():
+    """Each update is enqueued and the next frame fused at once: its
+    allocation, which links new branches and bricks into the tree and stamps
+    the bricks it queues, runs beside the update's partition and marking.
+    The update must mesh the map as it was when it was enqueued (links to
+    nodes past the counts it was launched with read as empty), so its mesh,
+    read after the next frame returned, equals the oracle's mesh of the map
+    before that frame."""
+    sc, R, t, cfg = synth.config_c1(n_frames=200)
+    cfg = dict(cfg, w=320, h=240)
+    K = (262.5, 262.5, 159.5, 119.5)
+    frames = list(range(0, 48, 3))
+    gpu, ora = make_pair(cfg["scale"], cfg["threshold"])
+    w, h = cfg["w"], cfg["h"]
+
+    def frame(i):
+        return sc.render(R[i].reshape(3, 3), t[i], K, w, h, noise=cfg["noise"], dropout=cfg["dropout"],
+                         seed=1000 + i)
+
+    d, bgr = frame(frames[0])
+    gpu.addMap(d, R[frames[0]], t[frames[0]], K, bgr, 1 / 5000.0, cfg["maxcam"])
+    ora.add_map(d, R[frames[0]], t[frames[0]], K, bgr, 1 / 5000.0, cfg["maxcam"])
+    grew = 0
+    for a, b in zip(frames[:-1], frames[1:]):
+        before = gpu.counts()["n_branches_used"]
+        gpu.updateMeshes()
+        d, bgr = frame(b)
+        nq = gpu.addMap(d, R[b], t[b], K, bgr, 1 / 5000.0, cfg["maxcam"])
+        grew += gpu.counts()["n_branches_used"] > before
+        compare_mesh(gpu, ora, cfg["scale"], f"update of frame {a}, frame {b} beside it")
+        assert nq == ora.add_map(d, R[b], t[b], K, bgr, 1 / 5000.0, cfg["maxcam"])
+    assert grew >= 5  # the frames beside the updates linked new branches
+    gpu.updateMeshes()
+    compare_mesh(gpu, ora, cfg["scale"], f"frame {frames[-1]}")
+
+
 def test_mesh_pools_too_small_grow():
     """Pools of 4 096 vertices, far below the live mesh: updates overflow,
     their compactions lose geometry, so the pools double and the next update
