@@ -26,6 +26,7 @@
 #include <hip/hip_ext.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -1273,16 +1274,21 @@ __global__ __launch_bounds__(512) void k_sdf(const uint32_t *__restrict__ queue,
     // blocks loop over the queued bricks; the count comes from the device
     const bool open = sdf_gate_open(gv, fd->n, gate);
     if (hostFc && blockIdx.x == 0 && threadIdx.x == 0) {
-        // the frame's final counters to host memory (the host reads them
-        // once the stream is done), with whether this update ran
+        // the frame's final counters to host memory, with whether this
+        // update runs, and the previous mesh update's counters (its kernels
+        // precede this one on the stream), in place of separate copies; the
+        // frame's stamp last, after a system-scope fence: a host that sees
+        // the stamp sees the rest (it need not wait for the update's end)
         uint4 *h = reinterpret_cast<uint4 *>(hostFc);
         h[0] = gv.w[0]; h[1] = gv.w[1]; h[2] = gv.w[2];
-        h[3] = make_uint4(open ? 1u : 0u, 0u, 0u, 0u);
-    }
-    if (carrySrc && blockIdx.x == 0 && threadIdx.x < 2) {
-        // the previous mesh update's counters (its kernels precede this one
-        // on the stream) to host memory, in place of a separate copy
-        carryDst[threadIdx.x] = carrySrc[threadIdx.x];
+        if (carrySrc) {
+            const uint4 a = carrySrc[0], b = carrySrc[1];
+            carryDst[0] = a;
+            carryDst[1] = b;
+        }
+        hostFc->sdfRan = open ? 1u : 0u;
+        __threadfence_system();
+        *reinterpret_cast<volatile uint32_t *>(&hostFc->pad[0]) = fd->stamp;
     }
     if (!open) return;
     const uint16_t *__restrict__ depth = fd->depth;
@@ -1565,6 +1571,7 @@ int ensure_image(ff_ctx *c, int W, int H)
     if (c->capW >= W && c->capH >= H) return FF_OK;
     const size_t npix = (size_t)W * H;
     const int words = (W + 31) / 32;
+    frame_join(c);  // the frame in flight reads the buffers freed here
     c->freeFrameBuffers();
     HIPCHK(hipMalloc(&c->d_depth, npix * 2));
     HIPCHK(hipMalloc(&c->d_bgr, npix * 3));
@@ -1721,20 +1728,19 @@ ff_ctx *ff_create(const ff_params *prm)
         delete c;
         return nullptr;
     }
-    if (hipStreamCreate(&c->side) != hipSuccess ||
-        hipEventCreateWithFlags(&c->evMc, hipEventDisableTiming) != hipSuccess) {
+    int waitValue = 0;
+    c->seqOk = hipDeviceGetAttribute(&waitValue, hipDeviceAttributeCanUseStreamWaitValue, c->device) == hipSuccess &&
+               waitValue != 0 && hipMalloc(&c->d_seq, sizeof(c->seq)) == hipSuccess &&
+               hipMemset(c->d_seq, 0, sizeof(c->seq)) == hipSuccess;
+    if (!c->seqOk || hipStreamCreate(&c->side) != hipSuccess) {
         // the frame then runs on the main stream
         if (c->side) hipStreamDestroy(c->side);
         c->side = nullptr;
-        c->evMc = nullptr;
         (void)hipGetLastError();
     }
-    if (c->side && (hipStreamCreate(&c->aux) != hipSuccess ||
-                    hipEventCreateWithFlags(&c->evConf, hipEventDisableTiming) != hipSuccess)) {
+    if (c->side && hipStreamCreate(&c->aux) != hipSuccess) {
         // the confidence then runs on the side stream with the frame
-        if (c->aux) hipStreamDestroy(c->aux);
         c->aux = nullptr;
-        c->evConf = nullptr;
         (void)hipGetLastError();
     }
     const size_t NL = c->nLeavesTotal;
@@ -1797,6 +1803,10 @@ void ff_destroy(ff_ctx *c)
 {
     if (!c) return;
     hipSetDevice(c->device);
+    // every stream drained before anything is freed (a frame's SDF update
+    // may still run on the side stream)
+    if (c->side) hipStreamSynchronize(c->side);
+    if (c->aux) hipStreamSynchronize(c->aux);
     if (c->stream) hipStreamSynchronize(c->stream);
     batch_free(c);
     mesh_free(c);
@@ -1816,12 +1826,11 @@ void ff_destroy(ff_ctx *c)
         hipStreamSynchronize(c->side);
         hipStreamDestroy(c->side);
     }
-    if (c->evMc) hipEventDestroy(c->evMc);
+    hipFree(c->d_seq);
     if (c->aux) {
         hipStreamSynchronize(c->aux);
         hipStreamDestroy(c->aux);
     }
-    if (c->evConf) hipEventDestroy(c->evConf);
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;
 }
@@ -2008,7 +2017,7 @@ static bool capture(ff_ctx *c, void (*fn)(ff_ctx *, int, int), int W, int H, hip
 // this size (the first frame and a new size prepare on the main stream)
 static bool front_overlap(const ff_ctx *c, int W, int H)
 {
-    return c->side && c->evMc && !c->useGraphs && !c->timingAll && c->n && c->imgW == W &&
+    return c->side && c->seqOk && !c->useGraphs && !c->timingAll && c->n && c->imgW == W &&
            c->imgH == H;
 }
 
@@ -2022,25 +2031,26 @@ static int run_frame(ff_ctx *c, int W, int H, bool overlap)
         // hash turnover may still run beside it
         // the learned confidence reads only the image: it starts at once on
         // the third stream, beside the de-dup and the descent
-        const bool confAside = c->learned && c->aux && c->evConf;
+        bool confAside = c->learned && c->aux;
         if (confAside) {
-            if (c->uploadOnSide) {
-                // the frame's host images go up on the side stream first
-                HIPCHK(hipEventRecord(c->evConf, c->side));
-                HIPCHK(hipStreamWaitEvent(c->aux, c->evConf, 0));
-            }
+            // (the previous frame's SDF update, which reads the
+            // confidences, may still run)
+            seq_wait(c, FF_SEQ_SDF, c->aux);
+            // the frame's host images go up on the side stream first
+            if (c->uploadOnSide && seq_signal(c, FF_SEQ_UP, c->side)) seq_wait(c, FF_SEQ_UP, c->aux);
             learned_launch_direct(c, W, H, c->aux);
-            HIPCHK(hipEventRecord(c->evConf, c->aux));
+            confAside = seq_signal(c, FF_SEQ_CONF, c->aux);  // (else aux has drained)
         }
         launch_front(c, W, H, c->side, true);  // carries the frame's parameters up
         launch_back(c, W, H, c->side);
         if (c->learned) {
-            if (confAside) HIPCHK(hipStreamWaitEvent(c->side, c->evConf, 0));
-            else learned_launch(c, W, H, c->side);
+            if (confAside) seq_wait(c, FF_SEQ_CONF, c->side);
+            else if (!c->aux) learned_launch(c, W, H, c->side);
         }
-        if (c->mcPending) HIPCHK(hipStreamWaitEvent(c->side, c->evMc, 0));
+        if (c->mcPending) seq_wait(c, FF_SEQ_MC, c->side);
         c->mcPending = false;
         launch_sdf_only(c, W, H, true, c->side);  // which leaves the counters in h_fc
+        c->sdfInFlight = seq_signal(c, FF_SEQ_SDF, c->side);  // (else the side stream has drained)
         HIPCHK(hipGetLastError());
         return FF_OK;
     }
@@ -2178,14 +2188,19 @@ int frame_finish(ff_ctx *c, int W, int H, bool timed)
             c->phaseMs[0] += ms;
             c->phaseCalls[0]++;
         }
-        if (c->sdfTimed && hf->sdfRan && hipEventElapsedTime(&ms, c->ev[2], c->ev[3]) == hipSuccess) {
-            c->phaseMs[1] += ms;
-            c->phaseCalls[1]++;
-            c->phaseBytes[1] += sdf_bytes(c, hf->nQueued, W, H);
-            c->phaseUnits[1] += hf->nQueued;
+        if (c->sdfTimed && hf->sdfRan) {
+            // read once the update has ended (at the next frame or join)
+            c->sdfTimedPending = true;
+            c->sdfTimedBytes = sdf_bytes(c, hf->nQueued, W, H);
+            c->sdfTimedUnits = hf->nQueued;
         }
     }
     c->sdfTimed = false;
+    if (!hf->sdfRan || (hf->flags & (FF_FLAG_ORDER_BIG | FF_FLAG_HASH_FULL)) || hf->listCount > c->hv.listCap) {
+        // the host's turn (an SDF update that stood down, a large or an
+        // overflowing frame): the side stream drained first
+        frame_join(c);
+    }
     if (hf->flags & FF_FLAG_ORDER_BIG) {
         const int r = order_big_frame(c, W, H);
         if (r) return r;
@@ -2251,9 +2266,96 @@ int frame_finish(ff_ctx *c, int W, int H, bool timed)
     return (int)c->nQueued;
 }
 
+bool await_word(const uint32_t *w, uint32_t want, hipStream_t st)
+{
+    const volatile uint32_t *v = w;
+    for (uint32_t k = 1;; k++) {
+        if (*v == want) break;
+        if ((k & 255u) == 0u) {
+            const hipError_t e = hipStreamQuery(st);
+            if (e != hipErrorNotReady) {
+                (void)hipGetLastError();
+                if (*v != want) return false;
+                break;
+            }
+        }
+        __builtin_ia32_pause();
+    }
+    std::atomic_thread_fence(std::memory_order_acquire);  // what was written before the word
+    return true;
+}
+
+// the SDF phase's time of a timed frame whose update had not ended when the
+// frame returned (the caller knows it has ended by now)
+static void sdf_timing_collect(ff_ctx *c)
+{
+    if (!c->sdfTimedPending) return;
+    c->sdfTimedPending = false;
+    float ms = 0.0f;
+    if (hipEventSynchronize(c->ev[3]) == hipSuccess && hipEventElapsedTime(&ms, c->ev[2], c->ev[3]) == hipSuccess) {
+        c->phaseMs[1] += ms;
+        c->phaseCalls[1]++;
+        c->phaseBytes[1] += c->sdfTimedBytes;
+        c->phaseUnits[1] += c->sdfTimedUnits;
+    } else {
+        (void)hipGetLastError();
+    }
+}
+
+bool seq_signal(ff_ctx *c, int k, hipStream_t st)
+{
+    const uint32_t v = c->seq[k] + 1u;
+    if (hipStreamWriteValue32(st, c->d_seq + k, v, 0) == hipSuccess) {
+        c->seq[k] = v;
+        return true;
+    }
+    (void)hipGetLastError();
+    hipStreamSynchronize(st);
+    return false;
+}
+
+void seq_wait(ff_ctx *c, int k, hipStream_t st)
+{
+    if (hipStreamWaitValue32(st, c->d_seq + k, c->seq[k], hipStreamWaitValueGte, 0xffffffffu) != hipSuccess) {
+        (void)hipGetLastError();
+        hipDeviceSynchronize();
+    }
+}
+
+// the sequence words back to zero long before a value could wrap (with
+// every stream of the context drained, no wait is pending)
+static void seq_rebase(ff_ctx *c)
+{
+    if (c->seq[0] < 0x40000000u && c->seq[1] < 0x40000000u && c->seq[2] < 0x40000000u && c->seq[3] < 0x40000000u)
+        return;
+    frame_join(c);
+    hipStreamSynchronize(c->stream);
+    if (c->aux) hipStreamSynchronize(c->aux);
+    hipMemset(c->d_seq, 0, sizeof(c->seq));
+    for (uint32_t &v : c->seq) v = 0;
+}
+
+void frame_join(ff_ctx *c, bool stream)
+{
+    if (!c->sdfInFlight) return;
+    c->sdfInFlight = false;
+    if (stream) {
+        seq_wait(c, FF_SEQ_SDF, c->stream);
+        return;
+    }
+    hipStreamSynchronize(c->side);
+    sdf_timing_collect(c);
+}
+
 static int add_map_impl(ff_ctx *c, const uint16_t *depth, const uint8_t *bgr, int W, int H, const double R[9],
                         const double t[3], const double K[4], float scaling, float maxcam, bool overlap)
 {
+    // a frame on the main stream waits there for the previous frame's SDF
+    // update; an overlapped one follows it on the side stream, and the
+    // previous update's timing events are read before they are recorded again
+    if (!overlap) frame_join(c, true);
+    sdf_timing_collect(c);
+    seq_rebase(c);
     int rc = frame_prepare(c, depth, bgr, W, H, R, t, K, scaling, maxcam, !overlap);
     if (rc) return rc;
     c->lastPbits = c->d_pbits;
@@ -2263,9 +2365,19 @@ static int add_map_impl(ff_ctx *c, const uint16_t *depth, const uint8_t *bgr, in
     } else {
         launch_empty(c, W, H);
     }
-    // (an overlapped frame ran on the side stream; work enqueued on the main
-    // stream from here on starts after it)
-    HIPCHK(hipStreamSynchronize(overlap && c->n ? c->side : c->stream));
+    if (overlap && c->n) {
+        // the frame ran on the side stream: its counters (and the previous
+        // mesh update's) are in host memory once its SDF update has started,
+        // which may still run when the call returns
+        if (!await_word(&c->h_fc->pad[0], c->stampCounter, c->side)) {
+            c->sdfInFlight = false;
+            HIPCHK(hipStreamSynchronize(c->side));
+            c->err = "the frame's SDF update did not report its counters";
+            return FF_E_DEVICE;
+        }
+    } else {
+        HIPCHK(hipStreamSynchronize(c->stream));
+    }
     return frame_finish(c, W, H, true);
 }
 
@@ -2288,6 +2400,7 @@ int ff_add_map(ff_ctx *c, const uint16_t *depth, size_t depth_stride, const uint
     if (!bgr_stride) bgr_stride = (size_t)W * 3;
     // the images go up on the stream that first reads them
     const bool overlap = front_overlap(c, W, H);
+    if (!overlap) frame_join(c, true);  // the previous frame may still read c->d_depth, c->d_bgr
     hipStream_t us = overlap ? c->side : c->stream;
     c->uploadOnSide = overlap;
     HIPCHK(hipMemcpy2DAsync(c->d_depth, (size_t)W * 2, depth, depth_stride, (size_t)W * 2, H, hipMemcpyHostToDevice,
@@ -2310,7 +2423,8 @@ int ff_add_map_device(ff_ctx *c, const uint16_t *depth_dev, const uint8_t *bgr_d
     hipSetDevice(c->device);
     int rc = ensure_image(c, W, H);
     if (rc) return rc;
-    // read in place: the call returns after the frame's kernels completed
+    // read in place: the frame's SDF update may still read the images when
+    // the call returns (until the next frame's counters are in)
     return add_map_impl(c, depth_dev, c->useColor ? bgr_dev : nullptr, W, H, R, t, K, scaling, maxcam,
                         front_overlap(c, W, H));
 }
@@ -2394,6 +2508,8 @@ int ff_get_timing(ff_ctx *c, double *ms, uint64_t *calls, double *bytes, uint64_
     const int rc = mesh_complete(c);
     if (rc != FF_OK) return rc;
     HIPCHK(hipStreamSynchronize(c->stream));
+    if (c->side) HIPCHK(hipStreamSynchronize(c->side));
+    sdf_timing_collect(c);
     for (int i = 0; i < 3; i++) {
         if (ms) ms[i] = c->phaseMs[i];
         if (calls) calls[i] = c->phaseCalls[i];
@@ -2408,6 +2524,8 @@ int ff_reset_timing(ff_ctx *c)
     if (!c) return FF_E_ARG;
     const int rc = mesh_complete(c);  // its counts belong before the reset
     if (rc != FF_OK) return rc;
+    if (c->side) HIPCHK(hipStreamSynchronize(c->side));
+    c->sdfTimedPending = false;
     for (int i = 0; i < 3; i++) {
         c->phaseMs[i] = 0.0;
         c->phaseCalls[i] = 0;
@@ -2448,6 +2566,7 @@ int ff_get_counts(ff_ctx *c, ff_counts *o)
 int ff_copy_tree(ff_ctx *c, uint32_t *out, size_t n)
 {
     if (!c || !out || n > c->nBranchesTotal) return FF_E_ARG;
+    frame_join(c);
     HIPCHK(hipStreamSynchronize(c->stream));
     HIPCHK(hipMemcpy(out, c->d_tree, n * sizeof(uint32_t), hipMemcpyDeviceToHost));
     return FF_OK;
@@ -2456,6 +2575,7 @@ int ff_copy_tree(ff_ctx *c, uint32_t *out, size_t n)
 int ff_copy_leaves(ff_ctx *c, uint16_t *pos3, uint16_t *scale, uint32_t *parent, size_t n)
 {
     if (!c || n > c->nLeavesTotal) return FF_E_ARG;
+    frame_join(c);
     HIPCHK(hipStreamSynchronize(c->stream));
     std::vector<ushort4> info(n);
     HIPCHK(hipMemcpy(info.data(), c->d_leafinfo, n * sizeof(ushort4), hipMemcpyDeviceToHost));
@@ -2470,6 +2590,7 @@ int ff_copy_leaves(ff_ctx *c, uint16_t *pos3, uint16_t *scale, uint32_t *parent,
 int ff_copy_bricks(ff_ctx *c, const uint32_t *leaves, size_t n, float *dist, float *weight, uint16_t *color)
 {
     if (!c || (!leaves && n)) return FF_E_ARG;
+    frame_join(c);
     HIPCHK(hipStreamSynchronize(c->stream));
     for (size_t i = 0; i < n; i++) {
         const uint32_t l = leaves[i];
@@ -2486,6 +2607,7 @@ int ff_copy_bricks(ff_ctx *c, const uint32_t *leaves, size_t n, float *dist, flo
 int ff_copy_queue(ff_ctx *c, uint32_t *out, size_t cap)
 {
     if (!c || !out) return FF_E_ARG;
+    frame_join(c);
     HIPCHK(hipStreamSynchronize(c->stream));
     const size_t n = std::min<size_t>(cap, c->nQueued);
     HIPCHK(hipMemcpy(out, c->d_queue, n * sizeof(uint32_t), hipMemcpyDeviceToHost));
@@ -2496,6 +2618,7 @@ int ff_copy_queue(ff_ctx *c, uint32_t *out, size_t cap)
 int ff_copy_processed(ff_ctx *c, uint8_t *out, size_t n)
 {
     if (!c || !out || n < (size_t)c->imgW * c->imgH) return FF_E_ARG;
+    frame_join(c);
     HIPCHK(hipStreamSynchronize(c->stream));
     const int W = c->imgW, H = c->imgH, words = (W + 31) / 32;
     (void)words;

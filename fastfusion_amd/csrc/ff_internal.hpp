@@ -143,20 +143,39 @@ struct ff_ctx {
     hipStream_t stream = nullptr;
     // a frame (bit planes, de-dup, descent, node commit, queue, SDF update)
     // runs on `side`, beside the previous mesh update on `stream`, and only
-    // the SDF update waits: for evMc, recorded after the update's marching
-    // cubes (the last of its kernels to read bricks).  The descent and the
+    // the SDF update waits: for the update's marching cubes (the last of its
+    // kernels to read bricks; sequence word FF_SEQ_MC).  The descent and the
     // allocation may run beside the update's partition and marking: the
     // update reads the tree only through the bricks and branch slots counted
     // when it was launched (a later frame's links point past them), and a
     // brick the next frame stamps can only make a cell dirty, never clean
     hipStream_t side = nullptr;
-    hipEvent_t evMc = nullptr;
+    // Order between the streams: sequence words in device memory, one per
+    // kind of producer; the producing stream writes the word's next value
+    // after its work (hipStreamWriteValue32) and a consuming stream waits
+    // until the word reaches it (hipStreamWaitValue32).  Measured on the
+    // MI355X: ~2 us of host time each and ~4 us from the producer's end to
+    // the consumer's start, where an event wait on a pending event costs
+    // ~16 us of host time and ~10 us.  seq[k]: the last value enqueued.
+    uint32_t *d_seq = nullptr;
+    uint32_t seq[4] = {0, 0, 0, 0};
+    bool seqOk = false;  // the runtime takes stream memory operations
+    // such a frame returns once its SDF update has started (the update's
+    // first block writes the frame's counters, then the frame's stamp, to
+    // mapped host memory); the next mesh update waits for FF_SEQ_SDF on
+    // `stream`, and every other reader of the map joins the frame first
+    // (frame_join); sdfInFlight until then
+    bool sdfInFlight = false;
+    // an SDF update timed by its dispatch's events whose frame returned
+    // before it ended: its time is read at the next frame or join
+    bool sdfTimedPending = false;
+    double sdfTimedBytes = 0.0;
+    uint32_t sdfTimedUnits = 0;
     // the learned confidence of an overlapped frame runs on `aux`, beside the
-    // frame's de-dup and descent; its SDF update waits for evConf
+    // frame's de-dup and descent; its SDF update waits for FF_SEQ_CONF
     hipStream_t aux = nullptr;
-    hipEvent_t evConf = nullptr;
     bool uploadOnSide = false;  // the frame in flight had its host images copied up on `side`
-    bool mcPending = false;  // evMc recorded since the last frame
+    bool mcPending = false;  // a mesh update signalled FF_SEQ_MC since the last frame
     // map arrays
     uint32_t *d_tree = nullptr, *d_treeTmp = nullptr, *d_parent = nullptr, *d_stamp = nullptr, *d_queue = nullptr;
     ushort4 *d_leafinfo = nullptr;
@@ -245,6 +264,23 @@ void launch_back(ff_ctx *c, int W, int H, hipStream_t st);
 void launch_sdf(ff_ctx *c, int W, int H);
 void launch_empty(ff_ctx *c, int W, int H);
 int frame_finish(ff_ctx *c, int W, int H, bool timed);
+// the frame in flight joined (a no-op when none is): stream = true enqueues a
+// wait for its SDF update on `stream` (the mesh update's path), false waits
+// on the host; later work on `stream` and host reads see the whole frame
+void frame_join(ff_ctx *c, bool stream = false);
+// waits until the mapped host word *w equals want (a kernel's report);
+// false when the stream drained or failed without writing it
+bool await_word(const uint32_t *w, uint32_t want, hipStream_t st);
+// the sequence words (ff_ctx::d_seq): after the mesh update's marching
+// cubes, after a frame's SDF update, after the learned confidence, after a
+// frame's host images went up on the side stream
+enum { FF_SEQ_MC = 0, FF_SEQ_SDF = 1, FF_SEQ_CONF = 2, FF_SEQ_UP = 3 };
+// enqueues on st: word k takes its next value; false (st drained instead)
+// when the runtime refused
+bool seq_signal(ff_ctx *c, int k, hipStream_t st);
+// enqueues on st a wait until word k holds the last value signalled (one
+// already signalled: a wait for work enqueued before)
+void seq_wait(ff_ctx *c, int k, hipStream_t st);
 int ensure_image(ff_ctx *c, int W, int H);
 // addMap's pInv: voxel -> camera with the map offset folded in
 // (geometryfusion_mipmap_cpu.cpp:1771-1793)

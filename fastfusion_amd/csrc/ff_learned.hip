@@ -770,6 +770,7 @@ const float *learned_prepare(ff_ctx *c, int W, int H)
     if (!L) return nullptr;
     const size_t np = (size_t)W * H;
     if (L->confCap < np) {
+        frame_join(c);  // the frame in flight may still read the confidences
         hipFree(L->conf);
         L->conf = nullptr;
         L->confCap = 0;
@@ -937,6 +938,7 @@ int ff_set_learned_weighting(ff_ctx *c, const float *params, size_t n)
              "hipMemcpyAsync(params)"))
         return FF_E_DEVICE;
     c->learned->w.pack(c->stream);
+    frame_join(c);
     if (!lok(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize")) return FF_E_DEVICE;
     return FF_OK;
 }
@@ -950,6 +952,7 @@ int ff_copy_confidence(ff_ctx *c, float *out, size_t n)
     }
     const size_t np = (size_t)c->imgW * c->imgH;
     if (n < np) return FF_E_ARG;
+    frame_join(c);
     if (!lok(c, hipStreamSynchronize(c->stream), "hipStreamSynchronize")) return FF_E_DEVICE;
     if (!lok(c, hipMemcpy(out, c->lastConf, sizeof(float) * np, hipMemcpyDeviceToHost), "hipMemcpy")) return FF_E_DEVICE;
     return FF_OK;
@@ -1012,6 +1015,7 @@ int ff_trainer_collect(ff_trainer *tr, ff_ctx *map, const uint16_t *depth, int W
     uint16_t *d_depth = nullptr;
     TCHK(hipMalloc(&d_depth, sizeof(uint16_t) * (size_t)W * H));
     hipError_t e = hipMemcpyAsync(d_depth, depth, sizeof(uint16_t) * (size_t)W * H, hipMemcpyHostToDevice, tr->stream);
+    frame_join(map);
     if (e == hipSuccess) e = hipStreamSynchronize(map->stream);  // the map is complete
     uint32_t before = 0, after = 0;
     if (e == hipSuccess) e = hipMemcpyAsync(&before, tr->count, 4, hipMemcpyDeviceToHost, tr->stream);

@@ -407,6 +407,7 @@ static int loop_fill(ff_loop *lp, uint32_t *p, size_t n, uint32_t v)
 // cleared, as after allocateMemoryCPU
 static int loop_clear(ff_loop *lp)
 {
+    frame_join(lp->ctx);
     hipStream_t st = lp->ctx->stream;
     const size_t v = (size_t)lp->total * FF_BRICK;
     int rc;
@@ -441,6 +442,7 @@ static int frame_check(ff_loop *lp, const ff_loop_frame &f, int k)
 // synchronisation; inputs in device memory
 static int loop_submit(ff_loop *lp, const ff_loop_frame *fr, int nf, size_t *applied)
 {
+    frame_join(lp->ctx);  // the map the frames are subtracted from is complete
     ff_ctx *c = lp->ctx;
     for (int k = 0; k < nf; k++) {
         if (applied) applied[k] = 0;
@@ -606,6 +608,7 @@ ff_loop *ff_loop_create(ff_ctx *ctx, uint32_t max_leaves)
 void ff_loop_destroy(ff_loop *lp)
 {
     if (!lp) return;
+    frame_join(lp->ctx);
     hipStreamSynchronize(lp->ctx->stream);
     loop_free(lp);
     delete lp;

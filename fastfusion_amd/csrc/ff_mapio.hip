@@ -103,6 +103,7 @@ int ff_save_map(ff_ctx *c, const char *path)
 {
     if (!c || !path) return FF_E_ARG;
     hipSetDevice(c->device);
+    frame_join(c);
     if (!io_ok(c, hipStreamSynchronize(c->stream), "ff_save_map")) return FF_E_DEVICE;
     const uint32_t nB = std::min(c->nBranchesUsed, c->nBranchesTotal);
     const uint32_t nL = std::min(c->nLeavesUsed, c->nLeavesTotal);
@@ -181,6 +182,7 @@ int ff_load_map(ff_ctx *c, const char *path, const float offset[3], uint32_t n)
     if ((nB && fread(tree.data(), 4, nB, f) != nB) || (nL && fread(pos.data(), 2, pos.size(), f) != pos.size()) ||
         (nL && fread(sc.data(), 2, nL, f) != nL))
         return fail("truncated tree or bricks");
+    frame_join(c);
     if (!io_ok(c, hipStreamSynchronize(c->stream), "ff_load_map")) {
         fclose(f);
         return FF_E_DEVICE;

@@ -2257,14 +2257,7 @@ void mesh_launch_mc(ff_ctx *c)
         k_mc<<<2048, 256, 0, c->stream>>>(s->d_cells[s->nxt], s->d_geom, s->d_mc, s->d_sbuf, s->m, pools(s),
                                           s->cellCap);
     // the bricks are read: the next frame's SDF update may start
-    if (c->evMc) {
-        if (hipEventRecord(c->evMc, c->stream) == hipSuccess) {
-            c->mcPending = true;
-        } else {
-            (void)hipGetLastError();
-            hipStreamSynchronize(c->stream);
-        }
-    }
+    if (c->seqOk) c->mcPending = seq_signal(c, FF_SEQ_MC, c->stream);  // (else the stream has drained)
 }
 
 void mesh_launch_post(ff_ctx *c, bool timed)
@@ -2453,6 +2446,7 @@ int mesh_finish(ff_ctx *c, bool timed)
 // the host part of an enqueued update, once its device work completed
 int mesh_complete(ff_ctx *c, bool maintain)
 {
+    frame_join(c);  // every caller reads or changes the map next
     MeshState *s = c->mesh;
     if (!s) return FF_OK;
     if (!s->pending) return maintain ? mesh_maintain(c) : FF_OK;
@@ -2475,9 +2469,12 @@ int mesh_complete(ff_ctx *c, bool maintain)
     return maintain ? mesh_maintain(c) : FF_OK;
 }
 
-// enqueues the update; the previous one's host part runs first
+// enqueues the update; the previous one's host part runs first (a frame
+// whose SDF update may still run is joined on the device: the update's
+// kernels wait for it on the stream)
 static int update_meshes(ff_ctx *c)
 {
+    frame_join(c, true);
     const int rc = mesh_complete(c);
     if (rc != FF_OK) return rc;
     if (!mesh_launch_pre(c, true)) return FF_OK;
