@@ -1837,7 +1837,12 @@ void ff_destroy(ff_ctx *c)
 
 const char *ff_last_error(const ff_ctx *c) { return c ? c->err.c_str() : "null context"; }
 
-void *ff_stream(ff_ctx *c) { return c ? (void *)c->stream : nullptr; }
+void *ff_stream(ff_ctx *c)
+{
+    if (!c) return nullptr;
+    frame_join(c, true);  // work the caller enqueues on it sees the whole frame
+    return (void *)c->stream;
+}
 
 }  // extern "C"
 

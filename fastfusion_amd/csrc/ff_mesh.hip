@@ -40,8 +40,9 @@ struct CellRec {
     uint32_t ll[8];   // lastLeaf per side
     uint32_t v0, nv, f0, nf;
     uint32_t hslot;   // slot of the key in the lookup hash
-    uint32_t pad[3];
+    uint32_t pad[3];  // the update that last meshed it, chain depth, owner offset
 };
+static_assert(sizeof(CellRec) == 72, "a cell record is nine 8-byte words (k_cells stages it as such)");
 
 struct MeshCounters {
     uint32_t nTerm, nCells, nDirty, flags, nTiles, pad[3];
@@ -155,6 +156,13 @@ struct MeshState {
     uint32_t termCap = 0;
     CellRec *d_cells[2] = {nullptr, nullptr};
     uint32_t cellCap = 0;
+    // incremental partition (an update whose tree grew by a few bricks and
+    // branches since the current table was built): the terminal nodes the
+    // change touches, the cells they own (fresh), and counters: [0] touched
+    // nodes, [1] fresh cells
+    uint4 *d_termAff = nullptr;
+    CellRec *d_fresh = nullptr;
+    uint32_t *d_inc = nullptr;
     uint32_t nCells[2] = {0, 0};
     int cur = 0;
     struct CellGeom *d_geom = nullptr;  // per dirty cell
@@ -418,15 +426,89 @@ __device__ __forceinline__ uint32_t block_rank(bool want, uint32_t *total)
 }
 
 // ---------------------------------------------------------------------------
+// incremental partition
+// ---------------------------------------------------------------------------
+// A cell depends only on its owner node p (size s) and on the terminal
+// nodes and deepest bricks at p's option points, which lie in [p - 1, p + s]
+// on each axis.  Bricks and branches are only ever added (a grow rebuilds the
+// partition), so between two tables the tree changes by the bricks and
+// branch blocks made since the older one: a node whose option box meets none
+// of their boxes has the same cells in both tables, and a cell changes owner
+// only between nodes whose option boxes meet a change.  An update with few
+// changes keeps the cells of untouched owners and recomputes those of
+// touched terminal nodes (their option points read as the full partition
+// reads them).
+#define FF_INC_BOXES 1024u  // changed bricks + branch blocks an incremental update takes
+#define FF_INC_NODES 4096u  // touched terminal nodes an incremental update takes (27 cells each at most)
+
+struct IncView {
+    uint32_t l0, l1;  // bricks made since the table was built
+    uint32_t b0, b1;  // branch blocks made since
+};
+
+// the change boxes (x, y, z, side) into LDS (every thread calls it; one
+// barrier): bricks from their position and scale, branch blocks from the
+// node geometry the allocation records
+__device__ __forceinline__ uint32_t load_changes(const IncView &v, const ushort4 *__restrict__ info,
+                                                 const ushort4 *__restrict__ binfo, uint4 *box)
+{
+    const uint32_t nl = v.l1 - v.l0, n = min(nl + (v.b1 - v.b0), FF_INC_BOXES);
+    for (uint32_t k = threadIdx.x; k < n; k += blockDim.x) {
+        if (k < nl) {
+            const ushort4 li = info[v.l0 + k];
+            box[k] = make_uint4(li.x, li.y, li.z, (uint32_t)li.w * FF_BL);
+        } else {
+            const ushort4 bi = binfo[v.b0 + (k - nl)];
+            box[k] = make_uint4(bi.x, bi.y, bi.z, 2u * bi.w);
+        }
+    }
+    __syncthreads();
+    return n;
+}
+
+// whether node (p, s) has an option point in one of the n change boxes
+__device__ __forceinline__ bool node_touched(const uint4 *box, uint32_t n, int px, int py, int pz, int s)
+{
+    for (uint32_t k = 0; k < n; k++) {
+        const uint4 b = box[k];
+        if (px - 1 < (int)(b.x + b.w) && (int)b.x <= px + s && py - 1 < (int)(b.y + b.w) && (int)b.y <= py + s &&
+            pz - 1 < (int)(b.z + b.w) && (int)b.z <= pz + s)
+            return true;
+    }
+    return false;
+}
+
+// the owner node of a cell: the key's origin plus the recorded offsets
+// (CellRec::pad[2], bit k: + size on axis k)
+__device__ __forceinline__ bool cell_owner_touched(uint64_t key, uint32_t own, const uint4 *box, uint32_t n)
+{
+    const int s = 1 << ((key >> 3) & 31u);
+    const int ox = (int)(key >> 48) + ((own & 1u) ? s : 0), oy = (int)((key >> 32) & 0xffffu) + ((own & 2u) ? s : 0),
+              oz = (int)((key >> 16) & 0xffffu) + ((own & 4u) ? s : 0);
+    return node_touched(box, n, ox, oy, oz, s);
+}
+
+// whether an update with these counts runs incrementally (the same on
+// every kernel of the update)
+__device__ __forceinline__ bool inc_mode(bool inc, const uint32_t *incCnt)
+{
+    return inc && incCnt[0] <= FF_INC_NODES;
+}
+
+// ---------------------------------------------------------------------------
 // k_terminal: covered terminal nodes
 // ---------------------------------------------------------------------------
 #define FF_TERM_THREADS 256  // block size of k_terminal (every thread of a block appends together)
+// box, nBox: the change boxes of an incremental update (nullptr: a full
+// one), whose touched terminal nodes also go to termAff (count incCnt[0])
 __device__ __forceinline__ void terminal_body(const uint32_t *__restrict__ tree, const ushort4 *__restrict__ binfo,
                                               uint32_t nBlocks, uint32_t nLeaves, uint32_t nSlots, uint32_t n,
-                                              uint4 *__restrict__ term, uint32_t termCap, MeshCounters *mc)
+                                              uint4 *__restrict__ term, uint32_t termCap, MeshCounters *mc,
+                                              const uint4 *box = nullptr, uint32_t nBox = 0,
+                                              uint4 *termAff = nullptr, uint32_t *incCnt = nullptr)
 {
     const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    bool want = false;
+    bool want = false, touched = false;
     uint4 rec = make_uint4(0, 0, 0, 0);
     if (t < nBlocks * 8u) {
         const uint32_t b = t >> 3, co = t & 7u;
@@ -441,6 +523,7 @@ __device__ __forceinline__ void terminal_body(const uint32_t *__restrict__ tree,
             if (sz == half && lt != FF_BRANCHINIT) {
                 want = true;
                 rec = make_uint4((uint32_t)x | ((uint32_t)y << 16), (uint32_t)z, half, lt);
+                touched = box && node_touched(box, nBox, x, y, z, (int)half);
             }
         }
     }
@@ -449,13 +532,21 @@ __device__ __forceinline__ void terminal_body(const uint32_t *__restrict__ tree,
         if (i < termCap) term[i] = rec;
         else atomicOr(&mc->flags, FF_CELL_OVERFLOW);
     }
+    if (box) {
+        // (more than FF_INC_NODES: the update runs in full, inc_mode)
+        const uint32_t j = block_append<FF_TERM_THREADS / 64>(&incCnt[0], touched);
+        if (touched && j < FF_INC_NODES) termAff[j] = rec;
+    }
 }
 
 // (blocks past the terminal candidates fill the jump table, 256 cells each)
+// inc: an incremental update (the change boxes from iv; incCnt zero)
 __global__ __launch_bounds__(FF_TERM_THREADS) void k_terminal(const uint32_t *__restrict__ tree, const ushort4 *__restrict__ binfo, uint32_t nBlocks,
                            uint32_t nLeaves, uint32_t nSlots, uint32_t n, uint4 *__restrict__ term, uint32_t termCap,
-                           MeshCounters *mc, uint4 *__restrict__ jump, MeshCounters *clearNext)
+                           MeshCounters *mc, uint4 *__restrict__ jump, MeshCounters *clearNext, bool inc,
+                           IncView iv, const ushort4 *__restrict__ info, uint4 *termAff, uint32_t *incCnt)
 {
+    __shared__ uint4 box[FF_INC_BOXES];
     if (blockIdx.x == 0 && threadIdx.x == 0) *clearNext = MeshCounters{};  // the next update's counters
     const uint32_t tb = (nBlocks * 8u + blockDim.x - 1u) / blockDim.x;
     if (blockIdx.x >= tb) {
@@ -463,7 +554,12 @@ __global__ __launch_bounds__(FF_TERM_THREADS) void k_terminal(const uint32_t *__
         if (cell < FF_JUMP_CELLS) jump[cell] = jump_entry(tree, nLeaves, nSlots, n, cell);
         return;
     }
-    terminal_body(tree, binfo, nBlocks, nLeaves, nSlots, n, term, termCap, mc);
+    if (!inc) {
+        terminal_body(tree, binfo, nBlocks, nLeaves, nSlots, n, term, termCap, mc);
+        return;
+    }
+    const uint32_t nBox = load_changes(iv, info, binfo, box);
+    terminal_body(tree, binfo, nBlocks, nLeaves, nSlots, n, term, termCap, mc, box, nBox, termAff, incCnt);
 }
 
 // the batched forms of the per-stream mesh kernels: stream blockIdx.z of S
@@ -487,28 +583,29 @@ __device__ __forceinline__ int opt_code(int ox, int oy, int oz) { return ox + 4 
 
 #define FF_CELL_STAGE 256  // cells a k_cells block stages before appending (8 nodes emit at most 216)
 
+// the cells of the nt terminal nodes term[] appended to cells (count *count,
+// capacity cap); each records its owner's offset from its origin (pad[2])
 __device__ __forceinline__ void cells_body(const uint4 *__restrict__ jump, const uint32_t *__restrict__ tree,
                                            uint32_t nLeaves, uint32_t nSlots, uint32_t n,
-                                           const uint4 *__restrict__ term, uint32_t termCap, MeshCounters *mc,
-                                           CellRec *cells, uint32_t cap)
+                                           const uint4 *__restrict__ term, uint32_t nt, MeshCounters *mc,
+                                           CellRec *cells, uint32_t cap, uint32_t *count)
 {
     __shared__ uint2 nb[8][64];  // (size, lastLeaf) per node and option point
     // the block's cells are staged here and appended to the table together,
     // one atomic per FF_CELL_STAGE cells (appends of every block to one
     // counter serialise on its address)
-    __shared__ uint4 stage[FF_CELL_STAGE][4];
+    __shared__ uint2 stage[FF_CELL_STAGE][9];
     __shared__ uint32_t nstage, sbase;
-    const uint32_t nt = min(mc->nTerm, termCap);
     const int tid = threadIdx.x;
     if (tid == 0) nstage = 0u;
     // (the first iteration's barriers order this before any use)
     auto flush = [&]() {
-        if (tid == 0) sbase = nstage ? atomicAdd(&mc->nCells, nstage) : 0u;
+        if (tid == 0) sbase = nstage ? atomicAdd(count, nstage) : 0u;
         __syncthreads();
         const uint32_t ns = nstage, b = sbase;
-        for (uint32_t e = (uint32_t)tid; e < ns * 4u; e += blockDim.x) {
-            const uint32_t k = e >> 2;
-            if (b + k < cap) reinterpret_cast<uint4 *>(cells + b + k)[e & 3u] = stage[k][e & 3u];
+        for (uint32_t e = (uint32_t)tid; e < ns * 9u; e += blockDim.x) {
+            const uint32_t k = e / 9u;
+            if (b + k < cap) reinterpret_cast<uint2 *>(cells + b + k)[e % 9u] = stage[k][e % 9u];
         }
         if (tid == 0 && ns && b + ns > cap) atomicOr(&mc->flags, FF_CELL_OVERFLOW);
         __syncthreads();  // stage is read before it is refilled
@@ -548,7 +645,7 @@ __device__ __forceinline__ void cells_body(const uint4 *__restrict__ jump, const
             const int cand = tid % 27;
             bool emit = false;
             int type = 0, o[3] = {0, 0, 0}, s = 1, nll = 0;
-            uint32_t ll[8];
+            uint32_t ll[8], own = 0;  // own: bit k, the owner lies at o + s on axis k
             if (tid < 8 * 27 && node < nt) {
                 const uint2 *N = nb[tid / 27];
                 const uint4 tn = term[node];
@@ -568,6 +665,7 @@ __device__ __forceinline__ void cells_body(const uint4 *__restrict__ jump, const
                     if (ns && (side ? (ns >= (uint32_t)s) : (ns > (uint32_t)s))) {
                         o[0] = p[0]; o[1] = p[1]; o[2] = p[2];
                         if (!side) o[a] = p[a] - s;
+                        own = side ? 0u : 1u << a;
                         ll[side ? 0 : 1] = lt;
                         ll[side ? 1 : 0] = r.y;
                         type = 1 + a; nll = 2;
@@ -590,6 +688,7 @@ __device__ __forceinline__ void cells_body(const uint4 *__restrict__ jump, const
                     }
                     if (ok) {
                         o[e] = p[e]; o[u] = p[u] + cu * s - s; o[v] = p[v] + cv * s - s;
+                        own = (cu ? 0u : 1u << u) | (cv ? 0u : 1u << v);
                         type = 4 + e; nll = 4;
                         emit = true;
                     }
@@ -604,6 +703,7 @@ __device__ __forceinline__ void cells_body(const uint4 *__restrict__ jump, const
                     }
                     if (ok) {
                         for (int k = 0; k < 3; k++) o[k] = p[k] + c3[k] * s - s;
+                        own = (c3[0] ? 0u : 1u) | (c3[1] ? 0u : 2u) | (c3[2] ? 0u : 4u);
                         type = 7; nll = 8;
                         emit = true;
                     }
@@ -623,9 +723,10 @@ __device__ __forceinline__ void cells_body(const uint4 *__restrict__ jump, const
                 for (int k = 0; k < 8; k++) r.ll[k] = k < nll ? ll[k] : FF_BRANCHINIT;
                 r.v0 = r.nv = r.f0 = r.nf = 0;
                 r.hslot = 0xffffffffu;
-                r.pad[0] = r.pad[1] = r.pad[2] = 0;
-                const uint4 *q = reinterpret_cast<const uint4 *>(&r);
-                for (int k = 0; k < 4; k++) stage[nstage + rank][k] = q[k];
+                r.pad[0] = r.pad[1] = 0;
+                r.pad[2] = own;
+                const uint2 *q = reinterpret_cast<const uint2 *>(&r);
+                for (int k = 0; k < 9; k++) stage[nstage + rank][k] = q[k];
             }
             __syncthreads();  // every stage write and nstage read are done
             if (tid == 0) nstage += total;
@@ -635,18 +736,25 @@ __device__ __forceinline__ void cells_body(const uint4 *__restrict__ jump, const
     flush();
 }
 
+// inc: an incremental update, which (inc_mode) takes the touched nodes
+// termAff and appends their cells to fresh (count incCnt[1])
 __global__ __launch_bounds__(512) void k_cells(const uint32_t *__restrict__ tree, uint32_t nLeaves, uint32_t nSlots,
                                                uint32_t n,
                                                const uint4 *__restrict__ term, uint32_t termCap, MeshCounters *mc,
-                                               CellRec *cells, uint32_t cap, const uint4 *__restrict__ jump)
+                                               CellRec *cells, uint32_t cap, const uint4 *__restrict__ jump, bool inc,
+                                               const uint4 *__restrict__ termAff, CellRec *fresh, uint32_t *incCnt)
 {
-    cells_body(jump, tree, nLeaves, nSlots, n, term, termCap, mc, cells, cap);
+    if (inc_mode(inc, incCnt))
+        cells_body(jump, tree, nLeaves, nSlots, n, termAff, incCnt[0], mc, fresh, FF_INC_NODES * 27u, &incCnt[1]);
+    else
+        cells_body(jump, tree, nLeaves, nSlots, n, term, min(mc->nTerm, termCap), mc, cells, cap, &mc->nCells);
 }
 
 __global__ __launch_bounds__(512) void k_cells_b(const MeshBatchStream *__restrict__ S)
 {
     const MeshBatchStream &a = S[blockIdx.z];
-    cells_body(nullptr, a.tree, a.nLeaves, a.nSlots, a.n, a.term, a.termCap, a.mc, a.cells, a.cellCap);
+    cells_body(nullptr, a.tree, a.nLeaves, a.nSlots, a.n, a.term, min(a.mc->nTerm, a.termCap), a.mc, a.cells,
+               a.cellCap, &a.mc->nCells);
 }
 
 // ---------------------------------------------------------------------------
@@ -1169,6 +1277,91 @@ __global__ __launch_bounds__(FF_MARK_THREADS) void k_mark_b(const MeshBatchStrea
               a.cellCap, a.m, a.tiles, a.tileCap, false, a.updateId);
 }
 
+// An incremental update's table and marks: the previous table's cells whose
+// owners no change touches, kept with their meshes and re-marked as
+// k_mark_same re-marks them, then the fresh cells of the touched nodes,
+// matched with the previous table by key as k_mark matches them; appended
+// to the new table in that order (one atomic per block).  In full mode
+// (inc_mode false: too many touched nodes) the table k_cells built, marked
+// by k_mark's pass.
+__global__ __launch_bounds__(FF_MARK_THREADS) void k_mark_inc(CellRec *cells, const CellRec *__restrict__ prev, uint32_t nPrev,
+                           const uint64_t *__restrict__ hkeys, const uint32_t *__restrict__ hval, uint32_t hmask,
+                           const uint32_t *__restrict__ stamp, const uint32_t *__restrict__ parent, uint32_t since,
+                           MeshCounters *mc, CellGeom *geom, uint32_t cap, MapRO m, TileRec *tiles, uint32_t tileCap,
+                           uint32_t updateId, IncView iv, const ushort4 *__restrict__ binfo,
+                           const CellRec *__restrict__ fresh, const uint32_t *__restrict__ incCnt)
+{
+    if (!inc_mode(true, incCnt)) {
+        mark_body(cells, prev, hkeys, hval, hmask, true, stamp, parent, since, mc, geom, cap, m, tiles, tileCap, true,
+                  updateId);
+        return;
+    }
+    __shared__ uint4 box[FF_INC_BOXES];
+    const uint32_t nBox = load_changes(iv, m.info, binfo, box);
+    const uint32_t nItems = nPrev + min(incCnt[1], FF_INC_NODES * 27u);
+    const uint32_t stride = gridDim.x * blockDim.x;
+    // block-uniform iterations (the table and marking slots are per block)
+    for (uint32_t ib = blockIdx.x * blockDim.x; ib < nItems; ib += stride) {
+        const uint32_t i = ib + threadIdx.x;
+        bool keep = false, isDirty = false;
+        CellRec c;
+        if (i < nPrev) {
+            c = prev[i];
+            keep = !cell_owner_touched(c.key, c.pad[2], box, nBox);
+            if (keep) isDirty = chains_dirty(stamp, parent, c.ll, since, c.pad[1]);
+        } else if (i < nItems) {
+            c = fresh[i - nPrev];
+            keep = true;
+            int found = -1;
+            uint32_t h = hkey_hash(c.key, hmask);
+            for (uint32_t pr = 0; pr <= hmask; pr++) {
+                const uint64_t k = hkeys[h];
+                if (k == c.key) { found = (int)hval[h]; break; }
+                if (k == FF_HEMPTY) break;
+                h = (h + 1) & hmask;
+            }
+            bool clean = found >= 0;
+            if (clean) {
+                const CellRec &p = prev[found];
+                for (int k = 0; k < 8; k++) clean = clean && (p.ll[k] == c.ll[k]);
+                if (clean && chains_dirty(stamp, parent, c.ll, since, p.pad[1])) clean = false;
+                if (clean) {
+                    c.v0 = p.v0; c.nv = p.nv; c.f0 = p.f0; c.nf = p.nf;
+                    c.pad[0] = p.pad[0];
+                    c.pad[1] = p.pad[1];
+                }
+            }
+            isDirty = !clean;
+        }
+        const uint32_t j = block_append<FF_MARK_THREADS / 64>(&mc->nCells, keep);
+        CellGeom g;
+        uint32_t nt = 0;
+        if (isDirty) {
+            // a fresh cell's record has an empty mesh range; a kept one's is
+            // cleared, like k_mark_same's (k_mc fills it unless the mesh is empty)
+            c.v0 = c.nv = c.f0 = c.nf = 0u;
+            c.pad[0] = updateId;  // the update that meshed it (CellUpdate)
+            cell_geom(m, c.key, c.ll, g);
+            g.cell = j;
+            g.pad = 0;
+            nt = ((uint32_t)geom_total(g) + 63u) >> 6;
+            c.pad[1] = nt ? 1u : 0u;  // chain depth: raised by the sampling
+        }
+        if (keep) {
+            if (j < cap) cells[j] = c;
+            else atomicOr(&mc->flags, FF_CELL_OVERFLOW);
+        }
+        const MarkSlot sl = block_mark_slots<FF_MARK_THREADS / 64>(mc, isDirty && j < cap, nt, cap, true);
+        if (isDirty && j < cap) {
+            const uint32_t t0 = sl.t0;
+            const bool fits = t0 + nt <= tileCap;
+            g.tile0 = nt == 0 ? FF_EMPTY_CELL : (fits ? t0 : FF_NO_TILE);
+            put_tiles(tiles, tileCap, t0, nt, g, c.ll);
+            geom[sl.d] = g;
+        }
+    }
+}
+
 // The table of the previous update re-marked in place, when the tree has not
 // changed since it was built: every cell then exists with the same bricks,
 // so it is clean exactly when none of its bricks' parent chains was updated
@@ -1274,10 +1467,12 @@ __device__ __forceinline__ void hash_clear_cells_body(const CellRec *__restrict_
 // the turnover of an update that built a new table: the previous table's
 // slots emptied in its hash, the new table's keys into the other (empty)
 // hash; the two parts touch different memory, so one pass does both
+// (incCnt: the incremental counters, zero for the next update)
 __global__ void k_hash_turnover(const CellRec *__restrict__ prev, uint32_t nPrev, uint64_t *prevKeys, CellRec *cells,
                                 MeshCounters *mc, uint32_t cap, uint64_t *hkeys, uint32_t *hval, uint32_t hmask,
-                                const unsigned long long *arena)
+                                const unsigned long long *arena, uint32_t *incCnt)
 {
+    if (blockIdx.x == 0 && threadIdx.x < 2) incCnt[threadIdx.x] = 0u;
     hash_clear_cells_body(prev, nPrev, prevKeys);
     hash_build_body(cells, mc, cap, hkeys, hval, hmask, arena);
 }
@@ -1910,6 +2105,10 @@ int mesh_init(ff_ctx *c)
     MCHK(hipMalloc(&s->d_jump, sizeof(uint4) * FF_JUMP_CELLS));
     s->cellCap = std::max<uint32_t>(1u << 18, std::min<uint32_t>(c->nLeavesTotal * 16u, 1u << 25));
     for (int k = 0; k < 2; k++) MCHK(hipMalloc(&s->d_cells[k], sizeof(CellRec) * s->cellCap));
+    MCHK(hipMalloc(&s->d_termAff, sizeof(uint4) * FF_INC_NODES));
+    MCHK(hipMalloc(&s->d_fresh, sizeof(CellRec) * FF_INC_NODES * 27u));
+    MCHK(hipMalloc(&s->d_inc, sizeof(uint32_t) * 4));
+    MCHK(hipMemset(s->d_inc, 0, sizeof(uint32_t) * 4));
     MCHK(hipMalloc(&s->d_geom, sizeof(CellGeom) * s->cellCap));
     s->tileCap = std::max<uint32_t>(1u << 16, std::min<uint32_t>(c->nLeavesTotal * 8u, 1u << 19));
     MCHK(hipMalloc(&s->d_tiles, sizeof(TileRec) * s->tileCap));
@@ -1949,6 +2148,7 @@ void mesh_free(ff_ctx *c)
     MeshState *s = c->mesh;
     if (!s) return;
     hipFree(s->d_binfo); hipFree(s->d_term); hipFree(s->d_jump);
+    hipFree(s->d_termAff); hipFree(s->d_fresh); hipFree(s->d_inc);
     hipFree(s->d_cells[0]); hipFree(s->d_cells[1]);
     hipFree(s->d_geom); hipFree(s->d_tiles); hipFree(s->d_sbuf);
     for (int k = 0; k < 2; k++) { hipFree(s->d_hkeysBuf[k]); hipFree(s->d_hvalBuf[k]); }
@@ -2171,18 +2371,38 @@ bool mesh_launch_pre(ff_ctx *c, bool timed)
     // the tree as of this update (the next frame's allocation may link new
     // nodes while it runs)
     const uint32_t nLeaves = std::min(c->nLeavesUsed, c->nLeavesTotal), nSlots = c->nBranchesUsed;
+    // incremental when the current table was built for this tree less the
+    // bricks and branch blocks made since, and those are few
+    const bool havePrev = s->nCells[s->cur] > 0 && !s->fullNext;
+    IncView iv;
+    iv.l0 = std::min(s->builtNL, nLeaves);
+    iv.l1 = nLeaves;
+    iv.b0 = std::min(s->builtNB, nSlots) / 16u;
+    iv.b1 = nBlocks;
+    const bool inc = havePrev && s->builtGrows == c->grows && s->builtNL <= c->nLeavesUsed &&
+                     s->builtNB <= nSlots && (iv.l1 - iv.l0) + (iv.b1 - iv.b0) <= FF_INC_BOXES;
     // (plus FF_JUMP_CELLS / 256 blocks that fill the jump table)
     k_terminal<<<(nBlocks * 8 + FF_TERM_THREADS - 1) / FF_TERM_THREADS + FF_JUMP_CELLS / FF_TERM_THREADS,
                  FF_TERM_THREADS, 0, c->stream>>>(
         c->d_tree, s->d_binfo, nBlocks, nLeaves, nSlots, c->n, s->d_term, s->termCap, s->d_mc, s->d_jump,
-        mc_other(s));
+        mc_other(s), inc, iv, c->d_leafinfo, s->d_termAff, s->d_inc);
     k_cells<<<1024, 512, 0, c->stream>>>(c->d_tree, nLeaves, nSlots, c->n, s->d_term, s->termCap, s->d_mc,
-                                         s->d_cells[nxt], s->cellCap, s->d_jump);
+                                         s->d_cells[nxt], s->cellCap, s->d_jump, inc, s->d_termAff, s->d_fresh,
+                                         s->d_inc);
     const MapRO m = s->m;
-    k_mark<<<1024, FF_MARK_THREADS, 0, c->stream>>>(s->d_cells[nxt], s->d_cells[s->cur], s->d_hkeys, s->d_hval, s->hmask,
-                                        s->nCells[s->cur] > 0 && !s->fullNext, c->d_stamp, c->d_parent,
-                                        s->lastMeshStamp, s->d_mc,
-                                        s->d_geom, s->cellCap, m, s->d_tiles, s->tileCap, s->launchId);
+    if (inc) {
+        const uint32_t nPrev = std::min(s->nCells[s->cur], s->cellCap);
+        k_mark_inc<<<std::max(1024u, std::min(4096u, (nPrev + FF_MARK_THREADS - 1) / FF_MARK_THREADS + 16u)),
+                     FF_MARK_THREADS, 0, c->stream>>>(
+            s->d_cells[nxt], s->d_cells[s->cur], nPrev, s->d_hkeys, s->d_hval, s->hmask, c->d_stamp, c->d_parent,
+            s->lastMeshStamp, s->d_mc, s->d_geom, s->cellCap, m, s->d_tiles, s->tileCap, s->launchId, iv,
+            s->d_binfo, s->d_fresh, s->d_inc);
+    } else {
+        k_mark<<<1024, FF_MARK_THREADS, 0, c->stream>>>(s->d_cells[nxt], s->d_cells[s->cur], s->d_hkeys, s->d_hval,
+                                                        s->hmask, havePrev, c->d_stamp, c->d_parent,
+                                                        s->lastMeshStamp, s->d_mc, s->d_geom, s->cellCap, m,
+                                                        s->d_tiles, s->tileCap, s->launchId);
+    }
     k_sample<<<4096, 256, 0, c->stream>>>(s->d_geom, s->d_cells[nxt], s->d_tiles, s->d_mc, s->tileCap, m, s->d_sbuf);
     mc_flip(s);
         return true;
@@ -2272,7 +2492,8 @@ void mesh_launch_post(ff_ctx *c, bool timed)
         const int b = 1 - s->hIdx;
         k_hash_turnover<<<1024, 256, 0, c->stream>>>(s->d_cells[s->cur], s->nCells[s->cur], s->d_hkeys,
                                                      s->d_cells[nxt], s->d_mc, s->cellCap, s->d_hkeysBuf[b],
-                                                     s->d_hvalBuf[b], s->hmask, c->timingAll ? s->d_arena : nullptr);
+                                                     s->d_hvalBuf[b], s->hmask, c->timingAll ? s->d_arena : nullptr,
+                                                     s->d_inc);
         s->hIdx = b;  // the next update looks the new table up there
         s->d_hkeys = s->d_hkeysBuf[b];
         s->d_hval = s->d_hvalBuf[b];

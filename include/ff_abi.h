@@ -69,7 +69,13 @@ int ff_add_map(ff_ctx *ctx, const uint16_t *depth, size_t depth_stride, const ui
                size_t bgr_stride, int width, int height, const double R[9], const double t[3],
                const double K[4], float scaling, float maxcam);
 
-/* Same, with depth and bgr already resident in device memory (dense rows). */
+/* Same, with depth and bgr already resident in device memory (dense rows).
+ * The call returns once the frame's counts are known, possibly while its
+ * voxel update still reads the images: they must stay unchanged until the
+ * next ff_add_map* call on the context returns (or any call that reads the
+ * map, which waits for the frame).  Every later call sees the whole frame,
+ * as after ff_add_map, which keeps no reference to the caller's host images
+ * once it returns. */
 int ff_add_map_device(ff_ctx *ctx, const uint16_t *depth_dev, const uint8_t *bgr_dev, int width,
                       int height, const double R[9], const double t[3], const double K[4],
                       float scaling, float maxcam);
