@@ -507,12 +507,12 @@ def run_walk(args):
     # frame in 8 (each event costs launch time)
     fz = make()
     fuse(fz, untimed, "untimed")
-    hiprt.synchronize()
+    fz.sync()  # (the last requested mesh update enqueued and done)
     fz.reset_timing()
     fz.set_timing(True, sdf_only=True, every=8)
     t0 = time.perf_counter()
     nq_sum = fuse(fz, timed, "timed")
-    hiprt.synchronize()
+    fz.sync()  # the last frame's mesh update included
     dt = time.perf_counter() - t0
     tm = fz.timing()
     mc = fz.mc_timing()
@@ -542,11 +542,11 @@ def run_walk(args):
     # the replay: every phase evented over the same timed frames
     fr = make()
     fuse(fr, untimed, "replay untimed")
-    hiprt.synchronize()
+    fr.sync()
     fr.reset_timing()  # the counts cover the timed frames only
     fr.set_timing(True)
     nq_replay = fuse(fr, timed, "replay")
-    hiprt.synchronize()
+    fr.sync()
     rp = fr.timing()
     rmc = fr.mc_timing()
     rc = fr.counts()

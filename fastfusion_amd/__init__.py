@@ -62,6 +62,7 @@ ABI = {
     "ff_add_map_device_noalloc": (ctypes.c_int, [_P, _P, _P, ctypes.c_int, ctypes.c_int, _DP, _DP, _DP,
                                                  ctypes.c_float, ctypes.c_float]),
     "ff_update_meshes": (ctypes.c_int, [_P]),
+    "ff_sync": (ctypes.c_int, [_P]),
     "ff_mesh_full": (ctypes.c_int, [_P]),
     "ff_mesh_sizes": (ctypes.c_int, [_P, ctypes.POINTER(_SZ), ctypes.POINTER(_SZ)]),
     "ff_copy_mesh": (ctypes.c_int, [_P, _P, _P, _P]),
@@ -312,7 +313,15 @@ class FusionMipMapGPU:
 
     # -- updateMeshes, :2884-2954
     def updateMeshes(self):
+        """Requests the update of the map as it stands; its device work is
+        enqueued by the next call on the context (the next frame's, right
+        after that frame's first kernels)."""
         return self._check(self._lib.ff_update_meshes(self._ctx))
+
+    def sync(self):
+        """Enqueues a requested update and waits for all of the context's
+        device work (no counterpart in the reference)."""
+        return self._check(self._lib.ff_sync(self._ctx))
 
     def meshFull(self):
         """Every cell meshed from scratch now (getMeshRecursive's counterpart,

@@ -2047,6 +2047,10 @@ static int run_frame(ff_ctx *c, int W, int H, bool overlap)
             confAside = seq_signal(c, FF_SEQ_CONF, c->aux);  // (else aux has drained)
         }
         launch_front(c, W, H, c->side, true);  // carries the frame's parameters up
+        // a requested mesh update (of the map before this frame) goes on the
+        // main stream now: the front reads only the new frame
+        const int rf = mesh_flush(c);
+        if (rf != FF_OK) return rf;
         launch_back(c, W, H, c->side);
         if (c->learned) {
             if (confAside) seq_wait(c, FF_SEQ_CONF, c->side);
@@ -2358,7 +2362,13 @@ static int add_map_impl(ff_ctx *c, const uint16_t *depth, const uint8_t *bgr, in
     // a frame on the main stream waits there for the previous frame's SDF
     // update; an overlapped one follows it on the side stream, and the
     // previous update's timing events are read before they are recorded again
-    if (!overlap) frame_join(c, true);
+    if (!overlap) {
+        // a frame on the main stream: a requested mesh update first, then a
+        // wait there for the previous frame's SDF update
+        const int rf = mesh_flush(c);
+        if (rf != FF_OK) return rf;
+        frame_join(c, true);
+    }
     sdf_timing_collect(c);
     seq_rebase(c);
     int rc = frame_prepare(c, depth, bgr, W, H, R, t, K, scaling, maxcam, !overlap);

@@ -299,6 +299,9 @@ int mesh_init(ff_ctx *c);
 // maintain = false (inside a frame) leaves its allocating part (buffer
 // growth, pool compaction) to the next call with maintain = true
 int mesh_complete(ff_ctx *c, bool maintain = true);
+// enqueues a requested mesh update (updateMeshes records the request; the
+// next frame enqueues it after its first kernels, any other call first)
+int mesh_flush(ff_ctx *c);
 void mesh_free(ff_ctx *c);
 void mesh_on_initial_volume(ff_ctx *c);
 int mesh_on_load(ff_ctx *c, const std::vector<ushort4> &binfo);

@@ -220,6 +220,12 @@ struct MeshState {
     // an update whose device work is enqueued and whose host part
     // (mesh_finish) runs at the next mesh call, query or state change
     bool pending = false, pendingTimed = false;
+    // an update requested (updateMeshes) but not yet enqueued: the next
+    // frame enqueues it right after its first kernels, every other call on
+    // the context first; reqStamp is the frame stamp it meshes
+    bool requested = false;
+    uint32_t reqStamp = 0;
+    uint32_t stampFor = 0;  // the stamp of the update being enqueued
     bool carryRequested = false;  // the update's counters left for the next SDF update to take up
     // the map the update in flight meshes: the frame stamp and the tree
     // (frames fused before its host part runs must not be attributed to it)
@@ -2332,7 +2338,7 @@ static bool mesh_pre_host(ff_ctx *c, bool allowReuse)
     s->d_mc = s->d_mcBuf[s->mcIdx];
     s->mcZeroNow = s->mcNextZero;
     s->mcNextZero = false;
-    s->launchStamp = c->stampCounter;
+    s->launchStamp = s->stampFor;
     s->launchId = ++s->updateSeq;
     s->launchNB = c->nBranchesUsed;
     s->launchNL = c->nLeavesUsed;
@@ -2386,7 +2392,10 @@ bool mesh_launch_pre(ff_ctx *c, bool timed)
                  FF_TERM_THREADS, 0, c->stream>>>(
         c->d_tree, s->d_binfo, nBlocks, nLeaves, nSlots, c->n, s->d_term, s->termCap, s->d_mc, s->d_jump,
         mc_other(s), inc, iv, c->d_leafinfo, s->d_termAff, s->d_inc);
-    k_cells<<<1024, 512, 0, c->stream>>>(c->d_tree, nLeaves, nSlots, c->n, s->d_term, s->termCap, s->d_mc,
+    // (an incremental update meshes a few touched nodes: a small grid, whose
+    // blocks retire at once, lets the marking start sooner; should the
+    // touched nodes exceed FF_INC_NODES, the same grid runs the full pass)
+    k_cells<<<inc ? 64 : 1024, 512, 0, c->stream>>>(c->d_tree, nLeaves, nSlots, c->n, s->d_term, s->termCap, s->d_mc,
                                          s->d_cells[nxt], s->cellCap, s->d_jump, inc, s->d_termAff, s->d_fresh,
                                          s->d_inc);
     const MapRO m = s->m;
@@ -2664,12 +2673,24 @@ int mesh_finish(ff_ctx *c, bool timed)
     return (int)nDirty;
 }
 
+static int update_meshes(ff_ctx *c, uint32_t stamp);
+
+int mesh_flush(ff_ctx *c)
+{
+    MeshState *s = c->mesh;
+    if (!s || !s->requested) return FF_OK;
+    s->requested = false;
+    return update_meshes(c, s->reqStamp);
+}
+
 // the host part of an enqueued update, once its device work completed
 int mesh_complete(ff_ctx *c, bool maintain)
 {
     frame_join(c);  // every caller reads or changes the map next
     MeshState *s = c->mesh;
     if (!s) return FF_OK;
+    const int rf = mesh_flush(c);
+    if (rf != FF_OK) return rf;
     if (!s->pending) return maintain ? mesh_maintain(c) : FF_OK;
     s->pending = false;
     // counters taken up by a frame's SDF update, which follows the update's
@@ -2690,14 +2711,15 @@ int mesh_complete(ff_ctx *c, bool maintain)
     return maintain ? mesh_maintain(c) : FF_OK;
 }
 
-// enqueues the update; the previous one's host part runs first (a frame
-// whose SDF update may still run is joined on the device: the update's
-// kernels wait for it on the stream)
-static int update_meshes(ff_ctx *c)
+// enqueues the update of the map as of frame stamp `stamp`; the previous
+// one's host part runs first (a frame whose SDF update may still run is
+// joined on the device: the update's kernels wait for it on the stream)
+static int update_meshes(ff_ctx *c, uint32_t stamp)
 {
     frame_join(c, true);
     const int rc = mesh_complete(c);
     if (rc != FF_OK) return rc;
+    c->mesh->stampFor = stamp;
     if (!mesh_launch_pre(c, true)) return FF_OK;
     mesh_launch_mc(c);
     mesh_launch_post(c, true);
@@ -2816,7 +2838,25 @@ int ff_update_meshes(ff_ctx *c)
     if (!c) return FF_E_ARG;
     hipSetDevice(c->device);
     if (!c->incremental) return 0;
-    return update_meshes(c);
+    MeshState *s = c->mesh;
+    // (a request not yet enqueued goes first)
+    const int rc = mesh_flush(c);
+    if (rc != FF_OK) return rc;
+    s->requested = true;
+    s->reqStamp = c->stampCounter;
+    return FF_OK;
+}
+
+int ff_sync(ff_ctx *c)
+{
+    if (!c) return FF_E_ARG;
+    hipSetDevice(c->device);
+    const int rc = mesh_flush(c);
+    if (rc != FF_OK) return rc;
+    frame_join(c);
+    if (c->aux) MCHK(hipStreamSynchronize(c->aux));
+    MCHK(hipStreamSynchronize(c->stream));
+    return FF_OK;
 }
 
 int ff_mesh_full(ff_ctx *c)
@@ -2827,7 +2867,7 @@ int ff_mesh_full(ff_ctx *c)
     if (rc != FF_OK) return rc;
     MeshState *s = c->mesh;
     s->fullNext = true;
-    rc = update_meshes(c);
+    rc = update_meshes(c, c->stampCounter);
     s->fullNext = false;
     if (rc != FF_OK) return rc;
     return mesh_complete(c);

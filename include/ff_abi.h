@@ -98,13 +98,22 @@ int ff_add_map_device_noalloc(ff_ctx *ctx, const uint16_t *depth_dev, const uint
                               float scaling, float maxcam);
 
 /* FusionMipMapCPU::updateMeshes (geometryfusion_mipmap_cpu.cpp:2884-2954):
- * re-meshes the cells touched by the frames since the last call.  The work
- * is enqueued on the context's stream and the call returns (the reference's
- * updateMeshes returns true; its meshing thread also runs behind the
- * caller); every mesh query below waits for it and sees its result.  Its
- * host bookkeeping runs at the next mesh call or query, which also returns
- * its errors (FF_E_POOL_FULL: cell table overflow).  Returns FF_OK or FF_E*. */
+ * re-meshes the cells touched by the frames since the last call, as the map
+ * stands at the call.  The call records the request and returns (the
+ * reference's updateMeshes returns true; its meshing thread also runs behind
+ * the caller): the device work is enqueued by the next call on the context
+ * -- the next ff_add_map*, right after that frame's first kernels (which
+ * read only the new images), or any other call first -- so that it never
+ * sits between a frame and the next frame's start.  Every mesh query below
+ * sees its result.  Its host bookkeeping runs at the next mesh call or
+ * query, which also returns its errors (FF_E_POOL_FULL: cell table
+ * overflow).  Returns FF_OK or FF_E*. */
 int ff_update_meshes(ff_ctx *ctx);
+
+/* No counterpart in the reference: enqueues a requested mesh update and
+ * waits until all of the context's device work has completed (after it the
+ * device images of the last frame are no longer read). */
+int ff_sync(ff_ctx *ctx);
 
 /* The whole map meshed from scratch, the counterpart of the reference's
  * non-incremental getMeshRecursive (geometryfusion_mipmap_cpu.cpp:3658-3674):
