@@ -1035,10 +1035,30 @@ static inline uint32_t queue_grid(uint32_t nLeavesUsed)
 // ---------------------------------------------------------------------------
 #define FF_TAIL_SPANS 4
 
+// The frame's counters, once its allocation is done, to mapped host memory
+// (one thread): the first 48 bytes, then, after a system-scope fence, the
+// frame's stamp in pad[0], so that a host that sees the stamp sees them.
+// The host then returns from the frame: the SDF update's gate is evaluated
+// on the host from the same counters.
+__device__ __forceinline__ void frame_report(const FrameCounters *fc, FrameCounters *hostFc, uint32_t stamp)
+{
+    const uint4 *s = reinterpret_cast<const uint4 *>(fc);
+    uint4 *d = reinterpret_cast<uint4 *>(hostFc);
+    const uint4 a = s[0], b = s[1], c = s[2];
+    d[0] = a;
+    d[1] = b;
+    d[2] = c;
+    __threadfence_system();
+    *reinterpret_cast<volatile uint32_t *>(&hostFc->pad[0]) = stamp;
+}
+
+// hostFc: where the frame's counters are reported (nullptr: not here -- a
+// separate k_queue follows, and k_frame_report after it)
 __global__ __launch_bounds__(FF_ORDER_THREADS) void k_alloc_tail(HashView hv, MapView m, FrameCounters *fc,
                                                              const uint32_t *__restrict__ pl0,
                                                              const FrameDev *__restrict__ fd, MapCounters *mapc,
-                                                             ushort4 *binfo, uint32_t binfoBlocks, int queue, int H)
+                                                             ushort4 *binfo, uint32_t binfoBlocks, int queue, int H,
+                                                             FrameCounters *hostFc)
 {
     __shared__ uint32_t big;
     __shared__ uint64_t key[FF_ORDER_MAX];  // the ordering's sort keys, then the queue's band sort
@@ -1047,7 +1067,11 @@ __global__ __launch_bounds__(FF_ORDER_THREADS) void k_alloc_tail(HashView hv, Ma
     if (threadIdx.x == 0)
         big = __hip_atomic_load(&fc->flags, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & FF_FLAG_ORDER_BIG;
     __syncthreads();
-    if (big) return;  // the host orders the frame
+    if (big) {
+        // the host orders the frame (it sees the flag in the report)
+        if (threadIdx.x == 0 && hostFc) frame_report(fc, hostFc, fd->stamp);
+        return;
+    }
     link_body(hv, m, fc, pl0, fd, fc, true);
     __syncthreads();  // every lookup of the node table is done before its slots are reset
     alloc_end_body(hv, fc, mapc, binfo, binfoBlocks, fd, true);
@@ -1056,6 +1080,14 @@ __global__ __launch_bounds__(FF_ORDER_THREADS) void k_alloc_tail(HashView hv, Ma
     queue_body(m.stamp, mapc, m.nLeavesTotal, fd, m.queue, fc, true);
     __syncthreads();  // the queue and its count are written
     queue_band_sort(m.leafinfo, fd, H, m.queue, fc, key);
+    __syncthreads();
+    if (threadIdx.x == 0 && hostFc) frame_report(fc, hostFc, fd->stamp);
+}
+
+// the report after a separate k_queue
+__global__ void k_frame_report(const FrameCounters *fc, FrameCounters *hostFc, const FrameDev *__restrict__ fd)
+{
+    if (threadIdx.x == 0) frame_report(fc, hostFc, fd->stamp);
 }
 
 // ---------------------------------------------------------------------------
@@ -1259,10 +1291,12 @@ __device__ __forceinline__ bool sdf_gate_open(const GateView &v, uint32_t n, con
     return g.force || !(held | (out & grows));
 }
 
+// carrySrc: the previous mesh update's counters, to carryDst (mapped host
+// memory) and then this frame's stamp to *carrySeq
 __global__ __launch_bounds__(512) void k_sdf(const uint32_t *__restrict__ queue, MapView m,
                                              const FrameDev *__restrict__ fd, int W, int H, FrameCounters *fc,
                                              const MapCounters *__restrict__ mapc, SdfGate gate,
-                                             FrameCounters *hostFc, const uint4 *carrySrc, uint4 *carryDst)
+                                             const uint4 *carrySrc, uint4 *carryDst, uint32_t *carrySeq)
 {
     rz_mode_on();  // first: every float operation below rounds toward zero
     // the counters, the rcp table's entries and the first queue slot are
@@ -1273,22 +1307,15 @@ __global__ __launch_bounds__(512) void k_sdf(const uint32_t *__restrict__ queue,
     for (int k = 0; k < 2048 / 512; k++) tv[k] = c_rcp_table[threadIdx.x + 512 * k];
     // blocks loop over the queued bricks; the count comes from the device
     const bool open = sdf_gate_open(gv, fd->n, gate);
-    if (hostFc && blockIdx.x == 0 && threadIdx.x == 0) {
-        // the frame's final counters to host memory, with whether this
-        // update runs, and the previous mesh update's counters (its kernels
-        // precede this one on the stream), in place of separate copies; the
-        // frame's stamp last, after a system-scope fence: a host that sees
-        // the stamp sees the rest (it need not wait for the update's end)
-        uint4 *h = reinterpret_cast<uint4 *>(hostFc);
-        h[0] = gv.w[0]; h[1] = gv.w[1]; h[2] = gv.w[2];
-        if (carrySrc) {
-            const uint4 a = carrySrc[0], b = carrySrc[1];
-            carryDst[0] = a;
-            carryDst[1] = b;
-        }
-        hostFc->sdfRan = open ? 1u : 0u;
+    if (carrySrc && blockIdx.x == 0 && threadIdx.x == 0) {
+        // the previous mesh update's counters (its kernels precede this one
+        // on the stream) to host memory, in place of a separate copy, then
+        // the frame's stamp after a system-scope fence
+        const uint4 a = carrySrc[0], b = carrySrc[1];
+        carryDst[0] = a;
+        carryDst[1] = b;
         __threadfence_system();
-        *reinterpret_cast<volatile uint32_t *>(&hostFc->pad[0]) = fd->stamp;
+        *reinterpret_cast<volatile uint32_t *>(carrySeq) = fd->stamp;
     }
     if (!open) return;
     const uint16_t *__restrict__ depth = fd->depth;
@@ -1753,7 +1780,8 @@ ff_ctx *ff_create(const ff_params *prm)
               hipMalloc(&c->d_dist, sizeof(float) * NL * FF_BRICK) == hipSuccess &&
               hipMalloc(&c->d_wgt, sizeof(float) * NL * FF_BRICK) == hipSuccess &&
               hipMalloc(&c->d_fc, sizeof(FrameCounters)) == hipSuccess &&
-              hipHostMalloc(&c->h_fc, sizeof(FrameCounters), hipHostMallocMapped | hipHostMallocCoherent) ==
+              // (the frame's counters, then the mesh counters' carry sequence word)
+              hipHostMalloc(&c->h_fc, 2 * sizeof(FrameCounters), hipHostMallocMapped | hipHostMallocCoherent) ==
                   hipSuccess &&
               hipHostGetDevicePointer(reinterpret_cast<void **>(&c->d_hfc), c->h_fc, 0) == hipSuccess &&
               hipMalloc(&c->d_fd, sizeof(FrameDev)) == hipSuccess &&
@@ -1774,6 +1802,9 @@ ff_ctx *ff_create(const ff_params *prm)
         ff_destroy(c);
         return nullptr;
     }
+    memset(c->h_fc, 0, 2 * sizeof(FrameCounters));
+    c->h_carrySeq = reinterpret_cast<uint32_t *>(c->h_fc + 1);
+    c->carrySeq = reinterpret_cast<uint32_t *>(c->d_hfc + 1);
     c->hv.mask = hcap - 1;
     c->hv.listCap = hcap / 2;
     hipMemsetAsync(c->hv.keys, 0xff, sizeof(uint64_t) * hcap, c->stream);
@@ -1889,10 +1920,12 @@ void launch_back(ff_ctx *c, int W, int H, hipStream_t st)
     // this frame's new bricks) is a few spans
     const bool small = (uint64_t)c->nLeavesUsed + FF_ORDER_MAX <= (uint64_t)FF_TAIL_SPANS * FF_QUEUE_SPAN;
     k_alloc_tail<<<1, FF_ORDER_THREADS, 0, st>>>(c->hv, mv, c->d_fc, c->d_pl0, fd, c->d_mapc, binfo, binfoBlocks,
-                                                 small ? 1 : 0, H);
-    if (!small)
+                                                 small ? 1 : 0, H, small ? c->d_hfc : nullptr);
+    if (!small) {
         k_queue<<<queue_grid(c->nLeavesUsed), FF_QUEUE_THREADS, 0, st>>>(c->d_stamp, c->d_mapc, c->nLeavesTotal, fd,
                                                                          c->d_queue, c->d_fc, 0);
+        k_frame_report<<<1, 64, 0, st>>>(c->d_fc, c->d_hfc, fd);
+    }
 }
 
 // A frame with more new nodes than k_order ranks (flagged; its link, commit
@@ -1981,16 +2014,17 @@ static void launch_sdf_only(ff_ctx *c, int W, int H, bool carry, hipStream_t st)
     gate.force = 0;
     const uint4 *cs = carry ? static_cast<const uint4 *>(c->carrySrc) : nullptr;
     uint4 *cd = carry ? static_cast<uint4 *>(c->carryDst) : nullptr;
+    if (cs) c->carryStamp = c->stampCounter;  // this frame's update takes them up
     // (timed on one frame in timeEvery)
     const bool timedNow = carry && c->timing && c->frames % c->timeEvery == 0;
     if (carry) c->sdfTimed = timedNow;
     if (timedNow)
         hipExtLaunchKernelGGL(k_sdf, dim3(FF_SDF_GRID), dim3(512), 0, st, c->ev[2], c->ev[3], 0,
                               (const uint32_t *)c->d_queue, c->mapView(), (const FrameDev *)c->d_fd, W, H, c->d_fc,
-                              (const MapCounters *)c->d_mapc, gate, c->d_hfc, cs, cd);
+                              (const MapCounters *)c->d_mapc, gate, cs, cd, c->carrySeq);
     else
         k_sdf<<<FF_SDF_GRID, 512, 0, st>>>(c->d_queue, c->mapView(), c->d_fd, W, H, c->d_fc, c->d_mapc, gate,
-                                           c->d_hfc, cs, cd);
+                                           cs, cd, c->carrySeq);
     if (cs) c->carrySrc = c->carryDst = nullptr;
 }
 
@@ -2275,16 +2309,17 @@ int frame_finish(ff_ctx *c, int W, int H, bool timed)
     return (int)c->nQueued;
 }
 
-bool await_word(const uint32_t *w, uint32_t want, hipStream_t st)
+bool await_word(const uint32_t *w, uint32_t want, hipStream_t st, bool atLeast)
 {
     const volatile uint32_t *v = w;
+    auto reached = [&]() { const uint32_t x = *v; return atLeast ? (int32_t)(x - want) >= 0 : x == want; };
     for (uint32_t k = 1;; k++) {
-        if (*v == want) break;
+        if (reached()) break;
         if ((k & 255u) == 0u) {
             const hipError_t e = hipStreamQuery(st);
             if (e != hipErrorNotReady) {
                 (void)hipGetLastError();
-                if (*v != want) return false;
+                if (!reached()) return false;
                 break;
             }
         }
@@ -2292,6 +2327,24 @@ bool await_word(const uint32_t *w, uint32_t want, hipStream_t st)
     }
     std::atomic_thread_fence(std::memory_order_acquire);  // what was written before the word
     return true;
+}
+
+// k_sdf's gate (sdf_gate_open), evaluated on the host from the counters the
+// allocation reported: whether the frame's SDF update runs or stands down
+// for the host (a node table that overflowed, a frame that must grow the
+// volume first)
+static bool sdf_gate_host(const ff_ctx *c)
+{
+    const FrameCounters &f = *c->h_fc;
+    const bool held = (f.flags & (FF_FLAG_HASH_FULL | FF_FLAG_ORDER_BIG)) != 0u || f.listCount > c->hv.listCap;
+    if (held) return false;
+    const int n = (int)c->n;
+    const bool out = f.bmin[0] < 0 || f.bmin[1] < 0 || f.bmin[2] < 0 || f.bmax[0] >= n || f.bmax[1] >= n ||
+                     f.bmax[2] >= n;
+    // the used counts the allocation committed
+    const uint64_t nl = (uint64_t)c->nLeavesUsed + f.newL, nb = (uint64_t)c->nBranchesUsed + 16ull * f.newB;
+    const bool grows = nl < c->nLeavesTotal && nb < c->nBranchesTotal;
+    return !(out && grows);
 }
 
 // the SDF phase's time of a timed frame whose update had not ended when the
@@ -2381,18 +2434,20 @@ static int add_map_impl(ff_ctx *c, const uint16_t *depth, const uint8_t *bgr, in
         launch_empty(c, W, H);
     }
     if (overlap && c->n) {
-        // the frame ran on the side stream: its counters (and the previous
-        // mesh update's) are in host memory once its SDF update has started,
-        // which may still run when the call returns
+        // the frame ran on the side stream: its counters are in host memory
+        // once its allocation is done (its SDF update, which follows the
+        // previous mesh update's marching cubes, may not have started when
+        // the call returns)
         if (!await_word(&c->h_fc->pad[0], c->stampCounter, c->side)) {
             c->sdfInFlight = false;
             HIPCHK(hipStreamSynchronize(c->side));
-            c->err = "the frame's SDF update did not report its counters";
+            c->err = "the frame's allocation did not report its counters";
             return FF_E_DEVICE;
         }
     } else {
         HIPCHK(hipStreamSynchronize(c->stream));
     }
+    if (c->n) c->h_fc->sdfRan = sdf_gate_host(c) ? 1u : 0u;
     return frame_finish(c, W, H, true);
 }
 

@@ -200,6 +200,10 @@ struct ff_ctx {
     // (mapped) host memory: the enqueued mesh update's counters
     const void *carrySrc = nullptr;
     void *carryDst = nullptr;
+    // the SDF update that took them up writes its frame's stamp here after
+    // them (mapped host memory; carrySeq as the device addresses it)
+    uint32_t *h_carrySeq = nullptr, *carrySeq = nullptr;
+    uint32_t carryStamp = 0;  // the stamp of that frame
     ff::FrameDev *d_fd = nullptr, *h_fd = nullptr;     // h_fd: pinned staging
     ff::MapCounters *d_mapc = nullptr, *h_mapc = nullptr;
     // the captured per-frame launches: [0] allocation, [1] confidence + SDF
@@ -270,7 +274,7 @@ int frame_finish(ff_ctx *c, int W, int H, bool timed);
 void frame_join(ff_ctx *c, bool stream = false);
 // waits until the mapped host word *w equals want (a kernel's report);
 // false when the stream drained or failed without writing it
-bool await_word(const uint32_t *w, uint32_t want, hipStream_t st);
+bool await_word(const uint32_t *w, uint32_t want, hipStream_t st, bool atLeast = false);
 // the sequence words (ff_ctx::d_seq): after the mesh update's marching
 // cubes, after a frame's SDF update, after the learned confidence, after a
 // frame's host images went up on the side stream

@@ -2699,6 +2699,11 @@ int mesh_complete(ff_ctx *c, bool maintain)
     // update's hash turnover)
     const bool carried = s->carryRequested && !c->carrySrc;
     s->carryRequested = false;
+    if (carried && !await_word(c->h_carrySeq, c->carryStamp, c->side, true)) {
+        MCHK(hipStreamSynchronize(c->side));
+        c->err = "the SDF update did not carry the mesh counters up";
+        return FF_E_DEVICE;
+    }
     if (!carried) {
         MCHK(hipStreamSynchronize(c->stream));
         if (c->carrySrc) {
