@@ -1291,32 +1291,24 @@ __device__ __forceinline__ bool sdf_gate_open(const GateView &v, uint32_t n, con
     return g.force || !(held | (out & grows));
 }
 
-// carrySrc: the previous mesh update's counters, to carryDst (mapped host
-// memory) and then this frame's stamp to *carrySeq
+// tiles: the tile count of the mesh update this launch follows the sampling
+// of (nullptr: it follows the whole update); above tileCap, the update's
+// marching cubes read bricks, and the launch stands down for the host
 __global__ __launch_bounds__(512) void k_sdf(const uint32_t *__restrict__ queue, MapView m,
                                              const FrameDev *__restrict__ fd, int W, int H, FrameCounters *fc,
                                              const MapCounters *__restrict__ mapc, SdfGate gate,
-                                             const uint4 *carrySrc, uint4 *carryDst, uint32_t *carrySeq)
+                                             const uint32_t *tiles, uint32_t tileCap)
 {
     rz_mode_on();  // first: every float operation below rounds toward zero
     // the counters, the rcp table's entries and the first queue slot are
     // all requested before anything waits on them
     const GateView gv = gate_load(fc, mapc);
+    const uint32_t nTiles = tiles ? *tiles : 0u;
     uint32_t tv[2048 / 512];
 #pragma unroll
     for (int k = 0; k < 2048 / 512; k++) tv[k] = c_rcp_table[threadIdx.x + 512 * k];
     // blocks loop over the queued bricks; the count comes from the device
-    const bool open = sdf_gate_open(gv, fd->n, gate);
-    if (carrySrc && blockIdx.x == 0 && threadIdx.x == 0) {
-        // the previous mesh update's counters (its kernels precede this one
-        // on the stream) to host memory, in place of a separate copy, then
-        // the frame's stamp after a system-scope fence
-        const uint4 a = carrySrc[0], b = carrySrc[1];
-        carryDst[0] = a;
-        carryDst[1] = b;
-        __threadfence_system();
-        *reinterpret_cast<volatile uint32_t *>(carrySeq) = fd->stamp;
-    }
+    const bool open = sdf_gate_open(gv, fd->n, gate) && nTiles <= tileCap;
     if (!open) return;
     const uint16_t *__restrict__ depth = fd->depth;
     const uint8_t *__restrict__ bgr = fd->bgr;
@@ -1780,8 +1772,7 @@ ff_ctx *ff_create(const ff_params *prm)
               hipMalloc(&c->d_dist, sizeof(float) * NL * FF_BRICK) == hipSuccess &&
               hipMalloc(&c->d_wgt, sizeof(float) * NL * FF_BRICK) == hipSuccess &&
               hipMalloc(&c->d_fc, sizeof(FrameCounters)) == hipSuccess &&
-              // (the frame's counters, then the mesh counters' carry sequence word)
-              hipHostMalloc(&c->h_fc, 2 * sizeof(FrameCounters), hipHostMallocMapped | hipHostMallocCoherent) ==
+              hipHostMalloc(&c->h_fc, sizeof(FrameCounters), hipHostMallocMapped | hipHostMallocCoherent) ==
                   hipSuccess &&
               hipHostGetDevicePointer(reinterpret_cast<void **>(&c->d_hfc), c->h_fc, 0) == hipSuccess &&
               hipMalloc(&c->d_fd, sizeof(FrameDev)) == hipSuccess &&
@@ -1802,9 +1793,7 @@ ff_ctx *ff_create(const ff_params *prm)
         ff_destroy(c);
         return nullptr;
     }
-    memset(c->h_fc, 0, 2 * sizeof(FrameCounters));
-    c->h_carrySeq = reinterpret_cast<uint32_t *>(c->h_fc + 1);
-    c->carrySeq = reinterpret_cast<uint32_t *>(c->d_hfc + 1);
+    memset(c->h_fc, 0, sizeof(FrameCounters));
     c->hv.mask = hcap - 1;
     c->hv.listCap = hcap / 2;
     hipMemsetAsync(c->hv.keys, 0xff, sizeof(uint64_t) * hcap, c->stream);
@@ -2002,30 +1991,28 @@ void launch_alloc_batch(hipStream_t st, const AllocStream *d_as, const AllocStre
 }
 
 // the SDF update waits for the host only when the node table overflowed or
-// the volume must grow first; carry: a direct launch, which also takes the
-// previous mesh update's counters up and, with timing on, has the dispatch
-// itself record the SDF phase's events (they then time the kernel alone)
-static void launch_sdf_only(ff_ctx *c, int W, int H, bool carry, hipStream_t st)
+// the volume must grow first (or, following a mesh update's sampling, when
+// that update's marching cubes read bricks: tiles); direct: a direct launch,
+// which with timing on has the dispatch itself record the SDF phase's events
+// (they then time the kernel alone)
+static void launch_sdf_only(ff_ctx *c, int W, int H, bool direct, hipStream_t st, const uint32_t *tiles = nullptr,
+                            uint32_t tileCap = 0)
 {
     SdfGate gate;
     gate.listCap = c->hv.listCap;
     gate.leavesTotal = c->nLeavesTotal;
     gate.branchesTotal = c->nBranchesTotal;
     gate.force = 0;
-    const uint4 *cs = carry ? static_cast<const uint4 *>(c->carrySrc) : nullptr;
-    uint4 *cd = carry ? static_cast<uint4 *>(c->carryDst) : nullptr;
-    if (cs) c->carryStamp = c->stampCounter;  // this frame's update takes them up
     // (timed on one frame in timeEvery)
-    const bool timedNow = carry && c->timing && c->frames % c->timeEvery == 0;
-    if (carry) c->sdfTimed = timedNow;
+    const bool timedNow = direct && c->timing && c->frames % c->timeEvery == 0;
+    if (direct) c->sdfTimed = timedNow;
     if (timedNow)
         hipExtLaunchKernelGGL(k_sdf, dim3(FF_SDF_GRID), dim3(512), 0, st, c->ev[2], c->ev[3], 0,
                               (const uint32_t *)c->d_queue, c->mapView(), (const FrameDev *)c->d_fd, W, H, c->d_fc,
-                              (const MapCounters *)c->d_mapc, gate, cs, cd, c->carrySeq);
+                              (const MapCounters *)c->d_mapc, gate, tiles, tileCap);
     else
-        k_sdf<<<FF_SDF_GRID, 512, 0, st>>>(c->d_queue, c->mapView(), c->d_fd, W, H, c->d_fc, c->d_mapc, gate,
-                                           cs, cd, c->carrySeq);
-    if (cs) c->carrySrc = c->carryDst = nullptr;
+        k_sdf<<<FF_SDF_GRID, 512, 0, st>>>(c->d_queue, c->mapView(), c->d_fd, W, H, c->d_fc, c->d_mapc, gate, tiles,
+                                           tileCap);
 }
 
 // (the learned confidence first when on; the counters' copy back follows
@@ -2064,10 +2051,11 @@ static int run_frame(ff_ctx *c, int W, int H, bool overlap)
 {
     if (overlap) {
         // the frame on the side stream, beside the mesh update still in
-        // flight on the main stream: the front at once, the descent once the
-        // update has passed over the tree and the stamps, the SDF update once
-        // its marching cubes (which read the bricks) are done; the update's
-        // hash turnover may still run beside it
+        // flight on the main stream: the front at once, the descent beside
+        // the update's partition and marking (which read the tree as counted
+        // at their launch), the SDF update once the update's sampling is
+        // done (its marching cubes may still run); the update's hash
+        // turnover follows the SDF update on the side stream
         // the learned confidence reads only the image: it starts at once on
         // the third stream, beside the de-dup and the descent
         bool confAside = c->learned && c->aux;
@@ -2083,16 +2071,26 @@ static int run_frame(ff_ctx *c, int W, int H, bool overlap)
         launch_front(c, W, H, c->side, true);  // carries the frame's parameters up
         // a requested mesh update (of the map before this frame) goes on the
         // main stream now: the front reads only the new frame
+        c->frameInFlush = true;
         const int rf = mesh_flush(c);
+        c->frameInFlush = false;
         if (rf != FF_OK) return rf;
         launch_back(c, W, H, c->side);
         if (c->learned) {
             if (confAside) seq_wait(c, FF_SEQ_CONF, c->side);
             else if (!c->aux) learned_launch(c, W, H, c->side);
         }
-        if (c->mcPending) seq_wait(c, FF_SEQ_MC, c->side);
-        c->mcPending = false;
-        launch_sdf_only(c, W, H, true, c->side);  // which leaves the counters in h_fc
+        const uint32_t *tiles = nullptr;
+        c->sdfTileId = 0;
+        if (c->samplePending) {
+            seq_wait(c, FF_SEQ_SAMPLE, c->side);
+            tiles = c->pendingTiles;
+            c->sdfTileId = c->pendingId;
+            c->sdfTileCap = c->pendingTileCap;
+            c->samplePending = false;
+        }
+        launch_sdf_only(c, W, H, true, c->side, tiles, c->sdfTileCap);
+        mesh_launch_deferred_turnover(c, c->side);
         c->sdfInFlight = seq_signal(c, FF_SEQ_SDF, c->side);  // (else the side stream has drained)
         HIPCHK(hipGetLastError());
         return FF_OK;
@@ -2282,7 +2280,7 @@ int frame_finish(ff_ctx *c, int W, int H, bool timed)
         gate.force = 1;
         if (c->timing) hipEventRecord(c->ev[4], c->stream);
         k_sdf<<<FF_SDF_GRID, 512, 0, c->stream>>>(c->d_queue, c->mapView(), c->d_fd, W, H, c->d_fc, c->d_mapc, gate,
-                                                  nullptr, nullptr, nullptr);
+                                                  nullptr, 0u);
         if (c->timing) hipEventRecord(c->ev[5], c->stream);
         HIPCHK(hipGetLastError());
         HIPCHK(hipStreamSynchronize(c->stream));
@@ -2421,7 +2419,9 @@ static int add_map_impl(ff_ctx *c, const uint16_t *depth, const uint8_t *bgr, in
         const int rf = mesh_flush(c);
         if (rf != FF_OK) return rf;
         frame_join(c, true);
+        c->samplePending = false;  // the frame follows the whole update on the stream
     }
+    c->sdfTileId = 0;
     sdf_timing_collect(c);
     seq_rebase(c);
     int rc = frame_prepare(c, depth, bgr, W, H, R, t, K, scaling, maxcam, !overlap);
@@ -2447,7 +2447,22 @@ static int add_map_impl(ff_ctx *c, const uint16_t *depth, const uint8_t *bgr, in
     } else {
         HIPCHK(hipStreamSynchronize(c->stream));
     }
-    if (c->n) c->h_fc->sdfRan = sdf_gate_host(c) ? 1u : 0u;
+    if (c->n) {
+        bool ran = sdf_gate_host(c);
+        if (ran && c->sdfTileId) {
+            // and the tile count of the mesh update it followed
+            uint32_t nt = 0;
+            if (!mesh_sample_report(c, c->sdfTileId, &nt)) {
+                c->sdfInFlight = false;
+                HIPCHK(hipStreamSynchronize(c->side));
+                HIPCHK(hipStreamSynchronize(c->stream));
+                c->err = "the mesh update did not report its tile count";
+                return FF_E_DEVICE;
+            }
+            ran = nt <= c->sdfTileCap;
+        }
+        c->h_fc->sdfRan = ran ? 1u : 0u;
+    }
     return frame_finish(c, W, H, true);
 }
 

@@ -143,8 +143,12 @@ struct ff_ctx {
     hipStream_t stream = nullptr;
     // a frame (bit planes, de-dup, descent, node commit, queue, SDF update)
     // runs on `side`, beside the previous mesh update on `stream`, and only
-    // the SDF update waits: for the update's marching cubes (the last of its
-    // kernels to read bricks; sequence word FF_SEQ_MC).  The descent and the
+    // the SDF update waits: for the update's sampling (sequence word
+    // FF_SEQ_SAMPLE), after which its marching cubes read the sample buffer
+    // and no brick, unless some cells' samples did not fit the buffer (the
+    // update's tile count above its capacity): then the SDF update stands
+    // down for the host, the device and the host both deciding from that
+    // count (pendingTiles, reported to h_rep).  The descent and the
     // allocation may run beside the update's partition and marking: the
     // update reads the tree only through the bricks and branch slots counted
     // when it was launched (a later frame's links point past them), and a
@@ -175,7 +179,18 @@ struct ff_ctx {
     // frame's de-dup and descent; its SDF update waits for FF_SEQ_CONF
     hipStream_t aux = nullptr;
     bool uploadOnSide = false;  // the frame in flight had its host images copied up on `side`
-    bool mcPending = false;  // a mesh update signalled FF_SEQ_MC since the last frame
+    // the last mesh update's sampling signalled FF_SEQ_SAMPLE and no frame
+    // has waited for it yet; its tile count (device), the buffer's capacity
+    // then, and its number
+    bool samplePending = false;
+    const uint32_t *pendingTiles = nullptr;
+    uint32_t pendingTileCap = 0, pendingId = 0;
+    // the SDF update of the frame in flight was gated on the tile count of
+    // update sdfTileId (0: on none)
+    uint32_t sdfTileId = 0, sdfTileCap = 0;
+    // a frame is enqueueing a requested mesh update: the update leaves its
+    // hash turnover for the frame to launch on `side` after its SDF update
+    bool frameInFlush = false;
     // map arrays
     uint32_t *d_tree = nullptr, *d_treeTmp = nullptr, *d_parent = nullptr, *d_stamp = nullptr, *d_queue = nullptr;
     ushort4 *d_leafinfo = nullptr;
@@ -196,14 +211,6 @@ struct ff_ctx {
     uint64_t *d_pref = nullptr, *d_bsum = nullptr;
     ff::FrameCounters *d_fc = nullptr, *h_fc = nullptr;
     ff::FrameCounters *d_hfc = nullptr;  // h_fc (mapped host memory) as the device addresses it
-    // 32 bytes the next directly launched SDF update copies from device to
-    // (mapped) host memory: the enqueued mesh update's counters
-    const void *carrySrc = nullptr;
-    void *carryDst = nullptr;
-    // the SDF update that took them up writes its frame's stamp here after
-    // them (mapped host memory; carrySeq as the device addresses it)
-    uint32_t *h_carrySeq = nullptr, *carrySeq = nullptr;
-    uint32_t carryStamp = 0;  // the stamp of that frame
     ff::FrameDev *d_fd = nullptr, *h_fd = nullptr;     // h_fd: pinned staging
     ff::MapCounters *d_mapc = nullptr, *h_mapc = nullptr;
     // the captured per-frame launches: [0] allocation, [1] confidence + SDF
@@ -275,10 +282,11 @@ void frame_join(ff_ctx *c, bool stream = false);
 // waits until the mapped host word *w equals want (a kernel's report);
 // false when the stream drained or failed without writing it
 bool await_word(const uint32_t *w, uint32_t want, hipStream_t st, bool atLeast = false);
-// the sequence words (ff_ctx::d_seq): after the mesh update's marching
-// cubes, after a frame's SDF update, after the learned confidence, after a
-// frame's host images went up on the side stream
-enum { FF_SEQ_MC = 0, FF_SEQ_SDF = 1, FF_SEQ_CONF = 2, FF_SEQ_UP = 3 };
+// the sequence words (ff_ctx::d_seq): after the mesh update's sampling,
+// after a frame's SDF update (and the hash turnover launched behind it),
+// after the learned confidence, after a frame's host images went up on the
+// side stream
+enum { FF_SEQ_SAMPLE = 0, FF_SEQ_SDF = 1, FF_SEQ_CONF = 2, FF_SEQ_UP = 3 };
 // enqueues on st: word k takes its next value; false (st drained instead)
 // when the runtime refused
 bool seq_signal(ff_ctx *c, int k, hipStream_t st);
@@ -306,6 +314,11 @@ int mesh_complete(ff_ctx *c, bool maintain = true);
 // enqueues a requested mesh update (updateMeshes records the request; the
 // next frame enqueues it after its first kernels, any other call first)
 int mesh_flush(ff_ctx *c);
+// the tile count update `id` reported when its sampling began (false: the
+// stream drained without the report)
+bool mesh_sample_report(ff_ctx *c, uint32_t id, uint32_t *nTiles);
+// the hash turnover an update enqueued by a frame left to it, on st
+void mesh_launch_deferred_turnover(ff_ctx *c, hipStream_t st);
 void mesh_free(ff_ctx *c);
 void mesh_on_initial_volume(ff_ctx *c);
 int mesh_on_load(ff_ctx *c, const std::vector<ushort4> &binfo);

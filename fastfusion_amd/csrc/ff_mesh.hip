@@ -226,7 +226,16 @@ struct MeshState {
     bool requested = false;
     uint32_t reqStamp = 0;
     uint32_t stampFor = 0;  // the stamp of the update being enqueued
-    bool carryRequested = false;  // the update's counters left for the next SDF update to take up
+    // reports in mapped host memory after h_mc: h_rep[0] the update's tile
+    // count and h_rep[1] its number (k_sample, when the marking is done),
+    // h_rep[2] its number once its counters are in h_mc (k_mesh_report,
+    // after the marching cubes); d_rep as the device addresses them
+    uint32_t *h_rep = nullptr, *d_rep = nullptr;
+    bool doneMapped = false;  // the update in flight reports its counters itself
+    // a hash turnover of the update in flight that the next frame launches
+    // on its side stream after its SDF update (off the main stream)
+    bool turnPending = false;
+    int turnTable = 0;
     // the map the update in flight meshes: the frame stamp and the tree
     // (frames fused before its host part runs must not be attributed to it)
     uint32_t launchStamp = 0, launchNB = 0, launchNL = 0, launchGrows = 0;
@@ -1526,11 +1535,37 @@ __device__ __forceinline__ void sample_body(const CellGeom *__restrict__ geom, C
     }
 }
 
+// rep: where the update's tile count and then its number go (mapped host
+// memory), so that the host knows whether the next frame's SDF update,
+// which follows this sampling, stands down (cells whose samples did not fit
+// the buffer are sampled from the bricks by the marching cubes)
 __global__ __launch_bounds__(256) void k_sample(const CellGeom *__restrict__ geom, CellRec *cells,
                                                 const TileRec *__restrict__ tiles, const MeshCounters *mc,
-                                                uint32_t tileCap, MapRO m, uint4 *__restrict__ sbuf)
+                                                uint32_t tileCap, MapRO m, uint4 *__restrict__ sbuf,
+                                                uint32_t *rep, uint32_t updateId)
 {
+    if (rep && blockIdx.x == 0 && threadIdx.x == 0) {
+        rep[0] = mc->nTiles;
+        __threadfence_system();
+        *reinterpret_cast<volatile uint32_t *>(&rep[1]) = updateId;
+    }
     sample_body(geom, cells, tiles, mc, tileCap, m, sbuf);
+}
+
+// the update's counters to mapped host memory once its marching cubes are
+// done, then (after a system-scope fence) its number: a host that sees the
+// number sees the counters
+__global__ void k_mesh_report(const MeshCounters *__restrict__ mc, MeshCounters *hmc, uint32_t *seq, uint32_t id)
+{
+    if (threadIdx.x == 0) {
+        const uint4 *src = reinterpret_cast<const uint4 *>(mc);
+        uint4 *dst = reinterpret_cast<uint4 *>(hmc);
+        const uint4 a = src[0], b = src[1];
+        dst[0] = a;
+        dst[1] = b;
+        __threadfence_system();
+        *reinterpret_cast<volatile uint32_t *>(seq) = id;
+    }
 }
 
 __global__ __launch_bounds__(256) void k_sample_b(const MeshBatchStream *__restrict__ S)
@@ -2142,9 +2177,12 @@ int mesh_init(ff_ctx *c)
         MCHK(hipMemset(s->d_mcBuf[k], 0, sizeof(MeshCounters)));
     }
     s->d_mc = s->d_mcBuf[0];
-    MCHK(hipHostMalloc(&s->h_mc, sizeof(MeshCounters), hipHostMallocMapped | hipHostMallocCoherent));
+    // the counters, then the report words
+    MCHK(hipHostMalloc(&s->h_mc, 2 * sizeof(MeshCounters), hipHostMallocMapped | hipHostMallocCoherent));
     MCHK(hipHostGetDevicePointer(reinterpret_cast<void **>(&s->d_hmc), s->h_mc, 0));
-    memset(s->h_mc, 0, sizeof(MeshCounters));
+    memset(s->h_mc, 0, 2 * sizeof(MeshCounters));
+    s->h_rep = reinterpret_cast<uint32_t *>(s->h_mc + 1);
+    s->d_rep = reinterpret_cast<uint32_t *>(s->d_hmc + 1);
 
     return FF_OK;
 }
@@ -2353,6 +2391,27 @@ static bool mesh_pre_host(ff_ctx *c, bool allowReuse)
     return true;
 }
 
+// after the sampling: the next frame's SDF update may start (the marching
+// cubes read the sample buffer; when some cells did not fit it, which the
+// update's tile count shows, the SDF update stands down)
+static void mesh_samples_taken(ff_ctx *c)
+{
+    MeshState *s = c->mesh;
+    if (!c->seqOk) return;
+    c->samplePending = seq_signal(c, FF_SEQ_SAMPLE, c->stream);  // (else the stream has drained)
+    c->pendingTiles = &s->d_mc->nTiles;
+    c->pendingTileCap = s->tileCap;
+    c->pendingId = s->launchId;
+}
+
+bool mesh_sample_report(ff_ctx *c, uint32_t id, uint32_t *nTiles)
+{
+    MeshState *s = c->mesh;
+    if (!await_word(&s->h_rep[1], id, c->stream)) return false;
+    *nTiles = s->h_rep[0];
+    return true;
+}
+
 bool mesh_launch_pre(ff_ctx *c, bool timed)
 {
     MeshState *s = c->mesh;
@@ -2368,7 +2427,8 @@ bool mesh_launch_pre(ff_ctx *c, bool timed)
             s->d_cells[nxt], c->d_stamp, c->d_parent, s->lastMeshStamp, s->d_mc, nc, mc_other(s), s->d_geom,
             s->cellCap, m, s->d_tiles, s->tileCap, s->launchId);
         k_sample<<<4096, 256, 0, c->stream>>>(s->d_geom, s->d_cells[nxt], s->d_tiles, s->d_mc, s->tileCap, m,
-                                              s->d_sbuf);
+                                              s->d_sbuf, s->d_rep, s->launchId);
+        mesh_samples_taken(c);
         mc_flip(s);
         return true;
     }
@@ -2412,9 +2472,11 @@ bool mesh_launch_pre(ff_ctx *c, bool timed)
                                                         s->lastMeshStamp, s->d_mc, s->d_geom, s->cellCap, m,
                                                         s->d_tiles, s->tileCap, s->launchId);
     }
-    k_sample<<<4096, 256, 0, c->stream>>>(s->d_geom, s->d_cells[nxt], s->d_tiles, s->d_mc, s->tileCap, m, s->d_sbuf);
+    k_sample<<<4096, 256, 0, c->stream>>>(s->d_geom, s->d_cells[nxt], s->d_tiles, s->d_mc, s->tileCap, m, s->d_sbuf,
+                                          s->d_rep, s->launchId);
+    mesh_samples_taken(c);
     mc_flip(s);
-        return true;
+    return true;
 }
 
 // the descriptor of a context in a batched update (after mesh_pre_host)
@@ -2485,8 +2547,31 @@ void mesh_launch_mc(ff_ctx *c)
     else
         k_mc<<<2048, 256, 0, c->stream>>>(s->d_cells[s->nxt], s->d_geom, s->d_mc, s->d_sbuf, s->m, pools(s),
                                           s->cellCap);
-    // the bricks are read: the next frame's SDF update may start
-    if (c->seqOk) c->mcPending = seq_signal(c, FF_SEQ_MC, c->stream);  // (else the stream has drained)
+    // the counters are final (the hash turnover does not change them)
+    s->doneMapped = !c->timingAll;
+    if (s->doneMapped) k_mesh_report<<<1, 64, 0, c->stream>>>(s->d_mc, s->d_hmc, &s->d_rep[2], s->launchId);
+}
+
+// the hash turnover of an update that built a new table: the previous
+// table's keys leave the lookup hash, the new ones enter the other one
+static void launch_turnover(ff_ctx *c, int nxt, hipStream_t st)
+{
+    MeshState *s = c->mesh;
+    const int b = 1 - s->hIdx;
+    k_hash_turnover<<<1024, 256, 0, st>>>(s->d_cells[s->cur], s->nCells[s->cur], s->d_hkeys, s->d_cells[nxt], s->d_mc,
+                                          s->cellCap, s->d_hkeysBuf[b], s->d_hvalBuf[b], s->hmask,
+                                          c->timingAll ? s->d_arena : nullptr, s->d_inc);
+    s->hIdx = b;  // the next update looks the new table up there
+    s->d_hkeys = s->d_hkeysBuf[b];
+    s->d_hval = s->d_hvalBuf[b];
+}
+
+void mesh_launch_deferred_turnover(ff_ctx *c, hipStream_t st)
+{
+    MeshState *s = c->mesh;
+    if (!s || !s->turnPending) return;
+    s->turnPending = false;
+    launch_turnover(c, s->turnTable, st);
 }
 
 void mesh_launch_post(ff_ctx *c, bool timed)
@@ -2496,27 +2581,18 @@ void mesh_launch_post(ff_ctx *c, bool timed)
     if (s->reuse) {
         // same table, same keys: the lookup hash stands
         if (c->timingAll) k_arena_sum<<<1, 64, 0, c->stream>>>(s->d_arena, s->d_mc);
+    } else if (c->frameInFlush) {
+        // the frame that enqueues this update launches it on its side stream
+        // after its SDF update (it reads only the tables, which the marching
+        // cubes leave alone but for the mesh ranges); the next update waits
+        // for that frame's SDF sequence word, which follows it
+        s->turnPending = true;
+        s->turnTable = nxt;
     } else {
-        // the previous table's keys leave the lookup hash, the new ones enter it
-        const int b = 1 - s->hIdx;
-        k_hash_turnover<<<1024, 256, 0, c->stream>>>(s->d_cells[s->cur], s->nCells[s->cur], s->d_hkeys,
-                                                     s->d_cells[nxt], s->d_mc, s->cellCap, s->d_hkeysBuf[b],
-                                                     s->d_hvalBuf[b], s->hmask, c->timingAll ? s->d_arena : nullptr,
-                                                     s->d_inc);
-        s->hIdx = b;  // the next update looks the new table up there
-        s->d_hkeys = s->d_hkeysBuf[b];
-        s->d_hval = s->d_hvalBuf[b];
+        launch_turnover(c, nxt, c->stream);
     }
     if (timed && c->timingAll) hipEventRecord(c->ev[7], c->stream);
-    if (c->timingAll) {
-        hipMemcpyAsync(s->h_mc, s->d_mc, sizeof(MeshCounters), hipMemcpyDeviceToHost, c->stream);
-    } else {
-        // the next SDF update carries the counters up (the host part
-        // copies them itself if none was launched by then)
-        c->carrySrc = s->d_mc;
-        c->carryDst = s->d_hmc;
-        s->carryRequested = true;
-    }
+    if (c->timingAll) hipMemcpyAsync(s->h_mc, s->d_mc, sizeof(MeshCounters), hipMemcpyDeviceToHost, c->stream);
 }
 
 // the allocating part of an update's host part (see maintPending)
@@ -2693,23 +2769,20 @@ int mesh_complete(ff_ctx *c, bool maintain)
     if (rf != FF_OK) return rf;
     if (!s->pending) return maintain ? mesh_maintain(c) : FF_OK;
     s->pending = false;
-    // counters taken up by a frame's SDF update, which follows the update's
-    // marching cubes and had completed when its frame returned: the host
-    // part needs no wait (later work on the stream stays ordered after the
-    // update's hash turnover)
-    const bool carried = s->carryRequested && !c->carrySrc;
-    s->carryRequested = false;
-    if (carried && !await_word(c->h_carrySeq, c->carryStamp, c->side, true)) {
-        MCHK(hipStreamSynchronize(c->side));
-        c->err = "the SDF update did not carry the mesh counters up";
-        return FF_E_DEVICE;
-    }
-    if (!carried) {
-        MCHK(hipStreamSynchronize(c->stream));
-        if (c->carrySrc) {
-            MCHK(hipMemcpy(s->h_mc, s->d_mc, sizeof(MeshCounters), hipMemcpyDeviceToHost));
-            c->carrySrc = c->carryDst = nullptr;
+    // (a turnover left for a frame that did not launch it goes now)
+    mesh_launch_deferred_turnover(c, c->stream);
+    if (s->doneMapped) {
+        // the counters are in host memory once the marching cubes are done:
+        // no wait for the rest of the stream (later work on it stays
+        // ordered after the update's hash turnover)
+        s->doneMapped = false;
+        if (!await_word(&s->h_rep[2], s->launchId, c->stream)) {
+            MCHK(hipStreamSynchronize(c->stream));
+            c->err = "the mesh update did not report its counters";
+            return FF_E_DEVICE;
         }
+    } else {
+        MCHK(hipStreamSynchronize(c->stream));
     }
     const int r = mesh_finish(c, s->pendingTimed);
     if (r < 0) return r;

@@ -96,6 +96,27 @@ def test_mesh_sample_buffer_overflow():
     assert nface > 1000
 
 
+def test_mesh_sample_buffer_overflow_beside_next_frame():
+    """The same 8-tile buffer with the meshes read only every third frame:
+    the first update is enqueued by the next frame, whose SDF update follows
+    that update's sampling and must stand down for the host (the cells that
+    did not fit are sampled from the bricks by the marching cubes, which may
+    still run); the meshes read later still equal the oracle's."""
+    sc, R, t, cfg = synth.config_c1(n_frames=200)
+    K = (262.5, 262.5, 159.5, 119.5)
+    gpu, ora = make_pair(cfg["scale"], cfg["threshold"])
+    gpu.setMeshSampleTiles(8)
+    for k in range(7):
+        d, bgr = sc.render(R[k].reshape(3, 3), t[k], K, 320, 240, noise=cfg["noise"], dropout=cfg["dropout"],
+                           seed=1000 + k)
+        assert gpu.addMap(d, R[k], t[k], K, bgr, 1 / 5000.0, cfg["maxcam"]) == \
+            ora.add_map(d, R[k], t[k], K, bgr, 1 / 5000.0, cfg["maxcam"])
+        gpu.updateMeshes()
+        if k in (3, 6):
+            ncell, nface = compare_mesh(gpu, ora, cfg["scale"], f"frame {k}")
+    assert nface > 1000
+
+
 def test_mesh_unchanged_tree_remarks_in_place():
     """Frames that create no brick or branch leave the cell partition as it
     was: the update then re-marks the previous table in place.  Re-fusing
