@@ -253,17 +253,24 @@ __global__ __launch_bounds__(64) void k_planes_b(const AllocStream *__restrict__
 // lane is ring row s at the lane's column, read with constant offsets; rows
 // outside the image are written as zeros, so no step tests bounds.  Four
 // bands of R diagonals, written three bands ahead of the sweep, one block
-// barrier per band.  pbits rows are FF_PB_STRIDE words apart with
-// FF_PB_PAD rows of padding above and below, so every step stores
-// unconditionally.
+// barrier per band.  The sweep's results go to a second LDS ring (128 steps
+// of 64 lanes, one conflict-free store per step); after each band the other
+// waves write the rows it completed to pbits, a row's words together.
+// Storing a step's 64 words straight to pbits (64 rows, 64 lines) cost the
+// computing wave about twice its instruction time.
 // ---------------------------------------------------------------------------
 #define FF_DD_NB 4
 #define FF_DD_R 32
 #define FF_DD_THREADS 1024
+#define FF_DD_PSTEPS 128  // the result ring's steps (covers the rows a band completes and the next band)
 
 // ring columns: 20, 40 or 64 words (images up to 640, 1280, 2048 pixels wide)
 __host__ __device__ static inline int dedup_wmax(int words) { return words <= 20 ? 20 : (words <= 40 ? 40 : 64); }
-size_t dedup_lds_bytes(int words) { return (size_t)FF_DD_NB * FF_DD_R * dedup_wmax(words) * sizeof(uint4); }
+// the plane ring, then the result ring
+size_t dedup_lds_bytes(int words)
+{
+    return (size_t)FF_DD_NB * FF_DD_R * dedup_wmax(words) * sizeof(uint4) + FF_DD_PSTEPS * 64 * sizeof(uint32_t);
+}
 int dedup_plane_rows(int H) { return H; }
 
 // the ring entries one loader thread moves per band
@@ -313,10 +320,31 @@ __device__ __forceinline__ void dedup_put(const DedupLoad<WMAX> &L, uint4 *ring,
     }
 }
 
+// the rows band `it` completed (their last word, at step row + words - 1,
+// in the band) from the result ring to pbits; rows outside the image are
+// skipped.  A row's first word is at most 63 steps older (two bands back);
+// while this runs the computing wave fills band it + 1, whose ring slots
+// held band it - 3, and it reaches band it + 2 (band it - 2's slots) only
+// after the next barrier, which waits for these LDS reads.
+template <int WMAX>
+__device__ __forceinline__ void dedup_writeout(const uint32_t *pring, uint32_t *__restrict__ pbits, int H, int words,
+                                               int it, int lt)
+{
+    const int r0 = it * FF_DD_R - words + 1;
+    for (int e = lt; e < FF_DD_R * WMAX; e += FF_DD_THREADS - 64) {
+        const int j = e / WMAX, w = e - j * WMAX;
+        const int r = r0 + j;
+        if (w < words && r >= 0 && r < H)
+            pbits[(size_t)(FF_PB_PAD + r) * FF_PB_STRIDE + w] = pring[((r + w) & (FF_DD_PSTEPS - 1)) * 64 + w];
+    }
+}
+
 template <int WMAX>
 __device__ __forceinline__ void dedup_body(uint4 *ring, const uint32_t *__restrict__ planes32, int H, int words,
                                            uint32_t *__restrict__ pbits, FrameCounters *fc)
 {
+    // (the result ring follows the plane ring)
+    uint32_t *pring = reinterpret_cast<uint32_t *>(ring + FF_DD_NB * FF_DD_R * WMAX);
     constexpr int R = FF_DD_R, RING = FF_DD_NB * R;
     constexpr uint32_t EVEN = 0x55555555u;
     const uint4 *planes = reinterpret_cast<const uint4 *>(planes32);
@@ -340,7 +368,7 @@ __device__ __forceinline__ void dedup_body(uint4 *ring, const uint32_t *__restri
     __syncthreads();  // the first three bands are in the ring
     const int l = tid & 63;
     const int lw = l < WMAX ? l : WMAX - 1;  // lanes past the ring's columns read a column they ignore
-    uint32_t *out = pbits + (size_t)(FF_PB_PAD - l) * FF_PB_STRIDE + l;  // row 0 - l: the lane's row at step 0
+    uint32_t *pw = pring + l;  // the lane's column of the result ring
     uint32_t pprev = 0u, pin = 0u, nproc = 0u;
     // The computing wave and the loaders run separate loops with one barrier
     // per iteration each, niter rounded up to even in both (the computing
@@ -362,7 +390,7 @@ __device__ __forceinline__ void dedup_body(uint4 *ring, const uint32_t *__restri
             uint4 rq[R];
 #pragma unroll
             for (int k = 0; k < R; k++) rq[k] = q[k * WMAX];
-            uint32_t *o = out + (size_t)it * R * FF_PB_STRIDE;
+            uint32_t *o = pw + ((it * R) & (FF_DD_PSTEPS - 1)) * 64;
 #pragma unroll
             for (int k = 0; k < R; k++) {
                 const uint32_t va = rq[k].x, el = rq[k].y, eu = rq[k].z;
@@ -375,7 +403,7 @@ __device__ __forceinline__ void dedup_body(uint4 *ring, const uint32_t *__restri
                 const uint32_t t = (ev << 1) + cont;                   // carries run through each such run
                 const uint32_t E = (~t & cont) | ev;                   // the bits of runs with an even start
                 const uint32_t P = a & ~(E ^ EVEN);                    // even distance from the run's start
-                o[k * FF_PB_STRIDE] = P;
+                o[k * 64] = P;
                 nproc += (uint32_t)__popc(P);
                 pprev = P;
                 pin = (uint32_t)__builtin_amdgcn_mov_dpp((int)P, 0x138, 0xf, 0xf, true);  // wave_shr:1, lane 0 gets 0
@@ -391,9 +419,11 @@ __device__ __forceinline__ void dedup_body(uint4 *ring, const uint32_t *__restri
             dedup_put<WMAX>(pre[1], ring, it + 3, lt);
             dedup_fetch<WMAX>(pre[1], planes, H, words, it + 5, lt);
             __syncthreads();
+            dedup_writeout<WMAX>(pring, pbits, H, words, it, lt);
             dedup_put<WMAX>(pre[0], ring, it + 4, lt);
             dedup_fetch<WMAX>(pre[0], planes, H, words, it + 6, lt);
             __syncthreads();
+            dedup_writeout<WMAX>(pring, pbits, H, words, it + 1, lt);
         }
     }
     if (compute) {

@@ -1969,10 +1969,16 @@ __device__ void mc_cell(WaveLds &L, const uint64_t *tri, int lane, CellRec *cell
 }
 
 // (dirty cells in slot order as k_mark / k_mark_same split them: the long
-// ones first)
+// ones first).  sampleSeq: the stream starts this launch only once the
+// sampling before it has ended, so its first thread writes sampleVal there,
+// the sequence word a frame's SDF update waits for (in place of a stream
+// memory operation, which cost ~13 us of host time on this stream)
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k_mc(CellRec *cells, const CellGeom *__restrict__ geom, MeshCounters *mc,
-                                            const uint4 *__restrict__ sbuf, MapRO m, Pools P, uint32_t geomCap)
+                                            const uint4 *__restrict__ sbuf, MapRO m, Pools P, uint32_t geomCap,
+                                            uint32_t *sampleSeq, uint32_t sampleVal)
 {
+    if (sampleSeq && blockIdx.x == 0 && threadIdx.x == 0)
+        __hip_atomic_store(sampleSeq, sampleVal, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
     __shared__ WaveLds lds[4];
     __shared__ uint64_t tri[256];
     tri[threadIdx.x] = c_tri64[threadIdx.x];  // 256 threads
@@ -2391,14 +2397,14 @@ static bool mesh_pre_host(ff_ctx *c, bool allowReuse)
     return true;
 }
 
-// after the sampling: the next frame's SDF update may start (the marching
+// after the sampling the next frame's SDF update may start (the marching
 // cubes read the sample buffer; when some cells did not fit it, which the
-// update's tile count shows, the SDF update stands down)
+// update's tile count shows, the SDF update stands down): the update's tile
+// count and number for that frame's gate (the marching cubes' launch
+// signals the sampling's end)
 static void mesh_samples_taken(ff_ctx *c)
 {
     MeshState *s = c->mesh;
-    if (!c->seqOk) return;
-    c->samplePending = seq_signal(c, FF_SEQ_SAMPLE, c->stream);  // (else the stream has drained)
     c->pendingTiles = &s->d_mc->nTiles;
     c->pendingTileCap = s->tileCap;
     c->pendingId = s->launchId;
@@ -2540,13 +2546,25 @@ void mesh_launch_mc(ff_ctx *c)
     MeshState *s = c->mesh;
     // (timed on one update in timeEvery, by events its own dispatch records)
     s->mcTimed = c->timing && c->ev[8] && (c->updates++ % c->timeEvery) == 0;
+    // the launch writes the sampling's sequence word as it starts
+    uint32_t *sw = nullptr;
+    uint32_t sv = 0;
+    if (c->seqOk) {
+        sw = c->d_seq + FF_SEQ_SAMPLE;
+        sv = c->seq[FF_SEQ_SAMPLE] + 1u;
+    }
     if (s->mcTimed)
         hipExtLaunchKernelGGL(k_mc, dim3(2048), dim3(256), 0, c->stream, c->ev[8], c->ev[9], 0,
                               (CellRec *)s->d_cells[s->nxt], (const CellGeom *)s->d_geom, s->d_mc,
-                              (const uint4 *)s->d_sbuf, s->m, pools(s), s->cellCap);
+                              (const uint4 *)s->d_sbuf, s->m, pools(s), s->cellCap, sw, sv);
     else
         k_mc<<<2048, 256, 0, c->stream>>>(s->d_cells[s->nxt], s->d_geom, s->d_mc, s->d_sbuf, s->m, pools(s),
-                                          s->cellCap);
+                                          s->cellCap, sw, sv);
+    // a frame waits for the word only once the launch is in the stream
+    if (sw && hipPeekAtLastError() == hipSuccess) {
+        c->seq[FF_SEQ_SAMPLE] = sv;
+        c->samplePending = true;
+    }
     // the counters are final (the hash turnover does not change them)
     s->doneMapped = !c->timingAll;
     if (s->doneMapped) k_mesh_report<<<1, 64, 0, c->stream>>>(s->d_mc, s->d_hmc, &s->d_rep[2], s->launchId);
