@@ -1978,7 +1978,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void k
                                             uint32_t *sampleSeq, uint32_t sampleVal)
 {
     if (sampleSeq && blockIdx.x == 0 && threadIdx.x == 0)
-        __hip_atomic_store(sampleSeq, sampleVal, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+        __hip_atomic_store(sampleSeq, sampleVal, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
     __shared__ WaveLds lds[4];
     __shared__ uint64_t tri[256];
     tri[threadIdx.x] = c_tri64[threadIdx.x];  // 256 threads
