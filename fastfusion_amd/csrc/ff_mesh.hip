@@ -3176,8 +3176,12 @@ int ff_update_meshes_batch(ff_ctx *const *ctxs, int n)
         ctxs[i]->stream = st;
     }
     std::vector<int> act;
-    for (int i = 0; i < n; i++)
+    for (int i = 0; i < n; i++) {
+        // each context is meshed as of its last frame (the stamp its next
+        // update compares the bricks' stamps against)
+        ctxs[i]->mesh->stampFor = ctxs[i]->stampCounter;
         if (ctxs[i]->incremental && mesh_pre_host(ctxs[i], false)) act.push_back(i);
+    }
     const int na = (int)act.size();
     if (na) {
         // one upload of all descriptors, one launch per kernel, one copy back

@@ -84,5 +84,10 @@ def test_batch_equals_single_streams(config):
         batch.updateMeshes()
         for s in range(len(STREAMS)):
             _same_map(single[s], batched[s], f"tick {tick} stream {s}")
+            # the batched update re-meshes the cells the single one does (a
+            # stale stamp would re-mesh every cell: equal meshes, more work)
+            ua, ub = single[s].getMeshCellsUpdate(), batched[s].getMeshCellsUpdate()
+            assert np.array_equal(ua.desc, ub.desc), \
+                f"tick {tick} stream {s}: {len(ub.desc)} cells re-meshed batched, {len(ua.desc)} single"
     nf = [_same_mesh(single[s], batched[s], f"stream {s}") for s in range(len(STREAMS))]
     assert min(nf) > 100
