@@ -2392,14 +2392,22 @@ static void sdf_timing_collect(ff_ctx *c)
     }
 }
 
+// one thread writes a sequence word's next value: the stream starts the
+// launch only once the work before it has ended.  A kernel launch costs the
+// host ~2.5 us, the runtime's stream write operation several times that.
+__global__ void k_seq_write(uint32_t *word, uint32_t v)
+{
+    if (threadIdx.x == 0) __hip_atomic_store(word, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
 bool seq_signal(ff_ctx *c, int k, hipStream_t st)
 {
     const uint32_t v = c->seq[k] + 1u;
-    if (hipStreamWriteValue32(st, c->d_seq + k, v, 0) == hipSuccess) {
+    k_seq_write<<<1, 64, 0, st>>>(c->d_seq + k, v);
+    if (hipGetLastError() == hipSuccess) {
         c->seq[k] = v;
         return true;
     }
-    (void)hipGetLastError();
     hipStreamSynchronize(st);
     return false;
 }
