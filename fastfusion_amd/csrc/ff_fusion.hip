@@ -2090,9 +2090,9 @@ static int run_frame(ff_ctx *c, int W, int H, bool overlap)
         // the third stream, beside the de-dup and the descent
         bool confAside = c->learned && c->aux;
         if (confAside) {
-            // (the previous frame's SDF update, which reads the
-            // confidences, may still run)
-            seq_wait(c, FF_SEQ_SDF, c->aux);
+            // (the previous frame's SDF update, which reads its
+            // confidences, may still run: a wait when they share a buffer)
+            if (c->lastConf == c->prevConf) seq_wait(c, FF_SEQ_SDF, c->aux);
             // the frame's host images go up on the side stream first
             if (c->uploadOnSide && seq_signal(c, FF_SEQ_UP, c->side)) seq_wait(c, FF_SEQ_UP, c->aux);
             learned_launch_direct(c, W, H, c->aux);
@@ -2161,7 +2161,7 @@ static int run_frame(ff_ctx *c, int W, int H, bool overlap)
 // volume, column/row tables, pose).  depth / bgr: the frame in device
 // memory, dense rows.
 int frame_prepare(ff_ctx *c, const uint16_t *depth, const uint8_t *bgr, int W, int H, const double R[9],
-                  const double t[3], const double K[4], float scaling, float maxcam, bool upload)
+                  const double t[3], const double K[4], float scaling, float maxcam, bool upload, bool overlap)
 {
     if (c->broken) {
         c->err = "a frame overflowed the per-frame node table: the context is unusable";
@@ -2215,9 +2215,10 @@ int frame_prepare(ff_ctx *c, const uint16_t *depth, const uint8_t *bgr, int W, i
     f.sp.useColor = c->useColor ? 1 : 0;
     f.sp.conf = nullptr;
     if (c->learned) {
-        f.sp.conf = learned_prepare(c, W, H);
+        f.sp.conf = learned_prepare(c, W, H, overlap);
         if (!f.sp.conf) return FF_E_DEVICE;
     }
+    c->prevConf = c->lastConf;
     c->lastConf = f.sp.conf;
     f.depth = depth;
     f.bgr = bgr;
@@ -2462,7 +2463,7 @@ static int add_map_impl(ff_ctx *c, const uint16_t *depth, const uint8_t *bgr, in
     c->sdfTileId = 0;
     sdf_timing_collect(c);
     seq_rebase(c);
-    int rc = frame_prepare(c, depth, bgr, W, H, R, t, K, scaling, maxcam, !overlap);
+    int rc = frame_prepare(c, depth, bgr, W, H, R, t, K, scaling, maxcam, !overlap, overlap);
     if (rc) return rc;
     c->lastPbits = c->d_pbits;
     if (c->n) {

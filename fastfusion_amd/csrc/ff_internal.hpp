@@ -205,6 +205,7 @@ struct ff_ctx {
     // ff_add_map_device_noalloc)
     const uint32_t *lastPbits = nullptr;
     const float *lastConf = nullptr;
+    const float *prevConf = nullptr;  // the frame before's (its SDF update may still read it)
     bool directOnly = false;  // the frame in flight launches directly (no graph capture or replay)
     uint32_t *d_planes = nullptr, *d_pbits = nullptr, *d_cntB = nullptr, *d_cntL = nullptr, *d_firstD = nullptr,
              *d_pl0 = nullptr;
@@ -254,7 +255,7 @@ namespace ff {
 // learned confidence (ff_learned.hip): the per-pixel buffer for a W x H
 // frame (allocated on first use), and the launch filling it from c->d_fd
 void learned_free(ff_ctx *c);
-const float *learned_prepare(ff_ctx *c, int W, int H);
+const float *learned_prepare(ff_ctx *c, int W, int H, bool alternate);
 void learned_launch(ff_ctx *c, int W, int H, hipStream_t st);
 void learned_launch_direct(ff_ctx *c, int W, int H, hipStream_t st);
 void learned_swap_conf(ff_ctx *c, float *buf, size_t cap, float **old, size_t *oldCap);
@@ -264,8 +265,11 @@ void drop_frame_graphs(ff_ctx *c);
 // device work (allocation, then confidence + SDF; or the empty-frame path),
 // and the host part after the device work completed
 // upload = false: the frame block is left in c->h_fd for the caller to copy
+// overlap: the frame runs on the side stream (its confidences then go to
+// the other of two buffers)
 int frame_prepare(ff_ctx *c, const uint16_t *depth, const uint8_t *bgr, int W, int H, const double R[9],
-                  const double t[3], const double K[4], float scaling, float maxcam, bool upload = true);
+                  const double t[3], const double K[4], float scaling, float maxcam, bool upload = true,
+                  bool overlap = false);
 void launch_alloc(ff_ctx *c, int W, int H);
 // launch_alloc in two parts: the front reads the frame and writes only the
 // bit planes, the processed-pixel bitmap and the frame counters (nothing of

@@ -741,8 +741,19 @@ struct MlpBuffers {
 
 struct LearnedState {
     MlpBuffers w;
+    // the buffer the current frame's confidence pass writes: one of bufs,
+    // or a caller's buffer swapped in for one frame (external)
     float *conf = nullptr;
     size_t confCap = 0;
+    // overlapped frames alternate between two buffers: a frame's pass then
+    // writes the buffer the frame before last read, whose SDF update has
+    // ended (it precedes the previous frame's allocation on the side
+    // stream, whose report the host awaited), so the pass need not wait for
+    // the previous frame's SDF update, which reads the other buffer
+    float *bufs[2] = {nullptr, nullptr};
+    size_t bufCap[2] = {0, 0};
+    int cur = 0;
+    bool external = false;
 };
 
 static bool lok(ff_ctx *c, hipError_t e, const char *what)
@@ -757,27 +768,37 @@ void learned_free(ff_ctx *c)
     if (!c->learned) return;
     c->bufGen++;
     c->learned->w.free_all();
-    hipFree(c->learned->conf);
+    hipFree(c->learned->bufs[0]);
+    hipFree(c->learned->bufs[1]);
     delete c->learned;
     c->learned = nullptr;
     c->lastConf = nullptr;
 }
 
-// the confidence buffer for a W x H frame, or nullptr on failure
-const float *learned_prepare(ff_ctx *c, int W, int H)
+// the confidence buffer for a W x H frame, or nullptr on failure;
+// alternate: take the other of the two buffers (an overlapped frame)
+const float *learned_prepare(ff_ctx *c, int W, int H, bool alternate)
 {
     LearnedState *L = c->learned;
     if (!L) return nullptr;
     const size_t np = (size_t)W * H;
-    if (L->confCap < np) {
+    if (L->external) return L->confCap >= np ? L->conf : nullptr;
+    if (alternate) {
+        L->cur ^= 1;
+        c->bufGen++;  // (a captured graph holds the other buffer)
+    }
+    const int k = L->cur;
+    if (L->bufCap[k] < np) {
         frame_join(c);  // the frame in flight may still read the confidences
-        hipFree(L->conf);
-        L->conf = nullptr;
-        L->confCap = 0;
-        if (!lok(c, hipMalloc(&L->conf, sizeof(float) * np), "hipMalloc(confidence)")) return nullptr;
-        L->confCap = np;
+        hipFree(L->bufs[k]);
+        L->bufs[k] = nullptr;
+        L->bufCap[k] = 0;
+        if (!lok(c, hipMalloc(&L->bufs[k], sizeof(float) * np), "hipMalloc(confidence)")) return nullptr;
+        L->bufCap[k] = np;
         c->bufGen++;
     }
+    L->conf = L->bufs[k];
+    L->confCap = L->bufCap[k];
     return L->conf;
 }
 
@@ -791,6 +812,7 @@ void learned_swap_conf(ff_ctx *c, float *buf, size_t cap, float **old, size_t *o
     if (oldCap) *oldCap = L->confCap;
     L->conf = buf;
     L->confCap = cap;
+    L->external = buf != L->bufs[0] && buf != L->bufs[1];
 }
 
 // per-pixel confidence of the frame described by c->d_fd
