@@ -331,6 +331,18 @@ class FusionMipMapGPU:
     def setIncrementalMeshing(self, on=True):
         self._check(self._lib.ff_set_incremental_meshing(self._ctx, int(bool(on))))
 
+    def setThreadMeshing(self, on=True):
+        """setThreadMeshing (geometryfusion_mipmap_cpu.cpp:1110-1114): accepted
+        and ignored; the meshing is a device pass that runs behind the caller."""
+        self.threadMeshing = bool(on)
+
+    def setDepthChecks(self, depthchecks):
+        """setDepthChecks (geometryfusion_mipmap_cpu.cpp:1116-1120): the
+        reference stores the number clamped at 0 and never reads it (its
+        backward depth checks in addMap are commented out, :1629-1703), so
+        the value is kept and has no effect."""
+        self.depthChecks = max(int(depthchecks), 0)
+
     def setMeshSampleTiles(self, tiles):
         """Capacity of the meshing sample buffer in 64-sample tiles (no
         counterpart in the reference; cells beyond it are meshed in place)."""

@@ -133,7 +133,8 @@ public:
                                 cam.t, K, scaling, maxcamdistance));
     }
 
-    // bool updateMeshes(): synchronous; always true
+    // bool updateMeshes(): records the request (the next call enqueues it on
+    // the device and runs beside it); always true
     bool updateMeshes()
     {
         check(ff_update_meshes(ctx_));
@@ -171,7 +172,10 @@ public:
         return ff_load_map(ctx_, path.c_str(), offset, n) == FF_OK;
     }
     void setThreadMeshing(bool) {}  // meshing is a device pass; there is no meshing thread
-    void setDepthChecks(int) {}     // the reference's depth checks are commented out (:1627-1700)
+    // the reference stores the count clamped at 0 and never reads it (its
+    // backward depth checks are commented out, :1629-1703)
+    void setDepthChecks(int depthchecks) { depthChecks_ = depthchecks < 0 ? 0 : depthchecks; }
+    int depthChecks() const { return depthChecks_; }
 
     ff_ctx *handle() { return ctx_; }
 
@@ -194,6 +198,7 @@ private:
         return rc;
     }
     ff_ctx *ctx_ = nullptr;
+    int depthChecks_ = 0;
 };
 
 // The loop-closure partner map (FusionMipMapCPU(FusionMipMapCPU*,

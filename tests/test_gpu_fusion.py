@@ -13,6 +13,12 @@ pytestmark = pytest.mark.gpu
 def test_first_frame_c1_640():
     sc, R, t, cfg = synth.config_c1(n_frames=200)
     gpu, ora = make_pair(cfg["scale"], cfg["threshold"])
+    # the reference's two settings without effect on the fused map: the
+    # depth-check count is clamped at 0 and stored, the meshing thread flag
+    # is accepted (geometryfusion_mipmap_cpu.cpp:1110-1120)
+    gpu.setDepthChecks(-2)
+    gpu.setThreadMeshing(True)
+    assert gpu.depthChecks == 0
     run_pair(sc, R, t, [0], cfg, gpu, ora)
 
 
