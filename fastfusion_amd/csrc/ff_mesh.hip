@@ -1161,7 +1161,7 @@ __device__ __forceinline__ int geom_coord(const CellGeom &g, int k, int gk)
 }
 
 // sample s (raster order over n) of a cell; ll points at the cell's lastLeafs
-__device__ Sample sample_cell(const MapRO &m, const CellGeom &g, const uint32_t *ll, int s)
+__device__ __forceinline__ Sample sample_cell(const MapRO &m, const CellGeom &g, const uint32_t *ll, int s)
 {
     const int n0 = g.n[0], n1 = g.n[1];
     const int i = s % n0, j = (s / n0) % n1, l = s / (n0 * n1);
@@ -1513,20 +1513,32 @@ __device__ __forceinline__ void raise_depth(CellRec *cells, uint32_t cell, uint3
 }
 
 // (the grid's stride is a multiple of 64, so a wave's lanes are one tile)
-__device__ __forceinline__ void sample_body(const CellGeom *__restrict__ geom, CellRec *cells,
+__device__ __forceinline__ void sample_body(const CellGeom *__restrict__ geom, CellRec *__restrict__ cells,
                                             const TileRec *__restrict__ tiles, const MeshCounters *mc,
                                             uint32_t tileCap, const MapRO &m, uint4 *__restrict__ sbuf)
 {
     const uint32_t nt = min(mc->nTiles, tileCap);
     for (uint32_t t = blockIdx.x * blockDim.x + threadIdx.x; t < nt * 64u; t += gridDim.x * blockDim.x) {
         // one record: the geometry and the bricks (read where they lie, by
-        // the index the sample picks) come in together
+        // the index the sample picks); the geometry's two 16-byte words are
+        // requested together, before any test waits on a field (behind an
+        // early exit for the in-place tiles they were three dependent round
+        // trips)
         const TileRec *r = tiles + (t >> 6);
-        const CellGeom g = r->g;
-        if (g.tile0 == FF_NO_TILE) continue;  // the whole tile: its cell is sampled in place
+        const uint4 *gw = reinterpret_cast<const uint4 *>(r);
+        const uint4 w0 = gw[0], w1 = gw[1];  // CellGeom (28 B) and the spare word
+        // (an empty statement that takes every word: the compiler keeps the
+        // two loads here instead of splitting them around the tests below)
+        asm volatile("" ::"v"(w0.x), "v"(w0.y), "v"(w0.z), "v"(w0.w), "v"(w1.x), "v"(w1.y), "v"(w1.z));
+        static_assert(sizeof(CellGeom) == 28, "the geometry is the record's first 28 bytes");
+        CellGeom g;
+        memcpy(&g, &w0, 16);
+        memcpy(reinterpret_cast<unsigned char *>(&g) + 16, &w1, 12);
         const int s = (int)(g.pad * 64u + (t & 63u));
         uint32_t lv = 0;
-        if (s < geom_total(g)) {
+        // (a tile of a cell sampled in place is skipped; both tests taken
+        // together, so that both words are waited for once)
+        if ((g.tile0 != FF_NO_TILE) & (s < geom_total(g))) {
             const Sample sm = sample_cell(m, g, r->ll, s);
             sbuf[t] = pack_sample(sm);
             lv = sm.lv;
@@ -1539,10 +1551,10 @@ __device__ __forceinline__ void sample_body(const CellGeom *__restrict__ geom, C
 // memory), so that the host knows whether the next frame's SDF update,
 // which follows this sampling, stands down (cells whose samples did not fit
 // the buffer are sampled from the bricks by the marching cubes)
-__global__ __launch_bounds__(256) void k_sample(const CellGeom *__restrict__ geom, CellRec *cells,
+__global__ __launch_bounds__(256) void k_sample(const CellGeom *__restrict__ geom, CellRec *__restrict__ cells,
                                                 const TileRec *__restrict__ tiles, const MeshCounters *mc,
                                                 uint32_t tileCap, MapRO m, uint4 *__restrict__ sbuf,
-                                                uint32_t *rep, uint32_t updateId)
+                                                uint32_t *__restrict__ rep, uint32_t updateId)
 {
     if (rep && blockIdx.x == 0 && threadIdx.x == 0) {
         rep[0] = mc->nTiles;
