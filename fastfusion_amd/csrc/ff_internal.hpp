@@ -156,11 +156,12 @@ struct ff_ctx {
     hipStream_t side = nullptr;
     // Order between the streams: sequence words in device memory, one per
     // kind of producer; the producing stream writes the word's next value
-    // after its work (hipStreamWriteValue32) and a consuming stream waits
-    // until the word reaches it (hipStreamWaitValue32).  Measured on the
-    // MI355X: ~2 us of host time each and ~4 us from the producer's end to
-    // the consumer's start, where an event wait on a pending event costs
-    // ~16 us of host time and ~10 us.  seq[k]: the last value enqueued.
+    // after its work (a one-thread kernel, or the first thread of the
+    // kernel that follows the work there) and a consuming stream waits
+    // until the word reaches it (hipStreamWaitValue32, ~2 us of host time;
+    // ~4 us from the producer's end to the consumer's start, where an event
+    // wait on a pending event costs ~16 us of host time and ~10 us).
+    // seq[k]: the last value enqueued.
     uint32_t *d_seq = nullptr;
     uint32_t seq[4] = {0, 0, 0, 0};
     bool seqOk = false;  // the runtime takes stream memory operations
