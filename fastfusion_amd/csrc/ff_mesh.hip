@@ -1104,7 +1104,8 @@ __device__ __forceinline__ bool axis_full(int type, int k)
 
 // sampling grid of a cell (per type, from the scales of its lastLeaf bricks;
 // mesh_interleaved_meshcell.cpp interior / wall / edge / corner meshers)
-__device__ void cell_geom(const MapRO &m, uint64_t key, const uint32_t ll[8], CellGeom &g)
+template <typename Info>
+__device__ __forceinline__ void cell_geom_t(uint64_t key, Info info, CellGeom &g)
 {
     const int type = (int)(key & 7u);
     const int size = 1 << ((key >> 3) & 31u);
@@ -1112,26 +1113,26 @@ __device__ void cell_geom(const MapRO &m, uint64_t key, const uint32_t ll[8], Ce
     bool empty = false;
     int sizeMin = 1, stretch = 1;
     if (type == 0) {
-        const ushort4 li = m.info[ll[0]];
+        const ushort4 li = info(0);
         sizeMin = li.w;
         if (sizeMin > size || size / sizeMin > FF_BL) empty = true;
         const int lp[3] = {li.x, li.y, li.z}, ext = sizeMin * FF_BL;
         for (int k = 0; k < 3; k++)
             if (o[k] + size > lp[k] + ext || o[k] < lp[k]) empty = true;
     } else if (type <= 3) {
-        const int s0 = m.info[ll[0]].w, s1 = m.info[ll[1]].w;
+        const int s0 = info(0).w, s1 = info(1).w;
         sizeMin = min(s0, s1);
         stretch = s0;
         if (stretch > size) empty = true;
     } else if (type <= 6) {
         int sc[4];
-        for (int k = 0; k < 4; k++) sc[k] = m.info[ll[k]].w;
+        for (int k = 0; k < 4; k++) sc[k] = info(k).w;
         stretch = max(max(sc[0], sc[1]), sc[2]);
         sizeMin = min(min(sc[0], sc[1]), min(sc[2], sc[3]));
         if (stretch > size) empty = true;
     } else {
-        stretch = m.info[ll[0]].w;
-        for (int k = 1; k < 7; k++) stretch = max(stretch, (int)m.info[ll[k]].w);
+        stretch = info(0).w;
+        for (int k = 1; k < 7; k++) stretch = max(stretch, (int)info(k).w);
         sizeMin = stretch;
         if (stretch > size) empty = true;
     }
@@ -1148,6 +1149,12 @@ __device__ void cell_geom(const MapRO &m, uint64_t key, const uint32_t ll[8], Ce
     g.size = (uint16_t)size;
     g.sizeMin = (uint16_t)sizeMin;
     g.stretch = (uint16_t)stretch;
+}
+
+// (the bricks' records read where they lie)
+__device__ void cell_geom(const MapRO &m, uint64_t key, const uint32_t ll[8], CellGeom &g)
+{
+    cell_geom_t(key, [&](int k) { return m.info[ll[k]]; }, g);
 }
 
 __device__ __forceinline__ int geom_total(const CellGeom &g) { return (int)g.n[0] * g.n[1] * g.n[2]; }
@@ -1320,12 +1327,19 @@ __global__ __launch_bounds__(FF_MARK_THREADS) void k_mark_inc(CellRec *cells, co
         const uint32_t i = ib + threadIdx.x;
         bool keep = false, isDirty = false;
         CellRec c;
+        // (the bricks' records, requested beside the stamps: a dirty cell's
+        // geometry needs them)
+        ushort4 li[8];
         if (i < nPrev) {
             c = prev[i];
+#pragma unroll
+            for (int k = 0; k < 8; k++) li[k] = m.info[c.ll[k] < FF_BRANCHINIT ? c.ll[k] : 0u];
             keep = !cell_owner_touched(c.key, c.pad[2], box, nBox);
             if (keep) isDirty = chains_dirty(stamp, parent, c.ll, since, c.pad[1]);
         } else if (i < nItems) {
             c = fresh[i - nPrev];
+#pragma unroll
+            for (int k = 0; k < 8; k++) li[k] = m.info[c.ll[k] < FF_BRANCHINIT ? c.ll[k] : 0u];
             keep = true;
             int found = -1;
             uint32_t h = hkey_hash(c.key, hmask);
@@ -1356,7 +1370,7 @@ __global__ __launch_bounds__(FF_MARK_THREADS) void k_mark_inc(CellRec *cells, co
             // cleared, like k_mark_same's (k_mc fills it unless the mesh is empty)
             c.v0 = c.nv = c.f0 = c.nf = 0u;
             c.pad[0] = updateId;  // the update that meshed it (CellUpdate)
-            cell_geom(m, c.key, c.ll, g);
+            cell_geom_t(c.key, [&](int k) { return li[k]; }, g);
             g.cell = j;
             g.pad = 0;
             nt = ((uint32_t)geom_total(g) + 63u) >> 6;
@@ -1406,11 +1420,17 @@ __global__ __launch_bounds__(FF_MARK_THREADS) void k_mark_same(CellRec *cells, c
         if (i < nc) {
             CellRec &c = cells[i];
             for (int k = 0; k < 8; k++) ll[k] = c.ll[k];
+            // the bricks' records requested beside their stamps (a dirty
+            // cell's geometry needs them; behind the test they were one
+            // more dependent round trip)
+            ushort4 li[8];
+#pragma unroll
+            for (int k = 0; k < 8; k++) li[k] = m.info[ll[k] < FF_BRANCHINIT ? ll[k] : 0u];
             isDirty = chains_dirty(stamp, parent, ll, since, c.pad[1]);
             if (isDirty) {
                 c.v0 = c.nv = c.f0 = c.nf = 0u;
                 c.pad[0] = updateId;
-                cell_geom(m, c.key, ll, g);
+                cell_geom_t(c.key, [&](int k) { return li[k]; }, g);
                 g.cell = i;
                 g.pad = 0;
                 nt = ((uint32_t)geom_total(g) + 63u) >> 6;
