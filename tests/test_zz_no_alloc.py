@@ -55,6 +55,38 @@ def test_noalloc_equals_add_map_device():
     assert np.array_equal(ma.faces, mb.faces) and np.array_equal(ma.colors, mb.colors)
 
 
+def test_noalloc_learned_mixed_with_add_map_device():
+    """Learned weighting on, each frame's mesh update requested: one context
+    takes every frame through ff_add_map_device, the other alternates the
+    allocation-free entry point (its confidences in the scratch array) with
+    ff_add_map_device (its own two alternating confidence buffers); maps and
+    meshes must stay bit-identical."""
+    from fastfusion_amd import learned
+    sc, R, t, cfg = synth.config_c2(n_frames=20)
+    w = learned.load_weights()
+    a = FusionMipMapGPU(0, 0, 0, cfg["scale"], cfg["threshold"], 0, True, max_leaves=1 << 17)
+    b = FusionMipMapGPU(0, 0, 0, cfg["scale"], cfg["threshold"], 0, True, max_leaves=1 << 17)
+    a.setLearnedWeighting(w)
+    b.setLearnedWeighting(w)
+    bufs = []
+    for k, i in enumerate(range(0, 12, 2)):
+        d, bgr = sc.render(R[i].reshape(3, 3), t[i], K320, 320, 240, seed=1000 + i)
+        db, cb = hiprt.DeviceBuffer.from_array(d), hiprt.DeviceBuffer.from_array(bgr)
+        bufs += [db, cb]
+        args = (db.addr(), cb.addr(), 320, 240, R[i], t[i], K320, 1 / 5000.0, cfg["maxcam"])
+        n_a = a.addMapDevice(*args)
+        n_b = b.addMapDeviceNoAlloc(*args) if k % 2 == 0 else b.addMapDevice(*args)
+        assert n_a == n_b and n_a > 0, (k, n_a, n_b)
+        a.updateMeshes()
+        b.updateMeshes()
+        if k % 2 == 1:
+            _same(a, b, 320, 240, f"frame {i}")
+    ma, mb = a.getMeshInterleavedMarchingCubes(), b.getMeshInterleavedMarchingCubes()
+    assert len(ma.faces) > 0
+    assert np.array_equal(ma.vertices.view(np.uint32), mb.vertices.view(np.uint32))
+    assert np.array_equal(ma.faces, mb.faces) and np.array_equal(ma.colors, mb.colors)
+
+
 def test_noalloc_rejects_oversize_image():
     fz = FusionMipMapGPU(0, 0, 0, 0.01, 0.03, 0, True, max_leaves=1 << 12)
     d = hiprt.DeviceBuffer(2048 * 2048 * 2)
