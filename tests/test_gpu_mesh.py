@@ -60,6 +60,14 @@ def test_mesh_c1_mixed_scales():
     assert nface > 1000
 
 
+def test_mesh_c1_640():
+    """Configuration 1 at its own 640x480 resolution (the mixed-scale test
+    above runs it at 320x240): meshes equal the oracle's every third frame."""
+    sc, R, t, cfg = synth.config_c1(n_frames=200)
+    ncell, nface = run_mesh(sc, R, t, list(range(0, 40, 4)), cfg, synth.K_DEFAULT, check_every=3)
+    assert nface > 10000
+
+
 def test_mesh_c2_walk_5mm():
     sc, R, t, cfg = synth.config_c2(n_frames=60)
     ncell, nface = run_mesh(sc, R, t, list(range(0, 16)), cfg, synth.K_DEFAULT, check_every=5, max_leaves=1 << 18)
