@@ -1023,6 +1023,13 @@ static_assert(sizeof(TileRec) == 64, "a tile record is four 16-byte words");
 // per block, not per wave: the counters are shared by every block of the
 // launch, and appends to one address serialise.
 #define FF_MARK_THREADS 256  // block size of the marking kernels
+// blocks of k_sample (256 threads each, striding over the tiles): a 1280x960
+// frame at 2.5 mm leaves tens of thousands of tiles, whose sampling is a
+// chain of dependent brick reads per lane, so more lanes in flight hide
+// more of the latency (measured: 8192 blocks +3% frames/s over 4096 there
+// and no change at 640x480; 1024 or 2048 lose 6-11% there, 16384 gains
+// nothing more)
+#define FF_SAMPLE_BLOCKS 8192
 
 struct MarkSlot {
     uint32_t d, t0;
@@ -2464,7 +2471,7 @@ bool mesh_launch_pre(ff_ctx *c, bool timed)
                       c->stream>>>(
             s->d_cells[nxt], c->d_stamp, c->d_parent, s->lastMeshStamp, s->d_mc, nc, mc_other(s), s->d_geom,
             s->cellCap, m, s->d_tiles, s->tileCap, s->launchId);
-        k_sample<<<4096, 256, 0, c->stream>>>(s->d_geom, s->d_cells[nxt], s->d_tiles, s->d_mc, s->tileCap, m,
+        k_sample<<<FF_SAMPLE_BLOCKS, 256, 0, c->stream>>>(s->d_geom, s->d_cells[nxt], s->d_tiles, s->d_mc, s->tileCap, m,
                                               s->d_sbuf, s->d_rep, s->launchId);
         mesh_samples_taken(c);
         mc_flip(s);
@@ -2510,7 +2517,7 @@ bool mesh_launch_pre(ff_ctx *c, bool timed)
                                                         s->lastMeshStamp, s->d_mc, s->d_geom, s->cellCap, m,
                                                         s->d_tiles, s->tileCap, s->launchId);
     }
-    k_sample<<<4096, 256, 0, c->stream>>>(s->d_geom, s->d_cells[nxt], s->d_tiles, s->d_mc, s->tileCap, m, s->d_sbuf,
+    k_sample<<<FF_SAMPLE_BLOCKS, 256, 0, c->stream>>>(s->d_geom, s->d_cells[nxt], s->d_tiles, s->d_mc, s->tileCap, m, s->d_sbuf,
                                           s->d_rep, s->launchId);
     mesh_samples_taken(c);
     mc_flip(s);
